@@ -1,0 +1,256 @@
+/*
+ * vfrag.h — C ABI of libvfrag, the MI355X (gfx950) implementation of
+ * VoxelFragmentML's voxelize -> fracture path.
+ *
+ * Every function is extern "C", takes plain pointers and sizes, and returns an
+ * int status (VF_OK or a negative VF_ERR_*); vf_last_error() gives the
+ * thread-local message of the last failure.  No exceptions cross this ABI.
+ * Host arrays are borrowed for the duration of a call; grids, batches and
+ * servers are opaque handles owned by the library.  Grid values follow the
+ * reference: uint16 per cell, linear index x*ny*nz + y*nz + z, 0 = EMPTY,
+ * 1 = FREE, >= 2 = fragment label, bit 15 = boundary mask.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference root, MF = MeshFragments).
+ */
+#ifndef VFRAG_H
+#define VFRAG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define VF_OK 0
+#define VF_ERR_INVALID_ARG (-1)   /* bad dims, pointer, seed or enum value */
+#define VF_ERR_OUT_OF_MEMORY (-2) /* device or host allocation failed */
+#define VF_ERR_DEVICE (-3)        /* a HIP call failed */
+#define VF_ERR_TOO_MANY_SEEDS (-4)/* flood labels must fit the 8-bit id field (2..255) */
+#define VF_ERR_BUDGET (-5)        /* job does not fit the server's HBM budget */
+#define VF_ERR_CANCELLED (-6)     /* job cancelled before completion */
+#define VF_ERR_NOT_READY (-7)     /* poll: job still running */
+#define VF_ERR_OVERFLOW (-8)      /* a packed key or counter would overflow */
+#define VF_ERR_SEEDING (-9)       /* fewer candidate cells than seeds (SeederSearchError) */
+#define VF_ERR_POISON (-10)       /* job flagged as poison by the caller (failure-path tests) */
+
+/* FractureParameters::DistanceFunction / fracturer::DistanceFunction
+ * (MF/Source/Graphics/Core/FractureParameters.h:21, MF/Source/Fracturer/Fracturer.h:10-14) */
+#define VF_EUCLIDEAN 0
+#define VF_MANHATTAN 1
+#define VF_CHEBYSHEV 2
+
+/* FractureParameters::FractureAlgorithm (FractureParameters.h:18) */
+#define VF_NAIVE 0
+#define VF_FLOOD 1
+
+/* flood neighbourhood: VF_NEIGH_FROM_DISTANCE follows FloodFracturer::build
+ * (FloodFracturer.cpp:114: MANHATTAN -> 6 neighbours, anything else -> 26) */
+#define VF_NEIGH_FROM_DISTANCE 0
+#define VF_NEIGH_VON_NEUMANN 6
+#define VF_NEIGH_MOORE 26
+
+/* Seeder::Location (MF/Source/Fracturer/Seeder.h:25) */
+#define VF_SEED_BOTH 0
+#define VF_SEED_OUTER 1
+#define VF_SEED_INNER 2
+
+#define VF_ALL_LABELS 0xFFFFFFFFu
+
+typedef struct vf_grid vf_grid;
+
+/* glm::uvec4 {x, y, z, label} seed layout (Seeder.h:69-76, Fracturer.h:35) */
+typedef struct {
+    uint32_t x, y, z, w;
+} vf_seed;
+
+/* ---- library ------------------------------------------------------------- */
+const char* vf_last_error(void);
+int vf_version(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int vf_device_count(void);
+
+/* ---- grid handles ---------------------------------------------------------- */
+/* RegularGrid(const AABB&, const ivec3&) (MF/Source/DataStructures/RegularGrid.h:152,
+ * RegularGrid.cpp:21-27): device-resident uint16 grid, cleared to EMPTY. */
+int vf_grid_create(const uint32_t dims[3], const float aabb_min[3], const float aabb_max[3], vf_grid** out);
+int vf_grid_destroy(vf_grid* g);
+/* RegularGrid::setAABB (RegularGrid.cpp:426-441): new box and dims, grid cleared. */
+int vf_grid_set_aabb(vf_grid* g, const float aabb_min[3], const float aabb_max[3], const uint32_t dims[3]);
+/* RegularGrid::getNumSubdivisions (RegularGrid.h:314) */
+int vf_grid_dims(const vf_grid* g, uint32_t dims[3]);
+/* RegularGrid::updateGrid / data() (RegularGrid.cpp:505-509, :518-521): device -> host copy */
+int vf_grid_read(vf_grid* g, uint16_t* out);
+/* RegularGrid::updateSSBO (RegularGrid.cpp:511-514): host -> device copy */
+int vf_grid_write(vf_grid* g, const uint16_t* in);
+/* uint8 occupancy (1 = cell not EMPTY) */
+int vf_occupancy_read(vf_grid* g, uint8_t* out);
+/* number of non-EMPTY cells (RegularGrid::numOccupiedVoxels, RegularGrid.h:233) */
+int vf_grid_count_occupied(vf_grid* g, uint64_t* out);
+/* device pointer to the uint16 cell array (RegularGrid::ssbo(), RegularGrid.h:263) */
+int vf_grid_device_ptr(vf_grid* g, void** ptr);
+/* block until every operation queued on the grid has finished */
+int vf_grid_synchronize(vf_grid* g);
+
+/* ---- voxelization ---------------------------------------------------------- */
+/* RegularGrid::fill(Model3D*) (RegularGrid.cpp:173-212): conservative FP32
+ * triangle/box SAT shell plus z-parity interior; cells become FREE.
+ * xyz: nv*3 floats, tri: nt*3 vertex indices (host memory). */
+int vf_voxelize(vf_grid* g, const float* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt);
+/* same, with the mesh already in device memory */
+int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32_t* d_tri, uint32_t nt);
+
+/* ---- seeding --------------------------------------------------------------- */
+/* Seeder::uniform (MF/Source/Fracturer/Seeder.cpp:154-208), counter-based: S
+ * distinct candidate cells drawn uniformly, in (x,y,z) order, labels 2..S+1. */
+int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed* out);
+
+/* ---- fracture -------------------------------------------------------------- */
+/* NaiveFracturer::build (MF/Source/Fracturer/NaiveFracturer.cpp:215-225,
+ * buildCPU :26-68, naiveFracturer-comp.glsl:19-42): every non-EMPTY cell takes
+ * the label of the nearest seed, ties to the lowest seed index. */
+int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance);
+/* NaiveFracturer::removeIsolatedRegionsCPU (NaiveFracturer.cpp:111-150) */
+int vf_remove_isolated_regions(vf_grid* g, const vf_seed* seeds, uint32_t S);
+/* FloodFracturer::build (MF/Source/Fracturer/FloodFracturer.cpp:98-191): region
+ * growing from the seeds, deterministic (distance, seed index) claims, node
+ * weights 1 + hash noise in [0, noise_amp]. */
+int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance, int neighbourhood,
+                      uint64_t noise_seed, uint32_t noise_amp);
+
+/* ---- post-fracture --------------------------------------------------------- */
+/* RegularGrid::detectBoundaries (RegularGrid.cpp:64-80, detectBoundaries-comp.glsl:18-43) */
+int vf_detect_boundaries(vf_grid* g, int size);
+/* RegularGrid::undoMask (RegularGrid.cpp:488-503) */
+int vf_undo_mask(vf_grid* g);
+/* RegularGrid::homogenize (RegularGrid.cpp:533-541) */
+int vf_homogenize(vf_grid* g);
+/* RegularGrid::countValues (RegularGrid.cpp:601-625): counts[v & 0x7FFF] for v > FREE.
+ * counts has ncounts entries (65536 covers every label); values >= ncounts are
+ * not reported. */
+int vf_histogram(vf_grid* g, uint32_t* counts, uint32_t ncounts);
+
+/* ---- marching cubes -------------------------------------------------------- */
+/* RegularGrid::toTriangleMesh -> MarchingCubes::triangulateFieldGPU
+ * (RegularGrid.cpp:443-486, MF/Source/Graphics/Core/MarchingCubes.cpp:364-433,
+ * marchingCubes-comp.glsl:96-156), all labels in one pass with an exact
+ * lattice-edge weld.  Two-call protocol: vf_marching_cubes computes the mesh
+ * on the device and reports sizes; vf_mesh_read copies it out. */
+typedef struct {
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    uint32_t num_labels;
+} vf_mesh_info;
+int vf_marching_cubes(vf_grid* g, uint32_t label /* VF_ALL_LABELS or one label */, vf_mesh_info* info);
+/* xyz: 3*nv FP32; tri: 3*nt vertex indices local to their label's vertex range;
+ * tri_flag: nt boundary flags (may be NULL); labels: nl; vert_offsets and
+ * tri_offsets: nl+1 each (may be NULL). */
+int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels,
+                 uint32_t* vert_offsets, uint32_t* tri_offsets);
+
+/* ---- grid file formats ----------------------------------------------------- */
+/* RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims + {uint16 value,
+ * uint32 count} runs.  Writes at most cap bytes; *size gets the full size. */
+int vf_grid_export_rle(vf_grid* g, uint8_t* out, size_t cap, size_t* size);
+/* inverse of the above (docs/decompress/decompress_grid.py:16-32); dims must match */
+int vf_grid_import_rle(vf_grid* g, const uint8_t* in, size_t n);
+
+/* ---- synthetic input ------------------------------------------------------- */
+/* Closed lathe-revolved vessel (outer wall, inner wall, bottom disks, rim),
+ * spline profile from seed, normalised like CADModel::load
+ * (MF/Source/Graphics/Core/CADModel.cpp:148-156): largest half extent 1.
+ * Pass NULL buffers to query nv/nt. */
+int vf_vessel_mesh(uint64_t seed, uint32_t radial, uint32_t height_segments, float* xyz, uint32_t* nv,
+                   uint32_t* tri, uint32_t* nt);
+
+/* ---- timing ---------------------------------------------------------------- */
+/* Per-kernel HIP-event timing on the stream each kernel is launched on.
+ * When enabled, every launch of a named kernel is bracketed by events;
+ * vf_timing_read returns total milliseconds and launch count per name. */
+int vf_timing_enable(int enable);
+int vf_timing_reset(void);
+int vf_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
+
+/* ---- batched form ------------------------------------------------------------ */
+/* One fracture job: voxelize a mesh, seed, fracture, then boundaries, histogram
+ * and (optionally) marching cubes, with the grid resident on the device.
+ * The same sequence CADScene::fractureModel + prepareScene drives
+ * (MF/Source/Graphics/Application/CADScene.cpp:624-691, :791-833). */
+typedef struct {
+    const float* xyz;      /* host mesh (or device when mesh_on_device) */
+    const uint32_t* tri;
+    uint32_t nv, nt;
+    uint32_t dims[3];
+    float aabb_min[3], aabb_max[3];
+    uint32_t num_seeds;
+    uint64_t seed;         /* seeding RNG key */
+    int seed_location;     /* VF_SEED_* */
+    int algorithm;         /* VF_NAIVE / VF_FLOOD */
+    int distance;          /* VF_EUCLIDEAN / VF_MANHATTAN / VF_CHEBYSHEV */
+    int neighbourhood;     /* flood only, VF_NEIGH_* */
+    uint64_t noise_seed;   /* flood only */
+    uint32_t noise_amp;    /* flood only */
+    int boundary_size;     /* detectBoundaries size; 0 skips the stage */
+    int marching_cubes;    /* 1: extract all labels */
+    int mesh_on_device;
+    int poison;            /* failure-path testing: the job fails with VF_ERR_POISON */
+    uint16_t* labels_out;  /* optional host copy of the final grid (V entries) */
+    uint32_t* histogram_out; /* optional host copy of counts[0..65535] */
+} vf_job;
+
+typedef struct {
+    int status;
+    uint64_t voxels;        /* V = nx*ny*nz */
+    uint64_t occupied;      /* non-EMPTY cells after voxelization */
+    uint64_t checksum;      /* order-independent checksum of the final grid */
+    uint32_t num_labels;    /* distinct labels with a nonzero count */
+    uint32_t mc_vertices, mc_triangles;
+    uint32_t flood_levels;  /* BFS levels run (flood) */
+    double latency_ms;      /* submit -> completion (server) or batch wall time */
+    double queue_ms;        /* submit -> admission (server) */
+} vf_result;
+
+typedef struct vf_batch vf_batch;
+/* Tuning: tile/block shapes come from the autotuner's table when tune_path is
+ * non-NULL and exists; otherwise defaults. */
+int vf_batch_create(vf_batch** out);
+int vf_batch_destroy(vf_batch* b);
+/* Runs n jobs together: every stage is one launch over all jobs' tiles (the
+ * flood runs all jobs' BFS levels in shared launches), captured in a HIP graph
+ * per launch shape.  results: n entries. */
+int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results);
+/* Plain form: the same jobs one after another, one job per launch. */
+int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results);
+int vf_batch_set_graphs(vf_batch* b, int use_graphs);
+
+/* ---- served form ------------------------------------------------------------- */
+/* Continuous batching: submitted jobs are admitted against an HBM budget as
+ * slots and memory free up, join the running batch at the next BFS level, and
+ * leave when done.  Thread-safe submit/poll. */
+typedef struct vf_server vf_server;
+int vf_server_create(size_t hbm_budget_bytes, uint32_t max_slots, vf_server** out);
+int vf_server_destroy(vf_server* s);
+/* The job (and its mesh) is copied; ticket identifies it. */
+int vf_server_submit(vf_server* s, const vf_job* job, uint64_t* ticket);
+int vf_server_cancel(vf_server* s, uint64_t ticket);
+/* VF_ERR_NOT_READY while running; otherwise fills result (result->status holds
+ * the job's own status) and forgets the ticket. */
+int vf_server_poll(vf_server* s, uint64_t ticket, vf_result* result);
+/* blocks until the ticket completes */
+int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result);
+/* bytes currently reserved by admitted jobs and number of jobs in flight */
+int vf_server_stats(vf_server* s, size_t* reserved_bytes, uint32_t* in_flight, uint64_t* completed);
+
+/* ---- autotuner ---------------------------------------------------------------- */
+/* Times each tunable stage over candidate block/tile shapes per grid-edge
+ * bucket on this device and writes the winners to path (text); vf_tune_load
+ * installs a table for subsequent launches. */
+int vf_tune_run(const char* path, int quick);
+int vf_tune_load(const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

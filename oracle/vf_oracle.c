@@ -1,0 +1,878 @@
+/*
+ * vf_oracle.c — CPU oracle for the voxelize -> fracture path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see vf_oracle.h).  Plain C, single threaded except
+ * where noted, compiled with -ffp-contract=off so every FP32 expression rounds
+ * exactly as written.  Reference file:line citations are relative to the
+ * reference repository root (MeshFragments/ is abbreviated MF/).
+ */
+#include "vf_oracle.h"
+#include "vfo_mc_table.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define VFO_EMPTY 0u
+#define VFO_FREE 1u
+#define VFO_MASK15 0x7FFFu
+
+static inline size_t gidx(size_t x, size_t y, size_t z, const uint32_t* d)
+{
+    /* RegularGrid::getPositionIndex (MF/Source/DataStructures/RegularGrid.cpp:1031-1034) */
+    return (x * d[1] + y) * d[2] + z;
+}
+
+/* ========================================================================= */
+/* Voxelization                                                              */
+/* ========================================================================= */
+
+/*
+ * Triangle/box overlap by the separating-axis theorem (Akenine-Moller), the
+ * spec named for the voxelizer.  Same tests and the same FP32 arithmetic per
+ * test as Intersections3D::intersect(Triangle3D&, AABB&) and its axis helpers
+ * (MF/Source/Geometry/3D/Intersections3D.h:204-420): 9 edge-cross-axis tests,
+ * 3 box-face tests, then the triangle-plane test.  Rejection uses strict '>'
+ * so touching counts as overlap.
+ */
+static int axis_sep(float pa, float pb, float rad)
+{
+    float mn = pa < pb ? pa : pb;
+    float mx = pa < pb ? pb : pa;
+    return (mn > rad || mx < -rad);
+}
+
+int vfo_sat_triangle_box(const float a[3], const float b[3], const float c[3],
+                         const float center[3], const float half[3])
+{
+    float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+    for (int q = 0; q < 3; ++q) {
+        v0[q] = a[q] - center[q];
+        v1[q] = b[q] - center[q];
+        v2[q] = c[q] - center[q];
+    }
+    for (int q = 0; q < 3; ++q) {
+        e0[q] = v1[q] - v0[q];
+        e1[q] = v2[q] - v1[q];
+        e2[q] = v0[q] - v2[q];
+    }
+    float fx, fy, fz, pa, pb, rad;
+
+    /* edge 0 */
+    fx = fabsf(e0[0]); fy = fabsf(e0[1]); fz = fabsf(e0[2]);
+    pa = e0[2] * v0[1] - e0[1] * v0[2]; pb = e0[2] * v2[1] - e0[1] * v2[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = -e0[2] * v0[0] + e0[0] * v0[2]; pb = -e0[2] * v2[0] + e0[0] * v2[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = e0[1] * v1[0] - e0[0] * v1[1]; pb = e0[1] * v2[0] - e0[0] * v2[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return 0;
+
+    /* edge 1 */
+    fx = fabsf(e1[0]); fy = fabsf(e1[1]); fz = fabsf(e1[2]);
+    pa = e1[2] * v0[1] - e1[1] * v0[2]; pb = e1[2] * v2[1] - e1[1] * v2[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = -e1[2] * v0[0] + e1[0] * v0[2]; pb = -e1[2] * v2[0] + e1[0] * v2[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = e1[1] * v0[0] - e1[0] * v0[1]; pb = e1[1] * v1[0] - e1[0] * v1[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return 0;
+
+    /* edge 2 */
+    fx = fabsf(e2[0]); fy = fabsf(e2[1]); fz = fabsf(e2[2]);
+    pa = e2[2] * v0[1] - e2[1] * v0[2]; pb = e2[2] * v1[1] - e2[1] * v1[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = -e2[2] * v0[0] + e2[0] * v0[2]; pb = -e2[2] * v1[0] + e2[0] * v1[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return 0;
+    pa = e2[1] * v1[0] - e2[0] * v1[1]; pb = e2[1] * v2[0] - e2[0] * v2[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return 0;
+
+    /* box faces: triangle bounds against the box extent */
+    for (int q = 0; q < 3; ++q) {
+        float mn = v0[q], mx = v0[q];
+        if (v1[q] < mn) mn = v1[q];
+        if (v1[q] > mx) mx = v1[q];
+        if (v2[q] < mn) mn = v2[q];
+        if (v2[q] > mx) mx = v2[q];
+        if (mn > half[q] || mx < -half[q]) return 0;
+    }
+
+    /* triangle plane (planeBoxOverlap, Intersections3D.h:270-294) */
+    float n[3];
+    n[0] = e0[1] * e1[2] - e1[1] * e0[2];
+    n[1] = e0[2] * e1[0] - e1[2] * e0[0];
+    n[2] = e0[0] * e1[1] - e1[0] * e0[1];
+    float vmin[3], vmax[3];
+    for (int q = 0; q < 3; ++q) {
+        if (n[q] > 0.0f) { vmin[q] = -half[q] - v0[q]; vmax[q] = half[q] - v0[q]; }
+        else             { vmin[q] = half[q] - v0[q];  vmax[q] = -half[q] - v0[q]; }
+    }
+    float dmin = n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2];
+    if (dmin > 0.0f) return 0;
+    float dmax = n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2];
+    return dmax >= 0.0f;
+}
+
+/* Cell box of index i along one axis: faces at aabb_min + i*cs, shared exactly
+ * by neighbouring cells; centre/extent as AABB::center()/extent()
+ * (MF/Source/Geometry/3D/AABB.h:41,51). */
+static void cell_box(int i, float mn, float cs, float* center, float* half)
+{
+    float lo = mn + (float)i * cs;
+    float hi = mn + (float)(i + 1) * cs;
+    *center = (hi + lo) / 2.0f;
+    *half = hi - *center;
+}
+
+#define VFO_SUB 256          /* fixed-point sub-cell resolution for the parity scan */
+#define VFO_SNAP_LIM 524288.0f
+
+static int32_t snap_coord(float x, float mn, float cs)
+{
+    float q = (x - mn) / cs;
+    q = q * (float)VFO_SUB;
+    if (!(q >= -VFO_SNAP_LIM)) q = -VFO_SNAP_LIM;
+    if (q > VFO_SNAP_LIM) q = VFO_SNAP_LIM;
+    return (int32_t)rintf(q);
+}
+
+static int64_t floordiv64(int64_t a, int64_t b) /* b > 0 */
+{
+    int64_t q = a / b;
+    if ((a % b) != 0 && a < 0) q -= 1;
+    return q;
+}
+
+static inline int32_t floordiv_sub(int32_t a) { return a >> 8; } /* floor(a/256) */
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* top-left tie rule for a CCW edge with direction (dx, dy) */
+static inline int edge_incl(int64_t e, int64_t dx, int64_t dy)
+{
+    return e > 0 || (e == 0 && (dy < 0 || (dy == 0 && dx < 0)));
+}
+
+int64_t vfo_voxelize(const float* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt,
+                     const float aabb_min[3], const float aabb_max[3], const uint32_t dims[3],
+                     uint8_t* occ, uint8_t* shell_out, uint8_t* parity_out)
+{
+    (void)nv;
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    float cs[3];
+    for (int q = 0; q < 3; ++q) cs[q] = (aabb_max[q] - aabb_min[q]) / (float)dims[q];
+
+    uint8_t* shell = (uint8_t*)calloc(V, 1);
+    uint8_t* toggle = (uint8_t*)calloc(V, 1);
+    if (!shell || !toggle) { free(shell); free(toggle); return -1; }
+
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float* p[3] = { xyz + 3 * (size_t)tri[3 * t], xyz + 3 * (size_t)tri[3 * t + 1],
+                              xyz + 3 * (size_t)tri[3 * t + 2] };
+        int finite = 1;
+        for (int k = 0; k < 3; ++k)
+            for (int q = 0; q < 3; ++q) finite &= isfinite(p[k][q]) != 0;
+        if (!finite) continue; /* non-finite triangles mark nothing */
+
+        /* --- shell: SAT against every cell of the triangle bbox grown by one cell --- */
+        int lo[3], hi[3], empty = 0;
+        for (int q = 0; q < 3; ++q) {
+            float mn = p[0][q], mx = p[0][q];
+            if (p[1][q] < mn) mn = p[1][q];
+            if (p[1][q] > mx) mx = p[1][q];
+            if (p[2][q] < mn) mn = p[2][q];
+            if (p[2][q] > mx) mx = p[2][q];
+            float fl = floorf((mn - aabb_min[q]) / cs[q]);
+            float fh = floorf((mx - aabb_min[q]) / cs[q]);
+            const float lim = (float)dims[q] + 1.0f;
+            fl = fl < -2.0f ? -2.0f : (fl > lim ? lim : fl);
+            fh = fh < -2.0f ? -2.0f : (fh > lim ? lim : fh);
+            if ((int)fh + 1 < 0 || (int)fl - 1 > (int)dims[q] - 1) empty = 1;
+            lo[q] = clampi((int)fl - 1, 0, (int)dims[q] - 1);
+            hi[q] = clampi((int)fh + 1, 0, (int)dims[q] - 1);
+        }
+        if (empty) { lo[0] = 1; hi[0] = 0; }
+        for (int x = lo[0]; x <= hi[0]; ++x)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    size_t id = gidx(x, y, z, dims);
+                    if (shell[id]) continue;
+                    float c[3], h[3];
+                    cell_box(x, aabb_min[0], cs[0], &c[0], &h[0]);
+                    cell_box(y, aabb_min[1], cs[1], &c[1], &h[1]);
+                    cell_box(z, aabb_min[2], cs[2], &c[2], &h[2]);
+                    if (vfo_sat_triangle_box(p[0], p[1], p[2], c, h)) shell[id] = 1;
+                }
+
+        /* --- z-parity: toggle the first cell centre above each crossing --- */
+        int64_t X[3], Y[3], Z[3];
+        for (int k = 0; k < 3; ++k) {
+            X[k] = snap_coord(p[k][0], aabb_min[0], cs[0]);
+            Y[k] = snap_coord(p[k][1], aabb_min[1], cs[1]);
+            Z[k] = snap_coord(p[k][2], aabb_min[2], cs[2]);
+        }
+        int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (Y[1] - Y[0]) * (X[2] - X[0]);
+        if (area == 0) continue;
+        if (area < 0) {
+            int64_t tx = X[1], ty = Y[1], tz = Z[1];
+            X[1] = X[2]; Y[1] = Y[2]; Z[1] = Z[2];
+            X[2] = tx; Y[2] = ty; Z[2] = tz;
+            area = -area;
+        }
+        int32_t mnx = (int32_t)X[0], mxx = (int32_t)X[0], mny = (int32_t)Y[0], mxy = (int32_t)Y[0];
+        for (int k = 1; k < 3; ++k) {
+            if (X[k] < mnx) mnx = (int32_t)X[k];
+            if (X[k] > mxx) mxx = (int32_t)X[k];
+            if (Y[k] < mny) mny = (int32_t)Y[k];
+            if (Y[k] > mxy) mxy = (int32_t)Y[k];
+        }
+        int i0 = floordiv_sub(mnx - VFO_SUB / 2 + VFO_SUB - 1), i1 = floordiv_sub(mxx - VFO_SUB / 2);
+        int j0 = floordiv_sub(mny - VFO_SUB / 2 + VFO_SUB - 1), j1 = floordiv_sub(mxy - VFO_SUB / 2);
+        if (i0 < 0) i0 = 0;
+        if (j0 < 0) j0 = 0;
+        if (i1 > (int)dims[0] - 1) i1 = (int)dims[0] - 1;
+        if (j1 > (int)dims[1] - 1) j1 = (int)dims[1] - 1;
+        const int64_t d0x = X[2] - X[1], d0y = Y[2] - Y[1];
+        const int64_t d1x = X[0] - X[2], d1y = Y[0] - Y[2];
+        const int64_t d2x = X[1] - X[0], d2y = Y[1] - Y[0];
+        for (int i = i0; i <= i1; ++i) {
+            const int64_t px = (int64_t)i * VFO_SUB + VFO_SUB / 2;
+            for (int j = j0; j <= j1; ++j) {
+                const int64_t py = (int64_t)j * VFO_SUB + VFO_SUB / 2;
+                int64_t E0 = d0x * (py - Y[1]) - d0y * (px - X[1]);
+                int64_t E1 = d1x * (py - Y[2]) - d1y * (px - X[2]);
+                int64_t E2 = d2x * (py - Y[0]) - d2y * (px - X[0]);
+                if (!edge_incl(E0, d0x, d0y) || !edge_incl(E1, d1x, d1y) || !edge_incl(E2, d2x, d2y))
+                    continue;
+                int64_t num = E0 * Z[0] + E1 * Z[1] + E2 * Z[2];
+                int64_t k = floordiv64(num - (int64_t)(VFO_SUB / 2) * area, (int64_t)VFO_SUB * area) + 1;
+                if (k < 0) k = 0;
+                if (k >= (int64_t)dims[2]) continue;
+                toggle[gidx(i, j, (size_t)k, dims)] ^= 1;
+            }
+        }
+    }
+
+    int64_t count = 0;
+    for (uint32_t x = 0; x < dims[0]; ++x)
+        for (uint32_t y = 0; y < dims[1]; ++y) {
+            uint8_t par = 0;
+            for (uint32_t z = 0; z < dims[2]; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                par ^= toggle[id];
+                uint8_t o = (uint8_t)(shell[id] | par);
+                occ[id] = o;
+                count += o;
+                if (shell_out) shell_out[id] = shell[id];
+                if (parity_out) parity_out[id] = par;
+            }
+        }
+    free(shell);
+    free(toggle);
+    return count;
+}
+
+/* ========================================================================= */
+/* Grid helpers                                                              */
+/* ========================================================================= */
+
+void vfo_occ_to_grid(const uint8_t* occ, uint16_t* grid, size_t n)
+{
+    /* RegularGrid::fill marks occupied cells VOXEL_FREE (RegularGrid.cpp:185-201) */
+    for (size_t i = 0; i < n; ++i) grid[i] = occ[i] ? VFO_FREE : VFO_EMPTY;
+}
+
+void vfo_homogenize(uint16_t* grid, size_t n)
+{
+    /* RegularGrid::homogenize (RegularGrid.cpp:533-541) */
+    for (size_t i = 0; i < n; ++i)
+        if (grid[i] != VFO_EMPTY) grid[i] = VFO_FREE;
+}
+
+void vfo_undo_mask(uint16_t* grid, size_t n)
+{
+    /* undoMask-comp.glsl:19-23 unmaskBit(.., 15) via RegularGrid::undoMask (RegularGrid.cpp:488-503) */
+    for (size_t i = 0; i < n; ++i) grid[i] = (uint16_t)(grid[i] & VFO_MASK15);
+}
+
+void vfo_unmask_rightmost(uint16_t* grid, size_t n, uint32_t position)
+{
+    /* undoMask-comp.glsl:25-29 unmaskRightMost */
+    for (size_t i = 0; i < n; ++i) grid[i] = (uint16_t)(grid[i] & (uint16_t)((1u << position) - 1u));
+}
+
+void vfo_detect_boundaries(const uint16_t* in, uint16_t* out, const uint32_t dims[3], int b)
+{
+    /* detectBoundaries-comp.glsl:18-43: a labelled cell is boundary when the
+     * clamped (2b+1)^3 window holds a value > FREE (bit 15 cleared) that differs
+     * from the cell's own raw value. */
+    const int nx = (int)dims[0], ny = (int)dims[1], nz = (int)dims[2];
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                uint16_t c = in[id];
+                out[id] = c;
+                if (c <= VFO_FREE) continue;
+                int x0 = clampi(x - b, 0, nx - 1), x1 = clampi(x + b, 0, nx - 1);
+                int y0 = clampi(y - b, 0, ny - 1), y1 = clampi(y + b, 0, ny - 1);
+                int z0 = clampi(z - b, 0, nz - 1), z1 = clampi(z + b, 0, nz - 1);
+                int bd = 0;
+                for (int xx = x0; xx <= x1 && !bd; ++xx)
+                    for (int yy = y0; yy <= y1 && !bd; ++yy)
+                        for (int zz = z0; zz <= z1 && !bd; ++zz) {
+                            uint16_t v = (uint16_t)(in[gidx(xx, yy, zz, dims)] & VFO_MASK15);
+                            if (v > VFO_FREE && v != c) bd = 1;
+                        }
+                if (bd) out[id] = (uint16_t)(c | 0x8000u);
+            }
+}
+
+void vfo_histogram(const uint16_t* grid, size_t n, uint32_t* counts)
+{
+    /* RegularGrid::countValues (RegularGrid.cpp:601-625): count unmask(value) for value > FREE */
+    memset(counts, 0, 65536 * sizeof(uint32_t));
+    for (size_t i = 0; i < n; ++i)
+        if (grid[i] > VFO_FREE) counts[grid[i] & VFO_MASK15]++;
+}
+
+/* ========================================================================= */
+/* Naive Voronoi fracture                                                    */
+/* ========================================================================= */
+
+static inline int64_t seed_dist(int64_t dx, int64_t dy, int64_t dz, int dist)
+{
+    /* NaiveFracturer distance lambdas (MF/Source/Fracturer/NaiveFracturer.cpp:12-23) and
+     * distance.glsl:1-20 in their exact integer form: squared Euclidean (sqrt is
+     * monotone and injective on these integers), L1 and L-infinity. */
+    if (dx < 0) dx = -dx;
+    if (dy < 0) dy = -dy;
+    if (dz < 0) dz = -dz;
+    if (dist == VFO_MANHATTAN) return dx + dy + dz;
+    if (dist == VFO_CHEBYSHEV) return dx > dy ? (dx > dz ? dx : dz) : (dy > dz ? dy : dz);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+void vfo_naive(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
+               const uint32_t* seeds, uint32_t S, int dist)
+{
+    /* NaiveFracturer::buildCPU (NaiveFracturer.cpp:26-68) / naiveFracturer-comp.glsl:19-42:
+     * every non-EMPTY cell takes the label of the first seed (list order) with the
+     * strictly smallest distance. */
+    for (uint32_t x = 0; x < dims[0]; ++x)
+        for (uint32_t y = 0; y < dims[1]; ++y)
+            for (uint32_t z = 0; z < dims[2]; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                out[id] = in[id];
+                if (in[id] == VFO_EMPTY) continue;
+                int64_t best = INT64_MAX;
+                for (uint32_t s = 0; s < S; ++s) {
+                    int64_t d = seed_dist((int64_t)x - seeds[4 * s], (int64_t)y - seeds[4 * s + 1],
+                                          (int64_t)z - seeds[4 * s + 2], dist);
+                    if (d < best) { best = d; out[id] = (uint16_t)seeds[4 * s + 3]; }
+                }
+            }
+}
+
+/* ========================================================================= */
+/* Flood (region-growing) fracture                                           */
+/* ========================================================================= */
+
+static inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16; h *= 0x85EBCA6Bu;
+    h ^= h >> 13; h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+uint32_t vfo_hash32(uint64_t key, uint32_t counter)
+{
+    /* counter-based hash: murmur3 finaliser applied twice, keyed by 64 bits */
+    uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    return fmix32(fmix32(counter ^ lo) + hi * 0x9E3779B1u + 0x7F4A7C15u);
+}
+
+uint32_t vfo_noise_delta(uint64_t noise_seed, uint32_t idx, uint32_t amp)
+{
+    return (uint32_t)(((uint64_t)vfo_hash32(noise_seed, idx) * (uint64_t)(amp + 1u)) >> 32);
+}
+
+static const int NB26[26][3] = {
+    /* FloodFracturer::MOORE order (FloodFracturer.cpp:13-27) */
+    {-1,-1,-1},{1,1,1},{-1,-1,0},{1,1,0},{-1,-1,1},{1,1,-1},{0,-1,-1},{0,1,1},{0,-1,0},{0,1,0},
+    {0,-1,1},{0,1,-1},{1,-1,-1},{-1,1,1},{1,-1,0},{-1,1,0},{1,-1,1},{-1,1,-1},{-1,0,-1},{1,0,1},
+    {-1,0,0},{1,0,0},{-1,0,1},{1,0,-1},{0,0,-1},{0,0,1}};
+static const int NB6[6][3] = {
+    /* FloodFracturer::VON_NEUMANN order (FloodFracturer.cpp:8-11) */
+    {1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+
+static int flood_validate(const uint32_t dims[3], const uint32_t* seeds, uint32_t S, int nb)
+{
+    if (nb != 6 && nb != 26) return -1;
+    for (uint32_t s = 0; s < S; ++s) {
+        if (seeds[4 * s] >= dims[0] || seeds[4 * s + 1] >= dims[1] || seeds[4 * s + 2] >= dims[2]) return -1;
+        if (seeds[4 * s + 3] < 2 || seeds[4 * s + 3] > 255) return -1;
+    }
+    return 0;
+}
+
+/* A seed acts as a source unless a later seed stamps the same cell
+ * (FloodFracturer.cpp:99-105 writes seeds in list order). */
+static uint8_t* flood_active_seeds(const uint32_t dims[3], const uint32_t* seeds, uint32_t S)
+{
+    uint8_t* act = (uint8_t*)malloc(S ? S : 1);
+    for (uint32_t s = 0; s < S; ++s) {
+        act[s] = 1;
+        for (uint32_t t = s + 1; t < S; ++t)
+            if (seeds[4 * t] == seeds[4 * s] && seeds[4 * t + 1] == seeds[4 * s + 1] &&
+                seeds[4 * t + 2] == seeds[4 * s + 2]) { act[s] = 0; break; }
+    }
+    (void)dims;
+    return act;
+}
+
+typedef struct { uint32_t* v; size_t n, cap; } u32vec;
+static void u32push(u32vec* a, uint32_t x)
+{
+    if (a->n == a->cap) { a->cap = a->cap ? 2 * a->cap : 64; a->v = (uint32_t*)realloc(a->v, a->cap * 4); }
+    a->v[a->n++] = x;
+}
+
+/*
+ * Deterministic flood.  The reference grows all seeds one BFS step at a time
+ * and resolves simultaneous claims by a race (floodFracturer-comp.glsl:24-63,
+ * FloodFracturer::build FloodFracturer.cpp:98-191).  The deterministic
+ * refinement computed here: label(v) = label of argmin over seeds s of
+ * (D_s(v), s), where D_s is the node-weighted geodesic distance in the 6- or
+ * 26-connected graph of occupied cells, weight(v) = 1 + noise_delta(v).  With
+ * zero noise this is one of the race's possible outcomes.  Unreached occupied
+ * cells stay FREE; the final unmaskRightMost keeps the low 8 bits.
+ * Computed by Dial's bucket algorithm over the lexicographic key (D, s).
+ */
+int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
+              const uint32_t* seeds, uint32_t S, int nb,
+              uint64_t noise_seed, uint32_t amp)
+{
+    if (flood_validate(dims, seeds, S, nb)) return -1;
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    const int (*off)[3] = nb == 6 ? NB6 : NB26;
+    uint64_t* key = (uint64_t*)malloc(V * sizeof(uint64_t));
+    for (size_t i = 0; i < V; ++i) key[i] = UINT64_MAX;
+    uint8_t* act = flood_active_seeds(dims, seeds, S);
+    u32vec* bucket = NULL;
+    size_t nbuckets = 0;
+#define BUCKET(d) (((d) >= nbuckets ? (bucket = (u32vec*)realloc(bucket, ((d) + 64) * sizeof(u32vec)), \
+                     memset(bucket + nbuckets, 0, ((d) + 64 - nbuckets) * sizeof(u32vec)), nbuckets = (d) + 64) : 0), &bucket[d])
+    for (uint32_t s = 0; s < S; ++s) {
+        if (!act[s]) continue;
+        size_t id = gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims);
+        key[id] = (uint64_t)s;
+        u32push(BUCKET(0), (uint32_t)id);
+    }
+    for (size_t d = 0; d < nbuckets; ++d) {
+        for (size_t e = 0; e < bucket[d].n; ++e) {
+            uint32_t v = bucket[d].v[e];
+            if ((key[v] >> 32) != d) continue;
+            uint64_t s = key[v] & 0xFFFFFFFFu;
+            int z = (int)(v % dims[2]), y = (int)((v / dims[2]) % dims[1]), x = (int)(v / ((size_t)dims[1] * dims[2]));
+            for (int k = 0; k < nb; ++k) {
+                int xx = x + off[k][0], yy = y + off[k][1], zz = z + off[k][2];
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)dims[0] || yy >= (int)dims[1] || zz >= (int)dims[2]) continue;
+                size_t u = gidx(xx, yy, zz, dims);
+                if (in[u] == VFO_EMPTY) continue;
+                uint64_t du = d + 1 + vfo_noise_delta(noise_seed, (uint32_t)u, amp);
+                uint64_t cand = (du << 32) | s;
+                if (cand < key[u]) {
+                    if ((key[u] >> 32) != du) u32push(BUCKET(du), (uint32_t)u);
+                    key[u] = cand;
+                }
+            }
+        }
+    }
+#undef BUCKET
+    for (size_t i = 0; i < V; ++i) {
+        if (key[i] != UINT64_MAX) out[i] = (uint16_t)(seeds[4 * (key[i] & 0xFFFFFFFFu) + 3] & 0xFFu);
+        else out[i] = in[i] != VFO_EMPTY ? VFO_FREE : VFO_EMPTY;
+    }
+    for (size_t d = 0; d < nbuckets; ++d) free(bucket[d].v);
+    free(bucket);
+    free(key);
+    free(act);
+    return 0;
+}
+
+/* binary heap of (dist, cell) for the brute-force check */
+typedef struct { uint64_t d; uint32_t v; } hent;
+static void hpush(hent** h, size_t* n, size_t* cap, hent e)
+{
+    if (*n == *cap) { *cap = *cap ? *cap * 2 : 256; *h = (hent*)realloc(*h, *cap * sizeof(hent)); }
+    size_t i = (*n)++;
+    while (i > 0) {
+        size_t p = (i - 1) / 2;
+        if ((*h)[p].d <= e.d) break;
+        (*h)[i] = (*h)[p];
+        i = p;
+    }
+    (*h)[i] = e;
+}
+static hent hpop(hent* h, size_t* n)
+{
+    hent top = h[0], last = h[--(*n)];
+    size_t i = 0;
+    for (;;) {
+        size_t l = 2 * i + 1, r = l + 1, m = i;
+        uint64_t md = last.d;
+        if (l < *n && h[l].d < md) { m = l; md = h[l].d; }
+        if (r < *n && h[r].d < md) { m = r; }
+        if (m == i) break;
+        h[i] = h[m];
+        i = m;
+    }
+    if (*n) h[i] = last;
+    return top;
+}
+
+int vfo_flood_dijkstra(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
+                       const uint32_t* seeds, uint32_t S, int nb,
+                       uint64_t noise_seed, uint32_t amp)
+{
+    if (flood_validate(dims, seeds, S, nb)) return -1;
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    const int (*off)[3] = nb == 6 ? NB6 : NB26;
+    uint64_t* best = (uint64_t*)malloc(V * sizeof(uint64_t));
+    uint32_t* bests = (uint32_t*)malloc(V * sizeof(uint32_t));
+    uint64_t* dist = (uint64_t*)malloc(V * sizeof(uint64_t));
+    for (size_t i = 0; i < V; ++i) { best[i] = UINT64_MAX; bests[i] = UINT32_MAX; }
+    uint8_t* act = flood_active_seeds(dims, seeds, S);
+    hent* heap = NULL;
+    size_t hn = 0, hcap = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        if (!act[s]) continue;
+        for (size_t i = 0; i < V; ++i) dist[i] = UINT64_MAX;
+        size_t src = gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims);
+        dist[src] = 0;
+        hn = 0;
+        hpush(&heap, &hn, &hcap, (hent){ 0, (uint32_t)src });
+        while (hn) {
+            hent e = hpop(heap, &hn);
+            if (e.d != dist[e.v]) continue;
+            int z = (int)(e.v % dims[2]), y = (int)((e.v / dims[2]) % dims[1]), x = (int)(e.v / ((size_t)dims[1] * dims[2]));
+            for (int k = 0; k < nb; ++k) {
+                int xx = x + off[k][0], yy = y + off[k][1], zz = z + off[k][2];
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)dims[0] || yy >= (int)dims[1] || zz >= (int)dims[2]) continue;
+                size_t u = gidx(xx, yy, zz, dims);
+                if (in[u] == VFO_EMPTY || u == src) continue;
+                uint64_t nd = e.d + 1 + vfo_noise_delta(noise_seed, (uint32_t)u, amp);
+                if (nd < dist[u]) { dist[u] = nd; hpush(&heap, &hn, &hcap, (hent){ nd, (uint32_t)u }); }
+            }
+        }
+        for (size_t i = 0; i < V; ++i)
+            if (dist[i] < best[i] || (dist[i] == best[i] && dist[i] != UINT64_MAX && s < bests[i])) {
+                best[i] = dist[i];
+                bests[i] = s;
+            }
+    }
+    for (size_t i = 0; i < V; ++i) {
+        if (bests[i] != UINT32_MAX) out[i] = (uint16_t)(seeds[4 * bests[i] + 3] & 0xFFu);
+        else out[i] = in[i] != VFO_EMPTY ? VFO_FREE : VFO_EMPTY;
+    }
+    free(heap); free(best); free(bests); free(dist); free(act);
+    return 0;
+}
+
+void vfo_remove_isolated(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
+                         const uint32_t* seeds, uint32_t S)
+{
+    /* NaiveFracturer::removeIsolatedRegionsCPU (NaiveFracturer.cpp:111-150): a FIFO
+     * BFS from the seeds over 6-neighbours holding the seed's label; cells it
+     * does not reach become EMPTY. */
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    memset(out, 0, V * sizeof(uint16_t));
+    u32vec q = { 0 }, ql = { 0 };
+    for (uint32_t s = 0; s < S; ++s) out[gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims)] = (uint16_t)seeds[4 * s + 3];
+    for (uint32_t s = 0; s < S; ++s) {
+        u32push(&q, (uint32_t)gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims));
+        u32push(&ql, seeds[4 * s + 3]);
+    }
+    for (size_t h = 0; h < q.n; ++h) {
+        uint32_t v = q.v[h], w = ql.v[h];
+        int z = (int)(v % dims[2]), y = (int)((v / dims[2]) % dims[1]), x = (int)(v / ((size_t)dims[1] * dims[2]));
+        for (int k = 0; k < 6; ++k) {
+            int xx = x + NB6[k][0], yy = y + NB6[k][1], zz = z + NB6[k][2];
+            if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)dims[0] || yy >= (int)dims[1] || zz >= (int)dims[2]) continue;
+            size_t u = gidx(xx, yy, zz, dims);
+            if (in[u] == (uint16_t)w && (uint32_t)in[u] == w && out[u] == VFO_EMPTY) {
+                out[u] = (uint16_t)w;
+                u32push(&q, (uint32_t)u);
+                u32push(&ql, w);
+            }
+        }
+    }
+    free(q.v);
+    free(ql.v);
+}
+
+/* ========================================================================= */
+/* Seeding                                                                   */
+/* ========================================================================= */
+
+static int is_surface(const uint16_t* g, const uint32_t* d, int x, int y, int z)
+{
+    /* RegularGrid::isBoundary(x, y, z, 1) (RegularGrid.cpp:543-558): an EMPTY cell in the clamped 3^3 window */
+    int x0 = clampi(x - 1, 0, (int)d[0] - 1), x1 = clampi(x + 1, 0, (int)d[0] - 1);
+    int y0 = clampi(y - 1, 0, (int)d[1] - 1), y1 = clampi(y + 1, 0, (int)d[1] - 1);
+    int z0 = clampi(z - 1, 0, (int)d[2] - 1), z1 = clampi(z + 1, 0, (int)d[2] - 1);
+    for (int a = x0; a <= x1; ++a)
+        for (int b = y0; b <= y1; ++b)
+            for (int c = z0; c <= z1; ++c)
+                if (g[gidx(a, b, c, d)] == VFO_EMPTY) return 1;
+    return 0;
+}
+
+static int cmp_u32(const void* a, const void* b)
+{
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+int vfo_seed_uniform(const uint16_t* grid, const uint32_t dims[3], uint32_t S,
+                     uint64_t seed, int location, uint32_t* seeds_out)
+{
+    /* Seeder::uniform (MF/Source/Fracturer/Seeder.cpp:154-208) with a counter-based
+     * draw: S distinct candidate cells (occupied; OUTER/INNER filter by
+     * isBoundary), uniform over candidate ranks, emitted in (x,y,z) order with
+     * labels 2, 3, ... */
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    uint32_t* cand = (uint32_t*)malloc(V * sizeof(uint32_t));
+    size_t O = 0;
+    for (uint32_t x = 0; x < dims[0]; ++x)
+        for (uint32_t y = 0; y < dims[1]; ++y)
+            for (uint32_t z = 0; z < dims[2]; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                if (grid[id] == VFO_EMPTY) continue;
+                if (location != VFO_SEED_BOTH) {
+                    int sf = is_surface(grid, dims, (int)x, (int)y, (int)z);
+                    if ((location == VFO_SEED_OUTER) != (sf != 0)) continue;
+                }
+                cand[O++] = (uint32_t)id;
+            }
+    if (O < S || O > 0xFFFFFFFFull) { free(cand); return -1; }
+    uint32_t* ranks = (uint32_t*)malloc((S ? S : 1) * sizeof(uint32_t));
+    uint32_t got = 0;
+    for (uint32_t j = 0; got < S; ++j) {
+        if (j == 1000000u) { free(cand); free(ranks); return -2; }
+        uint32_t r = (uint32_t)(((uint64_t)vfo_hash32(seed, j) * (uint64_t)O) >> 32);
+        int dup = 0;
+        for (uint32_t k = 0; k < got; ++k) if (ranks[k] == r) { dup = 1; break; }
+        if (!dup) ranks[got++] = r;
+    }
+    qsort(ranks, S, sizeof(uint32_t), cmp_u32);
+    for (uint32_t k = 0; k < S; ++k) {
+        uint32_t id = cand[ranks[k]];
+        seeds_out[4 * k + 0] = id / (dims[1] * dims[2]);
+        seeds_out[4 * k + 1] = (id / dims[2]) % dims[1];
+        seeds_out[4 * k + 2] = id % dims[2];
+        seeds_out[4 * k + 3] = 2 + k;
+    }
+    free(cand);
+    free(ranks);
+    return 0;
+}
+
+/* ========================================================================= */
+/* Marching cubes                                                            */
+/* ========================================================================= */
+
+/* cube corner offsets (marchingCubes-comp.glsl:28-38) and edge corner pairs (:40-54) */
+static const int MC_NB[8][3] = { {0,0,0},{0,0,1},{-1,0,1},{-1,0,0},{0,1,0},{0,1,1},{-1,1,1},{-1,1,0} };
+static const int MC_EDGE[12][2] = { {0,1},{1,2},{2,3},{3,0},{4,5},{5,6},{6,7},{7,4},{0,4},{1,5},{2,6},{3,7} };
+
+uint64_t vfo_mc_packed(uint32_t config) { return vfo_mc_tri_packed[config & 255u]; }
+
+typedef struct { uint64_t* v; size_t n, cap; } u64vec;
+static void u64push(u64vec* a, uint64_t x)
+{
+    if (a->n == a->cap) { a->cap = a->cap ? 2 * a->cap : 256; a->v = (uint64_t*)realloc(a->v, a->cap * 8); }
+    a->v[a->n++] = x;
+}
+static int cmp_u64(const void* a, const void* b)
+{
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/*
+ * Per-label marching cubes over the grid padded by one FREE cell
+ * (MarchingCubes::setGrid MarchingCubes.cpp:516-533, march() in
+ * marchingCubes-comp.glsl:96-156).  Labels are those countValues reports
+ * (RegularGrid::toTriangleMesh RegularGrid.cpp:443-486).  Field f = (cell &
+ * 0x7FFF) == label, isolevel 0.5, so each vertex is an exact edge midpoint;
+ * triangle corners are emitted in table order (0, 2, 1); the flag is bit 15
+ * of the cube's corner 0.  Output is canonical: per label, triangles in cube
+ * order then table order, and vertices welded exactly by lattice edge, in
+ * edge-id order.  World position = aabb_min + cell_size * (p - 1).
+ */
+int vfo_marching_cubes(const uint16_t* grid, const uint32_t dims[3],
+                       const float aabb_min[3], const float aabb_max[3],
+                       uint32_t want, vfo_mesh* m)
+{
+    memset(m, 0, sizeof(*m));
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    uint32_t* counts = (uint32_t*)malloc(65536 * sizeof(uint32_t));
+    vfo_histogram(grid, V, counts);
+    const int64_t N[3] = { (int64_t)dims[0] + 2, (int64_t)dims[1] + 2, (int64_t)dims[2] + 2 };
+    float cs[3];
+    double cs64[3];
+    for (int q = 0; q < 3; ++q) {
+        cs[q] = (aabb_max[q] - aabb_min[q]) / (float)dims[q];
+        cs64[q] = ((double)aabb_max[q] - (double)aabb_min[q]) / (double)dims[q];
+    }
+#define PADV(px, py, pz) (((px) <= 0 || (py) <= 0 || (pz) <= 0 || (px) >= N[0] - 1 || (py) >= N[1] - 1 || (pz) >= N[2] - 1) \
+                            ? (uint16_t)VFO_FREE : grid[gidx((px) - 1, (py) - 1, (pz) - 1, dims)])
+
+    u64vec tri_edges = { 0 }, tri_meta = { 0 }, verts = { 0 };
+    uint32_t nl = 0;
+    uint16_t* labels = (uint16_t*)malloc(65536 * sizeof(uint16_t));
+    uint32_t* voff = (uint32_t*)malloc(65537 * sizeof(uint32_t));
+    uint32_t* toff = (uint32_t*)malloc(65537 * sizeof(uint32_t));
+    voff[0] = toff[0] = 0;
+    for (uint32_t L = 0; L < 65536; ++L) {
+        if (!counts[L]) continue;
+        if (want != 0xFFFFFFFFu && want != L) continue;
+        labels[nl] = (uint16_t)L;
+        size_t tbegin = tri_meta.n;
+        for (int64_t cx = 1; cx <= N[0] - 1; ++cx)
+            for (int64_t cy = 0; cy <= N[1] - 2; ++cy)
+                for (int64_t cz = 0; cz <= N[2] - 2; ++cz) {
+                    int conf = 0;
+                    uint16_t c0 = 0;
+                    for (int i = 0; i < 8; ++i) {
+                        uint16_t raw = PADV(cx + MC_NB[i][0], cy + MC_NB[i][1], cz + MC_NB[i][2]);
+                        if (i == 0) c0 = raw;
+                        if ((uint32_t)(raw & VFO_MASK15) != L) conf |= 1 << i;
+                    }
+                    uint64_t w = vfo_mc_tri_packed[conf];
+                    int ntri = (int)(w >> 60);
+                    uint64_t cube = (uint64_t)((cx * N[1] + cy) * N[2] + cz);
+                    for (int t = 0; t < ntri; ++t) {
+                        int es[3] = { (int)((w >> (12 * t)) & 15), (int)((w >> (12 * t + 8)) & 15),
+                                      (int)((w >> (12 * t + 4)) & 15) }; /* corner order 0, 2, 1 */
+                        for (int k = 0; k < 3; ++k) {
+                            int a = MC_EDGE[es[k]][0], b = MC_EDGE[es[k]][1];
+                            int64_t pa[3] = { cx + MC_NB[a][0], cy + MC_NB[a][1], cz + MC_NB[a][2] };
+                            int64_t pb[3] = { cx + MC_NB[b][0], cy + MC_NB[b][1], cz + MC_NB[b][2] };
+                            int axis = pa[0] != pb[0] ? 0 : (pa[1] != pb[1] ? 1 : 2);
+                            int64_t lo[3] = { pa[0] < pb[0] ? pa[0] : pb[0], pa[1] < pb[1] ? pa[1] : pb[1],
+                                              pa[2] < pb[2] ? pa[2] : pb[2] };
+                            uint64_t eid = 3ull * (uint64_t)((lo[0] * N[1] + lo[1]) * N[2] + lo[2]) + (uint64_t)axis;
+                            u64push(&tri_edges, eid);
+                            u64push(&verts, eid);
+                        }
+                        u64push(&tri_meta, (cube << 1) | (uint64_t)((c0 >> 15) & 1u));
+                    }
+                }
+        /* exact weld: the label's vertices are its distinct lattice edges, in id order */
+        size_t vb = voff[nl];
+        qsort(verts.v + vb, verts.n - vb, sizeof(uint64_t), cmp_u64);
+        size_t u = vb;
+        for (size_t i = vb; i < verts.n; ++i)
+            if (i == vb || verts.v[i] != verts.v[u - 1]) verts.v[u++] = verts.v[i];
+        verts.n = u;
+        voff[nl + 1] = (uint32_t)verts.n;
+        toff[nl + 1] = (uint32_t)tri_meta.n;
+        (void)tbegin;
+        ++nl;
+    }
+    m->nl = nl;
+    m->nv = (uint32_t)verts.n;
+    m->nt = (uint32_t)tri_meta.n;
+    m->labels = labels;
+    m->voff = voff;
+    m->toff = toff;
+    m->edge = verts.v;
+    m->xyz = (float*)malloc((m->nv ? m->nv : 1) * 3 * sizeof(float));
+    m->xyz64 = (double*)malloc((m->nv ? m->nv : 1) * 3 * sizeof(double));
+    for (uint32_t i = 0; i < m->nv; ++i) {
+        uint64_t eid = verts.v[i];
+        int axis = (int)(eid % 3);
+        uint64_t p = eid / 3;
+        int64_t pz = (int64_t)(p % (uint64_t)N[2]), py = (int64_t)((p / (uint64_t)N[2]) % (uint64_t)N[1]),
+                px = (int64_t)(p / ((uint64_t)N[1] * (uint64_t)N[2]));
+        int64_t P[3] = { px, py, pz };
+        for (int q = 0; q < 3; ++q) {
+            float t = (float)P[q] - 1.0f + (q == axis ? 0.5f : 0.0f);
+            m->xyz[3 * i + q] = aabb_min[q] + cs[q] * t;
+            m->xyz64[3 * i + q] = (double)aabb_min[q] + cs64[q] * ((double)P[q] - 1.0 + (q == axis ? 0.5 : 0.0));
+        }
+    }
+    m->tri = (uint32_t*)malloc((m->nt ? m->nt : 1) * 3 * sizeof(uint32_t));
+    m->tri_flag = (uint8_t*)malloc(m->nt ? m->nt : 1);
+    m->tri_cube = (uint64_t*)malloc((m->nt ? m->nt : 1) * sizeof(uint64_t));
+    for (uint32_t l = 0; l < nl; ++l)
+        for (uint32_t t = toff[l]; t < toff[l + 1]; ++t) {
+            m->tri_flag[t] = (uint8_t)(tri_meta.v[t] & 1u);
+            m->tri_cube[t] = tri_meta.v[t] >> 1;
+            for (int k = 0; k < 3; ++k) {
+                uint64_t eid = tri_edges.v[3 * (size_t)t + k];
+                size_t lo = voff[l], hi = voff[l + 1];
+                while (lo < hi) {
+                    size_t mid = (lo + hi) / 2;
+                    if (verts.v[mid] < eid) lo = mid + 1; else hi = mid;
+                }
+                m->tri[3 * (size_t)t + k] = (uint32_t)(lo - voff[l]);
+            }
+        }
+    free(tri_edges.v);
+    free(tri_meta.v);
+    free(counts);
+#undef PADV
+    return 0;
+}
+
+void vfo_mesh_free(vfo_mesh* m)
+{
+    free(m->xyz); free(m->xyz64); free(m->edge); free(m->tri); free(m->tri_flag);
+    free(m->tri_cube); free(m->labels); free(m->voff); free(m->toff);
+    memset(m, 0, sizeof(*m));
+}
+
+/* ========================================================================= */
+/* RLE grid format                                                           */
+/* ========================================================================= */
+
+size_t vfo_rle_encode(const uint16_t* grid, const uint32_t dims[3], uint8_t* out, size_t cap)
+{
+    /* RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims, then maximal
+     * runs as packed little-endian {uint16 value, uint32 count}. */
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    size_t pos = 0;
+    if (cap >= 12) memcpy(out, dims, 12);
+    pos = 12;
+    size_t i = 0;
+    while (i < V) {
+        uint16_t v = grid[i];
+        size_t j = i;
+        while (j < V && grid[j] == v && j - i < 0xFFFFFFFFu) ++j;
+        uint32_t n = (uint32_t)(j - i);
+        if (pos + 6 <= cap) { memcpy(out + pos, &v, 2); memcpy(out + pos + 2, &n, 4); }
+        pos += 6;
+        i = j;
+    }
+    return pos;
+}
+
+int vfo_rle_decode(const uint8_t* in, size_t n, uint32_t dims_out[3], uint16_t* grid, size_t cap)
+{
+    /* docs/decompress/decompress_grid.py:16-32 */
+    if (n < 12 || (n - 12) % 6) return -1;
+    memcpy(dims_out, in, 12);
+    size_t V = (size_t)dims_out[0] * dims_out[1] * dims_out[2], pos = 0;
+    if (V > cap) return -2;
+    for (size_t o = 12; o < n; o += 6) {
+        uint16_t v; uint32_t c;
+        memcpy(&v, in + o, 2);
+        memcpy(&c, in + o + 2, 4);
+        if (pos + c > V) return -3;
+        for (uint32_t k = 0; k < c; ++k) grid[pos++] = v;
+    }
+    return pos == V ? 0 : -4;
+}

@@ -1,0 +1,628 @@
+// C ABI for single grids: RegularGrid lifetime, copies, voxelization, seeding,
+// fracture and the post-fracture stages.  Each entry point validates its
+// arguments on the host (so no kernel is ever launched on shapes it does not
+// support), launches the multi-job kernels with a one-job table, and reports
+// failures through vf_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "grid.hpp"
+
+namespace vf {
+
+constexpr uint32_t kMaxEdge = 2048;
+
+static void validate_dims(const uint32_t d[3])
+{
+    if (!d) fail(VF_ERR_INVALID_ARG, "dims is NULL");
+    for (int q = 0; q < 3; ++q)
+        if (d[q] == 0 || d[q] > kMaxEdge) fail(VF_ERR_INVALID_ARG, "grid edge %u out of range [1, %u]", d[q], kMaxEdge);
+    if ((uint64_t)d[0] * d[1] * d[2] >= (1ull << 31)) fail(VF_ERR_INVALID_ARG, "grid has 2^31 cells or more");
+}
+
+static void validate_box(const float mn[3], const float mx[3])
+{
+    if (!mn || !mx) fail(VF_ERR_INVALID_ARG, "aabb pointer is NULL");
+    for (int q = 0; q < 3; ++q)
+        if (!std::isfinite(mn[q]) || !std::isfinite(mx[q]) || !(mx[q] > mn[q]))
+            fail(VF_ERR_INVALID_ARG, "aabb axis %d is empty or not finite", q);
+}
+
+uint32_t grid_tiles(const vf_grid* g) { return ceil_div(g->V, kTileCells); }
+
+uint32_t key_bits_for(uint32_t S)
+{
+    uint32_t b = 1;
+    while ((1ull << b) < S) ++b;
+    return b;
+}
+
+static void grid_setup_shape(vf_grid* g, const uint32_t dims[3], const float mn[3], const float mx[3])
+{
+    for (int q = 0; q < 3; ++q) {
+        g->dims[q] = dims[q];
+        g->mn[q] = mn[q];
+        g->mx[q] = mx[q];
+        g->cs[q] = (mx[q] - mn[q]) / (float)dims[q];  // RegularGrid::setAABB, RegularGrid.cpp:426-441
+    }
+    g->V = (uint64_t)dims[0] * dims[1] * dims[2];
+    g->labels.ensure(g->V * 2 + 64);
+    g->occ.ensure(g->V + 64);
+    VF_HIP(hipMemsetAsync(g->labels.p, 0, g->V * 2, g->stream));
+    VF_HIP(hipMemsetAsync(g->occ.p, 0, g->V, g->stream));
+    g->labels_from_occ = false;
+    g->clean = true;
+    g->mc_valid = false;
+    JobDev& J = g->job;
+    J = JobDev{};
+    J.nx = dims[0];
+    J.ny = dims[1];
+    J.nz = dims[2];
+    J.wp = next_pow2(ceil_div(dims[2], 32));
+    J.V = g->V;
+    for (int q = 0; q < 3; ++q) {
+        J.mn[q] = g->mn[q];
+        J.mx[q] = g->mx[q];
+        J.cs[q] = g->cs[q];
+    }
+    J.labels = g->labels.as<uint16_t>();
+    J.occ = g->occ.as<uint8_t>();
+    J.stats = g->stats.as<unsigned long long>();
+    J.hist = g->hist.as<uint32_t>();
+    J.active = 1;
+    g->job_dirty = true;
+    uint32_t pre[2] = { 0, grid_tiles(g) };
+    VF_HIP(hipMemcpyAsync(g->prefix.p, pre, sizeof pre, hipMemcpyHostToDevice, g->stream));
+    VF_HIP(hipStreamSynchronize(g->stream));
+}
+
+Launch grid_launch(vf_grid* g)
+{
+    if (g->job_dirty) {
+        std::memcpy(g->pinned + 256, &g->job, sizeof(JobDev));
+        VF_HIP(hipMemcpyAsync(g->jobdev.p, g->pinned + 256, sizeof(JobDev), hipMemcpyHostToDevice, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+        g->job_dirty = false;
+    }
+    Launch L;
+    L.jobs = g->jobdev.as<JobDev>();
+    L.njobs = 1;
+    L.tile_prefix = g->prefix.as<uint32_t>();
+    L.total_tiles = grid_tiles(g);
+    L.s = g->stream;
+    return L;
+}
+
+void grid_materialize(vf_grid* g)
+{
+    if (!g->labels_from_occ) return;
+    launch_occ_to_labels(grid_launch(g));
+    g->labels_from_occ = false;
+}
+
+uint64_t read_stat(vf_grid* g, int slot)
+{
+    VF_HIP(hipMemcpyAsync(g->pinned, g->stats.as<unsigned long long>() + slot, 8, hipMemcpyDeviceToHost, g->stream));
+    VF_HIP(hipStreamSynchronize(g->stream));
+    uint64_t v;
+    std::memcpy(&v, g->pinned, 8);
+    return v;
+}
+
+void check_device_error(vf_grid* g, const char* stage)
+{
+    uint64_t e = read_stat(g, kStatError);
+    if (!e) return;
+    VF_HIP(hipMemsetAsync(g->stats.as<unsigned long long>() + kStatError, 0, 8, g->stream));
+    if (e & kDevErrSeeding) fail(VF_ERR_SEEDING, "%s: fewer candidate cells than seeds", stage);
+    if (e & kDevErrKeyOverflow) fail(VF_ERR_OVERFLOW, "%s: BFS level exceeds the packed key range", stage);
+    fail(VF_ERR_OVERFLOW, "%s: frontier bucket overflow", stage);
+}
+
+static void validate_seeds(const vf_grid* g, const vf_seed* seeds, uint32_t S)
+{
+    if (S && !seeds) fail(VF_ERR_INVALID_ARG, "seeds is NULL");
+    for (uint32_t s = 0; s < S; ++s)
+        if (seeds[s].x >= g->dims[0] || seeds[s].y >= g->dims[1] || seeds[s].z >= g->dims[2])
+            fail(VF_ERR_INVALID_ARG, "seed %u at (%u,%u,%u) lies outside the %ux%ux%u grid", s, seeds[s].x, seeds[s].y,
+                 seeds[s].z, g->dims[0], g->dims[1], g->dims[2]);
+}
+
+static void upload_seeds(vf_grid* g, const vf_seed* seeds, uint32_t S)
+{
+    g->seeds.ensure((size_t)std::max<uint32_t>(S, 1) * sizeof(uint4));
+    if (S) VF_HIP(hipMemcpyAsync(g->seeds.p, seeds, (size_t)S * sizeof(uint4), hipMemcpyHostToDevice, g->stream));
+    if (g->job.seeds != g->seeds.as<uint4>() || g->job.S != S) {
+        g->job.seeds = g->seeds.as<uint4>();
+        g->job.S = S;
+        g->job_dirty = true;
+    }
+    VF_HIP(hipStreamSynchronize(g->stream));
+}
+
+static vf_grid* checked(vf_grid* g)
+{
+    if (!g) fail(VF_ERR_INVALID_ARG, "grid is NULL");
+    VF_HIP(hipSetDevice(g->device));
+    return g;
+}
+
+}  // namespace vf
+
+using namespace vf;
+
+extern "C" {
+
+int vf_grid_create(const uint32_t dims[3], const float aabb_min[3], const float aabb_max[3], vf_grid** out)
+{
+    return guard([&] {
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        validate_dims(dims);
+        validate_box(aabb_min, aabb_max);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            (void)hipGetLastError();
+            fail(VF_ERR_DEVICE, "no HIP device available");
+        }
+        auto* g = new vf_grid();
+        try {
+            VF_HIP(hipGetDevice(&g->device));
+            VF_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+            VF_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pinned), 4096));
+            g->jobdev.ensure(sizeof(JobDev));
+            g->prefix.ensure(64);
+            g->stats.ensure(kStatCount * 8);
+            g->hist.ensure(65536 * 4);
+            g->ctl.ensure(256);
+            VF_HIP(hipMemsetAsync(g->stats.p, 0, kStatCount * 8, g->stream));
+            grid_setup_shape(g, dims, aabb_min, aabb_max);
+        } catch (...) {
+            vf_grid_destroy(g);
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int vf_grid_destroy(vf_grid* g)
+{
+    return guard([&] {
+        if (!g) return;
+        (void)hipSetDevice(g->device);
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        if (g->pinned) (void)hipHostFree(g->pinned);
+        if (g->stream) (void)hipStreamDestroy(g->stream);
+        delete g;
+    });
+}
+
+int vf_grid_set_aabb(vf_grid* g, const float aabb_min[3], const float aabb_max[3], const uint32_t dims[3])
+{
+    return guard([&] {
+        checked(g);
+        validate_dims(dims);
+        validate_box(aabb_min, aabb_max);
+        grid_setup_shape(g, dims, aabb_min, aabb_max);
+    });
+}
+
+int vf_grid_dims(const vf_grid* g, uint32_t dims[3])
+{
+    return guard([&] {
+        if (!g || !dims) fail(VF_ERR_INVALID_ARG, "NULL argument");
+        for (int q = 0; q < 3; ++q) dims[q] = g->dims[q];
+    });
+}
+
+int vf_grid_read(vf_grid* g, uint16_t* out)
+{
+    return guard([&] {
+        checked(g);
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        grid_materialize(g);
+        VF_HIP(hipMemcpyAsync(out, g->labels.p, g->V * 2, hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_grid_write(vf_grid* g, const uint16_t* in)
+{
+    return guard([&] {
+        checked(g);
+        if (!in) fail(VF_ERR_INVALID_ARG, "in is NULL");
+        VF_HIP(hipMemcpyAsync(g->labels.p, in, g->V * 2, hipMemcpyHostToDevice, g->stream));
+        g->labels_from_occ = false;
+        g->clean = false;
+        g->mc_valid = false;
+        launch_labels_to_occ(grid_launch(g));
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_occupancy_read(vf_grid* g, uint8_t* out)
+{
+    return guard([&] {
+        checked(g);
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        VF_HIP(hipMemcpyAsync(out, g->occ.p, g->V, hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_grid_count_occupied(vf_grid* g, uint64_t* out)
+{
+    return guard([&] {
+        checked(g);
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        VF_HIP(hipMemsetAsync(g->stats.as<unsigned long long>() + kStatOccupied, 0, 8, g->stream));
+        launch_count_occupied(grid_launch(g));
+        *out = read_stat(g, kStatOccupied);
+    });
+}
+
+int vf_grid_device_ptr(vf_grid* g, void** ptr)
+{
+    return guard([&] {
+        checked(g);
+        if (!ptr) fail(VF_ERR_INVALID_ARG, "ptr is NULL");
+        grid_materialize(g);
+        VF_HIP(hipStreamSynchronize(g->stream));
+        *ptr = g->labels.p;
+    });
+}
+
+int vf_grid_synchronize(vf_grid* g)
+{
+    return guard([&] {
+        checked(g);
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32_t* d_tri, uint32_t nt)
+{
+    return guard([&] {
+        checked(g);
+        if (nt && (!d_xyz || !d_tri)) fail(VF_ERR_INVALID_ARG, "mesh pointer is NULL");
+        JobDev& J = g->job;
+        const uint64_t words = (uint64_t)J.nx * J.ny * J.wp;
+        g->bits.ensure(words * 8 + 64);
+        J.shell_bits = g->bits.as<uint32_t>();
+        J.toggle_bits = g->bits.as<uint32_t>() + words;
+        J.xyz = d_xyz;
+        J.tri = d_tri;
+        J.nt = nt;
+        // RegularGrid::fill only sets occupied cells to FREE; on a grid that
+        // already holds values the new occupancy is merged afterwards.
+        const bool merge = !g->clean;
+        if (merge) {
+            g->occ_tmp.ensure(g->V + 64);
+            J.occ = g->occ_tmp.as<uint8_t>();
+        }
+        g->job_dirty = true;
+        const uint32_t groups = ceil_div(words, 256);
+        uint32_t pre[4] = { 0, nt, 0, groups };
+        g->vox_setup.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup));
+        g->vox_counts.ensure((size_t)std::max<uint32_t>(nt, 1) * 8);
+        g->vox_shell_off.ensure(((size_t)nt + 1) * 8);
+        g->vox_col_off.ensure(((size_t)nt + 1) * 8);
+        g->scan_tmp.ensure(scan_tmp_bytes(nt) + 64);
+        g->ctl.ensure(256);
+        std::memcpy(g->pinned + 512, pre, sizeof pre);
+        VF_HIP(hipMemcpyAsync(g->ctl.p, g->pinned + 512, sizeof pre, hipMemcpyHostToDevice, g->stream));
+        Launch L = grid_launch(g);
+        VoxScratch X;
+        X.setup = g->vox_setup.as<TriSetup>();
+        X.counts = g->vox_counts.as<uint32_t>();
+        X.shell_off = g->vox_shell_off.as<unsigned long long>();
+        X.col_off = g->vox_col_off.as<unsigned long long>();
+        X.tri_prefix = g->ctl.as<uint32_t>();
+        X.total_tris = nt;
+        X.scan_tmp = g->scan_tmp.p;
+        X.scan_tmp_bytes = g->scan_tmp.bytes;
+        launch_voxelize(L, X, g->ctl.as<uint32_t>() + 2, groups);
+        (void)nv;
+        if (merge) {
+            // labels = new ? FREE : old, occupancy = old | new
+            std::vector<uint16_t> old(g->V);
+            std::vector<uint8_t> nw(g->V);
+            grid_materialize(g);
+            VF_HIP(hipMemcpyAsync(old.data(), g->labels.p, g->V * 2, hipMemcpyDeviceToHost, g->stream));
+            VF_HIP(hipMemcpyAsync(nw.data(), g->occ_tmp.p, g->V, hipMemcpyDeviceToHost, g->stream));
+            VF_HIP(hipStreamSynchronize(g->stream));
+            for (uint64_t i = 0; i < g->V; ++i)
+                if (nw[i]) old[i] = kFree;
+            J.occ = g->occ.as<uint8_t>();
+            g->job_dirty = true;
+            VF_HIP(hipMemcpyAsync(g->labels.p, old.data(), g->V * 2, hipMemcpyHostToDevice, g->stream));
+            launch_labels_to_occ(grid_launch(g));
+            g->labels_from_occ = false;
+        } else {
+            g->labels_from_occ = true;
+        }
+        g->clean = false;
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_voxelize(vf_grid* g, const float* xyz, uint32_t nv, const uint32_t* tri, uint32_t nt)
+{
+    return guard([&] {
+        checked(g);
+        if (nt && (!xyz || !tri)) fail(VF_ERR_INVALID_ARG, "mesh pointer is NULL");
+        for (uint64_t i = 0; i < 3ull * nt; ++i)
+            if (tri[i] >= nv) fail(VF_ERR_INVALID_ARG, "triangle %llu references vertex %u of %u", (unsigned long long)(i / 3), tri[i], nv);
+        g->mesh_xyz.ensure((size_t)std::max<uint32_t>(nv, 1) * 12);
+        g->mesh_tri.ensure((size_t)std::max<uint32_t>(nt, 1) * 12);
+        if (nv) VF_HIP(hipMemcpyAsync(g->mesh_xyz.p, xyz, (size_t)nv * 12, hipMemcpyHostToDevice, g->stream));
+        if (nt) VF_HIP(hipMemcpyAsync(g->mesh_tri.p, tri, (size_t)nt * 12, hipMemcpyHostToDevice, g->stream));
+        int rc = vf_voxelize_device(g, g->mesh_xyz.as<float>(), nv, g->mesh_tri.as<uint32_t>(), nt);
+        if (rc != VF_OK) throw Exception{ rc };
+    });
+}
+
+int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed* out)
+{
+    return guard([&] {
+        checked(g);
+        if (S && !out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        if (location != VF_SEED_BOTH && location != VF_SEED_OUTER && location != VF_SEED_INNER)
+            fail(VF_ERR_INVALID_ARG, "unknown seed location %d", location);
+        if (S > kMaxDeviceSeeds) fail(VF_ERR_INVALID_ARG, "at most %u seeds can be drawn", kMaxDeviceSeeds);
+        if (S == 0) return;
+        g->seeds.ensure((size_t)S * sizeof(uint4));
+        g->job.seeds = g->seeds.as<uint4>();
+        g->job.S = S;
+        g->job.seed_key = seed;
+        g->job.seed_location = location;
+        g->job_dirty = true;
+        const uint32_t tiles = grid_tiles(g);
+        g->seed_tile_cnt.ensure((size_t)tiles * 4 + 64);
+        g->seed_tile_off.ensure(((size_t)tiles + 1) * 8 + 64);
+        g->scan_tmp.ensure(scan_tmp_bytes(tiles) + 64);
+        Launch L = grid_launch(g);
+        launch_seed_count(L, g->seed_tile_cnt.as<uint32_t>());
+        scan_exclusive_u32_to_u64(g->seed_tile_cnt.as<uint32_t>(), g->seed_tile_off.as<unsigned long long>(), tiles,
+                                  g->scan_tmp.p, g->stream);
+        launch_seed_pick(L, g->seed_tile_off.as<unsigned long long>());
+        check_device_error(g, "seed_uniform");
+        VF_HIP(hipMemcpyAsync(out, g->seeds.p, (size_t)S * sizeof(uint4), hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance)
+{
+    return guard([&] {
+        checked(g);
+        if (distance < VF_EUCLIDEAN || distance > VF_CHEBYSHEV) fail(VF_ERR_INVALID_ARG, "unknown distance %d", distance);
+        validate_seeds(g, seeds, S);
+        if (S == 0) return;  // buildCPU leaves every cell untouched without seeds
+        upload_seeds(g, seeds, S);
+        const uint32_t b = key_bits_for(S);
+        const uint64_t ex = g->dims[0] - 1, ey = g->dims[1] - 1, ez = g->dims[2] - 1;
+        uint64_t dmax = distance == VF_MANHATTAN ? ex + ey + ez
+                        : distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
+                                                   : ex * ex + ey * ey + ez * ez;
+        const bool key64 = ((dmax + 1) << b) > 0xFFFFFFFFull;
+        g->job.key_bits = b;
+        g->job.dist = distance;
+        g->job_dirty = true;
+        const bool fast = (g->dims[2] % 8) == 0;
+        launch_naive_shaped(grid_launch(g), distance, fast, key64);
+        g->labels_from_occ = false;
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_remove_isolated_regions(vf_grid* g, const vf_seed* seeds, uint32_t S)
+{
+    return guard([&] {
+        checked(g);
+        validate_seeds(g, seeds, S);
+        if (S > g->V) fail(VF_ERR_INVALID_ARG, "more seeds than cells");
+        upload_seeds(g, seeds, S);
+        grid_materialize(g);
+        g->ri_a.ensure(g->V * 8 + 64);
+        g->ri_b.ensure(g->V * 8 + 64);
+        g->ri_cnt.ensure(64);
+        g->ri_vis.ensure((g->V + 31) / 32 * 4 + 64);
+        g->ri_out.ensure(g->V * 2 + 64);
+        launch_remove_isolated(g->stream, g->labels.as<uint16_t>(), g->ri_out.as<uint16_t>(), g->ri_a.as<uint32_t>(),
+                               g->ri_b.as<uint32_t>(), g->ri_cnt.as<uint32_t>(), g->ri_vis.as<uint32_t>(), g->dims,
+                               g->seeds.as<uint4>(), S, g->pinned);
+        VF_HIP(hipMemcpyAsync(g->labels.p, g->ri_out.p, g->V * 2, hipMemcpyDeviceToDevice, g->stream));
+        launch_labels_to_occ(grid_launch(g));
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance, int neighbourhood, uint64_t noise_seed,
+                      uint32_t noise_amp)
+{
+    return guard([&] {
+        checked(g);
+        if (distance < VF_EUCLIDEAN || distance > VF_CHEBYSHEV) fail(VF_ERR_INVALID_ARG, "unknown distance %d", distance);
+        int nb = neighbourhood;
+        if (nb == VF_NEIGH_FROM_DISTANCE) nb = distance == VF_MANHATTAN ? 6 : 26;  // FloodFracturer.cpp:114
+        if (nb != 6 && nb != 26) fail(VF_ERR_INVALID_ARG, "neighbourhood must be 0, 6 or 26");
+        if (noise_amp > kMaxNoiseAmp) fail(VF_ERR_INVALID_ARG, "noise amplitude %u above %u", noise_amp, kMaxNoiseAmp);
+        validate_seeds(g, seeds, S);
+        for (uint32_t s = 0; s < S; ++s)
+            if (seeds[s].w < 2 || seeds[s].w > 255)
+                fail(VF_ERR_TOO_MANY_SEEDS, "flood label %u outside 2..255 (8-bit fragment id, Seeder.h:12)", seeds[s].w);
+        upload_seeds(g, seeds, S);
+        // bucket capacity: every cell is pushed at most once
+        uint64_t occupied = 0;
+        int rc = vf_grid_count_occupied(g, &occupied);
+        if (rc != VF_OK) throw Exception{ rc };
+        const uint64_t cap = occupied + S + 64;
+        JobDev& J = g->job;
+        g->keys.ensure(g->V * 4 + 64);
+        g->buckets.ensure(cap * 4 * kLevelRing);
+        g->fctl.ensure(64 * 4);
+        J.keys = g->keys.as<uint32_t>();
+        J.buckets = g->buckets.as<uint32_t>();
+        J.bucket_cap = (uint32_t)cap;
+        J.fctl = g->fctl.as<uint32_t>();
+        J.key_bits = key_bits_for(S);
+        J.neighbourhood = nb;
+        J.noise_seed = noise_seed;
+        J.noise_amp = noise_amp;
+        g->job_dirty = true;
+        Launch L = grid_launch(g);
+        VF_HIP(hipMemsetAsync(g->fctl.as<uint32_t>() + 32, 0, 4, g->stream));  // blocks_done
+        launch_flood_init(L);
+        launch_flood_seed(L, S);
+        uint32_t chunk = 4, levels = 0;
+        for (;;) {
+            for (uint32_t k = 0; k < chunk; ++k) launch_flood_level(L, g->fctl.as<uint32_t>() + 32, 512);
+            levels += chunk;
+            VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
+            VF_HIP(hipStreamSynchronize(g->stream));
+            if (g->pinned[1] != 0) break;
+            chunk = std::min<uint32_t>(chunk * 2, 64);
+            // a cell's distance is at most 3 x (cells on its path), so the BFS must drain by then
+            if ((uint64_t)levels > 3ull * (g->V + S) + 16) fail(VF_ERR_OVERFLOW, "flood did not drain");
+        }
+        g->last_flood_levels = g->pinned[1];
+        check_device_error(g, "flood");
+        launch_flood_finish(L);
+        g->labels_from_occ = false;
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_detect_boundaries(vf_grid* g, int size)
+{
+    return guard([&] {
+        checked(g);
+        if (size < 0 || size > 64) fail(VF_ERR_INVALID_ARG, "boundary size %d out of range", size);
+        grid_materialize(g);
+        launch_boundaries(grid_launch(g), size, 0);
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_undo_mask(vf_grid* g)
+{
+    return guard([&] {
+        checked(g);
+        grid_materialize(g);
+        launch_undo_mask(grid_launch(g), 0x7FFF);
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_homogenize(vf_grid* g)
+{
+    return guard([&] {
+        checked(g);
+        grid_materialize(g);
+        launch_homogenize(grid_launch(g));
+        g->mc_valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_histogram(vf_grid* g, uint32_t* counts, uint32_t ncounts)
+{
+    return guard([&] {
+        checked(g);
+        if (ncounts && !counts) fail(VF_ERR_INVALID_ARG, "counts is NULL");
+        grid_materialize(g);
+        Launch L = grid_launch(g);
+        launch_clear_hist(L);
+        launch_histogram(L);
+        const uint32_t n = std::min<uint32_t>(ncounts, 65536);
+        if (n) VF_HIP(hipMemcpyAsync(counts, g->hist.p, (size_t)n * 4, hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+        for (uint32_t i = n; i < ncounts; ++i) counts[i] = 0;
+    });
+}
+
+int vf_marching_cubes(vf_grid* g, uint32_t label, vf_mesh_info* info)
+{
+    return guard([&] {
+        checked(g);
+        if (!info) fail(VF_ERR_INVALID_ARG, "info is NULL");
+        grid_materialize(g);
+        grid_marching_cubes(g, label, info);
+    });
+}
+
+int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* vert_offsets,
+                 uint32_t* tri_offsets)
+{
+    return guard([&] {
+        checked(g);
+        if (!g->mc_valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes result: call vf_marching_cubes first");
+        grid_mesh_read(g, xyz, tri, tri_flag, labels, vert_offsets, tri_offsets);
+    });
+}
+
+int vf_grid_export_rle(vf_grid* g, uint8_t* out, size_t cap, size_t* size)
+{
+    // RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims, then maximal runs
+    return guard([&] {
+        checked(g);
+        std::vector<uint16_t> h(g->V);
+        int rc = vf_grid_read(g, h.data());
+        if (rc != VF_OK) throw Exception{ rc };
+        size_t pos = 0;
+        auto put = [&](const void* p, size_t n) {
+            if (out && pos + n <= cap) std::memcpy(out + pos, p, n);
+            pos += n;
+        };
+        put(g->dims, 12);
+        uint64_t i = 0;
+        while (i < g->V) {
+            uint16_t v = h[i];
+            uint64_t j = i;
+            while (j < g->V && h[j] == v && j - i < 0xFFFFFFFFull) ++j;
+            uint32_t n = (uint32_t)(j - i);
+            put(&v, 2);
+            put(&n, 4);
+            i = j;
+        }
+        if (size) *size = pos;
+    });
+}
+
+int vf_grid_import_rle(vf_grid* g, const uint8_t* in, size_t n)
+{
+    return guard([&] {
+        checked(g);
+        if (!in || n < 12 || (n - 12) % 6) fail(VF_ERR_INVALID_ARG, "RLE stream is truncated");
+        uint32_t d[3];
+        std::memcpy(d, in, 12);
+        if (d[0] != g->dims[0] || d[1] != g->dims[1] || d[2] != g->dims[2])
+            fail(VF_ERR_INVALID_ARG, "RLE dims %ux%ux%u differ from the grid's", d[0], d[1], d[2]);
+        std::vector<uint16_t> h(g->V);
+        uint64_t pos = 0;
+        for (size_t o = 12; o < n; o += 6) {
+            uint16_t v;
+            uint32_t c;
+            std::memcpy(&v, in + o, 2);
+            std::memcpy(&c, in + o + 2, 4);
+            if (pos + c > g->V) fail(VF_ERR_INVALID_ARG, "RLE runs exceed the grid size");
+            std::fill(h.begin() + (ptrdiff_t)pos, h.begin() + (ptrdiff_t)(pos + c), v);
+            pos += c;
+        }
+        if (pos != g->V) fail(VF_ERR_INVALID_ARG, "RLE runs cover %llu of %llu cells", (unsigned long long)pos,
+                              (unsigned long long)g->V);
+        int rc = vf_grid_write(g, h.data());
+        if (rc != VF_OK) throw Exception{ rc };
+    });
+}
+
+}  // extern "C"

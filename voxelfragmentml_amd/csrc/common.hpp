@@ -1,0 +1,136 @@
+// Shared host/device infrastructure for libvfrag: error propagation across the
+// C ABI, HIP error checks, per-kernel event timing, and the small device
+// helpers (index math, counter hash) every stage uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <new>
+#include <string>
+
+#include "vfrag.h"
+
+namespace vf {
+
+// ---------------------------------------------------------------- errors ---
+struct Exception {
+    int code;
+};
+
+void set_error(const char* fmt, ...);
+[[noreturn]] void fail(int code, const char* fmt, ...);
+[[noreturn]] void fail_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define VF_HIP(call)                                                  \
+    do {                                                              \
+        hipError_t vf_e_ = (call);                                    \
+        if (vf_e_ != hipSuccess) ::vf::fail_hip(vf_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define VF_CHECK_LAUNCH() VF_HIP(hipGetLastError())
+
+// Runs f, converting library exceptions into status codes for the C ABI.
+template <class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return VF_OK;
+    } catch (const Exception& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return VF_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        set_error("unexpected C++ exception");
+        return VF_ERR_INVALID_ARG;
+    }
+}
+
+// ---------------------------------------------------------------- timing ---
+// When enabled, launches wrapped in TimedLaunch are bracketed by HIP events
+// recorded on the launch stream; totals accumulate per kernel name.
+bool timing_enabled();
+void timing_begin(const char* name, hipStream_t s);
+void timing_end(const char* name, hipStream_t s);
+bool capturing(hipStream_t s);
+
+struct TimedLaunch {
+    const char* name;
+    hipStream_t s;
+    bool on;
+    TimedLaunch(const char* n, hipStream_t st) : name(n), s(st), on(timing_enabled() && !capturing(st))
+    {
+        if (on) timing_begin(name, s);
+    }
+    ~TimedLaunch()
+    {
+        if (on) timing_end(name, s);
+    }
+};
+
+// ------------------------------------------------------------- device mem ---
+// A grow-only device buffer owned by a grid / batch.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint32_t next_pow2(uint32_t v)
+{
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// ------------------------------------------------------------ device math ---
+__host__ __device__ inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Counter-based hash: murmur3's finaliser applied twice, keyed by 64 bits.
+// Drives the flood boundary noise and the seed draws.
+__host__ __device__ inline uint32_t hash32(uint64_t key, uint32_t counter)
+{
+    uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    return fmix32(fmix32(counter ^ lo) + hi * 0x9E3779B1u + 0x7F4A7C15u);
+}
+
+// Uniform integer in [0, amp] from the cell's hash (multiply-shift range map).
+__host__ __device__ inline uint32_t noise_delta(uint64_t seed, uint32_t idx, uint32_t amp)
+{
+    return (uint32_t)(((uint64_t)hash32(seed, idx) * (uint64_t)(amp + 1u)) >> 32);
+}
+
+constexpr uint16_t kEmpty = 0;
+constexpr uint16_t kFree = 1;
+constexpr uint16_t kMask15 = 0x7FFF;
+
+// Pick the per-launch grid for a grid-stride kernel.
+inline uint32_t stride_blocks(uint64_t items, uint32_t threads, uint32_t cap = 4096)
+{
+    uint64_t b = (items + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (uint32_t)b;
+}
+
+}  // namespace vf

@@ -1,0 +1,303 @@
+// Flood (region-growing) fracture: FloodFracturer::build
+// (MF/Source/Fracturer/FloodFracturer.cpp:98-191, floodFracturer-comp.glsl:24-63,
+// undoMask-comp.glsl unmaskRightMost).
+//
+// The reference grows every seed one BFS step per dispatch and resolves
+// simultaneous claims by a race.  Here every cell gets the label of the seed
+// minimising (D, seed index), D the node-weighted geodesic distance in the 6-
+// or 26-neighbour graph of occupied cells with weight 1 + noise(cell) in
+// {1, 2, 3}.  Because weights belong to the target cell, the first level that
+// reaches a cell fixes its D, and all claims of that level race only on the
+// seed index, which atomicMin on the packed key (D << b | index) settles
+// exactly.  So the result is deterministic and equals a brute-force
+// per-seed Dijkstra.
+//
+// Levels are processed level-synchronously with a ring of 4 bucket arrays
+// (D, D+1, D+2, D+3 are the only live levels).  The level kernel is
+// self-advancing: its last block bumps each job's level, resets the counter
+// slot that level D+4 will use and marks drained jobs, so the host (or a
+// captured graph) simply replays the same launch.
+#include "cellops.hpp"
+
+namespace vf {
+
+namespace {
+
+constexpr uint32_t kFloodThreads = 256;
+constexpr uint32_t kEmptyKey = 0u;
+constexpr uint32_t kUnreached = 0xFFFFFFFFu;
+
+__constant__ int c_nb26[26][3] = {
+    { -1, -1, -1 }, { 1, 1, 1 }, { -1, -1, 0 }, { 1, 1, 0 }, { -1, -1, 1 }, { 1, 1, -1 }, { 0, -1, -1 },
+    { 0, 1, 1 },    { 0, -1, 0 }, { 0, 1, 0 },  { 0, -1, 1 }, { 0, 1, -1 }, { 1, -1, -1 }, { -1, 1, 1 },
+    { 1, -1, 0 },   { -1, 1, 0 }, { 1, -1, 1 }, { -1, 1, -1 }, { -1, 0, -1 }, { 1, 0, 1 }, { -1, 0, 0 },
+    { 1, 0, 0 },    { -1, 0, 1 }, { 1, 0, -1 }, { 0, 0, -1 }, { 0, 0, 1 }
+};
+__constant__ int c_nb6[6][3] = { { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
+
+__global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
+{
+    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
+        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J->V) continue;
+        const uint64_t o = load8b(J->occ, c0, J->V);
+        if (c0 + 8 <= J->V) {
+            uint4 a, b;
+            a.x = (o & 0xFFull) ? kUnreached : kEmptyKey;
+            a.y = (o & 0xFF00ull) ? kUnreached : kEmptyKey;
+            a.z = (o & 0xFF0000ull) ? kUnreached : kEmptyKey;
+            a.w = (o & 0xFF000000ull) ? kUnreached : kEmptyKey;
+            b.x = (o & 0xFF00000000ull) ? kUnreached : kEmptyKey;
+            b.y = (o & 0xFF0000000000ull) ? kUnreached : kEmptyKey;
+            b.z = (o & 0xFF000000000000ull) ? kUnreached : kEmptyKey;
+            b.w = (o & 0xFF00000000000000ull) ? kUnreached : kEmptyKey;
+            reinterpret_cast<uint4*>(J->keys + c0)[0] = a;
+            reinterpret_cast<uint4*>(J->keys + c0)[1] = b;
+        } else {
+            for (int k = 0; k < 8; ++k)
+                if (c0 + k < J->V) J->keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
+        }
+    }
+}
+
+// One block per job: stamp the seeds (a later seed on the same cell wins, as
+// FloodFracturer.cpp:99-105 writes them in list order), mark seed cells
+// occupied, and push level 1 (level 0 is never used so that 0 can mean EMPTY).
+__global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
+{
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active) return;
+    const uint32_t S = J->S, b = J->key_bits;
+    for (uint32_t i = threadIdx.x; i < kFloodCtlWords; i += blockDim.x) J->fctl[i] = 0;
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        atomicMax(J->keys + ((size_t)q.x * J->ny + q.y) * J->nz + q.z, (1u << b) | s);
+    }
+    __syncthreads();
+    __shared__ uint32_t npush;
+    if (threadIdx.x == 0) npush = 0;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
+        if (J->keys[c] == ((1u << b) | s)) {
+            const uint32_t slot = atomicAdd(&npush, 1u);
+            J->buckets[(size_t)(1 % kLevelRing) * J->bucket_cap + slot] = (uint32_t)c;
+            J->occ[c] = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        J->fctl[0] = 1;
+        J->fctl[1] = npush ? 0u : 1u;
+        J->fctl[2 + 1 % 8] = npush;
+    }
+}
+
+// Wave-aggregated append of cell u to bucket (job j, level du).  All lanes of
+// the wave call it; `want` marks lanes with something to push.
+__device__ inline void push_aggregated(bool want, const JobDev* jobs, uint32_t j, uint32_t du, uint32_t u)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint64_t pending = __ballot(want);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1;
+        const uint32_t lj = __shfl(j, leader, 64), ld = __shfl(du, leader, 64);
+        const bool mine = want && j == lj && du == ld;
+        const uint64_t grp = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(jobs[lj].fctl + 2 + (ld & 7u), (uint32_t)__popcll(grp));
+        base = __shfl(base, leader, 64);
+        if (mine) {
+            const JobDev* J = jobs + j;
+            const uint32_t slot = base + (uint32_t)__popcll(grp & lt);
+            if (slot < J->bucket_cap) J->buckets[(size_t)(du % kLevelRing) * J->bucket_cap + slot] = u;
+            else atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrBucket);
+            want = false;
+        }
+        pending &= ~grp;
+    }
+}
+
+__global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_t* blocks_done)
+{
+    __shared__ uint32_t pre[kFloodMaxJobs + 1];
+    __shared__ uint32_t lev[kFloodMaxJobs];
+    __shared__ uint32_t wsum[kFloodThreads / 64];
+    const uint32_t nj = L.njobs;
+    // per-job frontier sizes and their prefix (every block computes it)
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nj; base += kFloodThreads) {
+        const uint32_t j = base + threadIdx.x;
+        uint32_t n = 0;
+        if (j < nj) {
+            const JobDev* J = L.jobs + j;
+            const uint32_t d = J->fctl[0];
+            lev[j] = d;
+            n = (J->active && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
+            if (n > J->bucket_cap) n = J->bucket_cap;
+        }
+        // block exclusive scan of n
+        uint32_t x = n;
+        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        uint32_t wofs = 0, tot = 0;
+        for (uint32_t w = 0; w < kFloodThreads / 64; ++w) {
+            if (w < wave) wofs += wsum[w];
+            tot += wsum[w];
+        }
+        if (j < nj) pre[j] = carry + wofs + x - n;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pre[nj] = carry;
+    __syncthreads();
+    const uint32_t total = pre[nj];
+
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // iterate so that every lane of a wave runs the same number of rounds
+    for (uint32_t e0 = blockIdx.x * blockDim.x; e0 < total; e0 += stride) {
+        const uint32_t e = e0 + threadIdx.x;
+        const bool have = e < total;
+        uint32_t j = 0, v = 0, d = 0, idx = 0, b = 0;
+        const JobDev* J = L.jobs;
+        if (have) {
+            uint32_t lo = 0, hi = nj;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pre[mid] <= e) lo = mid; else hi = mid;
+            }
+            j = lo;
+            J = L.jobs + j;
+            d = lev[j];
+            v = J->buckets[(size_t)(d % kLevelRing) * J->bucket_cap + (e - pre[j])];
+            b = J->key_bits;
+            idx = J->keys[v] & ((1u << b) - 1u);
+        }
+        uint32_t x = 0, y = 0, z = 0;
+        if (have) {
+            const uint32_t yz = J->ny * J->nz;
+            x = v / yz;
+            const uint32_t r = v - x * yz;
+            y = r / J->nz;
+            z = r - y * J->nz;
+        }
+        const int nb = have ? J->neighbourhood : 0;
+        const int nbmax = 26;
+        for (int k = 0; k < nbmax; ++k) {
+            bool want = false;
+            uint32_t du = 0, u = 0;
+            if (k < nb) {
+                const int* o = nb == 6 ? c_nb6[k] : c_nb26[k];
+                const int xx = (int)x + o[0], yy = (int)y + o[1], zz = (int)z + o[2];
+                if (xx >= 0 && yy >= 0 && zz >= 0 && xx < (int)J->nx && yy < (int)J->ny && zz < (int)J->nz) {
+                    u = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
+                    const uint32_t ku = J->keys[u];
+                    // quick reject: EMPTY (0) or already reached at a level <= d
+                    if (ku != kEmptyKey && (ku >> b) > d) {
+                        du = d + 1u + noise_delta(J->noise_seed, u, J->noise_amp);
+                        if (du >= (1u << (32 - b)) - 1u) {
+                            atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrKeyOverflow);
+                        } else {
+                            const uint32_t cand = (du << b) | idx;
+                            if (cand < ku) {
+                                const uint32_t old = atomicMin(J->keys + u, cand);
+                                want = cand < old && (old >> b) != du;
+                            }
+                        }
+                    }
+                }
+            }
+            if (__ballot(k < nb) == 0) break;
+            push_aggregated(want, L.jobs, j, du, u);
+        }
+    }
+
+    // last block: advance every job to the next level
+    __threadfence();
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) last = atomicAdd(blocks_done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (uint32_t j = threadIdx.x; j < nj; j += blockDim.x) {
+        const JobDev* J = L.jobs + j;
+        if (!J->active || J->fctl[1] != 0) continue;
+        const uint32_t d = lev[j];
+        volatile uint32_t* c = J->fctl + 2;
+        c[(d + 4) & 7u] = 0u;
+        J->fctl[0] = d + 1;
+        if (c[(d + 1) & 7u] == 0u && c[(d + 2) & 7u] == 0u && c[(d + 3) & 7u] == 0u) J->fctl[1] = d + 1;
+    }
+    if (threadIdx.x == 0) *blocks_done = 0;
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
+{
+    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
+        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J->V) continue;
+        const uint32_t mask = (1u << J->key_bits) - 1u;
+        uint16_t out[8];
+        for (int k = 0; k < 8; ++k) {
+            uint32_t key = c0 + k < J->V ? J->keys[c0 + k] : 0u;
+            if (key == kEmptyKey) out[k] = kEmpty;
+            else if (key == kUnreached) out[k] = kFree;
+            else out[k] = (uint16_t)(J->seeds[key & mask].w & 0xFFu);  // unmaskRightMost(value, 8)
+        }
+        store8(J->labels, c0, J->V, out);
+    }
+}
+
+}  // namespace
+
+void launch_flood_init(const Launch& L)
+{
+    TimedLaunch tl("flood_init", L.s);
+    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    k_flood_init<<<blocks, kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_flood_seed(const Launch& L, uint32_t /*max_seeds*/)
+{
+    TimedLaunch tl("flood_seed", L.s);
+    k_flood_seed<<<L.njobs, kFloodThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_flood_level(const Launch& L, uint32_t* blocks_done, uint32_t blocks)
+{
+    if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
+    TimedLaunch tl("flood_level", L.s);
+    k_flood_level<<<blocks, kFloodThreads, 0, L.s>>>(L, blocks_done);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_flood_finish(const Launch& L)
+{
+    TimedLaunch tl("flood_finish", L.s);
+    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    k_flood_finish<<<blocks, kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+}  // namespace vf

@@ -1,0 +1,314 @@
+// Cell-parallel grid stages: occupancy <-> labels, homogenize, mask removal,
+// boundary detection (optionally fused with the per-fragment histogram),
+// histogram, occupied count and the order-independent grid checksum.
+// All are HBM-bound streaming passes; each thread owns 8 cells.
+#include "cellops.hpp"
+
+namespace vf {
+
+namespace {
+
+template <class F>
+__device__ inline void for_each_tile(const Launch& L, F&& f)
+{
+    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
+        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        f(J, c0);
+    }
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_occ_to_labels(Launch L)
+{
+    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
+        if (c0 >= J->V) return;
+        uint64_t o = load8b(J->occ, c0, J->V);
+        uint16_t v[8];
+        for (int k = 0; k < 8; ++k) v[k] = (uint16_t)((o >> (8 * k)) & 1u);
+        store8(J->labels, c0, J->V, v);
+    });
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_labels_to_occ(Launch L)
+{
+    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
+        if (c0 >= J->V) return;
+        uint16_t v[8];
+        load8(J->labels, c0, J->V, v);
+        uint64_t o = 0;
+        for (int k = 0; k < 8; ++k) o |= (uint64_t)(v[k] != 0) << (8 * k);
+        store8b(J->occ, c0, J->V, o);
+    });
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_homogenize(Launch L)
+{
+    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
+        if (c0 >= J->V) return;
+        uint16_t v[8];
+        load8(J->labels, c0, J->V, v);
+        for (int k = 0; k < 8; ++k) v[k] = v[k] != kEmpty ? kFree : kEmpty;
+        store8(J->labels, c0, J->V, v);
+    });
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_undo_mask(Launch L, uint16_t keep)
+{
+    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
+        if (c0 >= J->V) return;
+        uint16_t v[8];
+        load8(J->labels, c0, J->V, v);
+        for (int k = 0; k < 8; ++k) v[k] = (uint16_t)(v[k] & keep);
+        store8(J->labels, c0, J->V, v);
+    });
+}
+
+__global__ void k_clear_hist(Launch L)
+{
+    for (uint32_t j = 0; j < L.njobs; ++j) {
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 65536; i += gridDim.x * blockDim.x) J->hist[i] = 0;
+    }
+}
+
+constexpr uint32_t kLdsBins = 4096;
+
+// Boundary bit of one labelled cell (detectBoundaries-comp.glsl:18-43).
+__device__ inline bool is_boundary_cell(const JobDev* J, uint32_t x, uint32_t y, uint32_t z, uint16_t c, int b)
+{
+    const int nx = (int)J->nx, ny = (int)J->ny, nz = (int)J->nz;
+    int x0 = max((int)x - b, 0), x1 = min((int)x + b, nx - 1);
+    int y0 = max((int)y - b, 0), y1 = min((int)y + b, ny - 1);
+    int z0 = max((int)z - b, 0), z1 = min((int)z + b, nz - 1);
+    const uint16_t* g = J->labels;
+    for (int xx = x0; xx <= x1; ++xx)
+        for (int yy = y0; yy <= y1; ++yy) {
+            const uint16_t* row = g + ((size_t)xx * ny + yy) * nz;
+            for (int zz = z0; zz <= z1; ++zz) {
+                uint16_t v = (uint16_t)(row[zz] & kMask15);
+                if (v > kFree && v != c) return true;
+            }
+        }
+    return false;
+}
+
+// HIST: accumulate the per-fragment histogram; BOUND: set the boundary bit.
+template <bool HIST, bool BOUND>
+__global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize)
+{
+    __shared__ uint32_t bins[HIST ? kLdsBins : 1];
+    uint32_t tb, te;
+    block_tile_range(L.total_tiles, tb, te);
+    if (tb >= te) return;
+    uint32_t cur_job = find_job(L.tile_prefix, L.njobs, tb);
+    if (HIST) {
+        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) bins[i] = 0;
+        __syncthreads();
+    }
+    auto flush = [&](uint32_t j) {
+        if (!HIST) return;
+        __syncthreads();
+        uint32_t* h = L.jobs[j].hist;
+        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) {
+            uint32_t c = bins[i];
+            if (c) {
+                atomicAdd(h + i, c);
+                bins[i] = 0;
+            }
+        }
+        __syncthreads();
+    };
+    for (uint32_t t = tb; t < te; ++t) {
+        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        if (j != cur_job) {
+            flush(cur_job);
+            cur_job = j;
+        }
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J->V) continue;
+        uint16_t v[8];
+        load8(J->labels, c0, J->V, v);
+        bool any = false;
+        for (int k = 0; k < 8; ++k) any |= v[k] > kFree;
+        if (!any) continue;
+        bool changed = false;
+        if (BOUND) {
+            uint32_t x, y, z;
+            decompose(c0, J, x, y, z);
+            for (int k = 0; k < 8; ++k) {
+                if (c0 + k < J->V && v[k] > kFree && is_boundary_cell(J, x, y, z, v[k], bsize)) {
+                    v[k] = (uint16_t)(v[k] | 0x8000u);
+                    changed = true;
+                }
+                advance(x, y, z, J);
+            }
+        }
+        if (HIST) {
+            for (int k = 0; k < 8; ++k) {
+                if (c0 + k < J->V && v[k] > kFree) {
+                    uint32_t m = v[k] & kMask15;
+                    if (m < kLdsBins) atomicAdd(&bins[m], 1u);
+                    else atomicAdd(J->hist + m, 1u);
+                }
+            }
+        }
+        if (changed) store8(J->labels, c0, J->V, v);
+    }
+    flush(cur_job);
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_checksum(Launch L)
+{
+    __shared__ unsigned long long part[kCellThreads / 64];
+    uint32_t tb, te;
+    block_tile_range(L.total_tiles, tb, te);
+    if (tb >= te) return;
+    uint32_t cur_job = find_job(L.tile_prefix, L.njobs, tb);
+    unsigned long long acc = 0;
+    auto flush = [&](uint32_t j) {
+        unsigned long long s = wave_sum_u64(acc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
+            atomicAdd(L.jobs[j].stats + kStatChecksum, tot);
+        }
+        __syncthreads();
+        acc = 0;
+    };
+    for (uint32_t t = tb; t < te; ++t) {
+        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        if (j != cur_job) {
+            flush(cur_job);
+            cur_job = j;
+        }
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J->V) continue;
+        uint16_t v[8];
+        load8(J->labels, c0, J->V, v);
+        for (int k = 0; k < 8; ++k) {
+            if (c0 + k >= J->V) break;
+            uint32_t c = (uint32_t)(c0 + k);
+            acc += ((unsigned long long)hash32(v[k], c) << 32) | hash32(c, v[k]);
+        }
+    }
+    flush(cur_job);
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_count_occupied(Launch L)
+{
+    __shared__ unsigned long long part[kCellThreads / 64];
+    uint32_t tb, te;
+    block_tile_range(L.total_tiles, tb, te);
+    if (tb >= te) return;
+    uint32_t cur_job = find_job(L.tile_prefix, L.njobs, tb);
+    unsigned long long acc = 0;
+    auto flush = [&](uint32_t j) {
+        unsigned long long s = wave_sum_u64(acc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
+            if (tot) atomicAdd(L.jobs[j].stats + kStatOccupied, tot);
+        }
+        __syncthreads();
+        acc = 0;
+    };
+    for (uint32_t t = tb; t < te; ++t) {
+        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        if (j != cur_job) {
+            flush(cur_job);
+            cur_job = j;
+        }
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J->V) continue;
+        uint64_t o = load8b(J->occ, c0, J->V);
+        acc += __popcll(o & 0x0101010101010101ull);
+    }
+    flush(cur_job);
+}
+
+inline uint32_t grid_for(const Launch& L) { return L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048; }
+
+}  // namespace
+
+void launch_occ_to_labels(const Launch& L)
+{
+    TimedLaunch tl("occ_to_labels", L.s);
+    k_occ_to_labels<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_labels_to_occ(const Launch& L)
+{
+    TimedLaunch tl("labels_to_occ", L.s);
+    k_labels_to_occ<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_homogenize(const Launch& L)
+{
+    TimedLaunch tl("homogenize", L.s);
+    k_homogenize<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_undo_mask(const Launch& L, uint16_t keep)
+{
+    TimedLaunch tl("undo_mask", L.s);
+    k_undo_mask<<<grid_for(L), kCellThreads, 0, L.s>>>(L, keep);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_clear_hist(const Launch& L)
+{
+    TimedLaunch tl("clear_hist", L.s);
+    k_clear_hist<<<64, 256, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_histogram(const Launch& L)
+{
+    TimedLaunch tl("histogram", L.s);
+    k_cells_post<true, false><<<grid_for(L), kCellThreads, 0, L.s>>>(L, 0);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_boundaries(const Launch& L, int size, int with_hist)
+{
+    if (with_hist) {
+        TimedLaunch tl("boundaries_hist", L.s);
+        k_cells_post<true, true><<<grid_for(L), kCellThreads, 0, L.s>>>(L, size);
+    } else {
+        TimedLaunch tl("boundaries", L.s);
+        k_cells_post<false, true><<<grid_for(L), kCellThreads, 0, L.s>>>(L, size);
+    }
+    VF_CHECK_LAUNCH();
+}
+
+void launch_checksum(const Launch& L)
+{
+    TimedLaunch tl("checksum", L.s);
+    k_checksum<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_count_occupied(const Launch& L)
+{
+    TimedLaunch tl("count_occupied", L.s);
+    k_count_occupied<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+}  // namespace vf

@@ -1,0 +1,161 @@
+// Device-side job descriptor and the stage launchers.
+//
+// Every stage kernel is written once in multi-job form: it receives an array
+// of JobDev descriptors plus a prefix table mapping global tiles (or
+// triangles) to jobs, and walks the tiles with a persistent grid-stride loop.
+// The single-grid API launches it with one job; the batched form launches it
+// once for all jobs of a batch, so uneven grids share one launch.
+#pragma once
+
+#include "common.hpp"
+
+namespace vf {
+
+constexpr uint32_t kLevelRing = 4;   // flood bucket ring (node weights are 1..3, levels d..d+3 live)
+constexpr uint32_t kMaxNoiseAmp = 2;  // weights 1 + [0, amp] must stay within the ring
+constexpr uint32_t kTileCells = 2048; // cells per tile for cell-parallel stages (256 threads x 8 cells)
+
+enum StatSlot : int {
+    kStatOccupied = 0,   // non-EMPTY cells after voxelization
+    kStatChecksum = 1,   // order-independent grid checksum
+    kStatShellPairs = 2, // (triangle, cell) SAT candidates
+    kStatColPairs = 3,   // (triangle, column) parity candidates
+    kStatError = 4,      // first error flag raised on the device
+    kStatFloodMax = 5,   // deepest BFS level with a pushed cell
+    kStatFloodDone = 6,  // level at which the flood drained
+    kStatCount = 8
+};
+
+// Device error flags (kStatError)
+enum DevError : uint32_t {
+    kDevErrNone = 0,
+    kDevErrSeeding = 1,     // fewer candidates than seeds
+    kDevErrKeyOverflow = 2, // flood level no longer fits the packed key
+    kDevErrBucket = 3,      // flood bucket capacity exceeded
+};
+
+struct TriSetup {
+    float P[3][3];             // FP32 vertices (SAT input)
+    int32_t X[3], Y[3], Z[3];  // snapped fixed-point vertices (CCW in xy)
+    int64_t area;              // twice the projected area (> 0) or 0 = no parity work
+    int32_t lo[3], hi[3];      // SAT candidate cell range (inclusive)
+    int32_t i0, i1, j0, j1;    // parity column range (inclusive)
+    uint32_t job;
+};
+
+struct JobDev {
+    uint32_t nx, ny, nz, wp;  // dims; wp = 32-bit words per z column in the bitmasks (power of two)
+    uint64_t V;
+    float mn[3], mx[3], cs[3];
+    uint16_t* labels;
+    uint8_t* occ;
+    // voxelizer input / scratch
+    const float* xyz;
+    const uint32_t* tri;
+    uint32_t nt;
+    uint32_t* shell_bits;   // nx*ny*wp words
+    uint32_t* toggle_bits;  // nx*ny*wp words
+    // seeds {x,y,z,label}
+    uint4* seeds;
+    uint32_t S;
+    int dist;
+    uint32_t key_bits;      // bits for the seed index in packed keys
+    int neighbourhood;      // 6 or 26
+    uint64_t noise_seed;
+    uint32_t noise_amp;
+    int seed_location;
+    uint64_t seed_key;
+    uint32_t* keys;         // flood keys, V: 0 = EMPTY, ~0 = unreached, else (level << key_bits) | seed
+    uint32_t* buckets;      // kLevelRing buckets of bucket_cap cells
+    uint32_t bucket_cap;
+    uint32_t* fctl;         // flood control: [0] level, [1] drained-at level (0 = running), [2..9] level counters
+    unsigned long long* stats;  // kStatCount slots
+    uint32_t* hist;         // 65536 counts
+    int active;             // 0: the job is skipped by multi-job launches
+};
+
+__device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t njobs, uint32_t t)
+{
+    // largest j with prefix[j] <= t  (prefix has njobs+1 entries)
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint64_t find_job64(const unsigned long long* prefix, uint32_t n, unsigned long long t)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// --------------------------------------------------------------- launchers
+// All launchers enqueue on stream s; none synchronises.  `jobs` and the
+// prefix tables are device pointers.
+struct Launch {
+    const JobDev* jobs;          // device array
+    uint32_t njobs;
+    const uint32_t* tile_prefix; // device, njobs+1 (kTileCells tiles)
+    uint32_t total_tiles;
+    hipStream_t s;
+};
+
+// voxelize.hip
+struct VoxScratch {
+    TriSetup* setup;                   // total triangles
+    unsigned long long* shell_off;     // total triangles + 1 exclusive prefix of the shell pair counts
+    unsigned long long* col_off;
+    uint32_t* counts;                  // 2*total triangles: shell pair counts, then column pair counts
+    const uint32_t* tri_prefix;        // njobs+1 prefix over triangles (device)
+    uint32_t total_tris;
+    void* scan_tmp;                    // scan scratch
+    size_t scan_tmp_bytes;
+};
+size_t vox_scan_tmp_bytes(uint32_t total_tris);
+void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_col_groups);
+void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_col_groups);
+
+// grid_ops.hip
+void launch_occ_to_labels(const Launch& L);
+void launch_labels_to_occ(const Launch& L);
+void launch_homogenize(const Launch& L);
+void launch_undo_mask(const Launch& L, uint16_t keep_mask);
+void launch_clear_hist(const Launch& L);
+void launch_histogram(const Launch& L);
+void launch_boundaries(const Launch& L, int size, int with_hist);
+void launch_checksum(const Launch& L);
+void launch_count_occupied(const Launch& L);
+
+// naive.hip
+void launch_naive_shaped(const Launch& L, int dist, bool fast, bool key64);
+void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint32_t* frontier_a, uint32_t* frontier_b,
+                            uint32_t* counters, uint32_t* visited, const uint32_t dims[3], const uint4* seeds, uint32_t S,
+                            uint32_t* host_counter);
+
+// flood.hip
+constexpr uint32_t kFloodCtlWords = 10;
+constexpr uint32_t kFloodMaxJobs = 1024;  // jobs per multi-job flood launch
+void launch_flood_init(const Launch& L);
+void launch_flood_seed(const Launch& L, uint32_t max_seeds);
+// one BFS level for every active job; self-advancing (the last block bumps
+// each job's level), so the same launch can be replayed from a graph
+void launch_flood_level(const Launch& L, uint32_t* blocks_done, uint32_t blocks);
+void launch_flood_finish(const Launch& L);
+
+// seed.hip
+constexpr uint32_t kMaxDeviceSeeds = 4096;
+void launch_seed_count(const Launch& L, uint32_t* tile_cnt);
+// tile_off: exclusive prefix over the global tiles (total_tiles + 1 entries)
+void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
+
+// scan (scan.hip): exclusive prefix sums, graph-capturable, no host sync
+size_t scan_tmp_bytes(uint64_t n);
+void scan_exclusive_u32_to_u64(const uint32_t* in, unsigned long long* out, uint64_t n, void* tmp, hipStream_t s);
+
+}  // namespace vf

@@ -1,0 +1,221 @@
+// Naive Voronoi fracture (NaiveFracturer::buildCPU, NaiveFracturer.cpp:26-68;
+// naiveFracturer-comp.glsl:19-42) and the optional isolated-region removal
+// (removeIsolatedRegionsCPU, NaiveFracturer.cpp:111-150).
+//
+// Every non-EMPTY cell takes the label of argmin over seeds of (distance,
+// seed index).  Distances are exact integers: squared Euclidean, L1, L-inf
+// (the reference's float distances on integer offsets order identically).
+// The argmin is a single unsigned min over packed keys (distance << b | i),
+// so ties go to the lowest index exactly as the reference's strict '<' does.
+// Seeds are staged through LDS in chunks; each thread owns 8 cells of one z
+// run, so the x/y part of the distance is computed once per seed.
+#include "cellops.hpp"
+
+namespace vf {
+
+namespace {
+
+constexpr uint32_t kSeedChunk = 1024;
+
+template <int DIST, typename K>
+__device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
+{
+    return ((K)d << b) | (K)i;
+}
+
+// FAST: the thread's 8 cells share (x, y) (nz % 8 == 0)
+template <int DIST, bool FAST, typename K>
+__global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
+{
+    __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
+    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
+        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
+        const JobDev* J = L.jobs + j;
+        if (!J->active || J->S == 0) continue;
+        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
+        const bool live = c0 < J->V;
+        const uint64_t occ = live ? load8b(J->occ, c0, J->V) : 0ull;
+        const uint32_t b = J->key_bits;
+        const uint32_t S = J->S;
+        uint32_t x = 0, y = 0, z = 0;
+        if (live) decompose(c0, J, x, y, z);
+        uint32_t cx[8], cy[8], cz[8];
+        if (!FAST) {
+            uint32_t a = x, bb = y, c = z;
+            for (int k = 0; k < 8; ++k) {
+                cx[k] = a; cy[k] = bb; cz[k] = c;
+                advance(a, bb, c, J);
+            }
+        }
+        K best[8];
+        for (int k = 0; k < 8; ++k) best[k] = ~(K)0;
+        const bool work = occ != 0;
+        for (uint32_t base = 0; base < S; base += kSeedChunk) {
+            const uint32_t n = min(kSeedChunk, S - base);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                uint4 s = J->seeds[base + i];
+                sx[i] = s.x; sy[i] = s.y; sz[i] = s.z;
+            }
+            __syncthreads();
+            if (!work) continue;
+            if (FAST) {
+                for (uint32_t i = 0; i < n; ++i) {
+                    const uint32_t id = base + i;
+                    const uint32_t ax = __usad(x, sx[i], 0u), ay = __usad(y, sy[i], 0u);
+                    if (DIST == VF_MANHATTAN) {
+                        const K kb = pack_key<DIST, K>(ax + ay, id, b);
+                        for (int k = 0; k < 8; ++k) {
+                            const K key = kb + ((K)__usad(z + k, sz[i], 0u) << b);
+                            best[k] = key < best[k] ? key : best[k];
+                        }
+                    } else if (DIST == VF_CHEBYSHEV) {
+                        const K kb = pack_key<DIST, K>(max(ax, ay), id, b);
+                        for (int k = 0; k < 8; ++k) {
+                            K key = pack_key<DIST, K>(__usad(z + k, sz[i], 0u), id, b);
+                            key = key > kb ? key : kb;
+                            best[k] = key < best[k] ? key : best[k];
+                        }
+                    } else {
+                        const K kb = pack_key<DIST, K>(ax * ax + ay * ay, id, b);
+                        for (int k = 0; k < 8; ++k) {
+                            const uint32_t dz = __usad(z + k, sz[i], 0u);
+                            const K key = kb + ((K)(dz * dz) << b);
+                            best[k] = key < best[k] ? key : best[k];
+                        }
+                    }
+                }
+            } else {
+                for (uint32_t i = 0; i < n; ++i) {
+                    const uint32_t id = base + i;
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t ax = __usad(cx[k], sx[i], 0u), ay = __usad(cy[k], sy[i], 0u),
+                                       az = __usad(cz[k], sz[i], 0u);
+                        uint32_t d;
+                        if (DIST == VF_MANHATTAN) d = ax + ay + az;
+                        else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
+                        else d = ax * ax + ay * ay + az * az;
+                        const K key = pack_key<DIST, K>(d, id, b);
+                        best[k] = key < best[k] ? key : best[k];
+                    }
+                }
+            }
+        }
+        if (!live) continue;
+        uint16_t out[8];
+        const K mask = ((K)1 << b) - 1;
+        for (int k = 0; k < 8; ++k) {
+            const bool o = ((occ >> (8 * k)) & 0xFFu) != 0;
+            out[k] = o ? (uint16_t)J->seeds[(uint32_t)(best[k] & mask)].w : (uint16_t)0;
+        }
+        store8(J->labels, c0, J->V, out);
+    }
+}
+
+template <int DIST, typename K>
+void launch_naive_t(const Launch& L, bool fast)
+{
+    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    if (fast) k_naive<DIST, true, K><<<blocks, kCellThreads, 0, L.s>>>(L);
+    else k_naive<DIST, false, K><<<blocks, kCellThreads, 0, L.s>>>(L);
+}
+
+// ---------------------------------------------------------------------------
+// Isolated-region removal: label-propagation BFS from the seeds through cells
+// holding the seed's label.  Each cell can be claimed by one label only (its
+// own), so the reached set, and thus the result, is independent of order.
+__global__ void k_ri_init(const uint16_t* labels, uint16_t* out, uint32_t* visited, uint64_t V, uint64_t words)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        visited[i] = 0;
+}
+
+__global__ void k_ri_seed(uint16_t* out, uint32_t* visited, uint32_t* frontier, uint32_t* fl, uint32_t* counters,
+                          uint32_t ny, uint32_t nz, const uint4* seeds, uint32_t S)
+{
+    // seeds stamped in list order: the last seed on a cell keeps it
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (uint32_t s = 0; s < S; ++s) {
+            uint32_t c = (seeds[s].x * ny + seeds[s].y) * nz + seeds[s].z;
+            out[c] = (uint16_t)seeds[s].w;
+            visited[c >> 5] |= 1u << (c & 31);
+            frontier[s] = c;
+            fl[s] = seeds[s].w;
+        }
+        counters[0] = S;
+        counters[1] = 0;
+    }
+}
+
+__global__ void k_ri_step(const uint16_t* labels, uint16_t* out, uint32_t* visited, const uint32_t* fa, const uint32_t* la,
+                          uint32_t* fb, uint32_t* lb, uint32_t* counters, uint32_t cur, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint32_t n = counters[cur];
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t v = fa[e], w = la[e];
+        const uint32_t yz = ny * nz;
+        const uint32_t x = v / yz, r = v - x * yz, y = r / nz, z = r - y * nz;
+        const int dx[6] = { 1, -1, 0, 0, 0, 0 }, dy[6] = { 0, 0, 1, -1, 0, 0 }, dz[6] = { 0, 0, 0, 0, 1, -1 };
+        for (int k = 0; k < 6; ++k) {
+            const int xx = (int)x + dx[k], yy = (int)y + dy[k], zz = (int)z + dz[k];
+            if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)nx || yy >= (int)ny || zz >= (int)nz) continue;
+            const uint32_t u = ((uint32_t)xx * ny + (uint32_t)yy) * nz + (uint32_t)zz;
+            if ((uint32_t)labels[u] != w) continue;
+            const uint32_t bit = 1u << (u & 31);
+            if (visited[u >> 5] & bit) continue;
+            const uint32_t old = atomicOr(visited + (u >> 5), bit);
+            if (old & bit) continue;
+            out[u] = (uint16_t)w;
+            const uint32_t slot = atomicAdd(counters + (cur ^ 1), 1u);
+            fb[slot] = u;
+            lb[slot] = w;
+        }
+    }
+}
+
+__global__ void k_ri_reset(uint32_t* counters, uint32_t idx) { counters[idx] = 0; }
+
+}  // namespace
+
+void launch_naive_shaped(const Launch& L, int dist, bool fast, bool key64)
+{
+    TimedLaunch tl("naive", L.s);
+    if (key64) {
+        if (dist == VF_MANHATTAN) launch_naive_t<VF_MANHATTAN, unsigned long long>(L, fast);
+        else if (dist == VF_CHEBYSHEV) launch_naive_t<VF_CHEBYSHEV, unsigned long long>(L, fast);
+        else launch_naive_t<VF_EUCLIDEAN, unsigned long long>(L, fast);
+    } else {
+        if (dist == VF_MANHATTAN) launch_naive_t<VF_MANHATTAN, uint32_t>(L, fast);
+        else if (dist == VF_CHEBYSHEV) launch_naive_t<VF_CHEBYSHEV, uint32_t>(L, fast);
+        else launch_naive_t<VF_EUCLIDEAN, uint32_t>(L, fast);
+    }
+    VF_CHECK_LAUNCH();
+}
+
+void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint32_t* fa, uint32_t* fb, uint32_t* counters,
+                            uint32_t* visited, const uint32_t dims[3], const uint4* seeds, uint32_t S, uint32_t* host_counter)
+{
+    // fa/fb each hold 2*V uint32 (cell, label) halves; host_counter is pinned memory
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2];
+    uint32_t* la = fa + V;
+    uint32_t* lb = fb + V;
+    TimedLaunch tl("remove_isolated", s);
+    k_ri_init<<<1024, 256, 0, s>>>(labels, out, visited, V, (V + 31) / 32);
+    k_ri_seed<<<1, 64, 0, s>>>(out, visited, fa, la, counters, dims[1], dims[2], seeds, S);
+    VF_CHECK_LAUNCH();
+    uint32_t cur = 0;
+    for (;;) {
+        k_ri_step<<<1024, 256, 0, s>>>(labels, out, visited, cur ? fb : fa, cur ? lb : la, cur ? fa : fb, cur ? la : lb,
+                                       counters, cur, dims[0], dims[1], dims[2]);
+        VF_CHECK_LAUNCH();
+        k_ri_reset<<<1, 1, 0, s>>>(counters, cur);
+        VF_HIP(hipMemcpyAsync(host_counter, counters + (cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        VF_HIP(hipStreamSynchronize(s));
+        cur ^= 1;
+        if (*host_counter == 0) break;
+    }
+}
+
+}  // namespace vf

@@ -1,0 +1,381 @@
+// Conservative voxelization: FP32 triangle/box SAT shell plus a z-parity
+// interior fill, replacing the reference's raster Tetravoxelizer
+// (RegularGrid::fill, RegularGrid.cpp:173-212; Tetravoxelizer.cpp:264-315).
+//
+// Pipeline (all jobs of a launch together, no host synchronisation):
+//   1. k_vox_setup   per triangle: SAT candidate cell box, snapped fixed-point
+//                    vertices, parity column range, pair counts
+//   2. scans         exclusive prefix of the pair counts (load balancing)
+//   3. k_vox_shell   one thread per (triangle, cell) pair: SAT
+//                    (Intersections3D.h:204-420 arithmetic), atomicOr shell bit
+//   4. k_vox_parity  one thread per (triangle, column) pair: top-left covered
+//                    column centres toggle the first cell centre above the
+//                    crossing (exact int64 edge functions), atomicXor
+//   5. k_vox_fill    one lane per 32-cell column word: prefix-XOR in the word,
+//                    carry across the column by wave ballot, occupancy bytes
+//                    = shell | parity, occupied count
+// Compiled with -ffp-contract=off: FP32 results equal the CPU oracle's.
+#include "cellops.hpp"
+
+namespace vf {
+
+namespace {
+
+constexpr int kSub = 256;
+constexpr float kSnapLim = 524288.0f;
+constexpr uint32_t kVoxThreads = 256;
+
+__device__ inline bool axis_sep(float pa, float pb, float rad)
+{
+    float mn = pa < pb ? pa : pb;
+    float mx = pa < pb ? pb : pa;
+    return mn > rad || mx < -rad;
+}
+
+__device__ bool sat_triangle_box(const float* a, const float* b, const float* c, const float ctr[3], const float half[3])
+{
+    float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        v0[q] = a[q] - ctr[q];
+        v1[q] = b[q] - ctr[q];
+        v2[q] = c[q] - ctr[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        e0[q] = v1[q] - v0[q];
+        e1[q] = v2[q] - v1[q];
+        e2[q] = v0[q] - v2[q];
+    }
+    float fx, fy, fz, pa, pb, rad;
+    fx = fabsf(e0[0]); fy = fabsf(e0[1]); fz = fabsf(e0[2]);
+    pa = e0[2] * v0[1] - e0[1] * v0[2]; pb = e0[2] * v2[1] - e0[1] * v2[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = -e0[2] * v0[0] + e0[0] * v0[2]; pb = -e0[2] * v2[0] + e0[0] * v2[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = e0[1] * v1[0] - e0[0] * v1[1]; pb = e0[1] * v2[0] - e0[0] * v2[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
+
+    fx = fabsf(e1[0]); fy = fabsf(e1[1]); fz = fabsf(e1[2]);
+    pa = e1[2] * v0[1] - e1[1] * v0[2]; pb = e1[2] * v2[1] - e1[1] * v2[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = -e1[2] * v0[0] + e1[0] * v0[2]; pb = -e1[2] * v2[0] + e1[0] * v2[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = e1[1] * v0[0] - e1[0] * v0[1]; pb = e1[1] * v1[0] - e1[0] * v1[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
+
+    fx = fabsf(e2[0]); fy = fabsf(e2[1]); fz = fabsf(e2[2]);
+    pa = e2[2] * v0[1] - e2[1] * v0[2]; pb = e2[2] * v1[1] - e2[1] * v1[2];
+    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = -e2[2] * v0[0] + e2[0] * v0[2]; pb = -e2[2] * v1[0] + e2[0] * v1[2];
+    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
+    pa = e2[1] * v1[0] - e2[0] * v1[1]; pb = e2[1] * v2[0] - e2[0] * v2[1];
+    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
+
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float mn = v0[q], mx = v0[q];
+        if (v1[q] < mn) mn = v1[q];
+        if (v1[q] > mx) mx = v1[q];
+        if (v2[q] < mn) mn = v2[q];
+        if (v2[q] > mx) mx = v2[q];
+        if (mn > half[q] || mx < -half[q]) return false;
+    }
+
+    float n[3];
+    n[0] = e0[1] * e1[2] - e1[1] * e0[2];
+    n[1] = e0[2] * e1[0] - e1[2] * e0[0];
+    n[2] = e0[0] * e1[1] - e1[0] * e0[1];
+    float vmin[3], vmax[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (n[q] > 0.0f) { vmin[q] = -half[q] - v0[q]; vmax[q] = half[q] - v0[q]; }
+        else             { vmin[q] = half[q] - v0[q];  vmax[q] = -half[q] - v0[q]; }
+    }
+    float dmin = n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2];
+    if (dmin > 0.0f) return false;
+    float dmax = n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2];
+    return dmax >= 0.0f;
+}
+
+__device__ inline void cell_box(int i, float mn, float cs, float* center, float* half)
+{
+    float lo = mn + (float)i * cs;
+    float hi = mn + (float)(i + 1) * cs;
+    *center = (hi + lo) / 2.0f;
+    *half = hi - *center;
+}
+
+__device__ inline int32_t snap_coord(float x, float mn, float cs)
+{
+    float q = (x - mn) / cs;
+    q = q * (float)kSub;
+    if (!(q >= -kSnapLim)) q = -kSnapLim;
+    if (q > kSnapLim) q = kSnapLim;
+    return (int32_t)rintf(q);
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ inline bool edge_incl(int64_t e, int64_t dx, int64_t dy)
+{
+    return e > 0 || (e == 0 && (dy < 0 || (dy == 0 && dx < 0)));
+}
+
+__device__ inline int64_t floordiv64(int64_t a, int64_t b)
+{
+    int64_t q = a / b;
+    if ((a % b) != 0 && a < 0) q -= 1;
+    return q;
+}
+
+__global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, uint32_t njobs, const uint32_t* tri_prefix,
+                                                           uint32_t total, TriSetup* setup, uint32_t* cnt_shell, uint32_t* cnt_col)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        const uint32_t j = find_job(tri_prefix, njobs, g);
+        const JobDev* J = jobs + j;
+        TriSetup S;
+        S.job = j;
+        S.area = 0;
+        uint32_t nshell = 0, ncol = 0;
+        const uint32_t t = g - tri_prefix[j];
+        if (J->active) {
+            const float* p[3] = { J->xyz + 3 * (size_t)J->tri[3 * (size_t)t], J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 1],
+                                  J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 2] };
+            float P[3][3];
+            bool finite = true;
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < 3; ++q) {
+                    P[k][q] = p[k][q];
+                    S.P[k][q] = P[k][q];
+                    finite &= __builtin_isfinite(P[k][q]);
+                }
+            const uint32_t dims[3] = { J->nx, J->ny, J->nz };
+            if (finite) {
+                bool empty = false;
+                for (int q = 0; q < 3; ++q) {
+                    float mn = P[0][q], mx = P[0][q];
+                    if (P[1][q] < mn) mn = P[1][q];
+                    if (P[1][q] > mx) mx = P[1][q];
+                    if (P[2][q] < mn) mn = P[2][q];
+                    if (P[2][q] > mx) mx = P[2][q];
+                    float fl = floorf((mn - J->mn[q]) / J->cs[q]);
+                    float fh = floorf((mx - J->mn[q]) / J->cs[q]);
+                    const float lim = (float)dims[q] + 1.0f;
+                    fl = fl < -2.0f ? -2.0f : (fl > lim ? lim : fl);
+                    fh = fh < -2.0f ? -2.0f : (fh > lim ? lim : fh);
+                    if ((int)fh + 1 < 0 || (int)fl - 1 > (int)dims[q] - 1) empty = true;
+                    S.lo[q] = clampi((int)fl - 1, 0, (int)dims[q] - 1);
+                    S.hi[q] = clampi((int)fh + 1, 0, (int)dims[q] - 1);
+                }
+                if (!empty)
+                    nshell = (uint32_t)(S.hi[0] - S.lo[0] + 1) * (uint32_t)(S.hi[1] - S.lo[1] + 1) *
+                             (uint32_t)(S.hi[2] - S.lo[2] + 1);
+                int64_t X[3], Y[3], Z[3];
+                for (int k = 0; k < 3; ++k) {
+                    X[k] = snap_coord(P[k][0], J->mn[0], J->cs[0]);
+                    Y[k] = snap_coord(P[k][1], J->mn[1], J->cs[1]);
+                    Z[k] = snap_coord(P[k][2], J->mn[2], J->cs[2]);
+                }
+                int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (Y[1] - Y[0]) * (X[2] - X[0]);
+                if (area != 0) {
+                    if (area < 0) {
+                        int64_t tx = X[1], ty = Y[1], tz = Z[1];
+                        X[1] = X[2]; Y[1] = Y[2]; Z[1] = Z[2];
+                        X[2] = tx; Y[2] = ty; Z[2] = tz;
+                        area = -area;
+                    }
+                    int32_t mnx = (int32_t)X[0], mxx = (int32_t)X[0], mny = (int32_t)Y[0], mxy = (int32_t)Y[0];
+                    for (int k = 1; k < 3; ++k) {
+                        mnx = min(mnx, (int32_t)X[k]); mxx = max(mxx, (int32_t)X[k]);
+                        mny = min(mny, (int32_t)Y[k]); mxy = max(mxy, (int32_t)Y[k]);
+                    }
+                    int i0 = (mnx - kSub / 2 + kSub - 1) >> 8, i1 = (mxx - kSub / 2) >> 8;
+                    int j0 = (mny - kSub / 2 + kSub - 1) >> 8, j1 = (mxy - kSub / 2) >> 8;
+                    i0 = max(i0, 0); j0 = max(j0, 0);
+                    i1 = min(i1, (int)dims[0] - 1); j1 = min(j1, (int)dims[1] - 1);
+                    for (int k = 0; k < 3; ++k) {
+                        S.X[k] = (int32_t)X[k]; S.Y[k] = (int32_t)Y[k]; S.Z[k] = (int32_t)Z[k];
+                    }
+                    S.area = area;
+                    S.i0 = i0; S.i1 = i1; S.j0 = j0; S.j1 = j1;
+                    if (i1 >= i0 && j1 >= j0) ncol = (uint32_t)(i1 - i0 + 1) * (uint32_t)(j1 - j0 + 1);
+                }
+            }
+        }
+        setup[g] = S;
+        cnt_shell[g] = nshell;
+        cnt_col[g] = ncol;
+    }
+}
+
+__global__ __launch_bounds__(kVoxThreads) void k_vox_shell(const JobDev* jobs, const TriSetup* setup,
+                                                           const unsigned long long* off, uint32_t ntris)
+{
+    const unsigned long long total = off[ntris];
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < total;
+         p += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint32_t g = (uint32_t)find_job64(off, ntris + 1, p);
+        const TriSetup* S = setup + g;
+        const JobDev* J = jobs + S->job;
+        const uint64_t local = p - off[g];
+        const uint32_t ez = (uint32_t)(S->hi[2] - S->lo[2] + 1), ey = (uint32_t)(S->hi[1] - S->lo[1] + 1);
+        const uint32_t z = (uint32_t)S->lo[2] + (uint32_t)(local % ez);
+        const uint64_t rest = local / ez;
+        const uint32_t y = (uint32_t)S->lo[1] + (uint32_t)(rest % ey);
+        const uint32_t x = (uint32_t)S->lo[0] + (uint32_t)(rest / ey);
+        const size_t word = ((size_t)x * J->ny + y) * J->wp + (z >> 5);
+        const uint32_t bit = 1u << (z & 31);
+        if (J->shell_bits[word] & bit) continue;  // already marked by another triangle
+        float c[3], h[3];
+        cell_box((int)x, J->mn[0], J->cs[0], &c[0], &h[0]);
+        cell_box((int)y, J->mn[1], J->cs[1], &c[1], &h[1]);
+        cell_box((int)z, J->mn[2], J->cs[2], &c[2], &h[2]);
+        if (sat_triangle_box(S->P[0], S->P[1], S->P[2], c, h)) atomicOr(J->shell_bits + word, bit);
+    }
+}
+
+__global__ __launch_bounds__(kVoxThreads) void k_vox_parity(const JobDev* jobs, const TriSetup* setup,
+                                                            const unsigned long long* off, uint32_t ntris)
+{
+    const unsigned long long total = off[ntris];
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < total;
+         p += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint32_t g = (uint32_t)find_job64(off, ntris + 1, p);
+        const TriSetup* S = setup + g;
+        const JobDev* J = jobs + S->job;
+        const uint64_t local = p - off[g];
+        const uint32_t nj = (uint32_t)(S->j1 - S->j0 + 1);
+        const int i = S->i0 + (int)(local / nj);
+        const int jj = S->j0 + (int)(local % nj);
+        const int64_t X0 = S->X[0], X1 = S->X[1], X2 = S->X[2];
+        const int64_t Y0 = S->Y[0], Y1 = S->Y[1], Y2 = S->Y[2];
+        const int64_t px = (int64_t)i * kSub + kSub / 2, py = (int64_t)jj * kSub + kSub / 2;
+        const int64_t d0x = X2 - X1, d0y = Y2 - Y1, d1x = X0 - X2, d1y = Y0 - Y2, d2x = X1 - X0, d2y = Y1 - Y0;
+        const int64_t E0 = d0x * (py - Y1) - d0y * (px - X1);
+        const int64_t E1 = d1x * (py - Y2) - d1y * (px - X2);
+        const int64_t E2 = d2x * (py - Y0) - d2y * (px - X0);
+        if (!edge_incl(E0, d0x, d0y) || !edge_incl(E1, d1x, d1y) || !edge_incl(E2, d2x, d2y)) continue;
+        const int64_t num = E0 * (int64_t)S->Z[0] + E1 * (int64_t)S->Z[1] + E2 * (int64_t)S->Z[2];
+        int64_t k = floordiv64(num - (int64_t)(kSub / 2) * S->area, (int64_t)kSub * S->area) + 1;
+        if (k < 0) k = 0;
+        if (k >= (int64_t)J->nz) continue;
+        const size_t word = ((size_t)i * J->ny + (size_t)jj) * J->wp + (size_t)(k >> 5);
+        atomicXor(J->toggle_bits + word, 1u << (k & 31));
+    }
+}
+
+// 4 bits -> 4 bytes of 0/1 (bit i -> byte i)
+__device__ inline uint32_t spread4(uint32_t b) { return ((b & 0xFu) * 0x00204081u) & 0x01010101u; }
+
+__global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
+{
+    for (uint32_t blk = blockIdx.x; blk < total_groups; blk += gridDim.x) {
+        const uint32_t j = find_job(col_prefix, L.njobs, blk);
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        const uint64_t nwords = (uint64_t)J->nx * J->ny * J->wp;
+        const uint64_t w = (uint64_t)(blk - col_prefix[j]) * kVoxThreads + threadIdx.x;
+        const bool valid = w < nwords;
+        const uint32_t tg = valid ? J->toggle_bits[w] : 0u;
+        const uint32_t sh = valid ? J->shell_bits[w] : 0u;
+        uint32_t p = tg;
+        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8; p ^= p << 16;
+        const uint64_t par = __ballot(__popc(tg) & 1);
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t seg = lane & ~(J->wp - 1);
+        const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull) & ~(seg ? (~0ull >> (64 - seg)) : 0ull);
+        if (__popcll(par & below) & 1) p = ~p;
+        uint32_t bits = 0;
+        uint32_t nb = 0;
+        uint64_t base = 0;
+        if (valid) {
+            const uint64_t col = w / J->wp;
+            const uint32_t z0 = (uint32_t)(w % J->wp) * 32u;
+            if (z0 < J->nz) {
+                nb = min(32u, J->nz - z0);
+                bits = (sh | p) & (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u));
+                base = col * J->nz + z0;
+            }
+        }
+        if (nb) {
+            uint8_t* o = J->occ + base;
+            if ((J->nz & 15u) == 0 && nb == 32) {
+                uint4 a, b;
+                a.x = spread4(bits); a.y = spread4(bits >> 4); a.z = spread4(bits >> 8); a.w = spread4(bits >> 12);
+                b.x = spread4(bits >> 16); b.y = spread4(bits >> 20); b.z = spread4(bits >> 24); b.w = spread4(bits >> 28);
+                reinterpret_cast<uint4*>(o)[0] = a;
+                reinterpret_cast<uint4*>(o)[1] = b;
+            } else if ((J->nz & 3u) == 0) {
+                for (uint32_t q = 0; q < nb / 4; ++q) reinterpret_cast<uint32_t*>(o)[q] = spread4(bits >> (4 * q));
+            } else {
+                for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
+            }
+        }
+        unsigned long long cnt = wave_sum_u64((unsigned long long)__popc(bits));
+        if (lane == 0 && cnt) atomicAdd(J->stats + kStatOccupied, cnt);
+    }
+}
+
+__global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
+{
+    for (uint32_t blk = blockIdx.x; blk < total_groups; blk += gridDim.x) {
+        const uint32_t j = find_job(col_prefix, L.njobs, blk);
+        const JobDev* J = L.jobs + j;
+        if (!J->active) continue;
+        const uint64_t nwords = (uint64_t)J->nx * J->ny * J->wp;
+        const uint64_t w = (uint64_t)(blk - col_prefix[j]) * kVoxThreads + threadIdx.x;
+        if (w < nwords) {
+            J->toggle_bits[w] = 0u;
+            J->shell_bits[w] = 0u;
+        }
+        if (w == 0) J->stats[kStatOccupied] = 0;
+    }
+}
+
+}  // namespace
+
+size_t vox_scan_tmp_bytes(uint32_t total_tris) { return scan_tmp_bytes(total_tris); }
+
+void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_groups)
+{
+    TimedLaunch tl("vox_clear", L.s);
+    uint32_t blocks = total_groups < 4096 ? (total_groups ? total_groups : 1) : 4096;
+    k_vox_clear<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
+    VF_CHECK_LAUNCH();
+}
+
+// X.setup / X.counts / X.shell_off / X.col_off / X.scan_tmp sized for X.total_tris.
+void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_groups)
+{
+    launch_clear_bits(L, col_prefix, total_groups);
+    if (X.total_tris) {
+        uint32_t* cnt_shell = X.counts;
+        uint32_t* cnt_col = cnt_shell + X.total_tris;
+        {
+            TimedLaunch tl("vox_setup", L.s);
+            k_vox_setup<<<stride_blocks(X.total_tris, kVoxThreads), kVoxThreads, 0, L.s>>>(
+                L.jobs, L.njobs, X.tri_prefix, X.total_tris, X.setup, cnt_shell, cnt_col);
+            VF_CHECK_LAUNCH();
+        }
+        scan_exclusive_u32_to_u64(cnt_shell, X.shell_off, X.total_tris, X.scan_tmp, L.s);
+        scan_exclusive_u32_to_u64(cnt_col, X.col_off, X.total_tris, X.scan_tmp, L.s);
+        {
+            TimedLaunch tl("vox_shell", L.s);
+            k_vox_shell<<<4096, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
+            VF_CHECK_LAUNCH();
+        }
+        {
+            TimedLaunch tl("vox_parity", L.s);
+            k_vox_parity<<<2048, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off, X.total_tris);
+            VF_CHECK_LAUNCH();
+        }
+    }
+    TimedLaunch tl("vox_fill", L.s);
+    uint32_t blocks = total_groups < 4096 ? (total_groups ? total_groups : 1) : 4096;
+    k_vox_fill<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
+    VF_CHECK_LAUNCH();
+}
+
+}  // namespace vf

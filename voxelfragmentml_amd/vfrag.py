@@ -1,0 +1,268 @@
+"""ctypes binding of libvfrag's C ABI (include/vfrag.h) for Python callers.
+
+This is plumbing for tests, the bench and scripting; the product is the C ABI
+and the HIP kernels behind it.  Loading fails loudly when libvfrag.so has not
+been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libvfrag.so"
+
+EUCLIDEAN, MANHATTAN, CHEBYSHEV = 0, 1, 2
+NAIVE, FLOOD = 0, 1
+NEIGH_FROM_DISTANCE, NEIGH_VON_NEUMANN, NEIGH_MOORE = 0, 6, 26
+SEED_BOTH, SEED_OUTER, SEED_INNER = 0, 1, 2
+ALL_LABELS = 0xFFFFFFFF
+
+ERRORS = {
+    -1: "VF_ERR_INVALID_ARG", -2: "VF_ERR_OUT_OF_MEMORY", -3: "VF_ERR_DEVICE", -4: "VF_ERR_TOO_MANY_SEEDS",
+    -5: "VF_ERR_BUDGET", -6: "VF_ERR_CANCELLED", -7: "VF_ERR_NOT_READY", -8: "VF_ERR_OVERFLOW",
+    -9: "VF_ERR_SEEDING", -10: "VF_ERR_POISON",
+}
+
+
+class VfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_labels", C.c_uint32)]
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("xyz", C.c_void_p), ("tri", C.c_void_p), ("nv", C.c_uint32), ("nt", C.c_uint32),
+        ("dims", C.c_uint32 * 3), ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3),
+        ("num_seeds", C.c_uint32), ("seed", C.c_uint64), ("seed_location", C.c_int),
+        ("algorithm", C.c_int), ("distance", C.c_int), ("neighbourhood", C.c_int),
+        ("noise_seed", C.c_uint64), ("noise_amp", C.c_uint32), ("boundary_size", C.c_int),
+        ("marching_cubes", C.c_int), ("mesh_on_device", C.c_int), ("poison", C.c_int),
+        ("labels_out", C.c_void_p), ("histogram_out", C.c_void_p),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("voxels", C.c_uint64), ("occupied", C.c_uint64), ("checksum", C.c_uint64),
+        ("num_labels", C.c_uint32), ("mc_vertices", C.c_uint32), ("mc_triangles", C.c_uint32),
+        ("flood_levels", C.c_uint32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+    ]
+
+
+def lib():
+    """Load libvfrag.so (built in-tree by voxelfragmentml_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m voxelfragmentml_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    P, U32, U64, I, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+    sig = {
+        "vf_last_error": (C.c_char_p, []), "vf_version": (I, []), "vf_device_count": (I, []),
+        "vf_grid_create": (I, [P, P, P, C.POINTER(P)]), "vf_grid_destroy": (I, [P]),
+        "vf_grid_set_aabb": (I, [P, P, P, P]), "vf_grid_dims": (I, [P, P]),
+        "vf_grid_read": (I, [P, P]), "vf_grid_write": (I, [P, P]), "vf_occupancy_read": (I, [P, P]),
+        "vf_grid_count_occupied": (I, [P, C.POINTER(U64)]), "vf_grid_device_ptr": (I, [P, C.POINTER(P)]),
+        "vf_grid_synchronize": (I, [P]),
+        "vf_voxelize": (I, [P, P, U32, P, U32]), "vf_voxelize_device": (I, [P, P, U32, P, U32]),
+        "vf_seed_uniform": (I, [P, U32, U64, I, P]),
+        "vf_fracture_naive": (I, [P, P, U32, I]), "vf_remove_isolated_regions": (I, [P, P, U32]),
+        "vf_fracture_flood": (I, [P, P, U32, I, I, U64, U32]),
+        "vf_detect_boundaries": (I, [P, I]), "vf_undo_mask": (I, [P]), "vf_homogenize": (I, [P]),
+        "vf_histogram": (I, [P, P, U32]),
+        "vf_marching_cubes": (I, [P, U32, C.POINTER(MeshInfo)]),
+        "vf_mesh_read": (I, [P, P, P, P, P, P, P]),
+        "vf_grid_export_rle": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "vf_grid_import_rle": (I, [P, P, C.c_size_t]),
+        "vf_vessel_mesh": (I, [U64, U32, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
+        "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
+        "vf_timing_read": (I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise VfError(rc, lib().vf_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    return lib().vf_device_count()
+
+
+def vessel_mesh(seed: int, radial: int, height_segments: int):
+    """Closed lathe vessel: (xyz float32 (nv,3), tri uint32 (nt,3))."""
+    L = lib()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    check(L.vf_vessel_mesh(seed, radial, height_segments, None, C.byref(nv), None, C.byref(nt)))
+    xyz = np.empty((nv.value, 3), np.float32)
+    tri = np.empty((nt.value, 3), np.uint32)
+    check(L.vf_vessel_mesh(seed, radial, height_segments, _ptr(xyz), C.byref(nv), _ptr(tri), C.byref(nt)))
+    return xyz, tri
+
+
+def vessel_segments(target_triangles: int):
+    """(radial, height) segment counts giving about target_triangles (height = 2.5 x radial)."""
+    r = max(3, int(round((target_triangles / 10.0) ** 0.5)))
+    h = max(1, int(round(2.5 * r)))
+    return r, h
+
+
+def seeds_array(seeds) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32).reshape(-1, 4))
+    return a
+
+
+class Grid:
+    """RegularGrid on the device (MF/Source/DataStructures/RegularGrid.h)."""
+
+    def __init__(self, dims, aabb_min, aabb_max):
+        self.dims = tuple(int(d) for d in dims)
+        self._h = C.c_void_p()
+        d = np.asarray(self.dims, np.uint32)
+        mn = np.asarray(aabb_min, np.float32)
+        mx = np.asarray(aabb_max, np.float32)
+        check(lib().vf_grid_create(_ptr(d), _ptr(mn), _ptr(mx), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().vf_grid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def V(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def set_aabb(self, aabb_min, aabb_max, dims):
+        d = np.asarray(dims, np.uint32)
+        check(lib().vf_grid_set_aabb(self._h, _ptr(np.asarray(aabb_min, np.float32)),
+                                     _ptr(np.asarray(aabb_max, np.float32)), _ptr(d)))
+        self.dims = tuple(int(x) for x in dims)
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.dims, np.uint16)
+        check(lib().vf_grid_read(self._h, _ptr(out)))
+        return out
+
+    def write(self, grid: np.ndarray):
+        g = np.ascontiguousarray(grid, dtype=np.uint16)
+        assert g.size == self.V
+        check(lib().vf_grid_write(self._h, _ptr(g)))
+
+    def occupancy(self) -> np.ndarray:
+        out = np.empty(self.dims, np.uint8)
+        check(lib().vf_occupancy_read(self._h, _ptr(out)))
+        return out
+
+    def count_occupied(self) -> int:
+        n = C.c_uint64()
+        check(lib().vf_grid_count_occupied(self._h, C.byref(n)))
+        return n.value
+
+    def voxelize(self, xyz: np.ndarray, tri: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        tri = np.ascontiguousarray(tri, np.uint32)
+        check(lib().vf_voxelize(self._h, _ptr(xyz), xyz.shape[0], _ptr(tri), tri.shape[0]))
+
+    def seed_uniform(self, n: int, seed: int, location: int = SEED_BOTH) -> np.ndarray:
+        out = np.zeros((n, 4), np.uint32)
+        check(lib().vf_seed_uniform(self._h, n, seed, location, _ptr(out)))
+        return out
+
+    def fracture_naive(self, seeds, distance: int):
+        s = seeds_array(seeds)
+        check(lib().vf_fracture_naive(self._h, _ptr(s), s.shape[0], distance))
+
+    def remove_isolated_regions(self, seeds):
+        s = seeds_array(seeds)
+        check(lib().vf_remove_isolated_regions(self._h, _ptr(s), s.shape[0]))
+
+    def fracture_flood(self, seeds, distance: int, neighbourhood: int = NEIGH_FROM_DISTANCE,
+                       noise_seed: int = 0, noise_amp: int = 0):
+        s = seeds_array(seeds)
+        check(lib().vf_fracture_flood(self._h, _ptr(s), s.shape[0], distance, neighbourhood, noise_seed, noise_amp))
+
+    def detect_boundaries(self, size: int = 1):
+        check(lib().vf_detect_boundaries(self._h, size))
+
+    def undo_mask(self):
+        check(lib().vf_undo_mask(self._h))
+
+    def homogenize(self):
+        check(lib().vf_homogenize(self._h))
+
+    def histogram(self, n: int = 65536) -> np.ndarray:
+        out = np.zeros(n, np.uint32)
+        check(lib().vf_histogram(self._h, _ptr(out), n))
+        return out
+
+    def marching_cubes(self, label: int = ALL_LABELS) -> dict:
+        info = MeshInfo()
+        check(lib().vf_marching_cubes(self._h, label, C.byref(info)))
+        nv, nt, nl = info.num_vertices, info.num_triangles, info.num_labels
+        xyz = np.empty((nv, 3), np.float32)
+        tri = np.empty((nt, 3), np.uint32)
+        flag = np.empty(nt, np.uint8)
+        labels = np.empty(nl, np.uint16)
+        voff = np.empty(nl + 1, np.uint32)
+        toff = np.empty(nl + 1, np.uint32)
+        check(lib().vf_mesh_read(self._h, _ptr(xyz), _ptr(tri), _ptr(flag), _ptr(labels), _ptr(voff), _ptr(toff)))
+        return {"xyz": xyz, "tri": tri, "flag": flag, "labels": labels, "voff": voff, "toff": toff}
+
+    def export_rle(self) -> bytes:
+        size = C.c_size_t()
+        check(lib().vf_grid_export_rle(self._h, None, 0, C.byref(size)))
+        buf = np.empty(size.value, np.uint8)
+        check(lib().vf_grid_export_rle(self._h, _ptr(buf), buf.size, C.byref(size)))
+        return buf.tobytes()
+
+    def import_rle(self, data: bytes):
+        buf = np.frombuffer(data, np.uint8)
+        check(lib().vf_grid_import_rle(self._h, _ptr(buf), buf.size))
+
+
+def timing_enable(on: bool = True):
+    check(lib().vf_timing_enable(1 if on else 0))
+
+
+def timing_reset():
+    check(lib().vf_timing_reset())
+
+
+def timing_read(kernel: str):
+    ms, n = C.c_double(), C.c_uint64()
+    check(lib().vf_timing_read(kernel.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
