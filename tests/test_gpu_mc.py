@@ -1,0 +1,71 @@
+"""GPU marching cubes against the oracle: labels, offsets, welded vertices and
+triangles must match exactly; FP32 vertices within 1e-5 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
+
+
+def compare(m, r):
+    np.testing.assert_array_equal(m["labels"], r["labels"])
+    np.testing.assert_array_equal(m["voff"], r["voff"])
+    np.testing.assert_array_equal(m["toff"], r["toff"])
+    np.testing.assert_array_equal(m["tri"], r["tri"])
+    np.testing.assert_array_equal(m["flag"], r["flag"])
+    np.testing.assert_array_equal(m["xyz"], r["xyz"])
+    scale = np.maximum(np.abs(r["xyz64"]), 2.06)
+    assert np.all(np.abs(m["xyz"].astype(np.float64) - r["xyz64"]) <= 1e-5 * scale)
+
+
+@pytest.mark.parametrize("dims,S", [((64, 64, 64), 12), ((45, 52, 37), 5)])
+def test_mc_fractured_vessel(vf, dims, S):
+    xyz, tri = vf.vessel_mesh(2, *vf.vessel_segments(12000))
+    g = vf.Grid(dims, *BOX)
+    g.voxelize(xyz, tri)
+    base = g.read()
+    seeds = O.seed_uniform(base, S, 3)
+    g.fracture_naive(seeds, 0)
+    g.detect_boundaries(1)
+    grid = g.read()
+    m = g.marching_cubes()
+    r = O.marching_cubes(grid, *BOX)
+    assert len(r["labels"]) == S
+    compare(m, r)
+    # single-label extraction equals that label's slice of the all-label mesh
+    L = int(r["labels"][1])
+    m1 = g.marching_cubes(L)
+    r1 = O.marching_cubes(grid, *BOX, label=L)
+    compare(m1, r1)
+
+
+def test_mc_random_labels_and_box(vf):
+    rng = np.random.default_rng(6)
+    dims = (20, 18, 16)
+    grid = np.zeros(dims, np.uint16)
+    grid[2:18, 2:16, 2:14] = rng.integers(2, 9, size=(16, 14, 12))
+    grid[rng.random(dims) < 0.05] |= 0x8000
+    g = vf.Grid(dims, [-1.3, -0.7, 0.2], [2.1, 1.9, 3.3])
+    g.write(grid)
+    m = g.marching_cubes()
+    compare_box = O.marching_cubes(grid, [-1.3, -0.7, 0.2], [2.1, 1.9, 3.3])
+    np.testing.assert_array_equal(m["tri"], compare_box["tri"])
+    np.testing.assert_array_equal(m["xyz"], compare_box["xyz"])
+    np.testing.assert_array_equal(m["flag"], compare_box["flag"])
+    box = np.zeros((9, 8, 10), np.uint16)
+    box[2:6, 2:5, 2:9] = 7
+    g2 = vf.Grid(box.shape, [0, 0, 0], list(box.shape))
+    g2.write(box)
+    m2 = g2.marching_cubes()
+    a, b, c = 4, 3, 7
+    T = 4 * ((a - 1) * (b - 1) + (b - 1) * (c - 1) + (c - 1) * (a - 1)) + 8 * ((a - 1) + (b - 1) + (c - 1)) + 8
+    assert m2["tri"].shape[0] == T and m2["xyz"].shape[0] == 2 + T // 2
+
+
+def test_mc_empty_grid(vf):
+    g = vf.Grid((10, 10, 10), *BOX)
+    m = g.marching_cubes()
+    assert m["tri"].shape[0] == 0 and m["labels"].shape[0] == 0
