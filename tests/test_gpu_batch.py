@@ -1,0 +1,139 @@
+"""Batched, plain and served forms: each job's result (grid checksum,
+histogram, occupied count, labels) must equal the single-grid path and the
+oracle, alone and in a mix of uneven jobs; failing jobs must not disturb the
+others and must release their slot and memory."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOX = ([-1.03] * 3, [1.03] * 3)
+
+
+def mixed_jobs(vf, n, seed, keep, edges=(40, 96), mc_every=0):
+    rng = np.random.default_rng(seed)
+    jobs, meta = [], []
+    for i in range(n):
+        e = [int(rng.integers(edges[0], edges[1] + 1)) for _ in range(3)]
+        tris = int(rng.integers(3000, 12000))
+        xyz, tri = vf.vessel_mesh(int(rng.integers(1, 1 << 30)), *vf.vessel_segments(tris))
+        algo = int(rng.integers(0, 2))
+        dist = int(rng.integers(0, 3))
+        S = int(rng.integers(2, 25))
+        amp = int(rng.integers(0, 3)) if algo == 1 else 0
+        labels = np.zeros(e, np.uint16)
+        hist = np.zeros(65536, np.uint32)
+        keep.extend([labels, hist])
+        j = vf.make_job(xyz, tri, e, *BOX, S, int(rng.integers(1, 1 << 40)), algo, dist,
+                        noise_seed=int(rng.integers(1, 1 << 40)), noise_amp=amp,
+                        boundary_size=1, marching_cubes=1 if (mc_every and i % mc_every == 0) else 0, keep=keep)
+        j.labels_out = labels.ctypes.data
+        j.histogram_out = hist.ctypes.data
+        jobs.append(j)
+        meta.append(dict(xyz=xyz, tri=tri, dims=e, S=S, algo=algo, dist=dist, amp=amp, labels=labels, hist=hist,
+                         seed=j.seed, noise_seed=j.noise_seed))
+    return jobs, meta
+
+
+def oracle_job(m):
+    occ = O.voxelize(m["xyz"], m["tri"], *BOX, m["dims"])
+    g = O.occ_to_grid(occ)
+    seeds = O.seed_uniform(g, m["S"], m["seed"])
+    if m["algo"] == 0:
+        lab = O.naive(g, seeds, m["dist"])
+    else:
+        lab = O.flood(g, seeds, 6 if m["dist"] == 1 else 26, m["noise_seed"], m["amp"])
+    return O.detect_boundaries(lab, 1), int(occ.sum())
+
+
+def test_batch_matches_oracle_and_plain(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 10, 1, keep)
+    b = vf.Batch()
+    res = b.run(jobs)
+    for m, r in zip(meta, res):
+        assert r["status"] == 0, r
+        ref, occ = oracle_job(m)
+        np.testing.assert_array_equal(m["labels"], ref)
+        np.testing.assert_array_equal(m["hist"], O.histogram(ref))
+        assert r["occupied"] == occ
+        assert r["checksum"] == O.checksum(ref)
+        assert r["num_labels"] == int(np.count_nonzero(O.histogram(ref)[2:]))
+    # replay (graphs) gives identical results
+    res2 = b.run(jobs)
+    assert [r["checksum"] for r in res2] == [r["checksum"] for r in res]
+    # plain form, one job per launch, same answers
+    plain = vf.plain_run(jobs)
+    assert [r["checksum"] for r in plain] == [r["checksum"] for r in res]
+    # without graphs
+    b2 = vf.Batch(graphs=False)
+    assert [r["checksum"] for r in b2.run(jobs)] == [r["checksum"] for r in res]
+
+
+def test_batch_alone_equals_in_mix(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 12, 7, keep)
+    b = vf.Batch()
+    mix = b.run(jobs)
+    for i in (0, 5, 11):
+        alone = vf.Batch().run([jobs[i]])[0]
+        assert alone["checksum"] == mix[i]["checksum"] and alone["occupied"] == mix[i]["occupied"]
+
+
+def test_batch_failures_isolated(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 6, 3, keep)
+    jobs[1].poison = 1
+    jobs[3].num_seeds = 300
+    jobs[3].algorithm = 1  # flood with more than 254 labels
+    res = vf.Batch().run(jobs)
+    assert res[1]["status"] == -10 and res[3]["status"] == -4
+    good = vf.Batch().run([jobs[i] for i in (0, 2, 4, 5)])
+    assert [res[i]["checksum"] for i in (0, 2, 4, 5)] == [r["checksum"] for r in good]
+
+
+def test_batch_marching_cubes_counts(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 3, 11, keep, edges=(48, 64), mc_every=1)
+    res = vf.Batch().run(jobs)
+    for m, r in zip(meta, res):
+        ref, _ = oracle_job(m)
+        mesh = O.marching_cubes(ref, *BOX)
+        assert r["mc_triangles"] == mesh["tri"].shape[0] and r["mc_vertices"] == mesh["xyz"].shape[0]
+
+
+def test_server_results_match_alone(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 16, 5, keep)
+    alone = vf.Batch().run(jobs)
+    srv = vf.Server(64 << 30, max_slots=6)
+    tickets = [srv.submit(j) for j in jobs]
+    out = [srv.wait(t) for t in tickets]
+    for a, o in zip(alone, out):
+        assert o["status"] == 0 and o["checksum"] == a["checksum"] and o["occupied"] == a["occupied"]
+        assert o["latency_ms"] >= o["queue_ms"] >= 0
+    st = srv.stats()
+    assert st["in_flight"] == 0 and st["reserved_bytes"] == 0 and st["completed"] == 16
+    srv.close()
+
+
+def test_server_budget_poison_cancel(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 6, 9, keep)
+    big = vf.make_job(meta[0]["xyz"], meta[0]["tri"], (1024, 1024, 1024), *BOX, 4, 1, 1, 2, keep=keep)
+    jobs[2].poison = 1
+    srv = vf.Server(2 << 30, max_slots=2)
+    t_big = srv.submit(big)
+    tickets = [srv.submit(j) for j in jobs]
+    assert srv.wait(t_big)["status"] == -5
+    out = [srv.wait(t) for t in tickets]
+    assert out[2]["status"] == -10
+    ok = [o for i, o in enumerate(out) if i != 2]
+    assert all(o["status"] == 0 for o in ok)
+    ref = vf.Batch().run([jobs[i] for i in range(6) if i != 2])
+    assert [o["checksum"] for o in ok] == [r["checksum"] for r in ref]
+    st = srv.stats()
+    assert st["in_flight"] == 0 and st["reserved_bytes"] == 0
+    srv.close()

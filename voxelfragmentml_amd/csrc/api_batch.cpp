@@ -1,0 +1,545 @@
+// C ABI for the batched, plain and served forms.
+//
+//   vf_batch_run   all jobs share every stage launch; the front, the flood
+//                  level chunk and the post stages are captured once into HIP
+//                  graphs and replayed while the job set is unchanged.
+//   vf_plain_run   the same stages one job at a time, one job per launch, with
+//                  a host check after every few flood levels (the reference's
+//                  per-dispatch round trip, FloodFracturer.cpp:143-158, batched
+//                  to chunks).
+//   vf_server_*    continuous batching: a scheduler thread admits queued jobs
+//                  against an HBM budget and a slot limit, runs their front
+//                  stages together, lets new flood jobs join the shared level
+//                  launches at their own level, and retires drained jobs, so
+//                  work of jobs that arrived at different times overlaps.
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+using namespace vf;
+
+namespace {
+
+constexpr uint32_t kLevelChunk = 16;
+
+uint64_t mix64(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+uint64_t job_signature(const vf_job* jobs, uint32_t n)
+{
+    uint64_t h = n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(jobs + i);
+        for (size_t k = 0; k + 8 <= sizeof(vf_job); k += 8) {
+            uint64_t v;
+            std::memcpy(&v, p + k, 8);
+            h = mix64(h, v);
+        }
+    }
+    return h;
+}
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    void reset()
+    {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
+    }
+    ~Graph() { reset(); }
+};
+
+template <class F>
+void run_captured(hipStream_t s, Graph& G, bool use, F&& body)
+{
+    if (!use) {
+        body();
+        return;
+    }
+    if (!G.x) {
+        VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(s, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        VF_HIP(hipStreamEndCapture(s, &G.g));
+        VF_HIP(hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0));
+    }
+    VF_HIP(hipGraphLaunch(G.x, s));
+}
+
+bool all_drained(const Table& t, const uint32_t* w)
+{
+    for (size_t i = 0; i < t.jobs.size(); ++i)
+        if (t.jobs[i]->d.algo == VF_FLOOD && w[16 * i + 2 * kStatFloodDone] == 0 &&
+            w[16 * i + 2 * kStatFloodDone + 1] == 0 && w[16 * i + 2 * kStatError] == 0)
+            return false;
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- batch ---
+struct vf_batch {
+    Engine eng;
+    bool use_graphs = true;
+    uint64_t sig = 0;
+    std::vector<EJob> jobs;          // every submitted job (invalid ones keep their status)
+    std::vector<EJob*> live;         // jobs on the device
+    Table* table = nullptr;
+    Graph g_front, g_levels, g_post;
+
+    void clear()
+    {
+        g_front.reset();
+        g_levels.reset();
+        g_post.reset();
+        if (table) eng.drop_table(table);
+        table = nullptr;
+        for (auto* e : live) eng.release(*e);
+        live.clear();
+        jobs.clear();
+        (void)hipStreamSynchronize(eng.s);
+        sig = 0;
+    }
+    ~vf_batch() { clear(); }
+};
+
+static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
+{
+    const uint64_t sig = job_signature(jobs, n);
+    if (sig == b->sig && b->jobs.size() == n) return;
+    b->clear();
+    b->jobs.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        EJob& e = b->jobs[i];
+        e.spec = jobs[i];
+        std::string why;
+        e.res = vf_result{};
+        e.res.status = job_validate(e.spec, why);
+        e.res.voxels = (uint64_t)e.spec.dims[0] * e.spec.dims[1] * e.spec.dims[2];
+        if (e.res.status == VF_OK) b->live.push_back(&e);
+    }
+    if (b->live.size() > Engine::kMaxTableJobs) fail(VF_ERR_INVALID_ARG, "at most %u jobs per batch", Engine::kMaxTableJobs);
+    for (auto* e : b->live) b->eng.alloc(*e);
+    if (!b->live.empty()) {
+        b->table = b->eng.make_table(b->live);
+        b->eng.reserve_front(*b->table);
+    }
+    b->sig = sig;
+}
+
+extern "C" {
+
+int vf_batch_create(vf_batch** out)
+{
+    return guard([&] {
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
+        *out = new vf_batch();
+    });
+}
+
+int vf_batch_destroy(vf_batch* b)
+{
+    return guard([&] { delete b; });
+}
+
+int vf_batch_set_graphs(vf_batch* b, int use_graphs)
+{
+    return guard([&] {
+        if (!b) fail(VF_ERR_INVALID_ARG, "batch is NULL");
+        b->clear();
+        b->use_graphs = use_graphs != 0;
+    });
+}
+
+int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results)
+{
+    return guard([&] {
+        if (!b || (n && (!jobs || !results))) fail(VF_ERR_INVALID_ARG, "NULL argument");
+        const double t0 = now_ms();
+        batch_prepare(b, jobs, n);
+        Engine& E = b->eng;
+        if (b->table) {
+            Table& T = *b->table;
+            run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T); });
+            if (T.any_flood) {
+                // the first chunks run without a check; then one readback per chunk
+                uint32_t chunks = 0;
+                for (;;) {
+                    run_captured(E.s, b->g_levels, b->use_graphs, [&] { E.levels(T, kLevelChunk); });
+                    if (++chunks < 2) continue;
+                    E.gather(T, E.pinned);
+                    VF_HIP(hipStreamSynchronize(E.s));
+                    if (all_drained(T, E.pinned)) break;
+                    if ((uint64_t)chunks * kLevelChunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
+                }
+            }
+            run_captured(E.s, b->g_post, b->use_graphs, [&] { E.post(T); });
+            VF_HIP(hipMemcpyAsync(E.pinned, T.gather, 16 * 4 * (size_t)T.L.njobs, hipMemcpyDeviceToHost, E.s));
+            VF_HIP(hipStreamSynchronize(E.s));
+            std::vector<uint32_t> w(E.pinned, E.pinned + 16 * T.L.njobs);
+            for (size_t i = 0; i < T.jobs.size(); ++i) E.finish(*T.jobs[i], w.data() + 16 * i);
+        }
+        const double t1 = now_ms();
+        for (uint32_t i = 0; i < n; ++i) {
+            results[i] = b->jobs[i].res;
+            results[i].latency_ms = t1 - t0;
+            results[i].queue_ms = 0;
+        }
+    });
+}
+
+int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results)
+{
+    return guard([&] {
+        if (n && (!jobs || !results)) fail(VF_ERR_INVALID_ARG, "NULL argument");
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        static Engine* E = nullptr;
+        if (!E) E = new Engine();
+        for (uint32_t i = 0; i < n; ++i) {
+            const double t0 = now_ms();
+            EJob e;
+            e.spec = jobs[i];
+            std::string why;
+            e.res = vf_result{};
+            e.res.voxels = (uint64_t)e.spec.dims[0] * e.spec.dims[1] * e.spec.dims[2];
+            e.res.status = job_validate(e.spec, why);
+            if (e.res.status == VF_OK) {
+                E->alloc(e);
+                std::vector<EJob*> one{ &e };
+                Table* T = E->make_table(one);
+                E->reserve_front(*T);
+                E->front(*T);
+                if (T->any_flood) {
+                    uint32_t chunk = 4, total = 0;
+                    for (;;) {
+                        E->levels(*T, chunk);
+                        total += chunk;
+                        E->gather(*T, E->pinned);
+                        VF_HIP(hipStreamSynchronize(E->s));
+                        if (all_drained(*T, E->pinned)) break;
+                        chunk = std::min<uint32_t>(chunk * 2, 64);
+                        if (total > (1u << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
+                    }
+                }
+                E->post(*T);
+                VF_HIP(hipMemcpyAsync(E->pinned, T->gather, 64, hipMemcpyDeviceToHost, E->s));
+                VF_HIP(hipStreamSynchronize(E->s));
+                std::vector<uint32_t> w(E->pinned, E->pinned + 16);
+                E->finish(e, w.data());
+                E->drop_table(T);
+                E->release(e);
+                VF_HIP(hipStreamSynchronize(E->s));
+            }
+            results[i] = e.res;
+            results[i].latency_ms = now_ms() - t0;
+            results[i].queue_ms = 0;
+        }
+    });
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------- server ---
+struct vf_server {
+    size_t budget = 0;
+    uint32_t max_slots = 0;
+    std::mutex mu;
+    std::condition_variable cv;       // wakes the scheduler
+    std::condition_variable done_cv;  // wakes waiters
+    std::deque<EJob*> queue;
+    std::unordered_map<uint64_t, EJob*> jobs;  // every ticket not yet collected
+    size_t reserved = 0;
+    uint32_t in_flight = 0;
+    uint64_t completed = 0;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    int device = 0;
+    std::thread th;
+    std::string fatal;  // scheduler failure, reported to every caller
+
+    void loop();
+};
+
+void vf_server::loop()
+{
+    (void)hipSetDevice(device);
+    try {
+        Engine E;
+        std::vector<EJob*> leveling;  // flood jobs between front and drain
+        Table* lev = nullptr;
+        bool lev_dirty = false;
+        auto complete = [&](EJob* e, int status) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (e->mem) {
+                reserved -= e->bytes;
+                --in_flight;
+            }
+            e->res.status = e->cancelled ? VF_ERR_CANCELLED : status;
+            e->res.latency_ms = now_ms() - e->t_submit;
+            e->res.queue_ms = (e->t_admit > 0 ? e->t_admit : now_ms()) - e->t_submit;
+            e->phase = 2;
+            ++completed;
+            done_cv.notify_all();
+        };
+        for (;;) {
+            std::vector<EJob*> admitted;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (queue.empty() && leveling.empty())
+                    cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !queue.empty(); });
+                if (stop && queue.empty() && leveling.empty()) break;
+                while (!queue.empty() && in_flight + admitted.size() < max_slots) {
+                    EJob* e = queue.front();
+                    if (e->cancelled) {
+                        queue.pop_front();
+                        e->res.status = VF_ERR_CANCELLED;
+                        e->res.latency_ms = now_ms() - e->t_submit;
+                        e->phase = 2;
+                        ++completed;
+                        done_cv.notify_all();
+                        continue;
+                    }
+                    const size_t fp = job_footprint(e->spec);
+                    if (fp > budget) {  // can never fit
+                        queue.pop_front();
+                        e->res.status = VF_ERR_BUDGET;
+                        e->res.latency_ms = now_ms() - e->t_submit;
+                        e->phase = 2;
+                        ++completed;
+                        done_cv.notify_all();
+                        continue;
+                    }
+                    if (reserved + fp > budget) break;  // wait for memory to free up
+                    queue.pop_front();
+                    reserved += fp;
+                    admitted.push_back(e);
+                }
+                in_flight += (uint32_t)admitted.size();
+            }
+            // front stages of the newly admitted jobs, in one set of launches
+            if (!admitted.empty()) {
+                for (auto* e : admitted) {
+                    e->t_admit = now_ms();
+                    E.alloc(*e);
+                }
+                Table* T = E.make_table(admitted);
+                E.reserve_front(*T);
+                E.front(*T);
+                E.drop_table(T);
+                std::vector<EJob*> naive_done;
+                for (auto* e : admitted) {
+                    if (e->d.algo == VF_FLOOD) {
+                        leveling.push_back(e);
+                        lev_dirty = true;
+                    } else {
+                        naive_done.push_back(e);
+                    }
+                }
+                if (!naive_done.empty()) {
+                    Table* P = E.make_table(naive_done);
+                    E.post(*P);
+                    VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * naive_done.size(), hipMemcpyDeviceToHost, E.s));
+                    VF_HIP(hipStreamSynchronize(E.s));
+                    std::vector<uint32_t> w(E.pinned, E.pinned + 16 * naive_done.size());
+                    for (size_t i = 0; i < naive_done.size(); ++i) {
+                        E.finish(*naive_done[i], w.data() + 16 * i);
+                        E.release(*naive_done[i]);
+                        complete(naive_done[i], naive_done[i]->res.status);
+                    }
+                    E.drop_table(P);
+                }
+            }
+            // one chunk of BFS levels for every flood job in flight
+            if (!leveling.empty()) {
+                if (lev_dirty || !lev) {
+                    if (lev) E.drop_table(lev);
+                    lev = E.make_table(leveling);
+                    lev_dirty = false;
+                }
+                E.levels(*lev, 8);
+                E.gather(*lev, E.pinned);
+                VF_HIP(hipStreamSynchronize(E.s));
+                std::vector<uint32_t> w(E.pinned, E.pinned + 16 * leveling.size());
+                std::vector<EJob*> drained, still;
+                for (size_t i = 0; i < leveling.size(); ++i) {
+                    const uint32_t* wi = w.data() + 16 * i;
+                    const bool done = wi[2 * kStatFloodDone] || wi[2 * kStatFloodDone + 1] || wi[2 * kStatError];
+                    (done ? drained : still).push_back(leveling[i]);
+                }
+                if (!drained.empty()) {
+                    Table* P = E.make_table(drained);
+                    E.post(*P);
+                    VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * drained.size(), hipMemcpyDeviceToHost, E.s));
+                    VF_HIP(hipStreamSynchronize(E.s));
+                    std::vector<uint32_t> wd(E.pinned, E.pinned + 16 * drained.size());
+                    for (size_t i = 0; i < drained.size(); ++i) {
+                        E.finish(*drained[i], wd.data() + 16 * i);
+                        E.release(*drained[i]);
+                        complete(drained[i], drained[i]->res.status);
+                    }
+                    E.drop_table(P);
+                    leveling.swap(still);
+                    lev_dirty = true;
+                }
+            }
+        }
+        if (lev) E.drop_table(lev);
+        VF_HIP(hipStreamSynchronize(E.s));
+    } catch (const Exception& ex) {
+        std::lock_guard<std::mutex> lk(mu);
+        fatal = vf_last_error();
+        for (auto& kv : jobs)
+            if (kv.second->phase != 2) {
+                kv.second->res.status = ex.code;
+                kv.second->phase = 2;
+            }
+        done_cv.notify_all();
+    }
+}
+
+extern "C" {
+
+int vf_server_create(size_t hbm_budget_bytes, uint32_t max_slots, vf_server** out)
+{
+    return guard([&] {
+        if (!out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        if (max_slots == 0 || max_slots > Engine::kMaxTableJobs) fail(VF_ERR_INVALID_ARG, "max_slots out of range");
+        if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
+        auto* s = new vf_server();
+        s->budget = hbm_budget_bytes;
+        s->max_slots = max_slots;
+        VF_HIP(hipGetDevice(&s->device));
+        s->th = std::thread([s] { s->loop(); });
+        *out = s;
+    });
+}
+
+int vf_server_destroy(vf_server* s)
+{
+    return guard([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        if (s->th.joinable()) s->th.join();
+        for (auto& kv : s->jobs) delete kv.second;
+        delete s;
+    });
+}
+
+int vf_server_submit(vf_server* s, const vf_job* job, uint64_t* ticket)
+{
+    return guard([&] {
+        if (!s || !job || !ticket) fail(VF_ERR_INVALID_ARG, "NULL argument");
+        auto* e = new EJob();
+        e->spec = *job;
+        std::string why;
+        const int st = job_validate(e->spec, why);
+        if (!e->spec.mesh_on_device && e->spec.nt) {
+            e->xyz_copy.assign(job->xyz, job->xyz + 3 * (size_t)job->nv);
+            e->tri_copy.assign(job->tri, job->tri + 3 * (size_t)job->nt);
+            e->spec.xyz = e->xyz_copy.data();
+            e->spec.tri = e->tri_copy.data();
+        }
+        e->spec.labels_out = job->labels_out;
+        e->spec.histogram_out = job->histogram_out;
+        e->t_submit = now_ms();
+        e->res.voxels = (uint64_t)job->dims[0] * job->dims[1] * job->dims[2];
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->fatal.empty()) fail(VF_ERR_DEVICE, "server stopped: %s", s->fatal.c_str());
+        e->ticket = s->next_ticket++;
+        *ticket = e->ticket;
+        s->jobs[e->ticket] = e;
+        if (st != VF_OK) {
+            e->res.status = st;
+            e->phase = 2;
+            ++s->completed;
+            s->done_cv.notify_all();
+            return;
+        }
+        s->queue.push_back(e);
+        s->cv.notify_all();
+    });
+}
+
+int vf_server_cancel(vf_server* s, uint64_t ticket)
+{
+    return guard([&] {
+        if (!s) fail(VF_ERR_INVALID_ARG, "server is NULL");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(ticket);
+        if (it == s->jobs.end()) fail(VF_ERR_INVALID_ARG, "unknown ticket");
+        it->second->cancelled = true;
+        s->cv.notify_all();
+    });
+}
+
+static int collect(vf_server* s, uint64_t ticket, vf_result* r, bool block)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) {
+        set_error("unknown ticket %llu", (unsigned long long)ticket);
+        return VF_ERR_INVALID_ARG;
+    }
+    EJob* e = it->second;
+    if (block) s->done_cv.wait(lk, [&] { return e->phase == 2; });
+    if (e->phase != 2) return VF_ERR_NOT_READY;
+    if (r) *r = e->res;
+    s->jobs.erase(it);
+    delete e;
+    return VF_OK;
+}
+
+int vf_server_poll(vf_server* s, uint64_t ticket, vf_result* result)
+{
+    if (!s) {
+        set_error("server is NULL");
+        return VF_ERR_INVALID_ARG;
+    }
+    return collect(s, ticket, result, false);
+}
+
+int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result)
+{
+    if (!s) {
+        set_error("server is NULL");
+        return VF_ERR_INVALID_ARG;
+    }
+    return collect(s, ticket, result, true);
+}
+
+int vf_server_stats(vf_server* s, size_t* reserved_bytes, uint32_t* in_flight, uint64_t* completed)
+{
+    return guard([&] {
+        if (!s) fail(VF_ERR_INVALID_ARG, "server is NULL");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (reserved_bytes) *reserved_bytes = s->reserved;
+        if (in_flight) *in_flight = s->in_flight;
+        if (completed) *completed = s->completed;
+    });
+}
+
+}  // extern "C"

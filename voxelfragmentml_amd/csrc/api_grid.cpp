@@ -54,7 +54,7 @@ static void grid_setup_shape(vf_grid* g, const uint32_t dims[3], const float mn[
     VF_HIP(hipMemsetAsync(g->occ.p, 0, g->V, g->stream));
     g->labels_from_occ = false;
     g->clean = true;
-    g->mc_valid = false;
+    g->mc.valid = false;
     JobDev& J = g->job;
     J = JobDev{};
     J.nx = dims[0];
@@ -236,7 +236,7 @@ int vf_grid_write(vf_grid* g, const uint16_t* in)
         VF_HIP(hipMemcpyAsync(g->labels.p, in, g->V * 2, hipMemcpyHostToDevice, g->stream));
         g->labels_from_occ = false;
         g->clean = false;
-        g->mc_valid = false;
+        g->mc.valid = false;
         launch_labels_to_occ(grid_launch(g));
         VF_HIP(hipStreamSynchronize(g->stream));
     });
@@ -344,7 +344,7 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
             g->labels_from_occ = true;
         }
         g->clean = false;
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -411,11 +411,12 @@ int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         const bool key64 = ((dmax + 1) << b) > 0xFFFFFFFFull;
         g->job.key_bits = b;
         g->job.dist = distance;
+        g->job.algo = VF_NAIVE;
+        g->job.naive_key64 = key64 ? 1 : 0;
         g->job_dirty = true;
-        const bool fast = (g->dims[2] % 8) == 0;
-        launch_naive_shaped(grid_launch(g), distance, fast, key64);
+        launch_naive(grid_launch(g), key64);
         g->labels_from_occ = false;
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -438,7 +439,7 @@ int vf_remove_isolated_regions(vf_grid* g, const vf_seed* seeds, uint32_t S)
                                g->seeds.as<uint4>(), S, g->pinned);
         VF_HIP(hipMemcpyAsync(g->labels.p, g->ri_out.p, g->V * 2, hipMemcpyDeviceToDevice, g->stream));
         launch_labels_to_occ(grid_launch(g));
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -475,6 +476,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         J.neighbourhood = nb;
         J.noise_seed = noise_seed;
         J.noise_amp = noise_amp;
+        J.algo = VF_FLOOD;
         g->job_dirty = true;
         Launch L = grid_launch(g);
         VF_HIP(hipMemsetAsync(g->fctl.as<uint32_t>() + 32, 0, 4, g->stream));  // blocks_done
@@ -495,7 +497,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         check_device_error(g, "flood");
         launch_flood_finish(L);
         g->labels_from_occ = false;
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -507,7 +509,7 @@ int vf_detect_boundaries(vf_grid* g, int size)
         if (size < 0 || size > 64) fail(VF_ERR_INVALID_ARG, "boundary size %d out of range", size);
         grid_materialize(g);
         launch_boundaries(grid_launch(g), size, 0);
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -518,7 +520,7 @@ int vf_undo_mask(vf_grid* g)
         checked(g);
         grid_materialize(g);
         launch_undo_mask(grid_launch(g), 0x7FFF);
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -529,7 +531,7 @@ int vf_homogenize(vf_grid* g)
         checked(g);
         grid_materialize(g);
         launch_homogenize(grid_launch(g));
-        g->mc_valid = false;
+        g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });
 }
@@ -565,7 +567,7 @@ int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint1
 {
     return guard([&] {
         checked(g);
-        if (!g->mc_valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes result: call vf_marching_cubes first");
+        if (!g->mc.valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes result: call vf_marching_cubes first");
         grid_mesh_read(g, xyz, tri, tri_flag, labels, vert_offsets, tri_offsets);
     });
 }

@@ -1,0 +1,330 @@
+// Engine implementation: per-job device blocks, table upload, and the stage
+// sequences.  All work is enqueued on the engine's stream; only gather
+// readbacks synchronise.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace vf {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct Layout {
+    size_t labels, occ, bits, seeds, hist, stats, fctl, keys, buckets, mesh_xyz, mesh_tri, total;
+    uint64_t V, words, cap;
+};
+
+int resolved_neigh(const vf_job& j)
+{
+    if (j.neighbourhood == VF_NEIGH_FROM_DISTANCE) return j.distance == VF_MANHATTAN ? 6 : 26;
+    return j.neighbourhood;
+}
+
+Layout layout_of(const vf_job& j)
+{
+    Layout L{};
+    L.V = (uint64_t)j.dims[0] * j.dims[1] * j.dims[2];
+    const uint32_t wp = next_pow2(ceil_div(j.dims[2], 32));
+    L.words = (uint64_t)j.dims[0] * j.dims[1] * wp;
+    const bool flood = j.algorithm == VF_FLOOD;
+    // a level's frontier holds at most the occupied cells; vessels occupy a few
+    // percent of the box, so a quarter of V (plus the seeds) is ample and an
+    // overflow is detected on the device
+    L.cap = std::min<uint64_t>(L.V + j.num_seeds + 64, L.V / 4 + j.num_seeds + 4096);
+    size_t o = 0;
+    L.labels = o; o = align_up(o + L.V * 2);
+    L.occ = o; o = align_up(o + L.V);
+    L.bits = o; o = align_up(o + L.words * 8);
+    L.seeds = o; o = align_up(o + (size_t)std::max<uint32_t>(j.num_seeds, 1) * 16);
+    L.hist = o; o = align_up(o + 65536 * 4);
+    L.stats = o; o = align_up(o + kStatCount * 8);
+    L.fctl = o; o = align_up(o + 64 * 4);
+    L.keys = o; if (flood) o = align_up(o + L.V * 4);
+    L.buckets = o; if (flood) o = align_up(o + L.cap * 4 * kLevelRing);
+    L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
+    L.mesh_tri = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nt * 12);
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+size_t job_footprint(const vf_job& j) { return layout_of(j).total; }
+
+int job_validate(const vf_job& j, std::string& why)
+{
+    if (j.poison) { why = "job flagged as poison"; return VF_ERR_POISON; }
+    for (int q = 0; q < 3; ++q)
+        if (j.dims[q] == 0 || j.dims[q] > 2048) { why = "grid edge out of range"; return VF_ERR_INVALID_ARG; }
+    if ((uint64_t)j.dims[0] * j.dims[1] * j.dims[2] >= (1ull << 31)) { why = "grid too large"; return VF_ERR_INVALID_ARG; }
+    for (int q = 0; q < 3; ++q)
+        if (!(j.aabb_max[q] > j.aabb_min[q])) { why = "empty aabb"; return VF_ERR_INVALID_ARG; }
+    if (j.nt && (!j.xyz || !j.tri)) { why = "mesh pointer is NULL"; return VF_ERR_INVALID_ARG; }
+    if (j.num_seeds == 0 || j.num_seeds > kMaxDeviceSeeds) { why = "seed count out of range"; return VF_ERR_INVALID_ARG; }
+    if (j.algorithm != VF_NAIVE && j.algorithm != VF_FLOOD) { why = "unknown algorithm"; return VF_ERR_INVALID_ARG; }
+    if (j.distance < VF_EUCLIDEAN || j.distance > VF_CHEBYSHEV) { why = "unknown distance"; return VF_ERR_INVALID_ARG; }
+    if (j.seed_location < VF_SEED_BOTH || j.seed_location > VF_SEED_INNER) { why = "unknown seed location"; return VF_ERR_INVALID_ARG; }
+    if (j.boundary_size < 0 || j.boundary_size > 64) { why = "boundary size out of range"; return VF_ERR_INVALID_ARG; }
+    if (j.algorithm == VF_FLOOD) {
+        if (j.num_seeds > 254) { why = "flood labels must fit 8 bits"; return VF_ERR_TOO_MANY_SEEDS; }
+        const int nb = resolved_neigh(j);
+        if (nb != 6 && nb != 26) { why = "neighbourhood must be 0, 6 or 26"; return VF_ERR_INVALID_ARG; }
+        if (j.noise_amp > kMaxNoiseAmp) { why = "noise amplitude too large"; return VF_ERR_INVALID_ARG; }
+    }
+    if (!j.mesh_on_device)
+        for (uint64_t i = 0; i < 3ull * j.nt; ++i)
+            if (j.tri[i] >= j.nv) { why = "triangle index out of range"; return VF_ERR_INVALID_ARG; }
+    return VF_OK;
+}
+
+Engine::Engine()
+{
+    VF_HIP(hipGetDevice(&device));
+    VF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    VF_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), 16 * 4 * kMaxTableJobs));
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t thr = ~0ull;  // keep freed blocks cached in the pool
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    (void)hipGetLastError();
+}
+
+Engine::~Engine()
+{
+    if (s) (void)hipStreamSynchronize(s);
+    if (pinned) (void)hipHostFree(pinned);
+    if (s) (void)hipStreamDestroy(s);
+}
+
+void Engine::alloc(EJob& e)
+{
+    const vf_job& j = e.spec;
+    const Layout Lo = layout_of(j);
+    e.bytes = Lo.total;
+    VF_HIP(hipMallocAsync(&e.mem, e.bytes, s));
+    auto* base = static_cast<uint8_t*>(e.mem);
+    JobDev& d = e.d;
+    d = JobDev{};
+    d.nx = j.dims[0];
+    d.ny = j.dims[1];
+    d.nz = j.dims[2];
+    d.wp = next_pow2(ceil_div(j.dims[2], 32));
+    d.V = Lo.V;
+    for (int q = 0; q < 3; ++q) {
+        d.mn[q] = j.aabb_min[q];
+        d.mx[q] = j.aabb_max[q];
+        d.cs[q] = (j.aabb_max[q] - j.aabb_min[q]) / (float)j.dims[q];  // RegularGrid::setAABB
+    }
+    d.labels = reinterpret_cast<uint16_t*>(base + Lo.labels);
+    d.occ = base + Lo.occ;
+    d.shell_bits = reinterpret_cast<uint32_t*>(base + Lo.bits);
+    d.toggle_bits = d.shell_bits + Lo.words;
+    d.seeds = reinterpret_cast<uint4*>(base + Lo.seeds);
+    d.S = j.num_seeds;
+    d.dist = j.distance;
+    d.key_bits = key_bits_for(j.num_seeds);
+    d.neighbourhood = resolved_neigh(j);
+    d.noise_seed = j.noise_seed;
+    d.noise_amp = j.noise_amp;
+    d.seed_location = j.seed_location;
+    d.seed_key = j.seed;
+    d.stats = reinterpret_cast<unsigned long long*>(base + Lo.stats);
+    d.hist = reinterpret_cast<uint32_t*>(base + Lo.hist);
+    d.fctl = reinterpret_cast<uint32_t*>(base + Lo.fctl);
+    d.active = 1;
+    d.algo = j.algorithm;
+    d.bsize = j.boundary_size;
+    if (j.algorithm == VF_FLOOD) {
+        d.keys = reinterpret_cast<uint32_t*>(base + Lo.keys);
+        d.buckets = reinterpret_cast<uint32_t*>(base + Lo.buckets);
+        d.bucket_cap = (uint32_t)Lo.cap;
+    } else {
+        const uint64_t ex = d.nx - 1, ey = d.ny - 1, ez = d.nz - 1;
+        const uint64_t dmax = j.distance == VF_MANHATTAN ? ex + ey + ez
+                              : j.distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
+                                                           : ex * ex + ey * ey + ez * ez;
+        d.naive_key64 = ((dmax + 1) << d.key_bits) > 0xFFFFFFFFull ? 1 : 0;
+    }
+    if (j.mesh_on_device) {
+        d.xyz = j.xyz;
+        d.tri = j.tri;
+    } else {
+        d.xyz = reinterpret_cast<const float*>(base + Lo.mesh_xyz);
+        d.tri = reinterpret_cast<const uint32_t*>(base + Lo.mesh_tri);
+        if (j.nv) VF_HIP(hipMemcpyAsync(base + Lo.mesh_xyz, j.xyz, (size_t)j.nv * 12, hipMemcpyHostToDevice, s));
+        if (j.nt) VF_HIP(hipMemcpyAsync(base + Lo.mesh_tri, j.tri, (size_t)j.nt * 12, hipMemcpyHostToDevice, s));
+    }
+    d.nt = j.nt;
+    e.words = Lo.words;
+    e.tiles = ceil_div(Lo.V, kTileCells);
+    e.groups = ceil_div(Lo.words, 256);
+}
+
+void Engine::release(EJob& e)
+{
+    if (e.mem) VF_HIP(hipFreeAsync(e.mem, s));
+    e.mem = nullptr;
+}
+
+Table* Engine::make_table(const std::vector<EJob*>& js)
+{
+    const uint32_t n = (uint32_t)js.size();
+    if (n == 0 || n > kMaxTableJobs) fail(VF_ERR_INVALID_ARG, "table of %u jobs (1..%u)", n, kMaxTableJobs);
+    auto* t = new Table();
+    t->jobs = js;
+    const size_t o_jobs = 0;
+    const size_t o_tile = align_up(o_jobs + n * sizeof(JobDev));
+    const size_t o_tri = align_up(o_tile + (n + 1) * 4);
+    const size_t o_col = align_up(o_tri + (n + 1) * 4);
+    const size_t o_gather = align_up(o_col + (n + 1) * 4);
+    const size_t o_done = align_up(o_gather + 16 * 4 * (size_t)n);
+    const size_t total = align_up(o_done + 64);
+    std::vector<uint8_t> img(total, 0);
+    auto* jd = reinterpret_cast<JobDev*>(img.data() + o_jobs);
+    auto* tp = reinterpret_cast<uint32_t*>(img.data() + o_tile);
+    auto* rp = reinterpret_cast<uint32_t*>(img.data() + o_tri);
+    auto* cp = reinterpret_cast<uint32_t*>(img.data() + o_col);
+    uint64_t tiles = 0, tris = 0, groups = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        EJob& e = *js[i];
+        jd[i] = e.d;
+        tp[i] = (uint32_t)tiles;
+        rp[i] = (uint32_t)tris;
+        cp[i] = (uint32_t)groups;
+        tiles += e.tiles;
+        tris += e.d.nt;
+        groups += e.groups;
+        t->any_flood |= e.d.algo == VF_FLOOD;
+        t->any_naive32 |= e.d.algo == VF_NAIVE && !e.d.naive_key64;
+        t->any_naive64 |= e.d.algo == VF_NAIVE && e.d.naive_key64;
+    }
+    if (tiles > 0xFFFFFFFFull || tris > 0xFFFFFFFFull || groups > 0xFFFFFFFFull)
+        fail(VF_ERR_INVALID_ARG, "table too large for 32-bit prefixes");
+    tp[n] = (uint32_t)tiles;
+    rp[n] = (uint32_t)tris;
+    cp[n] = (uint32_t)groups;
+    VF_HIP(hipMallocAsync(&t->dev, total, s));
+    VF_HIP(hipMemcpyAsync(t->dev, img.data(), total, hipMemcpyHostToDevice, s));
+    VF_HIP(hipStreamSynchronize(s));  // the pageable source must outlive the copy
+    auto* base = static_cast<uint8_t*>(t->dev);
+    t->L.jobs = reinterpret_cast<const JobDev*>(base + o_jobs);
+    t->L.njobs = n;
+    t->L.tile_prefix = reinterpret_cast<const uint32_t*>(base + o_tile);
+    t->L.total_tiles = (uint32_t)tiles;
+    t->L.s = s;
+    t->tri_prefix = reinterpret_cast<const uint32_t*>(base + o_tri);
+    t->col_prefix = reinterpret_cast<const uint32_t*>(base + o_col);
+    t->total_tris = (uint32_t)tris;
+    t->total_groups = (uint32_t)groups;
+    t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
+    t->blocks_done = reinterpret_cast<uint32_t*>(base + o_done);
+    return t;
+}
+
+void Engine::drop_table(Table* t)
+{
+    if (!t) return;
+    if (t->dev) VF_HIP(hipFreeAsync(t->dev, s));
+    delete t;
+}
+
+void Engine::reserve_front(const Table& t)
+{
+    const uint32_t nt = t.total_tris;
+    vox_setup_.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup));
+    vox_counts_.ensure((size_t)std::max<uint32_t>(nt, 1) * 8);
+    vox_shell_off_.ensure(((size_t)nt + 1) * 8);
+    vox_col_off_.ensure(((size_t)nt + 1) * 8);
+    scan_tmp_.ensure(std::max(scan_tmp_bytes(nt), scan_tmp_bytes(t.L.total_tiles)) + 64);
+    seed_cnt_.ensure((size_t)t.L.total_tiles * 4 + 64);
+    seed_off_.ensure(((size_t)t.L.total_tiles + 1) * 8 + 64);
+}
+
+// Launch-only (safe inside stream capture): reserve_front must have sized the scratch.
+void Engine::front(Table& t)
+{
+    const Launch& L = t.L;
+    launch_job_init(L);
+    VoxScratch X;
+    const uint32_t nt = t.total_tris;
+    X.setup = vox_setup_.as<TriSetup>();
+    X.counts = vox_counts_.as<uint32_t>();
+    X.shell_off = vox_shell_off_.as<unsigned long long>();
+    X.col_off = vox_col_off_.as<unsigned long long>();
+    X.tri_prefix = t.tri_prefix;
+    X.total_tris = nt;
+    X.scan_tmp = scan_tmp_.p;
+    X.scan_tmp_bytes = scan_tmp_.bytes;
+    launch_voxelize(L, X, t.col_prefix, t.total_groups);
+    launch_seed_count(L, seed_cnt_.as<uint32_t>());
+    scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
+    launch_seed_pick(L, seed_off_.as<unsigned long long>());
+    if (t.any_naive32) launch_naive(L, false);
+    if (t.any_naive64) launch_naive(L, true);
+    if (t.any_flood) {
+        launch_flood_init(L);
+        launch_flood_seed(L, 0);
+    }
+}
+
+void Engine::levels(Table& t, uint32_t n)
+{
+    for (uint32_t i = 0; i < n; ++i) launch_flood_level(t.L, t.blocks_done, level_blocks);
+}
+
+void Engine::post(Table& t)
+{
+    const Launch& L = t.L;
+    if (t.any_flood) launch_flood_finish(L);
+    launch_boundaries(L, -1, 1);
+    launch_hist_summary(L);
+    launch_checksum(L);
+    launch_gather(L, t.gather);
+}
+
+void Engine::gather(Table& t, uint32_t* host16)
+{
+    launch_gather(t.L, t.gather);
+    VF_HIP(hipMemcpyAsync(host16, t.gather, 16 * 4 * (size_t)t.L.njobs, hipMemcpyDeviceToHost, s));
+}
+
+void Engine::finish(EJob& e, const uint32_t* w)
+{
+    auto stat = [&](int k) { return (uint64_t)w[2 * k] | ((uint64_t)w[2 * k + 1] << 32); };
+    vf_result& r = e.res;
+    r.voxels = e.d.V;
+    r.occupied = stat(kStatOccupied);
+    r.checksum = stat(kStatChecksum);
+    r.num_labels = (uint32_t)stat(kStatLabels);
+    r.flood_levels = (uint32_t)stat(kStatFloodDone);
+    const uint64_t err = stat(kStatError);
+    r.status = VF_OK;
+    if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;
+    else if (err) r.status = VF_ERR_OVERFLOW;
+    if (r.status == VF_OK && e.spec.marching_cubes) {
+        vf_mesh_info info{};
+        const uint32_t dims[3] = { e.d.nx, e.d.ny, e.d.nz };
+        mc_run(mc_, s, e.d.labels, dims, e.d.mn, e.d.cs, e.d.hist, VF_ALL_LABELS, &info);
+        r.mc_vertices = info.num_vertices;
+        r.mc_triangles = info.num_triangles;
+    }
+    if (e.spec.labels_out)
+        VF_HIP(hipMemcpyAsync(e.spec.labels_out, e.d.labels, e.d.V * 2, hipMemcpyDeviceToHost, s));
+    if (e.spec.histogram_out)
+        VF_HIP(hipMemcpyAsync(e.spec.histogram_out, e.d.hist, 65536 * 4, hipMemcpyDeviceToHost, s));
+    if (e.spec.labels_out || e.spec.histogram_out) VF_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace vf

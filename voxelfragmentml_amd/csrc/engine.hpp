@@ -1,0 +1,87 @@
+// Multi-job execution engine shared by the batched, plain and served forms.
+//
+// A fracture job runs voxelize -> seed -> (naive | flood levels) -> flood
+// finish -> boundaries + histogram -> checksum [-> marching cubes], the
+// sequence CADScene::fractureModel + prepareScene drives
+// (MF/Source/Graphics/Application/CADScene.cpp:624-691, :791-833), with the
+// grid resident on the device throughout.  Every stage is one launch over a
+// Table: a compact array of job descriptors plus prefix tables, so any subset
+// of jobs (one for the plain form, a whole batch, or whichever jobs a server
+// tick has in that phase) shares each launch.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "jobs.hpp"
+
+namespace vf {
+
+struct EJob {
+    vf_job spec{};
+    std::vector<float> xyz_copy;     // server: the job owns a copy of its mesh
+    std::vector<uint32_t> tri_copy;
+    JobDev d{};
+    void* mem = nullptr;             // one device block per job
+    size_t bytes = 0;
+    uint64_t words = 0;
+    uint32_t tiles = 0, groups = 0;
+    vf_result res{};
+    uint64_t ticket = 0;
+    double t_submit = 0, t_admit = 0;
+    int phase = 0;
+    bool cancelled = false;
+};
+
+// Device bytes a job needs while resident (grid, occupancy, bitmasks, flood
+// keys and buckets, seeds, histogram, mesh copy).
+size_t job_footprint(const vf_job& j);
+// VF_OK or the status the job fails with before touching the device.
+int job_validate(const vf_job& j, std::string& why);
+
+struct Table {
+    std::vector<EJob*> jobs;
+    void* dev = nullptr;
+    Launch L{};
+    const uint32_t* tri_prefix = nullptr;
+    const uint32_t* col_prefix = nullptr;
+    uint32_t total_tris = 0, total_groups = 0;
+    uint32_t* gather = nullptr;       // 16 words per job
+    uint32_t* blocks_done = nullptr;
+    bool any_flood = false, any_naive32 = false, any_naive64 = false;
+};
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    hipStream_t s = nullptr;
+    int device = 0;
+
+    void alloc(EJob& j);    // stream-ordered allocation + descriptor + mesh upload
+    void release(EJob& j);  // stream-ordered free
+    Table* make_table(const std::vector<EJob*>& js);
+    void drop_table(Table* t);
+    void reserve_front(const Table& t);  // size the front scratch (not capturable)
+    void front(Table& t);               // launches only
+    void levels(Table& t, uint32_t n);
+    void post(Table& t);
+    void gather(Table& t, uint32_t* host16);  // async copy of the gathered stats
+    // fill j.res from 16 gathered words; runs marching cubes and optional copies
+    void finish(EJob& j, const uint32_t* w16);
+
+    uint32_t* pinned = nullptr;     // gathered stats staging (16 words x kMaxTableJobs)
+    static constexpr uint32_t kMaxTableJobs = 1024;
+    uint32_t level_blocks = 512;
+
+private:
+    DevBuf vox_setup_, vox_counts_, vox_shell_off_, vox_col_off_, scan_tmp_, seed_cnt_, seed_off_;
+    McScratch mc_;
+};
+
+double now_ms();
+
+}  // namespace vf

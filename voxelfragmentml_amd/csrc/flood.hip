@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
     for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
         const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
         const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
+        if (!J->active || J->algo != VF_FLOOD) continue;
         const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J->V) continue;
         const uint64_t o = load8b(J->occ, c0, J->V);
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
 __global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
 {
     const JobDev* J = L.jobs + blockIdx.x;
-    if (!J->active) return;
+    if (!J->active || J->algo != VF_FLOOD) return;
     const uint32_t S = J->S, b = J->key_bits;
     for (uint32_t i = threadIdx.x; i < kFloodCtlWords; i += blockDim.x) J->fctl[i] = 0;
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
@@ -98,6 +98,7 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
     if (threadIdx.x == 0) {
         J->fctl[0] = 1;
         J->fctl[1] = npush ? 0u : 1u;
+        J->stats[kStatFloodDone] = npush ? 0ull : 1ull;
         J->fctl[2 + 1 % 8] = npush;
     }
 }
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_
             const JobDev* J = L.jobs + j;
             const uint32_t d = J->fctl[0];
             lev[j] = d;
-            n = (J->active && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
+            n = (J->active && J->algo == VF_FLOOD && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
             if (n > J->bucket_cap) n = J->bucket_cap;
         }
         // block exclusive scan of n
@@ -237,12 +238,15 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_
     __threadfence();
     for (uint32_t j = threadIdx.x; j < nj; j += blockDim.x) {
         const JobDev* J = L.jobs + j;
-        if (!J->active || J->fctl[1] != 0) continue;
+        if (!J->active || J->algo != VF_FLOOD || J->fctl[1] != 0) continue;
         const uint32_t d = lev[j];
         volatile uint32_t* c = J->fctl + 2;
         c[(d + 4) & 7u] = 0u;
         J->fctl[0] = d + 1;
-        if (c[(d + 1) & 7u] == 0u && c[(d + 2) & 7u] == 0u && c[(d + 3) & 7u] == 0u) J->fctl[1] = d + 1;
+        if (c[(d + 1) & 7u] == 0u && c[(d + 2) & 7u] == 0u && c[(d + 3) & 7u] == 0u) {
+            J->fctl[1] = d + 1;
+            J->stats[kStatFloodDone] = d + 1;
+        }
     }
     if (threadIdx.x == 0) *blocks_done = 0;
 }
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
     for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
         const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
         const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
+        if (!J->active || J->algo != VF_FLOOD) continue;
         const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J->V) continue;
         const uint32_t mask = (1u << J->key_bits) - 1u;

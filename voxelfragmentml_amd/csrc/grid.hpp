@@ -18,10 +18,7 @@ struct vf_grid {
     vf::DevBuf seed_tile_cnt, seed_tile_off;
     vf::DevBuf keys, buckets, fctl, ctl;
     vf::DevBuf ri_a, ri_b, ri_cnt, ri_vis, ri_out;
-    // marching cubes results (mc.hip)
-    vf::DevBuf mc_scratch[16];
-    uint32_t mc_nv = 0, mc_nt = 0, mc_nl = 0;
-    bool mc_valid = false;
+    vf::McScratch mc;              // marching-cubes workspace and last result
 
     bool labels_from_occ = false;  // labels equal (occ ? FREE : EMPTY) but are not written yet
     bool clean = true;             // every cell EMPTY
@@ -40,7 +37,6 @@ uint32_t key_bits_for(uint32_t S);
 uint64_t read_stat(vf_grid* g, int slot);
 void check_device_error(vf_grid* g, const char* stage);
 
-// mc.hip
 void grid_marching_cubes(vf_grid* g, uint32_t label, vf_mesh_info* info);
 void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
                     uint32_t* toff);

@@ -138,10 +138,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
         if (!any) continue;
         bool changed = false;
         if (BOUND) {
+            const int bs = bsize >= 0 ? bsize : J->bsize;
             uint32_t x, y, z;
             decompose(c0, J, x, y, z);
             for (int k = 0; k < 8; ++k) {
-                if (c0 + k < J->V && v[k] > kFree && is_boundary_cell(J, x, y, z, v[k], bsize)) {
+                if (c0 + k < J->V && v[k] > kFree && is_boundary_cell(J, x, y, z, v[k], bs)) {
                     v[k] = (uint16_t)(v[k] | 0x8000u);
                     changed = true;
                 }
@@ -239,6 +240,43 @@ __global__ __launch_bounds__(kCellThreads) void k_count_occupied(Launch L)
     flush(cur_job);
 }
 
+// One block per job: number of labels >= 2 with a nonzero count.
+__global__ __launch_bounds__(256) void k_hist_summary(Launch L)
+{
+    __shared__ uint32_t wsum[4];
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active) return;
+    uint32_t n = 0;
+    for (uint32_t i = 2 + threadIdx.x; i < 65536; i += blockDim.x) n += J->hist[i] != 0;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) J->stats[kStatLabels] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Block per job: zero the per-job statistics, histogram and flood control.
+__global__ __launch_bounds__(256) void k_job_init(Launch L)
+{
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active) return;
+    for (uint32_t i = threadIdx.x; i < 65536; i += blockDim.x) J->hist[i] = 0;
+    if (threadIdx.x < kStatCount) J->stats[threadIdx.x] = 0;
+    if (J->fctl && threadIdx.x < 64) J->fctl[threadIdx.x] = 0;
+}
+
+// Block per job: stats and flood state into a compact table for one readback.
+__global__ void k_gather(Launch L, uint32_t* out)
+{
+    const JobDev* J = L.jobs + blockIdx.x;
+    uint32_t* o = out + 16 * (size_t)blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    if (t < kStatCount) {
+        const unsigned long long v = J->active ? J->stats[t] : 0ull;
+        o[2 * t] = (uint32_t)v;
+        o[2 * t + 1] = (uint32_t)(v >> 32);
+    }
+}
+
 inline uint32_t grid_for(const Launch& L) { return L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048; }
 
 }  // namespace
@@ -301,6 +339,26 @@ void launch_checksum(const Launch& L)
 {
     TimedLaunch tl("checksum", L.s);
     k_checksum<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_job_init(const Launch& L)
+{
+    TimedLaunch tl("job_init", L.s);
+    k_job_init<<<L.njobs, 256, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_gather(const Launch& L, uint32_t* out)
+{
+    k_gather<<<L.njobs, 64, 0, L.s>>>(L, out);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_hist_summary(const Launch& L)
+{
+    TimedLaunch tl("hist_summary", L.s);
+    k_hist_summary<<<L.njobs, 256, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

@@ -23,6 +23,7 @@ enum StatSlot : int {
     kStatError = 4,      // first error flag raised on the device
     kStatFloodMax = 5,   // deepest BFS level with a pushed cell
     kStatFloodDone = 6,  // level at which the flood drained
+    kStatLabels = 7,     // labels >= 2 with a nonzero count
     kStatCount = 8
 };
 
@@ -72,7 +73,12 @@ struct JobDev {
     unsigned long long* stats;  // kStatCount slots
     uint32_t* hist;         // 65536 counts
     int active;             // 0: the job is skipped by multi-job launches
+    int algo;               // VF_NAIVE / VF_FLOOD: selects which fracture launches process the job
+    int bsize;              // detectBoundaries window half-size used when a launch passes size < 0
+    int naive_key64;        // naive packed keys need 64 bits
 };
+
+uint32_t key_bits_for(uint32_t S);  // bits of the seed index in packed keys
 
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t njobs, uint32_t t)
 {
@@ -128,12 +134,17 @@ void launch_homogenize(const Launch& L);
 void launch_undo_mask(const Launch& L, uint16_t keep_mask);
 void launch_clear_hist(const Launch& L);
 void launch_histogram(const Launch& L);
-void launch_boundaries(const Launch& L, int size, int with_hist);
+void launch_boundaries(const Launch& L, int size /* < 0: per-job bsize */, int with_hist);
+void launch_job_init(const Launch& L);     // zero stats, histogram and flood control of every job
+void launch_gather(const Launch& L, uint32_t* out);  // 16 words per job: stats (low 32 bits) + flood state
 void launch_checksum(const Launch& L);
 void launch_count_occupied(const Launch& L);
+void launch_hist_summary(const Launch& L);  // block per job: kStatLabels
 
 // naive.hip
-void launch_naive_shaped(const Launch& L, int dist, bool fast, bool key64);
+// runs every job with algo == VF_NAIVE and matching key width; distance and
+// the z-run fast path are chosen per job inside the kernel
+void launch_naive(const Launch& L, bool key64);
 void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint32_t* frontier_a, uint32_t* frontier_b,
                             uint32_t* counters, uint32_t* visited, const uint32_t dims[3], const uint4* seeds, uint32_t S,
                             uint32_t* host_counter);
@@ -153,6 +164,18 @@ constexpr uint32_t kMaxDeviceSeeds = 4096;
 void launch_seed_count(const Launch& L, uint32_t* tile_cnt);
 // tile_off: exclusive prefix over the global tiles (total_tiles + 1 entries)
 void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
+
+// mc.hip: all-labels marching cubes on a raw device grid.  hist_dev holds the
+// grid's countValues histogram (65536 counts); results stay in W until read.
+struct McScratch {
+    DevBuf buf[16];
+    uint32_t nv = 0, nt = 0, nl = 0;
+    bool valid = false;
+};
+void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
+            const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info);
+void mc_read(McScratch& W, hipStream_t s, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
+             uint32_t* toff);
 
 // scan (scan.hip): exclusive prefix sums, graph-capturable, no host sync
 size_t scan_tmp_bytes(uint64_t n);

@@ -331,19 +331,16 @@ bool g_table_loaded = false;
 
 }  // namespace
 
-void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
+void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
+            const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info)
 {
-    hipStream_t s = g->stream;
     if (!g_table_loaded) {
         VF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri), kMcTriPacked, sizeof(kMcTriPacked)));
         g_table_loaded = true;
     }
     // target labels: countValues (RegularGrid.cpp:601-625) -> labels with a nonzero count
-    Launch L = grid_launch(g);
-    launch_clear_hist(L);
-    launch_histogram(L);
     std::vector<uint32_t> hist(65536);
-    VF_HIP(hipMemcpyAsync(hist.data(), g->hist.p, 65536 * 4, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipMemcpyAsync(hist.data(), hist_dev, 65536 * 4, hipMemcpyDeviceToHost, s));
     VF_HIP(hipStreamSynchronize(s));
     std::vector<uint16_t> labels, lindex(65536, (uint16_t)kMcNoLabel);
     for (uint32_t l = 0; l < 65536; ++l)
@@ -353,12 +350,13 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
         }
     const uint32_t nl = (uint32_t)labels.size();
     if (nl > kMcMaxLabels) fail(VF_ERR_INVALID_ARG, "marching cubes supports at most %u labels (grid has %u)", kMcMaxLabels, nl);
-    const uint64_t Nx = g->dims[0] + 2ull, Ny = g->dims[1] + 2ull, Nz = g->dims[2] + 2ull;
+    const uint64_t Nx = dims[0] + 2ull, Ny = dims[1] + 2ull, Nz = dims[2] + 2ull;
     const uint64_t P = Nx * Ny * Nz;
     if (3 * P >= (1ull << 40)) fail(VF_ERR_INVALID_ARG, "grid too large for marching cubes");
-    g->mc_nl = nl;
-    g->mc_nv = g->mc_nt = 0;
-    auto& B = g->mc_scratch;
+    W.nl = nl;
+    W.nv = W.nt = 0;
+    W.valid = false;
+    auto& B = W.buf;
     B[0].ensure(65536 * 2);
     B[1].ensure((size_t)std::max<uint32_t>(nl, 1) * 2);
     VF_HIP(hipMemcpyAsync(B[0].p, lindex.data(), 65536 * 2, hipMemcpyHostToDevice, s));
@@ -369,7 +367,7 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
         VF_HIP(hipMemcpyAsync(B[11].p, z.data(), 16, hipMemcpyHostToDevice, s));
         VF_HIP(hipStreamSynchronize(s));
         info->num_vertices = info->num_triangles = info->num_labels = 0;
-        g->mc_valid = true;
+        W.valid = true;
         return;
     }
     // tiles: keep the (label, tile) matrices near 2M entries
@@ -382,11 +380,11 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
     B[4].ensure((size_t)tiles * 4 + 64);
     B[5].ensure((mat + 1) * 8 + 64);
     B[6].ensure((mat + 1) * 8 + 64);
-    g->scan_tmp.ensure(scan_tmp_bytes(mat) + 64);
+    B[13].ensure(scan_tmp_bytes(mat) + 64);
 
     McDev M{};
-    M.grid = g->labels.as<uint16_t>();
-    M.nx = g->dims[0]; M.ny = g->dims[1]; M.nz = g->dims[2];
+    M.grid = grid;
+    M.nx = dims[0]; M.ny = dims[1]; M.nz = dims[2];
     M.Nx = (uint32_t)Nx; M.Ny = (uint32_t)Ny; M.Nz = (uint32_t)Nz;
     M.P = P;
     M.tiles = tiles;
@@ -400,8 +398,8 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
     M.otri = B[5].as<unsigned long long>();
     M.overt = B[6].as<unsigned long long>();
     for (int q = 0; q < 3; ++q) {
-        M.mn[q] = g->mn[q];
-        M.cs[q] = g->cs[q];
+        M.mn[q] = mn[q];
+        M.cs[q] = cs[q];
     }
     {
         TimedLaunch tl("mc_count", s);
@@ -409,8 +407,8 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
         k_mc_count<<<blocks, kMcThreads, 2 * nl * sizeof(uint32_t), s>>>(M);
         VF_CHECK_LAUNCH();
     }
-    scan_exclusive_u32_to_u64(M.mtri, B[5].as<unsigned long long>(), mat, g->scan_tmp.p, s);
-    scan_exclusive_u32_to_u64(M.mvert, B[6].as<unsigned long long>(), mat, g->scan_tmp.p, s);
+    scan_exclusive_u32_to_u64(M.mtri, B[5].as<unsigned long long>(), mat, B[13].p, s);
+    scan_exclusive_u32_to_u64(M.mvert, B[6].as<unsigned long long>(), mat, B[13].p, s);
     unsigned long long tot[2];
     VF_HIP(hipMemcpyAsync(&tot[0], B[5].as<unsigned long long>() + mat, 8, hipMemcpyDeviceToHost, s));
     VF_HIP(hipMemcpyAsync(&tot[1], B[6].as<unsigned long long>() + mat, 8, hipMemcpyDeviceToHost, s));
@@ -446,23 +444,22 @@ void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
         VF_CHECK_LAUNCH();
     }
     VF_HIP(hipStreamSynchronize(s));
-    g->mc_nv = M.nv;
-    g->mc_nt = M.nt;
+    W.nv = M.nv;
+    W.nt = M.nt;
     info->num_vertices = M.nv;
     info->num_triangles = M.nt;
     info->num_labels = nl;
-    g->mc_valid = true;
+    W.valid = true;
 }
 
-void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
-                    uint32_t* toff)
+void mc_read(McScratch& W, hipStream_t s, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
+             uint32_t* toff)
 {
-    hipStream_t s = g->stream;
-    auto& B = g->mc_scratch;
-    const uint32_t nl = g->mc_nl;
-    if (xyz && g->mc_nv) VF_HIP(hipMemcpyAsync(xyz, B[10].p, (size_t)g->mc_nv * 12, hipMemcpyDeviceToHost, s));
-    if (tri && g->mc_nt) VF_HIP(hipMemcpyAsync(tri, B[12].p, (size_t)g->mc_nt * 12, hipMemcpyDeviceToHost, s));
-    if (tri_flag && g->mc_nt) VF_HIP(hipMemcpyAsync(tri_flag, B[8].p, (size_t)g->mc_nt, hipMemcpyDeviceToHost, s));
+    auto& B = W.buf;
+    const uint32_t nl = W.nl;
+    if (xyz && W.nv) VF_HIP(hipMemcpyAsync(xyz, B[10].p, (size_t)W.nv * 12, hipMemcpyDeviceToHost, s));
+    if (tri && W.nt) VF_HIP(hipMemcpyAsync(tri, B[12].p, (size_t)W.nt * 12, hipMemcpyDeviceToHost, s));
+    if (tri_flag && W.nt) VF_HIP(hipMemcpyAsync(tri_flag, B[8].p, (size_t)W.nt, hipMemcpyDeviceToHost, s));
     if (labels && nl) VF_HIP(hipMemcpyAsync(labels, B[1].p, (size_t)nl * 2, hipMemcpyDeviceToHost, s));
     std::vector<unsigned long long> off(2 * (size_t)(nl + 1));
     VF_HIP(hipMemcpyAsync(off.data(), B[11].p, off.size() * 8, hipMemcpyDeviceToHost, s));
@@ -471,6 +468,21 @@ void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, ui
         if (voff) voff[l] = (uint32_t)off[l];
         if (toff) toff[l] = (uint32_t)off[nl + 1 + l];
     }
+}
+
+void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
+{
+    Launch L = grid_launch(g);
+    launch_clear_hist(L);
+    launch_histogram(L);
+    g->mc.valid = false;
+    mc_run(g->mc, g->stream, g->labels.as<uint16_t>(), g->dims, g->mn, g->cs, g->hist.as<uint32_t>(), want, info);
+}
+
+void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
+                    uint32_t* toff)
+{
+    mc_read(g->mc, g->stream, xyz, tri, tri_flag, labels, voff, toff);
 }
 
 }  // namespace vf

@@ -23,24 +23,79 @@ __device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
     return ((K)d << b) | (K)i;
 }
 
-// FAST: the thread's 8 cells share (x, y) (nz % 8 == 0)
+// Distance argmin for one thread's 8 cells against seeds [base, base+n) staged
+// in LDS.  FAST: the 8 cells share (x, y) (nz % 8 == 0).
 template <int DIST, bool FAST, typename K>
+__device__ inline void naive_chunk(K best[8], uint32_t x, uint32_t y, uint32_t z, const uint32_t cx[8], const uint32_t cy[8],
+                                   const uint32_t cz[8], const uint32_t* sx, const uint32_t* sy, const uint32_t* sz,
+                                   uint32_t base, uint32_t n, uint32_t b)
+{
+    if (FAST) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t id = base + i;
+            const uint32_t ax = __usad(x, sx[i], 0u), ay = __usad(y, sy[i], 0u);
+            if (DIST == VF_MANHATTAN) {
+                const K kb = pack_key<DIST, K>(ax + ay, id, b);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const K key = kb + ((K)__usad(z + k, sz[i], 0u) << b);
+                    best[k] = key < best[k] ? key : best[k];
+                }
+            } else if (DIST == VF_CHEBYSHEV) {
+                const K kb = pack_key<DIST, K>(max(ax, ay), id, b);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    K key = pack_key<DIST, K>(__usad(z + k, sz[i], 0u), id, b);
+                    key = key > kb ? key : kb;
+                    best[k] = key < best[k] ? key : best[k];
+                }
+            } else {
+                const K kb = pack_key<DIST, K>(ax * ax + ay * ay, id, b);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t dz = __usad(z + k, sz[i], 0u);
+                    const K key = kb + ((K)(dz * dz) << b);
+                    best[k] = key < best[k] ? key : best[k];
+                }
+            }
+        }
+    } else {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t id = base + i;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t ax = __usad(cx[k], sx[i], 0u), ay = __usad(cy[k], sy[i], 0u), az = __usad(cz[k], sz[i], 0u);
+                uint32_t d;
+                if (DIST == VF_MANHATTAN) d = ax + ay + az;
+                else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
+                else d = ax * ax + ay * ay + az * az;
+                const K key = pack_key<DIST, K>(d, id, b);
+                best[k] = key < best[k] ? key : best[k];
+            }
+        }
+    }
+}
+
+template <typename K>
 __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
 {
     __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
+    constexpr int kWant = sizeof(K) == 8 ? 1 : 0;
     for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
         const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
         const JobDev* J = L.jobs + j;
-        if (!J->active || J->S == 0) continue;
+        if (!J->active || J->algo != VF_NAIVE || J->naive_key64 != kWant || J->S == 0) continue;
         const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         const bool live = c0 < J->V;
         const uint64_t occ = live ? load8b(J->occ, c0, J->V) : 0ull;
         const uint32_t b = J->key_bits;
         const uint32_t S = J->S;
+        const int dist = J->dist;
+        const bool fast = (J->nz & 7u) == 0;
         uint32_t x = 0, y = 0, z = 0;
         if (live) decompose(c0, J, x, y, z);
         uint32_t cx[8], cy[8], cz[8];
-        if (!FAST) {
+        {
             uint32_t a = x, bb = y, c = z;
             for (int k = 0; k < 8; ++k) {
                 cx[k] = a; cy[k] = bb; cz[k] = c;
@@ -54,51 +109,19 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
             const uint32_t n = min(kSeedChunk, S - base);
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                uint4 s = J->seeds[base + i];
+                const uint4 s = J->seeds[base + i];
                 sx[i] = s.x; sy[i] = s.y; sz[i] = s.z;
             }
             __syncthreads();
             if (!work) continue;
-            if (FAST) {
-                for (uint32_t i = 0; i < n; ++i) {
-                    const uint32_t id = base + i;
-                    const uint32_t ax = __usad(x, sx[i], 0u), ay = __usad(y, sy[i], 0u);
-                    if (DIST == VF_MANHATTAN) {
-                        const K kb = pack_key<DIST, K>(ax + ay, id, b);
-                        for (int k = 0; k < 8; ++k) {
-                            const K key = kb + ((K)__usad(z + k, sz[i], 0u) << b);
-                            best[k] = key < best[k] ? key : best[k];
-                        }
-                    } else if (DIST == VF_CHEBYSHEV) {
-                        const K kb = pack_key<DIST, K>(max(ax, ay), id, b);
-                        for (int k = 0; k < 8; ++k) {
-                            K key = pack_key<DIST, K>(__usad(z + k, sz[i], 0u), id, b);
-                            key = key > kb ? key : kb;
-                            best[k] = key < best[k] ? key : best[k];
-                        }
-                    } else {
-                        const K kb = pack_key<DIST, K>(ax * ax + ay * ay, id, b);
-                        for (int k = 0; k < 8; ++k) {
-                            const uint32_t dz = __usad(z + k, sz[i], 0u);
-                            const K key = kb + ((K)(dz * dz) << b);
-                            best[k] = key < best[k] ? key : best[k];
-                        }
-                    }
-                }
+            if (fast) {
+                if (dist == VF_MANHATTAN) naive_chunk<VF_MANHATTAN, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
+                else if (dist == VF_CHEBYSHEV) naive_chunk<VF_CHEBYSHEV, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
+                else naive_chunk<VF_EUCLIDEAN, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
             } else {
-                for (uint32_t i = 0; i < n; ++i) {
-                    const uint32_t id = base + i;
-                    for (int k = 0; k < 8; ++k) {
-                        const uint32_t ax = __usad(cx[k], sx[i], 0u), ay = __usad(cy[k], sy[i], 0u),
-                                       az = __usad(cz[k], sz[i], 0u);
-                        uint32_t d;
-                        if (DIST == VF_MANHATTAN) d = ax + ay + az;
-                        else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
-                        else d = ax * ax + ay * ay + az * az;
-                        const K key = pack_key<DIST, K>(d, id, b);
-                        best[k] = key < best[k] ? key : best[k];
-                    }
-                }
+                if (dist == VF_MANHATTAN) naive_chunk<VF_MANHATTAN, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
+                else if (dist == VF_CHEBYSHEV) naive_chunk<VF_CHEBYSHEV, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
+                else naive_chunk<VF_EUCLIDEAN, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
             }
         }
         if (!live) continue;
@@ -110,14 +133,6 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         }
         store8(J->labels, c0, J->V, out);
     }
-}
-
-template <int DIST, typename K>
-void launch_naive_t(const Launch& L, bool fast)
-{
-    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
-    if (fast) k_naive<DIST, true, K><<<blocks, kCellThreads, 0, L.s>>>(L);
-    else k_naive<DIST, false, K><<<blocks, kCellThreads, 0, L.s>>>(L);
 }
 
 // ---------------------------------------------------------------------------
@@ -179,18 +194,12 @@ __global__ void k_ri_reset(uint32_t* counters, uint32_t idx) { counters[idx] = 0
 
 }  // namespace
 
-void launch_naive_shaped(const Launch& L, int dist, bool fast, bool key64)
+void launch_naive(const Launch& L, bool key64)
 {
-    TimedLaunch tl("naive", L.s);
-    if (key64) {
-        if (dist == VF_MANHATTAN) launch_naive_t<VF_MANHATTAN, unsigned long long>(L, fast);
-        else if (dist == VF_CHEBYSHEV) launch_naive_t<VF_CHEBYSHEV, unsigned long long>(L, fast);
-        else launch_naive_t<VF_EUCLIDEAN, unsigned long long>(L, fast);
-    } else {
-        if (dist == VF_MANHATTAN) launch_naive_t<VF_MANHATTAN, uint32_t>(L, fast);
-        else if (dist == VF_CHEBYSHEV) launch_naive_t<VF_CHEBYSHEV, uint32_t>(L, fast);
-        else launch_naive_t<VF_EUCLIDEAN, uint32_t>(L, fast);
-    }
+    const uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    TimedLaunch tl(key64 ? "naive64" : "naive", L.s);
+    if (key64) k_naive<unsigned long long><<<blocks, kCellThreads, 0, L.s>>>(L);
+    else k_naive<uint32_t><<<blocks, kCellThreads, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

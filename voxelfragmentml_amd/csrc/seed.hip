@@ -85,8 +85,13 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
     const unsigned long long base = tile_off[t0];
     const unsigned long long O = tile_off[t1] - base;
     if (S == 0) return;
-    if (O < S || O > 0xFFFFFFFFull || S > kMaxDeviceSeeds) {
+    // on failure every seed is parked on cell (0,0,0) so later stages stay in bounds
+    auto park = [&]() {
         if (threadIdx.x == 0) atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) J->seeds[i] = make_uint4(0u, 0u, 0u, 2u + (i & 0xFFu));
+    };
+    if (O < S || O > 0xFFFFFFFFull || S > kMaxDeviceSeeds) {
+        park();
         return;
     }
     if (threadIdx.x == 0) s_got = 0;
@@ -122,7 +127,7 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
         __syncthreads();
     }
     if (s_got < S) {
-        if (threadIdx.x == 0) atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
+        park();
         return;
     }
     // sort the distinct ranks: position = number of smaller ranks
@@ -156,8 +161,12 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
             }
             advance(x, y, z, J);
         }
-        if (found) J->seeds[i] = make_uint4(x, y, z, 2u + i);
-        else atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
+        if (found) {
+            J->seeds[i] = make_uint4(x, y, z, 2u + i);
+        } else {
+            J->seeds[i] = make_uint4(0u, 0u, 0u, 2u + i);
+            atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
+        }
     }
 }
 

@@ -87,6 +87,13 @@ def lib():
         "vf_grid_export_rle": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "vf_grid_import_rle": (I, [P, P, C.c_size_t]),
         "vf_vessel_mesh": (I, [U64, U32, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
+        "vf_batch_create": (I, [C.POINTER(P)]), "vf_batch_destroy": (I, [P]),
+        "vf_batch_run": (I, [P, P, U32, P]), "vf_batch_set_graphs": (I, [P, I]),
+        "vf_plain_run": (I, [P, U32, P]),
+        "vf_server_create": (I, [C.c_size_t, U32, C.POINTER(P)]), "vf_server_destroy": (I, [P]),
+        "vf_server_submit": (I, [P, P, C.POINTER(U64)]), "vf_server_cancel": (I, [P, U64]),
+        "vf_server_poll": (I, [P, U64, P]), "vf_server_wait": (I, [P, U64, P]),
+        "vf_server_stats": (I, [P, C.POINTER(C.c_size_t), C.POINTER(U32), C.POINTER(U64)]),
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
         "vf_timing_read": (I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),
     }
@@ -266,3 +273,119 @@ def timing_read(kernel: str):
     ms, n = C.c_double(), C.c_uint64()
     check(lib().vf_timing_read(kernel.encode(), C.byref(ms), C.byref(n)))
     return ms.value, n.value
+
+
+# ------------------------------------------------------------ batched / served
+def make_job(xyz, tri, dims, aabb_min, aabb_max, num_seeds, seed, algorithm, distance,
+             neighbourhood=NEIGH_FROM_DISTANCE, noise_seed=0, noise_amp=0, boundary_size=1,
+             marching_cubes=0, seed_location=SEED_BOTH, poison=0, keep=None):
+    """Fill a vf_job.  `keep` (a list) receives the arrays the job points to so
+    they outlive the call."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    tri = np.ascontiguousarray(tri, np.uint32)
+    if keep is not None:
+        keep.extend([xyz, tri])
+    j = Job()
+    j.xyz = xyz.ctypes.data
+    j.tri = tri.ctypes.data
+    j.nv, j.nt = xyz.shape[0], tri.shape[0]
+    j.dims[:] = [int(d) for d in dims]
+    j.aabb_min[:] = [float(v) for v in aabb_min]
+    j.aabb_max[:] = [float(v) for v in aabb_max]
+    j.num_seeds, j.seed, j.seed_location = num_seeds, seed, seed_location
+    j.algorithm, j.distance, j.neighbourhood = algorithm, distance, neighbourhood
+    j.noise_seed, j.noise_amp = noise_seed, noise_amp
+    j.boundary_size, j.marching_cubes, j.poison = boundary_size, marching_cubes, poison
+    j.mesh_on_device = 0
+    return j
+
+
+def _job_array(jobs):
+    arr = (Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j
+    return arr
+
+
+def result_dict(r: Result) -> dict:
+    return {k: getattr(r, k) for k, _ in Result._fields_}
+
+
+class Batch:
+    """Batched form: every stage is one launch over all jobs (vf_batch_run)."""
+
+    def __init__(self, graphs=True):
+        self._h = C.c_void_p()
+        check(lib().vf_batch_create(C.byref(self._h)))
+        if not graphs:
+            check(lib().vf_batch_set_graphs(self._h, 0))
+
+    def run(self, jobs):
+        arr = _job_array(jobs)
+        res = (Result * len(jobs))()
+        check(lib().vf_batch_run(self._h, arr, len(jobs), res))
+        return [result_dict(r) for r in res]
+
+    def close(self):
+        if self._h:
+            lib().vf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plain_run(jobs):
+    """Plain form: one job per launch (vf_plain_run)."""
+    arr = _job_array(jobs)
+    res = (Result * len(jobs))()
+    check(lib().vf_plain_run(arr, len(jobs), res))
+    return [result_dict(r) for r in res]
+
+
+class Server:
+    """Served form: continuous batching with HBM-budget admission."""
+
+    def __init__(self, hbm_budget_bytes: int, max_slots: int = 64):
+        self._h = C.c_void_p()
+        check(lib().vf_server_create(hbm_budget_bytes, max_slots, C.byref(self._h)))
+
+    def submit(self, job) -> int:
+        t = C.c_uint64()
+        check(lib().vf_server_submit(self._h, C.byref(job), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        r = Result()
+        rc = lib().vf_server_poll(self._h, ticket, C.byref(r))
+        if rc == -7:
+            return None
+        check(rc)
+        return result_dict(r)
+
+    def wait(self, ticket):
+        r = Result()
+        check(lib().vf_server_wait(self._h, ticket, C.byref(r)))
+        return result_dict(r)
+
+    def cancel(self, ticket):
+        check(lib().vf_server_cancel(self._h, ticket))
+
+    def stats(self):
+        b, f, c = C.c_size_t(), C.c_uint32(), C.c_uint64()
+        check(lib().vf_server_stats(self._h, C.byref(b), C.byref(f), C.byref(c)))
+        return {"reserved_bytes": b.value, "in_flight": f.value, "completed": c.value}
+
+    def close(self):
+        if self._h:
+            lib().vf_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
