@@ -165,6 +165,14 @@ int vf_grid_import_rle(vf_grid* g, const uint8_t* in, size_t n);
 int vf_vessel_mesh(uint64_t seed, uint32_t radial, uint32_t height_segments, float* xyz, uint32_t* nv,
                    uint32_t* tri, uint32_t* nt);
 
+/* ---- device memory ----------------------------------------------------------- */
+/* Plain device buffers for callers that keep meshes resident in HBM
+ * (vf_job.mesh_on_device, vf_voxelize_device).  No reference counterpart: the
+ * reference keeps meshes in GL buffers owned by Model3D. */
+int vf_device_alloc(size_t bytes, void** ptr);
+int vf_device_free(void* ptr);
+int vf_memcpy_htod(void* dst, const void* src, size_t bytes);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Per-kernel HIP-event timing on the stream each kernel is launched on.
  * When enabled, every launch of a named kernel is bracketed by events;

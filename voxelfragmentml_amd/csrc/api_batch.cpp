@@ -290,9 +290,10 @@ void vf_server::loop()
         bool lev_dirty = false;
         auto complete = [&](EJob* e, int status) {
             std::lock_guard<std::mutex> lk(mu);
-            if (e->mem) {
-                reserved -= e->bytes;
+            if (e->admitted) {  // return the slot and the budget (the block itself is freed stream-ordered)
+                reserved -= job_footprint(e->spec);
                 --in_flight;
+                e->admitted = false;
             }
             e->res.status = e->cancelled ? VF_ERR_CANCELLED : status;
             e->res.latency_ms = now_ms() - e->t_submit;
@@ -308,6 +309,10 @@ void vf_server::loop()
                 if (queue.empty() && leveling.empty())
                     cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !queue.empty(); });
                 if (stop && queue.empty() && leveling.empty()) break;
+                // queued work that cannot be admitted yet and nothing in flight to wait on:
+                // sleep briefly instead of spinning (only a cancel or a new job changes this)
+                if (!queue.empty() && leveling.empty() && in_flight >= max_slots)
+                    cv.wait_for(lk, std::chrono::milliseconds(1));
                 while (!queue.empty() && in_flight + admitted.size() < max_slots) {
                     EJob* e = queue.front();
                     if (e->cancelled) {
@@ -332,6 +337,7 @@ void vf_server::loop()
                     if (reserved + fp > budget) break;  // wait for memory to free up
                     queue.pop_front();
                     reserved += fp;
+                    e->admitted = true;
                     admitted.push_back(e);
                 }
                 in_flight += (uint32_t)admitted.size();

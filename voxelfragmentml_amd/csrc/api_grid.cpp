@@ -628,3 +628,36 @@ int vf_grid_import_rle(vf_grid* g, const uint8_t* in, size_t n)
 }
 
 }  // extern "C"
+
+// ---- device memory helpers (mesh residency for jobs with mesh_on_device) ----
+extern "C" {
+
+int vf_device_alloc(size_t bytes, void** ptr)
+{
+    return guard([&] {
+        if (!ptr) fail(VF_ERR_INVALID_ARG, "ptr is NULL");
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *ptr = nullptr;
+            fail(VF_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed", bytes);
+        }
+    });
+}
+
+int vf_device_free(void* ptr)
+{
+    return guard([&] {
+        if (ptr) VF_HIP(hipFree(ptr));
+    });
+}
+
+int vf_memcpy_htod(void* dst, const void* src, size_t bytes)
+{
+    return guard([&] {
+        if (bytes && (!dst || !src)) fail(VF_ERR_INVALID_ARG, "NULL pointer");
+        if (bytes) VF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+}  // extern "C"

@@ -31,6 +31,7 @@ struct EJob {
     double t_submit = 0, t_admit = 0;
     int phase = 0;
     bool cancelled = false;
+    bool admitted = false;           // server: holds a slot and budget until completed
 };
 
 // Device bytes a job needs while resident (grid, occupancy, bitmasks, flood

@@ -94,6 +94,8 @@ def lib():
         "vf_server_submit": (I, [P, P, C.POINTER(U64)]), "vf_server_cancel": (I, [P, U64]),
         "vf_server_poll": (I, [P, U64, P]), "vf_server_wait": (I, [P, U64, P]),
         "vf_server_stats": (I, [P, C.POINTER(C.c_size_t), C.POINTER(U32), C.POINTER(U64)]),
+        "vf_device_alloc": (I, [C.c_size_t, C.POINTER(P)]), "vf_device_free": (I, [P]),
+        "vf_memcpy_htod": (I, [P, P, C.c_size_t]),
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
         "vf_timing_read": (I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),
     }
@@ -383,6 +385,37 @@ class Server:
         if self._h:
             lib().vf_server_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """A mesh copied once into HBM; jobs built from it set mesh_on_device."""
+
+    def __init__(self, xyz, tri):
+        self.xyz = np.ascontiguousarray(xyz, np.float32)
+        self.tri = np.ascontiguousarray(tri, np.uint32)
+        self.nv, self.nt = self.xyz.shape[0], self.tri.shape[0]
+        self.d_xyz, self.d_tri = C.c_void_p(), C.c_void_p()
+        check(lib().vf_device_alloc(self.xyz.nbytes, C.byref(self.d_xyz)))
+        check(lib().vf_device_alloc(self.tri.nbytes, C.byref(self.d_tri)))
+        check(lib().vf_memcpy_htod(self.d_xyz, _ptr(self.xyz), self.xyz.nbytes))
+        check(lib().vf_memcpy_htod(self.d_tri, _ptr(self.tri), self.tri.nbytes))
+
+    def job(self, *args, **kw):
+        j = make_job(self.xyz, self.tri, *args, **kw)
+        j.xyz, j.tri, j.mesh_on_device = self.d_xyz.value, self.d_tri.value, 1
+        return j
+
+    def close(self):
+        for p in (self.d_xyz, self.d_tri):
+            if p:
+                lib().vf_device_free(p)
+        self.d_xyz, self.d_tri = C.c_void_p(), C.c_void_p()
 
     def __del__(self):
         try:
