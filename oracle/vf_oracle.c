@@ -172,7 +172,7 @@ int64_t vfo_voxelize(const float* xyz, uint32_t nv, const uint32_t* tri, uint32_
             for (int q = 0; q < 3; ++q) finite &= isfinite(p[k][q]) != 0;
         if (!finite) continue; /* non-finite triangles mark nothing */
 
-        /* --- shell: SAT against every cell of the triangle bbox grown by one cell --- */
+        /* --- shell: SAT against every candidate cell of the triangle's bbox --- */
         int lo[3], hi[3], empty = 0;
         for (int q = 0; q < 3; ++q) {
             float mn = p[0][q], mx = p[0][q];
@@ -180,14 +180,16 @@ int64_t vfo_voxelize(const float* xyz, uint32_t nv, const uint32_t* tri, uint32_
             if (p[1][q] > mx) mx = p[1][q];
             if (p[2][q] < mn) mn = p[2][q];
             if (p[2][q] > mx) mx = p[2][q];
-            float fl = floorf((mn - aabb_min[q]) / cs[q]);
-            float fh = floorf((mx - aabb_min[q]) / cs[q]);
+            /* cells within 1/64 cell of the triangle's extent: a superset of the
+             * cells the FP32 SAT can accept (it only adds touching cells) */
+            float fl = floorf((mn - aabb_min[q]) / cs[q] - 0.015625f);
+            float fh = floorf((mx - aabb_min[q]) / cs[q] + 0.015625f);
             const float lim = (float)dims[q] + 1.0f;
             fl = fl < -2.0f ? -2.0f : (fl > lim ? lim : fl);
             fh = fh < -2.0f ? -2.0f : (fh > lim ? lim : fh);
-            if ((int)fh + 1 < 0 || (int)fl - 1 > (int)dims[q] - 1) empty = 1;
-            lo[q] = clampi((int)fl - 1, 0, (int)dims[q] - 1);
-            hi[q] = clampi((int)fh + 1, 0, (int)dims[q] - 1);
+            if ((int)fh < 0 || (int)fl > (int)dims[q] - 1) empty = 1;
+            lo[q] = clampi((int)fl, 0, (int)dims[q] - 1);
+            hi[q] = clampi((int)fh, 0, (int)dims[q] - 1);
         }
         if (empty) { lo[0] = 1; hi[0] = 0; }
         for (int x = lo[0]; x <= hi[0]; ++x)

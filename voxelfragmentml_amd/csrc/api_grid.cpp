@@ -73,7 +73,7 @@ static void grid_setup_shape(vf_grid* g, const uint32_t dims[3], const float mn[
     J.hist = g->hist.as<uint32_t>();
     J.active = 1;
     g->job_dirty = true;
-    uint32_t pre[2] = { 0, grid_tiles(g) };
+    uint32_t pre[4] = { 0, grid_tiles(g), 0, brick_count(dims[0], dims[1], dims[2]) };
     VF_HIP(hipMemcpyAsync(g->prefix.p, pre, sizeof pre, hipMemcpyHostToDevice, g->stream));
     VF_HIP(hipStreamSynchronize(g->stream));
 }
@@ -484,7 +484,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         launch_flood_seed(L, S);
         uint32_t chunk = 4, levels = 0;
         for (;;) {
-            for (uint32_t k = 0; k < chunk; ++k) launch_flood_level(L, g->fctl.as<uint32_t>() + 32, 512);
+            for (uint32_t k = 0; k < chunk; ++k) launch_flood_level(L, g->fctl.as<uint32_t>() + 32, 2048);
             levels += chunk;
             VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
             VF_HIP(hipStreamSynchronize(g->stream));
@@ -508,7 +508,13 @@ int vf_detect_boundaries(vf_grid* g, int size)
         checked(g);
         if (size < 0 || size > 64) fail(VF_ERR_INVALID_ARG, "boundary size %d out of range", size);
         grid_materialize(g);
-        launch_boundaries(grid_launch(g), size, 0);
+        if (g->job.bsize != size) {
+            g->job.bsize = size;
+            g->job_dirty = true;
+        }
+        Launch L = grid_launch(g);
+        if (size <= 1) launch_boundaries_tiled(L, g->prefix.as<uint32_t>() + 2, brick_count(g->dims[0], g->dims[1], g->dims[2]), false);
+        else launch_boundaries(L, size, 0);
         g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));
     });

@@ -188,7 +188,8 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     const size_t o_tile = align_up(o_jobs + n * sizeof(JobDev));
     const size_t o_tri = align_up(o_tile + (n + 1) * 4);
     const size_t o_col = align_up(o_tri + (n + 1) * 4);
-    const size_t o_gather = align_up(o_col + (n + 1) * 4);
+    const size_t o_bt = align_up(o_col + (n + 1) * 4);
+    const size_t o_gather = align_up(o_bt + (n + 1) * 4);
     const size_t o_done = align_up(o_gather + 16 * 4 * (size_t)n);
     const size_t total = align_up(o_done + 64);
     std::vector<uint8_t> img(total, 0);
@@ -196,13 +197,17 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     auto* tp = reinterpret_cast<uint32_t*>(img.data() + o_tile);
     auto* rp = reinterpret_cast<uint32_t*>(img.data() + o_tri);
     auto* cp = reinterpret_cast<uint32_t*>(img.data() + o_col);
-    uint64_t tiles = 0, tris = 0, groups = 0;
+    auto* bp = reinterpret_cast<uint32_t*>(img.data() + o_bt);
+    uint64_t tiles = 0, tris = 0, groups = 0, btiles = 0;
     for (uint32_t i = 0; i < n; ++i) {
         EJob& e = *js[i];
         jd[i] = e.d;
         tp[i] = (uint32_t)tiles;
         rp[i] = (uint32_t)tris;
         cp[i] = (uint32_t)groups;
+        bp[i] = (uint32_t)btiles;
+        btiles += brick_count(e.d.nx, e.d.ny, e.d.nz);
+        t->any_wide |= e.d.bsize > 1;
         tiles += e.tiles;
         tris += e.d.nt;
         groups += e.groups;
@@ -215,6 +220,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     tp[n] = (uint32_t)tiles;
     rp[n] = (uint32_t)tris;
     cp[n] = (uint32_t)groups;
+    bp[n] = (uint32_t)btiles;
     VF_HIP(hipMallocAsync(&t->dev, total, s));
     VF_HIP(hipMemcpyAsync(t->dev, img.data(), total, hipMemcpyHostToDevice, s));
     VF_HIP(hipStreamSynchronize(s));  // the pageable source must outlive the copy
@@ -226,6 +232,8 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->L.s = s;
     t->tri_prefix = reinterpret_cast<const uint32_t*>(base + o_tri);
     t->col_prefix = reinterpret_cast<const uint32_t*>(base + o_col);
+    t->bt_prefix = reinterpret_cast<const uint32_t*>(base + o_bt);
+    t->total_btiles = (uint32_t)btiles;
     t->total_tris = (uint32_t)tris;
     t->total_groups = (uint32_t)groups;
     t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
@@ -288,7 +296,7 @@ void Engine::post(Table& t)
 {
     const Launch& L = t.L;
     if (t.any_flood) launch_flood_finish(L);
-    launch_boundaries(L, -1, 1);
+    launch_boundaries_tiled(L, t.bt_prefix, t.total_btiles, t.any_wide);
     launch_hist_summary(L);
     launch_checksum(L);
     launch_gather(L, t.gather);

@@ -47,6 +47,9 @@ struct Table {
     const uint32_t* tri_prefix = nullptr;
     const uint32_t* col_prefix = nullptr;
     uint32_t total_tris = 0, total_groups = 0;
+    const uint32_t* bt_prefix = nullptr;  // boundary bricks per job
+    uint32_t total_btiles = 0;
+    bool any_wide = false;                // some job uses a boundary window wider than 1
     uint32_t* gather = nullptr;       // 16 words per job
     uint32_t* blocks_done = nullptr;
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
@@ -76,7 +79,7 @@ public:
 
     uint32_t* pinned = nullptr;     // gathered stats staging (16 words x kMaxTableJobs)
     static constexpr uint32_t kMaxTableJobs = 1024;
-    uint32_t level_blocks = 512;
+    uint32_t level_blocks = 2048;
 
 private:
     DevBuf vox_setup_, vox_counts_, vox_shell_off_, vox_col_off_, scan_tmp_, seed_cnt_, seed_off_;

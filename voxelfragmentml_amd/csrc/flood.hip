@@ -129,13 +129,15 @@ __device__ inline void push_aggregated(bool want, const JobDev* jobs, uint32_t j
     }
 }
 
-__global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_t* blocks_done)
+// One BFS level for every job: one thread per (frontier cell, neighbour)
+// pair, so a level costs one chain of dependent memory accesses (bucket entry,
+// its key, the neighbour's key, atomicMin, push) instead of one per neighbour.
+__global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
 {
-    __shared__ uint32_t pre[kFloodMaxJobs + 1];
+    __shared__ uint32_t pre[kFloodMaxJobs + 1];  // prefix of (entry, neighbour) pairs per job
     __shared__ uint32_t lev[kFloodMaxJobs];
     __shared__ uint32_t wsum[kFloodThreads / 64];
     const uint32_t nj = L.njobs;
-    // per-job frontier sizes and their prefix (every block computes it)
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nj; base += kFloodThreads) {
         const uint32_t j = base + threadIdx.x;
@@ -144,10 +146,10 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_
             const JobDev* J = L.jobs + j;
             const uint32_t d = J->fctl[0];
             lev[j] = d;
-            n = (J->active && J->algo == VF_FLOOD && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
-            if (n > J->bucket_cap) n = J->bucket_cap;
+            uint32_t ne = (J->active && J->algo == VF_FLOOD && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
+            if (ne > J->bucket_cap) ne = J->bucket_cap;
+            n = ne * (uint32_t)J->neighbourhood;
         }
-        // block exclusive scan of n
         uint32_t x = n;
         const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int o = 1; o < 64; o <<= 1) {
@@ -168,79 +170,63 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_
     if (threadIdx.x == 0) pre[nj] = carry;
     __syncthreads();
     const uint32_t total = pre[nj];
-
     const uint32_t stride = gridDim.x * blockDim.x;
-    // iterate so that every lane of a wave runs the same number of rounds
-    for (uint32_t e0 = blockIdx.x * blockDim.x; e0 < total; e0 += stride) {
-        const uint32_t e = e0 + threadIdx.x;
-        const bool have = e < total;
-        uint32_t j = 0, v = 0, d = 0, idx = 0, b = 0;
-        const JobDev* J = L.jobs;
-        if (have) {
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < total; p0 += stride) {
+        const uint32_t p = p0 + threadIdx.x;
+        bool want = false;
+        uint32_t j = 0, du = 0, u = 0;
+        if (p < total) {
             uint32_t lo = 0, hi = nj;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (pre[mid] <= e) lo = mid; else hi = mid;
+                if (pre[mid] <= p) lo = mid; else hi = mid;
             }
             j = lo;
-            J = L.jobs + j;
-            d = lev[j];
-            v = J->buckets[(size_t)(d % kLevelRing) * J->bucket_cap + (e - pre[j])];
-            b = J->key_bits;
-            idx = J->keys[v] & ((1u << b) - 1u);
-        }
-        uint32_t x = 0, y = 0, z = 0;
-        if (have) {
+            const JobDev* J = L.jobs + j;
+            const uint32_t d = lev[j];
+            const uint32_t local = p - pre[j];
+            const int nb = J->neighbourhood;
+            const uint32_t e = nb == 6 ? local / 6u : local / 26u;
+            const int k = (int)(local - e * (uint32_t)nb);
+            const uint32_t v = J->buckets[(size_t)(d % kLevelRing) * J->bucket_cap + e];
+            const int* o = nb == 6 ? c_nb6[k] : c_nb26[k];
             const uint32_t yz = J->ny * J->nz;
-            x = v / yz;
-            const uint32_t r = v - x * yz;
-            y = r / J->nz;
-            z = r - y * J->nz;
-        }
-        const int nb = have ? J->neighbourhood : 0;
-        const int nbmax = 26;
-        for (int k = 0; k < nbmax; ++k) {
-            bool want = false;
-            uint32_t du = 0, u = 0;
-            if (k < nb) {
-                const int* o = nb == 6 ? c_nb6[k] : c_nb26[k];
-                const int xx = (int)x + o[0], yy = (int)y + o[1], zz = (int)z + o[2];
-                if (xx >= 0 && yy >= 0 && zz >= 0 && xx < (int)J->nx && yy < (int)J->ny && zz < (int)J->nz) {
-                    u = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
-                    const uint32_t ku = J->keys[u];
-                    // quick reject: EMPTY (0) or already reached at a level <= d
-                    if (ku != kEmptyKey && (ku >> b) > d) {
-                        du = d + 1u + noise_delta(J->noise_seed, u, J->noise_amp);
-                        if (du >= (1u << (32 - b)) - 1u) {
-                            atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrKeyOverflow);
-                        } else {
-                            const uint32_t cand = (du << b) | idx;
-                            if (cand < ku) {
-                                const uint32_t old = atomicMin(J->keys + u, cand);
-                                want = cand < old && (old >> b) != du;
-                            }
+            const uint32_t x = v / yz, r = v - x * yz, y = r / J->nz, z = r - y * J->nz;
+            const int xx = (int)x + o[0], yy = (int)y + o[1], zz = (int)z + o[2];
+            if (xx >= 0 && yy >= 0 && zz >= 0 && xx < (int)J->nx && yy < (int)J->ny && zz < (int)J->nz) {
+                u = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
+                const uint32_t b = J->key_bits;
+                const uint32_t ku = J->keys[u];
+                // quick reject: EMPTY (0) or already reached at a level <= d
+                if (ku != kEmptyKey && (ku >> b) > d) {
+                    const uint32_t idx = J->keys[v] & ((1u << b) - 1u);
+                    du = d + 1u + noise_delta(J->noise_seed, u, J->noise_amp);
+                    if (du >= (1u << (32 - b)) - 1u) {
+                        atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrKeyOverflow);
+                    } else {
+                        const uint32_t cand = (du << b) | idx;
+                        if (cand < ku) {
+                            const uint32_t old = atomicMin(J->keys + u, cand);
+                            want = cand < old && (old >> b) != du;
                         }
                     }
                 }
             }
-            if (__ballot(k < nb) == 0) break;
-            push_aggregated(want, L.jobs, j, du, u);
         }
+        push_aggregated(want, L.jobs, j, du, u);
     }
+}
 
-    // last block: advance every job to the next level
-    __threadfence();
-    __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0) last = atomicAdd(blocks_done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (uint32_t j = threadIdx.x; j < nj; j += blockDim.x) {
+// Advance every flood job to its next level: recycle the counter slot that
+// level d+4 will use (last used by level d-4) and mark jobs whose next three
+// levels are empty as drained.
+__global__ void k_flood_advance(Launch L)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < L.njobs; j += gridDim.x * blockDim.x) {
         const JobDev* J = L.jobs + j;
         if (!J->active || J->algo != VF_FLOOD || J->fctl[1] != 0) continue;
-        const uint32_t d = lev[j];
-        volatile uint32_t* c = J->fctl + 2;
+        uint32_t* c = J->fctl + 2;
+        const uint32_t d = J->fctl[0];
         c[(d + 4) & 7u] = 0u;
         J->fctl[0] = d + 1;
         if (c[(d + 1) & 7u] == 0u && c[(d + 2) & 7u] == 0u && c[(d + 3) & 7u] == 0u) {
@@ -248,7 +234,6 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L, uint32_
             J->stats[kStatFloodDone] = d + 1;
         }
     }
-    if (threadIdx.x == 0) *blocks_done = 0;
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
@@ -288,11 +273,16 @@ void launch_flood_seed(const Launch& L, uint32_t /*max_seeds*/)
     VF_CHECK_LAUNCH();
 }
 
-void launch_flood_level(const Launch& L, uint32_t* blocks_done, uint32_t blocks)
+void launch_flood_level(const Launch& L, uint32_t* /*blocks_done*/, uint32_t blocks)
 {
     if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
-    TimedLaunch tl("flood_level", L.s);
-    k_flood_level<<<blocks, kFloodThreads, 0, L.s>>>(L, blocks_done);
+    {
+        TimedLaunch tl("flood_level", L.s);
+        k_flood_level<<<blocks, kFloodThreads, 0, L.s>>>(L);
+        VF_CHECK_LAUNCH();
+    }
+    TimedLaunch tl("flood_advance", L.s);
+    k_flood_advance<<<ceil_div(L.njobs, 256), 256, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

@@ -76,6 +76,8 @@ __global__ void k_clear_hist(Launch L)
 
 constexpr uint32_t kLdsBins = 4096;
 
+__device__ inline uint32_t ceil_div_d(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
 // Boundary bit of one labelled cell (detectBoundaries-comp.glsl:18-43).
 __device__ inline bool is_boundary_cell(const JobDev* J, uint32_t x, uint32_t y, uint32_t z, uint16_t c, int b)
 {
@@ -96,7 +98,9 @@ __device__ inline bool is_boundary_cell(const JobDev* J, uint32_t x, uint32_t y,
 }
 
 // HIST: accumulate the per-fragment histogram; BOUND: set the boundary bit.
-template <bool HIST, bool BOUND>
+// Generic path (any window size; used for jobs with size > 1).  ONLY_WIDE
+// restricts it to those jobs when the tiled kernel handles the rest.
+template <bool HIST, bool BOUND, bool ONLY_WIDE = false>
 __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize)
 {
     __shared__ uint32_t bins[HIST ? kLdsBins : 1];
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
             cur_job = j;
         }
         const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
+        if (!J->active || (ONLY_WIDE && J->bsize <= 1)) continue;
         uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J->V) continue;
         uint16_t v[8];
@@ -159,6 +163,149 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
             }
         }
         if (changed) store8(J->labels, c0, J->V, v);
+    }
+    flush(cur_job);
+}
+
+// Tiled detectBoundaries(1) + histogram.  A labelled cell c (2 <= raw < 0x8000)
+// is a boundary cell iff its clamped 3x3x3 window holds a label > FREE other than
+// its own, i.e. iff the window max of A = (m > 1 ? m : 0) or the window min of
+// B = (m > 1 ? m : 0xFFFF) differs from raw(c) (m = value & 0x7FFF; c itself is in
+// the window).  Cells whose bit 15 is already set never change.  Clamping the
+// window equals replicate padding for max/min, so the halo is loaded with
+// clamped coordinates.  8x8x64 bricks, separable passes z, y, x in LDS.
+constexpr int kBTX = 8, kBTY = 8, kBTZ = 64;
+constexpr int kBHX = kBTX + 2, kBHY = kBTY + 2, kBHZ = kBTZ + 2;
+
+__device__ inline uint32_t btiles_of(const JobDev* J)
+{
+    return ceil_div_d(J->nx, kBTX) * ceil_div_d(J->ny, kBTY) * ceil_div_d(J->nz, kBTZ);
+}
+
+template <bool HIST>
+__global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
+{
+    __shared__ uint16_t raw[kBHX * kBHY * kBHZ];
+    __shared__ uint16_t ax[kBHX * kBHY * kBTZ], bx[kBHX * kBHY * kBTZ];  // after the z pass
+    __shared__ uint16_t ay[kBHX * kBTY * kBTZ], by[kBHX * kBTY * kBTZ];  // after the y pass
+    __shared__ uint32_t bins[HIST ? kLdsBins : 1];
+    __shared__ int any_label;
+    uint32_t tb, te;
+    block_tile_range(total, tb, te);
+    if (tb >= te) return;
+    uint32_t cur_job = find_job(bt_prefix, L.njobs, tb);
+    if (HIST) {
+        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) bins[i] = 0;
+        __syncthreads();
+    }
+    auto flush = [&](uint32_t j) {
+        if (!HIST) return;
+        __syncthreads();
+        uint32_t* h = L.jobs[j].hist;
+        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) {
+            const uint32_t c = bins[i];
+            if (c) {
+                atomicAdd(h + i, c);
+                bins[i] = 0;
+            }
+        }
+        __syncthreads();
+    };
+    for (uint32_t t = tb; t < te; ++t) {
+        const uint32_t j = find_job(bt_prefix, L.njobs, t);
+        if (j != cur_job) {
+            flush(cur_job);
+            cur_job = j;
+        }
+        const JobDev* J = L.jobs + j;
+        if (!J->active || J->bsize > 1) continue;
+        const int nx = (int)J->nx, ny = (int)J->ny, nz = (int)J->nz;
+        const uint32_t local = t - bt_prefix[j];
+        const uint32_t tz = ceil_div_d(J->nz, kBTZ), ty = ceil_div_d(J->ny, kBTY);
+        const int x0 = (int)(local / (tz * ty)) * kBTX;
+        const int y0 = (int)((local / tz) % ty) * kBTY;
+        const int z0 = (int)(local % tz) * kBTZ;
+        const bool bound = J->bsize == 1;
+        if (threadIdx.x == 0) any_label = 0;
+        __syncthreads();
+        // load brick + halo, clamped
+        for (int i = threadIdx.x; i < kBHX * kBHY * kBHZ; i += blockDim.x) {
+            const int hz = i % kBHZ, hy = (i / kBHZ) % kBHY, hx = i / (kBHZ * kBHY);
+            const int gx = min(max(x0 + hx - 1, 0), nx - 1), gy = min(max(y0 + hy - 1, 0), ny - 1),
+                      gz = min(max(z0 + hz - 1, 0), nz - 1);
+            const uint16_t v = J->labels[((size_t)gx * ny + gy) * nz + gz];
+            raw[i] = v;
+            const bool inside = hx >= 1 && hx <= kBTX && hy >= 1 && hy <= kBTY && hz >= 1 && hz <= kBTZ &&
+                                x0 + hx - 1 < nx && y0 + hy - 1 < ny && z0 + hz - 1 < nz;
+            if (inside && v > kFree) any_label = 1;
+        }
+        __syncthreads();
+        if (!any_label) continue;
+        if (bound) {
+            // z pass over every (hx, hy) of the halo box
+            for (int i = threadIdx.x; i < kBHX * kBHY * kBTZ; i += blockDim.x) {
+                const int z = i % kBTZ, r = i / kBTZ;  // r = hx * kBHY + hy
+                uint16_t mx = 0, mn = 0xFFFF;
+                for (int dz = 0; dz < 3; ++dz) {
+                    const uint16_t m = raw[r * kBHZ + z + dz] & kMask15;
+                    const bool lab = m > kFree;
+                    mx = max(mx, lab ? m : (uint16_t)0);
+                    mn = min(mn, lab ? m : (uint16_t)0xFFFF);
+                }
+                ax[i] = mx;
+                bx[i] = mn;
+            }
+            __syncthreads();
+            // y pass
+            for (int i = threadIdx.x; i < kBHX * kBTY * kBTZ; i += blockDim.x) {
+                const int z = i % kBTZ, y = (i / kBTZ) % kBTY, hx = i / (kBTZ * kBTY);
+                uint16_t mx = 0, mn = 0xFFFF;
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int k = (hx * kBHY + y + dy) * kBTZ + z;
+                    mx = max(mx, ax[k]);
+                    mn = min(mn, bx[k]);
+                }
+                ay[i] = mx;
+                by[i] = mn;
+            }
+            __syncthreads();
+        }
+        // x pass + write-back: thread owns 16 z-consecutive cells of one (x, y) row
+        const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 16;
+        const int x = row / kBTY, y = row % kBTY;
+        const int gx = x0 + x, gy = y0 + y;
+        if (gx < nx && gy < ny) {
+            uint16_t out[16];
+            bool changed = false;
+            for (int q = 0; q < 16; ++q) {
+                const int z = seg + q;
+                const uint16_t c = raw[((x + 1) * kBHY + (y + 1)) * kBHZ + z + 1];
+                out[q] = c;
+                if (z0 + z >= nz || c <= kFree) continue;
+                if (HIST) {
+                    const uint32_t m = c & kMask15;
+                    if (m < kLdsBins) atomicAdd(&bins[m], 1u);
+                    else atomicAdd(J->hist + m, 1u);
+                }
+                if (!bound || (c & 0x8000u)) continue;
+                uint16_t mx = 0, mn = 0xFFFF;
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int k = ((x + dx) * kBTY + y) * kBTZ + z;
+                    mx = max(mx, ay[k]);
+                    mn = min(mn, by[k]);
+                }
+                if (mx != c || mn != c) {
+                    out[q] = (uint16_t)(c | 0x8000u);
+                    changed = true;
+                }
+            }
+            if (changed) {
+                uint16_t* dst = J->labels + ((size_t)gx * ny + gy) * nz + z0 + seg;
+                const int nzv = min(16, nz - (z0 + seg));
+                for (int q = 0; q < nzv; ++q) dst[q] = out[q];
+            }
+        }
+        __syncthreads();
     }
     flush(cur_job);
 }
@@ -333,6 +480,21 @@ void launch_boundaries(const Launch& L, int size, int with_hist)
         k_cells_post<false, true><<<grid_for(L), kCellThreads, 0, L.s>>>(L, size);
     }
     VF_CHECK_LAUNCH();
+}
+
+void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide)
+{
+    {
+        TimedLaunch tl("boundaries_hist", L.s);
+        const uint32_t blocks = total_btiles < 4096 ? (total_btiles ? total_btiles : 1) : 4096;
+        k_boundaries_tiled<true><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+        VF_CHECK_LAUNCH();
+    }
+    if (any_wide) {
+        TimedLaunch tl("boundaries_wide", L.s);
+        k_cells_post<true, true, true><<<grid_for(L), kCellThreads, 0, L.s>>>(L, -1);
+        VF_CHECK_LAUNCH();
+    }
 }
 
 void launch_checksum(const Launch& L)

@@ -135,6 +135,15 @@ void launch_undo_mask(const Launch& L, uint16_t keep_mask);
 void launch_clear_hist(const Launch& L);
 void launch_histogram(const Launch& L);
 void launch_boundaries(const Launch& L, int size /* < 0: per-job bsize */, int with_hist);
+// detectBoundaries(size <= 1 per job) fused with the histogram, 8x8x64 LDS
+// bricks; bt_prefix: njobs+1 prefix of bricks; jobs with size > 1 go through
+// the generic kernel when any_wide.
+constexpr uint32_t kBrickX = 8, kBrickY = 8, kBrickZ = 64;
+inline uint32_t brick_count(uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    return ((nx + kBrickX - 1) / kBrickX) * ((ny + kBrickY - 1) / kBrickY) * ((nz + kBrickZ - 1) / kBrickZ);
+}
+void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide);
 void launch_job_init(const Launch& L);     // zero stats, histogram and flood control of every job
 void launch_gather(const Launch& L, uint32_t* out);  // 16 words per job: stats (low 32 bits) + flood state
 void launch_checksum(const Launch& L);

@@ -24,6 +24,9 @@ namespace {
 constexpr int kSub = 256;
 constexpr float kSnapLim = 524288.0f;
 constexpr uint32_t kVoxThreads = 256;
+constexpr float kRangeMargin = 0.015625f;   // 1/64 cell
+constexpr uint32_t kInlineShell = 64;       // SAT candidates a setup thread tests itself
+constexpr uint32_t kInlineCols = 32;        // parity columns a setup thread toggles itself
 
 __device__ inline bool axis_sep(float pa, float pb, float rad)
 {
@@ -129,6 +132,27 @@ __device__ inline int64_t floordiv64(int64_t a, int64_t b)
     return q;
 }
 
+// Column (i, j): if its centre is covered by the triangle's xy projection
+// (top-left rule on exact int64 edge functions), toggle the parity bit of the
+// first cell whose centre lies above the crossing.
+__device__ inline void parity_column(const JobDev* J, const TriSetup& S, int i, int jj)
+{
+    const int64_t X0 = S.X[0], X1 = S.X[1], X2 = S.X[2];
+    const int64_t Y0 = S.Y[0], Y1 = S.Y[1], Y2 = S.Y[2];
+    const int64_t px = (int64_t)i * kSub + kSub / 2, py = (int64_t)jj * kSub + kSub / 2;
+    const int64_t d0x = X2 - X1, d0y = Y2 - Y1, d1x = X0 - X2, d1y = Y0 - Y2, d2x = X1 - X0, d2y = Y1 - Y0;
+    const int64_t E0 = d0x * (py - Y1) - d0y * (px - X1);
+    const int64_t E1 = d1x * (py - Y2) - d1y * (px - X2);
+    const int64_t E2 = d2x * (py - Y0) - d2y * (px - X0);
+    if (!edge_incl(E0, d0x, d0y) || !edge_incl(E1, d1x, d1y) || !edge_incl(E2, d2x, d2y)) return;
+    const int64_t num = E0 * (int64_t)S.Z[0] + E1 * (int64_t)S.Z[1] + E2 * (int64_t)S.Z[2];
+    int64_t k = floordiv64(num - (int64_t)(kSub / 2) * S.area, (int64_t)kSub * S.area) + 1;
+    if (k < 0) k = 0;
+    if (k >= (int64_t)J->nz) return;
+    const size_t word = ((size_t)i * J->ny + (size_t)jj) * J->wp + (size_t)(k >> 5);
+    atomicXor(J->toggle_bits + word, 1u << (k & 31));
+}
+
 __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, uint32_t njobs, const uint32_t* tri_prefix,
                                                            uint32_t total, TriSetup* setup, uint32_t* cnt_shell, uint32_t* cnt_col)
 {
@@ -160,14 +184,17 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                     if (P[1][q] > mx) mx = P[1][q];
                     if (P[2][q] < mn) mn = P[2][q];
                     if (P[2][q] > mx) mx = P[2][q];
-                    float fl = floorf((mn - J->mn[q]) / J->cs[q]);
-                    float fh = floorf((mx - J->mn[q]) / J->cs[q]);
+                    // candidate cells: those within 1/64 cell of the triangle's extent (the
+                    // SAT's FP32 box test can only accept touching cells, and its rounding
+                    // is far below the margin)
+                    float fl = floorf((mn - J->mn[q]) / J->cs[q] - kRangeMargin);
+                    float fh = floorf((mx - J->mn[q]) / J->cs[q] + kRangeMargin);
                     const float lim = (float)dims[q] + 1.0f;
                     fl = fl < -2.0f ? -2.0f : (fl > lim ? lim : fl);
                     fh = fh < -2.0f ? -2.0f : (fh > lim ? lim : fh);
-                    if ((int)fh + 1 < 0 || (int)fl - 1 > (int)dims[q] - 1) empty = true;
-                    S.lo[q] = clampi((int)fl - 1, 0, (int)dims[q] - 1);
-                    S.hi[q] = clampi((int)fh + 1, 0, (int)dims[q] - 1);
+                    if ((int)fh < 0 || (int)fl > (int)dims[q] - 1) empty = true;
+                    S.lo[q] = clampi((int)fl, 0, (int)dims[q] - 1);
+                    S.hi[q] = clampi((int)fh, 0, (int)dims[q] - 1);
                 }
                 if (!empty)
                     nshell = (uint32_t)(S.hi[0] - S.lo[0] + 1) * (uint32_t)(S.hi[1] - S.lo[1] + 1) *
@@ -203,6 +230,27 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                     if (i1 >= i0 && j1 >= j0) ncol = (uint32_t)(i1 - i0 + 1) * (uint32_t)(j1 - j0 + 1);
                 }
             }
+        }
+        // small triangles: test the candidates here; only large ones go to the pair kernels
+        if (nshell && nshell <= kInlineShell) {
+            for (int x = S.lo[0]; x <= S.hi[0]; ++x)
+                for (int y = S.lo[1]; y <= S.hi[1]; ++y)
+                    for (int z = S.lo[2]; z <= S.hi[2]; ++z) {
+                        const size_t word = ((size_t)x * J->ny + (size_t)y) * J->wp + (size_t)(z >> 5);
+                        const uint32_t bit = 1u << (z & 31);
+                        if (J->shell_bits[word] & bit) continue;
+                        float c[3], h[3];
+                        cell_box(x, J->mn[0], J->cs[0], &c[0], &h[0]);
+                        cell_box(y, J->mn[1], J->cs[1], &c[1], &h[1]);
+                        cell_box(z, J->mn[2], J->cs[2], &c[2], &h[2]);
+                        if (sat_triangle_box(S.P[0], S.P[1], S.P[2], c, h)) atomicOr(J->shell_bits + word, bit);
+                    }
+            nshell = 0;
+        }
+        if (ncol && ncol <= kInlineCols) {
+            for (int i = S.i0; i <= S.i1; ++i)
+                for (int jj = S.j0; jj <= S.j1; ++jj) parity_column(J, S, i, jj);
+            ncol = 0;
         }
         setup[g] = S;
         cnt_shell[g] = nshell;
@@ -247,22 +295,7 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_parity(const JobDev* jobs, 
         const JobDev* J = jobs + S->job;
         const uint64_t local = p - off[g];
         const uint32_t nj = (uint32_t)(S->j1 - S->j0 + 1);
-        const int i = S->i0 + (int)(local / nj);
-        const int jj = S->j0 + (int)(local % nj);
-        const int64_t X0 = S->X[0], X1 = S->X[1], X2 = S->X[2];
-        const int64_t Y0 = S->Y[0], Y1 = S->Y[1], Y2 = S->Y[2];
-        const int64_t px = (int64_t)i * kSub + kSub / 2, py = (int64_t)jj * kSub + kSub / 2;
-        const int64_t d0x = X2 - X1, d0y = Y2 - Y1, d1x = X0 - X2, d1y = Y0 - Y2, d2x = X1 - X0, d2y = Y1 - Y0;
-        const int64_t E0 = d0x * (py - Y1) - d0y * (px - X1);
-        const int64_t E1 = d1x * (py - Y2) - d1y * (px - X2);
-        const int64_t E2 = d2x * (py - Y0) - d2y * (px - X0);
-        if (!edge_incl(E0, d0x, d0y) || !edge_incl(E1, d1x, d1y) || !edge_incl(E2, d2x, d2y)) continue;
-        const int64_t num = E0 * (int64_t)S->Z[0] + E1 * (int64_t)S->Z[1] + E2 * (int64_t)S->Z[2];
-        int64_t k = floordiv64(num - (int64_t)(kSub / 2) * S->area, (int64_t)kSub * S->area) + 1;
-        if (k < 0) k = 0;
-        if (k >= (int64_t)J->nz) continue;
-        const size_t word = ((size_t)i * J->ny + (size_t)jj) * J->wp + (size_t)(k >> 5);
-        atomicXor(J->toggle_bits + word, 1u << (k & 31));
+        parity_column(J, *S, S->i0 + (int)(local / nj), S->j0 + (int)(local % nj));
     }
 }
 
