@@ -465,7 +465,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         if (rc != VF_OK) throw Exception{ rc };
         const uint64_t cap = occupied + S + 64;
         JobDev& J = g->job;
-        g->keys.ensure(g->V * 4 + 64);
+        g->keys.ensure(flood_key_cells(g->dims[0], g->dims[1], g->dims[2]) * 4 + 64);
         g->buckets.ensure(cap * 4 * kLevelRing);
         g->fctl.ensure(64 * 4);
         J.keys = g->keys.as<uint32_t>();

@@ -50,7 +50,7 @@ Layout layout_of(const vf_job& j)
     L.hist = o; o = align_up(o + 65536 * 4);
     L.stats = o; o = align_up(o + kStatCount * 8);
     L.fctl = o; o = align_up(o + 64 * 4);
-    L.keys = o; if (flood) o = align_up(o + L.V * 4);
+    L.keys = o; if (flood) o = align_up(o + flood_key_cells(j.dims[0], j.dims[1], j.dims[2]) * 4);
     L.buckets = o; if (flood) o = align_up(o + L.cap * 4 * kLevelRing);
     L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
     L.mesh_tri = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nt * 12);

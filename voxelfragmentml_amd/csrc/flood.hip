@@ -35,6 +35,26 @@ __constant__ int c_nb26[26][3] = {
 };
 __constant__ int c_nb6[6][3] = { { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
 
+// Keys live in 4x4x4 bricks (64 cells, 256 bytes): a cell's 26 neighbours fall
+// in one or two bricks instead of nine z-rows.  Dims are padded to multiples
+// of 4; padded cells are never visited (neighbour bounds use the real dims).
+__device__ inline uint32_t kidx(const JobDev* J, uint32_t x, uint32_t y, uint32_t z)
+{
+    const uint32_t BY = (J->ny + 3) >> 2, BZ = (J->nz + 3) >> 2;
+    return ((((x >> 2) * BY + (y >> 2)) * BZ + (z >> 2)) << 6) | ((x & 3u) << 4) | ((y & 3u) << 2) | (z & 3u);
+}
+
+__device__ inline void kdecode(const JobDev* J, uint32_t e, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+    const uint32_t BY = (J->ny + 3) >> 2, BZ = (J->nz + 3) >> 2;
+    const uint32_t brick = e >> 6, off = e & 63u;
+    const uint32_t bz = brick % BZ, r = brick / BZ;
+    const uint32_t by = r % BY, bx = r / BY;
+    x = (bx << 2) | (off >> 4);
+    y = (by << 2) | ((off >> 2) & 3u);
+    z = (bz << 2) | (off & 3u);
+}
+
 __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
 {
     for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
@@ -44,21 +64,12 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
         const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J->V) continue;
         const uint64_t o = load8b(J->occ, c0, J->V);
-        if (c0 + 8 <= J->V) {
-            uint4 a, b;
-            a.x = (o & 0xFFull) ? kUnreached : kEmptyKey;
-            a.y = (o & 0xFF00ull) ? kUnreached : kEmptyKey;
-            a.z = (o & 0xFF0000ull) ? kUnreached : kEmptyKey;
-            a.w = (o & 0xFF000000ull) ? kUnreached : kEmptyKey;
-            b.x = (o & 0xFF00000000ull) ? kUnreached : kEmptyKey;
-            b.y = (o & 0xFF0000000000ull) ? kUnreached : kEmptyKey;
-            b.z = (o & 0xFF000000000000ull) ? kUnreached : kEmptyKey;
-            b.w = (o & 0xFF00000000000000ull) ? kUnreached : kEmptyKey;
-            reinterpret_cast<uint4*>(J->keys + c0)[0] = a;
-            reinterpret_cast<uint4*>(J->keys + c0)[1] = b;
-        } else {
-            for (int k = 0; k < 8; ++k)
-                if (c0 + k < J->V) J->keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
+        uint32_t x, y, z;
+        decompose(c0, J, x, y, z);
+        for (int k = 0; k < 8; ++k) {
+            if (c0 + k >= J->V) break;
+            J->keys[kidx(J, x, y, z)] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
+            advance(x, y, z, J);
         }
     }
 }
@@ -74,12 +85,12 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
     for (uint32_t i = threadIdx.x; i < kFloodCtlWords; i += blockDim.x) J->fctl[i] = 0;
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
-        J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 0u;
+        J->keys[kidx(J, q.x, q.y, q.z)] = 0u;
     }
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
-        atomicMax(J->keys + ((size_t)q.x * J->ny + q.y) * J->nz + q.z, (1u << b) | s);
+        atomicMax(J->keys + kidx(J, q.x, q.y, q.z), (1u << b) | s);
     }
     __syncthreads();
     __shared__ uint32_t npush;
@@ -87,11 +98,11 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
-        const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
-        if (J->keys[c] == ((1u << b) | s)) {
+        const uint32_t kc = kidx(J, q.x, q.y, q.z);
+        if (J->keys[kc] == ((1u << b) | s)) {
             const uint32_t slot = atomicAdd(&npush, 1u);
-            J->buckets[(size_t)(1 % kLevelRing) * J->bucket_cap + slot] = (uint32_t)c;
-            J->occ[c] = 1;
+            J->buckets[(size_t)(1 % kLevelRing) * J->bucket_cap + slot] = kc;
+            J->occ[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 1;
         }
     }
     __syncthreads();
@@ -131,12 +142,23 @@ __device__ inline void push_aggregated(bool want, const JobDev* jobs, uint32_t j
 
 // One BFS level for every job: one thread per (frontier cell, neighbour)
 // pair, so a level costs one chain of dependent memory accesses (bucket entry,
-// its key, the neighbour's key, atomicMin, push) instead of one per neighbour.
+// keys, atomicMin) instead of one per neighbour.  Each block owns a contiguous
+// range of pairs (so it touches one or two jobs) and stages the cells it
+// discovers in LDS; a flush reserves bucket space with one global atomic per
+// (job, level) the block touched, instead of one per wave, which keeps the
+// per-job bucket counters free of contention.
+constexpr uint32_t kStage = 2048;     // staged pushes per block before a flush
+constexpr uint32_t kStageKeys = 16;   // distinct (job, level) groups per flush
+
 __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
 {
     __shared__ uint32_t pre[kFloodMaxJobs + 1];  // prefix of (entry, neighbour) pairs per job
     __shared__ uint32_t lev[kFloodMaxJobs];
     __shared__ uint32_t wsum[kFloodThreads / 64];
+    __shared__ uint32_t st_u[kStage], st_key[kStage], st_off[kStage];
+    __shared__ uint8_t st_grp[kStage];
+    __shared__ uint32_t g_key[kStageKeys], g_cnt[kStageKeys], g_base[kStageKeys];
+    __shared__ uint32_t st_n;
     const uint32_t nj = L.njobs;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nj; base += kFloodThreads) {
@@ -167,15 +189,68 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
         carry += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) pre[nj] = carry;
+    if (threadIdx.x == 0) {
+        pre[nj] = carry;
+        st_n = 0;
+    }
     __syncthreads();
     const uint32_t total = pre[nj];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < total; p0 += stride) {
+    // contiguous, 256-aligned pair range per block
+    const uint32_t per = (((total + gridDim.x - 1) / gridDim.x) + 255u) & ~255u;
+    const uint32_t pb = min(total, blockIdx.x * per), pe = min(total, pb + per);
+    if (pb >= pe) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+
+    auto flush = [&]() {
+        __syncthreads();
+        const uint32_t n = st_n;
+        if (threadIdx.x < kStageKeys) {
+            g_key[threadIdx.x] = 0xFFFFFFFFu;
+            g_cnt[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t k = st_key[e];
+            uint32_t slot = kStageKeys;
+            for (uint32_t q = 0; q < kStageKeys; ++q) {
+                uint32_t cur = g_key[q];
+                if (cur == 0xFFFFFFFFu) cur = atomicCAS(&g_key[q], 0xFFFFFFFFu, k) == 0xFFFFFFFFu ? k : g_key[q];
+                if (cur == k) {
+                    slot = q;
+                    break;
+                }
+            }
+            st_grp[e] = (uint8_t)slot;
+            if (slot < kStageKeys) st_off[e] = atomicAdd(&g_cnt[slot], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < kStageKeys && g_cnt[threadIdx.x]) {
+            const uint32_t k = g_key[threadIdx.x];
+            const uint32_t j = k >> 2, du = lev[j] + 1u + (k & 3u);
+            g_base[threadIdx.x] = atomicAdd(L.jobs[j].fctl + 2 + (du & 7u), g_cnt[threadIdx.x]);
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t k = st_key[e];
+            const uint32_t j = k >> 2, du = lev[j] + 1u + (k & 3u);
+            const JobDev* J = L.jobs + j;
+            uint32_t slot;
+            if (st_grp[e] < kStageKeys) slot = g_base[st_grp[e]] + st_off[e];
+            else slot = atomicAdd(J->fctl + 2 + (du & 7u), 1u);  // more than kStageKeys groups: rare
+            if (slot < J->bucket_cap) J->buckets[(size_t)(du % kLevelRing) * J->bucket_cap + slot] = st_u[e];
+            else atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrBucket);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) st_n = 0;
+        __syncthreads();
+    };
+
+    for (uint32_t p0 = pb; p0 < pe; p0 += blockDim.x) {
         const uint32_t p = p0 + threadIdx.x;
         bool want = false;
         uint32_t j = 0, du = 0, u = 0;
-        if (p < total) {
+        if (p < pe) {
             uint32_t lo = 0, hi = nj;
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
@@ -190,17 +265,18 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
             const int k = (int)(local - e * (uint32_t)nb);
             const uint32_t v = J->buckets[(size_t)(d % kLevelRing) * J->bucket_cap + e];
             const int* o = nb == 6 ? c_nb6[k] : c_nb26[k];
-            const uint32_t yz = J->ny * J->nz;
-            const uint32_t x = v / yz, r = v - x * yz, y = r / J->nz, z = r - y * J->nz;
+            uint32_t x, y, z;
+            kdecode(J, v, x, y, z);
             const int xx = (int)x + o[0], yy = (int)y + o[1], zz = (int)z + o[2];
             if (xx >= 0 && yy >= 0 && zz >= 0 && xx < (int)J->nx && yy < (int)J->ny && zz < (int)J->nz) {
-                u = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
+                u = kidx(J, (uint32_t)xx, (uint32_t)yy, (uint32_t)zz);
                 const uint32_t b = J->key_bits;
                 const uint32_t ku = J->keys[u];
                 // quick reject: EMPTY (0) or already reached at a level <= d
                 if (ku != kEmptyKey && (ku >> b) > d) {
                     const uint32_t idx = J->keys[v] & ((1u << b) - 1u);
-                    du = d + 1u + noise_delta(J->noise_seed, u, J->noise_amp);
+                    const uint32_t lin = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
+                    du = d + 1u + noise_delta(J->noise_seed, lin, J->noise_amp);
                     if (du >= (1u << (32 - b)) - 1u) {
                         atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrKeyOverflow);
                     } else {
@@ -213,8 +289,22 @@ __global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
                 }
             }
         }
-        push_aggregated(want, L.jobs, j, du, u);
+        // stage this chunk's discoveries in LDS (one LDS atomic per wave)
+        const uint64_t m = __ballot(want);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&st_n, (uint32_t)__popcll(m));
+            base = __shfl(base, 0, 64);
+            if (want) {
+                const uint32_t at = base + (uint32_t)__popcll(m & lt);
+                st_u[at] = u;
+                st_key[at] = (j << 2) | (du - lev[j] - 1u);
+            }
+        }
+        __syncthreads();
+        if (st_n > kStage - blockDim.x) flush();
     }
+    flush();
 }
 
 // Advance every flood job to its next level: recycle the counter slot that
@@ -246,8 +336,11 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
         if (c0 >= J->V) continue;
         const uint32_t mask = (1u << J->key_bits) - 1u;
         uint16_t out[8];
+        uint32_t x, y, z;
+        decompose(c0, J, x, y, z);
         for (int k = 0; k < 8; ++k) {
-            uint32_t key = c0 + k < J->V ? J->keys[c0 + k] : 0u;
+            const uint32_t key = c0 + k < J->V ? J->keys[kidx(J, x, y, z)] : 0u;
+            advance(x, y, z, J);
             if (key == kEmptyKey) out[k] = kEmpty;
             else if (key == kUnreached) out[k] = kFree;
             else out[k] = (uint16_t)(J->seeds[key & mask].w & 0xFFu);  // unmaskRightMost(value, 8)

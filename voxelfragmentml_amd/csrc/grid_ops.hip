@@ -173,14 +173,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
 // B = (m > 1 ? m : 0xFFFF) differs from raw(c) (m = value & 0x7FFF; c itself is in
 // the window).  Cells whose bit 15 is already set never change.  Clamping the
 // window equals replicate padding for max/min, so the halo is loaded with
-// clamped coordinates.  8x8x64 bricks, separable passes z, y, x in LDS.
-constexpr int kBTX = 8, kBTY = 8, kBTZ = 64;
+// clamped coordinates.  8x8x32 bricks: the interior is loaded first with one
+// 16-byte load per thread and bricks without a label stop there; live bricks
+// load the halo and run separable max/min passes z, y, x in LDS.
+constexpr int kBTX = 8, kBTY = 8, kBTZ = 32;
 constexpr int kBHX = kBTX + 2, kBHY = kBTY + 2, kBHZ = kBTZ + 2;
-
-__device__ inline uint32_t btiles_of(const JobDev* J)
-{
-    return ceil_div_d(J->nx, kBTX) * ceil_div_d(J->ny, kBTY) * ceil_div_d(J->nz, kBTZ);
-}
 
 template <bool HIST>
 __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
@@ -189,7 +186,6 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
     __shared__ uint16_t ax[kBHX * kBHY * kBTZ], bx[kBHX * kBHY * kBTZ];  // after the z pass
     __shared__ uint16_t ay[kBHX * kBTY * kBTZ], by[kBHX * kBTY * kBTZ];  // after the y pass
     __shared__ uint32_t bins[HIST ? kLdsBins : 1];
-    __shared__ int any_label;
     uint32_t tb, te;
     block_tile_range(total, tb, te);
     if (tb >= te) return;
@@ -211,6 +207,9 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         }
         __syncthreads();
     };
+    // this thread's 8 cells: row (x, y) = tid / 4, z segment (tid % 4) * 8
+    const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 8;
+    const int tx = row / kBTY, ty = row % kBTY;
     for (uint32_t t = tb; t < te; ++t) {
         const uint32_t j = find_job(bt_prefix, L.njobs, t);
         if (j != cur_job) {
@@ -221,88 +220,108 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         if (!J->active || J->bsize > 1) continue;
         const int nx = (int)J->nx, ny = (int)J->ny, nz = (int)J->nz;
         const uint32_t local = t - bt_prefix[j];
-        const uint32_t tz = ceil_div_d(J->nz, kBTZ), ty = ceil_div_d(J->ny, kBTY);
-        const int x0 = (int)(local / (tz * ty)) * kBTX;
-        const int y0 = (int)((local / tz) % ty) * kBTY;
+        const uint32_t tz = ceil_div_d(J->nz, kBTZ), tyn = ceil_div_d(J->ny, kBTY);
+        const int x0 = (int)(local / (tz * tyn)) * kBTX;
+        const int y0 = (int)((local / tz) % tyn) * kBTY;
         const int z0 = (int)(local % tz) * kBTZ;
         const bool bound = J->bsize == 1;
-        if (threadIdx.x == 0) any_label = 0;
-        __syncthreads();
-        // load brick + halo, clamped
+        // interior
+        const int gx = x0 + tx, gy = y0 + ty, gz = z0 + seg;
+        const bool row_ok = gx < nx && gy < ny;
+        const int nval = row_ok ? max(0, min(8, nz - gz)) : 0;
+        uint16_t v[8];
+        const uint16_t* src = J->labels + ((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz;
+        if (nval == 8 && (nz & 7) == 0) {
+            const uint4 q = *reinterpret_cast<const uint4*>(src);
+            v[0] = (uint16_t)q.x; v[1] = (uint16_t)(q.x >> 16); v[2] = (uint16_t)q.y; v[3] = (uint16_t)(q.y >> 16);
+            v[4] = (uint16_t)q.z; v[5] = (uint16_t)(q.z >> 16); v[6] = (uint16_t)q.w; v[7] = (uint16_t)(q.w >> 16);
+        } else {
+            for (int k = 0; k < 8; ++k) v[k] = k < nval ? src[k] : (uint16_t)0;
+        }
+        bool mine = false;
+        for (int k = 0; k < 8; ++k) mine |= v[k] > kFree;
+        if (!__syncthreads_or(mine)) continue;
+        if (HIST) {
+            // run-length over the 8 cells: one LDS atomic per run of equal labels
+            uint32_t cur = 0, run = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t m = v[k] > kFree ? (uint32_t)(v[k] & kMask15) : 0xFFFFFFFFu;
+                if (m != cur && run) {
+                    if (cur < kLdsBins) atomicAdd(&bins[cur], run);
+                    else atomicAdd(J->hist + cur, run);
+                    run = 0;
+                }
+                if (m != 0xFFFFFFFFu) {
+                    cur = m;
+                    ++run;
+                }
+            }
+            if (run) {
+                if (cur < kLdsBins) atomicAdd(&bins[cur], run);
+                else atomicAdd(J->hist + cur, run);
+            }
+        }
+        if (!bound) continue;
+        // whole box (interior + halo), clamped
         for (int i = threadIdx.x; i < kBHX * kBHY * kBHZ; i += blockDim.x) {
             const int hz = i % kBHZ, hy = (i / kBHZ) % kBHY, hx = i / (kBHZ * kBHY);
-            const int gx = min(max(x0 + hx - 1, 0), nx - 1), gy = min(max(y0 + hy - 1, 0), ny - 1),
-                      gz = min(max(z0 + hz - 1, 0), nz - 1);
-            const uint16_t v = J->labels[((size_t)gx * ny + gy) * nz + gz];
-            raw[i] = v;
-            const bool inside = hx >= 1 && hx <= kBTX && hy >= 1 && hy <= kBTY && hz >= 1 && hz <= kBTZ &&
-                                x0 + hx - 1 < nx && y0 + hy - 1 < ny && z0 + hz - 1 < nz;
-            if (inside && v > kFree) any_label = 1;
+            const int cx = min(max(x0 + hx - 1, 0), nx - 1), cy = min(max(y0 + hy - 1, 0), ny - 1),
+                      cz = min(max(z0 + hz - 1, 0), nz - 1);
+            raw[i] = J->labels[((size_t)cx * ny + cy) * nz + cz];
         }
         __syncthreads();
-        if (!any_label) continue;
-        if (bound) {
-            // z pass over every (hx, hy) of the halo box
-            for (int i = threadIdx.x; i < kBHX * kBHY * kBTZ; i += blockDim.x) {
-                const int z = i % kBTZ, r = i / kBTZ;  // r = hx * kBHY + hy
-                uint16_t mx = 0, mn = 0xFFFF;
-                for (int dz = 0; dz < 3; ++dz) {
-                    const uint16_t m = raw[r * kBHZ + z + dz] & kMask15;
-                    const bool lab = m > kFree;
-                    mx = max(mx, lab ? m : (uint16_t)0);
-                    mn = min(mn, lab ? m : (uint16_t)0xFFFF);
-                }
-                ax[i] = mx;
-                bx[i] = mn;
+        for (int i = threadIdx.x; i < kBHX * kBHY * kBTZ; i += blockDim.x) {
+            const int z = i % kBTZ, r = i / kBTZ;
+            uint16_t mx = 0, mn = 0xFFFF;
+            for (int dz = 0; dz < 3; ++dz) {
+                const uint16_t m = raw[r * kBHZ + z + dz] & kMask15;
+                const bool lab = m > kFree;
+                mx = max(mx, lab ? m : (uint16_t)0);
+                mn = min(mn, lab ? m : (uint16_t)0xFFFF);
             }
-            __syncthreads();
-            // y pass
-            for (int i = threadIdx.x; i < kBHX * kBTY * kBTZ; i += blockDim.x) {
-                const int z = i % kBTZ, y = (i / kBTZ) % kBTY, hx = i / (kBTZ * kBTY);
-                uint16_t mx = 0, mn = 0xFFFF;
-                for (int dy = 0; dy < 3; ++dy) {
-                    const int k = (hx * kBHY + y + dy) * kBTZ + z;
-                    mx = max(mx, ax[k]);
-                    mn = min(mn, bx[k]);
-                }
-                ay[i] = mx;
-                by[i] = mn;
-            }
-            __syncthreads();
+            ax[i] = mx;
+            bx[i] = mn;
         }
-        // x pass + write-back: thread owns 16 z-consecutive cells of one (x, y) row
-        const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 16;
-        const int x = row / kBTY, y = row % kBTY;
-        const int gx = x0 + x, gy = y0 + y;
-        if (gx < nx && gy < ny) {
-            uint16_t out[16];
-            bool changed = false;
-            for (int q = 0; q < 16; ++q) {
-                const int z = seg + q;
-                const uint16_t c = raw[((x + 1) * kBHY + (y + 1)) * kBHZ + z + 1];
-                out[q] = c;
-                if (z0 + z >= nz || c <= kFree) continue;
-                if (HIST) {
-                    const uint32_t m = c & kMask15;
-                    if (m < kLdsBins) atomicAdd(&bins[m], 1u);
-                    else atomicAdd(J->hist + m, 1u);
-                }
-                if (!bound || (c & 0x8000u)) continue;
-                uint16_t mx = 0, mn = 0xFFFF;
-                for (int dx = 0; dx < 3; ++dx) {
-                    const int k = ((x + dx) * kBTY + y) * kBTZ + z;
-                    mx = max(mx, ay[k]);
-                    mn = min(mn, by[k]);
-                }
-                if (mx != c || mn != c) {
-                    out[q] = (uint16_t)(c | 0x8000u);
-                    changed = true;
-                }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kBHX * kBTY * kBTZ; i += blockDim.x) {
+            const int z = i % kBTZ, y = (i / kBTZ) % kBTY, hx = i / (kBTZ * kBTY);
+            uint16_t mx = 0, mn = 0xFFFF;
+            for (int dy = 0; dy < 3; ++dy) {
+                const int k = (hx * kBHY + y + dy) * kBTZ + z;
+                mx = max(mx, ax[k]);
+                mn = min(mn, bx[k]);
             }
-            if (changed) {
-                uint16_t* dst = J->labels + ((size_t)gx * ny + gy) * nz + z0 + seg;
-                const int nzv = min(16, nz - (z0 + seg));
-                for (int q = 0; q < nzv; ++q) dst[q] = out[q];
+            ay[i] = mx;
+            by[i] = mn;
+        }
+        __syncthreads();
+        bool changed = false;
+        for (int k = 0; k < nval; ++k) {
+            const uint16_t c = v[k];
+            if (c <= kFree || (c & 0x8000u)) continue;
+            const int z = seg + k;
+            uint16_t mx = 0, mn = 0xFFFF;
+            for (int dx = 0; dx < 3; ++dx) {
+                const int q = ((tx + dx) * kBTY + ty) * kBTZ + z;
+                mx = max(mx, ay[q]);
+                mn = min(mn, by[q]);
+            }
+            if (mx != c || mn != c) {
+                v[k] = (uint16_t)(c | 0x8000u);
+                changed = true;
+            }
+        }
+        if (changed) {
+            uint16_t* dst = J->labels + ((size_t)gx * ny + gy) * nz + gz;
+            if (nval == 8 && (nz & 7) == 0) {
+                uint4 q;
+                q.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+                q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+                q.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+                q.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+                *reinterpret_cast<uint4*>(dst) = q;
+            } else {
+                for (int k = 0; k < nval; ++k) dst[k] = v[k];
             }
         }
         __syncthreads();

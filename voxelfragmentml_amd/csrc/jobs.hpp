@@ -11,7 +11,12 @@
 
 namespace vf {
 
-constexpr uint32_t kLevelRing = 4;   // flood bucket ring (node weights are 1..3, levels d..d+3 live)
+constexpr uint32_t kLevelRing = 4;
+// flood keys are stored in 4x4x4 bricks over dims padded to multiples of 4
+inline uint64_t flood_key_cells(uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    return (uint64_t)((nx + 3) & ~3u) * ((ny + 3) & ~3u) * ((nz + 3) & ~3u);
+}   // flood bucket ring (node weights are 1..3, levels d..d+3 live)
 constexpr uint32_t kMaxNoiseAmp = 2;  // weights 1 + [0, amp] must stay within the ring
 constexpr uint32_t kTileCells = 2048; // cells per tile for cell-parallel stages (256 threads x 8 cells)
 
@@ -66,7 +71,7 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys, V: 0 = EMPTY, ~0 = unreached, else (level << key_bits) | seed
+    uint32_t* keys;         // flood keys (bricked, flood_key_cells): 0 = EMPTY, ~0 = unreached, else (level << key_bits) | seed
     uint32_t* buckets;      // kLevelRing buckets of bucket_cap cells
     uint32_t bucket_cap;
     uint32_t* fctl;         // flood control: [0] level, [1] drained-at level (0 = running), [2..9] level counters
@@ -135,10 +140,10 @@ void launch_undo_mask(const Launch& L, uint16_t keep_mask);
 void launch_clear_hist(const Launch& L);
 void launch_histogram(const Launch& L);
 void launch_boundaries(const Launch& L, int size /* < 0: per-job bsize */, int with_hist);
-// detectBoundaries(size <= 1 per job) fused with the histogram, 8x8x64 LDS
+// detectBoundaries(size <= 1 per job) fused with the histogram, 8x8x32 LDS
 // bricks; bt_prefix: njobs+1 prefix of bricks; jobs with size > 1 go through
 // the generic kernel when any_wide.
-constexpr uint32_t kBrickX = 8, kBrickY = 8, kBrickZ = 64;
+constexpr uint32_t kBrickX = 8, kBrickY = 8, kBrickZ = 32;
 inline uint32_t brick_count(uint32_t nx, uint32_t ny, uint32_t nz)
 {
     return ((nx + kBrickX - 1) / kBrickX) * ((ny + kBrickY - 1) / kBrickY) * ((nz + kBrickZ - 1) / kBrickZ);
