@@ -215,9 +215,11 @@ typedef struct {
     uint64_t checksum;      /* order-independent checksum of the final grid */
     uint32_t num_labels;    /* distinct labels with a nonzero count */
     uint32_t mc_vertices, mc_triangles;
-    uint32_t flood_levels;  /* BFS levels run (flood) */
+    uint32_t flood_levels;  /* flood relaxation rounds run (brick worklist rounds until drained) */
     double latency_ms;      /* submit -> completion (server) or batch wall time */
     double queue_ms;        /* submit -> admission (server) */
+    uint32_t flood_bricks;  /* flood work: brick visits over all rounds */
+    uint32_t flood_sweeps;  /* flood work: in-brick relaxation sweeps over all visits */
 } vf_result;
 
 typedef struct vf_batch vf_batch;

@@ -1,16 +1,16 @@
 // C ABI for the batched, plain and served forms.
 //
 //   vf_batch_run   all jobs share every stage launch; the front, the flood
-//                  level chunk and the post stages are captured once into HIP
+//                  round chunk and the post stages are captured once into HIP
 //                  graphs and replayed while the job set is unchanged.
 //   vf_plain_run   the same stages one job at a time, one job per launch, with
-//                  a host check after every few flood levels (the reference's
+//                  a host check after every few flood rounds (the reference's
 //                  per-dispatch round trip, FloodFracturer.cpp:143-158, batched
 //                  to chunks).
 //   vf_server_*    continuous batching: a scheduler thread admits queued jobs
 //                  against an HBM budget and a slot limit, runs their front
-//                  stages together, lets new flood jobs join the shared level
-//                  launches at their own level, and retires drained jobs, so
+//                  stages together, lets new flood jobs join the shared round
+//                  launches at their own round, and retires drained jobs, so
 //                  work of jobs that arrived at different times overlaps.
 #include <condition_variable>
 #include <cstring>
@@ -26,7 +26,8 @@ using namespace vf;
 
 namespace {
 
-constexpr uint32_t kLevelChunk = 16;
+constexpr uint32_t kRoundChunk = 8;   // flood rounds per captured chunk
+constexpr uint32_t kServerRounds = 4; // flood rounds per scheduler tick
 
 uint64_t mix64(uint64_t h, uint64_t v)
 {
@@ -103,12 +104,12 @@ struct vf_batch {
     std::vector<EJob> jobs;          // every submitted job (invalid ones keep their status)
     std::vector<EJob*> live;         // jobs on the device
     Table* table = nullptr;
-    Graph g_front, g_levels, g_post;
+    Graph g_front, g_rounds, g_post;
 
     void clear()
     {
         g_front.reset();
-        g_levels.reset();
+        g_rounds.reset();
         g_post.reset();
         if (table) eng.drop_table(table);
         table = nullptr;
@@ -184,12 +185,12 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
                 // the first chunks run without a check; then one readback per chunk
                 uint32_t chunks = 0;
                 for (;;) {
-                    run_captured(E.s, b->g_levels, b->use_graphs, [&] { E.levels(T, kLevelChunk); });
-                    if (++chunks < 2) continue;
+                    run_captured(E.s, b->g_rounds, b->use_graphs, [&] { E.rounds(T, kRoundChunk); });
+                    if (++chunks < 3) continue;
                     E.gather(T, E.pinned);
                     VF_HIP(hipStreamSynchronize(E.s));
                     if (all_drained(T, E.pinned)) break;
-                    if ((uint64_t)chunks * kLevelChunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
+                    if ((uint64_t)chunks * kRoundChunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                 }
             }
             run_captured(E.s, b->g_post, b->use_graphs, [&] { E.post(T); });
@@ -232,12 +233,12 @@ int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results)
                 if (T->any_flood) {
                     uint32_t chunk = 4, total = 0;
                     for (;;) {
-                        E->levels(*T, chunk);
+                        E->rounds(*T, chunk);
                         total += chunk;
                         E->gather(*T, E->pinned);
                         VF_HIP(hipStreamSynchronize(E->s));
                         if (all_drained(*T, E->pinned)) break;
-                        chunk = std::min<uint32_t>(chunk * 2, 64);
+                        chunk = std::min<uint32_t>(chunk * 2, 32);
                         if (total > (1u << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                     }
                 }
@@ -375,14 +376,14 @@ void vf_server::loop()
                     E.drop_table(P);
                 }
             }
-            // one chunk of BFS levels for every flood job in flight
+            // one chunk of relaxation rounds for every flood job in flight
             if (!leveling.empty()) {
                 if (lev_dirty || !lev) {
                     if (lev) E.drop_table(lev);
                     lev = E.make_table(leveling);
                     lev_dirty = false;
                 }
-                E.levels(*lev, 8);
+                E.rounds(*lev, kServerRounds);
                 E.gather(*lev, E.pinned);
                 VF_HIP(hipStreamSynchronize(E.s));
                 std::vector<uint32_t> w(E.pinned, E.pinned + 16 * leveling.size());

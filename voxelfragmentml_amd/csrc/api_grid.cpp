@@ -117,8 +117,9 @@ void check_device_error(vf_grid* g, const char* stage)
     if (!e) return;
     VF_HIP(hipMemsetAsync(g->stats.as<unsigned long long>() + kStatError, 0, 8, g->stream));
     if (e & kDevErrSeeding) fail(VF_ERR_SEEDING, "%s: fewer candidate cells than seeds", stage);
-    if (e & kDevErrKeyOverflow) fail(VF_ERR_OVERFLOW, "%s: BFS level exceeds the packed key range", stage);
-    fail(VF_ERR_OVERFLOW, "%s: frontier bucket overflow", stage);
+    if (e & kDevErrKeyOverflow) fail(VF_ERR_OVERFLOW, "%s: flood distance exceeds the packed key range", stage);
+    if (e & kDevErrRelax) fail(VF_ERR_OVERFLOW, "%s: flood brick did not settle", stage);
+    fail(VF_ERR_OVERFLOW, "%s: device error flag 0x%llx", stage, (unsigned long long)e);
 }
 
 static void validate_seeds(const vf_grid* g, const vf_seed* seeds, uint32_t S)
@@ -459,18 +460,14 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
             if (seeds[s].w < 2 || seeds[s].w > 255)
                 fail(VF_ERR_TOO_MANY_SEEDS, "flood label %u outside 2..255 (8-bit fragment id, Seeder.h:12)", seeds[s].w);
         upload_seeds(g, seeds, S);
-        // bucket capacity: every cell is pushed at most once
-        uint64_t occupied = 0;
-        int rc = vf_grid_count_occupied(g, &occupied);
-        if (rc != VF_OK) throw Exception{ rc };
-        const uint64_t cap = occupied + S + 64;
         JobDev& J = g->job;
-        g->keys.ensure(flood_key_cells(g->dims[0], g->dims[1], g->dims[2]) * 4 + 64);
-        g->buckets.ensure(cap * 4 * kLevelRing);
+        const uint64_t rw = relax_words(g->dims[0], g->dims[1], g->dims[2]);
+        g->keys.ensure(g->V * 4 + 64);
+        g->buckets.ensure(rw * 4);
         g->fctl.ensure(64 * 4);
         J.keys = g->keys.as<uint32_t>();
-        J.buckets = g->buckets.as<uint32_t>();
-        J.bucket_cap = (uint32_t)cap;
+        J.bricks = g->buckets.as<uint32_t>();
+        J.nbricks = relax_bricks(g->dims[0], g->dims[1], g->dims[2]);
         J.fctl = g->fctl.as<uint32_t>();
         J.key_bits = key_bits_for(S);
         J.neighbourhood = nb;
@@ -479,19 +476,33 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         J.algo = VF_FLOOD;
         g->job_dirty = true;
         Launch L = grid_launch(g);
-        VF_HIP(hipMemsetAsync(g->fctl.as<uint32_t>() + 32, 0, 4, g->stream));  // blocks_done
+        // fctl words 32..33: chunk prefix of the one job; 40..47: worklist counters
+        const uint32_t chunks = relax_chunks(g->dims[0], g->dims[1], g->dims[2]);
+        FloodRoundScratch W{};
+        W.chunk_prefix = g->fctl.as<uint32_t>() + 32;
+        W.total_chunks = chunks;
+        W.list_cap = flood_list_cap(chunks);
+        g->flood_lists.ensure((size_t)kFloodXcds * W.list_cap * 8);
+        W.lists = g->flood_lists.as<unsigned long long>();
+        W.counts = g->fctl.as<uint32_t>() + 40;
+        std::memset(g->pinned, 0, 64);
+        g->pinned[1] = chunks;
+        VF_HIP(hipMemcpyAsync(g->fctl.as<uint32_t>() + 32, g->pinned, 64, hipMemcpyHostToDevice, g->stream));
+        VF_HIP(hipMemsetAsync(J.bricks, 0, (size_t)J.nbricks * 8, g->stream));  // both brick flag arrays
+        VF_HIP(hipStreamSynchronize(g->stream));  // the pinned words are reused below
         launch_flood_init(L);
-        launch_flood_seed(L, S);
-        uint32_t chunk = 4, levels = 0;
+        launch_flood_seed(L);
+        uint32_t chunk = 4, rounds = 0;
         for (;;) {
-            for (uint32_t k = 0; k < chunk; ++k) launch_flood_level(L, g->fctl.as<uint32_t>() + 32, 2048);
-            levels += chunk;
+            launch_flood_rounds(L, W, chunk, kFloodRoundBlocks);
+            rounds += chunk;
             VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
             VF_HIP(hipStreamSynchronize(g->stream));
             if (g->pinned[1] != 0) break;
-            chunk = std::min<uint32_t>(chunk * 2, 64);
-            // a cell's distance is at most 3 x (cells on its path), so the BFS must drain by then
-            if ((uint64_t)levels > 3ull * (g->V + S) + 16) fail(VF_ERR_OVERFLOW, "flood did not drain");
+            chunk = std::min<uint32_t>(chunk * 2, 32);
+            // every round lowers a cell's key at least one step along a shortest path, so the
+            // worklist drains within (cells + 1) rounds
+            if ((uint64_t)rounds > g->V + S + 16) fail(VF_ERR_OVERFLOW, "flood did not drain");
         }
         g->last_flood_levels = g->pinned[1];
         check_device_error(g, "flood");

@@ -21,8 +21,8 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-    size_t labels, occ, bits, seeds, hist, stats, fctl, keys, buckets, mesh_xyz, mesh_tri, total;
-    uint64_t V, words, cap;
+    size_t labels, occ, bits, seeds, hist, stats, fctl, keys, bricks, mesh_xyz, mesh_tri, total;
+    uint64_t V, words;
 };
 
 int resolved_neigh(const vf_job& j)
@@ -38,10 +38,6 @@ Layout layout_of(const vf_job& j)
     const uint32_t wp = next_pow2(ceil_div(j.dims[2], 32));
     L.words = (uint64_t)j.dims[0] * j.dims[1] * wp;
     const bool flood = j.algorithm == VF_FLOOD;
-    // a level's frontier holds at most the occupied cells; vessels occupy a few
-    // percent of the box, so a quarter of V (plus the seeds) is ample and an
-    // overflow is detected on the device
-    L.cap = std::min<uint64_t>(L.V + j.num_seeds + 64, L.V / 4 + j.num_seeds + 4096);
     size_t o = 0;
     L.labels = o; o = align_up(o + L.V * 2);
     L.occ = o; o = align_up(o + L.V);
@@ -50,8 +46,8 @@ Layout layout_of(const vf_job& j)
     L.hist = o; o = align_up(o + 65536 * 4);
     L.stats = o; o = align_up(o + kStatCount * 8);
     L.fctl = o; o = align_up(o + 64 * 4);
-    L.keys = o; if (flood) o = align_up(o + flood_key_cells(j.dims[0], j.dims[1], j.dims[2]) * 4);
-    L.buckets = o; if (flood) o = align_up(o + L.cap * 4 * kLevelRing);
+    L.keys = o; if (flood) o = align_up(o + L.V * 4);
+    L.bricks = o; if (flood) o = align_up(o + relax_words(j.dims[0], j.dims[1], j.dims[2]) * 4);
     L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
     L.mesh_tri = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nt * 12);
     L.total = o;
@@ -148,8 +144,8 @@ void Engine::alloc(EJob& e)
     d.bsize = j.boundary_size;
     if (j.algorithm == VF_FLOOD) {
         d.keys = reinterpret_cast<uint32_t*>(base + Lo.keys);
-        d.buckets = reinterpret_cast<uint32_t*>(base + Lo.buckets);
-        d.bucket_cap = (uint32_t)Lo.cap;
+        d.bricks = reinterpret_cast<uint32_t*>(base + Lo.bricks);
+        d.nbricks = relax_bricks(d.nx, d.ny, d.nz);
     } else {
         const uint64_t ex = d.nx - 1, ey = d.ny - 1, ez = d.nz - 1;
         const uint64_t dmax = j.distance == VF_MANHATTAN ? ex + ey + ez
@@ -191,14 +187,16 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     const size_t o_bt = align_up(o_col + (n + 1) * 4);
     const size_t o_gather = align_up(o_bt + (n + 1) * 4);
     const size_t o_done = align_up(o_gather + 16 * 4 * (size_t)n);
-    const size_t total = align_up(o_done + 64);
+    const size_t o_fcnt = align_up(o_done + 4 * ((size_t)n + 1));
+    const size_t total = align_up(o_fcnt + 4 * kFloodXcds);
     std::vector<uint8_t> img(total, 0);
     auto* jd = reinterpret_cast<JobDev*>(img.data() + o_jobs);
     auto* tp = reinterpret_cast<uint32_t*>(img.data() + o_tile);
     auto* rp = reinterpret_cast<uint32_t*>(img.data() + o_tri);
     auto* cp = reinterpret_cast<uint32_t*>(img.data() + o_col);
     auto* bp = reinterpret_cast<uint32_t*>(img.data() + o_bt);
-    uint64_t tiles = 0, tris = 0, groups = 0, btiles = 0;
+    auto* fp = reinterpret_cast<uint32_t*>(img.data() + o_done);
+    uint64_t tiles = 0, tris = 0, groups = 0, btiles = 0, fchunks = 0;
     for (uint32_t i = 0; i < n; ++i) {
         EJob& e = *js[i];
         jd[i] = e.d;
@@ -206,7 +204,9 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         rp[i] = (uint32_t)tris;
         cp[i] = (uint32_t)groups;
         bp[i] = (uint32_t)btiles;
+        fp[i] = (uint32_t)fchunks;
         btiles += brick_count(e.d.nx, e.d.ny, e.d.nz);
+        if (e.d.algo == VF_FLOOD) fchunks += relax_chunks(e.d.nx, e.d.ny, e.d.nz);
         t->any_wide |= e.d.bsize > 1;
         tiles += e.tiles;
         tris += e.d.nt;
@@ -215,12 +215,13 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         t->any_naive32 |= e.d.algo == VF_NAIVE && !e.d.naive_key64;
         t->any_naive64 |= e.d.algo == VF_NAIVE && e.d.naive_key64;
     }
-    if (tiles > 0xFFFFFFFFull || tris > 0xFFFFFFFFull || groups > 0xFFFFFFFFull)
+    if (tiles > 0xFFFFFFFFull || tris > 0xFFFFFFFFull || groups > 0xFFFFFFFFull || fchunks > 0xFFFFFFFFull)
         fail(VF_ERR_INVALID_ARG, "table too large for 32-bit prefixes");
     tp[n] = (uint32_t)tiles;
     rp[n] = (uint32_t)tris;
     cp[n] = (uint32_t)groups;
     bp[n] = (uint32_t)btiles;
+    fp[n] = (uint32_t)fchunks;
     VF_HIP(hipMallocAsync(&t->dev, total, s));
     VF_HIP(hipMemcpyAsync(t->dev, img.data(), total, hipMemcpyHostToDevice, s));
     VF_HIP(hipStreamSynchronize(s));  // the pageable source must outlive the copy
@@ -237,7 +238,14 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->total_tris = (uint32_t)tris;
     t->total_groups = (uint32_t)groups;
     t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
-    t->blocks_done = reinterpret_cast<uint32_t*>(base + o_done);
+    t->flood.chunk_prefix = reinterpret_cast<const uint32_t*>(base + o_done);
+    t->flood.total_chunks = (uint32_t)fchunks;
+    t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
+    if (fchunks) {
+        t->flood.list_cap = flood_list_cap((uint32_t)fchunks);
+        VF_HIP(hipMallocAsync(&t->flood_lists, (size_t)kFloodXcds * t->flood.list_cap * 8, s));
+        t->flood.lists = static_cast<unsigned long long*>(t->flood_lists);
+    }
     return t;
 }
 
@@ -245,6 +253,7 @@ void Engine::drop_table(Table* t)
 {
     if (!t) return;
     if (t->dev) VF_HIP(hipFreeAsync(t->dev, s));
+    if (t->flood_lists) VF_HIP(hipFreeAsync(t->flood_lists, s));
     delete t;
 }
 
@@ -283,13 +292,13 @@ void Engine::front(Table& t)
     if (t.any_naive64) launch_naive(L, true);
     if (t.any_flood) {
         launch_flood_init(L);
-        launch_flood_seed(L, 0);
+        launch_flood_seed(L);
     }
 }
 
-void Engine::levels(Table& t, uint32_t n)
+void Engine::rounds(Table& t, uint32_t n)
 {
-    for (uint32_t i = 0; i < n; ++i) launch_flood_level(t.L, t.blocks_done, level_blocks);
+    if (t.flood.total_chunks) launch_flood_rounds(t.L, t.flood, n, kFloodRoundBlocks);
 }
 
 void Engine::post(Table& t)
@@ -317,6 +326,8 @@ void Engine::finish(EJob& e, const uint32_t* w)
     r.checksum = stat(kStatChecksum);
     r.num_labels = (uint32_t)stat(kStatLabels);
     r.flood_levels = (uint32_t)stat(kStatFloodDone);
+    r.flood_bricks = (uint32_t)stat(kStatFloodWork);
+    r.flood_sweeps = (uint32_t)(stat(kStatFloodWork) >> 32);
     const uint64_t err = stat(kStatError);
     r.status = VF_OK;
     if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;

@@ -1,6 +1,6 @@
 // Multi-job execution engine shared by the batched, plain and served forms.
 //
-// A fracture job runs voxelize -> seed -> (naive | flood levels) -> flood
+// A fracture job runs voxelize -> seed -> (naive | flood rounds) -> flood
 // finish -> boundaries + histogram -> checksum [-> marching cubes], the
 // sequence CADScene::fractureModel + prepareScene drives
 // (MF/Source/Graphics/Application/CADScene.cpp:624-691, :791-833), with the
@@ -51,7 +51,8 @@ struct Table {
     uint32_t total_btiles = 0;
     bool any_wide = false;                // some job uses a boundary window wider than 1
     uint32_t* gather = nullptr;       // 16 words per job
-    uint32_t* blocks_done = nullptr;
+    FloodRoundScratch flood{};      // chunk prefix + worklists of the flood rounds
+    void* flood_lists = nullptr;    // separate allocation of flood.lists
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
 };
 
@@ -71,7 +72,7 @@ public:
     void drop_table(Table* t);
     void reserve_front(const Table& t);  // size the front scratch (not capturable)
     void front(Table& t);               // launches only
-    void levels(Table& t, uint32_t n);
+    void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
     void gather(Table& t, uint32_t* host16);  // async copy of the gathered stats
     // fill j.res from 16 gathered words; runs marching cubes and optional copies
@@ -79,7 +80,6 @@ public:
 
     uint32_t* pinned = nullptr;     // gathered stats staging (16 words x kMaxTableJobs)
     static constexpr uint32_t kMaxTableJobs = 1024;
-    uint32_t level_blocks = 2048;
 
 private:
     DevBuf vox_setup_, vox_counts_, vox_shell_off_, vox_col_off_, scan_tmp_, seed_cnt_, seed_off_;

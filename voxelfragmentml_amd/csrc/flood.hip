@@ -6,53 +6,388 @@
 // simultaneous claims by a race.  Here every cell gets the label of the seed
 // minimising (D, seed index), D the node-weighted geodesic distance in the 6-
 // or 26-neighbour graph of occupied cells with weight 1 + noise(cell) in
-// {1, 2, 3}.  Because weights belong to the target cell, the first level that
-// reaches a cell fixes its D, and all claims of that level race only on the
-// seed index, which atomicMin on the packed key (D << b | index) settles
-// exactly.  So the result is deterministic and equals a brute-force
+// {1, 2, 3}.  With the packed key K = (D << b) | index this is the unique
+// fixpoint of
+//     K(seed cell) = (1 << b) | index,   K(v) = min_u K(u) + (w(v) << b)
+// over the occupied neighbours u of v (positive weights make it unique), so
+// any relaxation order that reaches the fixpoint gives the same labels as a
 // per-seed Dijkstra.
 //
-// Levels are processed level-synchronously with a ring of 4 bucket arrays
-// (D, D+1, D+2, D+3 are the only live levels).  The level kernel is
-// self-advancing: its last block bumps each job's level, resets the counter
-// slot that level D+4 will use and marks drained jobs, so the host (or a
-// captured graph) simply replays the same launch.
+// The relaxation runs on 8x8x8 bricks.  A round processes a deduplicated
+// worklist of bricks; one wave loads a brick plus a one-cell halo of keys into
+// LDS and iterates min-plus sweeps (column minima from LDS, plus a z sweep in
+// registers) until the brick is locally stable, writes the cells that dropped
+// and queues the neighbour bricks that see a dropped cell through a face,
+// edge or corner.  A round needs no device-scope atomics per cell and the
+// rounds needed follow the geodesic depth in bricks, not in cells.
+//
+// Halo values read while a neighbour brick is being rewritten in the same
+// round may be old or new: both are upper bounds of the fixpoint, and a
+// brick that lowers a boundary cell queues its neighbours for the next round,
+// which reads the value after the kernel boundary.
 #include "cellops.hpp"
 
 namespace vf {
 
 namespace {
 
-constexpr uint32_t kFloodThreads = 256;
 constexpr uint32_t kEmptyKey = 0u;
 constexpr uint32_t kUnreached = 0xFFFFFFFFu;
+// tile values are key - 1: EMPTY becomes 0xFFFFFFFF and unreached 0xFFFFFFFE,
+// so a plain unsigned min over neighbours never selects either
+constexpr uint32_t kTileNone = 0xFFFFFFFEu;
+constexpr uint32_t kRowWords = 12;                 // halo z row: pad, z = -1..8, pad (48 B, 16 B aligned)
+constexpr uint32_t kTileWords = 10 * 10 * kRowWords;
+constexpr uint32_t kMaxSweeps = 2048;              // a brick of 512 cells is stable after at most 513 sweeps
+constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgroups are dealt round-robin
+constexpr uint32_t kGroupChunks = 16;              // consecutive 64-brick chunks kept on one XCD
 
-__constant__ int c_nb26[26][3] = {
-    { -1, -1, -1 }, { 1, 1, 1 }, { -1, -1, 0 }, { 1, 1, 0 }, { -1, -1, 1 }, { 1, 1, -1 }, { 0, -1, -1 },
-    { 0, 1, 1 },    { 0, -1, 0 }, { 0, 1, 0 },  { 0, -1, 1 }, { 0, 1, -1 }, { 1, -1, -1 }, { -1, 1, 1 },
-    { 1, -1, 0 },   { -1, 1, 0 }, { 1, -1, 1 }, { -1, 1, -1 }, { -1, 0, -1 }, { 1, 0, 1 }, { -1, 0, 0 },
-    { 1, 0, 0 },    { -1, 0, 1 }, { 1, 0, -1 }, { 0, 0, -1 }, { 0, 0, 1 }
-};
-__constant__ int c_nb6[6][3] = { { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
+__device__ inline uint32_t bdim(uint32_t n) { return (n + kRelaxBrick - 1) / kRelaxBrick; }
 
-// Keys live in 4x4x4 bricks (64 cells, 256 bytes): a cell's 26 neighbours fall
-// in one or two bricks instead of nine z-rows.  Dims are padded to multiples
-// of 4; padded cells are never visited (neighbour bounds use the real dims).
-__device__ inline uint32_t kidx(const JobDev* J, uint32_t x, uint32_t y, uint32_t z)
+// One block per job: stamp the seeds (a later seed on the same cell wins, as
+// FloodFracturer.cpp:99-105 writes them in list order), mark seed cells
+// occupied and flag, for round 1, every brick that holds a seed cell or one of
+// its neighbours.  Both flag arrays were zeroed by an earlier launch.
+__global__ __launch_bounds__(256) void k_flood_seed(Launch L)
 {
-    const uint32_t BY = (J->ny + 3) >> 2, BZ = (J->nz + 3) >> 2;
-    return ((((x >> 2) * BY + (y >> 2)) * BZ + (z >> 2)) << 6) | ((x & 3u) << 4) | ((y & 3u) << 2) | (z & 3u);
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active || J->algo != VF_FLOOD) return;
+    const uint32_t S = J->S, b = J->key_bits;
+    if (threadIdx.x < kFloodCtlWords) J->fctl[threadIdx.x] = 0;
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        atomicMax(J->keys + ((size_t)q.x * J->ny + q.y) * J->nz + q.z, (1u << b) | s);
+    }
+    __syncthreads();
+    const uint32_t nby = bdim(J->ny), nbz = bdim(J->nz);
+    uint32_t* flags1 = J->bricks + J->nbricks;  // round 1 consumes parity 1
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
+        if (J->keys[c] != ((1u << b) | s)) continue;
+        J->occ[c] = 1;
+        // bricks of the 3x3x3 neighbourhood: at most two brick indices per axis
+        const uint32_t bx0 = (q.x ? q.x - 1 : 0) / kRelaxBrick, bx1 = min(q.x + 1, J->nx - 1) / kRelaxBrick;
+        const uint32_t by0 = (q.y ? q.y - 1 : 0) / kRelaxBrick, by1 = min(q.y + 1, J->ny - 1) / kRelaxBrick;
+        const uint32_t bz0 = (q.z ? q.z - 1 : 0) / kRelaxBrick, bz1 = min(q.z + 1, J->nz - 1) / kRelaxBrick;
+        for (uint32_t bx = bx0; bx <= bx1; ++bx)
+            for (uint32_t by = by0; by <= by1; ++by)
+                for (uint32_t bz = bz0; bz <= bz1; ++bz) flags1[(bx * nby + by) * nbz + bz] = 1u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        J->fctl[0] = 1;
+        J->fctl[3] = 1;  // parity-1 flags are non-empty
+    }
 }
 
-__device__ inline void kdecode(const JobDev* J, uint32_t e, uint32_t& x, uint32_t& y, uint32_t& z)
+// Uniform per-job state of a round, held in scalar registers.
+struct RelaxJob {
+    const JobDev* J;
+    uint32_t nx, ny, nz, nby, nbz, nbx, nb, b, amp, r;
+    uint64_t noise_seed;
+    uint32_t* keys;
+    uint32_t* cur;   // flags consumed this round
+    uint32_t* nxt;   // flags for the next round
+    bool n26;
+};
+
+__device__ inline uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+template <typename T>
+__device__ inline T* uni_ptr(T* p)
 {
-    const uint32_t BY = (J->ny + 3) >> 2, BZ = (J->nz + 3) >> 2;
-    const uint32_t brick = e >> 6, off = e & 63u;
-    const uint32_t bz = brick % BZ, r = brick / BZ;
-    const uint32_t by = r % BY, bx = r / BY;
-    x = (bx << 2) | (off >> 4);
-    y = (by << 2) | ((off >> 2) & 3u);
-    z = (bz << 2) | (off & 3u);
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint64_t lo = uni((uint32_t)v), hi = uni((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(lo | (hi << 32));
+}
+
+// Relax one brick to local stability against its halo, write the cells that
+// dropped and flag the neighbour bricks that see one of them.  Returns the
+// sweeps run; *activated is set when a neighbour was flagged.
+__device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32_t* tile, bool& activated, bool& error)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nx = R.nx, ny = R.ny, nz = R.nz;
+    const uint32_t bz = brick % R.nbz, t1 = brick / R.nbz, by = t1 % R.nby, bx = t1 / R.nby;
+    const int X0 = (int)(bx * kRelaxBrick), Y0 = (int)(by * kRelaxBrick), Z0 = (int)(bz * kRelaxBrick);
+    // halo origin (X0-1, Y0-1, Z0-1): cell offsets below are relative to it and fit 32 bits
+    const int64_t org = ((int64_t)(X0 - 1) * ny + (Y0 - 1)) * nz + (Z0 - 1);
+    uint32_t* kb = R.keys + org;  // dereferenced only at in-grid cells
+
+    // brick + halo -> LDS as key - 1 (cells outside the grid are EMPTY), 8 loads in flight per lane
+#pragma unroll 1
+    for (uint32_t i1 = 0; i1 < 1024u; i1 += 512u) {
+#pragma unroll
+    for (uint32_t i0 = 0; i0 < 512u; i0 += 64u) {
+        const uint32_t i = i1 + i0 + lane;
+        const uint32_t hx = i / 100u, rem = i - hx * 100u, hy = rem / 10u, hz = rem - hy * 10u;
+        const uint32_t gx = (uint32_t)X0 + hx - 1u, gy = (uint32_t)Y0 + hy - 1u, gz = (uint32_t)Z0 + hz - 1u;
+        uint32_t v = kEmptyKey;
+        if (i < 1000u && gx < nx && gy < ny && gz < nz) v = kb[(hx * ny + hy) * nz + hz];
+        if (i < 1000u) tile[(hx * 10u + hy) * kRowWords + hz + 1u] = v - 1u;
+    }
+    }
+    __syncthreads();
+
+    // the lane's own z row of 8 cells
+    const uint32_t x = lane >> 3, y = lane & 7u;
+    const uint32_t own = ((x + 1u) * 10u + (y + 1u)) * kRowWords;
+    const uint32_t b = R.b;
+    const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
+    uint32_t k[8], w[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) k[z] = tile[own + 2 + z];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const bool occupied = k[z] != 0xFFFFFFFFu;  // out-of-grid cells load as EMPTY
+        const uint32_t cell = (uint32_t)(org + orow + z);
+        w[z] = occupied ? (1u + (R.amp ? noise_delta(R.noise_seed, cell, R.amp) : 0u)) << b : 0u;
+        if (!occupied) k[z] = kTileNone;  // never updated, never a source
+    }
+    uint32_t changed = 0;  // bit z: cell z dropped in this visit
+    bool overflow = false;
+    uint32_t sweeps = 0;
+    for (;;) {
+        uint32_t m[8];
+        if (R.n26) {
+            uint32_t cm[10];
+#pragma unroll
+            for (int o = 0; o < 10; ++o) cm[o] = 0xFFFFFFFFu;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const uint32_t* row = tile + ((x + dx) * 10u + (y + dy)) * kRowWords + 1u;
+#pragma unroll
+                    for (int o = 0; o < 10; ++o) cm[o] = min(cm[o], row[o]);
+                    __builtin_amdgcn_sched_barrier(0);  // keep one row of LDS values live at a time
+                }
+            // cm[o] covers halo z = o - 1; cell z sees cm[z], cm[z+1], cm[z+2]
+#pragma unroll
+            for (int z = 0; z < 8; ++z) m[z] = min(min(cm[z], cm[z + 1]), cm[z + 2]);
+        } else {
+            uint32_t cc[10], fm[8];
+            {
+                const uint32_t* row = tile + own + 1u;
+#pragma unroll
+                for (int o = 0; o < 10; ++o) cc[o] = row[o];
+            }
+#pragma unroll
+            for (int o = 0; o < 8; ++o) fm[o] = 0xFFFFFFFFu;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const uint32_t fx = f == 0 ? x : (f == 1 ? x + 2u : x + 1u);
+                const uint32_t fy = f < 2 ? y + 1u : (f == 2 ? y : y + 2u);
+                const uint32_t* row = tile + (fx * 10u + fy) * kRowWords + 2u;
+#pragma unroll
+                for (int o = 0; o < 8; ++o) fm[o] = min(fm[o], row[o]);
+            }
+#pragma unroll
+            for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]);
+        }
+        // candidate m + w must stay below kTileNone; m > kTileNone - 1 - w flags a key overflow
+        uint32_t ch = 0;
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (w[z] && m[z] < kTileNone) {
+                if (m[z] > kTileNone - 1u - w[z]) overflow = true;
+                else if (m[z] + w[z] < k[z]) { k[z] = m[z] + w[z]; ch |= 1u << z; }
+            }
+        }
+        // z sweeps in registers: the row's own neighbours along z
+#pragma unroll
+        for (int z = 1; z < 8; ++z) {
+            if (w[z] && k[z - 1] <= kTileNone - 1u - w[z] && k[z - 1] + w[z] < k[z]) {
+                k[z] = k[z - 1] + w[z];
+                ch |= 1u << z;
+            }
+        }
+#pragma unroll
+        for (int z = 6; z >= 0; --z) {
+            if (w[z] && k[z + 1] <= kTileNone - 1u - w[z] && k[z + 1] + w[z] < k[z]) {
+                k[z] = k[z + 1] + w[z];
+                ch |= 1u << z;
+            }
+        }
+        if (ch) {
+#pragma unroll
+            for (int z = 0; z < 8; ++z) tile[own + 2 + z] = k[z];
+        }
+        changed |= ch;
+        ++sweeps;
+        __syncthreads();
+        if (__ballot(ch != 0) == 0ull) break;
+        if (sweeps >= kMaxSweeps) {
+            error = true;
+            break;
+        }
+    }
+    if (overflow) error = true;
+
+    // write back the cells that dropped
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+        if (changed & (1u << z)) kb[orow + z] = k[z] + 1u;
+
+    // neighbour bricks that see a dropped cell: bit (dx+1)*9 + (dy+1)*3 + (dz+1)
+    uint32_t dirs = 0;
+    if (changed) {
+        const uint32_t sx = x == 0 ? 0u : (x == 7 ? 2u : 1u), sy = y == 0 ? 0u : (y == 7 ? 2u : 1u);
+        const uint32_t zlo = changed & 1u, zhi = (changed >> 7) & 1u;
+        if (R.n26) {
+            const uint32_t mz = 2u | zlo | (zhi << 2);  // bit iz for iz in {1, 0 if z = 0 dropped, 2 if z = 7 dropped}
+            dirs = (mz << 12) | (mz << (sx * 9u + 3u)) | (mz << (9u + sy * 3u)) | (mz << (sx * 9u + sy * 3u));
+            dirs &= ~(1u << 13);  // the brick itself
+        } else {
+            dirs = (sx != 1u ? 1u << (sx * 9u + 4u) : 0u) | (sy != 1u ? 1u << (10u + sy * 3u) : 0u) | (zlo << 12) | (zhi << 14);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dirs |= __shfl_xor(dirs, o, 64);
+    if (lane < 27u && (dirs & (1u << lane))) {
+        const int qx = (int)bx + (int)(lane / 9u) - 1, qy = (int)by + (int)((lane / 3u) % 3u) - 1,
+                  qz = (int)bz + (int)(lane % 3u) - 1;
+        if (qx >= 0 && qy >= 0 && qz >= 0 && (uint32_t)qx < R.nbx && (uint32_t)qy < R.nby && (uint32_t)qz < R.nbz)
+            R.nxt[((uint32_t)qx * R.nby + (uint32_t)qy) * R.nbz + (uint32_t)qz] = 1u;
+    }
+    activated |= dirs != 0;
+    __syncthreads();  // the tile is reused by the next brick
+    return sweeps;
+}
+
+// Load the uniform per-job state of a round into scalar registers.
+__device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
+{
+    const JobDev* J = L.jobs + j;
+    R.J = uni_ptr(J);
+    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u && uni(J->fctl[1]) == 0u)) return false;
+    R.nx = uni(J->nx); R.ny = uni(J->ny); R.nz = uni(J->nz);
+    R.nbx = bdim(R.nx); R.nby = bdim(R.ny); R.nbz = bdim(R.nz);
+    R.nb = uni(J->nbricks);
+    R.b = uni(J->key_bits);
+    R.amp = uni(J->noise_amp);
+    R.noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
+    R.keys = uni_ptr(J->keys);
+    R.r = uni(J->fctl[0]);
+    uint32_t* flags = uni_ptr(J->bricks);
+    R.cur = flags + (size_t)R.nb * (R.r & 1u);
+    R.nxt = flags + (size_t)R.nb * ((R.r + 1u) & 1u);
+    R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
+    return true;
+}
+
+// Round step 1: scan the flag arrays of every running job in chunks of 64
+// bricks (one flag per lane), clear the flags and append the flagged bricks
+// as (job << 32 | brick) to the worklist of the chunk's XCD.  Chunk groups of
+// kGroupChunks consecutive chunks belong to one XCD (workgroups are dealt to
+// the XCDs round-robin), so the relax step, which serves list x from the
+// workgroups of XCD x, finds the halos of neighbouring bricks in the same L2.
+// prefix: njobs + 1 chunk prefix over the jobs (0 chunks for non-flood jobs);
+// lists: kXcds lists of list_cap entries; counts: kXcds counters, zero on entry.
+__global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* prefix, uint32_t total_chunks,
+                                                      unsigned long long* lists, uint32_t list_cap, uint32_t* counts)
+{
+    __shared__ uint32_t sp[kFloodMaxJobs + 1];
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i <= L.njobs; i += 64u) sp[i] = prefix[i];
+    __syncthreads();
+    const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, local = blockIdx.x / kXcds;
+    if (local >= per_xcd) return;  // gridDim is a multiple of kXcds
+    RelaxJob R{};
+    uint32_t cur_job = 0xFFFFFFFFu;
+    bool running = false;
+    for (uint32_t k = local;; k += per_xcd) {
+        const uint32_t c = (xcd + kXcds * (k / kGroupChunks)) * kGroupChunks + k % kGroupChunks;
+        if (c >= total_chunks) break;
+        uint32_t lo = 0, hi = L.njobs;  // job of chunk c: largest j with sp[j] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sp[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint32_t j = uni(lo);
+        if (j != cur_job) {
+            cur_job = j;
+            running = load_relax_job(L, j, R);
+        }
+        if (!running) continue;
+        const uint32_t mine = (c - sp[j]) * 64u + lane;
+        const bool set = mine < R.nb && R.cur[mine] != 0u;
+        const uint64_t mask = __ballot(set);
+        if (!mask) continue;
+        if (set) R.cur[mine] = 0u;  // consumed
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counts + xcd, (uint32_t)__popcll(mask));
+        base = __shfl(base, 0, 64);
+        if (set) {
+            const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < list_cap) lists[(size_t)xcd * list_cap + slot] = ((unsigned long long)j << 32) | mine;
+        }
+    }
+}
+
+// Round step 2: relax the listed bricks; the workgroups of XCD x serve list x.
+__global__ __launch_bounds__(64) void k_flood_relax(Launch L, const unsigned long long* lists, uint32_t list_cap,
+                                                    const uint32_t* counts)
+{
+    __shared__ __align__(16) uint32_t tile[kTileWords];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, local = blockIdx.x / kXcds;
+    if (local >= per_xcd) return;
+    const uint32_t n = min(uni(counts[xcd]), list_cap);
+    const unsigned long long* list = lists + (size_t)xcd * list_cap;
+    RelaxJob R{};
+    uint32_t cur_job = 0xFFFFFFFFu;
+    bool activated = false, error = false;
+    unsigned long long work = 0;  // (sweeps << 32) + visits for cur_job
+    auto flush = [&]() {
+        if (cur_job == 0xFFFFFFFFu) return;
+        if (lane == 0) {
+            if (work) atomicAdd(R.J->stats + kStatFloodWork, work);
+            if (activated) R.J->fctl[2 + ((R.r + 1u) & 1u)] = 1u;
+            if (error) atomicOr(reinterpret_cast<unsigned int*>(R.J->stats + kStatError), (unsigned int)kDevErrRelax);
+        }
+        work = 0;
+        activated = error = false;
+    };
+    for (uint32_t e = local; e < n; e += per_xcd) {
+        const unsigned long long entry = list[e];
+        const uint32_t j = uni((uint32_t)(entry >> 32)), brick = uni((uint32_t)entry);
+        if (j != cur_job) {
+            flush();
+            cur_job = j;
+            load_relax_job(L, j, R);  // listed jobs are running
+        }
+        const uint32_t sw = relax_brick(R, brick, tile, activated, error);
+        work += ((unsigned long long)sw << 32) | 1ull;
+    }
+    flush();
+}
+
+// Round step 3 (single block): advance every running flood job to its next
+// round, or mark it drained when no brick was flagged for that round, and
+// clear the worklist counters.  The flags of the round just run were cleared
+// as they were listed; its non-empty word is cleared here because the round
+// after next writes it.  njobs <= 1024.
+__global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
+{
+    const uint32_t j = threadIdx.x;
+    if (j < kXcds) counts[j] = 0u;
+    if (j >= L.njobs) return;
+    const JobDev* J = L.jobs + j;
+    if (!J->active || J->algo != VF_FLOOD || J->fctl[1] != 0u) return;
+    const uint32_t r = J->fctl[0];
+    if (J->fctl[2 + ((r + 1u) & 1u)] == 0u) {
+        J->fctl[1] = r;
+        J->stats[kStatFloodDone] = r;
+    } else {
+        J->fctl[2 + (r & 1u)] = 0u;
+        J->fctl[0] = r + 1u;
+    }
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
@@ -64,264 +399,21 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
         const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J->V) continue;
         const uint64_t o = load8b(J->occ, c0, J->V);
-        uint32_t x, y, z;
-        decompose(c0, J, x, y, z);
-        for (int k = 0; k < 8; ++k) {
-            if (c0 + k >= J->V) break;
-            J->keys[kidx(J, x, y, z)] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
-            advance(x, y, z, J);
-        }
-    }
-}
-
-// One block per job: stamp the seeds (a later seed on the same cell wins, as
-// FloodFracturer.cpp:99-105 writes them in list order), mark seed cells
-// occupied, and push level 1 (level 0 is never used so that 0 can mean EMPTY).
-__global__ __launch_bounds__(kFloodThreads) void k_flood_seed(Launch L)
-{
-    const JobDev* J = L.jobs + blockIdx.x;
-    if (!J->active || J->algo != VF_FLOOD) return;
-    const uint32_t S = J->S, b = J->key_bits;
-    for (uint32_t i = threadIdx.x; i < kFloodCtlWords; i += blockDim.x) J->fctl[i] = 0;
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-        const uint4 q = J->seeds[s];
-        J->keys[kidx(J, q.x, q.y, q.z)] = 0u;
-    }
-    __syncthreads();
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-        const uint4 q = J->seeds[s];
-        atomicMax(J->keys + kidx(J, q.x, q.y, q.z), (1u << b) | s);
-    }
-    __syncthreads();
-    __shared__ uint32_t npush;
-    if (threadIdx.x == 0) npush = 0;
-    __syncthreads();
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-        const uint4 q = J->seeds[s];
-        const uint32_t kc = kidx(J, q.x, q.y, q.z);
-        if (J->keys[kc] == ((1u << b) | s)) {
-            const uint32_t slot = atomicAdd(&npush, 1u);
-            J->buckets[(size_t)(1 % kLevelRing) * J->bucket_cap + slot] = kc;
-            J->occ[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 1;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        J->fctl[0] = 1;
-        J->fctl[1] = npush ? 0u : 1u;
-        J->stats[kStatFloodDone] = npush ? 0ull : 1ull;
-        J->fctl[2 + 1 % 8] = npush;
-    }
-}
-
-// Wave-aggregated append of cell u to bucket (job j, level du).  All lanes of
-// the wave call it; `want` marks lanes with something to push.
-__device__ inline void push_aggregated(bool want, const JobDev* jobs, uint32_t j, uint32_t du, uint32_t u)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    uint64_t pending = __ballot(want);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1;
-        const uint32_t lj = __shfl(j, leader, 64), ld = __shfl(du, leader, 64);
-        const bool mine = want && j == lj && du == ld;
-        const uint64_t grp = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(jobs[lj].fctl + 2 + (ld & 7u), (uint32_t)__popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (mine) {
-            const JobDev* J = jobs + j;
-            const uint32_t slot = base + (uint32_t)__popcll(grp & lt);
-            if (slot < J->bucket_cap) J->buckets[(size_t)(du % kLevelRing) * J->bucket_cap + slot] = u;
-            else atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrBucket);
-            want = false;
-        }
-        pending &= ~grp;
-    }
-}
-
-// One BFS level for every job: one thread per (frontier cell, neighbour)
-// pair, so a level costs one chain of dependent memory accesses (bucket entry,
-// keys, atomicMin) instead of one per neighbour.  Each block owns a contiguous
-// range of pairs (so it touches one or two jobs) and stages the cells it
-// discovers in LDS; a flush reserves bucket space with one global atomic per
-// (job, level) the block touched, instead of one per wave, which keeps the
-// per-job bucket counters free of contention.
-constexpr uint32_t kStage = 2048;     // staged pushes per block before a flush
-constexpr uint32_t kStageKeys = 16;   // distinct (job, level) groups per flush
-
-__global__ __launch_bounds__(kFloodThreads) void k_flood_level(Launch L)
-{
-    __shared__ uint32_t pre[kFloodMaxJobs + 1];  // prefix of (entry, neighbour) pairs per job
-    __shared__ uint32_t lev[kFloodMaxJobs];
-    __shared__ uint32_t wsum[kFloodThreads / 64];
-    __shared__ uint32_t st_u[kStage], st_key[kStage], st_off[kStage];
-    __shared__ uint8_t st_grp[kStage];
-    __shared__ uint32_t g_key[kStageKeys], g_cnt[kStageKeys], g_base[kStageKeys];
-    __shared__ uint32_t st_n;
-    const uint32_t nj = L.njobs;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nj; base += kFloodThreads) {
-        const uint32_t j = base + threadIdx.x;
-        uint32_t n = 0;
-        if (j < nj) {
-            const JobDev* J = L.jobs + j;
-            const uint32_t d = J->fctl[0];
-            lev[j] = d;
-            uint32_t ne = (J->active && J->algo == VF_FLOOD && J->fctl[1] == 0) ? J->fctl[2 + (d & 7u)] : 0u;
-            if (ne > J->bucket_cap) ne = J->bucket_cap;
-            n = ne * (uint32_t)J->neighbourhood;
-        }
-        uint32_t x = n;
-        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        uint32_t wofs = 0, tot = 0;
-        for (uint32_t w = 0; w < kFloodThreads / 64; ++w) {
-            if (w < wave) wofs += wsum[w];
-            tot += wsum[w];
-        }
-        if (j < nj) pre[j] = carry + wofs + x - n;
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        pre[nj] = carry;
-        st_n = 0;
-    }
-    __syncthreads();
-    const uint32_t total = pre[nj];
-    // contiguous, 256-aligned pair range per block
-    const uint32_t per = (((total + gridDim.x - 1) / gridDim.x) + 255u) & ~255u;
-    const uint32_t pb = min(total, blockIdx.x * per), pe = min(total, pb + per);
-    if (pb >= pe) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-
-    auto flush = [&]() {
-        __syncthreads();
-        const uint32_t n = st_n;
-        if (threadIdx.x < kStageKeys) {
-            g_key[threadIdx.x] = 0xFFFFFFFFu;
-            g_cnt[threadIdx.x] = 0;
-        }
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t k = st_key[e];
-            uint32_t slot = kStageKeys;
-            for (uint32_t q = 0; q < kStageKeys; ++q) {
-                uint32_t cur = g_key[q];
-                if (cur == 0xFFFFFFFFu) cur = atomicCAS(&g_key[q], 0xFFFFFFFFu, k) == 0xFFFFFFFFu ? k : g_key[q];
-                if (cur == k) {
-                    slot = q;
-                    break;
-                }
-            }
-            st_grp[e] = (uint8_t)slot;
-            if (slot < kStageKeys) st_off[e] = atomicAdd(&g_cnt[slot], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x < kStageKeys && g_cnt[threadIdx.x]) {
-            const uint32_t k = g_key[threadIdx.x];
-            const uint32_t j = k >> 2, du = lev[j] + 1u + (k & 3u);
-            g_base[threadIdx.x] = atomicAdd(L.jobs[j].fctl + 2 + (du & 7u), g_cnt[threadIdx.x]);
-        }
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t k = st_key[e];
-            const uint32_t j = k >> 2, du = lev[j] + 1u + (k & 3u);
-            const JobDev* J = L.jobs + j;
-            uint32_t slot;
-            if (st_grp[e] < kStageKeys) slot = g_base[st_grp[e]] + st_off[e];
-            else slot = atomicAdd(J->fctl + 2 + (du & 7u), 1u);  // more than kStageKeys groups: rare
-            if (slot < J->bucket_cap) J->buckets[(size_t)(du % kLevelRing) * J->bucket_cap + slot] = st_u[e];
-            else atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrBucket);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) st_n = 0;
-        __syncthreads();
-    };
-
-    for (uint32_t p0 = pb; p0 < pe; p0 += blockDim.x) {
-        const uint32_t p = p0 + threadIdx.x;
-        bool want = false;
-        uint32_t j = 0, du = 0, u = 0;
-        if (p < pe) {
-            uint32_t lo = 0, hi = nj;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (pre[mid] <= p) lo = mid; else hi = mid;
-            }
-            j = lo;
-            const JobDev* J = L.jobs + j;
-            const uint32_t d = lev[j];
-            const uint32_t local = p - pre[j];
-            const int nb = J->neighbourhood;
-            const uint32_t e = nb == 6 ? local / 6u : local / 26u;
-            const int k = (int)(local - e * (uint32_t)nb);
-            const uint32_t v = J->buckets[(size_t)(d % kLevelRing) * J->bucket_cap + e];
-            const int* o = nb == 6 ? c_nb6[k] : c_nb26[k];
-            uint32_t x, y, z;
-            kdecode(J, v, x, y, z);
-            const int xx = (int)x + o[0], yy = (int)y + o[1], zz = (int)z + o[2];
-            if (xx >= 0 && yy >= 0 && zz >= 0 && xx < (int)J->nx && yy < (int)J->ny && zz < (int)J->nz) {
-                u = kidx(J, (uint32_t)xx, (uint32_t)yy, (uint32_t)zz);
-                const uint32_t b = J->key_bits;
-                const uint32_t ku = J->keys[u];
-                // quick reject: EMPTY (0) or already reached at a level <= d
-                if (ku != kEmptyKey && (ku >> b) > d) {
-                    const uint32_t idx = J->keys[v] & ((1u << b) - 1u);
-                    const uint32_t lin = ((uint32_t)xx * J->ny + (uint32_t)yy) * J->nz + (uint32_t)zz;
-                    du = d + 1u + noise_delta(J->noise_seed, lin, J->noise_amp);
-                    if (du >= (1u << (32 - b)) - 1u) {
-                        atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrKeyOverflow);
-                    } else {
-                        const uint32_t cand = (du << b) | idx;
-                        if (cand < ku) {
-                            const uint32_t old = atomicMin(J->keys + u, cand);
-                            want = cand < old && (old >> b) != du;
-                        }
-                    }
-                }
-            }
-        }
-        // stage this chunk's discoveries in LDS (one LDS atomic per wave)
-        const uint64_t m = __ballot(want);
-        if (m) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&st_n, (uint32_t)__popcll(m));
-            base = __shfl(base, 0, 64);
-            if (want) {
-                const uint32_t at = base + (uint32_t)__popcll(m & lt);
-                st_u[at] = u;
-                st_key[at] = (j << 2) | (du - lev[j] - 1u);
-            }
-        }
-        __syncthreads();
-        if (st_n > kStage - blockDim.x) flush();
-    }
-    flush();
-}
-
-// Advance every flood job to its next level: recycle the counter slot that
-// level d+4 will use (last used by level d-4) and mark jobs whose next three
-// levels are empty as drained.
-__global__ void k_flood_advance(Launch L)
-{
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < L.njobs; j += gridDim.x * blockDim.x) {
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->algo != VF_FLOOD || J->fctl[1] != 0) continue;
-        uint32_t* c = J->fctl + 2;
-        const uint32_t d = J->fctl[0];
-        c[(d + 4) & 7u] = 0u;
-        J->fctl[0] = d + 1;
-        if (c[(d + 1) & 7u] == 0u && c[(d + 2) & 7u] == 0u && c[(d + 3) & 7u] == 0u) {
-            J->fctl[1] = d + 1;
-            J->stats[kStatFloodDone] = d + 1;
+        if (c0 + 8 <= J->V) {
+            uint4 a, b;
+            a.x = (o & 0xFFull) ? kUnreached : kEmptyKey;
+            a.y = (o & 0xFF00ull) ? kUnreached : kEmptyKey;
+            a.z = (o & 0xFF0000ull) ? kUnreached : kEmptyKey;
+            a.w = (o & 0xFF000000ull) ? kUnreached : kEmptyKey;
+            b.x = (o & 0xFF00000000ull) ? kUnreached : kEmptyKey;
+            b.y = (o & 0xFF0000000000ull) ? kUnreached : kEmptyKey;
+            b.z = (o & 0xFF000000000000ull) ? kUnreached : kEmptyKey;
+            b.w = (o & 0xFF00000000000000ull) ? kUnreached : kEmptyKey;
+            reinterpret_cast<uint4*>(J->keys + c0)[0] = a;
+            reinterpret_cast<uint4*>(J->keys + c0)[1] = b;
+        } else {
+            for (int k = 0; k < 8; ++k)
+                if (c0 + k < J->V) J->keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
         }
     }
 }
@@ -336,11 +428,8 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
         if (c0 >= J->V) continue;
         const uint32_t mask = (1u << J->key_bits) - 1u;
         uint16_t out[8];
-        uint32_t x, y, z;
-        decompose(c0, J, x, y, z);
         for (int k = 0; k < 8; ++k) {
-            const uint32_t key = c0 + k < J->V ? J->keys[kidx(J, x, y, z)] : 0u;
-            advance(x, y, z, J);
+            uint32_t key = c0 + k < J->V ? J->keys[c0 + k] : 0u;
             if (key == kEmptyKey) out[k] = kEmpty;
             else if (key == kUnreached) out[k] = kFree;
             else out[k] = (uint16_t)(J->seeds[key & mask].w & 0xFFu);  // unmaskRightMost(value, 8)
@@ -359,24 +448,39 @@ void launch_flood_init(const Launch& L)
     VF_CHECK_LAUNCH();
 }
 
-void launch_flood_seed(const Launch& L, uint32_t /*max_seeds*/)
+void launch_flood_seed(const Launch& L)
 {
     TimedLaunch tl("flood_seed", L.s);
-    k_flood_seed<<<L.njobs, kFloodThreads, 0, L.s>>>(L);
+    k_flood_seed<<<L.njobs, 256, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 
-void launch_flood_level(const Launch& L, uint32_t* /*blocks_done*/, uint32_t blocks)
+uint32_t flood_list_cap(uint32_t total_chunks)
+{
+    const uint32_t groups = (total_chunks + kGroupChunks - 1) / kGroupChunks;
+    return (groups + kXcds - 1) / kXcds * kGroupChunks * 64u;
+}
+
+void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks)
 {
     if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
-    {
-        TimedLaunch tl("flood_level", L.s);
-        k_flood_level<<<blocks, kFloodThreads, 0, L.s>>>(L);
+    if (W.list_cap < flood_list_cap(W.total_chunks)) fail(VF_ERR_INVALID_ARG, "flood worklists too small");
+    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    for (uint32_t i = 0; i < rounds; ++i) {
+        {
+            TimedLaunch tl("flood_compact", L.s);
+            k_flood_compact<<<blocks, 64, 0, L.s>>>(L, W.chunk_prefix, W.total_chunks, W.lists, W.list_cap, W.counts);
+            VF_CHECK_LAUNCH();
+        }
+        {
+            TimedLaunch tl("flood_relax", L.s);
+            k_flood_relax<<<blocks, 64, 0, L.s>>>(L, W.lists, W.list_cap, W.counts);
+            VF_CHECK_LAUNCH();
+        }
+        TimedLaunch tl("flood_plan", L.s);
+        k_flood_plan<<<1, 1024, 0, L.s>>>(L, W.counts);
         VF_CHECK_LAUNCH();
     }
-    TimedLaunch tl("flood_advance", L.s);
-    k_flood_advance<<<ceil_div(L.njobs, 256), 256, 0, L.s>>>(L);
-    VF_CHECK_LAUNCH();
 }
 
 void launch_flood_finish(const Launch& L)

@@ -428,6 +428,10 @@ __global__ __launch_bounds__(256) void k_job_init(Launch L)
     for (uint32_t i = threadIdx.x; i < 65536; i += blockDim.x) J->hist[i] = 0;
     if (threadIdx.x < kStatCount) J->stats[threadIdx.x] = 0;
     if (J->fctl && threadIdx.x < 64) J->fctl[threadIdx.x] = 0;
+    if (J->algo == VF_FLOOD && J->bricks) {  // both brick flag arrays of the flood rounds
+        const uint32_t n4 = (2u * J->nbricks + 3u) >> 2;
+        for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) reinterpret_cast<uint4*>(J->bricks)[i] = make_uint4(0, 0, 0, 0);
+    }
 }
 
 // Block per job: stats and flood state into a compact table for one readback.

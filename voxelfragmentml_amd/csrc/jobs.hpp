@@ -11,13 +11,8 @@
 
 namespace vf {
 
-constexpr uint32_t kLevelRing = 4;
-// flood keys are stored in 4x4x4 bricks over dims padded to multiples of 4
-inline uint64_t flood_key_cells(uint32_t nx, uint32_t ny, uint32_t nz)
-{
-    return (uint64_t)((nx + 3) & ~3u) * ((ny + 3) & ~3u) * ((nz + 3) & ~3u);
-}   // flood bucket ring (node weights are 1..3, levels d..d+3 live)
-constexpr uint32_t kMaxNoiseAmp = 2;  // weights 1 + [0, amp] must stay within the ring
+constexpr uint32_t kMaxNoiseAmp = 2;  // flood cell weights are 1 + [0, amp]
+constexpr uint32_t kRelaxBrick = 8;   // flood relaxation brick edge (cells)
 constexpr uint32_t kTileCells = 2048; // cells per tile for cell-parallel stages (256 threads x 8 cells)
 
 enum StatSlot : int {
@@ -26,8 +21,8 @@ enum StatSlot : int {
     kStatShellPairs = 2, // (triangle, cell) SAT candidates
     kStatColPairs = 3,   // (triangle, column) parity candidates
     kStatError = 4,      // first error flag raised on the device
-    kStatFloodMax = 5,   // deepest BFS level with a pushed cell
-    kStatFloodDone = 6,  // level at which the flood drained
+    kStatFloodWork = 5,  // flood work: (sweeps << 32) + brick visits
+    kStatFloodDone = 6,  // flood relaxation rounds run (0 while running)
     kStatLabels = 7,     // labels >= 2 with a nonzero count
     kStatCount = 8
 };
@@ -36,8 +31,8 @@ enum StatSlot : int {
 enum DevError : uint32_t {
     kDevErrNone = 0,
     kDevErrSeeding = 1,     // fewer candidates than seeds
-    kDevErrKeyOverflow = 2, // flood level no longer fits the packed key
-    kDevErrBucket = 3,      // flood bucket capacity exceeded
+    kDevErrKeyOverflow = 2, // flood distance no longer fits the packed key
+    kDevErrRelax = 4,       // a flood brick did not settle within its sweep limit
 };
 
 struct TriSetup {
@@ -71,10 +66,10 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys (bricked, flood_key_cells): 0 = EMPTY, ~0 = unreached, else (level << key_bits) | seed
-    uint32_t* buckets;      // kLevelRing buckets of bucket_cap cells
-    uint32_t bucket_cap;
-    uint32_t* fctl;         // flood control: [0] level, [1] drained-at level (0 = running), [2..9] level counters
+    uint32_t* keys;         // flood keys, V: 0 = EMPTY, ~0 = unreached, else (distance << key_bits) | seed
+    uint32_t* bricks;       // 2 x nbricks: brick flags for even / odd rounds
+    uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
+    uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty
     unsigned long long* stats;  // kStatCount slots
     uint32_t* hist;         // 65536 counts
     int active;             // 0: the job is skipped by multi-job launches
@@ -164,13 +159,37 @@ void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint
                             uint32_t* host_counter);
 
 // flood.hip
-constexpr uint32_t kFloodCtlWords = 10;
+constexpr uint32_t kFloodCtlWords = 4;
 constexpr uint32_t kFloodMaxJobs = 1024;  // jobs per multi-job flood launch
+inline uint32_t relax_bricks(uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    return ((nx + kRelaxBrick - 1) / kRelaxBrick) * ((ny + kRelaxBrick - 1) / kRelaxBrick) *
+           ((nz + kRelaxBrick - 1) / kRelaxBrick);
+}
+// Device words (uint32) of a job's two brick flag arrays (the clear writes whole uint4).
+inline uint64_t relax_words(uint32_t nx, uint32_t ny, uint32_t nz) { return 2ull * relax_bricks(nx, ny, nz) + 4u; }
+// 64-brick flag chunks of a job (a round scans one chunk per wave step)
+inline uint32_t relax_chunks(uint32_t nx, uint32_t ny, uint32_t nz) { return (relax_bricks(nx, ny, nz) + 63u) / 64u; }
+// Both brick flag arrays of every flood job must be zero before
+// launch_flood_seed (launch_job_init clears them).
 void launch_flood_init(const Launch& L);
-void launch_flood_seed(const Launch& L, uint32_t max_seeds);
-// one BFS level for every active job; self-advancing (the last block bumps
-// each job's level), so the same launch can be replayed from a graph
-void launch_flood_level(const Launch& L, uint32_t* blocks_done, uint32_t blocks);
+void launch_flood_seed(const Launch& L);
+// Device state of the flood rounds of one launch table.
+struct FloodRoundScratch {
+    const uint32_t* chunk_prefix;  // njobs + 1: prefix of relax_chunks over the jobs (0 for non-flood jobs)
+    uint32_t total_chunks;
+    unsigned long long* lists;     // per-XCD brick worklists, list_cap entries each
+    uint32_t list_cap;             // >= flood_list_cap(total_chunks)
+    uint32_t* counts;              // per-XCD list lengths; zero before the first round
+};
+constexpr uint32_t kFloodXcds = 8;
+uint32_t flood_list_cap(uint32_t total_chunks);  // entries per XCD list
+// `rounds` relaxation rounds for every running flood job of L.  Each job
+// continues at its own round, so jobs may join a table between calls; rounds
+// after a job drained do nothing for it, so the launches can be replayed from
+// a graph.
+constexpr uint32_t kFloodRoundBlocks = 4096;  // one wave per block
+void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks);
 void launch_flood_finish(const Launch& L);
 
 // seed.hip

@@ -57,6 +57,7 @@ class Result(C.Structure):
         ("status", C.c_int), ("voxels", C.c_uint64), ("occupied", C.c_uint64), ("checksum", C.c_uint64),
         ("num_labels", C.c_uint32), ("mc_vertices", C.c_uint32), ("mc_triangles", C.c_uint32),
         ("flood_levels", C.c_uint32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+        ("flood_bricks", C.c_uint32), ("flood_sweeps", C.c_uint32),
     ]
 
 
