@@ -212,7 +212,8 @@ typedef struct {
     int status;
     uint64_t voxels;        /* V = nx*ny*nz */
     uint64_t occupied;      /* non-EMPTY cells after voxelization */
-    uint64_t checksum;      /* order-independent checksum of the final grid */
+    uint64_t checksum;      /* order-independent grid checksum: sum over non-EMPTY cells c (linear index) of
+                               (hash32(value, c) << 32) | hash32(c, value) */
     uint32_t num_labels;    /* distinct labels with a nonzero count */
     uint32_t mc_vertices, mc_triangles;
     uint32_t flood_levels;  /* flood relaxation rounds run (brick worklist rounds until drained) */

@@ -222,9 +222,11 @@ def rle_decode(data: bytes):
 
 
 def checksum(grid):
-    """Order-independent grid checksum, the same definition as the product's."""
+    """Order-independent grid checksum, the same definition as the product's:
+    the sum over non-EMPTY cells c of (hash32(value, c) << 32) | hash32(c, value)."""
     g = np.ascontiguousarray(grid, np.uint16).ravel().astype(np.uint64)
-    c = np.arange(g.size, dtype=np.uint64)
+    c = np.nonzero(g)[0].astype(np.uint64)
+    g = g[c]
     h1 = _hash32_np(g, c.astype(np.uint32))
     h2 = _hash32_np(c, g.astype(np.uint32))
     terms = (h1.astype(np.uint64) << np.uint64(32)) | h2.astype(np.uint64)

@@ -461,13 +461,13 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
                 fail(VF_ERR_TOO_MANY_SEEDS, "flood label %u outside 2..255 (8-bit fragment id, Seeder.h:12)", seeds[s].w);
         upload_seeds(g, seeds, S);
         JobDev& J = g->job;
-        const uint64_t rw = relax_words(g->dims[0], g->dims[1], g->dims[2]);
+        const uint32_t nbricks = relax_bricks(g->dims[0], g->dims[1], g->dims[2]);
         g->keys.ensure(g->V * 4 + 64);
-        g->buckets.ensure(rw * 4);
+        g->buckets.ensure(16ull * relax_words_per_parity(nbricks));
         g->fctl.ensure(64 * 4);
         J.keys = g->keys.as<uint32_t>();
-        J.bricks = g->buckets.as<uint32_t>();
-        J.nbricks = relax_bricks(g->dims[0], g->dims[1], g->dims[2]);
+        J.bflags = g->buckets.as<unsigned long long>();
+        J.nbricks = nbricks;
         J.fctl = g->fctl.as<uint32_t>();
         J.key_bits = key_bits_for(S);
         J.neighbourhood = nb;
@@ -476,19 +476,19 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         J.algo = VF_FLOOD;
         g->job_dirty = true;
         Launch L = grid_launch(g);
-        // fctl words 32..33: chunk prefix of the one job; 40..47: worklist counters
-        const uint32_t chunks = relax_chunks(g->dims[0], g->dims[1], g->dims[2]);
+        // fctl words 32..33: task prefix of the one job; 40..47: worklist counters
+        const uint32_t tasks = relax_tasks(g->dims[0], g->dims[1], g->dims[2]);
         FloodRoundScratch W{};
-        W.chunk_prefix = g->fctl.as<uint32_t>() + 32;
-        W.total_chunks = chunks;
-        W.list_cap = flood_list_cap(chunks);
+        W.task_prefix = g->fctl.as<uint32_t>() + 32;
+        W.total_tasks = tasks;
+        W.list_cap = flood_list_cap(tasks);
         g->flood_lists.ensure((size_t)kFloodXcds * W.list_cap * 8);
         W.lists = g->flood_lists.as<unsigned long long>();
         W.counts = g->fctl.as<uint32_t>() + 40;
         std::memset(g->pinned, 0, 64);
-        g->pinned[1] = chunks;
+        g->pinned[1] = tasks;
         VF_HIP(hipMemcpyAsync(g->fctl.as<uint32_t>() + 32, g->pinned, 64, hipMemcpyHostToDevice, g->stream));
-        VF_HIP(hipMemsetAsync(J.bricks, 0, (size_t)J.nbricks * 8, g->stream));  // both brick flag arrays
+        VF_HIP(hipMemsetAsync(J.bflags, 0, 16ull * relax_words_per_parity(nbricks), g->stream));  // both flag parities
         VF_HIP(hipStreamSynchronize(g->stream));  // the pinned words are reused below
         launch_flood_init(L);
         launch_flood_seed(L);

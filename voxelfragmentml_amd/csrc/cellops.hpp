@@ -75,6 +75,98 @@ __device__ inline void block_tile_range(uint32_t total, uint32_t& b, uint32_t& e
     if (e > total) e = total;
 }
 
+// Copy a job descriptor into LDS (block-wide; every thread must call it).
+__device__ inline void stage_job(JobDev* sJ, const JobDev* src)
+{
+    static_assert(sizeof(JobDev) % 4 == 0, "JobDev is copied as words");
+    __syncthreads();
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(sJ);
+    for (uint32_t i = threadIdx.x; i < sizeof(JobDev) / 4; i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
+
+// Walk this block's contiguous range of the global tiles (prefix: njobs + 1
+// tile prefix over the jobs).  The job is tracked incrementally and its
+// descriptor staged in LDS on every change, so a tile costs no lookup and no
+// dependent descriptor loads.  f(J, j, local_tile) runs block-uniformly;
+// leave(j) runs before the block moves off job j and at the end.
+template <class F, class G>
+__device__ inline void walk_tiles(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f, G&& leave)
+{
+    uint32_t tb, te;
+    block_tile_range(total, tb, te);
+    if (tb >= te) return;
+    uint32_t j = find_job(prefix, L.njobs, tb);
+    uint32_t jb = prefix[j], je = prefix[j + 1];
+    stage_job(sJ, L.jobs + j);
+    for (uint32_t t = tb; t < te; ++t) {
+        if (t >= je) {
+            leave(j);
+            do {
+                ++j;
+                jb = je;
+                je = prefix[j + 1];
+            } while (t >= je);
+            stage_job(sJ, L.jobs + j);
+        }
+        f(*sJ, j, t - jb, t);
+    }
+    leave(j);
+}
+
+template <class F>
+__device__ inline void walk_tiles(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f)
+{
+    walk_tiles(L, prefix, total, sJ, f, [](uint32_t) {});
+}
+
+// Grid-stride variant for compute-heavy stages whose tiles differ in cost:
+// tiles are dealt round-robin over the blocks (better balance); the tile
+// prefix (njobs <= 1024) is kept in LDS for the per-tile job search.
+constexpr uint32_t kWalkMaxJobs = 1024;
+template <class F>
+__device__ inline void walk_tiles_strided(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ,
+                                          uint32_t* sp, F&& f)
+{
+    for (uint32_t i = threadIdx.x; i <= L.njobs && i <= kWalkMaxJobs; i += blockDim.x) sp[i] = prefix[i];
+    __syncthreads();
+    uint32_t cur = 0xFFFFFFFFu;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t lo = 0, hi = L.njobs;  // largest j with sp[j] <= t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sp[mid] <= t) lo = mid; else hi = mid;
+        }
+        if (lo != cur) {
+            stage_job(sJ, L.jobs + lo);
+            cur = lo;
+        }
+        f(*sJ, lo, t - sp[lo], t);
+    }
+}
+
+__device__ inline void decompose(uint64_t c, const JobDev& J, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+    uint32_t yz = J.ny * J.nz;
+    uint32_t cc = (uint32_t)c;
+    x = cc / yz;
+    uint32_t r = cc - x * yz;
+    y = r / J.nz;
+    z = r - y * J.nz;
+}
+
+__device__ inline void advance(uint32_t& x, uint32_t& y, uint32_t& z, const JobDev& J)
+{
+    if (++z == J.nz) {
+        z = 0;
+        if (++y == J.ny) {
+            y = 0;
+            ++x;
+        }
+    }
+}
+
 __device__ inline void decompose(uint64_t c, const JobDev* J, uint32_t& x, uint32_t& y, uint32_t& z)
 {
     uint32_t yz = J->ny * J->nz;

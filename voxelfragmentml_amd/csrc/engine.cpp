@@ -47,7 +47,7 @@ Layout layout_of(const vf_job& j)
     L.stats = o; o = align_up(o + kStatCount * 8);
     L.fctl = o; o = align_up(o + 64 * 4);
     L.keys = o; if (flood) o = align_up(o + L.V * 4);
-    L.bricks = o; if (flood) o = align_up(o + relax_words(j.dims[0], j.dims[1], j.dims[2]) * 4);
+    L.bricks = o; if (flood) o = align_up(o + 16ull * relax_words_per_parity(relax_bricks(j.dims[0], j.dims[1], j.dims[2])));
     L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
     L.mesh_tri = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nt * 12);
     L.total = o;
@@ -144,7 +144,7 @@ void Engine::alloc(EJob& e)
     d.bsize = j.boundary_size;
     if (j.algorithm == VF_FLOOD) {
         d.keys = reinterpret_cast<uint32_t*>(base + Lo.keys);
-        d.bricks = reinterpret_cast<uint32_t*>(base + Lo.bricks);
+        d.bflags = reinterpret_cast<unsigned long long*>(base + Lo.bricks);
         d.nbricks = relax_bricks(d.nx, d.ny, d.nz);
     } else {
         const uint64_t ex = d.nx - 1, ey = d.ny - 1, ez = d.nz - 1;
@@ -206,7 +206,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         bp[i] = (uint32_t)btiles;
         fp[i] = (uint32_t)fchunks;
         btiles += brick_count(e.d.nx, e.d.ny, e.d.nz);
-        if (e.d.algo == VF_FLOOD) fchunks += relax_chunks(e.d.nx, e.d.ny, e.d.nz);
+        if (e.d.algo == VF_FLOOD) fchunks += relax_tasks(e.d.nx, e.d.ny, e.d.nz);
         t->any_wide |= e.d.bsize > 1;
         tiles += e.tiles;
         tris += e.d.nt;
@@ -238,8 +238,8 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->total_tris = (uint32_t)tris;
     t->total_groups = (uint32_t)groups;
     t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
-    t->flood.chunk_prefix = reinterpret_cast<const uint32_t*>(base + o_done);
-    t->flood.total_chunks = (uint32_t)fchunks;
+    t->flood.task_prefix = reinterpret_cast<const uint32_t*>(base + o_done);
+    t->flood.total_tasks = (uint32_t)fchunks;
     t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
     if (fchunks) {
         t->flood.list_cap = flood_list_cap((uint32_t)fchunks);
@@ -298,7 +298,7 @@ void Engine::front(Table& t)
 
 void Engine::rounds(Table& t, uint32_t n)
 {
-    if (t.flood.total_chunks) launch_flood_rounds(t.L, t.flood, n, kFloodRoundBlocks);
+    if (t.flood.total_tasks) launch_flood_rounds(t.L, t.flood, n, kFloodRoundBlocks);
 }
 
 void Engine::post(Table& t)

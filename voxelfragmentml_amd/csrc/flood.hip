@@ -40,7 +40,6 @@ constexpr uint32_t kRowWords = 12;                 // halo z row: pad, z = -1..8
 constexpr uint32_t kTileWords = 10 * 10 * kRowWords;
 constexpr uint32_t kMaxSweeps = 2048;              // a brick of 512 cells is stable after at most 513 sweeps
 constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgroups are dealt round-robin
-constexpr uint32_t kGroupChunks = 16;              // consecutive 64-brick chunks kept on one XCD
 
 __device__ inline uint32_t bdim(uint32_t n) { return (n + kRelaxBrick - 1) / kRelaxBrick; }
 
@@ -65,7 +64,7 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
     }
     __syncthreads();
     const uint32_t nby = bdim(J->ny), nbz = bdim(J->nz);
-    uint32_t* flags1 = J->bricks + J->nbricks;  // round 1 consumes parity 1
+    unsigned long long* flags1 = J->bflags + relax_words_per_parity(J->nbricks);  // round 1 consumes parity 1
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
         const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
@@ -77,7 +76,10 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
         const uint32_t bz0 = (q.z ? q.z - 1 : 0) / kRelaxBrick, bz1 = min(q.z + 1, J->nz - 1) / kRelaxBrick;
         for (uint32_t bx = bx0; bx <= bx1; ++bx)
             for (uint32_t by = by0; by <= by1; ++by)
-                for (uint32_t bz = bz0; bz <= bz1; ++bz) flags1[(bx * nby + by) * nbz + bz] = 1u;
+                for (uint32_t bz = bz0; bz <= bz1; ++bz) {
+                    const uint32_t id = (bx * nby + by) * nbz + bz;
+                    atomicOr(flags1 + (id >> 6), 1ull << (id & 63u));
+                }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -92,8 +94,9 @@ struct RelaxJob {
     uint32_t nx, ny, nz, nby, nbz, nbx, nb, b, amp, r;
     uint64_t noise_seed;
     uint32_t* keys;
-    uint32_t* cur;   // flags consumed this round
-    uint32_t* nxt;   // flags for the next round
+    uint32_t nw;                // flag words per parity
+    unsigned long long* cur;   // brick flag bits consumed this round
+    unsigned long long* nxt;   // brick flag bits for the next round
     bool n26;
 };
 
@@ -253,7 +256,10 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         const int qx = (int)bx + (int)(lane / 9u) - 1, qy = (int)by + (int)((lane / 3u) % 3u) - 1,
                   qz = (int)bz + (int)(lane % 3u) - 1;
         if (qx >= 0 && qy >= 0 && qz >= 0 && (uint32_t)qx < R.nbx && (uint32_t)qy < R.nby && (uint32_t)qz < R.nbz)
-            R.nxt[((uint32_t)qx * R.nby + (uint32_t)qy) * R.nbz + (uint32_t)qz] = 1u;
+        {
+            const uint32_t id = ((uint32_t)qx * R.nby + (uint32_t)qy) * R.nbz + (uint32_t)qz;
+            atomicOr(R.nxt + (id >> 6), 1ull << (id & 63u));
+        }
     }
     activated |= dirs != 0;
     __syncthreads();  // the tile is reused by the next brick
@@ -274,22 +280,24 @@ __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
     R.noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
     R.keys = uni_ptr(J->keys);
     R.r = uni(J->fctl[0]);
-    uint32_t* flags = uni_ptr(J->bricks);
-    R.cur = flags + (size_t)R.nb * (R.r & 1u);
-    R.nxt = flags + (size_t)R.nb * ((R.r + 1u) & 1u);
+    R.nw = relax_words_per_parity(R.nb);
+    unsigned long long* flags = uni_ptr(J->bflags);
+    R.cur = flags + (size_t)R.nw * (R.r & 1u);
+    R.nxt = flags + (size_t)R.nw * ((R.r + 1u) & 1u);
     R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
     return true;
 }
 
-// Round step 1: scan the flag arrays of every running job in chunks of 64
-// bricks (one flag per lane), clear the flags and append the flagged bricks
-// as (job << 32 | brick) to the worklist of the chunk's XCD.  Chunk groups of
-// kGroupChunks consecutive chunks belong to one XCD (workgroups are dealt to
-// the XCDs round-robin), so the relax step, which serves list x from the
-// workgroups of XCD x, finds the halos of neighbouring bricks in the same L2.
-// prefix: njobs + 1 chunk prefix over the jobs (0 chunks for non-flood jobs);
+// Round step 1: turn the flag bitmasks of every running job into worklists.
+// A task is 64 flag words (4096 bricks, one word per lane) of one job; each
+// lane clears its word and appends its flagged bricks as (job << 32 | brick)
+// to the worklist of the task's XCD, with one list reservation per wave.
+// Tasks are dealt to the XCDs round-robin like workgroups, so the relax step,
+// which serves list x from the workgroups of XCD x, finds the halos of
+// neighbouring bricks (mostly in the same task) in the same L2.
+// prefix: njobs + 1 task prefix over the jobs (0 tasks for non-flood jobs);
 // lists: kXcds lists of list_cap entries; counts: kXcds counters, zero on entry.
-__global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* prefix, uint32_t total_chunks,
+__global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* prefix, uint32_t total_tasks,
                                                       unsigned long long* lists, uint32_t list_cap, uint32_t* counts)
 {
     __shared__ uint32_t sp[kFloodMaxJobs + 1];
@@ -302,12 +310,12 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
     uint32_t cur_job = 0xFFFFFFFFu;
     bool running = false;
     for (uint32_t k = local;; k += per_xcd) {
-        const uint32_t c = (xcd + kXcds * (k / kGroupChunks)) * kGroupChunks + k % kGroupChunks;
-        if (c >= total_chunks) break;
-        uint32_t lo = 0, hi = L.njobs;  // job of chunk c: largest j with sp[j] <= c
+        const uint32_t task = xcd + kXcds * k;
+        if (task >= total_tasks) break;
+        uint32_t lo = 0, hi = L.njobs;  // job of the task: largest j with sp[j] <= task
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (sp[mid] <= c) lo = mid; else hi = mid;
+            if (sp[mid] <= task) lo = mid; else hi = mid;
         }
         const uint32_t j = uni(lo);
         if (j != cur_job) {
@@ -315,17 +323,30 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
             running = load_relax_job(L, j, R);
         }
         if (!running) continue;
-        const uint32_t mine = (c - sp[j]) * 64u + lane;
-        const bool set = mine < R.nb && R.cur[mine] != 0u;
-        const uint64_t mask = __ballot(set);
-        if (!mask) continue;
-        if (set) R.cur[mine] = 0u;  // consumed
+        const uint32_t w = (task - sp[j]) * 64u + lane;
+        const uint64_t m = w < R.nw ? R.cur[w] : 0ull;
+        const uint32_t n = (uint32_t)__popcll(m);
+        uint32_t inc = n;  // wave inclusive scan of the per-lane counts
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += t;
+        }
+        const uint32_t total = __shfl(inc, 63, 64);
+        if (total == 0) continue;
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(counts + xcd, (uint32_t)__popcll(mask));
-        base = __shfl(base, 0, 64);
-        if (set) {
-            const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-            if (slot < list_cap) lists[(size_t)xcd * list_cap + slot] = ((unsigned long long)j << 32) | mine;
+        if (lane == 0) base = atomicAdd(counts + xcd, total);
+        base = __shfl(base, 0, 64) + inc - n;
+        if (m) {
+            R.cur[w] = 0ull;  // consumed
+            uint64_t bits = m;
+            unsigned long long* dst = lists + (size_t)xcd * list_cap;
+            while (bits) {
+                const uint32_t bit = (uint32_t)__ffsll((long long)bits) - 1u;
+                bits &= bits - 1ull;
+                if (base < list_cap) dst[base] = ((unsigned long long)j << 32) | (w * 64u + bit);
+                ++base;
+            }
         }
     }
 }
@@ -392,14 +413,13 @@ __global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
 {
-    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
-        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->algo != VF_FLOOD) continue;
-        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J->V) continue;
-        const uint64_t o = load8b(J->occ, c0, J->V);
-        if (c0 + 8 <= J->V) {
+    __shared__ JobDev sJ;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+        if (!J.active || J.algo != VF_FLOOD) return;
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
+        const uint64_t o = load8b(J.occ, c0, J.V);
+        if (c0 + 8 <= J.V) {
             uint4 a, b;
             a.x = (o & 0xFFull) ? kUnreached : kEmptyKey;
             a.y = (o & 0xFF00ull) ? kUnreached : kEmptyKey;
@@ -409,33 +429,48 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
             b.y = (o & 0xFF0000000000ull) ? kUnreached : kEmptyKey;
             b.z = (o & 0xFF000000000000ull) ? kUnreached : kEmptyKey;
             b.w = (o & 0xFF00000000000000ull) ? kUnreached : kEmptyKey;
-            reinterpret_cast<uint4*>(J->keys + c0)[0] = a;
-            reinterpret_cast<uint4*>(J->keys + c0)[1] = b;
+            reinterpret_cast<uint4*>(J.keys + c0)[0] = a;
+            reinterpret_cast<uint4*>(J.keys + c0)[1] = b;
         } else {
             for (int k = 0; k < 8; ++k)
-                if (c0 + k < J->V) J->keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
+                if (c0 + k < J.V) J.keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
         }
-    }
+    });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
 {
-    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
-        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->algo != VF_FLOOD) continue;
-        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J->V) continue;
-        const uint32_t mask = (1u << J->key_bits) - 1u;
+    __shared__ JobDev sJ;
+    __shared__ uint16_t lab[kMaxDeviceSeeds];  // seed index -> 8-bit fragment label of the staged job
+    uint32_t staged = 0xFFFFFFFFu;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+        if (!J.active || J.algo != VF_FLOOD) return;
+        if (j != staged) {  // block-uniform
+            for (uint32_t i = threadIdx.x; i < J.S && i < kMaxDeviceSeeds; i += blockDim.x)
+                lab[i] = (uint16_t)(J.seeds[i].w & 0xFFu);  // unmaskRightMost(value, 8)
+            __syncthreads();
+            staged = j;
+        }
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
+        const uint32_t mask = (1u << J.key_bits) - 1u;
+        uint32_t key[8];
+        if (c0 + 8 <= J.V) {
+            const uint4 a = reinterpret_cast<const uint4*>(J.keys + c0)[0];
+            const uint4 b = reinterpret_cast<const uint4*>(J.keys + c0)[1];
+            key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
+            key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
+        } else {
+            for (int k = 0; k < 8; ++k) key[k] = c0 + k < J.V ? J.keys[c0 + k] : kEmptyKey;
+        }
         uint16_t out[8];
         for (int k = 0; k < 8; ++k) {
-            uint32_t key = c0 + k < J->V ? J->keys[c0 + k] : 0u;
-            if (key == kEmptyKey) out[k] = kEmpty;
-            else if (key == kUnreached) out[k] = kFree;
-            else out[k] = (uint16_t)(J->seeds[key & mask].w & 0xFFu);  // unmaskRightMost(value, 8)
+            if (key[k] == kEmptyKey) out[k] = kEmpty;
+            else if (key[k] == kUnreached) out[k] = kFree;
+            else out[k] = lab[key[k] & mask];
         }
-        store8(J->labels, c0, J->V, out);
-    }
+        store8(J.labels, c0, J.V, out);
+    });
 }
 
 }  // namespace
@@ -455,21 +490,17 @@ void launch_flood_seed(const Launch& L)
     VF_CHECK_LAUNCH();
 }
 
-uint32_t flood_list_cap(uint32_t total_chunks)
-{
-    const uint32_t groups = (total_chunks + kGroupChunks - 1) / kGroupChunks;
-    return (groups + kXcds - 1) / kXcds * kGroupChunks * 64u;
-}
+uint32_t flood_list_cap(uint32_t total_tasks) { return (total_tasks + kXcds - 1) / kXcds * 4096u; }
 
 void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks)
 {
     if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
-    if (W.list_cap < flood_list_cap(W.total_chunks)) fail(VF_ERR_INVALID_ARG, "flood worklists too small");
+    if (W.list_cap < flood_list_cap(W.total_tasks)) fail(VF_ERR_INVALID_ARG, "flood worklists too small");
     blocks = (blocks + kXcds - 1) / kXcds * kXcds;
     for (uint32_t i = 0; i < rounds; ++i) {
         {
             TimedLaunch tl("flood_compact", L.s);
-            k_flood_compact<<<blocks, 64, 0, L.s>>>(L, W.chunk_prefix, W.total_chunks, W.lists, W.list_cap, W.counts);
+            k_flood_compact<<<blocks, 64, 0, L.s>>>(L, W.task_prefix, W.total_tasks, W.lists, W.list_cap, W.counts);
             VF_CHECK_LAUNCH();
         }
         {

@@ -11,57 +11,52 @@ namespace {
 template <class F>
 __device__ inline void for_each_tile(const Launch& L, F&& f)
 {
-    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
-        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
-        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        f(J, c0);
-    }
+    __shared__ JobDev sJ;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+        if (!J.active) return;
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 < J.V) f(J, c0);
+    });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_occ_to_labels(Launch L)
 {
-    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
-        if (c0 >= J->V) return;
-        uint64_t o = load8b(J->occ, c0, J->V);
+    for_each_tile(L, [&](const JobDev& J, uint64_t c0) {
+        uint64_t o = load8b(J.occ, c0, J.V);
         uint16_t v[8];
         for (int k = 0; k < 8; ++k) v[k] = (uint16_t)((o >> (8 * k)) & 1u);
-        store8(J->labels, c0, J->V, v);
+        store8(J.labels, c0, J.V, v);
     });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_labels_to_occ(Launch L)
 {
-    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
-        if (c0 >= J->V) return;
+    for_each_tile(L, [&](const JobDev& J, uint64_t c0) {
         uint16_t v[8];
-        load8(J->labels, c0, J->V, v);
+        load8(J.labels, c0, J.V, v);
         uint64_t o = 0;
         for (int k = 0; k < 8; ++k) o |= (uint64_t)(v[k] != 0) << (8 * k);
-        store8b(J->occ, c0, J->V, o);
+        store8b(J.occ, c0, J.V, o);
     });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_homogenize(Launch L)
 {
-    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
-        if (c0 >= J->V) return;
+    for_each_tile(L, [&](const JobDev& J, uint64_t c0) {
         uint16_t v[8];
-        load8(J->labels, c0, J->V, v);
+        load8(J.labels, c0, J.V, v);
         for (int k = 0; k < 8; ++k) v[k] = v[k] != kEmpty ? kFree : kEmpty;
-        store8(J->labels, c0, J->V, v);
+        store8(J.labels, c0, J.V, v);
     });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_undo_mask(Launch L, uint16_t keep)
 {
-    for_each_tile(L, [&](const JobDev* J, uint64_t c0) {
-        if (c0 >= J->V) return;
+    for_each_tile(L, [&](const JobDev& J, uint64_t c0) {
         uint16_t v[8];
-        load8(J->labels, c0, J->V, v);
+        load8(J.labels, c0, J.V, v);
         for (int k = 0; k < 8; ++k) v[k] = (uint16_t)(v[k] & keep);
-        store8(J->labels, c0, J->V, v);
+        store8(J.labels, c0, J.V, v);
     });
 }
 
@@ -75,6 +70,7 @@ __global__ void k_clear_hist(Launch L)
 }
 
 constexpr uint32_t kLdsBins = 4096;
+constexpr uint32_t kTiledBins = 1024;  // LDS histogram bins of the tiled boundary kernel (larger labels go to HBM)
 
 __device__ inline uint32_t ceil_div_d(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
@@ -184,21 +180,16 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
 {
     __shared__ uint16_t raw[kBHX * kBHY * kBHZ];
     __shared__ uint16_t ax[kBHX * kBHY * kBTZ], bx[kBHX * kBHY * kBTZ];  // after the z pass
-    __shared__ uint16_t ay[kBHX * kBTY * kBTZ], by[kBHX * kBTY * kBTZ];  // after the y pass
-    __shared__ uint32_t bins[HIST ? kLdsBins : 1];
-    uint32_t tb, te;
-    block_tile_range(total, tb, te);
-    if (tb >= te) return;
-    uint32_t cur_job = find_job(bt_prefix, L.njobs, tb);
+    __shared__ uint32_t bins[HIST ? kTiledBins : 1];
+    __shared__ JobDev sJ;
     if (HIST) {
-        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) bins[i] = 0;
-        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kTiledBins; i += blockDim.x) bins[i] = 0;
     }
     auto flush = [&](uint32_t j) {
         if (!HIST) return;
         __syncthreads();
         uint32_t* h = L.jobs[j].hist;
-        for (uint32_t i = threadIdx.x; i < kLdsBins; i += blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < kTiledBins; i += blockDim.x) {
             const uint32_t c = bins[i];
             if (c) {
                 atomicAdd(h + i, c);
@@ -210,28 +201,23 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
     // this thread's 8 cells: row (x, y) = tid / 4, z segment (tid % 4) * 8
     const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 8;
     const int tx = row / kBTY, ty = row % kBTY;
-    for (uint32_t t = tb; t < te; ++t) {
-        const uint32_t j = find_job(bt_prefix, L.njobs, t);
-        if (j != cur_job) {
-            flush(cur_job);
-            cur_job = j;
-        }
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->bsize > 1) continue;
-        const int nx = (int)J->nx, ny = (int)J->ny, nz = (int)J->nz;
-        const uint32_t local = t - bt_prefix[j];
-        const uint32_t tz = ceil_div_d(J->nz, kBTZ), tyn = ceil_div_d(J->ny, kBTY);
+    walk_tiles(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+        if (!J.active || J.bsize > 1) return;
+        const int nx = (int)J.nx, ny = (int)J.ny, nz = (int)J.nz;
+        const uint32_t tz = ceil_div_d(J.nz, kBTZ), tyn = ceil_div_d(J.ny, kBTY);
         const int x0 = (int)(local / (tz * tyn)) * kBTX;
         const int y0 = (int)((local / tz) % tyn) * kBTY;
         const int z0 = (int)(local % tz) * kBTZ;
-        const bool bound = J->bsize == 1;
+        const bool bound = J.bsize == 1;
+        const uint16_t* lab = J.labels;
         // interior
         const int gx = x0 + tx, gy = y0 + ty, gz = z0 + seg;
         const bool row_ok = gx < nx && gy < ny;
         const int nval = row_ok ? max(0, min(8, nz - gz)) : 0;
         uint16_t v[8];
-        const uint16_t* src = J->labels + ((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz;
-        if (nval == 8 && (nz & 7) == 0) {
+        const uint16_t* src = lab + ((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz;
+        const bool vec = (nz & 7) == 0;
+        if (nval == 8 && vec) {
             const uint4 q = *reinterpret_cast<const uint4*>(src);
             v[0] = (uint16_t)q.x; v[1] = (uint16_t)(q.x >> 16); v[2] = (uint16_t)q.y; v[3] = (uint16_t)(q.y >> 16);
             v[4] = (uint16_t)q.z; v[5] = (uint16_t)(q.z >> 16); v[6] = (uint16_t)q.w; v[7] = (uint16_t)(q.w >> 16);
@@ -240,15 +226,15 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         }
         bool mine = false;
         for (int k = 0; k < 8; ++k) mine |= v[k] > kFree;
-        if (!__syncthreads_or(mine)) continue;
+        if (!__syncthreads_or(mine)) return;
         if (HIST) {
             // run-length over the 8 cells: one LDS atomic per run of equal labels
             uint32_t cur = 0, run = 0;
             for (int k = 0; k < 8; ++k) {
                 const uint32_t m = v[k] > kFree ? (uint32_t)(v[k] & kMask15) : 0xFFFFFFFFu;
                 if (m != cur && run) {
-                    if (cur < kLdsBins) atomicAdd(&bins[cur], run);
-                    else atomicAdd(J->hist + cur, run);
+                    if (cur < kTiledBins) atomicAdd(&bins[cur], run);
+                    else atomicAdd(J.hist + cur, run);
                     run = 0;
                 }
                 if (m != 0xFFFFFFFFu) {
@@ -257,17 +243,40 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
                 }
             }
             if (run) {
-                if (cur < kLdsBins) atomicAdd(&bins[cur], run);
-                else atomicAdd(J->hist + cur, run);
+                if (cur < kTiledBins) atomicAdd(&bins[cur], run);
+                else atomicAdd(J.hist + cur, run);
             }
         }
-        if (!bound) continue;
-        // whole box (interior + halo), clamped
-        for (int i = threadIdx.x; i < kBHX * kBHY * kBHZ; i += blockDim.x) {
-            const int hz = i % kBHZ, hy = (i / kBHZ) % kBHY, hx = i / (kBHZ * kBHY);
+        if (!bound) return;
+        auto clamped = [&](int hx, int hy, int hz) {
             const int cx = min(max(x0 + hx - 1, 0), nx - 1), cy = min(max(y0 + hy - 1, 0), ny - 1),
                       cz = min(max(z0 + hz - 1, 0), nz - 1);
-            raw[i] = J->labels[((size_t)cx * ny + cy) * nz + cz];
+            return lab[((size_t)cx * ny + cy) * nz + cz];
+        };
+        if (x0 + kBTX <= nx && y0 + kBTY <= ny && z0 + kBTZ <= nz) {
+            // interior from registers, then only the halo shell from HBM
+            uint16_t* dst = raw + ((tx + 1) * kBHY + ty + 1) * kBHZ + 1 + seg;
+            for (int k = 0; k < 8; ++k) dst[k] = v[k];
+            constexpr int kRing = 2 * kBHY + 2 * kBTX;  // rows with hx in {0, 9} or hy in {0, 9}
+            for (int i = threadIdx.x; i < kRing * kBHZ; i += blockDim.x) {
+                const int r = i / kBHZ, hz = i - r * kBHZ;
+                int hx, hy;
+                if (r < kBHY) { hx = 0; hy = r; }
+                else if (r < 2 * kBHY) { hx = kBHX - 1; hy = r - kBHY; }
+                else if (r < 2 * kBHY + kBTX) { hx = r - 2 * kBHY + 1; hy = 0; }
+                else { hx = r - 2 * kBHY - kBTX + 1; hy = kBHY - 1; }
+                raw[(hx * kBHY + hy) * kBHZ + hz] = clamped(hx, hy, hz);
+            }
+            for (int i = threadIdx.x; i < kBTX * kBTY * 2; i += blockDim.x) {
+                const int r = i >> 1, hx = r / kBTY + 1, hy = r % kBTY + 1, hz = (i & 1) ? kBHZ - 1 : 0;
+                raw[(hx * kBHY + hy) * kBHZ + hz] = clamped(hx, hy, hz);
+            }
+        } else {
+            // whole box (interior + halo), clamped
+            for (int i = threadIdx.x; i < kBHX * kBHY * kBHZ; i += blockDim.x) {
+                const int hz = i % kBHZ, hy = (i / kBHZ) % kBHY, hx = i / (kBHZ * kBHY);
+                raw[i] = clamped(hx, hy, hz);
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kBHX * kBHY * kBTZ; i += blockDim.x) {
@@ -275,45 +284,35 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             uint16_t mx = 0, mn = 0xFFFF;
             for (int dz = 0; dz < 3; ++dz) {
                 const uint16_t m = raw[r * kBHZ + z + dz] & kMask15;
-                const bool lab = m > kFree;
-                mx = max(mx, lab ? m : (uint16_t)0);
-                mn = min(mn, lab ? m : (uint16_t)0xFFFF);
+                const bool labd = m > kFree;
+                mx = max(mx, labd ? m : (uint16_t)0);
+                mn = min(mn, labd ? m : (uint16_t)0xFFFF);
             }
             ax[i] = mx;
             bx[i] = mn;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kBHX * kBTY * kBTZ; i += blockDim.x) {
-            const int z = i % kBTZ, y = (i / kBTZ) % kBTY, hx = i / (kBTZ * kBTY);
-            uint16_t mx = 0, mn = 0xFFFF;
-            for (int dy = 0; dy < 3; ++dy) {
-                const int k = (hx * kBHY + y + dy) * kBTZ + z;
-                mx = max(mx, ax[k]);
-                mn = min(mn, bx[k]);
-            }
-            ay[i] = mx;
-            by[i] = mn;
-        }
-        __syncthreads();
+        // x and y windows together, only for the labelled cells still without the bit
         bool changed = false;
         for (int k = 0; k < nval; ++k) {
             const uint16_t c = v[k];
             if (c <= kFree || (c & 0x8000u)) continue;
             const int z = seg + k;
             uint16_t mx = 0, mn = 0xFFFF;
-            for (int dx = 0; dx < 3; ++dx) {
-                const int q = ((tx + dx) * kBTY + ty) * kBTZ + z;
-                mx = max(mx, ay[q]);
-                mn = min(mn, by[q]);
-            }
+            for (int dx = 0; dx < 3; ++dx)
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int q = ((tx + dx) * kBHY + ty + dy) * kBTZ + z;
+                    mx = max(mx, ax[q]);
+                    mn = min(mn, bx[q]);
+                }
             if (mx != c || mn != c) {
                 v[k] = (uint16_t)(c | 0x8000u);
                 changed = true;
             }
         }
         if (changed) {
-            uint16_t* dst = J->labels + ((size_t)gx * ny + gy) * nz + gz;
-            if (nval == 8 && (nz & 7) == 0) {
+            uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
+            if (nval == 8 && vec) {
                 uint4 q;
                 q.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
                 q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
@@ -325,85 +324,60 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             }
         }
         __syncthreads();
-    }
-    flush(cur_job);
+    }, flush);
 }
 
-__global__ __launch_bounds__(kCellThreads) void k_checksum(Launch L)
-{
-    __shared__ unsigned long long part[kCellThreads / 64];
-    uint32_t tb, te;
-    block_tile_range(L.total_tiles, tb, te);
-    if (tb >= te) return;
-    uint32_t cur_job = find_job(L.tile_prefix, L.njobs, tb);
-    unsigned long long acc = 0;
-    auto flush = [&](uint32_t j) {
-        unsigned long long s = wave_sum_u64(acc);
+// Per-job sum of a per-thread value: wave sums, then one device atomic per block and job.
+struct BlockSum {
+    unsigned long long part[kCellThreads / 64];
+    __device__ void flush(unsigned long long& acc, unsigned long long* dst)
+    {
+        const unsigned long long s = wave_sum_u64(acc);
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned long long tot = 0;
             for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
-            atomicAdd(L.jobs[j].stats + kStatChecksum, tot);
+            if (tot) atomicAdd(dst, tot);
         }
         __syncthreads();
         acc = 0;
-    };
-    for (uint32_t t = tb; t < te; ++t) {
-        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        if (j != cur_job) {
-            flush(cur_job);
-            cur_job = j;
-        }
-        const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
-        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J->V) continue;
+    }
+};
+
+// Order-independent grid checksum: sum over the non-EMPTY cells c of
+// (hash32(value, c) << 32) | hash32(c, value).
+__global__ __launch_bounds__(kCellThreads) void k_checksum(Launch L)
+{
+    __shared__ BlockSum bs;
+    __shared__ JobDev sJ;
+    unsigned long long acc = 0;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+        if (!J.active) return;
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
         uint16_t v[8];
-        load8(J->labels, c0, J->V, v);
+        load8(J.labels, c0, J.V, v);
         for (int k = 0; k < 8; ++k) {
-            if (c0 + k >= J->V) break;
-            uint32_t c = (uint32_t)(c0 + k);
+            if (v[k] == 0) continue;  // EMPTY cells (and cells beyond V, loaded as 0) add nothing
+            const uint32_t c = (uint32_t)(c0 + k);
             acc += ((unsigned long long)hash32(v[k], c) << 32) | hash32(c, v[k]);
         }
-    }
-    flush(cur_job);
+    }, [&](uint32_t j) { bs.flush(acc, L.jobs[j].stats + kStatChecksum); });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_count_occupied(Launch L)
 {
-    __shared__ unsigned long long part[kCellThreads / 64];
-    uint32_t tb, te;
-    block_tile_range(L.total_tiles, tb, te);
-    if (tb >= te) return;
-    uint32_t cur_job = find_job(L.tile_prefix, L.njobs, tb);
+    __shared__ BlockSum bs;
+    __shared__ JobDev sJ;
     unsigned long long acc = 0;
-    auto flush = [&](uint32_t j) {
-        unsigned long long s = wave_sum_u64(acc);
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long tot = 0;
-            for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
-            if (tot) atomicAdd(L.jobs[j].stats + kStatOccupied, tot);
-        }
-        __syncthreads();
-        acc = 0;
-    };
-    for (uint32_t t = tb; t < te; ++t) {
-        uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        if (j != cur_job) {
-            flush(cur_job);
-            cur_job = j;
-        }
-        const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
-        uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J->V) continue;
-        uint64_t o = load8b(J->occ, c0, J->V);
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+        if (!J.active) return;
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
+        const uint64_t o = load8b(J.occ, c0, J.V);
         acc += __popcll(o & 0x0101010101010101ull);
-    }
-    flush(cur_job);
+    }, [&](uint32_t j) { bs.flush(acc, L.jobs[j].stats + kStatOccupied); });
 }
 
 // One block per job: number of labels >= 2 with a nonzero count.
@@ -428,9 +402,9 @@ __global__ __launch_bounds__(256) void k_job_init(Launch L)
     for (uint32_t i = threadIdx.x; i < 65536; i += blockDim.x) J->hist[i] = 0;
     if (threadIdx.x < kStatCount) J->stats[threadIdx.x] = 0;
     if (J->fctl && threadIdx.x < 64) J->fctl[threadIdx.x] = 0;
-    if (J->algo == VF_FLOOD && J->bricks) {  // both brick flag arrays of the flood rounds
-        const uint32_t n4 = (2u * J->nbricks + 3u) >> 2;
-        for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) reinterpret_cast<uint4*>(J->bricks)[i] = make_uint4(0, 0, 0, 0);
+    if (J->algo == VF_FLOOD && J->bflags) {  // both brick flag parities of the flood rounds
+        const uint32_t n = 2u * relax_words_per_parity(J->nbricks);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) J->bflags[i] = 0ull;
     }
 }
 

@@ -67,7 +67,7 @@ struct JobDev {
     int seed_location;
     uint64_t seed_key;
     uint32_t* keys;         // flood keys, V: 0 = EMPTY, ~0 = unreached, else (distance << key_bits) | seed
-    uint32_t* bricks;       // 2 x nbricks: brick flags for even / odd rounds
+    unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
     uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty
     unsigned long long* stats;  // kStatCount slots
@@ -166,24 +166,26 @@ inline uint32_t relax_bricks(uint32_t nx, uint32_t ny, uint32_t nz)
     return ((nx + kRelaxBrick - 1) / kRelaxBrick) * ((ny + kRelaxBrick - 1) / kRelaxBrick) *
            ((nz + kRelaxBrick - 1) / kRelaxBrick);
 }
-// Device words (uint32) of a job's two brick flag arrays (the clear writes whole uint4).
-inline uint64_t relax_words(uint32_t nx, uint32_t ny, uint32_t nz) { return 2ull * relax_bricks(nx, ny, nz) + 4u; }
-// 64-brick flag chunks of a job (a round scans one chunk per wave step)
-inline uint32_t relax_chunks(uint32_t nx, uint32_t ny, uint32_t nz) { return (relax_bricks(nx, ny, nz) + 63u) / 64u; }
-// Both brick flag arrays of every flood job must be zero before
+// 64-bit brick flag words per round parity, and compaction tasks (64 words) of a job
+__host__ __device__ inline uint32_t relax_words_per_parity(uint32_t nbricks) { return (nbricks + 63u) / 64u; }
+inline uint32_t relax_tasks(uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    return (relax_words_per_parity(relax_bricks(nx, ny, nz)) + 63u) / 64u;
+}
+// Both brick flag parities of every flood job must be zero before
 // launch_flood_seed (launch_job_init clears them).
 void launch_flood_init(const Launch& L);
 void launch_flood_seed(const Launch& L);
 // Device state of the flood rounds of one launch table.
 struct FloodRoundScratch {
-    const uint32_t* chunk_prefix;  // njobs + 1: prefix of relax_chunks over the jobs (0 for non-flood jobs)
-    uint32_t total_chunks;
+    const uint32_t* task_prefix;   // njobs + 1: prefix of relax_tasks over the jobs (0 for non-flood jobs)
+    uint32_t total_tasks;
     unsigned long long* lists;     // per-XCD brick worklists, list_cap entries each
-    uint32_t list_cap;             // >= flood_list_cap(total_chunks)
+    uint32_t list_cap;             // >= flood_list_cap(total_tasks)
     uint32_t* counts;              // per-XCD list lengths; zero before the first round
 };
 constexpr uint32_t kFloodXcds = 8;
-uint32_t flood_list_cap(uint32_t total_chunks);  // entries per XCD list
+uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
 // `rounds` relaxation rounds for every running flood job of L.  Each job
 // continues at its own round, so jobs may join a table between calls; rounds
 // after a job drained do nothing for it, so the launches can be replayed from

@@ -80,18 +80,33 @@ template <typename K>
 __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
 {
     __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
+    __shared__ uint16_t sw[kSeedChunk];
+    __shared__ JobDev sJ;
+    __shared__ uint32_t sp[kWalkMaxJobs + 1];
     constexpr int kWant = sizeof(K) == 8 ? 1 : 0;
-    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
-        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->algo != VF_NAIVE || J->naive_key64 != kWant || J->S == 0) continue;
-        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        const bool live = c0 < J->V;
-        const uint64_t occ = live ? load8b(J->occ, c0, J->V) : 0ull;
-        const uint32_t b = J->key_bits;
-        const uint32_t S = J->S;
-        const int dist = J->dist;
-        const bool fast = (J->nz & 7u) == 0;
+    uint32_t staged = 0xFFFFFFFFu;  // job whose seeds (one chunk) sit in LDS
+    auto stage = [&](const JobDev& J, uint32_t base, uint32_t n) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint4 q = J.seeds[base + i];
+            sx[i] = q.x; sy[i] = q.y; sz[i] = q.z; sw[i] = (uint16_t)q.w;
+        }
+        __syncthreads();
+    };
+    walk_tiles_strided(L, L.tile_prefix, L.total_tiles, &sJ, sp, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+        if (!J.active || J.algo != VF_NAIVE || J.naive_key64 != kWant || J.S == 0) return;
+        const uint32_t S = J.S;
+        const bool one_chunk = S <= kSeedChunk;
+        if (one_chunk && staged != j) {  // block-uniform
+            stage(J, 0, S);
+            staged = j;
+        }
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        const bool live = c0 < J.V;
+        const uint64_t occ = live ? load8b(J.occ, c0, J.V) : 0ull;
+        const uint32_t b = J.key_bits;
+        const int dist = J.dist;
+        const bool fast = (J.nz & 7u) == 0;
         uint32_t x = 0, y = 0, z = 0;
         if (live) decompose(c0, J, x, y, z);
         uint32_t cx[8], cy[8], cz[8];
@@ -107,12 +122,10 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         const bool work = occ != 0;
         for (uint32_t base = 0; base < S; base += kSeedChunk) {
             const uint32_t n = min(kSeedChunk, S - base);
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint4 s = J->seeds[base + i];
-                sx[i] = s.x; sy[i] = s.y; sz[i] = s.z;
+            if (!one_chunk) {
+                stage(J, base, n);
+                staged = 0xFFFFFFFFu;
             }
-            __syncthreads();
             if (!work) continue;
             if (fast) {
                 if (dist == VF_MANHATTAN) naive_chunk<VF_MANHATTAN, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
@@ -124,15 +137,16 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
                 else naive_chunk<VF_EUCLIDEAN, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
             }
         }
-        if (!live) continue;
+        if (!live) return;
         uint16_t out[8];
         const K mask = ((K)1 << b) - 1;
         for (int k = 0; k < 8; ++k) {
             const bool o = ((occ >> (8 * k)) & 0xFFu) != 0;
-            out[k] = o ? (uint16_t)J->seeds[(uint32_t)(best[k] & mask)].w : (uint16_t)0;
+            const uint32_t id = (uint32_t)(best[k] & mask);
+            out[k] = o ? (one_chunk ? sw[id] : (uint16_t)J.seeds[id].w) : (uint16_t)0;
         }
-        store8(J->labels, c0, J->V, out);
-    }
+        store8(J.labels, c0, J.V, out);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -196,6 +210,7 @@ __global__ void k_ri_reset(uint32_t* counters, uint32_t idx) { counters[idx] = 0
 
 void launch_naive(const Launch& L, bool key64)
 {
+    if (L.njobs > kWalkMaxJobs) fail(VF_ERR_INVALID_ARG, "naive launch with %u jobs (max %u)", L.njobs, kWalkMaxJobs);
     const uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
     TimedLaunch tl(key64 ? "naive64" : "naive", L.s);
     if (key64) k_naive<unsigned long long><<<blocks, kCellThreads, 0, L.s>>>(L);

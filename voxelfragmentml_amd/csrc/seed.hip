@@ -39,20 +39,20 @@ __device__ inline bool candidate(const JobDev* J, uint64_t c, uint32_t x, uint32
 __global__ __launch_bounds__(kCellThreads) void k_seed_count(Launch L, uint32_t* tile_cnt)
 {
     __shared__ uint32_t wsum[kCellThreads / 64];
-    for (uint32_t t = blockIdx.x; t < L.total_tiles; t += gridDim.x) {
-        const uint32_t j = find_job(L.tile_prefix, L.njobs, t);
-        const JobDev* J = L.jobs + j;
+    __shared__ JobDev sJ;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t t) {
         uint32_t n = 0;
-        const uint64_t c0 = (uint64_t)(t - L.tile_prefix[j]) * kTileCells + threadIdx.x * kCellsPerThread;
-        if (J->active && c0 < J->V) {
-            const uint64_t o = load8b(J->occ, c0, J->V);
-            if (J->seed_location == VF_SEED_BOTH) {
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (J.active && c0 < J.V) {
+            const uint64_t o = load8b(J.occ, c0, J.V);
+            if (J.seed_location == VF_SEED_BOTH) {
                 n = __popcll(o & 0x0101010101010101ull);
             } else if (o) {
+                const JobDev* Jg = L.jobs + j;
                 uint32_t x, y, z;
                 decompose(c0, J, x, y, z);
-                for (int k = 0; k < 8 && c0 + k < J->V; ++k) {
-                    if ((o >> (8 * k)) & 0xFFu) n += candidate(J, c0 + k, x, y, z) ? 1u : 0u;
+                for (int k = 0; k < 8 && c0 + k < J.V; ++k) {
+                    if ((o >> (8 * k)) & 0xFFu) n += candidate(Jg, c0 + k, x, y, z) ? 1u : 0u;
                     advance(x, y, z, J);
                 }
             }
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kCellThreads) void k_seed_count(Launch L, uint32_t*
             tile_cnt[t] = tot;
         }
         __syncthreads();
-    }
+    });
 }
 
 // One block per job.
