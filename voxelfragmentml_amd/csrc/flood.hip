@@ -36,7 +36,9 @@ constexpr uint32_t kUnreached = 0xFFFFFFFFu;
 // tile values are key - 1: EMPTY becomes 0xFFFFFFFF and unreached 0xFFFFFFFE,
 // so a plain unsigned min over neighbours never selects either
 constexpr uint32_t kTileNone = 0xFFFFFFFEu;
-constexpr uint32_t kRowWords = 12;                 // halo z row: pad, z = -1..8, pad (48 B, 16 B aligned)
+// Halo z rows of 10 words (z = -1..8) at an odd stride, so that the row starts
+// of a wave's 64 lanes fall on distinct LDS banks.
+constexpr uint32_t kRowWords = 11;
 constexpr uint32_t kTileWords = 10 * 10 * kRowWords;
 constexpr uint32_t kMaxSweeps = 2048;              // a brick of 512 cells is stable after at most 513 sweeps
 constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgroups are dealt round-robin
@@ -132,7 +134,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         const uint32_t gx = (uint32_t)X0 + hx - 1u, gy = (uint32_t)Y0 + hy - 1u, gz = (uint32_t)Z0 + hz - 1u;
         uint32_t v = kEmptyKey;
         if (i < 1000u && gx < nx && gy < ny && gz < nz) v = kb[(hx * ny + hy) * nz + hz];
-        if (i < 1000u) tile[(hx * 10u + hy) * kRowWords + hz + 1u] = v - 1u;
+        if (i < 1000u) tile[(hx * 10u + hy) * kRowWords + hz] = v - 1u;
     }
     }
     __syncthreads();
@@ -144,7 +146,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
     uint32_t k[8], w[8];
 #pragma unroll
-    for (int z = 0; z < 8; ++z) k[z] = tile[own + 2 + z];
+    for (int z = 0; z < 8; ++z) k[z] = tile[own + 1 + z];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
         const bool occupied = k[z] != 0xFFFFFFFFu;  // out-of-grid cells load as EMPTY
@@ -165,7 +167,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy) {
-                    const uint32_t* row = tile + ((x + dx) * 10u + (y + dy)) * kRowWords + 1u;
+                    const uint32_t* row = tile + ((x + dx) * 10u + (y + dy)) * kRowWords;
 #pragma unroll
                     for (int o = 0; o < 10; ++o) cm[o] = min(cm[o], row[o]);
                     __builtin_amdgcn_sched_barrier(0);  // keep one row of LDS values live at a time
@@ -176,7 +178,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         } else {
             uint32_t cc[10], fm[8];
             {
-                const uint32_t* row = tile + own + 1u;
+                const uint32_t* row = tile + own;
 #pragma unroll
                 for (int o = 0; o < 10; ++o) cc[o] = row[o];
             }
@@ -186,7 +188,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             for (int f = 0; f < 4; ++f) {
                 const uint32_t fx = f == 0 ? x : (f == 1 ? x + 2u : x + 1u);
                 const uint32_t fy = f < 2 ? y + 1u : (f == 2 ? y : y + 2u);
-                const uint32_t* row = tile + (fx * 10u + fy) * kRowWords + 2u;
+                const uint32_t* row = tile + (fx * 10u + fy) * kRowWords + 1u;
 #pragma unroll
                 for (int o = 0; o < 8; ++o) fm[o] = min(fm[o], row[o]);
             }
@@ -219,7 +221,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         }
         if (ch) {
 #pragma unroll
-            for (int z = 0; z < 8; ++z) tile[own + 2 + z] = k[z];
+            for (int z = 0; z < 8; ++z) tile[own + 1 + z] = k[z];
         }
         changed |= ch;
         ++sweeps;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
 __global__ __launch_bounds__(64) void k_flood_relax(Launch L, const unsigned long long* lists, uint32_t list_cap,
                                                     const uint32_t* counts)
 {
-    __shared__ __align__(16) uint32_t tile[kTileWords];
+    __shared__ uint32_t tile[kTileWords];
     const uint32_t lane = threadIdx.x;
     const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, local = blockIdx.x / kXcds;
     if (local >= per_xcd) return;

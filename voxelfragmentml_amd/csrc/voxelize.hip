@@ -25,8 +25,8 @@ constexpr int kSub = 256;
 constexpr float kSnapLim = 524288.0f;
 constexpr uint32_t kVoxThreads = 256;
 constexpr float kRangeMargin = 0.015625f;   // 1/64 cell
-constexpr uint32_t kInlineShell = 64;       // SAT candidates a setup thread tests itself
-constexpr uint32_t kInlineCols = 32;        // parity columns a setup thread toggles itself
+constexpr uint32_t kInlineShell = 8;        // SAT candidates a setup thread tests itself
+constexpr uint32_t kInlineCols = 8;         // parity columns a setup thread toggles itself
 
 __device__ inline bool axis_sep(float pa, float pb, float rad)
 {
@@ -156,14 +156,17 @@ __device__ inline void parity_column(const JobDev* J, const TriSetup& S, int i, 
 __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, uint32_t njobs, const uint32_t* tri_prefix,
                                                            uint32_t total, TriSetup* setup, uint32_t* cnt_shell, uint32_t* cnt_col)
 {
+    __shared__ uint32_t sp[kWalkMaxJobs + 1];
+    for (uint32_t i = threadIdx.x; i <= njobs; i += blockDim.x) sp[i] = tri_prefix[i];
+    __syncthreads();
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        const uint32_t j = find_job(tri_prefix, njobs, g);
+        const uint32_t j = find_job(sp, njobs, g);
         const JobDev* J = jobs + j;
         TriSetup S;
         S.job = j;
         S.area = 0;
         uint32_t nshell = 0, ncol = 0;
-        const uint32_t t = g - tri_prefix[j];
+        const uint32_t t = g - sp[j];
         if (J->active) {
             const float* p[3] = { J->xyz + 3 * (size_t)J->tri[3 * (size_t)t], J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 1],
                                   J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 2] };
@@ -238,7 +241,6 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                     for (int z = S.lo[2]; z <= S.hi[2]; ++z) {
                         const size_t word = ((size_t)x * J->ny + (size_t)y) * J->wp + (size_t)(z >> 5);
                         const uint32_t bit = 1u << (z & 31);
-                        if (J->shell_bits[word] & bit) continue;
                         float c[3], h[3];
                         cell_box(x, J->mn[0], J->cs[0], &c[0], &h[0]);
                         cell_box(y, J->mn[1], J->cs[1], &c[1], &h[1]);
@@ -258,45 +260,128 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
     }
 }
 
+// Largest g in [0, n) with off[g] <= p (off[0] == 0 <= p), searched by the
+// whole wave with 64 probes per step: four dependent loads cover 16M entries.
+__device__ inline uint32_t wave_search(const unsigned long long* off, uint32_t n, unsigned long long p)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 64u) {
+        const uint32_t step = (hi - lo + 63u) / 64u;
+        const uint32_t idx = lo + lane * step;
+        const bool ok = idx < hi && off[idx] <= p;
+        const uint64_t m = __ballot(ok);
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
+        lo = lo + last * step;
+        hi = min(lo + step, hi);
+    }
+    const uint32_t idx = lo + lane;
+    const bool ok = idx < hi && off[idx] <= p;
+    const uint64_t m = __ballot(ok);
+    return lo + (63u - (uint32_t)__clzll((long long)m));
+}
+
+// Pair kernels: a block takes chunks of kPairChunk consecutive pairs.  Two
+// waves locate the chunk's first and last triangle, the block caches their
+// offsets in LDS and each thread finds its triangle there.
+constexpr uint32_t kPairChunk = 4 * kVoxThreads;
+constexpr uint32_t kOffCache = 1024;
+
+template <class F>
+__device__ inline void for_each_pair(const unsigned long long* off, uint32_t ntris, F&& f)
+{
+    __shared__ unsigned long long soff[kOffCache + 1];
+    __shared__ uint32_t sg[2];
+    const unsigned long long total = off[ntris];
+    const uint32_t wave = threadIdx.x >> 6;
+    for (unsigned long long p0 = (unsigned long long)blockIdx.x * kPairChunk; p0 < total;
+         p0 += (unsigned long long)gridDim.x * kPairChunk) {
+        const unsigned long long p1 = min(p0 + kPairChunk, total);
+        if (wave < 2) {
+            const uint32_t g = wave_search(off, ntris + 1, wave == 0 ? p0 : p1 - 1);
+            if ((threadIdx.x & 63u) == 0) sg[wave] = g;
+        }
+        __syncthreads();
+        const uint32_t g0 = sg[0], n = sg[1] - g0 + 1;  // triangles g0 .. g0 + n - 1 hold the chunk
+        const bool cached = n <= kOffCache;
+        if (cached)
+            for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) soff[i] = off[g0 + i];
+        __syncthreads();
+        // every lane of a wave calls f in each step (valid == false past the chunk),
+        // so f may combine its result across the wave
+        for (unsigned long long base = p0; base < p1; base += blockDim.x) {
+            const unsigned long long p = base + threadIdx.x;
+            const bool valid = p < p1;
+            uint32_t lo = 0, hi = n;  // largest i with off[g0 + i] <= p
+            while (valid && hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((cached ? soff[mid] : off[g0 + mid]) <= p) lo = mid; else hi = mid;
+            }
+            const uint32_t g = g0 + lo;
+            f(g, valid ? p - (cached ? soff[lo] : off[g]) : 0ull, valid);
+        }
+        __syncthreads();
+    }
+}
+
+// Wave-combined bit update: lanes holding the same target word merge their
+// bits (OR, or XOR for toggles) and the first lane of each run of equal
+// words issues one device atomic.  All 64 lanes must call it.
+template <bool XOR>
+__device__ inline void wave_bits(uint32_t* word, uint32_t bits)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t key = reinterpret_cast<uint64_t>(word);
+    uint32_t acc = bits;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t other = __shfl_down(acc, o, 64);
+        const uint64_t okey = __shfl_down(key, o, 64);
+        if (lane + (uint32_t)o < 64u && okey == key) acc = XOR ? (acc ^ other) : (acc | other);
+    }
+    const uint64_t pkey = __shfl_up(key, 1, 64);
+    const bool leader = lane == 0 || pkey != key;
+    if (leader && acc && word) {
+        if (XOR) atomicXor(word, acc);
+        else atomicOr(word, acc);
+    }
+}
+
 __global__ __launch_bounds__(kVoxThreads) void k_vox_shell(const JobDev* jobs, const TriSetup* setup,
                                                            const unsigned long long* off, uint32_t ntris)
 {
-    const unsigned long long total = off[ntris];
-    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < total;
-         p += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t g = (uint32_t)find_job64(off, ntris + 1, p);
-        const TriSetup* S = setup + g;
-        const JobDev* J = jobs + S->job;
-        const uint64_t local = p - off[g];
-        const uint32_t ez = (uint32_t)(S->hi[2] - S->lo[2] + 1), ey = (uint32_t)(S->hi[1] - S->lo[1] + 1);
-        const uint32_t z = (uint32_t)S->lo[2] + (uint32_t)(local % ez);
-        const uint64_t rest = local / ez;
-        const uint32_t y = (uint32_t)S->lo[1] + (uint32_t)(rest % ey);
-        const uint32_t x = (uint32_t)S->lo[0] + (uint32_t)(rest / ey);
-        const size_t word = ((size_t)x * J->ny + y) * J->wp + (z >> 5);
-        const uint32_t bit = 1u << (z & 31);
-        if (J->shell_bits[word] & bit) continue;  // already marked by another triangle
-        float c[3], h[3];
-        cell_box((int)x, J->mn[0], J->cs[0], &c[0], &h[0]);
-        cell_box((int)y, J->mn[1], J->cs[1], &c[1], &h[1]);
-        cell_box((int)z, J->mn[2], J->cs[2], &c[2], &h[2]);
-        if (sat_triangle_box(S->P[0], S->P[1], S->P[2], c, h)) atomicOr(J->shell_bits + word, bit);
-    }
+    for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
+        uint32_t* word = nullptr;
+        uint32_t bit = 0;
+        if (valid) {
+            const TriSetup* S = setup + g;
+            const JobDev* J = jobs + S->job;
+            const uint32_t ez = (uint32_t)(S->hi[2] - S->lo[2] + 1), ey = (uint32_t)(S->hi[1] - S->lo[1] + 1);
+            const uint32_t z = (uint32_t)S->lo[2] + (uint32_t)(local % ez);
+            const uint64_t rest = local / ez;
+            const uint32_t y = (uint32_t)S->lo[1] + (uint32_t)(rest % ey);
+            const uint32_t x = (uint32_t)S->lo[0] + (uint32_t)(rest / ey);
+            float c[3], h[3];
+            cell_box((int)x, J->mn[0], J->cs[0], &c[0], &h[0]);
+            cell_box((int)y, J->mn[1], J->cs[1], &c[1], &h[1]);
+            cell_box((int)z, J->mn[2], J->cs[2], &c[2], &h[2]);
+            word = J->shell_bits + ((size_t)x * J->ny + y) * J->wp + (z >> 5);
+            if (sat_triangle_box(S->P[0], S->P[1], S->P[2], c, h)) bit = 1u << (z & 31);
+        }
+        wave_bits<false>(word, bit);
+    });
 }
 
 __global__ __launch_bounds__(kVoxThreads) void k_vox_parity(const JobDev* jobs, const TriSetup* setup,
                                                             const unsigned long long* off, uint32_t ntris)
 {
-    const unsigned long long total = off[ntris];
-    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < total;
-         p += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t g = (uint32_t)find_job64(off, ntris + 1, p);
+    for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
+        if (!valid) return;
         const TriSetup* S = setup + g;
         const JobDev* J = jobs + S->job;
-        const uint64_t local = p - off[g];
         const uint32_t nj = (uint32_t)(S->j1 - S->j0 + 1);
         parity_column(J, *S, S->i0 + (int)(local / nj), S->j0 + (int)(local % nj));
-    }
+    });
 }
 
 // 4 bits -> 4 bytes of 0/1 (bit i -> byte i)
@@ -304,67 +389,77 @@ __device__ inline uint32_t spread4(uint32_t b) { return ((b & 0xFu) * 0x00204081
 
 __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
 {
-    for (uint32_t blk = blockIdx.x; blk < total_groups; blk += gridDim.x) {
-        const uint32_t j = find_job(col_prefix, L.njobs, blk);
-        const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
-        const uint64_t nwords = (uint64_t)J->nx * J->ny * J->wp;
-        const uint64_t w = (uint64_t)(blk - col_prefix[j]) * kVoxThreads + threadIdx.x;
+    __shared__ JobDev sJ;
+    __shared__ unsigned long long part[kVoxThreads / 64];
+    unsigned long long acc = 0;
+    walk_tiles(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t group, uint32_t) {
+        if (!J.active) return;
+        const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
+        const uint64_t w = (uint64_t)group * kVoxThreads + threadIdx.x;
         const bool valid = w < nwords;
-        const uint32_t tg = valid ? J->toggle_bits[w] : 0u;
-        const uint32_t sh = valid ? J->shell_bits[w] : 0u;
+        const uint32_t tg = valid ? J.toggle_bits[w] : 0u;
+        const uint32_t sh = valid ? J.shell_bits[w] : 0u;
         uint32_t p = tg;
         p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8; p ^= p << 16;
         const uint64_t par = __ballot(__popc(tg) & 1);
         const uint32_t lane = threadIdx.x & 63;
-        const uint32_t seg = lane & ~(J->wp - 1);
+        const uint32_t seg = lane & ~(J.wp - 1);
         const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull) & ~(seg ? (~0ull >> (64 - seg)) : 0ull);
         if (__popcll(par & below) & 1) p = ~p;
         uint32_t bits = 0;
         uint32_t nb = 0;
         uint64_t base = 0;
         if (valid) {
-            const uint64_t col = w / J->wp;
-            const uint32_t z0 = (uint32_t)(w % J->wp) * 32u;
-            if (z0 < J->nz) {
-                nb = min(32u, J->nz - z0);
+            const uint64_t col = w / J.wp;
+            const uint32_t z0 = (uint32_t)(w % J.wp) * 32u;
+            if (z0 < J.nz) {
+                nb = min(32u, J.nz - z0);
                 bits = (sh | p) & (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u));
-                base = col * J->nz + z0;
+                base = col * J.nz + z0;
             }
         }
         if (nb) {
-            uint8_t* o = J->occ + base;
-            if ((J->nz & 15u) == 0 && nb == 32) {
+            uint8_t* o = J.occ + base;
+            if ((J.nz & 15u) == 0 && nb == 32) {
                 uint4 a, b;
                 a.x = spread4(bits); a.y = spread4(bits >> 4); a.z = spread4(bits >> 8); a.w = spread4(bits >> 12);
                 b.x = spread4(bits >> 16); b.y = spread4(bits >> 20); b.z = spread4(bits >> 24); b.w = spread4(bits >> 28);
                 reinterpret_cast<uint4*>(o)[0] = a;
                 reinterpret_cast<uint4*>(o)[1] = b;
-            } else if ((J->nz & 3u) == 0) {
+            } else if ((J.nz & 3u) == 0) {
                 for (uint32_t q = 0; q < nb / 4; ++q) reinterpret_cast<uint32_t*>(o)[q] = spread4(bits >> (4 * q));
             } else {
                 for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
             }
         }
-        unsigned long long cnt = wave_sum_u64((unsigned long long)__popc(bits));
-        if (lane == 0 && cnt) atomicAdd(J->stats + kStatOccupied, cnt);
-    }
+        acc += (unsigned long long)__popc(bits);
+    }, [&](uint32_t j) {
+        const unsigned long long s = wave_sum_u64(acc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (uint32_t w = 0; w < kVoxThreads / 64; ++w) tot += part[w];
+            if (tot) atomicAdd(L.jobs[j].stats + kStatOccupied, tot);
+        }
+        __syncthreads();
+        acc = 0;
+    });
 }
 
 __global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
 {
-    for (uint32_t blk = blockIdx.x; blk < total_groups; blk += gridDim.x) {
-        const uint32_t j = find_job(col_prefix, L.njobs, blk);
-        const JobDev* J = L.jobs + j;
-        if (!J->active) continue;
-        const uint64_t nwords = (uint64_t)J->nx * J->ny * J->wp;
-        const uint64_t w = (uint64_t)(blk - col_prefix[j]) * kVoxThreads + threadIdx.x;
+    __shared__ JobDev sJ;
+    walk_tiles(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t group, uint32_t) {
+        if (!J.active) return;
+        const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
+        const uint64_t w = (uint64_t)group * kVoxThreads + threadIdx.x;
         if (w < nwords) {
-            J->toggle_bits[w] = 0u;
-            J->shell_bits[w] = 0u;
+            J.toggle_bits[w] = 0u;
+            J.shell_bits[w] = 0u;
         }
-        if (w == 0) J->stats[kStatOccupied] = 0;
-    }
+        if (w == 0) J.stats[kStatOccupied] = 0;
+    });
 }
 
 }  // namespace
@@ -382,6 +477,7 @@ void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t tot
 // X.setup / X.counts / X.shell_off / X.col_off / X.scan_tmp sized for X.total_tris.
 void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_groups)
 {
+    if (L.njobs > kWalkMaxJobs) fail(VF_ERR_INVALID_ARG, "voxelize launch with %u jobs (max %u)", L.njobs, kWalkMaxJobs);
     launch_clear_bits(L, col_prefix, total_groups);
     if (X.total_tris) {
         uint32_t* cnt_shell = X.counts;
@@ -396,12 +492,12 @@ void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_p
         scan_exclusive_u32_to_u64(cnt_col, X.col_off, X.total_tris, X.scan_tmp, L.s);
         {
             TimedLaunch tl("vox_shell", L.s);
-            k_vox_shell<<<4096, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
+            k_vox_shell<<<8192, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
             VF_CHECK_LAUNCH();
         }
         {
             TimedLaunch tl("vox_parity", L.s);
-            k_vox_parity<<<2048, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off, X.total_tris);
+            k_vox_parity<<<4096, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off, X.total_tris);
             VF_CHECK_LAUNCH();
         }
     }
