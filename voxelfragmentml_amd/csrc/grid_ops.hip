@@ -174,12 +174,13 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
 // load the halo and run separable max/min passes z, y, x in LDS.
 constexpr int kBTX = 8, kBTY = 8, kBTZ = 32;
 constexpr int kBHX = kBTX + 2, kBHY = kBTY + 2, kBHZ = kBTZ + 2;
+constexpr int kAZ = kBTZ + 2;  // z-pass row stride (u16): 17 words, odd, so rows spread over the LDS banks
 
 template <bool HIST>
 __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
 {
     __shared__ uint16_t raw[kBHX * kBHY * kBHZ];
-    __shared__ uint16_t ax[kBHX * kBHY * kBTZ], bx[kBHX * kBHY * kBTZ];  // after the z pass
+    __shared__ uint16_t ax[kBHX * kBHY * kAZ], bx[kBHX * kBHY * kAZ];  // after the z pass
     __shared__ uint32_t bins[HIST ? kTiledBins : 1];
     __shared__ JobDev sJ;
     if (HIST) {
@@ -288,8 +289,8 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
                 mx = max(mx, labd ? m : (uint16_t)0);
                 mn = min(mn, labd ? m : (uint16_t)0xFFFF);
             }
-            ax[i] = mx;
-            bx[i] = mn;
+            ax[r * kAZ + z] = mx;
+            bx[r * kAZ + z] = mn;
         }
         __syncthreads();
         // x and y windows together, only for the labelled cells still without the bit
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             uint16_t mx = 0, mn = 0xFFFF;
             for (int dx = 0; dx < 3; ++dx)
                 for (int dy = 0; dy < 3; ++dy) {
-                    const int q = ((tx + dx) * kBHY + ty + dy) * kBTZ + z;
+                    const int q = ((tx + dx) * kBHY + ty + dy) * kAZ + z;
                     mx = max(mx, ax[q]);
                     mn = min(mn, bx[q]);
                 }

@@ -76,18 +76,58 @@ __device__ inline void naive_chunk(K best[8], uint32_t x, uint32_t y, uint32_t z
     }
 }
 
+// Distance key of one cell to one seed.
+template <int DIST, typename K>
+__device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, uint32_t sy, uint32_t sz, uint32_t id,
+                             uint32_t b)
+{
+    const uint32_t ax = __usad(x, sx, 0u), ay = __usad(y, sy, 0u), az = __usad(z, sz, 0u);
+    uint32_t d;
+    if (DIST == VF_MANHATTAN) d = ax + ay + az;
+    else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
+    else d = ax * ax + ay * ay + az * az;
+    return pack_key<DIST, K>(d, id, b);
+}
+
+// argmin over seeds [base, base + n) (staged in LDS) for a thread's nq <= 8
+// compacted cells
+template <int DIST, typename K>
+__device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8], uint32_t nq,
+                                   const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, uint32_t base, uint32_t n,
+                                   uint32_t b)
+{
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) {
+        if (q >= nq) break;
+        K m = best[q];
+        for (uint32_t i = 0; i < n; ++i) {
+            const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], sx[i], sy[i], sz[i], base + i, b);
+            m = key < m ? key : m;
+        }
+        best[q] = m;
+    }
+}
+
+// One tile per step: the tile's occupied cells are compacted in LDS, so the
+// seed loop runs over occupied cells only (shells occupy a few percent of
+// their box); labels return through LDS and leave as one 16-byte store per
+// thread.
 template <typename K>
 __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
 {
     __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
     __shared__ uint16_t sw[kSeedChunk];
+    __shared__ uint16_t cells[kTileCells];  // compacted offsets of the occupied cells
+    __shared__ uint16_t lab[kTileCells];    // the tile's output labels
+    __shared__ uint32_t wsum[kCellThreads / 64];
     __shared__ JobDev sJ;
     __shared__ uint32_t sp[kWalkMaxJobs + 1];
     constexpr int kWant = sizeof(K) == 8 ? 1 : 0;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint32_t staged = 0xFFFFFFFFu;  // job whose seeds (one chunk) sit in LDS
     auto stage = [&](const JobDev& J, uint32_t base, uint32_t n) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        for (uint32_t i = tid; i < n; i += blockDim.x) {
             const uint4 q = J.seeds[base + i];
             sx[i] = q.x; sy[i] = q.y; sz[i] = q.z; sw[i] = (uint16_t)q.w;
         }
@@ -101,51 +141,66 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
             stage(J, 0, S);
             staged = j;
         }
-        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
-        const bool live = c0 < J.V;
-        const uint64_t occ = live ? load8b(J.occ, c0, J.V) : 0ull;
-        const uint32_t b = J.key_bits;
-        const int dist = J.dist;
-        const bool fast = (J.nz & 7u) == 0;
-        uint32_t x = 0, y = 0, z = 0;
-        if (live) decompose(c0, J, x, y, z);
-        uint32_t cx[8], cy[8], cz[8];
-        {
-            uint32_t a = x, bb = y, c = z;
-            for (int k = 0; k < 8; ++k) {
-                cx[k] = a; cy[k] = bb; cz[k] = c;
-                advance(a, bb, c, J);
+        const uint64_t t0 = (uint64_t)local * kTileCells;
+        const uint64_t c0 = t0 + tid * kCellsPerThread;
+        const uint64_t occ = c0 < J.V ? load8b(J.occ, c0, J.V) : 0ull;
+        // block-wide exclusive scan of the occupied counts -> compacted positions
+        const uint32_t n = (uint32_t)__popcll(occ & 0x0101010101010101ull);
+        uint32_t inc = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += t;
+        }
+        if (lane == 63u) wsum[wave] = inc;
+        for (int k = 0; k < 8; ++k) lab[tid * 8 + k] = 0;  // EMPTY unless written below
+        __syncthreads();
+        uint32_t pos = inc - n, total = 0;
+        for (uint32_t w = 0; w < kCellThreads / 64; ++w) {
+            if (w < wave) pos += wsum[w];
+            total += wsum[w];
+        }
+        for (int k = 0; k < 8; ++k)
+            if ((occ >> (8 * k)) & 0xFFu) cells[pos++] = (uint16_t)(tid * 8 + k);
+        __syncthreads();
+        if (total) {
+            const uint32_t nq = tid < total ? (total - tid + kCellThreads - 1) / kCellThreads : 0u;
+            uint32_t cx[8], cy[8], cz[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                cx[q] = cy[q] = cz[q] = 0;
+                if (q < nq) decompose(t0 + cells[tid + q * kCellThreads], J, cx[q], cy[q], cz[q]);
+            }
+            K best[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) best[q] = ~(K)0;
+            const uint32_t b = J.key_bits;
+            const int dist = J.dist;
+            for (uint32_t base = 0; base < S; base += kSeedChunk) {
+                const uint32_t m = min(kSeedChunk, S - base);
+                if (!one_chunk) {
+                    stage(J, base, m);
+                    staged = 0xFFFFFFFFu;
+                }
+                if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
+                else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
+                else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
+            }
+            const K mask = ((K)1 << b) - 1;
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                if (q >= nq) break;
+                const uint32_t id = (uint32_t)(best[q] & mask);
+                lab[cells[tid + q * kCellThreads]] = one_chunk ? sw[id] : (uint16_t)J.seeds[id].w;
             }
         }
-        K best[8];
-        for (int k = 0; k < 8; ++k) best[k] = ~(K)0;
-        const bool work = occ != 0;
-        for (uint32_t base = 0; base < S; base += kSeedChunk) {
-            const uint32_t n = min(kSeedChunk, S - base);
-            if (!one_chunk) {
-                stage(J, base, n);
-                staged = 0xFFFFFFFFu;
-            }
-            if (!work) continue;
-            if (fast) {
-                if (dist == VF_MANHATTAN) naive_chunk<VF_MANHATTAN, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-                else if (dist == VF_CHEBYSHEV) naive_chunk<VF_CHEBYSHEV, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-                else naive_chunk<VF_EUCLIDEAN, true, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-            } else {
-                if (dist == VF_MANHATTAN) naive_chunk<VF_MANHATTAN, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-                else if (dist == VF_CHEBYSHEV) naive_chunk<VF_CHEBYSHEV, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-                else naive_chunk<VF_EUCLIDEAN, false, K>(best, x, y, z, cx, cy, cz, sx, sy, sz, base, n, b);
-            }
+        __syncthreads();
+        if (c0 < J.V) {
+            uint16_t out[8];
+            for (int k = 0; k < 8; ++k) out[k] = lab[tid * 8 + k];
+            store8(J.labels, c0, J.V, out);
         }
-        if (!live) return;
-        uint16_t out[8];
-        const K mask = ((K)1 << b) - 1;
-        for (int k = 0; k < 8; ++k) {
-            const bool o = ((occ >> (8 * k)) & 0xFFu) != 0;
-            const uint32_t id = (uint32_t)(best[k] & mask);
-            out[k] = o ? (one_chunk ? sw[id] : (uint16_t)J.seeds[id].w) : (uint16_t)0;
-        }
-        store8(J.labels, c0, J.V, out);
+        __syncthreads();  // lab and cells are reused by the next tile
     });
 }
 
