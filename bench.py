@@ -67,42 +67,49 @@ def build_jobs(vf, specs, mc=False):
     return meshes, jobs
 
 
-# algorithmic HBM bytes per launch of each kernel (stage definitions: voxelize
-# 36T + V, fracture 3V, boundaries 4V + histogram 2V, checksum 2V)
-def kernel_bytes(name, specs):
+# Algorithmic HBM bytes of each stage per step (the minimum traffic of the
+# stage's definition, not of this implementation): cell stages count the
+# bytes per cell they must read and write (labels 2 B, occupancy 1 B, flood
+# keys 4 B); the voxelizer counts the triangle vertices (36 B per triangle);
+# a flood relaxation round counts 4 KiB per brick visit (its 512 keys read
+# and written once).
+RELAX_BYTES_PER_VISIT = 512 * 4 * 2
+
+
+def kernel_bytes(name, specs, results=None):
     V = [s["edge"] ** 3 for s in specs]
     T = [s["tris"] for s in specs]
-    if name in ("naive", "naive64"):
-        return 3 * sum(v for v, s in zip(V, specs) if s["algo"] == 0)
-    if name == "boundaries_hist":
-        return 6 * sum(V)
-    if name == "checksum":
-        return 2 * sum(V)
-    if name == "vox_fill":
-        return sum(V)
-    if name in ("vox_shell", "vox_parity", "vox_setup"):
-        return 36 * sum(T)
-    if name == "seed_count":
-        return sum(V)
-    if name == "flood_init":
-        return sum(v for v, s in zip(V, specs) if s["algo"] == 1)
-    if name == "flood_finish":
-        return 2 * sum(v for v, s in zip(V, specs) if s["algo"] == 1)
-    return None
+    Vn = sum(v for v, s in zip(V, specs) if s["algo"] == 0)
+    Vf = sum(v for v, s in zip(V, specs) if s["algo"] == 1)
+    table = {
+        "naive": 3 * Vn, "naive64": 3 * Vn,             # occupancy in, labels out
+        "boundaries_hist": 4 * sum(V),                  # labels in and out
+        "checksum": 2 * sum(V),                         # labels in
+        "vox_fill": sum(V),                             # occupancy out
+        "vox_setup": 36 * sum(T), "vox_shell": 36 * sum(T), "vox_parity": 36 * sum(T),
+        "seed_count": sum(V),                           # occupancy in
+        "flood_init": 5 * Vf,                           # occupancy in, keys out
+        "flood_finish": 6 * Vf,                         # keys in, labels out
+    }
+    if name == "flood_relax" and results is not None:
+        return RELAX_BYTES_PER_VISIT * sum(r["flood_bricks"] for r in results)
+    return table.get(name)
 
 
 KERNELS = ["job_init", "vox_clear", "vox_setup", "scan", "vox_shell", "vox_parity", "vox_fill", "seed_count",
-           "seed_pick", "naive", "naive64", "flood_init", "flood_seed", "flood_level", "flood_finish",
-           "boundaries_hist", "hist_summary", "checksum", "mc_count", "mc_emit", "mc_finish"]
+           "seed_pick", "naive", "naive64", "flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan",
+           "flood_finish", "boundaries_hist", "hist_summary", "checksum", "mc_count", "mc_emit", "mc_finish"]
 
 
 def kernel_profile(vf, jobs, steps):
+    """Per-kernel device time from HIP events recorded around every launch on
+    the launch stream (graphs off, so each launch is timed)."""
     b = vf.Batch(graphs=False)
-    b.run(jobs)
+    res = b.run(jobs)
     vf.timing_reset()
     vf.timing_enable(True)
     for _ in range(steps):
-        b.run(jobs)
+        res = b.run(jobs)
     vf.timing_enable(False)
     prof = {}
     for k in KERNELS:
@@ -110,7 +117,7 @@ def kernel_profile(vf, jobs, steps):
         if n:
             prof[k] = {"total_ms": ms / steps, "launches": n / steps, "avg_ms": ms / n}
     b.close()
-    return prof
+    return prof, res
 
 
 def served(vf, specs, rate, n_req, budget, slots, seed):
@@ -206,7 +213,20 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,profile")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (a reference dispatch of known "
+                         "traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = a.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "dispatches": 5}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); jobs are independent, so several GPUs would "
               f"each run their own share (see DESIGN.md)", file=sys.stderr)
@@ -263,15 +283,20 @@ def main():
         log(f"plain: {pv:.2f} Gvoxels/s (batched/plain = {value / pv:.2f}x), bit-exact={same}")
 
     if "profile" not in skip:
-        prof = kernel_profile(vf, jobs, 2)
+        prof, pres = kernel_profile(vf, jobs, 2)
         out["kernels_ms_per_step"] = {k: round(v["total_ms"], 4) for k, v in prof.items()}
-        cand = {k: v for k, v in prof.items() if kernel_bytes(k, specs)}
+        cand = {k: v for k, v in prof.items() if kernel_bytes(k, specs, pres)}
         dom = max(cand, key=lambda k: cand[k]["total_ms"])
-        per_launch = kernel_bytes(dom, specs) / max(1.0, prof[dom]["launches"])
+        per_launch = kernel_bytes(dom, specs, pres) / max(1.0, prof[dom]["launches"])
         achieved = per_launch / (prof[dom]["avg_ms"] * 1e-3) / 1e9
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                           "bytes_per_launch": per_launch, "avg_launch_ms": prof[dom]["avg_ms"]}
+                           "bytes_per_launch": per_launch, "avg_launch_ms": prof[dom]["avg_ms"],
+                           "launches_per_step": prof[dom]["launches"]}
+        if dom == "flood_relax":
+            visits = sum(r["flood_bricks"] for r in pres)
+            out["roofline"]["unit_work"] = {"brick_visits_per_step": visits, "bytes_per_visit": RELAX_BYTES_PER_VISIT,
+                                            "sweeps_per_step": sum(r["flood_sweeps"] for r in pres)}
         log(f"dominant kernel {dom}: {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.3f} of peak)")
 
     if "served" not in skip:

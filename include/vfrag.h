@@ -52,10 +52,10 @@ extern "C" {
 #define VF_NEIGH_VON_NEUMANN 6
 #define VF_NEIGH_MOORE 26
 
-/* Seeder::Location (MF/Source/Fracturer/Seeder.h:25) */
-#define VF_SEED_BOTH 0
+/* Seeder::Location { INNER, OUTER, BOTH } (MF/Source/Fracturer/Seeder.h:24) */
+#define VF_SEED_INNER 0
 #define VF_SEED_OUTER 1
-#define VF_SEED_INNER 2
+#define VF_SEED_BOTH 2
 
 #define VF_ALL_LABELS 0xFFFFFFFFu
 

@@ -59,7 +59,7 @@ void vfo_remove_isolated(const uint16_t* in, uint16_t* out, const uint32_t dims[
                          const uint32_t* seeds, uint32_t S);
 
 /* ---- seeding ------------------------------------------------------------ */
-enum { VFO_SEED_BOTH = 0, VFO_SEED_OUTER = 1, VFO_SEED_INNER = 2 };
+enum { VFO_SEED_INNER = 0, VFO_SEED_OUTER = 1, VFO_SEED_BOTH = 2 };  /* Seeder::Location order (Seeder.h:24) */
 uint32_t vfo_hash32(uint64_t key, uint32_t counter);
 int vfo_seed_uniform(const uint16_t* grid, const uint32_t dims[3], uint32_t S,
                      uint64_t seed, int location, uint32_t* seeds_out);

@@ -160,7 +160,7 @@ def hash32(key, counter):
     return lib().vfo_hash32(key, counter)
 
 
-def seed_uniform(grid, n, seed, location=0):
+def seed_uniform(grid, n, seed, location=2):  # 2 = BOTH (Seeder::Location order)
     g = np.ascontiguousarray(grid, np.uint16)
     out = np.zeros((n, 4), np.uint32)
     rc = lib().vfo_seed_uniform(_p(g), _p(_dims(g.shape)), n, seed, location, _p(out))

@@ -1,0 +1,82 @@
+"""The drop-in C++ surface on the GPU: a caller written only against the
+reference-shaped API (voxelfragmentml_amd/dropin/fracture_scene.cpp, in
+CADScene's fractureGrid -> fractureModel -> prepareScene order) produces the
+oracle's grid, seeds, per-fragment counts and mesh sizes; the autotuner's
+table changes launch shapes but never results."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+DRIVER = ROOT / "voxelfragmentml_amd" / "dropin" / "fracture_scene"
+BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
+
+
+@pytest.mark.parametrize("algo,edge,S,dist,mesh_seed,tris,key,noise_seed,amp", [
+    ("naive", 96, 8, 1, 1, 20000, 11, 0, 0),
+    ("flood", 80, 12, 0, 2, 30000, 12, 77, 2),
+    ("flood", 72, 9, 1, 3, 8000, 13, 5, 1),
+])
+def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_seed, tris, key, noise_seed, amp):
+    assert DRIVER.exists(), "drop-in driver not built"
+    out = tmp_path / "scene"
+    args = [str(DRIVER), algo, str(edge), str(S), str(dist), str(mesh_seed), str(tris), str(key), str(noise_seed),
+            str(amp), str(out)]
+    print("dropin:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    grid = np.fromfile(str(out) + ".grid", np.uint16).reshape(edge, edge, edge)
+    seeds = np.fromfile(str(out) + ".seeds", np.uint32).reshape(-1, 4)
+    hist = np.fromfile(str(out) + ".hist", np.uint32).reshape(-1, 2)
+    mc = np.fromfile(str(out) + ".mc", np.uint32).reshape(-1, 3)
+
+    xyz, tri = vf.vessel_mesh(mesh_seed, *vf.vessel_segments(tris))
+    base = O.occ_to_grid(O.voxelize(xyz, tri, *BOX, (edge, edge, edge)))
+    ref_seeds = O.seed_uniform(base, S, key, 2)  # Seeder::BOTH
+    np.testing.assert_array_equal(seeds, ref_seeds)
+    if algo == "naive":
+        lab = O.naive(base, ref_seeds, dist)
+    else:
+        lab = O.flood(base, ref_seeds, 6 if dist == 1 else 26, noise_seed, amp)
+    ref = O.detect_boundaries(lab, 1)
+    np.testing.assert_array_equal(grid, ref)
+
+    counts = O.histogram(ref)
+    ref_hist = np.array([(v, counts[v]) for v in range(2, 65536) if counts[v]], np.uint32).reshape(-1, 2)
+    np.testing.assert_array_equal(hist, ref_hist)
+
+    m = O.marching_cubes(ref, *BOX)
+    ref_mc = np.stack([m["labels"].astype(np.uint32), np.diff(m["voff"]), np.diff(m["toff"])], axis=1)
+    np.testing.assert_array_equal(mc, ref_mc)
+
+
+def test_tuner_changes_shapes_not_results(vf, tmp_path):
+    specs = [(64, 6, 0, 0), (96, 9, 1, 1), (72, 4, 2, 1), (128, 12, 1, 0)]
+    meshes, jobs = [], []
+    for i, (e, S, dist, algo) in enumerate(specs):
+        xyz, tri = vf.vessel_mesh(40 + i, *vf.vessel_segments(10000))
+        m = vf.DeviceMesh(xyz, tri)
+        meshes.append(m)
+        jobs.append(m.job((e, e, e), *BOX, S, 100 + i, algo, dist, noise_seed=9 + i, noise_amp=2 if algo else 0))
+    with vf.Batch() as b:
+        before = [r["checksum"] for r in b.run(jobs)]
+    path = tmp_path / "tuning.txt"
+    vf.tune_run(path, quick=True)
+    table = vf.read_tuning(path)
+    assert sorted(table) == [0, 1, 2, 3]
+    vf.tune_load(path)
+    with vf.Batch() as b:
+        after = [r["checksum"] for r in b.run(jobs)]
+    assert after == before
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 64 1\n")
+    with pytest.raises(vf.VfError):
+        vf.tune_load(bad)
+    for m in meshes:
+        m.close()

@@ -69,7 +69,25 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_dropin(verbose)
     return LIB
+
+
+DROPIN = PKG / "dropin"
+DRIVER = DROPIN / "fracture_scene"
+
+
+def build_dropin(verbose: bool = False) -> Path:
+    """The drop-in C++ caller (host code only, g++), linked to libvfrag.so."""
+    src = DROPIN / "fracture_scene.cpp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(src), "-o", str(DRIVER),
+           f"-L{PKG}", "-lvfrag", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -101,6 +101,7 @@ struct vf_batch {
     Engine eng;
     bool use_graphs = true;
     uint64_t sig = 0;
+    uint64_t tune_gen = 0;           // launch shapes the graphs were captured with
     std::vector<EJob> jobs;          // every submitted job (invalid ones keep their status)
     std::vector<EJob*> live;         // jobs on the device
     Table* table = nullptr;
@@ -124,6 +125,13 @@ struct vf_batch {
 
 static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
 {
+    const uint64_t gen = tune_generation();
+    if (gen != b->tune_gen) {  // launch shapes changed: recapture
+        b->g_front.reset();
+        b->g_rounds.reset();
+        b->g_post.reset();
+        b->tune_gen = gen;
+    }
     const uint64_t sig = job_signature(jobs, n);
     if (sig == b->sig && b->jobs.size() == n) return;
     b->clear();

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "grid.hpp"
+#include "tune.hpp"
 
 namespace vf {
 
@@ -86,6 +87,7 @@ Launch grid_launch(vf_grid* g)
         VF_HIP(hipStreamSynchronize(g->stream));
         g->job_dirty = false;
     }
+    set_active_bucket(tune_bucket(std::max(g->dims[0], std::max(g->dims[1], g->dims[2]))));
     Launch L;
     L.jobs = g->jobdev.as<JobDev>();
     L.njobs = 1;
@@ -494,7 +496,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         launch_flood_seed(L);
         uint32_t chunk = 4, rounds = 0;
         for (;;) {
-            launch_flood_rounds(L, W, chunk, kFloodRoundBlocks);
+            launch_flood_rounds(L, W, chunk, active_shape().flood_blocks);
             rounds += chunk;
             VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
             VF_HIP(hipStreamSynchronize(g->stream));

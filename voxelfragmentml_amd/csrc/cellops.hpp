@@ -4,7 +4,10 @@
 // when the run is complete, element-wise at the tail.
 #pragma once
 
+#include <algorithm>
+
 #include "jobs.hpp"
+#include "tune.hpp"
 
 namespace vf {
 

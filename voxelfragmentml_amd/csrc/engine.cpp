@@ -70,7 +70,7 @@ int job_validate(const vf_job& j, std::string& why)
     if (j.num_seeds == 0 || j.num_seeds > kMaxDeviceSeeds) { why = "seed count out of range"; return VF_ERR_INVALID_ARG; }
     if (j.algorithm != VF_NAIVE && j.algorithm != VF_FLOOD) { why = "unknown algorithm"; return VF_ERR_INVALID_ARG; }
     if (j.distance < VF_EUCLIDEAN || j.distance > VF_CHEBYSHEV) { why = "unknown distance"; return VF_ERR_INVALID_ARG; }
-    if (j.seed_location < VF_SEED_BOTH || j.seed_location > VF_SEED_INNER) { why = "unknown seed location"; return VF_ERR_INVALID_ARG; }
+    if (j.seed_location < VF_SEED_INNER || j.seed_location > VF_SEED_BOTH) { why = "unknown seed location"; return VF_ERR_INVALID_ARG; }
     if (j.boundary_size < 0 || j.boundary_size > 64) { why = "boundary size out of range"; return VF_ERR_INVALID_ARG; }
     if (j.algorithm == VF_FLOOD) {
         if (j.num_seeds > 254) { why = "flood labels must fit 8 bits"; return VF_ERR_TOO_MANY_SEEDS; }
@@ -197,6 +197,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     auto* bp = reinterpret_cast<uint32_t*>(img.data() + o_bt);
     auto* fp = reinterpret_cast<uint32_t*>(img.data() + o_done);
     uint64_t tiles = 0, tris = 0, groups = 0, btiles = 0, fchunks = 0;
+    uint32_t max_edge = 1;
     for (uint32_t i = 0; i < n; ++i) {
         EJob& e = *js[i];
         jd[i] = e.d;
@@ -207,6 +208,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         fp[i] = (uint32_t)fchunks;
         btiles += brick_count(e.d.nx, e.d.ny, e.d.nz);
         if (e.d.algo == VF_FLOOD) fchunks += relax_tasks(e.d.nx, e.d.ny, e.d.nz);
+        max_edge = std::max(max_edge, std::max(e.d.nx, std::max(e.d.ny, e.d.nz)));
         t->any_wide |= e.d.bsize > 1;
         tiles += e.tiles;
         tris += e.d.nt;
@@ -238,6 +240,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->total_tris = (uint32_t)tris;
     t->total_groups = (uint32_t)groups;
     t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
+    t->bucket = tune_bucket(max_edge);
     t->flood.task_prefix = reinterpret_cast<const uint32_t*>(base + o_done);
     t->flood.total_tasks = (uint32_t)fchunks;
     t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
@@ -272,6 +275,7 @@ void Engine::reserve_front(const Table& t)
 // Launch-only (safe inside stream capture): reserve_front must have sized the scratch.
 void Engine::front(Table& t)
 {
+    set_active_bucket(t.bucket);
     const Launch& L = t.L;
     launch_job_init(L);
     VoxScratch X;
@@ -298,11 +302,13 @@ void Engine::front(Table& t)
 
 void Engine::rounds(Table& t, uint32_t n)
 {
-    if (t.flood.total_tasks) launch_flood_rounds(t.L, t.flood, n, kFloodRoundBlocks);
+    set_active_bucket(t.bucket);
+    if (t.flood.total_tasks) launch_flood_rounds(t.L, t.flood, n, active_shape().flood_blocks);
 }
 
 void Engine::post(Table& t)
 {
+    set_active_bucket(t.bucket);
     const Launch& L = t.L;
     if (t.any_flood) launch_flood_finish(L);
     launch_boundaries_tiled(L, t.bt_prefix, t.total_btiles, t.any_wide);

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "jobs.hpp"
+#include "tune.hpp"
 
 namespace vf {
 
@@ -52,6 +53,7 @@ struct Table {
     bool any_wide = false;                // some job uses a boundary window wider than 1
     uint32_t* gather = nullptr;       // 16 words per job
     FloodRoundScratch flood{};      // chunk prefix + worklists of the flood rounds
+    int bucket = 0;                 // launch-shape bucket (largest grid edge of the table)
     void* flood_lists = nullptr;    // separate allocation of flood.lists
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
 };

@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
 void launch_flood_init(const Launch& L)
 {
     TimedLaunch tl("flood_init", L.s);
-    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
     k_flood_init<<<blocks, kCellThreads, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
@@ -519,7 +519,7 @@ void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t r
 void launch_flood_finish(const Launch& L)
 {
     TimedLaunch tl("flood_finish", L.s);
-    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
     k_flood_finish<<<blocks, kCellThreads, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }

@@ -422,7 +422,7 @@ __global__ void k_gather(Launch L, uint32_t* out)
     }
 }
 
-inline uint32_t grid_for(const Launch& L) { return L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048; }
+inline uint32_t grid_for(const Launch& L) { return std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks)); }
 
 }  // namespace
 
@@ -484,7 +484,7 @@ void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_
 {
     {
         TimedLaunch tl("boundaries_hist", L.s);
-        const uint32_t blocks = total_btiles < 4096 ? (total_btiles ? total_btiles : 1) : 4096;
+        const uint32_t blocks = std::max(1u, std::min(total_btiles, active_shape().brick_blocks));
         k_boundaries_tiled<true><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
         VF_CHECK_LAUNCH();
     }

@@ -190,7 +190,7 @@ uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
 // continues at its own round, so jobs may join a table between calls; rounds
 // after a job drained do nothing for it, so the launches can be replayed from
 // a graph.
-constexpr uint32_t kFloodRoundBlocks = 4096;  // one wave per block
+// blocks: workgroups of one wave for the compaction and relaxation launches
 void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks);
 void launch_flood_finish(const Launch& L);
 

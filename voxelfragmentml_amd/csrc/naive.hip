@@ -266,7 +266,7 @@ __global__ void k_ri_reset(uint32_t* counters, uint32_t idx) { counters[idx] = 0
 void launch_naive(const Launch& L, bool key64)
 {
     if (L.njobs > kWalkMaxJobs) fail(VF_ERR_INVALID_ARG, "naive launch with %u jobs (max %u)", L.njobs, kWalkMaxJobs);
-    const uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
     TimedLaunch tl(key64 ? "naive64" : "naive", L.s);
     if (key64) k_naive<unsigned long long><<<blocks, kCellThreads, 0, L.s>>>(L);
     else k_naive<uint32_t><<<blocks, kCellThreads, 0, L.s>>>(L);

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
 void launch_seed_count(const Launch& L, uint32_t* tile_cnt)
 {
     TimedLaunch tl("seed_count", L.s);
-    uint32_t blocks = L.total_tiles < 2048 ? (L.total_tiles ? L.total_tiles : 1) : 2048;
+    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
     k_seed_count<<<blocks, kCellThreads, 0, L.s>>>(L, tile_cnt);
     VF_CHECK_LAUNCH();
 }

@@ -469,7 +469,7 @@ size_t vox_scan_tmp_bytes(uint32_t total_tris) { return scan_tmp_bytes(total_tri
 void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_groups)
 {
     TimedLaunch tl("vox_clear", L.s);
-    uint32_t blocks = total_groups < 4096 ? (total_groups ? total_groups : 1) : 4096;
+    const uint32_t blocks = std::max(1u, std::min(total_groups, active_shape().cell_blocks));
     k_vox_clear<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
     VF_CHECK_LAUNCH();
 }
@@ -492,17 +492,18 @@ void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_p
         scan_exclusive_u32_to_u64(cnt_col, X.col_off, X.total_tris, X.scan_tmp, L.s);
         {
             TimedLaunch tl("vox_shell", L.s);
-            k_vox_shell<<<8192, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
+            k_vox_shell<<<active_shape().pair_blocks, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
             VF_CHECK_LAUNCH();
         }
         {
             TimedLaunch tl("vox_parity", L.s);
-            k_vox_parity<<<4096, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off, X.total_tris);
+            k_vox_parity<<<std::max(64u, active_shape().pair_blocks / 2), kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off,
+                                                                                                X.total_tris);
             VF_CHECK_LAUNCH();
         }
     }
     TimedLaunch tl("vox_fill", L.s);
-    uint32_t blocks = total_groups < 4096 ? (total_groups ? total_groups : 1) : 4096;
+    const uint32_t blocks = std::max(1u, std::min(total_groups, active_shape().cell_blocks));
     k_vox_fill<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
     VF_CHECK_LAUNCH();
 }

@@ -17,7 +17,7 @@ LIB_PATH = _PKG / "libvfrag.so"
 EUCLIDEAN, MANHATTAN, CHEBYSHEV = 0, 1, 2
 NAIVE, FLOOD = 0, 1
 NEIGH_FROM_DISTANCE, NEIGH_VON_NEUMANN, NEIGH_MOORE = 0, 6, 26
-SEED_BOTH, SEED_OUTER, SEED_INNER = 0, 1, 2
+SEED_INNER, SEED_OUTER, SEED_BOTH = 0, 1, 2  # Seeder::Location order
 ALL_LABELS = 0xFFFFFFFF
 
 ERRORS = {
@@ -99,6 +99,7 @@ def lib():
         "vf_memcpy_htod": (I, [P, P, C.c_size_t]),
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
         "vf_timing_read": (I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),
+        "vf_tune_run": (I, [C.c_char_p, I]), "vf_tune_load": (I, [C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -278,6 +279,27 @@ def timing_read(kernel: str):
     return ms.value, n.value
 
 
+def tune_run(path, quick: bool = True):
+    """Run the autotuner on this device and write the launch-shape table to path."""
+    check(lib().vf_tune_run(str(path).encode(), 1 if quick else 0))
+
+
+def tune_load(path):
+    """Install a launch-shape table written by tune_run."""
+    check(lib().vf_tune_load(str(path).encode()))
+
+
+def read_tuning(path):
+    """The table of a tuning file: {bucket: {max_edge, cell_blocks, flood_blocks, pair_blocks, brick_blocks}}."""
+    out = {}
+    for line in Path(path).read_text().splitlines():
+        if line.startswith("#") or not line.strip():
+            continue
+        b, e, c, f, p, k = (int(x) for x in line.split())
+        out[b] = {"max_edge": e, "cell_blocks": c, "flood_blocks": f, "pair_blocks": p, "brick_blocks": k}
+    return out
+
+
 # ------------------------------------------------------------ batched / served
 def make_job(xyz, tri, dims, aabb_min, aabb_max, num_seeds, seed, algorithm, distance,
              neighbourhood=NEIGH_FROM_DISTANCE, noise_seed=0, noise_amp=0, boundary_size=1,
@@ -333,6 +355,12 @@ class Batch:
         if self._h:
             lib().vf_batch_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:
