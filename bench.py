@@ -96,6 +96,11 @@ def kernel_bytes(name, specs, results=None):
     return table.get(name)
 
 
+def path_bytes(specs):
+    """Algorithmic bytes of the whole path per step: voxelize 36 T + V, fracture 3 V, boundaries 4 V."""
+    return sum(8 * s["edge"] ** 3 + 36 * s["tris"] for s in specs)
+
+
 KERNELS = ["job_init", "vox_clear", "vox_setup", "scan", "vox_shell", "vox_parity", "vox_fill", "seed_count",
            "seed_pick", "naive", "naive64", "flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan",
            "flood_finish", "boundaries_hist", "hist_summary", "checksum", "mc_count", "mc_emit", "mc_finish"]
@@ -212,7 +217,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,profile")
+    ap.add_argument("--skip", default="", help="comma list of: tune,plain,served,cpu,profile")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a reference dispatch of known "
                          "traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -239,6 +244,16 @@ def main():
     specs = job_specs(a.jobs, 4)
     voxels = sum(s["edge"] ** 3 for s in specs)
     log(f"mixed workload: {len(specs)} jobs, {voxels / 1e9:.3f} Gvoxels per step")
+    tuning = None
+    if "tune" not in skip:
+        # launch shapes from the autotuner (its own synthetic jobs per edge bucket), outside the timed region
+        import tempfile
+        t0 = time.perf_counter()
+        with tempfile.TemporaryDirectory() as d:
+            path = Path(d) / "tuning.txt"
+            vf.tune_run(path, quick=True)
+            tuning = {"table": vf.read_tuning(path), "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotuner: {tuning}")
     meshes, jobs = build_jobs(vf, specs)
 
     # ---- batched form (the headline)
@@ -267,6 +282,11 @@ def main():
                                "distance uniform, naive/flood 50:50, flood noise 2, boundaries + histogram",
                    "jobs": len(specs), "voxels_per_step": voxels, "rng_seed": 4, "form": "batched (HIP graphs)"},
         "batched": {"gvoxels_per_s": value, "ms_per_step": ms_step, "deterministic": deterministic},
+        "path_bytes": {"algorithmic_bytes_per_step": path_bytes(specs),
+                       "achieved_gbs": round(path_bytes(specs) / (ms_step * 1e-3) / 1e9, 2),
+                       "frac_of_hbm_peak": round(path_bytes(specs) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "definition": "8 B per cell + 36 B per triangle per job"},
+        "tuning": tuning,
     }
 
     if "plain" not in skip:
