@@ -212,8 +212,9 @@ typedef struct {
     int status;
     uint64_t voxels;        /* V = nx*ny*nz */
     uint64_t occupied;      /* non-EMPTY cells after voxelization */
-    uint64_t checksum;      /* order-independent grid checksum: sum over non-EMPTY cells c (linear index) of
-                               (hash32(value, c) << 32) | hash32(c, value) */
+    uint64_t checksum;      /* order-independent grid checksum: sum (mod 2^64) over non-EMPTY cells c (linear
+                               index) holding v of (a << 32) | fmix32(a ^ c ^ 0x7F4A7C15), a = fmix32(c ^ v * 0x9E3779B1),
+                               fmix32 the MurmurHash3 finalizer */
     uint32_t num_labels;    /* distinct labels with a nonzero count */
     uint32_t mc_vertices, mc_triangles;
     uint32_t flood_levels;  /* flood relaxation rounds run (brick worklist rounds until drained) */

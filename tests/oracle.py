@@ -222,14 +222,16 @@ def rle_decode(data: bytes):
 
 
 def checksum(grid):
-    """Order-independent grid checksum, the same definition as the product's:
-    the sum over non-EMPTY cells c of (hash32(value, c) << 32) | hash32(c, value)."""
-    g = np.ascontiguousarray(grid, np.uint16).ravel().astype(np.uint64)
-    c = np.nonzero(g)[0].astype(np.uint64)
-    g = g[c]
-    h1 = _hash32_np(g, c.astype(np.uint32))
-    h2 = _hash32_np(c, g.astype(np.uint32))
-    terms = (h1.astype(np.uint64) << np.uint64(32)) | h2.astype(np.uint64)
+    """Order-independent grid checksum, the same definition as the product's: the sum
+    over non-EMPTY cells c holding v of (a << 32) | fmix32(a ^ c ^ 0x7F4A7C15) with
+    a = fmix32(c ^ v * 0x9E3779B1)."""
+    g = np.ascontiguousarray(grid, np.uint16).ravel()
+    c = np.nonzero(g)[0].astype(np.uint32)
+    v = g[c].astype(np.uint32)
+    with np.errstate(over="ignore"):
+        a = _fmix32(c ^ (v * np.uint32(0x9E3779B1)).astype(np.uint32))
+        b = _fmix32(a ^ c ^ np.uint32(0x7F4A7C15))
+    terms = (a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64)
     return int(np.sum(terms, dtype=np.uint64))
 
 

@@ -526,7 +526,8 @@ int vf_detect_boundaries(vf_grid* g, int size)
             g->job_dirty = true;
         }
         Launch L = grid_launch(g);
-        if (size <= 1) launch_boundaries_tiled(L, g->prefix.as<uint32_t>() + 2, brick_count(g->dims[0], g->dims[1], g->dims[2]), false);
+        if (size <= 1) launch_boundaries_tiled(L, g->prefix.as<uint32_t>() + 2, brick_count(g->dims[0], g->dims[1], g->dims[2]), false,
+                                              false);
         else launch_boundaries(L, size, 0);
         g->mc.valid = false;
         VF_HIP(hipStreamSynchronize(g->stream));

@@ -311,9 +311,8 @@ void Engine::post(Table& t)
     set_active_bucket(t.bucket);
     const Launch& L = t.L;
     if (t.any_flood) launch_flood_finish(L);
-    launch_boundaries_tiled(L, t.bt_prefix, t.total_btiles, t.any_wide);
+    launch_boundaries_tiled(L, t.bt_prefix, t.total_btiles, t.any_wide, true);  // + grid checksum
     launch_hist_summary(L);
-    launch_checksum(L);
     launch_gather(L, t.gather);
 }
 

@@ -124,18 +124,34 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const int64_t org = ((int64_t)(X0 - 1) * ny + (Y0 - 1)) * nz + (Z0 - 1);
     uint32_t* kb = R.keys + org;  // dereferenced only at in-grid cells
 
-    // brick + halo -> LDS as key - 1 (cells outside the grid are EMPTY), 8 loads in flight per lane
+    // brick + halo -> LDS as key - 1 (cells outside the grid are EMPTY).  Each
+    // lane owns one or two (y, z) positions of the 10 x 10 halo plane and walks
+    // the 10 x slices, so the index work is done once per visit.
+    {
+        const uint32_t p1 = lane, p2 = lane + 64u;  // p2 valid for lanes < 36
+        const uint32_t hy1 = p1 / 10u, hz1 = p1 - hy1 * 10u, hy2 = p2 / 10u, hz2 = p2 - hy2 * 10u;
+        const bool in1 = (uint32_t)Y0 + hy1 - 1u < ny && (uint32_t)Z0 + hz1 - 1u < nz;
+        const bool in2 = p2 < 100u && (uint32_t)Y0 + hy2 - 1u < ny && (uint32_t)Z0 + hz2 - 1u < nz;
+        const uint32_t o1 = hy1 * nz + hz1, o2 = hy2 * nz + hz2;              // cell offsets within an x slice
+        const uint32_t t1 = hy1 * kRowWords + hz1, t2 = hy2 * kRowWords + hz2;  // tile offsets within an x slice
+        const uint32_t slice = ny * nz;
 #pragma unroll 1
-    for (uint32_t i1 = 0; i1 < 1024u; i1 += 512u) {
+        for (uint32_t h0 = 0; h0 < 10u; h0 += 5u) {
+            uint32_t v1[5], v2[5];
 #pragma unroll
-    for (uint32_t i0 = 0; i0 < 512u; i0 += 64u) {
-        const uint32_t i = i1 + i0 + lane;
-        const uint32_t hx = i / 100u, rem = i - hx * 100u, hy = rem / 10u, hz = rem - hy * 10u;
-        const uint32_t gx = (uint32_t)X0 + hx - 1u, gy = (uint32_t)Y0 + hy - 1u, gz = (uint32_t)Z0 + hz - 1u;
-        uint32_t v = kEmptyKey;
-        if (i < 1000u && gx < nx && gy < ny && gz < nz) v = kb[(hx * ny + hy) * nz + hz];
-        if (i < 1000u) tile[(hx * 10u + hy) * kRowWords + hz] = v - 1u;
-    }
+            for (uint32_t k = 0; k < 5u; ++k) {
+                const uint32_t hx = h0 + k;
+                const bool xin = (uint32_t)X0 + hx - 1u < nx;
+                v1[k] = xin && in1 ? kb[hx * slice + o1] : kEmptyKey;
+                v2[k] = xin && in2 ? kb[hx * slice + o2] : kEmptyKey;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 5u; ++k) {
+                const uint32_t hx = h0 + k;
+                tile[hx * 10u * kRowWords + t1] = v1[k] - 1u;
+                if (p2 < 100u) tile[hx * 10u * kRowWords + t2] = v2[k] - 1u;
+            }
+        }
     }
     __syncthreads();
 

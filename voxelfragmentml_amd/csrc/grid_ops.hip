@@ -72,6 +72,34 @@ __global__ void k_clear_hist(Launch L)
 constexpr uint32_t kLdsBins = 4096;
 constexpr uint32_t kTiledBins = 1024;  // LDS histogram bins of the tiled boundary kernel (larger labels go to HBM)
 
+// Per-job sum of a per-thread value: wave sums, then one device atomic per block and job.
+struct BlockSum {
+    unsigned long long part[kCellThreads / 64];
+    __device__ void flush(unsigned long long& acc, unsigned long long* dst)
+    {
+        const unsigned long long s = wave_sum_u64(acc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
+            if (tot) atomicAdd(dst, tot);
+        }
+        __syncthreads();
+        acc = 0;
+    }
+};
+
+// Checksum term of one cell c (linear index) holding v: a = fmix32(c ^ v * 0x9E3779B1),
+// term = (a << 32) | fmix32(a ^ c ^ 0x7F4A7C15); EMPTY adds nothing.
+__device__ inline unsigned long long checksum_term(uint16_t v, uint32_t c)
+{
+    if (!v) return 0ull;
+    const uint32_t a = fmix32(c ^ ((uint32_t)v * 0x9E3779B1u));
+    return ((unsigned long long)a << 32) | fmix32(a ^ c ^ 0x7F4A7C15u);
+}
+
+
 __device__ inline uint32_t ceil_div_d(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Boundary bit of one labelled cell (detectBoundaries-comp.glsl:18-43).
@@ -174,19 +202,29 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
 // load the halo and run separable max/min passes z, y, x in LDS.
 constexpr int kBTX = 8, kBTY = 8, kBTZ = 32;
 constexpr int kBHX = kBTX + 2, kBHY = kBTY + 2, kBHZ = kBTZ + 2;
-constexpr int kAZ = kBTZ + 2;  // z-pass row stride (u16): 17 words, odd, so rows spread over the LDS banks
+constexpr int kAZ = kBTZ + 1;  // z-pass row stride (words): odd, so rows spread over the LDS banks
 
-template <bool HIST>
+// CSUM: also accumulate the grid checksum of the final cells (jobs with a
+// window size <= 1; the others get theirs from k_checksum<true>).
+//
+// Each thread owns an 8-cell z segment of one of the brick's 64 rows and loads
+// it with its two z neighbours (clamped coordinates = replicate padding), so
+// the z window is done in registers; 144 of the threads also do one segment
+// of the 36 halo rows.  Only the z-window max/min of the 100 rows goes
+// through LDS, for the 3x3 x/y window of each labelled cell.
+template <bool HIST, bool CSUM>
 __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
 {
-    __shared__ uint16_t raw[kBHX * kBHY * kBHZ];
-    __shared__ uint16_t ax[kBHX * kBHY * kAZ], bx[kBHX * kBHY * kAZ];  // after the z pass
+    __shared__ uint32_t zmm[kBHX * kBHY * kAZ];  // z-window max (low 16 bits) | min (high 16), row = hx * kBHY + hy
     __shared__ uint32_t bins[HIST ? kTiledBins : 1];
     __shared__ JobDev sJ;
+    __shared__ BlockSum csum_part;
+    unsigned long long csum = 0;
     if (HIST) {
         for (uint32_t i = threadIdx.x; i < kTiledBins; i += blockDim.x) bins[i] = 0;
     }
     auto flush = [&](uint32_t j) {
+        if (CSUM) csum_part.flush(csum, L.jobs[j].stats + kStatChecksum);
         if (!HIST) return;
         __syncthreads();
         uint32_t* h = L.jobs[j].hist;
@@ -199,9 +237,34 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         }
         __syncthreads();
     };
-    // this thread's 8 cells: row (x, y) = tid / 4, z segment (tid % 4) * 8
+    // this thread's interior segment: row (tx, ty) = tid / 4, z segment (tid % 4) * 8
     const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 8;
     const int tx = row / kBTY, ty = row % kBTY;
+    // halo task (threads < 144): ring row r = tid / 4 of the 36 rows with hx in {0, 9} or hy in {0, 9}
+    const bool halo_task = threadIdx.x < (2 * kBHY + 2 * kBTX) * 4;
+    int hhx = 0, hhy = 0;
+    {
+        const int r = threadIdx.x >> 2;
+        if (r < kBHY) { hhx = 0; hhy = r; }
+        else if (r < 2 * kBHY) { hhx = kBHX - 1; hhy = r - kBHY; }
+        else if (r < 2 * kBHY + kBTX) { hhx = r - 2 * kBHY + 1; hhy = 0; }
+        else { hhx = r - 2 * kBHY - kBTX + 1; hhy = kBHY - 1; }
+    }
+    // z-window max/min of 8 cells from their 10 raw values w[0..9] (w[k+1] is cell k)
+    auto zwin = [](const uint16_t w[10], uint32_t out[8]) {
+        uint16_t mx[10], mn[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            const uint16_t m = w[k] & kMask15;
+            const bool lab = m > kFree;
+            mx[k] = lab ? m : (uint16_t)0;
+            mn[k] = lab ? m : (uint16_t)0xFFFF;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            out[k] = (uint32_t)max(max(mx[k], mx[k + 1]), mx[k + 2]) |
+                     ((uint32_t)min(min(mn[k], mn[k + 1]), mn[k + 2]) << 16);
+    };
     walk_tiles(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
         if (!J.active || J.bsize > 1) return;
         const int nx = (int)J.nx, ny = (int)J.ny, nz = (int)J.nz;
@@ -211,23 +274,41 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         const int z0 = (int)(local % tz) * kBTZ;
         const bool bound = J.bsize == 1;
         const uint16_t* lab = J.labels;
+        // ten raw values of row (gx, gy) at z = zs - 1 .. zs + 8, clamped to the grid
+        auto load_row = [&](int gx, int gy, int zs, uint16_t w[10]) {
+            const int cx = min(max(gx, 0), nx - 1), cy = min(max(gy, 0), ny - 1);
+            const uint16_t* r = lab + ((size_t)cx * ny + cy) * nz;
+            if (zs >= 1 && zs + 9 <= nz && (nz & 7) == 0 && (zs & 7) == 0) {
+                const uint4 q = *reinterpret_cast<const uint4*>(r + zs);
+                w[1] = (uint16_t)q.x; w[2] = (uint16_t)(q.x >> 16); w[3] = (uint16_t)q.y; w[4] = (uint16_t)(q.y >> 16);
+                w[5] = (uint16_t)q.z; w[6] = (uint16_t)(q.z >> 16); w[7] = (uint16_t)q.w; w[8] = (uint16_t)(q.w >> 16);
+                w[0] = r[zs - 1];
+                w[9] = r[zs + 8];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 10; ++k) w[k] = r[min(max(zs - 1 + k, 0), nz - 1)];
+            }
+        };
         // interior
         const int gx = x0 + tx, gy = y0 + ty, gz = z0 + seg;
         const bool row_ok = gx < nx && gy < ny;
         const int nval = row_ok ? max(0, min(8, nz - gz)) : 0;
+        uint16_t w[10];
+        load_row(gx, gy, gz, w);
         uint16_t v[8];
-        const uint16_t* src = lab + ((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz;
-        const bool vec = (nz & 7) == 0;
-        if (nval == 8 && vec) {
-            const uint4 q = *reinterpret_cast<const uint4*>(src);
-            v[0] = (uint16_t)q.x; v[1] = (uint16_t)(q.x >> 16); v[2] = (uint16_t)q.y; v[3] = (uint16_t)(q.y >> 16);
-            v[4] = (uint16_t)q.z; v[5] = (uint16_t)(q.z >> 16); v[6] = (uint16_t)q.w; v[7] = (uint16_t)(q.w >> 16);
-        } else {
-            for (int k = 0; k < 8; ++k) v[k] = k < nval ? src[k] : (uint16_t)0;
-        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = k < nval ? w[k + 1] : (uint16_t)0;
         bool mine = false;
         for (int k = 0; k < 8; ++k) mine |= v[k] > kFree;
-        if (!__syncthreads_or(mine)) return;
+        const uint32_t c0 = (uint32_t)(((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz);
+        auto add_checksum = [&]() {
+            if (CSUM)
+                for (int k = 0; k < nval; ++k) csum += checksum_term(v[k], c0 + k);
+        };
+        if (!__syncthreads_or(mine)) {
+            add_checksum();
+            return;
+        }
         if (HIST) {
             // run-length over the 8 cells: one LDS atomic per run of equal labels
             uint32_t cur = 0, run = 0;
@@ -248,49 +329,25 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
                 else atomicAdd(J.hist + cur, run);
             }
         }
-        if (!bound) return;
-        auto clamped = [&](int hx, int hy, int hz) {
-            const int cx = min(max(x0 + hx - 1, 0), nx - 1), cy = min(max(y0 + hy - 1, 0), ny - 1),
-                      cz = min(max(z0 + hz - 1, 0), nz - 1);
-            return lab[((size_t)cx * ny + cy) * nz + cz];
-        };
-        if (x0 + kBTX <= nx && y0 + kBTY <= ny && z0 + kBTZ <= nz) {
-            // interior from registers, then only the halo shell from HBM
-            uint16_t* dst = raw + ((tx + 1) * kBHY + ty + 1) * kBHZ + 1 + seg;
-            for (int k = 0; k < 8; ++k) dst[k] = v[k];
-            constexpr int kRing = 2 * kBHY + 2 * kBTX;  // rows with hx in {0, 9} or hy in {0, 9}
-            for (int i = threadIdx.x; i < kRing * kBHZ; i += blockDim.x) {
-                const int r = i / kBHZ, hz = i - r * kBHZ;
-                int hx, hy;
-                if (r < kBHY) { hx = 0; hy = r; }
-                else if (r < 2 * kBHY) { hx = kBHX - 1; hy = r - kBHY; }
-                else if (r < 2 * kBHY + kBTX) { hx = r - 2 * kBHY + 1; hy = 0; }
-                else { hx = r - 2 * kBHY - kBTX + 1; hy = kBHY - 1; }
-                raw[(hx * kBHY + hy) * kBHZ + hz] = clamped(hx, hy, hz);
-            }
-            for (int i = threadIdx.x; i < kBTX * kBTY * 2; i += blockDim.x) {
-                const int r = i >> 1, hx = r / kBTY + 1, hy = r % kBTY + 1, hz = (i & 1) ? kBHZ - 1 : 0;
-                raw[(hx * kBHY + hy) * kBHZ + hz] = clamped(hx, hy, hz);
-            }
-        } else {
-            // whole box (interior + halo), clamped
-            for (int i = threadIdx.x; i < kBHX * kBHY * kBHZ; i += blockDim.x) {
-                const int hz = i % kBHZ, hy = (i / kBHZ) % kBHY, hx = i / (kBHZ * kBHY);
-                raw[i] = clamped(hx, hy, hz);
-            }
+        if (!bound) {
+            add_checksum();
+            return;
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < kBHX * kBHY * kBTZ; i += blockDim.x) {
-            const int z = i % kBTZ, r = i / kBTZ;
-            uint16_t mx = 0, mn = 0xFFFF;
-            for (int dz = 0; dz < 3; ++dz) {
-                const uint16_t m = raw[r * kBHZ + z + dz] & kMask15;
-                const bool labd = m > kFree;
-                mx = max(mx, labd ? m : (uint16_t)0);
-                mn = min(mn, labd ? m : (uint16_t)0xFFFF);
-            }
-            ax[r * kAZ + z] = mx;
-            bx[r * kAZ + z] = mn;
+        {
+            uint32_t zw[8];
+            zwin(w, zw);
+            uint32_t* dst = zmm + ((tx + 1) * kBHY + ty + 1) * kAZ + seg;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dst[k] = zw[k];
+        }
+        if (halo_task) {
+            uint16_t hw[10];
+            load_row(x0 + hhx - 1, y0 + hhy - 1, z0 + seg, hw);
+            uint32_t zw[8];
+            zwin(hw, zw);
+            uint32_t* dst = zmm + (hhx * kBHY + hhy) * kAZ + seg;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dst[k] = zw[k];
         }
         __syncthreads();
         // x and y windows together, only for the labelled cells still without the bit
@@ -302,18 +359,19 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             uint16_t mx = 0, mn = 0xFFFF;
             for (int dx = 0; dx < 3; ++dx)
                 for (int dy = 0; dy < 3; ++dy) {
-                    const int q = ((tx + dx) * kBHY + ty + dy) * kAZ + z;
-                    mx = max(mx, ax[q]);
-                    mn = min(mn, bx[q]);
+                    const uint32_t q = zmm[((tx + dx) * kBHY + ty + dy) * kAZ + z];
+                    mx = max(mx, (uint16_t)q);
+                    mn = min(mn, (uint16_t)(q >> 16));
                 }
             if (mx != c || mn != c) {
                 v[k] = (uint16_t)(c | 0x8000u);
                 changed = true;
             }
         }
+        add_checksum();
         if (changed) {
             uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
-            if (nval == 8 && vec) {
+            if (nval == 8 && (nz & 7) == 0) {
                 uint4 q;
                 q.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
                 q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
@@ -328,42 +386,22 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
     }, flush);
 }
 
-// Per-job sum of a per-thread value: wave sums, then one device atomic per block and job.
-struct BlockSum {
-    unsigned long long part[kCellThreads / 64];
-    __device__ void flush(unsigned long long& acc, unsigned long long* dst)
-    {
-        const unsigned long long s = wave_sum_u64(acc);
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long tot = 0;
-            for (int w = 0; w < (int)(kCellThreads / 64); ++w) tot += part[w];
-            if (tot) atomicAdd(dst, tot);
-        }
-        __syncthreads();
-        acc = 0;
-    }
-};
-
-// Order-independent grid checksum: sum over the non-EMPTY cells c of
-// (hash32(value, c) << 32) | hash32(c, value).
+// Order-independent grid checksum: sum over the non-EMPTY cells of checksum_term.
+// ONLY_WIDE: only jobs whose boundary window is wider than 1 (the tiled
+// boundary kernel sums the others).
+template <bool ONLY_WIDE>
 __global__ __launch_bounds__(kCellThreads) void k_checksum(Launch L)
 {
     __shared__ BlockSum bs;
     __shared__ JobDev sJ;
     unsigned long long acc = 0;
     walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
-        if (!J.active) return;
+        if (!J.active || (ONLY_WIDE && J.bsize <= 1)) return;
         const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J.V) return;
         uint16_t v[8];
         load8(J.labels, c0, J.V, v);
-        for (int k = 0; k < 8; ++k) {
-            if (v[k] == 0) continue;  // EMPTY cells (and cells beyond V, loaded as 0) add nothing
-            const uint32_t c = (uint32_t)(c0 + k);
-            acc += ((unsigned long long)hash32(v[k], c) << 32) | hash32(c, v[k]);
-        }
+        for (int k = 0; k < 8; ++k) acc += checksum_term(v[k], (uint32_t)(c0 + k));  // cells beyond V load as 0
     }, [&](uint32_t j) { bs.flush(acc, L.jobs[j].stats + kStatChecksum); });
 }
 
@@ -480,25 +518,32 @@ void launch_boundaries(const Launch& L, int size, int with_hist)
     VF_CHECK_LAUNCH();
 }
 
-void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide)
+void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide,
+                             bool with_checksum)
 {
     {
         TimedLaunch tl("boundaries_hist", L.s);
         const uint32_t blocks = std::max(1u, std::min(total_btiles, active_shape().brick_blocks));
-        k_boundaries_tiled<true><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+        if (with_checksum) k_boundaries_tiled<true, true><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+        else k_boundaries_tiled<true, false><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
         VF_CHECK_LAUNCH();
     }
     if (any_wide) {
         TimedLaunch tl("boundaries_wide", L.s);
         k_cells_post<true, true, true><<<grid_for(L), kCellThreads, 0, L.s>>>(L, -1);
         VF_CHECK_LAUNCH();
+        if (with_checksum) {
+            TimedLaunch tc("checksum", L.s);
+            k_checksum<true><<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+            VF_CHECK_LAUNCH();
+        }
     }
 }
 
 void launch_checksum(const Launch& L)
 {
     TimedLaunch tl("checksum", L.s);
-    k_checksum<<<grid_for(L), kCellThreads, 0, L.s>>>(L);
+    k_checksum<false><<<grid_for(L), kCellThreads, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

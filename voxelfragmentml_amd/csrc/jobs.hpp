@@ -143,7 +143,9 @@ inline uint32_t brick_count(uint32_t nx, uint32_t ny, uint32_t nz)
 {
     return ((nx + kBrickX - 1) / kBrickX) * ((ny + kBrickY - 1) / kBrickY) * ((nz + kBrickZ - 1) / kBrickZ);
 }
-void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide);
+// with_checksum: also sum every job's grid checksum into kStatChecksum
+void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_t total_btiles, bool any_wide,
+                             bool with_checksum);
 void launch_job_init(const Launch& L);     // zero stats, histogram and flood control of every job
 void launch_gather(const Launch& L, uint32_t* out);  // 16 words per job: stats (low 32 bits) + flood state
 void launch_checksum(const Launch& L);
