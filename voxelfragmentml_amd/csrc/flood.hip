@@ -31,11 +31,16 @@ namespace vf {
 
 namespace {
 
+// Flood key of a cell: 0 = EMPTY; otherwise (K << 2) | w with w = 1 + noise in
+// 1..3 the cell's node weight and K = (D << b) | seed index, or K = 0x3FFFFFFC
+// while unreached.  Keys compare by K first (the weight bits only break ties
+// between equal K, which never changes K), so the minimum over neighbours is
+// still the (distance, seed) minimum, and a cell's weight travels with it.
 constexpr uint32_t kEmptyKey = 0u;
-constexpr uint32_t kUnreached = 0xFFFFFFFFu;
-// tile values are key - 1: EMPTY becomes 0xFFFFFFFF and unreached 0xFFFFFFFE,
-// so a plain unsigned min over neighbours never selects either
-constexpr uint32_t kTileNone = 0xFFFFFFFEu;
+constexpr uint32_t kUnreachedKey = 0xFFFFFFF0u;  // | w
+// LDS tile values are the keys with EMPTY stored as 0xFFFFFFFF, so an unsigned
+// min over neighbours never selects an EMPTY or unreached (>= kUnreachedKey) cell
+constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
 // Halo z rows of 10 words (z = -1..8) at an odd stride, so that the row starts
 // of a wave's 64 lanes fall on distinct LDS banks.
 constexpr uint32_t kRowWords = 11;
@@ -60,9 +65,14 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
         J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 0u;
     }
     __syncthreads();
+    auto seed_key = [&](uint32_t s, size_t c) {
+        const uint32_t w = 1u + (J->noise_amp ? noise_delta(J->noise_seed, (uint32_t)c, J->noise_amp) : 0u);
+        return (((1u << b) | s) << 2) | w;
+    };
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
-        atomicMax(J->keys + ((size_t)q.x * J->ny + q.y) * J->nz + q.z, (1u << b) | s);
+        const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
+        atomicMax(J->keys + c, seed_key(s, c));
     }
     __syncthreads();
     const uint32_t nby = bdim(J->ny), nbz = bdim(J->nz);
@@ -70,7 +80,7 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
         const size_t c = ((size_t)q.x * J->ny + q.y) * J->nz + q.z;
-        if (J->keys[c] != ((1u << b) | s)) continue;
+        if (J->keys[c] != seed_key(s, c)) continue;
         J->occ[c] = 1;
         // bricks of the 3x3x3 neighbourhood: at most two brick indices per axis
         const uint32_t bx0 = (q.x ? q.x - 1 : 0) / kRelaxBrick, bx1 = min(q.x + 1, J->nx - 1) / kRelaxBrick;
@@ -148,8 +158,8 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
 #pragma unroll
             for (uint32_t k = 0; k < 5u; ++k) {
                 const uint32_t hx = h0 + k;
-                tile[hx * 10u * kRowWords + t1] = v1[k] - 1u;
-                if (p2 < 100u) tile[hx * 10u * kRowWords + t2] = v2[k] - 1u;
+                tile[hx * 10u * kRowWords + t1] = v1[k] ? v1[k] : kTileEmpty;
+                if (p2 < 100u) tile[hx * 10u * kRowWords + t2] = v2[k] ? v2[k] : kTileEmpty;
             }
         }
     }
@@ -160,15 +170,14 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t own = ((x + 1u) * 10u + (y + 1u)) * kRowWords;
     const uint32_t b = R.b;
     const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
-    uint32_t k[8], w[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) k[z] = tile[own + 1 + z];
+    // k: the row's keys (EMPTY as kTileEmpty); w: the weight bits of occupied cells (0 = EMPTY);
+    // wb: the weight added to K, already shifted into key position
+    uint32_t k[8], w[8], wb[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        const bool occupied = k[z] != 0xFFFFFFFFu;  // out-of-grid cells load as EMPTY
-        const uint32_t cell = (uint32_t)(org + orow + z);
-        w[z] = occupied ? (1u + (R.amp ? noise_delta(R.noise_seed, cell, R.amp) : 0u)) << b : 0u;
-        if (!occupied) k[z] = kTileNone;  // never updated, never a source
+        k[z] = tile[own + 1 + z];
+        w[z] = k[z] != kTileEmpty ? (k[z] & 3u) : 0u;  // out-of-grid cells load as EMPTY
+        wb[z] = w[z] << (b + 2u);
     }
     uint32_t changed = 0;  // bit z: cell z dropped in this visit
     bool overflow = false;
@@ -211,27 +220,31 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
 #pragma unroll
             for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]);
         }
-        // candidate m + w must stay below kTileNone; m > kTileNone - 1 - w flags a key overflow
+        // candidate (m's K + weight) with the cell's own weight bits; it must stay below
+        // kUnreachedKey, and m & ~3 >= kUnreachedKey - wb flags a key overflow
         uint32_t ch = 0;
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
-            if (w[z] && m[z] < kTileNone) {
-                if (m[z] > kTileNone - 1u - w[z]) overflow = true;
-                else if (m[z] + w[z] < k[z]) { k[z] = m[z] + w[z]; ch |= 1u << z; }
+            if (w[z] && m[z] < kUnreachedKey) {
+                const uint32_t mh = m[z] & ~3u;
+                if (mh >= kUnreachedKey - wb[z]) overflow = true;
+                else if (mh + wb[z] + w[z] < k[z]) { k[z] = mh + wb[z] + w[z]; ch |= 1u << z; }
             }
         }
         // z sweeps in registers: the row's own neighbours along z
 #pragma unroll
         for (int z = 1; z < 8; ++z) {
-            if (w[z] && k[z - 1] <= kTileNone - 1u - w[z] && k[z - 1] + w[z] < k[z]) {
-                k[z] = k[z - 1] + w[z];
+            const uint32_t mh = k[z - 1] & ~3u;
+            if (w[z] && k[z - 1] < kUnreachedKey && mh < kUnreachedKey - wb[z] && mh + wb[z] + w[z] < k[z]) {
+                k[z] = mh + wb[z] + w[z];
                 ch |= 1u << z;
             }
         }
 #pragma unroll
         for (int z = 6; z >= 0; --z) {
-            if (w[z] && k[z + 1] <= kTileNone - 1u - w[z] && k[z + 1] + w[z] < k[z]) {
-                k[z] = k[z + 1] + w[z];
+            const uint32_t mh = k[z + 1] & ~3u;
+            if (w[z] && k[z + 1] < kUnreachedKey && mh < kUnreachedKey - wb[z] && mh + wb[z] + w[z] < k[z]) {
+                k[z] = mh + wb[z] + w[z];
                 ch |= 1u << z;
             }
         }
@@ -253,7 +266,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     // write back the cells that dropped
 #pragma unroll
     for (int z = 0; z < 8; ++z)
-        if (changed & (1u << z)) kb[orow + z] = k[z] + 1u;
+        if (changed & (1u << z)) kb[orow + z] = k[z];
 
     // neighbour bricks that see a dropped cell: bit (dx+1)*9 + (dy+1)*3 + (dz+1)
     uint32_t dirs = 0;
@@ -437,21 +450,18 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
         const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J.V) return;
         const uint64_t o = load8b(J.occ, c0, J.V);
+        uint32_t key[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t w = 1u + (J.noise_amp ? noise_delta(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp) : 0u);
+            key[k] = ((o >> (8 * k)) & 0xFFu) ? (kUnreachedKey | w) : kEmptyKey;
+        }
         if (c0 + 8 <= J.V) {
-            uint4 a, b;
-            a.x = (o & 0xFFull) ? kUnreached : kEmptyKey;
-            a.y = (o & 0xFF00ull) ? kUnreached : kEmptyKey;
-            a.z = (o & 0xFF0000ull) ? kUnreached : kEmptyKey;
-            a.w = (o & 0xFF000000ull) ? kUnreached : kEmptyKey;
-            b.x = (o & 0xFF00000000ull) ? kUnreached : kEmptyKey;
-            b.y = (o & 0xFF0000000000ull) ? kUnreached : kEmptyKey;
-            b.z = (o & 0xFF000000000000ull) ? kUnreached : kEmptyKey;
-            b.w = (o & 0xFF00000000000000ull) ? kUnreached : kEmptyKey;
-            reinterpret_cast<uint4*>(J.keys + c0)[0] = a;
-            reinterpret_cast<uint4*>(J.keys + c0)[1] = b;
+            reinterpret_cast<uint4*>(J.keys + c0)[0] = make_uint4(key[0], key[1], key[2], key[3]);
+            reinterpret_cast<uint4*>(J.keys + c0)[1] = make_uint4(key[4], key[5], key[6], key[7]);
         } else {
             for (int k = 0; k < 8; ++k)
-                if (c0 + k < J.V) J.keys[c0 + k] = ((o >> (8 * k)) & 0xFFu) ? kUnreached : kEmptyKey;
+                if (c0 + k < J.V) J.keys[c0 + k] = key[k];
         }
     });
 }
@@ -484,8 +494,8 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
         uint16_t out[8];
         for (int k = 0; k < 8; ++k) {
             if (key[k] == kEmptyKey) out[k] = kEmpty;
-            else if (key[k] == kUnreached) out[k] = kFree;
-            else out[k] = lab[key[k] & mask];
+            else if (key[k] >= kUnreachedKey) out[k] = kFree;
+            else out[k] = lab[(key[k] >> 2) & mask];
         }
         store8(J.labels, c0, J.V, out);
     });

@@ -66,7 +66,7 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys, V: 0 = EMPTY, ~0 = unreached, else (distance << key_bits) | seed
+    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | weight, K = (distance << key_bits) | seed or 0x3FFFFFFC unreached
     unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
     uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty
