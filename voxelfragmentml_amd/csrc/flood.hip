@@ -172,13 +172,15 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
     // k: the row's keys (EMPTY as kTileEmpty); w: the weight bits of occupied cells (0 = EMPTY);
     // wb: the weight added to K, already shifted into key position
-    uint32_t k[8], w[8], wb[8];
+    uint32_t k[8];
+    uint32_t wmask = 0;  // 2 bits per cell: its weight (0 = EMPTY); out-of-grid cells load as EMPTY
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
         k[z] = tile[own + 1 + z];
-        w[z] = k[z] != kTileEmpty ? (k[z] & 3u) : 0u;  // out-of-grid cells load as EMPTY
-        wb[z] = w[z] << (b + 2u);
+        wmask |= (k[z] != kTileEmpty ? (k[z] & 3u) : 0u) << (2 * z);
     }
+    const uint32_t wsh = b + 2u;
+#define W_(z) ((wmask >> (2 * (z))) & 3u)
     uint32_t changed = 0;  // bit z: cell z dropped in this visit
     bool overflow = false;
     uint32_t sweeps = 0;
@@ -225,26 +227,27 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         uint32_t ch = 0;
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
-            if (w[z] && m[z] < kUnreachedKey) {
+            const uint32_t w = W_(z), wb = w << wsh;
+            if (w && m[z] < kUnreachedKey) {
                 const uint32_t mh = m[z] & ~3u;
-                if (mh >= kUnreachedKey - wb[z]) overflow = true;
-                else if (mh + wb[z] + w[z] < k[z]) { k[z] = mh + wb[z] + w[z]; ch |= 1u << z; }
+                if (mh >= kUnreachedKey - wb) overflow = true;
+                else if (mh + wb + w < k[z]) { k[z] = mh + wb + w; ch |= 1u << z; }
             }
         }
         // z sweeps in registers: the row's own neighbours along z
 #pragma unroll
         for (int z = 1; z < 8; ++z) {
-            const uint32_t mh = k[z - 1] & ~3u;
-            if (w[z] && k[z - 1] < kUnreachedKey && mh < kUnreachedKey - wb[z] && mh + wb[z] + w[z] < k[z]) {
-                k[z] = mh + wb[z] + w[z];
+            const uint32_t w = W_(z), wb = w << wsh, mh = k[z - 1] & ~3u;
+            if (w && k[z - 1] < kUnreachedKey && mh < kUnreachedKey - wb && mh + wb + w < k[z]) {
+                k[z] = mh + wb + w;
                 ch |= 1u << z;
             }
         }
 #pragma unroll
         for (int z = 6; z >= 0; --z) {
-            const uint32_t mh = k[z + 1] & ~3u;
-            if (w[z] && k[z + 1] < kUnreachedKey && mh < kUnreachedKey - wb[z] && mh + wb[z] + w[z] < k[z]) {
-                k[z] = mh + wb[z] + w[z];
+            const uint32_t w = W_(z), wb = w << wsh, mh = k[z + 1] & ~3u;
+            if (w && k[z + 1] < kUnreachedKey && mh < kUnreachedKey - wb && mh + wb + w < k[z]) {
+                k[z] = mh + wb + w;
                 ch |= 1u << z;
             }
         }
@@ -261,6 +264,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             break;
         }
     }
+#undef W_
     if (overflow) error = true;
 
     // write back the cells that dropped
