@@ -37,7 +37,7 @@ namespace {
 // between equal K, which never changes K), so the minimum over neighbours is
 // still the (distance, seed) minimum, and a cell's weight travels with it.
 constexpr uint32_t kEmptyKey = 0u;
-constexpr uint32_t kUnreachedKey = 0xFFFFFFF0u;  // | w
+constexpr uint32_t kUnreachedKey = kFloodUnreachedKey;  // | w
 // LDS tile values are the keys with EMPTY stored as 0xFFFFFFFF, so an unsigned
 // min over neighbours never selects an EMPTY or unreached (>= kUnreachedKey) cell
 constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
@@ -66,8 +66,7 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
     }
     __syncthreads();
     auto seed_key = [&](uint32_t s, size_t c) {
-        const uint32_t w = 1u + (J->noise_amp ? noise_delta(J->noise_seed, (uint32_t)c, J->noise_amp) : 0u);
-        return (((1u << b) | s) << 2) | w;
+        return (((1u << b) | s) << 2) | flood_weight(J->noise_seed, (uint32_t)c, J->noise_amp);
     };
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
@@ -457,8 +456,8 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
         uint32_t key[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const uint32_t w = 1u + (J.noise_amp ? noise_delta(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp) : 0u);
-            key[k] = ((o >> (8 * k)) & 0xFFu) ? (kUnreachedKey | w) : kEmptyKey;
+            key[k] = ((o >> (8 * k)) & 0xFFu) ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
+                                              : kEmptyKey;
         }
         if (c0 + 8 <= J.V) {
             reinterpret_cast<uint4*>(J.keys + c0)[0] = make_uint4(key[0], key[1], key[2], key[3]);

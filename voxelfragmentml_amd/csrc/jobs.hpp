@@ -78,6 +78,13 @@ struct JobDev {
     int naive_key64;        // naive packed keys need 64 bits
 };
 
+// Flood keys (see flood.hip): 0 = EMPTY, unreached occupied cells kFloodUnreachedKey | weight.
+constexpr uint32_t kFloodUnreachedKey = 0xFFFFFFF0u;
+__host__ __device__ inline uint32_t flood_weight(uint64_t noise_seed, uint32_t cell, uint32_t amp)
+{
+    return 1u + (amp ? noise_delta(noise_seed, cell, amp) : 0u);
+}
+
 uint32_t key_bits_for(uint32_t S);  // bits of the seed index in packed keys
 
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t njobs, uint32_t t)
