@@ -150,6 +150,21 @@ int vf_marching_cubes(vf_grid* g, uint32_t label /* VF_ALL_LABELS or one label *
 int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels,
                  uint32_t* vert_offsets, uint32_t* tri_offsets);
 
+/* Laplacian smoothing of the mesh computed by the last vf_marching_cubes call,
+ * in place (MarchingCubes::smoothSurface, MarchingCubes.cpp:395-401, 491-514;
+ * laplacianSmoothing-comp.glsl, finishLaplacianSmoothing-comp.glsl): first
+ * nonboundary_iterations steps moving the non-boundary vertices (faces whose
+ * three vertices are non-boundary) with nonboundary_weight, then
+ * boundary_iterations steps moving the boundary vertices (all faces) with
+ * boundary_weight.  A step moves a vertex to mix(p, average of its face
+ * neighbours, weight), the average taken over positions scaled by 10^4 and
+ * truncated to integers.  A vertex is a boundary vertex when any of its
+ * triangles is flagged.  The reference's iteration counts are
+ * (uint32_t)(max grid edge * 0.048f) with weights 0.9 and 0.2
+ * (FractureParameters.h:93-110). */
+int vf_mesh_smooth(vf_grid* g, uint32_t nonboundary_iterations, float nonboundary_weight, uint32_t boundary_iterations,
+                   float boundary_weight);
+
 /* ---- grid file formats ----------------------------------------------------- */
 /* RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims + {uint16 value,
  * uint32 count} runs.  Writes at most cap bytes; *size gets the full size. */

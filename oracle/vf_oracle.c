@@ -878,3 +878,61 @@ int vfo_rle_decode(const uint8_t* in, size_t n, uint32_t dims_out[3], uint16_t* 
     }
     return pos == V ? 0 : -4;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Laplacian smoothing.  Positions are scaled by 10^4 and truncated to int
+ * (ivec3(p * UINT_MULT)), neighbours are summed per face (each face adds its
+ * other two vertices to a vertex: laplacianSmoothing-comp.glsl:24-45), and
+ * the finish step moves a target vertex to mix(p, sum / count / 10^4, w)
+ * (finishLaplacianSmoothing-comp.glsl:18-30).  Integer sums make the result
+ * independent of accumulation order. */
+static void smooth_pass(vfo_mesh* m, const uint32_t* tri_g, const uint8_t* vflag, uint32_t iters, float weight,
+                        uint8_t target, int check)
+{
+    const uint32_t nv = m->nv, nt = m->nt;
+    int32_t* acc = (int32_t*)calloc((size_t)nv * 4 + 4, sizeof(int32_t));
+    for (uint32_t it = 0; it < iters; ++it) {
+        memset(acc, 0, (size_t)nv * 4 * sizeof(int32_t));
+        for (uint32_t t = 0; t < nt; ++t) {
+            const uint32_t* f = tri_g + 3 * (size_t)t;
+            if (check && !(vflag[f[0]] == target && vflag[f[1]] == target && vflag[f[2]] == target)) continue;
+            for (int i = 0; i < 3; ++i) {
+                const float* p = m->xyz + 3 * (size_t)f[i];
+                const int32_t vx = (int32_t)(p[0] * 10000.0f), vy = (int32_t)(p[1] * 10000.0f),
+                              vz = (int32_t)(p[2] * 10000.0f);
+                const uint32_t a = f[(i + 1) % 3], b = f[(i + 2) % 3];
+                acc[4 * (size_t)a + 0] += vx; acc[4 * (size_t)a + 1] += vy; acc[4 * (size_t)a + 2] += vz; acc[4 * (size_t)a + 3] += 1;
+                acc[4 * (size_t)b + 0] += vx; acc[4 * (size_t)b + 1] += vy; acc[4 * (size_t)b + 2] += vz; acc[4 * (size_t)b + 3] += 1;
+            }
+        }
+        for (uint32_t v = 0; v < nv; ++v) {
+            const int32_t* a = acc + 4 * (size_t)v;
+            if (vflag[v] != target || a[3] <= 0) continue;
+            float* p = m->xyz + 3 * (size_t)v;
+            for (int q = 0; q < 3; ++q) {
+                const float avg = (float)a[q] / (float)a[3] / 10000.0f;
+                p[q] = p[q] * (1.0f - weight) + avg * weight;  /* mix(p, avg, weight) */
+            }
+        }
+    }
+    free(acc);
+}
+
+void vfo_mesh_smooth(vfo_mesh* m, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight)
+{
+    const uint32_t nv = m->nv, nt = m->nt;
+    if (!nv || !nt) return;
+    uint32_t* tri_g = (uint32_t*)malloc((size_t)nt * 3 * sizeof(uint32_t));
+    uint8_t* vflag = (uint8_t*)calloc(nv, 1);
+    for (uint32_t l = 0; l < m->nl; ++l)
+        for (uint32_t t = m->toff[l]; t < m->toff[l + 1]; ++t)
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t g = m->voff[l] + m->tri[3 * (size_t)t + i];
+                tri_g[3 * (size_t)t + i] = g;
+                if (m->tri_flag[t]) vflag[g] = 1;
+            }
+    smooth_pass(m, tri_g, vflag, nb_iters, nb_weight, 0, 1);
+    smooth_pass(m, tri_g, vflag, b_iters, b_weight, 1, 0);
+    free(tri_g);
+    free(vflag);
+}

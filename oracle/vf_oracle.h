@@ -81,6 +81,14 @@ int vfo_marching_cubes(const uint16_t* grid, const uint32_t dims[3],
                        const float aabb_min[3], const float aabb_max[3],
                        uint32_t label /* 0xFFFFFFFF = all */, vfo_mesh* out);
 void vfo_mesh_free(vfo_mesh* m);
+/* Laplacian smoothing of every fragment mesh (MarchingCubes::smoothSurface,
+ * MarchingCubes.cpp:395-401, 491-514; laplacianSmoothing-comp.glsl,
+ * finishLaplacianSmoothing-comp.glsl): nb_iters Jacobi steps moving the
+ * non-boundary vertices (faces with three non-boundary vertices only) with
+ * weight nb_weight, then b_iters steps moving the boundary vertices (every
+ * face) with b_weight.  A vertex is a boundary vertex when any of its
+ * triangles is flagged.  Updates m->xyz in place (FP32). */
+void vfo_mesh_smooth(vfo_mesh* m, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight);
 uint64_t vfo_mc_packed(uint32_t config);
 
 /* ---- RLE grid format (RegularGrid::exportRLE) ----------------------------- */

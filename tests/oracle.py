@@ -52,6 +52,7 @@ def lib():
         "vfo_seed_uniform": (I, [P, P, U32, U64, I, P]),
         "vfo_marching_cubes": (I, [P, P, P, P, U32, C.POINTER(Mesh)]),
         "vfo_mesh_free": (None, [C.POINTER(Mesh)]),
+        "vfo_mesh_smooth": (None, [C.POINTER(Mesh), U32, C.c_float, U32, C.c_float]),
         "vfo_mc_packed": (U64, [U32]),
         "vfo_rle_encode": (C.c_size_t, [P, P, P, C.c_size_t]),
         "vfo_rle_decode": (I, [P, C.c_size_t, P, P, C.c_size_t]),
@@ -169,11 +170,14 @@ def seed_uniform(grid, n, seed, location=2):  # 2 = BOTH (Seeder::Location order
     return out
 
 
-def marching_cubes(grid, aabb_min, aabb_max, label=0xFFFFFFFF):
+def marching_cubes(grid, aabb_min, aabb_max, label=0xFFFFFFFF, smooth=None):
+    """smooth: optional (nonboundary_iterations, nonboundary_weight, boundary_iterations, boundary_weight)."""
     g = np.ascontiguousarray(grid, np.uint16)
     m = Mesh()
     lib().vfo_marching_cubes(_p(g), _p(_dims(g.shape)), _p(np.asarray(aabb_min, np.float32)),
                              _p(np.asarray(aabb_max, np.float32)), label, C.byref(m))
+    if smooth is not None:
+        lib().vfo_mesh_smooth(C.byref(m), int(smooth[0]), float(smooth[1]), int(smooth[2]), float(smooth[3]))
     nv, nt, nl = m.nv, m.nt, m.nl
 
     def arr(ptr, n, dt):

@@ -69,3 +69,23 @@ def test_mc_empty_grid(vf):
     g = vf.Grid((10, 10, 10), *BOX)
     m = g.marching_cubes()
     assert m["tri"].shape[0] == 0 and m["labels"].shape[0] == 0
+
+
+@pytest.mark.parametrize("dims,S,dist", [((64, 64, 64), 10, 0), ((50, 44, 58), 6, 1)])
+def test_mc_smoothing_matches_oracle(vf, dims, S, dist):
+    xyz, tri = vf.vessel_mesh(5, *vf.vessel_segments(12000))
+    g = vf.Grid(dims, *BOX)
+    g.voxelize(xyz, tri)
+    seeds = O.seed_uniform(g.read(), S, 7)
+    g.fracture_naive(seeds, dist)
+    g.detect_boundaries(1)
+    grid = g.read()
+    it = int(np.float32(max(dims)) * np.float32(0.048))  # the reference's default iteration count
+    sm = (it, 0.9, it, 0.2)
+    g.marching_cubes()
+    g.smooth_mesh(*sm)
+    m = g.read_mesh()
+    r = O.marching_cubes(grid, *BOX, smooth=sm)
+    np.testing.assert_array_equal(m["tri"], r["tri"])
+    np.testing.assert_array_equal(m["xyz"], r["xyz"])
+    g.close()

@@ -282,3 +282,82 @@ def test_rle_roundtrip_random():
     np.testing.assert_array_equal(O.rle_decode(enc), g)
     runs = (len(enc) - 12) // 6
     assert runs == 1 + np.count_nonzero(np.diff(g.ravel()))
+
+
+# ------------------------------------------------------------ mesh smoothing
+def _box_grid(n=12, lab=2, flag=False):
+    g = np.zeros((n, n, n), np.uint16)
+    g[3:n - 3, 3:n - 3, 3:n - 3] = lab
+    if flag:
+        g[3:n - 3, 3:n - 3, 3:n - 6] |= 0x8000
+    return g
+
+
+BOX1 = ([-1.0] * 3, [1.0] * 3)
+
+
+def test_smoothing_identity_cases():
+    g = _box_grid()
+    m0 = O.marching_cubes(g, *BOX1)
+    for sm in [(0, 0.9, 0, 0.2), (3, 0.0, 3, 0.0)]:
+        m = O.marching_cubes(g, *BOX1, smooth=sm)
+        np.testing.assert_array_equal(m["xyz"], m0["xyz"])
+
+
+def test_smoothing_one_full_step_is_the_truncated_neighbour_average():
+    # weight 1, one non-boundary step, no flags: p' = average over faces of the other two
+    # vertices' int(p * 1e4), / count / 1e4, in FP32 -- computed here independently
+    g = _box_grid()
+    m0 = O.marching_cubes(g, *BOX1)
+    m1 = O.marching_cubes(g, *BOX1, smooth=(1, 1.0, 0, 0.0))
+    xyz = m0["xyz"]
+    q = np.trunc(xyz.astype(np.float32) * np.float32(10000.0)).astype(np.int64)
+    acc = np.zeros((len(xyz), 3), np.int64)
+    cnt = np.zeros(len(xyz), np.int64)
+    for t in m0["tri"]:
+        for i in range(3):
+            for j in range(3):
+                if i != j:
+                    acc[t[j]] += q[t[i]]
+                    cnt[t[j]] += 1
+    avg = (acc.astype(np.float32) / cnt[:, None].astype(np.float32)) / np.float32(10000.0)
+    expect = xyz * np.float32(0.0) + avg * np.float32(1.0)
+    np.testing.assert_array_equal(m1["xyz"], expect.astype(np.float32))
+
+
+def test_smoothing_lowers_edge_energy_and_pulls_corners_in():
+    # (the table's fixed quad diagonals make the per-face weights orientation dependent, so
+    # a mirror-symmetric box does not stay exactly symmetric; energy and corners must move)
+    g = _box_grid(14)
+    m0 = O.marching_cubes(g, *BOX1)
+
+    def energy(m):
+        x = m["xyz"].astype(np.float64)
+        t = m["tri"]
+        return sum(((x[t[:, i]] - x[t[:, (i + 1) % 3]]) ** 2).sum() for i in range(3))
+
+    c = (m0["xyz"].min(axis=0).astype(np.float64) + m0["xyz"].max(axis=0)) / 2
+    r0 = np.abs(m0["xyz"] - c).max()
+    prev_e, prev_r = energy(m0), r0
+    for it in (1, 2, 4, 8):
+        m = O.marching_cubes(g, *BOX1, smooth=(it, 0.9, 0, 0.2))
+        e, r = energy(m), np.abs(m["xyz"] - c).max()
+        assert e < prev_e and r <= prev_r
+        prev_e, prev_r = e, r
+    assert prev_r < r0
+
+
+def test_smoothing_moves_only_the_target_vertex_class():
+    g = _box_grid(14, flag=True)
+    m0 = O.marching_cubes(g, *BOX1)
+    vflag = np.zeros(len(m0["xyz"]), bool)
+    for t, f in zip(m0["tri"], m0["flag"]):
+        if f:
+            vflag[t] = True
+    assert vflag.any() and (~vflag).any()
+    nb = O.marching_cubes(g, *BOX1, smooth=(2, 0.9, 0, 0.2))
+    moved = np.any(nb["xyz"] != m0["xyz"], axis=1)
+    assert not np.any(moved & vflag) and np.any(moved & ~vflag)
+    b = O.marching_cubes(g, *BOX1, smooth=(0, 0.9, 2, 0.5))
+    moved = np.any(b["xyz"] != m0["xyz"], axis=1)
+    assert not np.any(moved & ~vflag) and np.any(moved & vflag)

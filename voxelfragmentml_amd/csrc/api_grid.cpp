@@ -592,6 +592,15 @@ int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint1
     });
 }
 
+int vf_mesh_smooth(vf_grid* g, uint32_t nonboundary_iterations, float nonboundary_weight, uint32_t boundary_iterations,
+                   float boundary_weight)
+{
+    return guard([&] {
+        checked(g);
+        mc_smooth(g->mc, g->stream, nonboundary_iterations, nonboundary_weight, boundary_iterations, boundary_weight);
+    });
+}
+
 int vf_grid_export_rle(vf_grid* g, uint8_t* out, size_t cap, size_t* size)
 {
     // RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims, then maximal runs

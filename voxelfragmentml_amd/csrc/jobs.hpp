@@ -220,6 +220,8 @@ void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t di
             const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info);
 void mc_read(McScratch& W, hipStream_t s, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
              uint32_t* toff);
+// Laplacian smoothing of the last mesh in W, in place (non-boundary pass, then boundary pass)
+void mc_smooth(McScratch& W, hipStream_t s, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight);
 
 // scan (scan.hip): exclusive prefix sums, graph-capturable, no host sync
 size_t scan_tmp_bytes(uint64_t n);

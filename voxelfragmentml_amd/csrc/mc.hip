@@ -329,7 +329,143 @@ __global__ void k_mc_offsets(const unsigned long long* o, uint32_t nl, uint32_t 
 
 bool g_table_loaded = false;
 
+// ---------------------------------------------------------------- smoothing
+// Laplacian smoothing of the fragment meshes (MarchingCubes::smoothSurface,
+// MarchingCubes.cpp:491-514; laplacianSmoothing-comp.glsl:24-45,
+// finishLaplacianSmoothing-comp.glsl:18-30): a face adds each of its vertices,
+// scaled by 10^4 and truncated to int, to its other two vertices; a target
+// vertex then moves to mix(p, sum / count / 10^4, weight).  The sums are
+// gathered per vertex over a vertex -> triangle adjacency, in integers, so the
+// result does not depend on order.  A vertex is a boundary vertex when any of
+// its triangles is flagged.
+struct SmoothDev {
+    uint32_t nv, nt, nl;
+    const uint32_t* tri;                 // local indices
+    const uint8_t* tri_flag;
+    const unsigned long long* voff;      // per label
+    const unsigned long long* toff;
+    uint32_t* tri_g;                     // 3 * nt global vertex indices
+    uint32_t* vflag;                     // nv
+    uint32_t* deg;                       // nv incidence counts, then fill cursors
+    const unsigned long long* adj_off;   // nv + 1
+    uint32_t* adj;                       // 3 * nt triangle ids
+};
+
+__global__ void k_smooth_prep(SmoothDev D)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < D.nt; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = D.nl;  // label of triangle t: largest l with toff[l] <= t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (D.toff[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t base = (uint32_t)D.voff[lo];
+        for (int i = 0; i < 3; ++i) {
+            const uint32_t g = base + D.tri[3 * (size_t)t + i];
+            D.tri_g[3 * (size_t)t + i] = g;
+            atomicAdd(D.deg + g, 1u);
+            if (D.tri_flag[t]) atomicOr(D.vflag + g, 1u);
+        }
+    }
+}
+
+__global__ void k_smooth_fill(SmoothDev D)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < D.nt; t += gridDim.x * blockDim.x)
+        for (int i = 0; i < 3; ++i) {
+            const uint32_t g = D.tri_g[3 * (size_t)t + i];
+            const uint32_t slot = atomicAdd(D.deg + g, 1u);  // deg was reset to 0 after the scan
+            D.adj[D.adj_off[g] + slot] = t;
+        }
+}
+
+__global__ void k_smooth_step(SmoothDev D, const float* src, float* dst, uint32_t target, int check, float weight)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < D.nv; v += gridDim.x * blockDim.x) {
+        const float* p = src + 3 * (size_t)v;
+        float* o = dst + 3 * (size_t)v;
+        float px = p[0], py = p[1], pz = p[2];
+        if (D.vflag[v] == target) {
+            int32_t sx = 0, sy = 0, sz = 0, cnt = 0;
+            for (unsigned long long e = D.adj_off[v]; e < D.adj_off[v + 1]; ++e) {
+                const uint32_t* f = D.tri_g + 3 * (size_t)D.adj[e];
+                if (check && !(D.vflag[f[0]] == target && D.vflag[f[1]] == target && D.vflag[f[2]] == target)) continue;
+                for (int i = 0; i < 3; ++i) {
+                    if (f[i] == v) continue;
+                    const float* q = src + 3 * (size_t)f[i];
+                    sx += (int32_t)(q[0] * 10000.0f);
+                    sy += (int32_t)(q[1] * 10000.0f);
+                    sz += (int32_t)(q[2] * 10000.0f);
+                    ++cnt;
+                }
+            }
+            if (cnt > 0) {
+                const float ax = (float)sx / (float)cnt / 10000.0f, ay = (float)sy / (float)cnt / 10000.0f,
+                            az = (float)sz / (float)cnt / 10000.0f;
+                px = px * (1.0f - weight) + ax * weight;
+                py = py * (1.0f - weight) + ay * weight;
+                pz = pz * (1.0f - weight) + az * weight;
+            }
+        }
+        o[0] = px;
+        o[1] = py;
+        o[2] = pz;
+    }
+}
+
 }  // namespace
+
+void mc_smooth(McScratch& W, hipStream_t s, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight)
+{
+    if (!W.valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes mesh to smooth");
+    if (!(nb_weight >= 0.0f && nb_weight <= 1.0f && b_weight >= 0.0f && b_weight <= 1.0f))
+        fail(VF_ERR_INVALID_ARG, "smoothing weights must lie in [0, 1]");
+    if (W.nv == 0 || W.nt == 0 || (nb_iters == 0 && b_iters == 0)) return;
+    auto& B = W.buf;
+    SmoothDev D{};
+    D.nv = W.nv;
+    D.nt = W.nt;
+    D.nl = W.nl;
+    D.tri = B[12].as<uint32_t>();
+    D.tri_flag = B[8].as<uint8_t>();
+    D.voff = B[11].as<unsigned long long>();
+    D.toff = D.voff + (W.nl + 1);
+    B[14].ensure((size_t)W.nt * 12 + 64);
+    B[15].ensure((size_t)W.nv * 8 + 64);         // vflag, deg
+    B[13].ensure(std::max(scan_tmp_bytes(W.nv), (size_t)W.nt * 12) + 64);  // scan scratch, then adjacency
+    D.tri_g = B[14].as<uint32_t>();
+    D.vflag = B[15].as<uint32_t>();
+    D.deg = D.vflag + W.nv;
+    DevBuf& offs = W.buf[5];                     // reuse: nv + 1 adjacency offsets
+    offs.ensure(((size_t)W.nv + 1) * 8 + 64);
+    DevBuf& tmp = W.buf[2];                      // ping-pong positions
+    tmp.ensure((size_t)W.nv * 12 + 64);
+    VF_HIP(hipMemsetAsync(D.vflag, 0, (size_t)W.nv * 8, s));
+    const uint32_t gt = stride_blocks(W.nt, 256, 8192), gv = stride_blocks(W.nv, 256, 8192);
+    k_smooth_prep<<<gt, 256, 0, s>>>(D);
+    VF_CHECK_LAUNCH();
+    scan_exclusive_u32_to_u64(D.deg, offs.as<unsigned long long>(), W.nv, B[13].p, s);
+    VF_HIP(hipMemsetAsync(D.deg, 0, (size_t)W.nv * 4, s));
+    D.adj_off = offs.as<unsigned long long>();
+    D.adj = B[13].as<uint32_t>();                // the scan scratch is free again
+    k_smooth_fill<<<gt, 256, 0, s>>>(D);
+    VF_CHECK_LAUNCH();
+    float* a = B[10].as<float>();
+    float* b = tmp.as<float>();
+    auto pass = [&](uint32_t iters, float weight, uint32_t target, int check) {
+        for (uint32_t i = 0; i < iters; ++i) {
+            TimedLaunch tl("mc_smooth", s);
+            k_smooth_step<<<gv, 256, 0, s>>>(D, a, b, target, check, weight);
+            VF_CHECK_LAUNCH();
+            std::swap(a, b);
+        }
+    };
+    pass(nb_iters, nb_weight, 0, 1);
+    pass(b_iters, b_weight, 1, 0);
+    if (a != B[10].as<float>()) VF_HIP(hipMemcpyAsync(B[10].p, a, (size_t)W.nv * 12, hipMemcpyDeviceToDevice, s));
+    VF_HIP(hipStreamSynchronize(s));
+}
+
 
 void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
             const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info)

@@ -85,6 +85,7 @@ def lib():
         "vf_histogram": (I, [P, P, U32]),
         "vf_marching_cubes": (I, [P, U32, C.POINTER(MeshInfo)]),
         "vf_mesh_read": (I, [P, P, P, P, P, P, P]),
+        "vf_mesh_smooth": (I, [P, U32, F, U32, F]),
         "vf_grid_export_rle": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "vf_grid_import_rle": (I, [P, P, C.c_size_t]),
         "vf_vessel_mesh": (I, [U64, U32, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
@@ -244,9 +245,28 @@ class Grid:
         info = MeshInfo()
         check(lib().vf_marching_cubes(self._h, label, C.byref(info)))
         nv, nt, nl = info.num_vertices, info.num_triangles, info.num_labels
+        self._mesh_sizes = (nv, nt, nl)
         xyz = np.empty((nv, 3), np.float32)
         tri = np.empty((nt, 3), np.uint32)
         flag = np.empty(nt, np.uint8)
+        labels = np.empty(nl, np.uint16)
+        voff = np.empty(nl + 1, np.uint32)
+        toff = np.empty(nl + 1, np.uint32)
+        check(lib().vf_mesh_read(self._h, _ptr(xyz), _ptr(tri), _ptr(flag), _ptr(labels), _ptr(voff), _ptr(toff)))
+        return {"xyz": xyz, "tri": tri, "flag": flag, "labels": labels, "voff": voff, "toff": toff}
+
+    def smooth_mesh(self, nonboundary_iterations: int, nonboundary_weight: float, boundary_iterations: int,
+                    boundary_weight: float):
+        """Laplacian smoothing of the last marching_cubes mesh, in place (read it again afterwards)."""
+        check(lib().vf_mesh_smooth(self._h, nonboundary_iterations, nonboundary_weight, boundary_iterations,
+                                   boundary_weight))
+
+    def read_mesh(self) -> dict:
+        """The current mesh of the last marching_cubes call (after any smoothing)."""
+        info_nv, info_nt, nl = self._mesh_sizes
+        xyz = np.empty((info_nv, 3), np.float32)
+        tri = np.empty((info_nt, 3), np.uint32)
+        flag = np.empty(info_nt, np.uint8)
         labels = np.empty(nl, np.uint16)
         voff = np.empty(nl + 1, np.uint32)
         toff = np.empty(nl + 1, np.uint32)
