@@ -165,6 +165,17 @@ int vf_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint1
 int vf_mesh_smooth(vf_grid* g, uint32_t nonboundary_iterations, float nonboundary_weight, uint32_t boundary_iterations,
                    float boundary_weight);
 
+/* One fragment mesh of the last vf_marching_cubes call (index into its label
+ * list) in the reference's binary mesh format, CADModel::saveBinary
+ * (MF/Source/Graphics/Core/CADModel.cpp:839-855): uint32 vertex count, then per
+ * vertex the 64-byte Model3D::VertexGPUData {position, pad, normal, pad,
+ * texcoord, pad2, tangent, pad} (Model3D.h:37-50), uint32 face count, then per
+ * face the 16-byte FaceGPUData {3 vertex indices, component id}.  As in the
+ * reference's dataset export (endInsertionBatch without computeMeshData,
+ * CADModel.cpp:66-82) only positions are set; the other fields are zero.
+ * Writes at most cap bytes; *size gets the full size. */
+int vf_mesh_export_binm(vf_grid* g, uint32_t label_index, uint8_t* out, size_t cap, size_t* size);
+
 /* ---- grid file formats ----------------------------------------------------- */
 /* RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims + {uint16 value,
  * uint32 count} runs.  Writes at most cap bytes; *size gets the full size. */

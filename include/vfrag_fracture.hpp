@@ -91,6 +91,9 @@ struct FractureParameters {
     int _seed = 80;
     int _marchingCubesSubdivisions = 1;
     ivec3 _voxelizationSize = ivec3(128);
+    // marching-cubes smoothing: iterations = (unsigned)(max _voxelizationSize edge * factor)
+    float _boundaryMCWeight = 0.2f, _boundaryMCIterations = 0.048f;
+    float _nonBoundaryMCWeight = 0.9f, _nonBoundaryMCIterations = 0.048f;
 
     // Flood boundary noise: node weights 1 + hash(noise seed, cell) in
     // [0, amplitude].  Not in the reference (amplitude 0 = plain BFS order).
@@ -211,17 +214,51 @@ public:
 
     // Marching cubes over every label (RegularGrid::toTriangleMesh,
     // RegularGrid.cpp:443-486): one mesh per label, FP32 world coordinates.
+    // With parameters, the meshes are smoothed as MarchingCubes.cpp:368-401
+    // does: a non-boundary pass, then a boundary pass, each of
+    // (unsigned)(max _voxelizationSize edge * iteration factor) steps.
     struct LabelMesh {
         uint16_t label;
         std::vector<float> xyz;
         std::vector<uint32_t> tri;
         std::vector<uint8_t> boundary;  // per triangle: the boundary bit of its cell
     };
-    std::vector<LabelMesh> toTriangleMesh()
+    std::vector<LabelMesh> toTriangleMesh() { return triangulate(nullptr); }
+    std::vector<LabelMesh> toTriangleMesh(const FractureParameters& params) { return triangulate(&params); }
+
+    // One mesh of the last toTriangleMesh call (index into its list) in the
+    // layout CADModel::saveBinary writes (CADModel.cpp:839-855), as the
+    // reference's dataset export does: positions set, other vertex fields zero.
+    std::vector<uint8_t> exportBinaryMesh(uint32_t meshIndex)
+    {
+        size_t n = 0;
+        check(vf_mesh_export_binm(g_, meshIndex, nullptr, 0, &n));
+        std::vector<uint8_t> out(n);
+        check(vf_mesh_export_binm(g_, meshIndex, out.data(), n, &n));
+        return out;
+    }
+
+    vf_grid* handle()
+    {
+        sync_device();
+        return g_;
+    }
+    // the device owns the grid again (after device work through handle())
+    void invalidateHost() { host_valid_ = false; }
+
+private:
+    std::vector<LabelMesh> triangulate(const FractureParameters* params)
     {
         sync_device();
         vf_mesh_info info{};
         check(vf_marching_cubes(g_, VF_ALL_LABELS, &info));
+        if (params) {
+            const ivec3& v = params->_voxelizationSize;
+            const unsigned maxVoxels = (unsigned)std::max(v.x, std::max(v.y, v.z));
+            check(vf_mesh_smooth(g_, (unsigned)(maxVoxels * params->_nonBoundaryMCIterations),
+                                 params->_nonBoundaryMCWeight,
+                                 (unsigned)(maxVoxels * params->_boundaryMCIterations), params->_boundaryMCWeight));
+        }
         std::vector<float> xyz(3ull * info.num_vertices);
         std::vector<uint32_t> tri(3ull * info.num_triangles);
         std::vector<uint8_t> flag(info.num_triangles);
@@ -238,15 +275,6 @@ public:
         return out;
     }
 
-    vf_grid* handle()
-    {
-        sync_device();
-        return g_;
-    }
-    // the device owns the grid again (after device work through handle())
-    void invalidateHost() { host_valid_ = false; }
-
-private:
     size_t index(int x, int y, int z) const { return ((size_t)x * dims_.y + (size_t)y) * dims_.z + (size_t)z; }
     void pull()
     {

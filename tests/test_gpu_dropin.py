@@ -51,9 +51,20 @@ def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_se
     ref_hist = np.array([(v, counts[v]) for v in range(2, 65536) if counts[v]], np.uint32).reshape(-1, 2)
     np.testing.assert_array_equal(hist, ref_hist)
 
-    m = O.marching_cubes(ref, *BOX)
+    it = int(np.float32(edge) * np.float32(0.048))  # FractureParameters' default MC smoothing
+    m = O.marching_cubes(ref, *BOX, smooth=(it, 0.9, it, 0.2))
     ref_mc = np.stack([m["labels"].astype(np.uint32), np.diff(m["voff"]), np.diff(m["toff"])], axis=1)
     np.testing.assert_array_equal(mc, ref_mc)
+    for i, label in enumerate(m["labels"]):
+        data = (tmp_path / f"scene.{label}.binm").read_bytes()
+        v0, v1, t0, t1 = m["voff"][i], m["voff"][i + 1], m["toff"][i], m["toff"][i + 1]
+        nv = int(np.frombuffer(data, np.uint32, 1)[0])
+        assert nv == v1 - v0
+        vert = np.frombuffer(data, np.float32, 16 * nv, 4).reshape(nv, 16)
+        np.testing.assert_array_equal(vert[:, :3], m["xyz"][v0:v1])
+        nt = int(np.frombuffer(data, np.uint32, 1, 4 + 64 * nv)[0])
+        face = np.frombuffer(data, np.uint32, 4 * nt, 8 + 64 * nv).reshape(nt, 4)
+        np.testing.assert_array_equal(face[:, :3], m["tri"][t0:t1])
 
 
 def test_tuner_changes_shapes_not_results(vf, tmp_path):

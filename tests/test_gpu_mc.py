@@ -89,3 +89,42 @@ def test_mc_smoothing_matches_oracle(vf, dims, S, dist):
     np.testing.assert_array_equal(m["tri"], r["tri"])
     np.testing.assert_array_equal(m["xyz"], r["xyz"])
     g.close()
+
+
+def decode_binm(data):
+    """The layout CADModel::saveBinary writes (CADModel.cpp:839-855): u32 vertex count, 64-byte
+    vertices (position first), u32 face count, 16-byte faces (3 indices, component id)."""
+    nv = int(np.frombuffer(data, np.uint32, 1, 0)[0])
+    vert = np.frombuffer(data, np.float32, 16 * nv, 4).reshape(nv, 16)
+    o = 4 + 64 * nv
+    nt = int(np.frombuffer(data, np.uint32, 1, o)[0])
+    face = np.frombuffer(data, np.uint32, 4 * nt, o + 4).reshape(nt, 4)
+    assert len(data) == o + 4 + 16 * nt
+    return vert, face
+
+
+def test_mc_binm_export(vf):
+    xyz, tri = vf.vessel_mesh(3, *vf.vessel_segments(12000))
+    dims = (48, 40, 56)
+    g = vf.Grid(dims, *BOX)
+    g.voxelize(xyz, tri)
+    seeds = O.seed_uniform(g.read(), 5, 9)
+    g.fracture_flood(seeds, 0)
+    g.detect_boundaries(1)
+    grid = g.read()
+    it = int(np.float32(max(dims)) * np.float32(0.048))
+    sm = (it, 0.9, it, 0.2)
+    g.marching_cubes()
+    g.smooth_mesh(*sm)
+    r = O.marching_cubes(grid, *BOX, smooth=sm)
+    assert len(r["labels"]) == 5
+    for i in range(len(r["labels"])):
+        vert, face = decode_binm(g.export_binm(i))
+        v0, v1, t0, t1 = r["voff"][i], r["voff"][i + 1], r["toff"][i], r["toff"][i + 1]
+        np.testing.assert_array_equal(vert[:, :3], r["xyz"][v0:v1])
+        assert not vert[:, 3:].any()  # dataset export: normal, texcoord, tangent and padding stay zero
+        np.testing.assert_array_equal(face[:, :3], r["tri"][t0:t1])
+        assert not face[:, 3].any()
+    with pytest.raises(vf.VfError):
+        g.export_binm(5)
+    g.close()

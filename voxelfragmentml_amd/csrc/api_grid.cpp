@@ -601,6 +601,31 @@ int vf_mesh_smooth(vf_grid* g, uint32_t nonboundary_iterations, float nonboundar
     });
 }
 
+int vf_mesh_export_binm(vf_grid* g, uint32_t label_index, uint8_t* out, size_t cap, size_t* size)
+{
+    return guard([&] {
+        checked(g);
+        if (!g->mc.valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes result: call vf_marching_cubes first");
+        const uint32_t nl = g->mc.nl;
+        if (label_index >= nl) fail(VF_ERR_INVALID_ARG, "label index %u out of range (%u labels)", label_index, nl);
+        std::vector<uint32_t> voff(nl + 1), toff(nl + 1);
+        grid_mesh_read(g, nullptr, nullptr, nullptr, nullptr, voff.data(), toff.data());
+        const uint32_t nv = voff[label_index + 1] - voff[label_index], nt = toff[label_index + 1] - toff[label_index];
+        const size_t total = 4 + (size_t)nv * 64 + 4 + (size_t)nt * 16;
+        if (size) *size = total;
+        if (!out || cap < total) return;
+        std::vector<float> xyz(3ull * nv);
+        std::vector<uint32_t> tri(3ull * nt);
+        grid_mesh_slice(g, voff[label_index], nv, toff[label_index], nt, xyz.data(), tri.data());
+        std::memset(out, 0, total);
+        std::memcpy(out, &nv, 4);
+        for (uint32_t v = 0; v < nv; ++v) std::memcpy(out + 4 + (size_t)v * 64, &xyz[3ull * v], 12);
+        uint8_t* f = out + 4 + (size_t)nv * 64;
+        std::memcpy(f, &nt, 4);
+        for (uint32_t t = 0; t < nt; ++t) std::memcpy(f + 4 + (size_t)t * 16, &tri[3ull * t], 12);
+    });
+}
+
 int vf_grid_export_rle(vf_grid* g, uint8_t* out, size_t cap, size_t* size)
 {
     // RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims, then maximal runs

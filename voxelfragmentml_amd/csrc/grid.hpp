@@ -40,5 +40,7 @@ void check_device_error(vf_grid* g, const char* stage);
 void grid_marching_cubes(vf_grid* g, uint32_t label, vf_mesh_info* info);
 void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
                     uint32_t* toff);
+// vertices [v0, v0 + nv) and triangles [t0, t0 + nt) of the last mesh
+void grid_mesh_slice(vf_grid* g, uint32_t v0, uint32_t nv, uint32_t t0, uint32_t nt, float* xyz, uint32_t* tri);
 
 }  // namespace vf

@@ -621,4 +621,13 @@ void grid_mesh_read(vf_grid* g, float* xyz, uint32_t* tri, uint8_t* tri_flag, ui
     mc_read(g->mc, g->stream, xyz, tri, tri_flag, labels, voff, toff);
 }
 
+void grid_mesh_slice(vf_grid* g, uint32_t v0, uint32_t nv, uint32_t t0, uint32_t nt, float* xyz, uint32_t* tri)
+{
+    auto& B = g->mc.buf;
+    if ((uint64_t)v0 + nv > g->mc.nv || (uint64_t)t0 + nt > g->mc.nt) fail(VF_ERR_INVALID_ARG, "mesh slice out of range");
+    if (nv) VF_HIP(hipMemcpyAsync(xyz, B[10].as<float>() + 3ull * v0, (size_t)nv * 12, hipMemcpyDeviceToHost, g->stream));
+    if (nt) VF_HIP(hipMemcpyAsync(tri, B[12].as<uint32_t>() + 3ull * t0, (size_t)nt * 12, hipMemcpyDeviceToHost, g->stream));
+    VF_HIP(hipStreamSynchronize(g->stream));
+}
+
 }  // namespace vf

@@ -9,7 +9,9 @@
 // usage: fracture_scene naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX
 // writes OUT_PREFIX.grid (uint16 cells after detectBoundaries), .seeds
 // (uint32 x,y,z,label), .hist (uint32 label,count pairs) and .mc (uint32
-// label, vertices, triangles per fragment mesh); prints a one-line summary.
+// label, vertices, triangles per fragment mesh), and OUT_PREFIX.LABEL.binm per
+// fragment (the smoothed mesh, CADModel::saveBinary layout); prints a one-line
+// summary.
 #define VFRAG_REFERENCE_NAMES
 #include "vfrag_fracture.hpp"
 
@@ -93,14 +95,18 @@ int main(int argc, char** argv)
             hist.push_back(kv.second);
         }
         dump(out + ".hist", hist.data(), hist.size());
-        const std::vector<RegularGrid::LabelMesh> meshes = grid.toTriangleMesh();
+        const std::vector<RegularGrid::LabelMesh> meshes = grid.toTriangleMesh(params);
         std::vector<uint32_t> mc;
         size_t tris = 0;
-        for (const auto& m : meshes) {
+        for (uint32_t i = 0; i < meshes.size(); ++i) {
+            const auto& m = meshes[i];
             mc.push_back(m.label);
             mc.push_back((uint32_t)(m.xyz.size() / 3));
             mc.push_back((uint32_t)(m.tri.size() / 3));
             tris += m.tri.size() / 3;
+            // CADModel::save with BINARY_MESH (CADModel.cpp:372-381)
+            const std::vector<uint8_t> binm = grid.exportBinaryMesh(i);
+            dump(out + "." + std::to_string(m.label) + ".binm", binm.data(), binm.size());
         }
         dump(out + ".mc", mc.data(), mc.size());
         grid.undoMask();

@@ -86,6 +86,7 @@ def lib():
         "vf_marching_cubes": (I, [P, U32, C.POINTER(MeshInfo)]),
         "vf_mesh_read": (I, [P, P, P, P, P, P, P]),
         "vf_mesh_smooth": (I, [P, U32, F, U32, F]),
+        "vf_mesh_export_binm": (I, [P, U32, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "vf_grid_export_rle": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "vf_grid_import_rle": (I, [P, P, C.c_size_t]),
         "vf_vessel_mesh": (I, [U64, U32, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
@@ -272,6 +273,14 @@ class Grid:
         toff = np.empty(nl + 1, np.uint32)
         check(lib().vf_mesh_read(self._h, _ptr(xyz), _ptr(tri), _ptr(flag), _ptr(labels), _ptr(voff), _ptr(toff)))
         return {"xyz": xyz, "tri": tri, "flag": flag, "labels": labels, "voff": voff, "toff": toff}
+
+    def export_binm(self, label_index: int) -> bytes:
+        """One fragment mesh of the last marching_cubes call in the reference's .binm layout."""
+        size = C.c_size_t()
+        check(lib().vf_mesh_export_binm(self._h, label_index, None, 0, C.byref(size)))
+        buf = np.empty(size.value, np.uint8)
+        check(lib().vf_mesh_export_binm(self._h, label_index, _ptr(buf), buf.size, C.byref(size)))
+        return buf.tobytes()
 
     def export_rle(self) -> bytes:
         size = C.c_size_t()
