@@ -106,6 +106,26 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
 /* Seeder::uniform (MF/Source/Fracturer/Seeder.cpp:154-208), counter-based: S
  * distinct candidate cells drawn uniformly, in (x,y,z) order, labels 2..S+1. */
 int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed* out);
+/* Seeder::nearSeeds (Seeder.cpp:49-113), counter-based: impacts times, pick a
+ * fragment seed and a count of 1..pending-1 new seeds (of num_seeds in all),
+ * then draw cells around it by the biased sum of `spreading` uniform offsets
+ * per axis (RandomUtilities::getBiasedRandomInt), wrapped around the grid, and
+ * keep those within min(dims)/2 of it that are occupied, on the surface
+ * (RegularGrid::isBoundary) and new.  out (F + num_seeds entries) receives
+ * the F fragment seeds, then the new seeds in (x,y,z) order labelled after the
+ * last fragment's label; *count the number written.  spreading 1..64 and at
+ * most the smallest grid edge; F >= 1.  An impact that finds too few cells in
+ * 10^6 draws fails with VF_ERR_INVALID_ARG (the reference loops forever). */
+int vf_seed_near(vf_grid* g, const vf_seed* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
+                 uint32_t spreading, uint64_t seed, vf_seed* out, uint32_t* count);
+/* Seeder::mergeSeeds (Seeder.cpp:115-152), host only: each of seeds[0..N)
+ * takes the label of its nearest fragment seed (distance VF_*, first on ties)
+ * with that fragment's running sub-seed id (1, 2, ...) in bits 8..15.
+ * Fragment labels must fit 8 bits; more than 255 sub-seeds on one fragment is
+ * VF_ERR_TOO_MANY_SEEDS (the reference's uint16 cell would wrap).  The flood
+ * then keeps, per fragment, only the connected part holding its smallest id
+ * and regrows the rest (vf_fracture_flood). */
+int vf_seed_merge(const vf_seed* frags, uint32_t F, vf_seed* seeds, uint32_t N, int distance);
 
 /* ---- fracture -------------------------------------------------------------- */
 /* NaiveFracturer::build (MF/Source/Fracturer/NaiveFracturer.cpp:215-225,
@@ -116,7 +136,12 @@ int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
 int vf_remove_isolated_regions(vf_grid* g, const vf_seed* seeds, uint32_t S);
 /* FloodFracturer::build (MF/Source/Fracturer/FloodFracturer.cpp:98-191): region
  * growing from the seeds, deterministic (distance, seed index) claims, node
- * weights 1 + hash noise in [0, noise_amp]. */
+ * weights 1 + hash noise in [0, noise_amp].  Labels are fragment | sub-seed
+ * id << 8 (fragment 2..255): when a fragment's seeds carry different ids,
+ * each connected part of the fragment keeps the smallest id of its seeds, the
+ * parts without the fragment's smallest id are freed and regrown from all
+ * other labelled cells at once (FloodFracturer.cpp:134-177, ties to the lower
+ * owning seed index).  Cells end with the fragment id (unmaskRightMost). */
 int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance, int neighbourhood,
                       uint64_t noise_seed, uint32_t noise_amp);
 

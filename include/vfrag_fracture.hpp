@@ -11,6 +11,8 @@
 //   fracturer::NaiveFracturer        Fracturer/NaiveFracturer.h:13-70 (Singleton)
 //   fracturer::FloodFracturer        Fracturer/FloodFracturer.h:13-75 (Singleton)
 //   fracturer::Seeder::uniform       Fracturer/Seeder.h:24, 69-76
+//   fracturer::Seeder::nearSeeds     Fracturer/Seeder.h:62, Seeder.cpp:49-113
+//   fracturer::Seeder::mergeSeeds    Fracturer/Seeder.h:67, Seeder.cpp:115-152
 //
 // Differences a caller sees: the grid lives in device memory and data() /
 // at() / set() work on a host mirror that is synchronised explicitly, exactly
@@ -85,7 +87,11 @@ struct FractureParameters {
     int _distanceFunction = CHEBYSHEV;
     int _neighbourhoodType = VON_NEUMANN;  // the reference's flood ignores it (the stencil follows the distance)
     int _numSeeds = 8;
-    int _numExtraSeeds = 16;               // not on this path (extra seeds are not built)
+    int _numExtraSeeds = 16;               // BOTH seeds merged into the fragments as sub-seeds (Seeder::mergeSeeds)
+    int _mergeSeedsDistanceFunction = EUCLIDEAN;
+    int _numImpacts = 0;                   // Seeder::nearSeeds impacts; 0 = uniform fragment seeds only
+    int _biasSeeds = 32;                   // nearSeeds: seeds over all impacts
+    int _biasFocus = 5;                    // nearSeeds: uniform draws summed per axis (spreading)
     bool _removeIsolatedRegions = true;    // the reference's GPU pass is an identity; see NaiveFracturer below
     bool _launchGPU = true;
     int _seed = 80;
@@ -409,7 +415,35 @@ private:
 // them in (x, y, z) order labelled 2..nseeds+1.
 class Seeder {
 public:
+    static constexpr uint32_t VOXEL_ID_POSITION = 8;  // Seeder.h:12: sub-seed ids live above the 8-bit fragment id
     enum Location { INNER, OUTER, BOTH };
+
+    // nearSeeds(grid, frags, numImpacts, numSeeds, spreading): frags followed
+    // by impact seeds drawn around randomly chosen fragment seeds (occupied
+    // surface cells within half the smallest grid edge), in (x, y, z) order
+    // with labels after the last fragment's.  Counter-based, keyed by
+    // randomSeed; the reference's endless retry loop becomes an error after
+    // 10^6 draws per impact.
+    static std::vector<uvec4> nearSeeds(RegularGrid& grid, const std::vector<uvec4>& frags, unsigned numImpacts,
+                                        unsigned numSeeds, unsigned spreading, uint64_t randomSeed = 0)
+    {
+        std::vector<uvec4> out(frags.size() + numSeeds);
+        uint32_t n = 0;
+        check(vf_seed_near(FractureAccess::device(grid), reinterpret_cast<const vf_seed*>(frags.data()),
+                           (uint32_t)frags.size(), numImpacts, numSeeds, spreading, randomSeed,
+                           reinterpret_cast<vf_seed*>(out.data()), &n));
+        out.resize(n);
+        return out;
+    }
+
+    // mergeSeeds(frags, seeds, dfunc): each seed takes its nearest fragment
+    // seed's label plus that fragment's running sub-seed id << 8.
+    static void mergeSeeds(const std::vector<uvec4>& frags, std::vector<uvec4>& seeds, DistanceFunction dfunc)
+    {
+        check(vf_seed_merge(reinterpret_cast<const vf_seed*>(frags.data()), (uint32_t)frags.size(),
+                            reinterpret_cast<vf_seed*>(seeds.data()), (uint32_t)seeds.size(), (int)dfunc));
+    }
+
     static std::vector<uvec4> uniform(RegularGrid& grid, unsigned int nseeds, uint64_t randomSeed,
                                       Location location = OUTER)
     {

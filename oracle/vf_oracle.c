@@ -407,12 +407,14 @@ static const int NB6[6][3] = {
     /* FloodFracturer::VON_NEUMANN order (FloodFracturer.cpp:8-11) */
     {1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
 
-static int flood_validate(const uint32_t dims[3], const uint32_t* seeds, uint32_t S, int nb)
+/* max_label 255: plain fragment ids; 0xFFFF: fragment id in the low 8 bits
+ * (at least 2) and a sub-seed id in the high 8 bits (Seeder.h:12). */
+static int flood_validate(const uint32_t dims[3], const uint32_t* seeds, uint32_t S, int nb, uint32_t max_label)
 {
     if (nb != 6 && nb != 26) return -1;
     for (uint32_t s = 0; s < S; ++s) {
         if (seeds[4 * s] >= dims[0] || seeds[4 * s + 1] >= dims[1] || seeds[4 * s + 2] >= dims[2]) return -1;
-        if (seeds[4 * s + 3] < 2 || seeds[4 * s + 3] > 255) return -1;
+        if ((seeds[4 * s + 3] & 0xFFu) < 2 || seeds[4 * s + 3] > max_label) return -1;
     }
     return 0;
 }
@@ -439,37 +441,18 @@ static void u32push(u32vec* a, uint32_t x)
     a->v[a->n++] = x;
 }
 
-/*
- * Deterministic flood.  The reference grows all seeds one BFS step at a time
- * and resolves simultaneous claims by a race (floodFracturer-comp.glsl:24-63,
- * FloodFracturer::build FloodFracturer.cpp:98-191).  The deterministic
- * refinement computed here: label(v) = label of argmin over seeds s of
- * (D_s(v), s), where D_s is the node-weighted geodesic distance in the 6- or
- * 26-connected graph of occupied cells, weight(v) = 1 + noise_delta(v).  With
- * zero noise this is one of the race's possible outcomes.  Unreached occupied
- * cells stay FREE; the final unmaskRightMost keeps the low 8 bits.
- * Computed by Dial's bucket algorithm over the lexicographic key (D, s).
- */
-int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
-              const uint32_t* seeds, uint32_t S, int nb,
-              uint64_t noise_seed, uint32_t amp)
+/* Dial's bucket algorithm over the lexicographic key (D << 32 | s): key[] holds
+ * the sources at D = 0 (listed in src) and UINT64_MAX elsewhere; a path may
+ * enter the cells with occ[] set, each adding 1 + noise_delta(cell). */
+static void flood_dial(const uint8_t* occ, const uint32_t dims[3], int nb, uint64_t noise_seed, uint32_t amp,
+                       uint64_t* key, const u32vec* src)
 {
-    if (flood_validate(dims, seeds, S, nb)) return -1;
-    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
     const int (*off)[3] = nb == 6 ? NB6 : NB26;
-    uint64_t* key = (uint64_t*)malloc(V * sizeof(uint64_t));
-    for (size_t i = 0; i < V; ++i) key[i] = UINT64_MAX;
-    uint8_t* act = flood_active_seeds(dims, seeds, S);
     u32vec* bucket = NULL;
     size_t nbuckets = 0;
 #define BUCKET(d) (((d) >= nbuckets ? (bucket = (u32vec*)realloc(bucket, ((d) + 64) * sizeof(u32vec)), \
                      memset(bucket + nbuckets, 0, ((d) + 64 - nbuckets) * sizeof(u32vec)), nbuckets = (d) + 64) : 0), &bucket[d])
-    for (uint32_t s = 0; s < S; ++s) {
-        if (!act[s]) continue;
-        size_t id = gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims);
-        key[id] = (uint64_t)s;
-        u32push(BUCKET(0), (uint32_t)id);
-    }
+    for (size_t e = 0; e < src->n; ++e) u32push(BUCKET(0), src->v[e]);
     for (size_t d = 0; d < nbuckets; ++d) {
         for (size_t e = 0; e < bucket[d].n; ++e) {
             uint32_t v = bucket[d].v[e];
@@ -480,7 +463,7 @@ int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
                 int xx = x + off[k][0], yy = y + off[k][1], zz = z + off[k][2];
                 if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)dims[0] || yy >= (int)dims[1] || zz >= (int)dims[2]) continue;
                 size_t u = gidx(xx, yy, zz, dims);
-                if (in[u] == VFO_EMPTY) continue;
+                if (!occ[u]) continue;
                 uint64_t du = d + 1 + vfo_noise_delta(noise_seed, (uint32_t)u, amp);
                 uint64_t cand = (du << 32) | s;
                 if (cand < key[u]) {
@@ -491,13 +474,117 @@ int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
         }
     }
 #undef BUCKET
-    for (size_t i = 0; i < V; ++i) {
-        if (key[i] != UINT64_MAX) out[i] = (uint16_t)(seeds[4 * (key[i] & 0xFFFFFFFFu) + 3] & 0xFFu);
-        else out[i] = in[i] != VFO_EMPTY ? VFO_FREE : VFO_EMPTY;
-    }
     for (size_t d = 0; d < nbuckets; ++d) free(bucket[d].v);
     free(bucket);
+}
+
+/*
+ * Deterministic flood.  The reference grows all seeds one BFS step at a time
+ * and resolves simultaneous claims by a race (floodFracturer-comp.glsl:24-63,
+ * FloodFracturer::build FloodFracturer.cpp:98-191).  The deterministic
+ * refinement computed here: owner(v) = argmin over seeds s of (D_s(v), s),
+ * where D_s is the node-weighted geodesic distance in the 6- or 26-connected
+ * graph of occupied cells, weight(v) = 1 + noise_delta(v).  With zero noise
+ * this is one of the race's possible outcomes.  Computed by Dial's bucket
+ * algorithm over the lexicographic key (D, s).
+ *
+ * Sub-seeds (labels fragment | id << 8, as Seeder::mergeSeeds makes them,
+ * Seeder.cpp:115-152): while growing, the reference lowers the id of every
+ * cell to the smallest id met across same-fragment neighbours
+ * (floodFracturer-comp.glsl:49-62), so each connected component of a
+ * fragment's cells ends with the smallest id of its seeds; then every cell
+ * whose id is not the fragment's smallest becomes FREE
+ * (disjointSet-comp.glsl, disjointSetStack-comp.glsl) and the grid is flooded
+ * again from all remaining cells at once (FloodFracturer.cpp:134-177).  Here:
+ * components of {v : fragment(owner(v)) = f} under the flood's stencil, kept
+ * iff their smallest id equals the fragment's smallest; the freed cells are
+ * re-flooded from every kept cell at distance 0, ties to the smaller owner
+ * index.  A second pass frees nothing (each fragment is then one component
+ * carrying its smallest id), as in the reference's loop.
+ *
+ * Unreached occupied cells stay FREE; the final unmaskRightMost keeps the low
+ * 8 bits.
+ */
+int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
+              const uint32_t* seeds, uint32_t S, int nb,
+              uint64_t noise_seed, uint32_t amp)
+{
+    if (flood_validate(dims, seeds, S, nb, 0xFFFF)) return -1;
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    const int (*off)[3] = nb == 6 ? NB6 : NB26;
+    uint64_t* key = (uint64_t*)malloc(V * sizeof(uint64_t));
+    uint8_t* occ = (uint8_t*)calloc(V ? V : 1, 1);
+    for (size_t i = 0; i < V; ++i) { key[i] = UINT64_MAX; occ[i] = in[i] != VFO_EMPTY; }
+    uint8_t* act = flood_active_seeds(dims, seeds, S);
+    u32vec src = { 0 };
+    for (uint32_t s = 0; s < S; ++s) {
+        if (!act[s]) continue;
+        size_t id = gidx(seeds[4 * s], seeds[4 * s + 1], seeds[4 * s + 2], dims);
+        key[id] = (uint64_t)s;
+        occ[id] = 1;  /* grid.set makes a seed cell occupied */
+        u32push(&src, (uint32_t)id);
+    }
+    flood_dial(occ, dims, nb, noise_seed, amp, key, &src);
+
+    /* sub-seed merge: components of each fragment's cells, their smallest ids */
+#define OWNER(v) ((uint32_t)(key[v] & 0xFFFFFFFFu))
+#define FRAG(v) (seeds[4 * OWNER(v) + 3] & 0xFFu)
+#define SUBID(v) (seeds[4 * OWNER(v) + 3] >> 8)
+    uint32_t* comp = (uint32_t*)malloc(V * sizeof(uint32_t));
+    for (size_t i = 0; i < V; ++i) comp[i] = UINT32_MAX;
+    u32vec cmin = { 0 }, queue = { 0 };
+    uint32_t fmin[256];
+    for (int f = 0; f < 256; ++f) fmin[f] = UINT32_MAX;
+    for (size_t v0 = 0; v0 < V; ++v0) {
+        if (key[v0] == UINT64_MAX || comp[v0] != UINT32_MAX) continue;
+        const uint32_t c = (uint32_t)cmin.n, f = FRAG(v0);
+        uint32_t m = SUBID(v0);
+        queue.n = 0;
+        u32push(&queue, (uint32_t)v0);
+        comp[v0] = c;
+        for (size_t h = 0; h < queue.n; ++h) {
+            uint32_t v = queue.v[h];
+            if (SUBID(v) < m) m = SUBID(v);
+            int z = (int)(v % dims[2]), y = (int)((v / dims[2]) % dims[1]), x = (int)(v / ((size_t)dims[1] * dims[2]));
+            for (int k = 0; k < nb; ++k) {
+                int xx = x + off[k][0], yy = y + off[k][1], zz = z + off[k][2];
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= (int)dims[0] || yy >= (int)dims[1] || zz >= (int)dims[2]) continue;
+                size_t u = gidx(xx, yy, zz, dims);
+                if (key[u] == UINT64_MAX || comp[u] != UINT32_MAX || FRAG(u) != f) continue;
+                comp[u] = c;
+                u32push(&queue, (uint32_t)u);
+            }
+        }
+        u32push(&cmin, m);
+        if (m < fmin[f]) fmin[f] = m;
+    }
+    int freed = 0;
+    for (size_t v = 0; v < V && !freed; ++v)
+        if (key[v] != UINT64_MAX && cmin.v[comp[v]] != fmin[FRAG(v)]) freed = 1;
+    if (freed) {
+        /* re-flood the freed cells from every kept cell */
+        src.n = 0;
+        for (size_t v = 0; v < V; ++v) {
+            if (key[v] == UINT64_MAX) continue;
+            if (cmin.v[comp[v]] == fmin[FRAG(v)]) {
+                key[v] = OWNER(v);
+                u32push(&src, (uint32_t)v);
+            } else {
+                key[v] = UINT64_MAX;
+            }
+        }
+        flood_dial(occ, dims, nb, noise_seed, amp, key, &src);
+    }
+#undef OWNER
+#undef FRAG
+#undef SUBID
+    for (size_t i = 0; i < V; ++i) {
+        if (key[i] != UINT64_MAX) out[i] = (uint16_t)(seeds[4 * (key[i] & 0xFFFFFFFFu) + 3] & 0xFFu);
+        else out[i] = occ[i] ? VFO_FREE : VFO_EMPTY;
+    }
+    free(comp); free(cmin.v); free(queue.v); free(src.v);
     free(key);
+    free(occ);
     free(act);
     return 0;
 }
@@ -537,7 +624,7 @@ int vfo_flood_dijkstra(const uint16_t* in, uint16_t* out, const uint32_t dims[3]
                        const uint32_t* seeds, uint32_t S, int nb,
                        uint64_t noise_seed, uint32_t amp)
 {
-    if (flood_validate(dims, seeds, S, nb)) return -1;
+    if (flood_validate(dims, seeds, S, nb, 255)) return -1;
     const size_t V = (size_t)dims[0] * dims[1] * dims[2];
     const int (*off)[3] = nb == 6 ? NB6 : NB26;
     uint64_t* best = (uint64_t*)malloc(V * sizeof(uint64_t));
@@ -677,6 +764,115 @@ int vfo_seed_uniform(const uint16_t* grid, const uint32_t dims[3], uint32_t S,
     }
     free(cand);
     free(ranks);
+    return 0;
+}
+
+static int64_t seed_dist2(const uint32_t* a, const uint32_t* b, int dist)
+{
+    int64_t dx = (int64_t)a[0] - b[0], dy = (int64_t)a[1] - b[1], dz = (int64_t)a[2] - b[2];
+    return seed_dist(dx, dy, dz, dist);
+}
+
+int vfo_seed_merge(const uint32_t* frags, uint32_t F, uint32_t* seeds, uint32_t N, int dist)
+{
+    /* Seeder::mergeSeeds (Seeder.cpp:115-152): each seed takes the label of its
+     * nearest fragment seed (first on ties) with a per-fragment running sub-id
+     * in bits 8..15.  Integer distances order exactly as the reference's float
+     * ones (squared Euclidean for glm::distance). */
+    uint32_t next[256] = { 0 };
+    if (N && !F) return -1;
+    for (uint32_t f = 0; f < F; ++f)
+        if (frags[4 * f + 3] > 255) return -1;
+    for (uint32_t i = 0; i < N; ++i) {
+        int64_t best = INT64_MAX;
+        uint32_t nearest = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            int64_t d = seed_dist2(seeds + 4 * i, frags + 4 * f, dist);
+            if (d < best) { best = d; nearest = f; }
+        }
+        uint32_t w = frags[4 * nearest + 3];
+        if (++next[w] > 255) return -2;
+        seeds[4 * i + 3] = w | (next[w] << 8);
+    }
+    return 0;
+}
+
+/* floor(min + (max - min) * h / 2^32): static_cast<int>(min + (max - min) * u)
+ * of RandomUtilities::getUniformRandomInt (RandomUtilities.h:141-144) for
+ * u = h / 2^32 whenever the result is not negative */
+static int64_t uniform_int(int64_t mn, int64_t mx, uint32_t h)
+{
+    int64_t t = (mx - mn) * (int64_t)h;
+    return mn + (t >= 0 ? t >> 32 : -((-t + 0xFFFFFFFFll) >> 32));
+}
+
+uint64_t vfo_near_key(uint64_t seed, uint32_t impact)
+{
+    return seed + (uint64_t)(impact + 1u) * 0x9E3779B97F4A7C15ull;
+}
+
+int vfo_seed_near(const uint16_t* grid, const uint32_t dims[3], const uint32_t* frags, uint32_t F,
+                  uint32_t impacts, uint32_t num_seeds, uint32_t spreading, uint64_t seed,
+                  uint32_t* out, uint32_t* count)
+{
+    /* Seeder::nearSeeds (Seeder.cpp:49-113) with counter-based draws.  Impact i
+     * uses key k_i = vfo_near_key(seed, i): fragment index uniform_int(0, F-1,
+     * h(k_i, 0)), seed count min(uniform_int(1, pending, h(k_i, 1)), pending);
+     * candidate t draws axis a as n_a/2 - sum_{j<spreading} h(k_i, 2 + (3t+a)
+     * spreading + j) mod (n_a / spreading) (getBiasedRandomInt), wraps it
+     * around the fragment seed, and is accepted when within min(n)/2 of it,
+     * occupied, on the surface (isBoundary) and new.  New seeds follow the
+     * fragments in (x, y, z) order with labels after the last fragment's. */
+    if (!F || !spreading || spreading > 64) return -1;
+    for (int a = 0; a < 3; ++a)
+        if (dims[a] < spreading) return -1;
+    const uint32_t n[3] = { dims[0], dims[1], dims[2] };
+    const uint32_t m[3] = { n[0] / spreading, n[1] / spreading, n[2] / spreading };
+    const uint32_t mind = (n[0] < n[1] ? (n[0] < n[2] ? n[0] : n[2]) : (n[1] < n[2] ? n[1] : n[2])) / 2;
+    uint32_t* chosen = (uint32_t*)malloc((num_seeds ? num_seeds : 1) * sizeof(uint32_t));
+    uint32_t nchosen = 0, pending = num_seeds;
+    for (uint32_t i = 0; i < impacts; ++i) {
+        const uint64_t k = vfo_near_key(seed, i);
+        const uint32_t* fr = frags + 4 * (uint32_t)uniform_int(0, (int64_t)F - 1, vfo_hash32(k, 0));
+        int64_t want = uniform_int(1, pending, vfo_hash32(k, 1));
+        if (want > pending) want = pending;
+        if (want < 0) want = 0;
+        uint32_t got = 0;
+        for (uint32_t t = 0; got < (uint32_t)want; ++t) {
+            if (t == 1000000u) { free(chosen); return -2; }
+            uint32_t c[3];
+            int64_t d2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                uint32_t b = 0;
+                for (uint32_t j = 0; j < spreading; ++j) b += vfo_hash32(k, 2 + (3 * t + a) * spreading + j) % m[a];
+                int64_t o = (int64_t)(n[a] / 2) - b;
+                c[a] = (uint32_t)(((int64_t)fr[a] + o + n[a]) % n[a]);
+                d2 += ((int64_t)c[a] - fr[a]) * ((int64_t)c[a] - fr[a]);
+            }
+            if (d2 > (int64_t)mind * mind) continue;
+            size_t id = gidx(c[0], c[1], c[2], dims);
+            if (grid[id] == VFO_EMPTY || !is_surface(grid, dims, (int)c[0], (int)c[1], (int)c[2])) continue;
+            int dup = 0;
+            for (uint32_t q = 0; q < nchosen; ++q) if (chosen[q] == id) { dup = 1; break; }
+            if (dup) continue;
+            chosen[nchosen++] = (uint32_t)id;
+            ++got;
+        }
+        pending -= (uint32_t)want;
+    }
+    qsort(chosen, nchosen, sizeof(uint32_t), cmp_u32);
+    memcpy(out, frags, (size_t)F * 4 * sizeof(uint32_t));
+    uint32_t label = frags[4 * (F - 1) + 3];
+    for (uint32_t q = 0; q < nchosen; ++q) {
+        uint32_t id = chosen[q];
+        uint32_t* o = out + 4 * (F + q);
+        o[0] = id / (dims[1] * dims[2]);
+        o[1] = (id / dims[2]) % dims[1];
+        o[2] = id % dims[2];
+        o[3] = ++label;
+    }
+    *count = F + nchosen;
+    free(chosen);
     return 0;
 }
 

@@ -51,7 +51,8 @@ uint32_t vfo_noise_delta(uint64_t noise_seed, uint32_t idx, uint32_t amp);
 int vfo_flood(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
               const uint32_t* seeds, uint32_t S, int neighbourhood,
               uint64_t noise_seed, uint32_t amp);
-/* independent brute force: one Dijkstra per seed, then lexicographic argmin */
+/* independent brute force of the first growth: one Dijkstra per seed, then
+ * lexicographic argmin (labels 2..255, no sub-seed pass) */
 int vfo_flood_dijkstra(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
                        const uint32_t* seeds, uint32_t S, int neighbourhood,
                        uint64_t noise_seed, uint32_t amp);
@@ -63,6 +64,14 @@ enum { VFO_SEED_INNER = 0, VFO_SEED_OUTER = 1, VFO_SEED_BOTH = 2 };  /* Seeder::
 uint32_t vfo_hash32(uint64_t key, uint32_t counter);
 int vfo_seed_uniform(const uint16_t* grid, const uint32_t dims[3], uint32_t S,
                      uint64_t seed, int location, uint32_t* seeds_out);
+/* Seeder::mergeSeeds: seeds[i].label = nearest fragment's label | running sub-id << 8.
+ * 0, -1 bad input, -2 more than 255 sub-ids for one fragment. */
+int vfo_seed_merge(const uint32_t* frags, uint32_t F, uint32_t* seeds, uint32_t N, int dist);
+/* Seeder::nearSeeds with counter-based draws; out: F + num_seeds entries. 0, -1 bad input, -2 draw cap */
+uint64_t vfo_near_key(uint64_t seed, uint32_t impact);
+int vfo_seed_near(const uint16_t* grid, const uint32_t dims[3], const uint32_t* frags, uint32_t F,
+                  uint32_t impacts, uint32_t num_seeds, uint32_t spreading, uint64_t seed,
+                  uint32_t* out, uint32_t* count);
 
 /* ---- marching cubes ----------------------------------------------------- */
 typedef struct {

@@ -50,6 +50,9 @@ def lib():
         "vfo_flood_dijkstra": (I, [P, P, P, P, U32, I, U64, U32]),
         "vfo_remove_isolated": (None, [P, P, P, P, U32]),
         "vfo_seed_uniform": (I, [P, P, U32, U64, I, P]),
+        "vfo_seed_merge": (I, [P, U32, P, U32, I]),
+        "vfo_near_key": (U64, [U64, U32]),
+        "vfo_seed_near": (I, [P, P, P, U32, U32, U32, U32, U64, P, P]),
         "vfo_marching_cubes": (I, [P, P, P, P, U32, C.POINTER(Mesh)]),
         "vfo_mesh_free": (None, [C.POINTER(Mesh)]),
         "vfo_mesh_smooth": (None, [C.POINTER(Mesh), U32, C.c_float, U32, C.c_float]),
@@ -168,6 +171,29 @@ def seed_uniform(grid, n, seed, location=2):  # 2 = BOTH (Seeder::Location order
     if rc != 0:
         raise ValueError(f"oracle seeding failed ({rc})")
     return out
+
+
+def seed_merge(frags, seeds, dist):
+    """Seeder::mergeSeeds: returns a copy of seeds with merged labels."""
+    f = np.ascontiguousarray(np.asarray(frags, np.uint32).reshape(-1, 4))
+    out = np.array(np.asarray(seeds, np.uint32).reshape(-1, 4), copy=True)
+    rc = lib().vfo_seed_merge(_p(f), f.shape[0], _p(out), out.shape[0], dist)
+    if rc != 0:
+        raise ValueError(f"oracle merge failed ({rc})")
+    return out
+
+
+def seed_near(grid, frags, impacts, num_seeds, spreading, seed):
+    """Seeder::nearSeeds (counter-based): fragments followed by the impact seeds."""
+    g = np.ascontiguousarray(grid, np.uint16)
+    f = np.ascontiguousarray(np.asarray(frags, np.uint32).reshape(-1, 4))
+    out = np.zeros((f.shape[0] + num_seeds, 4), np.uint32)
+    n = C.c_uint32()
+    rc = lib().vfo_seed_near(_p(g), _p(_dims(g.shape)), _p(f), f.shape[0], impacts, num_seeds, spreading, seed,
+                             _p(out), C.byref(n))
+    if rc != 0:
+        raise ValueError(f"oracle near seeding failed ({rc})")
+    return out[: n.value]
 
 
 def marching_cubes(grid, aabb_min, aabb_max, label=0xFFFFFFFF, smooth=None):

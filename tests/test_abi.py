@@ -73,3 +73,24 @@ def test_dropin_driver_usage_without_device():
         pytest.skip("drop-in driver not built")
     r = subprocess.run([str(DRIVER)], capture_output=True, text=True, timeout=30)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.parametrize("dist", [0, 1, 2])
+def test_seed_merge_host_entry_matches_oracle(dist):
+    """vf_seed_merge is host code (the reference merges on the CPU too), so it
+    runs here without a device."""
+    if not LIB.exists():
+        pytest.skip("libvfrag.so not built")
+    import numpy as np
+    import oracle as O
+    from voxelfragmentml_amd import vfrag
+
+    rng = np.random.default_rng(40 + dist)
+    frags = np.concatenate([rng.integers(0, 256, (10, 3)), (2 + np.arange(10))[:, None]], 1).astype(np.uint32)
+    extra = np.concatenate([rng.integers(0, 256, (20, 3)), np.full((20, 1), 2)], 1).astype(np.uint32)
+    seeds = np.concatenate([frags, extra])
+    np.testing.assert_array_equal(vfrag.seed_merge(frags, seeds, dist), O.seed_merge(frags, seeds, dist))
+    many = np.zeros((256, 4), np.uint32)
+    with pytest.raises(vfrag.VfError) as e:
+        vfrag.seed_merge(frags[:1], many, dist)
+    assert e.value.code == -4

@@ -18,16 +18,20 @@ DRIVER = ROOT / "voxelfragmentml_amd" / "dropin" / "fracture_scene"
 BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
 
 
-@pytest.mark.parametrize("algo,edge,S,dist,mesh_seed,tris,key,noise_seed,amp", [
-    ("naive", 96, 8, 1, 1, 20000, 11, 0, 0),
-    ("flood", 80, 12, 0, 2, 30000, 12, 77, 2),
-    ("flood", 72, 9, 1, 3, 8000, 13, 5, 1),
+@pytest.mark.parametrize("algo,edge,S,dist,mesh_seed,tris,key,noise_seed,amp,extra,impacts", [
+    ("naive", 96, 8, 1, 1, 20000, 11, 0, 0, 0, 0),
+    ("flood", 80, 12, 0, 2, 30000, 12, 77, 2, 0, 0),
+    ("flood", 72, 9, 1, 3, 8000, 13, 5, 1, 0, 0),
+    ("flood", 80, 8, 1, 4, 20000, 14, 3, 1, 16, 0),   # CADScene: numExtraSeeds = 2 x fragments
+    ("flood", 88, 6, 2, 5, 20000, 15, 0, 0, 12, 2),   # plus two impacts
+    ("naive", 64, 5, 0, 6, 10000, 16, 0, 0, 10, 1),
 ])
-def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_seed, tris, key, noise_seed, amp):
+def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_seed, tris, key, noise_seed, amp, extra,
+                                      impacts):
     assert DRIVER.exists(), "drop-in driver not built"
     out = tmp_path / "scene"
     args = [str(DRIVER), algo, str(edge), str(S), str(dist), str(mesh_seed), str(tris), str(key), str(noise_seed),
-            str(amp), str(out)]
+            str(amp), str(out), str(extra), str(impacts)]
     print("dropin:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -39,6 +43,11 @@ def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_se
     xyz, tri = vf.vessel_mesh(mesh_seed, *vf.vessel_segments(tris))
     base = O.occ_to_grid(O.voxelize(xyz, tri, *BOX, (edge, edge, edge)))
     ref_seeds = O.seed_uniform(base, S, key, 2)  # Seeder::BOTH
+    if impacts:  # Seeder::nearSeeds with FractureParameters' _biasSeeds 32, _biasFocus 5
+        ref_seeds = O.seed_near(base, ref_seeds, impacts, 32, 5, key + 2)
+    if extra:  # CADScene.cpp:647-655
+        more = np.concatenate([ref_seeds, O.seed_uniform(base, extra, key + 1, 2)])
+        ref_seeds = np.concatenate([ref_seeds, O.seed_merge(ref_seeds, more, 0)])
     np.testing.assert_array_equal(seeds, ref_seeds)
     if algo == "naive":
         lab = O.naive(base, ref_seeds, dist)

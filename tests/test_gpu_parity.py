@@ -135,9 +135,10 @@ def test_flood_edge_cases(vf):
     g.write(base_h)
     g.fracture_flood(seeds, CHE, 26, noise_seed=5, noise_amp=2)
     np.testing.assert_array_equal(g.read(), O.flood(base_h, seeds, 26, 5, 2))
-    with pytest.raises(vf.VfError) as e:
-        g.fracture_flood(np.array([[0, 0, 0, 300]], np.uint32), CHE)
-    assert e.value.code == -4
+    for bad in (0x101, 0x10002):  # fragment id 1 (FREE); beyond 16 bits
+        with pytest.raises(vf.VfError) as e:
+            g.fracture_flood(np.array([[0, 0, 0, bad]], np.uint32), CHE)
+        assert e.value.code == -4
     # no seeds: every occupied cell stays FREE
     g.write(base)
     g.fracture_flood(np.zeros((0, 4), np.uint32), CHE)

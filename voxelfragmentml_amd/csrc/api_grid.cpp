@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <set>
 #include <vector>
 
 #include "grid.hpp"
@@ -398,6 +400,74 @@ int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed
     });
 }
 
+int vf_seed_near(vf_grid* g, const vf_seed* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
+                 uint32_t spreading, uint64_t seed, vf_seed* out, uint32_t* count)
+{
+    return guard([&] {
+        checked(g);
+        if (!frags || !F) fail(VF_ERR_INVALID_ARG, "near seeding needs at least one fragment seed");
+        if (!out || !count) fail(VF_ERR_INVALID_ARG, "out or count is NULL");
+        if (spreading == 0 || spreading > 64) fail(VF_ERR_INVALID_ARG, "spreading %u outside 1..64", spreading);
+        if (spreading > std::min(g->dims[0], std::min(g->dims[1], g->dims[2])))
+            fail(VF_ERR_INVALID_ARG, "spreading %u above the smallest grid edge", spreading);
+        if (num_seeds > kMaxDeviceSeeds || F > kMaxDeviceSeeds)
+            fail(VF_ERR_INVALID_ARG, "at most %u seeds", kMaxDeviceSeeds);
+        validate_seeds(g, frags, F);
+        grid_materialize(g);
+        g->near_in.ensure((size_t)F * sizeof(uint4) + 64);
+        g->near_out.ensure(((size_t)F + num_seeds) * sizeof(uint4) + 64);
+        VF_HIP(hipMemcpyAsync(g->near_in.p, frags, (size_t)F * sizeof(uint4), hipMemcpyHostToDevice, g->stream));
+        Launch L = grid_launch(g);
+        g->fctl.ensure(64 * 4);
+        uint32_t* dres = g->fctl.as<uint32_t>() + 48;  // words 48..49: count, draw-cap flag
+        launch_seed_near(L, g->near_in.as<uint4>(), F, impacts, num_seeds, spreading, seed, g->near_out.as<uint4>(), dres);
+        VF_HIP(hipMemcpyAsync(g->pinned, dres, 8, hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+        if (g->pinned[1]) fail(VF_ERR_INVALID_ARG, "an impact found too few free surface cells in 10^6 draws");
+        const uint32_t n = g->pinned[0];
+        VF_HIP(hipMemcpyAsync(out, g->near_out.p, (size_t)n * sizeof(uint4), hipMemcpyDeviceToHost, g->stream));
+        VF_HIP(hipStreamSynchronize(g->stream));
+        *count = n;
+    });
+}
+
+int vf_seed_merge(const vf_seed* frags, uint32_t F, vf_seed* seeds, uint32_t N, int distance)
+{
+    return guard([&] {
+        if (N == 0) return;
+        if (!seeds || !frags || F == 0) fail(VF_ERR_INVALID_ARG, "merging needs fragment seeds");
+        if (distance < VF_EUCLIDEAN || distance > VF_CHEBYSHEV) fail(VF_ERR_INVALID_ARG, "unknown distance %d", distance);
+        for (uint32_t f = 0; f < F; ++f)
+            if (frags[f].w > 255) fail(VF_ERR_INVALID_ARG, "fragment label %u does not fit 8 bits", frags[f].w);
+        // integer distances order as the reference's float ones (squared Euclidean for glm::distance:
+        // the float square root of an integer below 2^24 is strictly increasing)
+        auto d = [&](const vf_seed& a, const vf_seed& b) -> int64_t {
+            const int64_t dx = std::llabs((int64_t)a.x - b.x), dy = std::llabs((int64_t)a.y - b.y),
+                          dz = std::llabs((int64_t)a.z - b.z);
+            if (distance == VF_EUCLIDEAN) return dx * dx + dy * dy + dz * dz;
+            if (distance == VF_MANHATTAN) return dx + dy + dz;
+            return std::max(dx, std::max(dy, dz));
+        };
+        uint32_t next[256] = {};
+        std::vector<uint32_t> label(N);
+        for (uint32_t i = 0; i < N; ++i) {
+            uint32_t nearest = 0;
+            int64_t best = INT64_MAX;
+            for (uint32_t f = 0; f < F; ++f) {
+                const int64_t v = d(seeds[i], frags[f]);
+                if (v < best) {
+                    best = v;
+                    nearest = f;
+                }
+            }
+            const uint32_t w = frags[nearest].w;
+            if (++next[w] > 255) fail(VF_ERR_TOO_MANY_SEEDS, "fragment %u would get more than 255 sub-seeds", w);
+            label[i] = w | (next[w] << 8);
+        }
+        for (uint32_t i = 0; i < N; ++i) seeds[i].w = label[i];
+    });
+}
+
 int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance)
 {
     return guard([&] {
@@ -459,10 +529,24 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         if (noise_amp > kMaxNoiseAmp) fail(VF_ERR_INVALID_ARG, "noise amplitude %u above %u", noise_amp, kMaxNoiseAmp);
         validate_seeds(g, seeds, S);
         for (uint32_t s = 0; s < S; ++s)
-            if (seeds[s].w < 2 || seeds[s].w > 255)
-                fail(VF_ERR_TOO_MANY_SEEDS, "flood label %u outside 2..255 (8-bit fragment id, Seeder.h:12)", seeds[s].w);
+            if ((seeds[s].w & 0xFFu) < 2 || seeds[s].w > 0xFFFF)
+                fail(VF_ERR_TOO_MANY_SEEDS, "flood label 0x%x: the fragment id (low 8 bits, Seeder.h:12) must be 2..255 "
+                     "and a sub-seed id fits bits 8..15", seeds[s].w);
+        if (S > kMaxDeviceSeeds) fail(VF_ERR_TOO_MANY_SEEDS, "%u flood seeds (max %u)", S, kMaxDeviceSeeds);
+        // the sub-seed pass matters only when some fragment's live seeds carry different ids
+        bool sub = false;
+        {
+            std::map<uint64_t, uint32_t> last;  // cell -> last seed on it
+            for (uint32_t s = 0; s < S; ++s)
+                last[((uint64_t)seeds[s].x * g->dims[1] + seeds[s].y) * g->dims[2] + seeds[s].z] = s;
+            std::map<uint32_t, std::set<uint32_t>> ids;
+            for (auto& kv : last) ids[seeds[kv.second].w & 0xFFu].insert(seeds[kv.second].w >> 8);
+            for (auto& kv : ids) sub |= kv.second.size() > 1;
+        }
         upload_seeds(g, seeds, S);
         JobDev& J = g->job;
+        if (sub) g->flood_sub.ensure(8ull * S + 64);
+        J.sub = sub ? g->flood_sub.as<uint32_t>() : nullptr;
         const uint32_t nbricks = relax_bricks(g->dims[0], g->dims[1], g->dims[2]);
         g->keys.ensure(g->V * 4 + 64);
         g->buckets.ensure(16ull * relax_words_per_parity(nbricks));
@@ -494,19 +578,27 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         VF_HIP(hipStreamSynchronize(g->stream));  // the pinned words are reused below
         launch_flood_init(L);
         launch_flood_seed(L);
-        uint32_t chunk = 4, rounds = 0;
-        for (;;) {
-            launch_flood_rounds(L, W, chunk, active_shape().flood_blocks);
-            rounds += chunk;
+        auto drain = [&]() {
+            uint32_t chunk = 4, rounds = 0;
+            for (;;) {
+                launch_flood_rounds(L, W, chunk, active_shape().flood_blocks);
+                rounds += chunk;
+                VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
+                VF_HIP(hipStreamSynchronize(g->stream));
+                if (g->pinned[1] != 0) return g->pinned[1];
+                chunk = std::min<uint32_t>(chunk * 2, 32);
+                // every round lowers a cell's key at least one step along a shortest path, so the
+                // worklist drains within (cells + 1) rounds
+                if ((uint64_t)rounds > g->V + S + 16) fail(VF_ERR_OVERFLOW, "flood did not drain");
+            }
+        };
+        g->last_flood_levels = drain();
+        if (sub) {
+            launch_flood_sub(L);
             VF_HIP(hipMemcpyAsync(g->pinned, g->fctl.p, 8, hipMemcpyDeviceToHost, g->stream));
             VF_HIP(hipStreamSynchronize(g->stream));
-            if (g->pinned[1] != 0) break;
-            chunk = std::min<uint32_t>(chunk * 2, 32);
-            // every round lowers a cell's key at least one step along a shortest path, so the
-            // worklist drains within (cells + 1) rounds
-            if ((uint64_t)rounds > g->V + S + 16) fail(VF_ERR_OVERFLOW, "flood did not drain");
+            if (g->pinned[1] == 0) g->last_flood_levels += drain();  // regions were freed: second growth
         }
-        g->last_flood_levels = g->pinned[1];
         check_device_error(g, "flood");
         launch_flood_finish(L);
         g->labels_from_occ = false;

@@ -504,6 +504,195 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
     });
 }
 
+// ---- sub-seeds -------------------------------------------------------------
+// Seeds labelled fragment | id << 8 (Seeder::mergeSeeds, Seeder.cpp:115-152).
+// While growing, the reference lowers every cell's id to the smallest id met
+// across same-fragment neighbours (floodFracturer-comp.glsl:49-62), so each
+// connected component of a fragment's cells ends with the smallest id of its
+// seeds; cells whose id is not the fragment's smallest become FREE and the
+// grid is flooded again from all remaining cells (disjointSet-comp.glsl,
+// disjointSetStack-comp.glsl, FloodFracturer.cpp:134-177).
+//
+// After the first drain every seed's region is connected (each cell's
+// shortest path to its seed stays in the region), so a fragment's components
+// are unions of the regions of its seeds that touch.  A union-find over the
+// seeds joins touching same-fragment regions; a component is kept iff its
+// smallest live id equals the fragment's smallest.  Freed cells become
+// unreached, kept cells become sources (distance 1, their owner as the tie
+// index), and the rounds restart over the bricks holding freed cells.
+
+// Parents only ever move from a root to a smaller index, so chains strictly
+// decrease and both loops end; a stale read at worst costs a failed CAS.
+__device__ inline uint32_t uf_find(uint32_t* parent, uint32_t x)
+{
+    for (;;) {
+        const uint32_t p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+__device__ inline void uf_unite(uint32_t* parent, uint32_t a, uint32_t b)
+{
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    while (a != b) {
+        if (a > b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        const uint32_t old = atomicCAS(parent + b, b, a);
+        if (old == b) return;
+        b = uf_find(parent, old);  // b was linked meanwhile: continue from its new root
+        a = uf_find(parent, a);
+    }
+}
+
+__device__ inline bool reached_key(uint32_t k) { return k != kEmptyKey && k < kUnreachedKey; }
+
+// One block per job: every seed its own set.
+__global__ __launch_bounds__(256) void k_flood_sub_init(Launch L)
+{
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active || J->algo != VF_FLOOD || !J->sub) return;
+    for (uint32_t s = threadIdx.x; s < J->S; s += blockDim.x) J->sub[s] = s;
+}
+
+// Join the sets of seeds whose cells touch through the flood's stencil and
+// carry the same fragment id.  The relation is symmetric, so each cell looks
+// at the forward half of the stencil (3 of 6, 13 of 26 neighbours).
+__global__ __launch_bounds__(kCellThreads) void k_flood_sub_touch(Launch L)
+{
+    __shared__ JobDev sJ;
+    __shared__ uint8_t frag[kMaxDeviceSeeds];
+    uint32_t staged = 0xFFFFFFFFu;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+        if (!J.active || J.algo != VF_FLOOD || !J.sub) return;
+        if (j != staged) {  // block-uniform
+            for (uint32_t i = threadIdx.x; i < J.S && i < kMaxDeviceSeeds; i += blockDim.x)
+                frag[i] = (uint8_t)(J.seeds[i].w & 0xFFu);
+            __syncthreads();
+            staged = j;
+        }
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
+        const uint32_t mask = (1u << J.key_bits) - 1u;
+        const bool n26 = J.neighbourhood != 6;
+        uint32_t x, y, z;
+        decompose(c0, J, x, y, z);
+        uint32_t last_s = 0xFFFFFFFFu, last_t = 0xFFFFFFFFu;
+        for (int k = 0; k < kCellsPerThread && c0 + k < J.V; ++k, advance(x, y, z, J)) {
+            const uint32_t key = J.keys[c0 + k];
+            if (!reached_key(key)) continue;
+            const uint32_t s = (key >> 2) & mask;
+            for (int i = 0; i < (n26 ? 13 : 3); ++i) {
+                // forward offsets o: dx = 1 (o 0..8), then dx = 0, dy = 1 (o 9..11), then (0, 0, 1);
+                // the 6-neighbour ones are o = 4, 10, 12
+                const int o = n26 ? i : (i == 0 ? 4 : i == 1 ? 10 : 12);
+                int dx, dy, dz;
+                if (o < 9) { dx = 1; dy = o / 3 - 1; dz = o % 3 - 1; }
+                else if (o < 12) { dx = 0; dy = 1; dz = o - 10; }
+                else { dx = 0; dy = 0; dz = 1; }
+                const int xx = (int)x + dx, yy = (int)y + dy, zz = (int)z + dz;
+                if (xx >= (int)J.nx || yy < 0 || yy >= (int)J.ny || zz < 0 || zz >= (int)J.nz) continue;
+                const uint32_t kn = J.keys[((size_t)xx * J.ny + (uint32_t)yy) * J.nz + (uint32_t)zz];
+                if (!reached_key(kn)) continue;
+                const uint32_t t = (kn >> 2) & mask;
+                if (t == s || frag[t] != frag[s] || (s == last_s && t == last_t)) continue;
+                last_s = s;
+                last_t = t;
+                uf_unite(J.sub, s, t);
+            }
+        }
+    });
+}
+
+// One block per job: component minima of the live ids, the fragment minima,
+// the keep flags; when a live seed's component is not kept, restart the
+// rounds (both flag parities are clear after a drain).
+__global__ __launch_bounds__(1024) void k_flood_sub_keep(Launch L)
+{
+    __shared__ uint32_t cmin[kMaxDeviceSeeds];
+    __shared__ uint32_t fmin[256];
+    __shared__ uint32_t freed;
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active || J->algo != VF_FLOOD || !J->sub) return;
+    const uint32_t S = J->S, mask = (1u << J->key_bits) - 1u;
+    uint32_t* parent = J->sub;
+    uint32_t* keep = J->sub + S;
+    for (uint32_t i = threadIdx.x; i < kMaxDeviceSeeds; i += blockDim.x) cmin[i] = 0xFFFFFFFFu;
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) fmin[i] = 0xFFFFFFFFu;
+    if (threadIdx.x == 0) freed = 0;
+    __syncthreads();
+    // a seed is live when it owns its own cell (a later seed on the cell wins)
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        const uint4 q = J->seeds[s];
+        const uint32_t k = J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z];
+        const bool live = reached_key(k) && ((k >> 2) & mask) == s;
+        keep[s] = live ? 1u : 0u;
+        if (live) atomicMin(&cmin[uf_find(parent, s)], q.w >> 8);
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
+        if (keep[s]) atomicMin(&fmin[J->seeds[s].w & 0xFFu], cmin[uf_find(parent, s)]);
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        if (!keep[s]) continue;
+        if (cmin[uf_find(parent, s)] != fmin[J->seeds[s].w & 0xFFu]) {
+            keep[s] = 0u;
+            atomicAdd(&freed, 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && freed) {
+        J->fctl[0] = 1;
+        J->fctl[1] = 0;
+        J->fctl[2] = 0;
+        J->fctl[3] = 1;
+    }
+}
+
+// Restarted jobs: kept cells become sources at distance 1, freed cells
+// unreached, and the bricks holding freed cells are flagged for round 1.
+__global__ __launch_bounds__(kCellThreads) void k_flood_sub_reseed(Launch L)
+{
+    __shared__ JobDev sJ;
+    __shared__ uint8_t keep[kMaxDeviceSeeds];
+    uint32_t staged = 0xFFFFFFFFu;
+    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+        if (!J.active || J.algo != VF_FLOOD || !J.sub || J.fctl[1] != 0u) return;
+        if (j != staged) {  // block-uniform
+            for (uint32_t i = threadIdx.x; i < J.S && i < kMaxDeviceSeeds; i += blockDim.x)
+                keep[i] = (uint8_t)J.sub[J.S + i];
+            __syncthreads();
+            staged = j;
+        }
+        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
+        if (c0 >= J.V) return;
+        const uint32_t b = J.key_bits, mask = (1u << b) - 1u;
+        const uint32_t nby = bdim(J.ny), nbz = bdim(J.nz);
+        unsigned long long* flags1 = J.bflags + relax_words_per_parity(J.nbricks);  // round 1 consumes parity 1
+        uint32_t x, y, z, last = 0xFFFFFFFFu;
+        decompose(c0, J, x, y, z);
+        for (int k = 0; k < kCellsPerThread && c0 + k < J.V; ++k, advance(x, y, z, J)) {
+            const uint32_t key = J.keys[c0 + k];
+            if (!reached_key(key)) continue;
+            const uint32_t s = (key >> 2) & mask;
+            if (keep[s]) {
+                J.keys[c0 + k] = (((1u << b) | s) << 2) | (key & 3u);
+            } else {
+                J.keys[c0 + k] = kUnreachedKey | (key & 3u);
+                const uint32_t id = ((x / kRelaxBrick) * nby + y / kRelaxBrick) * nbz + z / kRelaxBrick;
+                if (id != last) {
+                    atomicOr(flags1 + (id >> 6), 1ull << (id & 63u));
+                    last = id;
+                }
+            }
+        }
+    });
+}
+
 }  // namespace
 
 void launch_flood_init(const Launch& L)
@@ -543,6 +732,29 @@ void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t r
         k_flood_plan<<<1, 1024, 0, L.s>>>(L, W.counts);
         VF_CHECK_LAUNCH();
     }
+}
+
+void launch_flood_sub(const Launch& L)
+{
+    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
+    {
+        TimedLaunch tl("flood_sub_init", L.s);
+        k_flood_sub_init<<<L.njobs, 256, 0, L.s>>>(L);
+        VF_CHECK_LAUNCH();
+    }
+    {
+        TimedLaunch tl("flood_sub_touch", L.s);
+        k_flood_sub_touch<<<blocks, kCellThreads, 0, L.s>>>(L);
+        VF_CHECK_LAUNCH();
+    }
+    {
+        TimedLaunch tl("flood_sub_keep", L.s);
+        k_flood_sub_keep<<<L.njobs, 1024, 0, L.s>>>(L);
+        VF_CHECK_LAUNCH();
+    }
+    TimedLaunch tl("flood_sub_reseed", L.s);
+    k_flood_sub_reseed<<<blocks, kCellThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
 }
 
 void launch_flood_finish(const Launch& L)

@@ -66,7 +66,8 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | weight, K = (distance << key_bits) | seed or 0x3FFFFFFC unreached
+    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | (weight - 1), K = (distance << key_bits) | seed; kFloodUnreachedKey | (weight - 1) unreached
+    uint32_t* sub;          // sub-seed pass (null when the seeds carry one id per fragment): S union-find parents, S keep flags
     unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
     uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty
@@ -202,12 +203,19 @@ uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
 // blocks: workgroups of one wave for the compaction and relaxation launches
 void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks);
 void launch_flood_finish(const Launch& L);
+// Sub-seed pass after the first drain (jobs with a non-null sub): merge touching
+// same-fragment seed regions, keep each fragment's smallest-id component, and,
+// where a region was freed, reset the keys and restart the rounds (fctl[1] = 0).
+void launch_flood_sub(const Launch& L);
 
 // seed.hip
 constexpr uint32_t kMaxDeviceSeeds = 4096;
 void launch_seed_count(const Launch& L, uint32_t* tile_cnt);
 // tile_off: exclusive prefix over the global tiles (total_tiles + 1 entries)
 void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
+// Seeder::nearSeeds on job 0 of L; out: F + num_seeds entries; result[2]: count, draw-cap flag
+void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
+                      uint32_t spreading, uint64_t key, uint4* out, uint32_t* result);
 
 // mc.hip: all-labels marching cubes on a raw device grid.  hist_dev holds the
 // grid's countValues histogram (65536 counts); results stay in W until read.

@@ -170,7 +170,126 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
     }
 }
 
+// floor(mn + (mx - mn) * h / 2^32): static_cast<int>(min + (max - min) * u) of
+// RandomUtilities::getUniformRandomInt (RandomUtilities.h:141-144) for
+// u = h / 2^32 wherever that is not negative.
+__device__ inline int64_t uniform_int(int64_t mn, int64_t mx, uint32_t h)
+{
+    const int64_t t = (mx - mn) * (int64_t)h;
+    return mn + (t >= 0 ? (t >> 32) : -((-t + 0xFFFFFFFFll) >> 32));
+}
+
+// Seeder::nearSeeds (Seeder.cpp:49-113) for one grid, one block.  Impact i
+// draws with key k_i = key + (i + 1) * 0x9E3779B97F4A7C15: the fragment seed
+// uniform_int(0, F - 1, h(k_i, 0)), the seed count min(uniform_int(1,
+// pending, h(k_i, 1)), pending), and candidate t's axis a as
+// n_a / 2 - sum_{j < spreading} h(k_i, 2 + (3t + a) spreading + j) mod
+// (n_a / spreading) (getBiasedRandomInt) wrapped around the fragment seed.  A
+// candidate is accepted, in draw order, when it lies within min(n) / 2 of the
+// fragment seed, is occupied, on the surface (isBoundary) and new.  Batches of
+// kSeedThreads candidates are tested in parallel; one lane accepts them in
+// order.  result: [0] seeds written (fragments first), [1] 1 if an impact hit
+// the draw cap.
+__global__ __launch_bounds__(kSeedThreads) void k_seed_near(const JobDev* J, const uint4* frags, uint32_t F,
+                                                             uint32_t impacts, uint32_t num_seeds, uint32_t spreading,
+                                                             uint64_t key, uint4* out, uint32_t* result)
+{
+    __shared__ uint32_t chosen[kMaxDeviceSeeds];
+    __shared__ uint32_t sorted[kMaxDeviceSeeds];
+    __shared__ uint32_t cand[kSeedThreads];
+    __shared__ uint32_t s_n, s_got, s_err;
+    const uint32_t n[3] = { J->nx, J->ny, J->nz };
+    const uint32_t m[3] = { n[0] / spreading, n[1] / spreading, n[2] / spreading };
+    const uint32_t mind = min(n[0], min(n[1], n[2])) / 2;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_err = 0;
+    }
+    __syncthreads();
+    uint32_t pending = num_seeds;
+    for (uint32_t i = 0; i < impacts && !s_err; ++i) {
+        const uint64_t k = key + (uint64_t)(i + 1u) * 0x9E3779B97F4A7C15ull;
+        const uint4 fr = frags[uniform_int(0, (int64_t)F - 1, hash32(k, 0))];
+        const int64_t w = uniform_int(1, pending, hash32(k, 1));
+        const uint32_t want = (uint32_t)max<int64_t>(0, min<int64_t>(w, pending));
+        if (threadIdx.x == 0) s_got = 0;
+        __syncthreads();
+        for (uint32_t t0 = 0; s_got < want; t0 += kSeedThreads) {
+            if (t0 >= kMaxDraws) {
+                if (threadIdx.x == 0) s_err = 1;
+                break;
+            }
+            const uint32_t t = t0 + threadIdx.x;
+            const uint32_t f[3] = { fr.x, fr.y, fr.z };
+            uint32_t c[3];
+            int64_t d2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                uint32_t b = 0;
+                for (uint32_t j = 0; j < spreading; ++j) b += hash32(k, 2u + (3u * t + (uint32_t)a) * spreading + j) % m[a];
+                const int64_t o = (int64_t)(n[a] / 2) - (int64_t)b;
+                c[a] = (uint32_t)(((int64_t)f[a] + o + n[a]) % n[a]);
+                d2 += ((int64_t)c[a] - f[a]) * ((int64_t)c[a] - f[a]);
+            }
+            uint32_t id = 0xFFFFFFFFu;
+            if (t < kMaxDraws && d2 <= (int64_t)mind * mind) {
+                const size_t cell = ((size_t)c[0] * n[1] + c[1]) * n[2] + c[2];
+                if (J->occ[cell] && surface_cell(J, c[0], c[1], c[2])) {
+                    id = (uint32_t)cell;
+                    for (uint32_t q = 0; q < s_n && id != 0xFFFFFFFFu; ++q)
+                        if (chosen[q] == id) id = 0xFFFFFFFFu;
+                }
+            }
+            cand[threadIdx.x] = id;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t first = s_n;
+                for (uint32_t q = 0; q < kSeedThreads && s_got < want; ++q) {
+                    const uint32_t v = cand[q];
+                    if (v == 0xFFFFFFFFu) continue;
+                    bool dup = false;
+                    for (uint32_t r = first; r < s_n && !dup; ++r) dup = chosen[r] == v;
+                    if (dup) continue;
+                    chosen[s_n++] = v;
+                    ++s_got;
+                }
+            }
+            __syncthreads();
+        }
+        pending -= want;
+        __syncthreads();
+    }
+    const uint32_t N = s_n;
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const uint32_t v = chosen[i];
+        uint32_t p = 0;
+        for (uint32_t q = 0; q < N; ++q) p += chosen[q] < v ? 1u : 0u;
+        sorted[p] = v;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) out[i] = frags[i];
+    const uint32_t label0 = frags[F - 1].w;
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const uint32_t v = sorted[i];
+        out[F + i] = make_uint4(v / (n[1] * n[2]), (v / n[2]) % n[1], v % n[2], label0 + 1u + i);
+    }
+    if (threadIdx.x == 0) {
+        result[0] = F + N;
+        result[1] = s_err;
+    }
+}
+
 }  // namespace
+
+void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
+                      uint32_t spreading, uint64_t key, uint4* out, uint32_t* result)
+{
+    if (!F || !spreading || spreading > 64 || num_seeds > kMaxDeviceSeeds)
+        fail(VF_ERR_INVALID_ARG, "near seeding needs fragment seeds, spreading 1..64 and at most %u seeds",
+             kMaxDeviceSeeds);
+    TimedLaunch tl("seed_near", L.s);
+    k_seed_near<<<1, kSeedThreads, 0, L.s>>>(L.jobs, frags, F, impacts, num_seeds, spreading, key, out, result);
+    VF_CHECK_LAUNCH();
+}
 
 void launch_seed_count(const Launch& L, uint32_t* tile_cnt)
 {

@@ -7,6 +7,9 @@
 // names of vfrag_fracture.hpp (RegularGrid, FractureParameters, fracturer::*).
 //
 // usage: fracture_scene naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX
+//        [EXTRA_SEEDS [IMPACTS]]
+// (fragment seeds are drawn with key SEED_KEY, extra seeds with SEED_KEY + 1,
+// impact seeds with SEED_KEY + 2)
 // writes OUT_PREFIX.grid (uint16 cells after detectBoundaries), .seeds
 // (uint32 x,y,z,label), .hist (uint32 label,count pairs) and .mc (uint32
 // label, vertices, triangles per fragment mesh), and OUT_PREFIX.LABEL.binm per
@@ -48,9 +51,10 @@ static void dump(const std::string& path, const T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc != 11) {
+    if (argc < 11 || argc > 13) {
         std::fprintf(stderr,
-                     "usage: %s naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX\n",
+                     "usage: %s naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX "
+                     "[EXTRA_SEEDS [IMPACTS]]\n",
                      argv[0]);
         return 2;
     }
@@ -63,6 +67,8 @@ int main(int argc, char** argv)
         params._distanceFunction = std::atoi(argv[4]);
         params._noiseSeed = std::strtoull(argv[8], nullptr, 10);
         params._noiseAmplitude = (uint32_t)std::atoi(argv[9]);
+        params._numExtraSeeds = argc > 11 ? std::atoi(argv[11]) : 0;
+        params._numImpacts = argc > 12 ? std::atoi(argv[12]) : 0;
         const vfrag::TriangleMesh mesh = vessel(std::strtoull(argv[5], nullptr, 10), (uint32_t)std::atoi(argv[6]));
 
         // allocateMeshGrid + RegularGrid::fill (CADScene.cpp:545-561)
@@ -71,8 +77,18 @@ int main(int argc, char** argv)
         grid.fill(mesh);
 
         // fractureModel (CADScene.cpp:624-691)
-        std::vector<vfrag::uvec4> seeds =
-            fracturer::Seeder::uniform(grid, params._numSeeds, std::strtoull(argv[7], nullptr, 10), fracturer::Seeder::BOTH);
+        const uint64_t key = std::strtoull(argv[7], nullptr, 10);
+        std::vector<vfrag::uvec4> seeds = fracturer::Seeder::uniform(grid, params._numSeeds, key, fracturer::Seeder::BOTH);
+        if (params._numImpacts > 0)
+            seeds = fracturer::Seeder::nearSeeds(grid, seeds, params._numImpacts, params._biasSeeds, params._biasFocus, key + 2);
+        if (params._numExtraSeeds > 0) {
+            std::vector<vfrag::uvec4> extra =
+                fracturer::Seeder::uniform(grid, params._numExtraSeeds, key + 1, fracturer::Seeder::BOTH);
+            extra.insert(extra.begin(), seeds.begin(), seeds.end());
+            fracturer::Seeder::mergeSeeds(seeds, extra,
+                                          static_cast<fracturer::DistanceFunction>(params._mergeSeedsDistanceFunction));
+            seeds.insert(seeds.end(), extra.begin(), extra.end());
+        }
         fracturer::Fracturer* fracturer = nullptr;
         if (params._fractureAlgorithm == FractureParameters::NAIVE) fracturer = fracturer::NaiveFracturer::getInstance();
         else fracturer = fracturer::FloodFracturer::getInstance();

@@ -79,6 +79,8 @@ def lib():
         "vf_grid_synchronize": (I, [P]),
         "vf_voxelize": (I, [P, P, U32, P, U32]), "vf_voxelize_device": (I, [P, P, U32, P, U32]),
         "vf_seed_uniform": (I, [P, U32, U64, I, P]),
+        "vf_seed_near": (I, [P, P, U32, U32, U32, U32, U64, P, C.POINTER(U32)]),
+        "vf_seed_merge": (I, [P, U32, P, U32, I]),
         "vf_fracture_naive": (I, [P, P, U32, I]), "vf_remove_isolated_regions": (I, [P, P, U32]),
         "vf_fracture_flood": (I, [P, P, U32, I, I, U64, U32]),
         "vf_detect_boundaries": (I, [P, I]), "vf_undo_mask": (I, [P]), "vf_homogenize": (I, [P]),
@@ -145,6 +147,14 @@ def vessel_segments(target_triangles: int):
 def seeds_array(seeds) -> np.ndarray:
     a = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32).reshape(-1, 4))
     return a
+
+
+def seed_merge(frags, seeds, distance: int) -> np.ndarray:
+    """Seeder::mergeSeeds: a copy of seeds labelled fragment | sub-seed id << 8."""
+    f = seeds_array(frags)
+    out = np.array(seeds_array(seeds), copy=True)
+    check(lib().vf_seed_merge(_ptr(f), f.shape[0], _ptr(out), out.shape[0], distance))
+    return out
 
 
 class Grid:
@@ -214,6 +224,15 @@ class Grid:
         out = np.zeros((n, 4), np.uint32)
         check(lib().vf_seed_uniform(self._h, n, seed, location, _ptr(out)))
         return out
+
+    def seed_near(self, frags, impacts: int, num_seeds: int, spreading: int, seed: int) -> np.ndarray:
+        """Seeder::nearSeeds: the fragment seeds followed by the impact seeds."""
+        f = seeds_array(frags)
+        out = np.zeros((f.shape[0] + num_seeds, 4), np.uint32)
+        n = C.c_uint32()
+        check(lib().vf_seed_near(self._h, _ptr(f), f.shape[0], impacts, num_seeds, spreading, seed, _ptr(out),
+                                 C.byref(n)))
+        return out[: n.value]
 
     def fracture_naive(self, seeds, distance: int):
         s = seeds_array(seeds)
