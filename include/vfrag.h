@@ -56,6 +56,10 @@ extern "C" {
 #define VF_SEED_INNER 0
 #define VF_SEED_OUTER 1
 #define VF_SEED_BOTH 2
+/* FractureParameters::ErosionType (FractureParameters.h:27) */
+#define VF_EROSION_SQUARE 0
+#define VF_EROSION_ELLIPSE 1
+#define VF_EROSION_CROSS 2
 
 #define VF_ALL_LABELS 0xFFFFFFFFu
 
@@ -150,6 +154,23 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
 int vf_detect_boundaries(vf_grid* g, int size);
 /* RegularGrid::undoMask (RegularGrid.cpp:488-503) */
 int vf_undo_mask(vf_grid* g);
+/* RegularGrid::erode (RegularGrid.cpp:82-159): iterations times
+ * detectBoundaries(1) then one erosion step (erodeGrid-comp.glsl:26-59); then
+ * removeIsolatedRegions.  A cell above FREE whose noise draw
+ * (hash32(noise_seed, cell) >> 8) / 2^24 is below probability becomes EMPTY
+ * when the mask-weighted count of window cells with its raw value (boundary
+ * bit included), over the in-bounds window cells, is below activations *
+ * threshold (FP32).  The window is size^3 (even sizes grow by one, at most
+ * 15): VF_EROSION_SQUARE, _CROSS or _ELLIPSE, as the reference builds them.
+ * Leaves the boundary bits of the last detection set. */
+int vf_erode(vf_grid* g, int convolution, uint32_t size, uint32_t iterations, float probability, float threshold,
+             uint64_t noise_seed);
+/* RegularGrid::removeIsolatedRegions (RegularGrid.cpp:1006-1015,
+ * removeIsolatedRegionsGrid-comp.glsl:16-38): a cell with fewer than 6 other
+ * cells of its raw value in its clamped 3x3x3 window becomes EMPTY (read from
+ * the grid before the pass: the shader's in-place update is thread-order
+ * dependent). */
+int vf_remove_isolated_cells(vf_grid* g);
 /* RegularGrid::homogenize (RegularGrid.cpp:533-541) */
 int vf_homogenize(vf_grid* g);
 /* RegularGrid::countValues (RegularGrid.cpp:601-625): counts[v & 0x7FFF] for v > FREE.

@@ -82,6 +82,7 @@ struct FractureParameters {
     enum FractureAlgorithm : uint8_t { NAIVE, FLOOD, VORONOI, BASE_ALGORITHMS };
     enum DistanceFunction : uint8_t { EUCLIDEAN, MANHATTAN, CHEBYSHEV, DISTANCE_FUNCTIONS };
     enum NeighbourhoodType { VON_NEUMANN, MOORE, NUM_NEIGHBOURHOODS };
+    enum ErosionType { SQUARE, ELLIPSE, CROSS, NUM_EROSION_CONVOLUTIONS };
 
     int _fractureAlgorithm = FLOOD;
     int _distanceFunction = CHEBYSHEV;
@@ -92,6 +93,13 @@ struct FractureParameters {
     int _numImpacts = 0;                   // Seeder::nearSeeds impacts; 0 = uniform fragment seeds only
     int _biasSeeds = 32;                   // nearSeeds: seeds over all impacts
     int _biasFocus = 5;                    // nearSeeds: uniform draws summed per axis (spreading)
+    // erosion instead of detectBoundaries at the end of fractureModel (CADScene.cpp:680-689)
+    bool _erode = false;
+    int _erosionConvolution = ELLIPSE;
+    float _erosionProbability = .5f;
+    int _erosionIterations = 3;
+    int _erosionSize = 3;
+    float _erosionThreshold = 0.5f;
     bool _removeIsolatedRegions = true;    // the reference's GPU pass is an identity; see NaiveFracturer below
     bool _launchGPU = true;
     int _seed = 80;
@@ -156,6 +164,16 @@ public:
     void detectBoundaries(int boundarySize) { check(vf_detect_boundaries(g_, boundarySize)); host_valid_ = false; }
     // RegularGrid::undoMask (RegularGrid.cpp:488-503)
     void undoMask() { check(vf_undo_mask(g_)); host_valid_ = false; }
+    // RegularGrid::erode (RegularGrid.cpp:82-159); the noise buffer is a
+    // counter-based draw keyed by noiseSeed
+    void erode(int erosionType, uint32_t convolutionSize, uint16_t numIterations, float erosionProbability,
+               float erosionThreshold, uint64_t noiseSeed = 0)
+    {
+        check(vf_erode(g_, erosionType, convolutionSize, numIterations, erosionProbability, erosionThreshold, noiseSeed));
+        host_valid_ = false;
+    }
+    // RegularGrid::removeIsolatedRegions (RegularGrid.cpp:1006-1015)
+    void removeIsolatedRegions() { check(vf_remove_isolated_cells(g_)); host_valid_ = false; }
 
     // RegularGrid::countValues (RegularGrid.cpp:601-625): count per label > FREE
     // (boundary bit removed); returns the number of counted cells

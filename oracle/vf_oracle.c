@@ -9,6 +9,7 @@
 #include "vf_oracle.h"
 #include "vfo_mc_table.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -698,6 +699,131 @@ void vfo_remove_isolated(const uint16_t* in, uint16_t* out, const uint32_t dims[
     }
     free(q.v);
     free(ql.v);
+}
+
+/* ========================================================================= */
+/* Erosion                                                                   */
+/* ========================================================================= */
+
+int vfo_erosion_mask(int type, uint32_t size, uint8_t* mask, float* activations)
+{
+    /* RegularGrid::erode (RegularGrid.cpp:82-121): an odd cube of side size
+     * (even sizes grow by one); SQUARE all cells, CROSS the three axis lines
+     * through the centre (activations = maskSize / 3 as the reference sets
+     * it), ELLIPSE the cells whose float distance to the centre is below
+     * centre + FLT_EPSILON.  Returns the side; activations = fraction. */
+    if (size % 2 == 0) ++size;
+    const uint32_t n = size * size * size, c = size / 2;
+    float act = 0.0f;
+    memset(mask, 0, n);
+    if (type == VFO_EROSION_SQUARE) {
+        memset(mask, 1, n);
+        act = (float)n;
+    } else if (type == VFO_EROSION_CROSS) {
+        for (uint32_t i = 0; i < size; ++i) {
+            mask[i * size * size + c * size + c] = 1;
+            mask[c * size * size + i * size + c] = 1;
+            mask[c * size * size + c * size + i] = 1;
+        }
+        act = 1.0f / 3.0f * (float)n;
+    } else if (type == VFO_EROSION_ELLIPSE) {
+        for (uint32_t x = 0; x < size; ++x)
+            for (uint32_t y = 0; y < size; ++y)
+                for (uint32_t z = 0; z < size; ++z) {
+                    float dx = (float)x - (float)c, dy = (float)y - (float)c, dz = (float)z - (float)c;
+                    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                    if (d < (float)c + FLT_EPSILON) {
+                        mask[x * size * size + y * size + z] = 1;
+                        act += 1.0f;
+                    }
+                }
+    } else {
+        return -1;
+    }
+    *activations = act / (float)n;
+    return (int)size;
+}
+
+float vfo_erosion_noise(uint64_t seed, uint32_t cell)
+{
+    /* the reference's uniform float noise buffer (RegularGrid::fillNoiseBuffer,
+     * RegularGrid.cpp:238-244) as a counter-based draw: 24 bits to [0, 1) */
+    return (float)(vfo_hash32(seed, cell) >> 8) * (1.0f / 16777216.0f);
+}
+
+void vfo_erode_step(const uint16_t* in, uint16_t* out, const uint32_t dims[3], const uint8_t* mask, uint32_t size,
+                    float limit, float probability, uint64_t noise_seed)
+{
+    /* erodeGrid-comp.glsl:26-59.  The shader's isBoundary is
+     * unmaskedBit(value, 15) != 0, i.e. any non-EMPTY cell, so every cell above
+     * FREE passing the noise gate is tested; neighbours count when their raw
+     * value (boundary bit included) equals the cell's and the mask is set;
+     * the cell empties when count / (in-bounds window cells) < limit
+     * (limit = activations * threshold, both FP32). */
+    const int nx = (int)dims[0], ny = (int)dims[1], nz = (int)dims[2], h = (int)size / 2;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                uint16_t v = in[id];
+                out[id] = v;
+                if (!(v > VFO_FREE && (v & VFO_MASK15) != 0 && vfo_erosion_noise(noise_seed, (uint32_t)id) < probability))
+                    continue;
+                uint32_t count = 0, global = 0;
+                for (int xx = clampi(x - h, 0, nx - 1); xx <= clampi(x + h, 0, nx - 1); ++xx)
+                    for (int yy = clampi(y - h, 0, ny - 1); yy <= clampi(y + h, 0, ny - 1); ++yy)
+                        for (int zz = clampi(z - h, 0, nz - 1); zz <= clampi(z + h, 0, nz - 1); ++zz) {
+                            if (in[gidx(xx, yy, zz, dims)] == v &&
+                                mask[(xx - (x - h)) * size * size + (yy - (y - h)) * size + (zz - (z - h))])
+                                ++count;
+                            ++global;
+                        }
+                if ((float)count / (float)global < limit) out[id] = VFO_EMPTY;
+            }
+}
+
+void vfo_remove_isolated_cells(const uint16_t* in, uint16_t* out, const uint32_t dims[3])
+{
+    /* removeIsolatedRegionsGrid-comp.glsl:16-38: a cell with fewer than 6
+     * other cells of its raw value in the clamped 3x3x3 window becomes EMPTY.
+     * The shader updates the grid in place, so its result depends on thread
+     * order; this reads the input only. */
+    const int nx = (int)dims[0], ny = (int)dims[1], nz = (int)dims[2];
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                size_t id = gidx(x, y, z, dims);
+                uint16_t v = in[id];
+                int count = -1;
+                for (int xx = clampi(x - 1, 0, nx - 1); xx <= clampi(x + 1, 0, nx - 1); ++xx)
+                    for (int yy = clampi(y - 1, 0, ny - 1); yy <= clampi(y + 1, 0, ny - 1); ++yy)
+                        for (int zz = clampi(z - 1, 0, nz - 1); zz <= clampi(z + 1, 0, nz - 1); ++zz)
+                            count += in[gidx(xx, yy, zz, dims)] == v;
+                out[id] = count < 6 ? VFO_EMPTY : v;
+            }
+}
+
+int vfo_erode(uint16_t* grid, const uint32_t dims[3], int type, uint32_t size, uint32_t iterations, float probability,
+              float threshold, uint64_t noise_seed)
+{
+    /* RegularGrid::erode (RegularGrid.cpp:82-159): per iteration
+     * detectBoundaries(1) then one erosion step; then removeIsolatedRegions. */
+    uint32_t side = size % 2 ? size : size + 1;
+    uint8_t* mask = (uint8_t*)malloc((size_t)side * side * side);
+    float act;
+    if (vfo_erosion_mask(type, size, mask, &act) < 0) { free(mask); return -1; }
+    const float limit = act * threshold;
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2];
+    uint16_t* tmp = (uint16_t*)malloc(V * sizeof(uint16_t));
+    for (uint32_t it = 0; it < iterations; ++it) {
+        vfo_detect_boundaries(grid, tmp, dims, 1);
+        vfo_erode_step(tmp, grid, dims, mask, side, limit, probability, noise_seed);
+    }
+    vfo_remove_isolated_cells(grid, tmp, dims);
+    memcpy(grid, tmp, V * sizeof(uint16_t));
+    free(tmp);
+    free(mask);
+    return 0;
 }
 
 /* ========================================================================= */

@@ -59,6 +59,17 @@ int vfo_flood_dijkstra(const uint16_t* in, uint16_t* out, const uint32_t dims[3]
 void vfo_remove_isolated(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
                          const uint32_t* seeds, uint32_t S);
 
+/* ---- erosion (RegularGrid::erode) --------------------------------------- */
+enum { VFO_EROSION_SQUARE = 0, VFO_EROSION_ELLIPSE = 1, VFO_EROSION_CROSS = 2 };  /* FractureParameters::ErosionType */
+/* mask: side^3 bytes (side = size rounded up to odd); returns side or -1 */
+int vfo_erosion_mask(int type, uint32_t size, uint8_t* mask, float* activations);
+float vfo_erosion_noise(uint64_t seed, uint32_t cell);
+void vfo_erode_step(const uint16_t* in, uint16_t* out, const uint32_t dims[3], const uint8_t* mask, uint32_t side,
+                    float limit, float probability, uint64_t noise_seed);
+void vfo_remove_isolated_cells(const uint16_t* in, uint16_t* out, const uint32_t dims[3]);
+int vfo_erode(uint16_t* grid, const uint32_t dims[3], int type, uint32_t size, uint32_t iterations, float probability,
+              float threshold, uint64_t noise_seed);
+
 /* ---- seeding ------------------------------------------------------------ */
 enum { VFO_SEED_INNER = 0, VFO_SEED_OUTER = 1, VFO_SEED_BOTH = 2 };  /* Seeder::Location order (Seeder.h:24) */
 uint32_t vfo_hash32(uint64_t key, uint32_t counter);

@@ -51,6 +51,11 @@ def lib():
         "vfo_remove_isolated": (None, [P, P, P, P, U32]),
         "vfo_seed_uniform": (I, [P, P, U32, U64, I, P]),
         "vfo_seed_merge": (I, [P, U32, P, U32, I]),
+        "vfo_erosion_mask": (I, [I, U32, P, C.POINTER(C.c_float)]),
+        "vfo_erosion_noise": (C.c_float, [U64, U32]),
+        "vfo_erode_step": (None, [P, P, P, P, U32, C.c_float, C.c_float, U64]),
+        "vfo_remove_isolated_cells": (None, [P, P, P]),
+        "vfo_erode": (I, [P, P, I, U32, U32, C.c_float, C.c_float, U64]),
         "vfo_near_key": (U64, [U64, U32]),
         "vfo_seed_near": (I, [P, P, P, U32, U32, U32, U32, U64, P, P]),
         "vfo_marching_cubes": (I, [P, P, P, P, U32, C.POINTER(Mesh)]),
@@ -194,6 +199,44 @@ def seed_near(grid, frags, impacts, num_seeds, spreading, seed):
     if rc != 0:
         raise ValueError(f"oracle near seeding failed ({rc})")
     return out[: n.value]
+
+
+def erosion_mask(kind, size):
+    """(mask[side, side, side] uint8, activations float32)"""
+    side = size if size % 2 else size + 1
+    m = np.zeros(side ** 3, np.uint8)
+    act = C.c_float()
+    if lib().vfo_erosion_mask(kind, size, _p(m), C.byref(act)) < 0:
+        raise ValueError("bad erosion type")
+    return m.reshape(side, side, side), np.float32(act.value)
+
+
+def erosion_noise(seed, cell):
+    return np.float32(lib().vfo_erosion_noise(seed, cell))
+
+
+def erode_step(grid, mask, limit, probability, noise_seed):
+    g = np.ascontiguousarray(grid, np.uint16)
+    m = np.ascontiguousarray(mask, np.uint8)
+    out = np.empty_like(g)
+    lib().vfo_erode_step(_p(g), _p(out), _p(_dims(g.shape)), _p(m), m.shape[0], float(limit), float(probability),
+                         noise_seed)
+    return out
+
+
+def remove_isolated_cells(grid):
+    g = np.ascontiguousarray(grid, np.uint16)
+    out = np.empty_like(g)
+    lib().vfo_remove_isolated_cells(_p(g), _p(out), _p(_dims(g.shape)))
+    return out
+
+
+def erode(grid, kind, size, iterations, probability, threshold, noise_seed):
+    g = np.array(grid, np.uint16, copy=True)
+    if lib().vfo_erode(_p(g), _p(_dims(g.shape)), kind, size, iterations, float(probability), float(threshold),
+                       noise_seed) != 0:
+        raise ValueError("bad erosion arguments")
+    return g
 
 
 def marching_cubes(grid, aabb_min, aabb_max, label=0xFFFFFFFF, smooth=None):

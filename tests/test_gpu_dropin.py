@@ -18,20 +18,21 @@ DRIVER = ROOT / "voxelfragmentml_amd" / "dropin" / "fracture_scene"
 BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
 
 
-@pytest.mark.parametrize("algo,edge,S,dist,mesh_seed,tris,key,noise_seed,amp,extra,impacts", [
-    ("naive", 96, 8, 1, 1, 20000, 11, 0, 0, 0, 0),
-    ("flood", 80, 12, 0, 2, 30000, 12, 77, 2, 0, 0),
-    ("flood", 72, 9, 1, 3, 8000, 13, 5, 1, 0, 0),
-    ("flood", 80, 8, 1, 4, 20000, 14, 3, 1, 16, 0),   # CADScene: numExtraSeeds = 2 x fragments
-    ("flood", 88, 6, 2, 5, 20000, 15, 0, 0, 12, 2),   # plus two impacts
-    ("naive", 64, 5, 0, 6, 10000, 16, 0, 0, 10, 1),
+@pytest.mark.parametrize("algo,edge,S,dist,mesh_seed,tris,key,noise_seed,amp,extra,impacts,erode", [
+    ("naive", 96, 8, 1, 1, 20000, 11, 0, 0, 0, 0, 0),
+    ("flood", 80, 12, 0, 2, 30000, 12, 77, 2, 0, 0, 0),
+    ("flood", 72, 9, 1, 3, 8000, 13, 5, 1, 0, 0, 0),
+    ("flood", 80, 8, 1, 4, 20000, 14, 3, 1, 16, 0, 0),   # CADScene: numExtraSeeds = 2 x fragments
+    ("flood", 88, 6, 2, 5, 20000, 15, 0, 0, 12, 2, 0),   # plus two impacts
+    ("naive", 64, 5, 0, 6, 10000, 16, 0, 0, 10, 1, 0),
+    ("flood", 80, 7, 2, 7, 20000, 17, 9, 1, 14, 0, 1),   # erosion instead of detectBoundaries
 ])
 def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_seed, tris, key, noise_seed, amp, extra,
-                                      impacts):
+                                      impacts, erode):
     assert DRIVER.exists(), "drop-in driver not built"
     out = tmp_path / "scene"
     args = [str(DRIVER), algo, str(edge), str(S), str(dist), str(mesh_seed), str(tris), str(key), str(noise_seed),
-            str(amp), str(out), str(extra), str(impacts)]
+            str(amp), str(out), str(extra), str(impacts), str(erode)]
     print("dropin:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -53,7 +54,10 @@ def test_dropin_driver_matches_oracle(vf, tmp_path, algo, edge, S, dist, mesh_se
         lab = O.naive(base, ref_seeds, dist)
     else:
         lab = O.flood(base, ref_seeds, 6 if dist == 1 else 26, noise_seed, amp)
-    ref = O.detect_boundaries(lab, 1)
+    if erode:  # FractureParameters defaults: ELLIPSE 3, 3 iterations, probability 0.5, threshold 0.5
+        ref = O.erode(lab, 1, 3, 3, 0.5, 0.5, noise_seed)
+    else:
+        ref = O.detect_boundaries(lab, 1)
     np.testing.assert_array_equal(grid, ref)
 
     counts = O.histogram(ref)

@@ -637,6 +637,56 @@ int vf_undo_mask(vf_grid* g)
     });
 }
 
+int vf_erode(vf_grid* g, int convolution, uint32_t size, uint32_t iterations, float probability, float threshold,
+             uint64_t noise_seed)
+{
+    return guard([&] {
+        checked(g);
+        std::vector<uint8_t> mask;
+        float act = 0.0f;
+        if (size == 0 || size > kMaxErosionSide) fail(VF_ERR_INVALID_ARG, "erosion size %u outside 1..%u", size, kMaxErosionSide);
+        const int side = erosion_mask(convolution, size, mask, act);
+        if (side < 0) fail(VF_ERR_INVALID_ARG, "unknown erosion convolution %d", convolution);
+        const float limit = act * threshold;
+        grid_materialize(g);
+        g->erode_tmp.ensure(g->V * 2 + 64);
+        g->erode_mask.ensure(mask.size() + 64);
+        VF_HIP(hipMemcpyAsync(g->erode_mask.p, mask.data(), mask.size(), hipMemcpyHostToDevice, g->stream));
+        if (g->job.bsize != 1) {
+            g->job.bsize = 1;
+            g->job_dirty = true;
+        }
+        Launch L = grid_launch(g);
+        const uint32_t bricks = brick_count(g->dims[0], g->dims[1], g->dims[2]);
+        for (uint32_t it = 0; it < iterations; ++it) {
+            launch_boundaries_tiled(L, g->prefix.as<uint32_t>() + 2, bricks, false, false);
+            launch_erode_step(L, g->erode_tmp.as<uint16_t>(), g->erode_mask.as<uint8_t>(), (uint32_t)side, limit,
+                              probability, noise_seed, g->V);
+            VF_HIP(hipMemcpyAsync(g->labels.p, g->erode_tmp.p, g->V * 2, hipMemcpyDeviceToDevice, g->stream));
+        }
+        launch_isolated_cells(L, g->erode_tmp.as<uint16_t>(), g->V);
+        VF_HIP(hipMemcpyAsync(g->labels.p, g->erode_tmp.p, g->V * 2, hipMemcpyDeviceToDevice, g->stream));
+        launch_labels_to_occ(L);
+        g->mc.valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
+int vf_remove_isolated_cells(vf_grid* g)
+{
+    return guard([&] {
+        checked(g);
+        grid_materialize(g);
+        g->erode_tmp.ensure(g->V * 2 + 64);
+        Launch L = grid_launch(g);
+        launch_isolated_cells(L, g->erode_tmp.as<uint16_t>(), g->V);
+        VF_HIP(hipMemcpyAsync(g->labels.p, g->erode_tmp.p, g->V * 2, hipMemcpyDeviceToDevice, g->stream));
+        launch_labels_to_occ(L);
+        g->mc.valid = false;
+        VF_HIP(hipStreamSynchronize(g->stream));
+    });
+}
+
 int vf_homogenize(vf_grid* g)
 {
     return guard([&] {

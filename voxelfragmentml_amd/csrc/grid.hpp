@@ -16,7 +16,7 @@ struct vf_grid {
     vf::DevBuf vox_setup, vox_counts, vox_shell_off, vox_col_off, scan_tmp, occ_tmp;
     vf::DevBuf mesh_xyz, mesh_tri;
     vf::DevBuf seed_tile_cnt, seed_tile_off;
-    vf::DevBuf keys, buckets, fctl, ctl, flood_lists, flood_sub, near_in, near_out;
+    vf::DevBuf keys, buckets, fctl, ctl, flood_lists, flood_sub, near_in, near_out, erode_tmp, erode_mask;
     vf::DevBuf ri_a, ri_b, ri_cnt, ri_vis, ri_out;
     vf::McScratch mc;              // marching-cubes workspace and last result
 

@@ -7,6 +7,8 @@
 // once for all jobs of a batch, so uneven grids share one launch.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace vf {
@@ -203,6 +205,14 @@ uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
 // blocks: workgroups of one wave for the compaction and relaxation launches
 void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t rounds, uint32_t blocks);
 void launch_flood_finish(const Launch& L);
+// erode.hip: RegularGrid::erode.  erosion_mask returns the odd window side (or -1) and fills
+// side^3 mask bytes and the active fraction; the stencils read L.jobs[0]'s labels.
+constexpr uint32_t kMaxErosionSide = 15;
+int erosion_mask(int type, uint32_t size, std::vector<uint8_t>& mask, float& activations);
+void launch_erode_step(const Launch& L, uint16_t* out, const uint8_t* mask, uint32_t side, float limit,
+                       float probability, uint64_t noise_seed, uint64_t V);
+void launch_isolated_cells(const Launch& L, uint16_t* out, uint64_t V);
+
 // Sub-seed pass after the first drain (jobs with a non-null sub): merge touching
 // same-fragment seed regions, keep each fragment's smallest-id component, and,
 // where a region was freed, reset the keys and restart the rounds (fctl[1] = 0).

@@ -7,9 +7,10 @@
 // names of vfrag_fracture.hpp (RegularGrid, FractureParameters, fracturer::*).
 //
 // usage: fracture_scene naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX
-//        [EXTRA_SEEDS [IMPACTS]]
+//        [EXTRA_SEEDS [IMPACTS [ERODE]]]
 // (fragment seeds are drawn with key SEED_KEY, extra seeds with SEED_KEY + 1,
-// impact seeds with SEED_KEY + 2)
+// impact seeds with SEED_KEY + 2; ERODE 1 erodes with the default parameters
+// and noise key NOISE_SEED instead of detecting boundaries)
 // writes OUT_PREFIX.grid (uint16 cells after detectBoundaries), .seeds
 // (uint32 x,y,z,label), .hist (uint32 label,count pairs) and .mc (uint32
 // label, vertices, triangles per fragment mesh), and OUT_PREFIX.LABEL.binm per
@@ -51,10 +52,10 @@ static void dump(const std::string& path, const T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc < 11 || argc > 13) {
+    if (argc < 11 || argc > 14) {
         std::fprintf(stderr,
                      "usage: %s naive|flood EDGE SEEDS DISTANCE MESH_SEED TRIANGLES SEED_KEY NOISE_SEED NOISE_AMP OUT_PREFIX "
-                     "[EXTRA_SEEDS [IMPACTS]]\n",
+                     "[EXTRA_SEEDS [IMPACTS [ERODE]]]\n",
                      argv[0]);
         return 2;
     }
@@ -69,6 +70,7 @@ int main(int argc, char** argv)
         params._noiseAmplitude = (uint32_t)std::atoi(argv[9]);
         params._numExtraSeeds = argc > 11 ? std::atoi(argv[11]) : 0;
         params._numImpacts = argc > 12 ? std::atoi(argv[12]) : 0;
+        params._erode = argc > 13 && std::atoi(argv[13]) != 0;
         const vfrag::TriangleMesh mesh = vessel(std::strtoull(argv[5], nullptr, 10), (uint32_t)std::atoi(argv[6]));
 
         // allocateMeshGrid + RegularGrid::fill (CADScene.cpp:545-561)
@@ -97,7 +99,11 @@ int main(int argc, char** argv)
             return 1;
         }
         fracturer->build(grid, seeds, &params);
-        grid.detectBoundaries(1);
+        if (params._erode)  // CADScene.cpp:680-689
+            grid.erode(params._erosionConvolution, params._erosionSize, params._erosionIterations,
+                       params._erosionProbability, params._erosionThreshold, params._noiseSeed);
+        else
+            grid.detectBoundaries(1);
         dump(out + ".grid", grid.data(), grid.length());
         dump(out + ".seeds", reinterpret_cast<const uint32_t*>(seeds.data()), 4 * seeds.size());
 

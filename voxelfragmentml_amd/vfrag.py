@@ -18,6 +18,7 @@ EUCLIDEAN, MANHATTAN, CHEBYSHEV = 0, 1, 2
 NAIVE, FLOOD = 0, 1
 NEIGH_FROM_DISTANCE, NEIGH_VON_NEUMANN, NEIGH_MOORE = 0, 6, 26
 SEED_INNER, SEED_OUTER, SEED_BOTH = 0, 1, 2  # Seeder::Location order
+EROSION_SQUARE, EROSION_ELLIPSE, EROSION_CROSS = 0, 1, 2  # FractureParameters::ErosionType order
 ALL_LABELS = 0xFFFFFFFF
 
 ERRORS = {
@@ -84,6 +85,7 @@ def lib():
         "vf_fracture_naive": (I, [P, P, U32, I]), "vf_remove_isolated_regions": (I, [P, P, U32]),
         "vf_fracture_flood": (I, [P, P, U32, I, I, U64, U32]),
         "vf_detect_boundaries": (I, [P, I]), "vf_undo_mask": (I, [P]), "vf_homogenize": (I, [P]),
+        "vf_erode": (I, [P, I, U32, U32, F, F, U64]), "vf_remove_isolated_cells": (I, [P]),
         "vf_histogram": (I, [P, P, U32]),
         "vf_marching_cubes": (I, [P, U32, C.POINTER(MeshInfo)]),
         "vf_mesh_read": (I, [P, P, P, P, P, P, P]),
@@ -252,6 +254,15 @@ class Grid:
 
     def undo_mask(self):
         check(lib().vf_undo_mask(self._h))
+
+    def erode(self, convolution: int, size: int, iterations: int, probability: float, threshold: float,
+              noise_seed: int = 0):
+        """RegularGrid::erode: boundary detection + erosion steps, then the isolated-cell cleanup."""
+        check(lib().vf_erode(self._h, convolution, size, iterations, probability, threshold, noise_seed))
+
+    def remove_isolated_cells(self):
+        """RegularGrid::removeIsolatedRegions (the grid pass)."""
+        check(lib().vf_remove_isolated_cells(self._h))
 
     def homogenize(self):
         check(lib().vf_homogenize(self._h))
