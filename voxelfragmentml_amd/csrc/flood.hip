@@ -33,7 +33,7 @@ namespace {
 
 // Flood key of a cell: 0 = EMPTY; otherwise (K << 2) | w with w = 1 + noise in
 // 1..3 the cell's node weight and K = (D << b) | seed index, or K = 0x3FFFFFFC
-// while unreached.  Keys compare by K first (the weight bits only break ties
+// (key kUnreachedKey | w) while unreached.  Keys compare by K first (the weight bits only break ties
 // between equal K, which never changes K), so the minimum over neighbours is
 // still the (distance, seed) minimum, and a cell's weight travels with it.
 constexpr uint32_t kEmptyKey = 0u;
@@ -167,21 +167,24 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     // the lane's own z row of 8 cells
     const uint32_t x = lane >> 3, y = lane & 7u;
     const uint32_t own = ((x + 1u) * 10u + (y + 1u)) * kRowWords;
-    const uint32_t b = R.b;
     const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
-    // k: the row's keys (EMPTY as kTileEmpty); w: the weight bits of occupied cells (0 = EMPTY);
-    // wb: the weight added to K, already shifted into key position
-    uint32_t k[8];
-    uint32_t wmask = 0;  // 2 bits per cell: its weight (0 = EMPTY); out-of-grid cells load as EMPTY
+    const uint32_t wsh = R.b + 2u;
+    // k: the row's keys (EMPTY as kTileEmpty).  add: what a neighbour's key (weight bits cleared)
+    // gains on entering the cell, (w << (b + 2)) + w, or all ones for EMPTY cells, so that
+    //     relaxed = min(k, sat_add(m & ~3, add))
+    // leaves EMPTY cells alone and never lets an EMPTY (~0) or unreached (>= kUnreachedKey) neighbour
+    // win: their sums either saturate or stay above every key they could replace.
+    uint32_t k[8], k0[8], add[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
         k[z] = tile[own + 1 + z];
-        wmask |= (k[z] != kTileEmpty ? (k[z] & 3u) : 0u) << (2 * z);
+        k0[z] = k[z];
+        const uint32_t w = k[z] & 3u;
+        add[z] = k[z] != kTileEmpty ? (w << wsh) + w : 0xFFFFFFFFu;
     }
-    const uint32_t wsh = b + 2u;
-#define W_(z) ((wmask >> (2 * (z))) & 3u)
-    uint32_t changed = 0;  // bit z: cell z dropped in this visit
-    bool overflow = false;
+    auto relax = [&](uint32_t cur, uint32_t nb, uint32_t a) {
+        return min(cur, __builtin_elementwise_add_sat(nb & ~3u, a));
+    };
     uint32_t sweeps = 0;
     for (;;) {
         uint32_t m[8];
@@ -221,50 +224,41 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
 #pragma unroll
             for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]);
         }
-        // candidate (m's K + weight) with the cell's own weight bits; it must stay below
-        // kUnreachedKey, and m & ~3 >= kUnreachedKey - wb flags a key overflow
-        uint32_t ch = 0;
+        uint32_t prev[8];
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
-            const uint32_t w = W_(z), wb = w << wsh;
-            if (w && m[z] < kUnreachedKey) {
-                const uint32_t mh = m[z] & ~3u;
-                if (mh >= kUnreachedKey - wb) overflow = true;
-                else if (mh + wb + w < k[z]) { k[z] = mh + wb + w; ch |= 1u << z; }
-            }
+            prev[z] = k[z];
+            k[z] = relax(k[z], m[z], add[z]);
         }
         // z sweeps in registers: the row's own neighbours along z
 #pragma unroll
-        for (int z = 1; z < 8; ++z) {
-            const uint32_t w = W_(z), wb = w << wsh, mh = k[z - 1] & ~3u;
-            if (w && k[z - 1] < kUnreachedKey && mh < kUnreachedKey - wb && mh + wb + w < k[z]) {
-                k[z] = mh + wb + w;
-                ch |= 1u << z;
-            }
-        }
+        for (int z = 1; z < 8; ++z) k[z] = relax(k[z], k[z - 1], add[z]);
 #pragma unroll
-        for (int z = 6; z >= 0; --z) {
-            const uint32_t w = W_(z), wb = w << wsh, mh = k[z + 1] & ~3u;
-            if (w && k[z + 1] < kUnreachedKey && mh < kUnreachedKey - wb && mh + wb + w < k[z]) {
-                k[z] = mh + wb + w;
-                ch |= 1u << z;
-            }
-        }
-        if (ch) {
+        for (int z = 6; z >= 0; --z) k[z] = relax(k[z], k[z + 1], add[z]);
+        uint32_t diff = 0;
+#pragma unroll
+        for (int z = 0; z < 8; ++z) diff |= k[z] ^ prev[z];
+        if (diff) {
 #pragma unroll
             for (int z = 0; z < 8; ++z) tile[own + 1 + z] = k[z];
         }
-        changed |= ch;
         ++sweeps;
         __syncthreads();
-        if (__ballot(ch != 0) == 0ull) break;
+        if (__ballot(diff != 0) == 0ull) break;
         if (sweeps >= kMaxSweeps) {
             error = true;
             break;
         }
     }
-#undef W_
-    if (overflow) error = true;
+    // Key overflow: a reached key within one step (at most weight 3) of kUnreachedKey could have a
+    // successor that does not fit; flag it (conservatively) where the key was made.
+    uint32_t changed = 0;  // bit z: cell z dropped in this visit
+    const uint32_t headroom = kUnreachedKey - ((3u << wsh) + 3u);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        changed |= (k[z] != k0[z] ? 1u : 0u) << z;
+        if (k[z] < kUnreachedKey && (k[z] & ~3u) >= headroom) error = true;
+    }
 
     // write back the cells that dropped
 #pragma unroll

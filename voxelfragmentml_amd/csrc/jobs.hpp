@@ -68,7 +68,7 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | (weight - 1), K = (distance << key_bits) | seed; kFloodUnreachedKey | (weight - 1) unreached
+    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | weight (1..3), K = (distance << key_bits) | seed; kFloodUnreachedKey | weight unreached
     uint32_t* sub;          // sub-seed pass (null when the seeds carry one id per fragment): S union-find parents, S keep flags
     unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
