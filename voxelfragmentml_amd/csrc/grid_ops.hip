@@ -278,12 +278,14 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         auto load_row = [&](int gx, int gy, int zs, uint16_t w[10]) {
             const int cx = min(max(gx, 0), nx - 1), cy = min(max(gy, 0), ny - 1);
             const uint16_t* r = lab + ((size_t)cx * ny + cy) * nz;
-            if (zs >= 1 && zs + 9 <= nz && (nz & 7) == 0 && (zs & 7) == 0) {
-                const uint4 q = *reinterpret_cast<const uint4*>(r + zs);
-                w[1] = (uint16_t)q.x; w[2] = (uint16_t)(q.x >> 16); w[3] = (uint16_t)q.y; w[4] = (uint16_t)(q.y >> 16);
-                w[5] = (uint16_t)q.z; w[6] = (uint16_t)(q.z >> 16); w[7] = (uint16_t)q.w; w[8] = (uint16_t)(q.w >> 16);
-                w[0] = r[zs - 1];
-                w[9] = r[zs + 8];
+            // zs is a multiple of 8, so with nz a multiple of 4 the 8 cells start 8-byte aligned
+            if (zs + 8 <= nz && (nz & 3) == 0) {
+                const uint2 a = *reinterpret_cast<const uint2*>(r + zs);
+                const uint2 q = *reinterpret_cast<const uint2*>(r + zs + 4);
+                w[1] = (uint16_t)a.x; w[2] = (uint16_t)(a.x >> 16); w[3] = (uint16_t)a.y; w[4] = (uint16_t)(a.y >> 16);
+                w[5] = (uint16_t)q.x; w[6] = (uint16_t)(q.x >> 16); w[7] = (uint16_t)q.y; w[8] = (uint16_t)(q.y >> 16);
+                w[0] = r[max(zs - 1, 0)];        // clamped window: z = -1 reads z = 0
+                w[9] = r[min(zs + 8, nz - 1)];
             } else {
 #pragma unroll
                 for (int k = 0; k < 10; ++k) w[k] = r[min(max(zs - 1 + k, 0), nz - 1)];
@@ -371,13 +373,11 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         add_checksum();
         if (changed) {
             uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
-            if (nval == 8 && (nz & 7) == 0) {
-                uint4 q;
-                q.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-                q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-                q.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-                q.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-                *reinterpret_cast<uint4*>(dst) = q;
+            if (nval == 8 && (nz & 3) == 0) {
+                reinterpret_cast<uint2*>(dst)[0] =
+                    make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+                reinterpret_cast<uint2*>(dst)[1] =
+                    make_uint2((uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16));
             } else {
                 for (int k = 0; k < nval; ++k) dst[k] = v[k];
             }
