@@ -212,10 +212,21 @@ constexpr int kAZ = kBTZ + 1;  // z-pass row stride (words): odd, so rows spread
 // the z window is done in registers; 144 of the threads also do one segment
 // of the 36 halo rows.  Only the z-window max/min of the 100 rows goes
 // through LDS, for the 3x3 x/y window of each labelled cell.
+// lane-wise max of two packed u16 pairs
+__device__ inline uint32_t pk_max_u16(uint32_t a, uint32_t b)
+{
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+// a label as the packed pair (label | 0xFFFF - label)
+__device__ inline uint32_t packed_self(uint32_t m) { return m | ((0xFFFFu - m) << 16); }
+
 template <bool HIST, bool CSUM>
 __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
 {
-    __shared__ uint32_t zmm[kBHX * kBHY * kAZ];  // z-window max (low 16 bits) | min (high 16), row = hx * kBHY + hy
+    // z-window max (low 16 bits) | 0xFFFF - min (high 16) over labelled cells (EMPTY / FREE contribute 0),
+    // so one packed u16 max combines both halves; row = hx * kBHY + hy
+    __shared__ uint32_t zmm[kBHX * kBHY * kAZ];
     __shared__ uint32_t bins[HIST ? kTiledBins : 1];
     __shared__ JobDev sJ;
     __shared__ BlockSum csum_part;
@@ -250,20 +261,16 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         else if (r < 2 * kBHY + kBTX) { hhx = r - 2 * kBHY + 1; hhy = 0; }
         else { hhx = r - 2 * kBHY - kBTX + 1; hhy = kBHY - 1; }
     }
-    // z-window max/min of 8 cells from their 10 raw values w[0..9] (w[k+1] is cell k)
+    // packed z-window (max | 0xFFFF - min) of 8 cells from their 10 raw values w[0..9] (w[k+1] is cell k)
     auto zwin = [](const uint16_t w[10], uint32_t out[8]) {
-        uint16_t mx[10], mn[10];
+        uint32_t p[10];
 #pragma unroll
         for (int k = 0; k < 10; ++k) {
-            const uint16_t m = w[k] & kMask15;
-            const bool lab = m > kFree;
-            mx[k] = lab ? m : (uint16_t)0;
-            mn[k] = lab ? m : (uint16_t)0xFFFF;
+            const uint32_t m = w[k] & kMask15;
+            p[k] = m > kFree ? packed_self(m) : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            out[k] = (uint32_t)max(max(mx[k], mx[k + 1]), mx[k + 2]) |
-                     ((uint32_t)min(min(mn[k], mn[k + 1]), mn[k + 2]) << 16);
+        for (int k = 0; k < 8; ++k) out[k] = pk_max_u16(pk_max_u16(p[k], p[k + 1]), p[k + 2]);
     };
     walk_tiles(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
         if (!J.active || J.bsize > 1) return;
@@ -352,23 +359,20 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             for (int k = 0; k < 8; ++k) dst[k] = zw[k];
         }
         __syncthreads();
-        // x and y windows together, only for the labelled cells still without the bit
+        // x and y windows together: a labelled cell without the bit is interior iff the packed
+        // window (max | 0xFFFF - min) equals its own packed value; cells beyond the grid are 0
         bool changed = false;
-        for (int k = 0; k < nval; ++k) {
-            const uint16_t c = v[k];
-            if (c <= kFree || (c & 0x8000u)) continue;
-            const int z = seg + k;
-            uint16_t mx = 0, mn = 0xFFFF;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t acc = 0;
+#pragma unroll
             for (int dx = 0; dx < 3; ++dx)
-                for (int dy = 0; dy < 3; ++dy) {
-                    const uint32_t q = zmm[((tx + dx) * kBHY + ty + dy) * kAZ + z];
-                    mx = max(mx, (uint16_t)q);
-                    mn = min(mn, (uint16_t)(q >> 16));
-                }
-            if (mx != c || mn != c) {
-                v[k] = (uint16_t)(c | 0x8000u);
-                changed = true;
-            }
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) acc = pk_max_u16(acc, zmm[((tx + dx) * kBHY + ty + dy) * kAZ + seg + k]);
+            const uint32_t c = v[k];
+            const bool bnd = c > kFree && (c & 0x8000u) == 0 && acc != packed_self(c);
+            v[k] = bnd ? (uint16_t)(c | 0x8000u) : (uint16_t)c;
+            changed |= bnd;
         }
         add_checksum();
         if (changed) {
