@@ -711,10 +711,12 @@ void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t r
     if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
     if (W.list_cap < flood_list_cap(W.total_tasks)) fail(VF_ERR_INVALID_ARG, "flood worklists too small");
     blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    // one compaction wave per task is enough (tasks are dealt round-robin over the XCDs)
+    const uint32_t cblocks = std::max(kXcds, std::min(blocks, (W.total_tasks + kXcds - 1) / kXcds * kXcds));
     for (uint32_t i = 0; i < rounds; ++i) {
         {
             TimedLaunch tl("flood_compact", L.s);
-            k_flood_compact<<<blocks, 64, 0, L.s>>>(L, W.task_prefix, W.total_tasks, W.lists, W.list_cap, W.counts);
+            k_flood_compact<<<cblocks, 64, 0, L.s>>>(L, W.task_prefix, W.total_tasks, W.lists, W.list_cap, W.counts);
             VF_CHECK_LAUNCH();
         }
         {

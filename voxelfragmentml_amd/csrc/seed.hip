@@ -138,7 +138,8 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
         sorted[p] = r;
     }
     __syncthreads();
-    // rank -> cell: tile by binary search over the tile prefix, then a scan of the tile
+    // rank -> tile: one binary search over the tile prefix per seed (the ranks are sorted, so
+    // chosen[] is free to hold the tile index)
     for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
         const unsigned long long want = base + sorted[i];
         uint32_t lo = t0, hi = t1;  // largest t with tile_off[t] <= want
@@ -146,24 +147,61 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
             const uint32_t mid = (lo + hi) >> 1;
             if (tile_off[mid] <= want) lo = mid; else hi = mid;
         }
-        uint32_t need = (uint32_t)(want - tile_off[lo]);
-        uint64_t c = (uint64_t)(lo - t0) * kTileCells;
-        uint32_t x, y, z;
-        decompose(c, J, x, y, z);
-        bool found = false;
-        for (; c < J->V; ++c) {
-            if (candidate(J, c, x, y, z)) {
-                if (need == 0) {
-                    found = true;
-                    break;
+        chosen[i] = lo;
+    }
+    __syncthreads();
+    // tile -> cell: one wave per seed; lane l counts the candidates among the tile's cells
+    // [32 l, 32 l + 32), a wave scan finds the lane holding the wanted one, and that lane walks
+    // its 32 cells.  Cells outside the grid are no candidates.
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t i = wave; i < S; i += kSeedThreads / 64u) {
+        const uint32_t t = chosen[i];
+        const uint32_t need = (uint32_t)(base + sorted[i] - tile_off[t]);
+        const uint64_t c0 = (uint64_t)(t - t0) * kTileCells + (uint64_t)lane * 32u;
+        uint32_t cnt = 0;
+        if (c0 < J->V) {
+            if (J->seed_location == VF_SEED_BOTH) {
+                if (c0 + 32u <= J->V) {
+                    const uint4 a = *reinterpret_cast<const uint4*>(J->occ + c0);
+                    const uint4 b = *reinterpret_cast<const uint4*>(J->occ + c0 + 16u);
+                    const uint64_t m = 0x0101010101010101ull;
+                    cnt = __popcll((((uint64_t)a.y << 32) | a.x) & m) + __popcll((((uint64_t)a.w << 32) | a.z) & m) +
+                          __popcll((((uint64_t)b.y << 32) | b.x) & m) + __popcll((((uint64_t)b.w << 32) | b.z) & m);
+                } else {
+                    for (uint64_t c = c0; c < J->V; ++c) cnt += J->occ[c] != 0;
                 }
-                --need;
+            } else {
+                uint32_t x, y, z;
+                decompose(c0, J, x, y, z);
+                for (uint64_t c = c0; c < c0 + 32u && c < J->V; ++c) {
+                    cnt += candidate(J, c, x, y, z) ? 1u : 0u;
+                    advance(x, y, z, J);
+                }
             }
-            advance(x, y, z, J);
         }
-        if (found) {
-            J->seeds[i] = make_uint4(x, y, z, 2u + i);
-        } else {
+        uint32_t inc = cnt;  // wave inclusive scan
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += v;
+        }
+        const bool holder = inc > need && inc - cnt <= need;
+        if (holder) {
+            uint32_t left = need - (inc - cnt);
+            uint32_t x, y, z;
+            decompose(c0, J, x, y, z);
+            for (uint64_t c = c0; c < c0 + 32u && c < J->V; ++c) {
+                if (candidate(J, c, x, y, z)) {
+                    if (left == 0) {
+                        J->seeds[i] = make_uint4(x, y, z, 2u + i);
+                        break;
+                    }
+                    --left;
+                }
+                advance(x, y, z, J);
+            }
+        }
+        if (__ballot(holder) == 0ull && lane == 0) {  // the counts disagree with the tile prefix
             J->seeds[i] = make_uint4(0u, 0u, 0u, 2u + i);
             atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
         }
