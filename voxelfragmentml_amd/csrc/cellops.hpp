@@ -22,13 +22,13 @@ struct Cell8 {
 __device__ inline void load8(const uint16_t* p, uint64_t c0, uint64_t V, uint16_t v[8])
 {
     if (c0 + 8 <= V) {
-        uint4 q = *reinterpret_cast<const uint4*>(p + c0);
+        const uint4 q = gload4(p + c0);
         v[0] = (uint16_t)q.x; v[1] = (uint16_t)(q.x >> 16);
         v[2] = (uint16_t)q.y; v[3] = (uint16_t)(q.y >> 16);
         v[4] = (uint16_t)q.z; v[5] = (uint16_t)(q.z >> 16);
         v[6] = (uint16_t)q.w; v[7] = (uint16_t)(q.w >> 16);
     } else {
-        for (int k = 0; k < 8; ++k) v[k] = c0 + k < V ? p[c0 + k] : (uint16_t)0;
+        for (int k = 0; k < 8; ++k) v[k] = c0 + k < V ? gcast<const uint16_t>(p)[c0 + k] : (uint16_t)0;
     }
 }
 
@@ -40,30 +40,30 @@ __device__ inline void store8(uint16_t* p, uint64_t c0, uint64_t V, const uint16
         q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
         q.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
         q.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-        *reinterpret_cast<uint4*>(p + c0) = q;
+        gstore4(p + c0, q);
     } else {
         for (int k = 0; k < 8; ++k)
-            if (c0 + k < V) p[c0 + k] = v[k];
+            if (c0 + k < V) gcast<uint16_t>(p)[c0 + k] = v[k];
     }
 }
 
 // 8 occupancy bytes packed little-endian into a uint64 (0 beyond V)
 __device__ inline uint64_t load8b(const uint8_t* p, uint64_t c0, uint64_t V)
 {
-    if (c0 + 8 <= V) return *reinterpret_cast<const uint64_t*>(p + c0);
+    if (c0 + 8 <= V) return *gcast<const uint64_t>(p + c0);
     uint64_t r = 0;
     for (int k = 0; k < 8; ++k)
-        if (c0 + k < V) r |= (uint64_t)p[c0 + k] << (8 * k);
+        if (c0 + k < V) r |= (uint64_t)gcast<const uint8_t>(p)[c0 + k] << (8 * k);
     return r;
 }
 
 __device__ inline void store8b(uint8_t* p, uint64_t c0, uint64_t V, uint64_t r)
 {
     if (c0 + 8 <= V) {
-        *reinterpret_cast<uint64_t*>(p + c0) = r;
+        *gcast<uint64_t>(p + c0) = r;
     } else {
         for (int k = 0; k < 8; ++k)
-            if (c0 + k < V) p[c0 + k] = (uint8_t)(r >> (8 * k));
+            if (c0 + k < V) gcast<uint8_t>(p)[c0 + k] = (uint8_t)(r >> (8 * k));
     }
 }
 

@@ -96,6 +96,33 @@ inline uint32_t next_pow2(uint32_t v)
 }
 
 // ------------------------------------------------------------ device math ---
+// A generic pointer to device memory as a global-address-space pointer, so the
+// access compiles to global_* instead of flat_* instructions.  Pointers read
+// from a job descriptor staged in LDS are generic; flat accesses also count
+// against the LDS wait counter, which makes every LDS read wait for earlier
+// outstanding stores.
+#define VF_AS_GLOBAL __attribute__((address_space(1)))
+template <typename U, typename T>
+__device__ inline VF_AS_GLOBAL U* gcast(T* p)
+{
+    return (VF_AS_GLOBAL U*)p;
+}
+// 16- and 8-byte global loads / stores (HIP's vector structs cannot bind global references)
+typedef unsigned int vf_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int vf_u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint4 gload4(const void* p)
+{
+    const vf_u32x4 v = *(const VF_AS_GLOBAL vf_u32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void gstore4(void* p, uint4 q) { *(VF_AS_GLOBAL vf_u32x4*)p = vf_u32x4{ q.x, q.y, q.z, q.w }; }
+__device__ inline uint2 gload2(const void* p)
+{
+    const vf_u32x2 v = *(const VF_AS_GLOBAL vf_u32x2*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ inline void gstore2(void* p, uint2 q) { *(VF_AS_GLOBAL vf_u32x2*)p = vf_u32x2{ q.x, q.y }; }
+
 __host__ __device__ inline uint32_t fmix32(uint32_t h)
 {
     h ^= h >> 16;

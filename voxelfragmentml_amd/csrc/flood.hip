@@ -104,10 +104,10 @@ struct RelaxJob {
     const JobDev* J;
     uint32_t nx, ny, nz, nby, nbz, nbx, nb, b, amp, r;
     uint64_t noise_seed;
-    uint32_t* keys;
+    VF_AS_GLOBAL uint32_t* keys;
     uint32_t nw;                // flag words per parity
-    unsigned long long* cur;   // brick flag bits consumed this round
-    unsigned long long* nxt;   // brick flag bits for the next round
+    VF_AS_GLOBAL unsigned long long* cur;  // brick flag bits consumed this round
+    VF_AS_GLOBAL unsigned long long* nxt;  // brick flag bits for the next round
     bool n26;
 };
 
@@ -131,7 +131,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const int X0 = (int)(bx * kRelaxBrick), Y0 = (int)(by * kRelaxBrick), Z0 = (int)(bz * kRelaxBrick);
     // halo origin (X0-1, Y0-1, Z0-1): cell offsets below are relative to it and fit 32 bits
     const int64_t org = ((int64_t)(X0 - 1) * ny + (Y0 - 1)) * nz + (Z0 - 1);
-    uint32_t* kb = R.keys + org;  // dereferenced only at in-grid cells
+    VF_AS_GLOBAL uint32_t* kb = R.keys + org;  // dereferenced only at in-grid cells
 
     // brick + halo -> LDS as key - 1 (cells outside the grid are EMPTY).  Each
     // lane owns one or two (y, z) positions of the 10 x 10 halo plane and walks
@@ -286,7 +286,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         if (qx >= 0 && qy >= 0 && qz >= 0 && (uint32_t)qx < R.nbx && (uint32_t)qy < R.nby && (uint32_t)qz < R.nbz)
         {
             const uint32_t id = ((uint32_t)qx * R.nby + (uint32_t)qy) * R.nbz + (uint32_t)qz;
-            atomicOr(R.nxt + (id >> 6), 1ull << (id & 63u));
+            __hip_atomic_fetch_or(R.nxt + (id >> 6), 1ull << (id & 63u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     activated |= dirs != 0;
@@ -306,12 +306,12 @@ __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
     R.b = uni(J->key_bits);
     R.amp = uni(J->noise_amp);
     R.noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
-    R.keys = uni_ptr(J->keys);
+    R.keys = gcast<uint32_t>(uni_ptr(J->keys));
     R.r = uni(J->fctl[0]);
     R.nw = relax_words_per_parity(R.nb);
     unsigned long long* flags = uni_ptr(J->bflags);
-    R.cur = flags + (size_t)R.nw * (R.r & 1u);
-    R.nxt = flags + (size_t)R.nw * ((R.r + 1u) & 1u);
+    R.cur = gcast<unsigned long long>(flags + (size_t)R.nw * (R.r & 1u));
+    R.nxt = gcast<unsigned long long>(flags + (size_t)R.nw * ((R.r + 1u) & 1u));
     R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
     return true;
 }
@@ -447,18 +447,21 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
         const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
         if (c0 >= J.V) return;
         const uint64_t o = load8b(J.occ, c0, J.V);
-        uint32_t key[8];
+        uint32_t key[8] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
+        if (o) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            key[k] = ((o >> (8 * k)) & 0xFFu) ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
-                                              : kEmptyKey;
+            for (int k = 0; k < 8; ++k) {
+                key[k] = ((o >> (8 * k)) & 0xFFu)
+                             ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
+                             : kEmptyKey;
+            }
         }
         if (c0 + 8 <= J.V) {
-            reinterpret_cast<uint4*>(J.keys + c0)[0] = make_uint4(key[0], key[1], key[2], key[3]);
-            reinterpret_cast<uint4*>(J.keys + c0)[1] = make_uint4(key[4], key[5], key[6], key[7]);
+            gstore4(J.keys + c0, make_uint4(key[0], key[1], key[2], key[3]));
+            gstore4(J.keys + c0 + 4, make_uint4(key[4], key[5], key[6], key[7]));
         } else {
             for (int k = 0; k < 8; ++k)
-                if (c0 + k < J.V) J.keys[c0 + k] = key[k];
+                if (c0 + k < J.V) gcast<uint32_t>(J.keys)[c0 + k] = key[k];
         }
     });
 }
@@ -481,12 +484,12 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
         const uint32_t mask = (1u << J.key_bits) - 1u;
         uint32_t key[8];
         if (c0 + 8 <= J.V) {
-            const uint4 a = reinterpret_cast<const uint4*>(J.keys + c0)[0];
-            const uint4 b = reinterpret_cast<const uint4*>(J.keys + c0)[1];
+            const uint4 a = gload4(J.keys + c0);
+            const uint4 b = gload4(J.keys + c0 + 4);
             key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
             key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
         } else {
-            for (int k = 0; k < 8; ++k) key[k] = c0 + k < J.V ? J.keys[c0 + k] : kEmptyKey;
+            for (int k = 0; k < 8; ++k) key[k] = c0 + k < J.V ? gcast<const uint32_t>(J.keys)[c0 + k] : kEmptyKey;
         }
         uint16_t out[8];
         for (int k = 0; k < 8; ++k) {

@@ -287,12 +287,12 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             const uint16_t* r = lab + ((size_t)cx * ny + cy) * nz;
             // zs is a multiple of 8, so with nz a multiple of 4 the 8 cells start 8-byte aligned
             if (zs + 8 <= nz && (nz & 3) == 0) {
-                const uint2 a = *reinterpret_cast<const uint2*>(r + zs);
-                const uint2 q = *reinterpret_cast<const uint2*>(r + zs + 4);
+                const uint2 a = gload2(r + zs);
+                const uint2 q = gload2(r + zs + 4);
                 w[1] = (uint16_t)a.x; w[2] = (uint16_t)(a.x >> 16); w[3] = (uint16_t)a.y; w[4] = (uint16_t)(a.y >> 16);
                 w[5] = (uint16_t)q.x; w[6] = (uint16_t)(q.x >> 16); w[7] = (uint16_t)q.y; w[8] = (uint16_t)(q.y >> 16);
-                w[0] = r[max(zs - 1, 0)];        // clamped window: z = -1 reads z = 0
-                w[9] = r[min(zs + 8, nz - 1)];
+                w[0] = gcast<const uint16_t>(r)[max(zs - 1, 0)];  // clamped window: z = -1 reads z = 0
+                w[9] = gcast<const uint16_t>(r)[min(zs + 8, nz - 1)];
             } else {
 #pragma unroll
                 for (int k = 0; k < 10; ++k) w[k] = r[min(max(zs - 1 + k, 0), nz - 1)];
@@ -312,7 +312,8 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         const uint32_t c0 = (uint32_t)(((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz);
         auto add_checksum = [&]() {
             if (CSUM)
-                for (int k = 0; k < nval; ++k) csum += checksum_term(v[k], c0 + k);
+                if (v[0] | v[1] | v[2] | v[3] | v[4] | v[5] | v[6] | v[7])  // EMPTY cells add nothing
+                    for (int k = 0; k < nval; ++k) csum += checksum_term(v[k], c0 + k);
         };
         if (!__syncthreads_or(mine)) {
             add_checksum();
@@ -378,10 +379,8 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
         if (changed) {
             uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
             if (nval == 8 && (nz & 3) == 0) {
-                reinterpret_cast<uint2*>(dst)[0] =
-                    make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
-                reinterpret_cast<uint2*>(dst)[1] =
-                    make_uint2((uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16));
+                gstore2(dst, make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16)));
+                gstore2(dst + 4, make_uint2((uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16)));
             } else {
                 for (int k = 0; k < nval; ++k) dst[k] = v[k];
             }

@@ -162,8 +162,8 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
         if (c0 < J->V) {
             if (J->seed_location == VF_SEED_BOTH) {
                 if (c0 + 32u <= J->V) {
-                    const uint4 a = *reinterpret_cast<const uint4*>(J->occ + c0);
-                    const uint4 b = *reinterpret_cast<const uint4*>(J->occ + c0 + 16u);
+                    const uint4 a = gload4(J->occ + c0);
+                    const uint4 b = gload4(J->occ + c0 + 16u);
                     const uint64_t m = 0x0101010101010101ull;
                     cnt = __popcll((((uint64_t)a.y << 32) | a.x) & m) + __popcll((((uint64_t)a.w << 32) | a.z) & m) +
                           __popcll((((uint64_t)b.y << 32) | b.x) & m) + __popcll((((uint64_t)b.w << 32) | b.z) & m);

@@ -397,8 +397,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
         const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
         const uint64_t w = (uint64_t)group * kVoxThreads + threadIdx.x;
         const bool valid = w < nwords;
-        const uint32_t tg = valid ? J.toggle_bits[w] : 0u;
-        const uint32_t sh = valid ? J.shell_bits[w] : 0u;
+        const uint32_t tg = valid ? gcast<const uint32_t>(J.toggle_bits)[w] : 0u;
+        const uint32_t sh = valid ? gcast<const uint32_t>(J.shell_bits)[w] : 0u;
         uint32_t p = tg;
         p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8; p ^= p << 16;
         const uint64_t par = __ballot(__popc(tg) & 1);
@@ -421,13 +421,13 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
         if (nb) {
             uint8_t* o = J.occ + base;
             if ((J.nz & 15u) == 0 && nb == 32) {
-                uint4 a, b;
-                a.x = spread4(bits); a.y = spread4(bits >> 4); a.z = spread4(bits >> 8); a.w = spread4(bits >> 12);
-                b.x = spread4(bits >> 16); b.y = spread4(bits >> 20); b.z = spread4(bits >> 24); b.w = spread4(bits >> 28);
-                reinterpret_cast<uint4*>(o)[0] = a;
-                reinterpret_cast<uint4*>(o)[1] = b;
+                gstore4(o, make_uint4(spread4(bits), spread4(bits >> 4), spread4(bits >> 8), spread4(bits >> 12)));
+                gstore4(o + 16, make_uint4(spread4(bits >> 16), spread4(bits >> 20), spread4(bits >> 24), spread4(bits >> 28)));
+            } else if ((J.nz & 7u) == 0 && nb == 32) {
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) gstore2(o + 8 * q, make_uint2(spread4(bits >> (8 * q)), spread4(bits >> (8 * q + 4))));
             } else if ((J.nz & 3u) == 0) {
-                for (uint32_t q = 0; q < nb / 4; ++q) reinterpret_cast<uint32_t*>(o)[q] = spread4(bits >> (4 * q));
+                for (uint32_t q = 0; q < nb / 4; ++q) gcast<uint32_t>(o)[q] = spread4(bits >> (4 * q));
             } else {
                 for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
             }
@@ -455,8 +455,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint3
         const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
         const uint64_t w = (uint64_t)group * kVoxThreads + threadIdx.x;
         if (w < nwords) {
-            J.toggle_bits[w] = 0u;
-            J.shell_bits[w] = 0u;
+            gcast<uint32_t>(J.toggle_bits)[w] = 0u;
+            gcast<uint32_t>(J.shell_bits)[w] = 0u;
         }
         if (w == 0) J.stats[kStatOccupied] = 0;
     });
