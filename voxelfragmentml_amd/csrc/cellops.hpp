@@ -124,6 +124,33 @@ __device__ inline void walk_tiles(const Launch& L, const uint32_t* prefix, uint3
     walk_tiles(L, prefix, total, sJ, f, [](uint32_t) {});
 }
 
+// The same walk in runs of up to N consecutive tiles of one job:
+// f(J, j, first_local_tile, n) handles tiles first .. first + n - 1 (n <= N),
+// so a streaming stage can issue all its loads before using any of them.
+template <uint32_t N, class F>
+__device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f)
+{
+    uint32_t tb, te;
+    block_tile_range(total, tb, te);
+    if (tb >= te) return;
+    uint32_t j = find_job(prefix, L.njobs, tb);
+    uint32_t jb = prefix[j], je = prefix[j + 1];
+    stage_job(sJ, L.jobs + j);
+    for (uint32_t t = tb; t < te;) {
+        if (t >= je) {
+            do {
+                ++j;
+                jb = je;
+                je = prefix[j + 1];
+            } while (t >= je);
+            stage_job(sJ, L.jobs + j);
+        }
+        const uint32_t n = min(N, min(te, je) - t);
+        f(*sJ, j, t - jb, n);
+        t += n;
+    }
+}
+
 // Grid-stride variant for compute-heavy stages whose tiles differ in cost:
 // tiles are dealt round-robin over the blocks (better balance); the tile
 // prefix (njobs <= 1024) is kept in LDS for the per-tile job search.

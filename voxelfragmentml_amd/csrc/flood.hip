@@ -441,37 +441,48 @@ __global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
 {
+    constexpr uint32_t kRun = 4;  // tiles per step: all occupancy loads are issued before any key is written
     __shared__ JobDev sJ;
-    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+    walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t n) {
         if (!J.active || J.algo != VF_FLOOD) return;
-        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J.V) return;
-        const uint64_t o = load8b(J.occ, c0, J.V);
-        uint32_t key[8] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
-        if (o) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
+        const uint64_t V = J.V;
+        uint64_t o[kRun];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                key[k] = ((o >> (8 * k)) & 0xFFu)
-                             ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
-                             : kEmptyKey;
-            }
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            o[r] = r < n && c0 < V ? load8b(J.occ, c0, V) : 0ull;
         }
-        if (c0 + 8 <= J.V) {
-            gstore4(J.keys + c0, make_uint4(key[0], key[1], key[2], key[3]));
-            gstore4(J.keys + c0 + 4, make_uint4(key[4], key[5], key[6], key[7]));
-        } else {
-            for (int k = 0; k < 8; ++k)
-                if (c0 + k < J.V) gcast<uint32_t>(J.keys)[c0 + k] = key[k];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            if (r >= n || c0 >= V) break;
+            uint32_t key[8] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
+            if (o[r]) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    key[k] = ((o[r] >> (8 * k)) & 0xFFu)
+                                 ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
+                                 : kEmptyKey;
+                }
+            }
+            if (c0 + 8 <= V) {
+                gstore4(J.keys + c0, make_uint4(key[0], key[1], key[2], key[3]));
+                gstore4(J.keys + c0 + 4, make_uint4(key[4], key[5], key[6], key[7]));
+            } else {
+                for (int k = 0; k < 8; ++k)
+                    if (c0 + k < V) gcast<uint32_t>(J.keys)[c0 + k] = key[k];
+            }
         }
     });
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
 {
+    constexpr uint32_t kRun = 2;  // tiles per step: all key loads are issued before any label is written
     __shared__ JobDev sJ;
     __shared__ uint16_t lab[kMaxDeviceSeeds];  // seed index -> 8-bit fragment label of the staged job
     uint32_t staged = 0xFFFFFFFFu;
-    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+    walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t n) {
         if (!J.active || J.algo != VF_FLOOD) return;
         if (j != staged) {  // block-uniform
             for (uint32_t i = threadIdx.x; i < J.S && i < kMaxDeviceSeeds; i += blockDim.x)
@@ -479,25 +490,34 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
             __syncthreads();
             staged = j;
         }
-        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
-        if (c0 >= J.V) return;
+        const uint64_t V = J.V;
         const uint32_t mask = (1u << J.key_bits) - 1u;
-        uint32_t key[8];
-        if (c0 + 8 <= J.V) {
-            const uint4 a = gload4(J.keys + c0);
-            const uint4 b = gload4(J.keys + c0 + 4);
-            key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
-            key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
-        } else {
-            for (int k = 0; k < 8; ++k) key[k] = c0 + k < J.V ? gcast<const uint32_t>(J.keys)[c0 + k] : kEmptyKey;
+        uint32_t key[kRun][8];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            if (r < n && c0 + 8 <= V) {
+                const uint4 a = gload4(J.keys + c0);
+                const uint4 b = gload4(J.keys + c0 + 4);
+                key[r][0] = a.x; key[r][1] = a.y; key[r][2] = a.z; key[r][3] = a.w;
+                key[r][4] = b.x; key[r][5] = b.y; key[r][6] = b.z; key[r][7] = b.w;
+            } else {
+                for (int k = 0; k < 8; ++k)
+                    key[r][k] = r < n && c0 + k < V ? gcast<const uint32_t>(J.keys)[c0 + k] : kEmptyKey;
+            }
         }
-        uint16_t out[8];
-        for (int k = 0; k < 8; ++k) {
-            if (key[k] == kEmptyKey) out[k] = kEmpty;
-            else if (key[k] >= kUnreachedKey) out[k] = kFree;
-            else out[k] = lab[(key[k] >> 2) & mask];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            if (r >= n || c0 >= V) break;
+            uint16_t out[8];
+            for (int k = 0; k < 8; ++k) {
+                if (key[r][k] == kEmptyKey) out[k] = kEmpty;
+                else if (key[r][k] >= kUnreachedKey) out[k] = kFree;
+                else out[k] = lab[(key[r][k] >> 2) & mask];
+            }
+            store8(J.labels, c0, V, out);
         }
-        store8(J.labels, c0, J.V, out);
     });
 }
 

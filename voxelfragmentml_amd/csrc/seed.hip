@@ -38,32 +38,43 @@ __device__ inline bool candidate(const JobDev* J, uint64_t c, uint32_t x, uint32
 
 __global__ __launch_bounds__(kCellThreads) void k_seed_count(Launch L, uint32_t* tile_cnt)
 {
-    __shared__ uint32_t wsum[kCellThreads / 64];
+    constexpr uint32_t kRun = 4;  // tiles per step: loads first, one block reduction for all of them
+    __shared__ uint32_t wsum[kRun][kCellThreads / 64];
     __shared__ JobDev sJ;
-    walk_tiles(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t t) {
-        uint32_t n = 0;
-        const uint64_t c0 = (uint64_t)local * kTileCells + threadIdx.x * kCellsPerThread;
-        if (J.active && c0 < J.V) {
-            const uint64_t o = load8b(J.occ, c0, J.V);
+    walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t nrun) {
+        uint32_t n[kRun];
+        uint64_t o[kRun];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            o[r] = J.active && r < nrun && c0 < J.V ? load8b(J.occ, c0, J.V) : 0ull;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            n[r] = 0;
             if (J.seed_location == VF_SEED_BOTH) {
-                n = __popcll(o & 0x0101010101010101ull);
-            } else if (o) {
+                n[r] = __popcll(o[r] & 0x0101010101010101ull);
+            } else if (o[r]) {
                 const JobDev* Jg = L.jobs + j;
+                const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
                 uint32_t x, y, z;
                 decompose(c0, J, x, y, z);
                 for (int k = 0; k < 8 && c0 + k < J.V; ++k) {
-                    if ((o >> (8 * k)) & 0xFFu) n += candidate(Jg, c0 + k, x, y, z) ? 1u : 0u;
+                    if ((o[r] >> (8 * k)) & 0xFFu) n[r] += candidate(Jg, c0 + k, x, y, z) ? 1u : 0u;
                     advance(x, y, z, J);
                 }
             }
+            for (int ofs = 32; ofs > 0; ofs >>= 1) n[r] += __shfl_xor(n[r], ofs, 64);
         }
-        for (int ofs = 32; ofs > 0; ofs >>= 1) n += __shfl_xor(n, ofs, 64);
-        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (uint32_t r = 0; r < kRun; ++r) wsum[r][threadIdx.x >> 6] = n[r];
+        }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < nrun) {
             uint32_t tot = 0;
-            for (uint32_t w = 0; w < kCellThreads / 64; ++w) tot += wsum[w];
-            tile_cnt[t] = tot;
+            for (uint32_t w = 0; w < kCellThreads / 64; ++w) tot += wsum[threadIdx.x][w];
+            tile_cnt[L.tile_prefix[j] + local + threadIdx.x] = tot;
         }
         __syncthreads();
     });
