@@ -129,37 +129,39 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t nx = R.nx, ny = R.ny, nz = R.nz;
     const uint32_t bz = brick % R.nbz, t1 = brick / R.nbz, by = t1 % R.nby, bx = t1 / R.nby;
     const int X0 = (int)(bx * kRelaxBrick), Y0 = (int)(by * kRelaxBrick), Z0 = (int)(bz * kRelaxBrick);
-    // halo origin (X0-1, Y0-1, Z0-1): cell offsets below are relative to it and fit 32 bits
-    const int64_t org = ((int64_t)(X0 - 1) * ny + (Y0 - 1)) * nz + (Z0 - 1);
-    VF_AS_GLOBAL uint32_t* kb = R.keys + org;  // dereferenced only at in-grid cells
 
-    // brick + halo -> LDS as key - 1 (cells outside the grid are EMPTY).  Each
-    // lane owns one or two (y, z) positions of the 10 x 10 halo plane and walks
-    // the 10 x slices, so the index work is done once per visit.
+    // brick + halo -> LDS (cells outside the grid and EMPTY cells as kTileEmpty).  Each lane owns
+    // one or two (y, z) positions of the 10 x 10 halo plane and loads them in all 10 x slices at
+    // once; addresses are clamped into the grid so every load is unconditional, and cells outside
+    // the grid are masked afterwards.
     {
-        const uint32_t p1 = lane, p2 = lane + 64u;  // p2 valid for lanes < 36
+        // lanes 36..63 have no second position and repeat their first (same cell, same word)
+        const uint32_t p1 = lane, p2 = lane < 36u ? lane + 64u : lane;
         const uint32_t hy1 = p1 / 10u, hz1 = p1 - hy1 * 10u, hy2 = p2 / 10u, hz2 = p2 - hy2 * 10u;
-        const bool in1 = (uint32_t)Y0 + hy1 - 1u < ny && (uint32_t)Z0 + hz1 - 1u < nz;
-        const bool in2 = p2 < 100u && (uint32_t)Y0 + hy2 - 1u < ny && (uint32_t)Z0 + hz2 - 1u < nz;
-        const uint32_t o1 = hy1 * nz + hz1, o2 = hy2 * nz + hz2;              // cell offsets within an x slice
+        const int gy1 = Y0 + (int)hy1 - 1, gz1 = Z0 + (int)hz1 - 1, gy2 = Y0 + (int)hy2 - 1, gz2 = Z0 + (int)hz2 - 1;
+        const bool in1 = (uint32_t)gy1 < ny && (uint32_t)gz1 < nz;
+        const bool in2 = (uint32_t)gy2 < ny && (uint32_t)gz2 < nz;
+        const uint32_t o1 = (uint32_t)min(max(gy1, 0), (int)ny - 1) * nz + (uint32_t)min(max(gz1, 0), (int)nz - 1);
+        const uint32_t o2 = (uint32_t)min(max(gy2, 0), (int)ny - 1) * nz + (uint32_t)min(max(gz2, 0), (int)nz - 1);
         const uint32_t t1 = hy1 * kRowWords + hz1, t2 = hy2 * kRowWords + hz2;  // tile offsets within an x slice
         const uint32_t slice = ny * nz;
-#pragma unroll 1
-        for (uint32_t h0 = 0; h0 < 10u; h0 += 5u) {
-            uint32_t v1[5], v2[5];
+        // byte offsets from the keys base fit 32 bits (grids below 2^30 cells): a uniform base plus a
+        // 32-bit lane offset per load
+        const VF_AS_GLOBAL uint8_t* kbytes = (const VF_AS_GLOBAL uint8_t*)R.keys;
+        uint32_t v1[10], v2[10];
 #pragma unroll
-            for (uint32_t k = 0; k < 5u; ++k) {
-                const uint32_t hx = h0 + k;
-                const bool xin = (uint32_t)X0 + hx - 1u < nx;
-                v1[k] = xin && in1 ? kb[hx * slice + o1] : kEmptyKey;
-                v2[k] = xin && in2 ? kb[hx * slice + o2] : kEmptyKey;
-            }
+        for (uint32_t hx = 0; hx < 10u; ++hx) {
+            const uint32_t xs = (uint32_t)min(max(X0 + (int)hx - 1, 0), (int)nx - 1) * slice;
+            v1[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o1) * 4u);
+            v2[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o2) * 4u);
+        }
+        // outside the grid -> all ones; EMPTY (0) -> kTileEmpty
+        const uint32_t bad1 = in1 ? 0u : kTileEmpty, bad2 = in2 ? 0u : kTileEmpty;
 #pragma unroll
-            for (uint32_t k = 0; k < 5u; ++k) {
-                const uint32_t hx = h0 + k;
-                tile[hx * 10u * kRowWords + t1] = v1[k] ? v1[k] : kTileEmpty;
-                if (p2 < 100u) tile[hx * 10u * kRowWords + t2] = v2[k] ? v2[k] : kTileEmpty;
-            }
+        for (uint32_t hx = 0; hx < 10u; ++hx) {
+            const uint32_t xbad = (uint32_t)(X0 + (int)hx - 1) < nx ? 0u : kTileEmpty;
+            tile[hx * 10u * kRowWords + t1] = (v1[hx] ? v1[hx] : kTileEmpty) | bad1 | xbad;
+            tile[hx * 10u * kRowWords + t2] = (v2[hx] ? v2[hx] : kTileEmpty) | bad2 | xbad;
         }
     }
     __syncthreads();
@@ -167,7 +169,6 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     // the lane's own z row of 8 cells
     const uint32_t x = lane >> 3, y = lane & 7u;
     const uint32_t own = ((x + 1u) * 10u + (y + 1u)) * kRowWords;
-    const uint32_t orow = ((x + 1u) * ny + (y + 1u)) * nz + 1u;  // offset of cell (x, y, 0) from the halo origin
     const uint32_t wsh = R.b + 2u;
     // k: the row's keys (EMPTY as kTileEmpty).  add: what a neighbour's key (weight bits cleared)
     // gains on entering the cell, (w << (b + 2)) + w, or all ones for EMPTY cells, so that
@@ -260,10 +261,23 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         if (k[z] < kUnreachedKey && (k[z] & ~3u) >= headroom) error = true;
     }
 
-    // write back the cells that dropped
+    // write back the rows with a dropped cell: the whole row in two 16-byte stores where it is
+    // aligned and complete (its other cells rewrite the values they hold: a brick is visited at
+    // most once per round, so nothing else writes them meanwhile), else cell by cell
+    if (changed) {
+        const uint32_t row = (((uint32_t)X0 + x) * ny + ((uint32_t)Y0 + y)) * nz + (uint32_t)Z0;
+        if ((uint32_t)Z0 + 8u <= nz && (nz & 3u) == 0) {
+            uint32_t out[8];
 #pragma unroll
-    for (int z = 0; z < 8; ++z)
-        if (changed & (1u << z)) kb[orow + z] = k[z];
+            for (int z = 0; z < 8; ++z) out[z] = k[z] == kTileEmpty ? kEmptyKey : k[z];
+            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row) = vf_u32x4{ out[0], out[1], out[2], out[3] };
+            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row + 4) = vf_u32x4{ out[4], out[5], out[6], out[7] };
+        } else {
+#pragma unroll
+            for (int z = 0; z < 8; ++z)
+                if (changed & (1u << z)) R.keys[row + z] = k[z];
+        }
+    }
 
     // neighbour bricks that see a dropped cell: bit (dx+1)*9 + (dy+1)*3 + (dz+1)
     uint32_t dirs = 0;
