@@ -45,6 +45,14 @@ constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
 // of a wave's 64 lanes fall on distinct LDS banks.
 constexpr uint32_t kRowWords = 11;
 constexpr uint32_t kTileWords = 10 * 10 * kRowWords;
+// 26-neighbour minima, separably: zt = z-window minima of the 100 halo-plane rows (8 per row),
+// yt = y-window minima of the 10 x-slabs' 8 rows; rows at an odd stride of 9 words.  yt reuses
+// the tile's words (the tile is read only while a visit starts).
+constexpr uint32_t kSepStride = 9;
+constexpr uint32_t kZtWords = 100 * kSepStride;
+constexpr uint32_t kYtWords = 80 * kSepStride;
+static_assert(kYtWords <= kTileWords, "yt reuses the tile");
+constexpr uint32_t kRelaxLdsWords = kTileWords + kZtWords;
 constexpr uint32_t kMaxSweeps = 2048;              // a brick of 512 cells is stable after at most 513 sweeps
 constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgroups are dealt round-robin
 
@@ -186,25 +194,56 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     auto relax = [&](uint32_t cur, uint32_t nb, uint32_t a) {
         return min(cur, __builtin_elementwise_add_sat(nb & ~3u, a));
     };
+    uint32_t* zt = tile + kTileWords;
+    uint32_t* yt = tile;
+    uint32_t hzlo = 0, hzhi = 0;  // the own row's halo keys at z = -1 and z = 8
+    if (R.n26) {
+        // own row's z halo, then the minima that stay fixed during the visit: zt of the 36 halo
+        // rows (x-slab 0 or 9, or y-row 0 or 9), then yt of the halo x-slabs 0 and 9
+        hzlo = tile[own];
+        hzhi = tile[own + 9];
+        if (lane < 36u) {
+            const uint32_t hx = lane < 10u ? 0u : (lane < 20u ? 9u : (lane < 28u ? lane - 19u : lane - 27u));
+            const uint32_t hy = lane < 10u ? lane : (lane < 20u ? lane - 10u : (lane < 28u ? 0u : 9u));
+            const uint32_t* row = tile + (hx * 10u + hy) * kRowWords;
+            uint32_t r[10];
+#pragma unroll
+            for (int o = 0; o < 10; ++o) r[o] = row[o];
+#pragma unroll
+            for (int z = 0; z < 8; ++z) zt[(hx * 10u + hy) * kSepStride + z] = min(min(r[z], r[z + 1]), r[z + 2]);
+        }
+        __syncthreads();  // the tile is read for the last time above; yt may now overwrite it
+        if (lane < 16u) {
+            const uint32_t sx = lane < 8u ? 0u : 9u, sy = lane & 7u;
+#pragma unroll
+            for (int z = 0; z < 8; ++z)
+                yt[(sx * 8u + sy) * kSepStride + z] = min(min(zt[(sx * 10u + sy) * kSepStride + z],
+                                                              zt[(sx * 10u + sy + 1u) * kSepStride + z]),
+                                                          zt[(sx * 10u + sy + 2u) * kSepStride + z]);
+        }
+    }
     uint32_t sweeps = 0;
     for (;;) {
         uint32_t m[8];
         if (R.n26) {
-            uint32_t cm[10];
+            // z pass in registers, then y and x passes through LDS; every window includes the cell
+            uint32_t zo[8], ym[8];
+            zo[0] = min(min(hzlo, k[0]), k[1]);
 #pragma unroll
-            for (int o = 0; o < 10; ++o) cm[o] = 0xFFFFFFFFu;
+            for (int z = 1; z < 7; ++z) zo[z] = min(min(k[z - 1], k[z]), k[z + 1]);
+            zo[7] = min(min(k[6], k[7]), hzhi);
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx)
+            for (int z = 0; z < 8; ++z) zt[((x + 1u) * 10u + y + 1u) * kSepStride + z] = zo[z];
+            __syncthreads();
 #pragma unroll
-                for (int dy = 0; dy < 3; ++dy) {
-                    const uint32_t* row = tile + ((x + dx) * 10u + (y + dy)) * kRowWords;
+            for (int z = 0; z < 8; ++z)
+                ym[z] = min(min(zt[((x + 1u) * 10u + y) * kSepStride + z], zo[z]), zt[((x + 1u) * 10u + y + 2u) * kSepStride + z]);
 #pragma unroll
-                    for (int o = 0; o < 10; ++o) cm[o] = min(cm[o], row[o]);
-                    __builtin_amdgcn_sched_barrier(0);  // keep one row of LDS values live at a time
-                }
-            // cm[o] covers halo z = o - 1; cell z sees cm[z], cm[z+1], cm[z+2]
+            for (int z = 0; z < 8; ++z) yt[((x + 1u) * 8u + y) * kSepStride + z] = ym[z];
+            __syncthreads();
 #pragma unroll
-            for (int z = 0; z < 8; ++z) m[z] = min(min(cm[z], cm[z + 1]), cm[z + 2]);
+            for (int z = 0; z < 8; ++z)
+                m[z] = min(min(yt[(x * 8u + y) * kSepStride + z], ym[z]), yt[((x + 2u) * 8u + y) * kSepStride + z]);
         } else {
             uint32_t cc[10], fm[8];
             {
@@ -239,7 +278,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         uint32_t diff = 0;
 #pragma unroll
         for (int z = 0; z < 8; ++z) diff |= k[z] ^ prev[z];
-        if (diff) {
+        if (diff && !R.n26) {  // the 26-neighbour passes read zt / yt, rebuilt every sweep
 #pragma unroll
             for (int z = 0; z < 8; ++z) tile[own + 1 + z] = k[z];
         }
@@ -394,10 +433,10 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
 }
 
 // Round step 2: relax the listed bricks; the workgroups of XCD x serve list x.
-__global__ __launch_bounds__(64) void k_flood_relax(Launch L, const unsigned long long* lists, uint32_t list_cap,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_flood_relax(Launch L, const unsigned long long* lists, uint32_t list_cap,
                                                     const uint32_t* counts)
 {
-    __shared__ uint32_t tile[kTileWords];
+    __shared__ uint32_t tile[kRelaxLdsWords];  // tile (yt reuses it), then zt
     const uint32_t lane = threadIdx.x;
     const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, local = blockIdx.x / kXcds;
     if (local >= per_xcd) return;
