@@ -178,30 +178,35 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t x = lane >> 3, y = lane & 7u;
     const uint32_t own = ((x + 1u) * 10u + (y + 1u)) * kRowWords;
     const uint32_t wsh = R.b + 2u;
-    // k: the row's keys (EMPTY as kTileEmpty).  add: what a neighbour's key (weight bits cleared)
-    // gains on entering the cell, (w << (b + 2)) + w, or all ones for EMPTY cells, so that
-    //     relaxed = min(k, sat_add(m & ~3, add))
-    // leaves EMPTY cells alone and never lets an EMPTY (~0) or unreached (>= kUnreachedKey) neighbour
-    // win: their sums either saturate or stay above every key they could replace.
-    uint32_t k[8], k0[8], add[8];
+    // During the sweeps keys are held with their weight bits cleared (EMPTY as 0xFFFFFFFC, unreached
+    // as kUnreachedKey).  k: the row's keys; wmask: 2 bits per cell, its weight (0 = EMPTY);
+    // add: what a neighbour's key gains on entering the cell, w << (b + 2), or all ones for EMPTY
+    // cells, so that
+    //     relaxed = min(k, sat_add(neighbour, add))
+    // leaves EMPTY cells alone and never lets an EMPTY or unreached neighbour win: their sums either
+    // saturate or stay at or above every key they could replace.
+    uint32_t k[8], add[8];
+    uint32_t wmask = 0;
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        k[z] = tile[own + 1 + z];
-        k0[z] = k[z];
-        const uint32_t w = k[z] & 3u;
-        add[z] = k[z] != kTileEmpty ? (w << wsh) + w : 0xFFFFFFFFu;
+        const uint32_t t = tile[own + 1 + z];
+        const uint32_t w = t != kTileEmpty ? (t & 3u) : 0u;
+        wmask |= w << (2 * z);
+        k[z] = t & ~3u;
+        add[z] = w ? (w << wsh) : 0xFFFFFFFFu;
     }
-    auto relax = [&](uint32_t cur, uint32_t nb, uint32_t a) {
-        return min(cur, __builtin_elementwise_add_sat(nb & ~3u, a));
-    };
+    // what the neighbour flags and the write-back need: whether any cell dropped, and the end cells
+    const uint32_t k0_first = k[0], k0_last = k[7];
+    uint32_t anydiff = 0;
+    auto relax = [&](uint32_t cur, uint32_t nb, uint32_t a) { return min(cur, __builtin_elementwise_add_sat(nb, a)); };
     uint32_t* zt = tile + kTileWords;
     uint32_t* yt = tile;
     uint32_t hzlo = 0, hzhi = 0;  // the own row's halo keys at z = -1 and z = 8
     if (R.n26) {
         // own row's z halo, then the minima that stay fixed during the visit: zt of the 36 halo
         // rows (x-slab 0 or 9, or y-row 0 or 9), then yt of the halo x-slabs 0 and 9
-        hzlo = tile[own];
-        hzhi = tile[own + 9];
+        hzlo = tile[own] & ~3u;
+        hzhi = tile[own + 9] & ~3u;
         if (lane < 36u) {
             const uint32_t hx = lane < 10u ? 0u : (lane < 20u ? 9u : (lane < 28u ? lane - 19u : lane - 27u));
             const uint32_t hy = lane < 10u ? lane : (lane < 20u ? lane - 10u : (lane < 28u ? 0u : 9u));
@@ -210,7 +215,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
 #pragma unroll
             for (int o = 0; o < 10; ++o) r[o] = row[o];
 #pragma unroll
-            for (int z = 0; z < 8; ++z) zt[(hx * 10u + hy) * kSepStride + z] = min(min(r[z], r[z + 1]), r[z + 2]);
+            for (int z = 0; z < 8; ++z) zt[(hx * 10u + hy) * kSepStride + z] = min(min(r[z], r[z + 1]), r[z + 2]) & ~3u;
         }
         __syncthreads();  // the tile is read for the last time above; yt may now overwrite it
         if (lane < 16u) {
@@ -262,7 +267,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
                 for (int o = 0; o < 8; ++o) fm[o] = min(fm[o], row[o]);
             }
 #pragma unroll
-            for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]);
+            for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]) & ~3u;
         }
         uint32_t prev[8];
 #pragma unroll
@@ -278,6 +283,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         uint32_t diff = 0;
 #pragma unroll
         for (int z = 0; z < 8; ++z) diff |= k[z] ^ prev[z];
+        anydiff |= diff;
         if (diff && !R.n26) {  // the 26-neighbour passes read zt / yt, rebuilt every sweep
 #pragma unroll
             for (int z = 0; z < 8; ++z) tile[own + 1 + z] = k[z];
@@ -292,29 +298,28 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     }
     // Key overflow: a reached key within one step (at most weight 3) of kUnreachedKey could have a
     // successor that does not fit; flag it (conservatively) where the key was made.
-    uint32_t changed = 0;  // bit z: cell z dropped in this visit
-    const uint32_t headroom = kUnreachedKey - ((3u << wsh) + 3u);
+    const bool changed = anydiff != 0;  // a cell of the row dropped in this visit
+    const uint32_t zlo = k[0] != k0_first ? 1u : 0u, zhi = k[7] != k0_last ? 1u : 0u;  // which end cells
+    const uint32_t headroom = kUnreachedKey - (3u << wsh);
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        changed |= (k[z] != k0[z] ? 1u : 0u) << z;
-        if (k[z] < kUnreachedKey && (k[z] & ~3u) >= headroom) error = true;
+        if (k[z] < kUnreachedKey && k[z] >= headroom) error = true;
+        const uint32_t w = (wmask >> (2 * z)) & 3u;
+        k[z] = w ? (k[z] | w) : kEmptyKey;  // the stored form: weight bits back, EMPTY as 0
     }
 
-    // write back the rows with a dropped cell: the whole row in two 16-byte stores where it is
-    // aligned and complete (its other cells rewrite the values they hold: a brick is visited at
-    // most once per round, so nothing else writes them meanwhile), else cell by cell
+    // write back the rows with a dropped cell, whole: the other cells rewrite the values they hold
+    // (a brick is visited at most once per round, so nothing else writes them meanwhile); two
+    // 16-byte stores where the row is aligned and complete, else cell by cell within the grid
     if (changed) {
         const uint32_t row = (((uint32_t)X0 + x) * ny + ((uint32_t)Y0 + y)) * nz + (uint32_t)Z0;
         if ((uint32_t)Z0 + 8u <= nz && (nz & 3u) == 0) {
-            uint32_t out[8];
-#pragma unroll
-            for (int z = 0; z < 8; ++z) out[z] = k[z] == kTileEmpty ? kEmptyKey : k[z];
-            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row) = vf_u32x4{ out[0], out[1], out[2], out[3] };
-            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row + 4) = vf_u32x4{ out[4], out[5], out[6], out[7] };
+            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row) = vf_u32x4{ k[0], k[1], k[2], k[3] };
+            *(VF_AS_GLOBAL vf_u32x4*)(R.keys + row + 4) = vf_u32x4{ k[4], k[5], k[6], k[7] };
         } else {
 #pragma unroll
             for (int z = 0; z < 8; ++z)
-                if (changed & (1u << z)) R.keys[row + z] = k[z];
+                if ((uint32_t)Z0 + (uint32_t)z < nz) R.keys[row + z] = k[z];
         }
     }
 
@@ -322,7 +327,6 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     uint32_t dirs = 0;
     if (changed) {
         const uint32_t sx = x == 0 ? 0u : (x == 7 ? 2u : 1u), sy = y == 0 ? 0u : (y == 7 ? 2u : 1u);
-        const uint32_t zlo = changed & 1u, zhi = (changed >> 7) & 1u;
         if (R.n26) {
             const uint32_t mz = 2u | zlo | (zhi << 2);  // bit iz for iz in {1, 0 if z = 0 dropped, 2 if z = 7 dropped}
             dirs = (mz << 12) | (mz << (sx * 9u + 3u)) | (mz << (9u + sy * 3u)) | (mz << (sx * 9u + sy * 3u));
