@@ -7,8 +7,8 @@
 // (the reference's float distances on integer offsets order identically).
 // The argmin is a single unsigned min over packed keys (distance << b | i),
 // so ties go to the lowest index exactly as the reference's strict '<' does.
-// Seeds are staged through LDS in chunks; each thread owns 8 cells of one z
-// run, so the x/y part of the distance is computed once per seed.
+// Seeds are staged through LDS in chunks; each tile's occupied cells are
+// compacted and a thread applies every seed to its few cells in registers.
 #include "cellops.hpp"
 
 namespace vf {
@@ -23,59 +23,6 @@ __device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
     return ((K)d << b) | (K)i;
 }
 
-// Distance argmin for one thread's 8 cells against seeds [base, base+n) staged
-// in LDS.  FAST: the 8 cells share (x, y) (nz % 8 == 0).
-template <int DIST, bool FAST, typename K>
-__device__ inline void naive_chunk(K best[8], uint32_t x, uint32_t y, uint32_t z, const uint32_t cx[8], const uint32_t cy[8],
-                                   const uint32_t cz[8], const uint32_t* sx, const uint32_t* sy, const uint32_t* sz,
-                                   uint32_t base, uint32_t n, uint32_t b)
-{
-    if (FAST) {
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t id = base + i;
-            const uint32_t ax = __usad(x, sx[i], 0u), ay = __usad(y, sy[i], 0u);
-            if (DIST == VF_MANHATTAN) {
-                const K kb = pack_key<DIST, K>(ax + ay, id, b);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const K key = kb + ((K)__usad(z + k, sz[i], 0u) << b);
-                    best[k] = key < best[k] ? key : best[k];
-                }
-            } else if (DIST == VF_CHEBYSHEV) {
-                const K kb = pack_key<DIST, K>(max(ax, ay), id, b);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    K key = pack_key<DIST, K>(__usad(z + k, sz[i], 0u), id, b);
-                    key = key > kb ? key : kb;
-                    best[k] = key < best[k] ? key : best[k];
-                }
-            } else {
-                const K kb = pack_key<DIST, K>(ax * ax + ay * ay, id, b);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t dz = __usad(z + k, sz[i], 0u);
-                    const K key = kb + ((K)(dz * dz) << b);
-                    best[k] = key < best[k] ? key : best[k];
-                }
-            }
-        }
-    } else {
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t id = base + i;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t ax = __usad(cx[k], sx[i], 0u), ay = __usad(cy[k], sy[i], 0u), az = __usad(cz[k], sz[i], 0u);
-                uint32_t d;
-                if (DIST == VF_MANHATTAN) d = ax + ay + az;
-                else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
-                else d = ax * ax + ay * ay + az * az;
-                const K key = pack_key<DIST, K>(d, id, b);
-                best[k] = key < best[k] ? key : best[k];
-            }
-        }
-    }
-}
-
 // Distance key of one cell to one seed.
 template <int DIST, typename K>
 __device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, uint32_t sy, uint32_t sz, uint32_t id,
@@ -85,28 +32,47 @@ __device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, ui
     uint32_t d;
     if (DIST == VF_MANHATTAN) d = ax + ay + az;
     else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
-    else d = ax * ax + ay * ay + az * az;
+    else d = __umul24(ax, ax) + __umul24(ay, ay) + __umul24(az, az);  // offsets < 2^24: full-rate multiplies
     return pack_key<DIST, K>(d, id, b);
 }
 
-// argmin over seeds [base, base + n) (staged in LDS) for a thread's nq <= 8
-// compacted cells
+// argmin over seeds [base, base + n) (staged in LDS) for Q compacted cells of
+// a thread: each seed is read once (an LDS broadcast) and applied to all Q
+// cells held in registers.
+template <int DIST, typename K, int Q>
+__device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8],
+                                     const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, uint32_t base, uint32_t n,
+                                     uint32_t b)
+{
+    K m[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) m[q] = best[q];
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t x = sx[i], y = sy[i], z = sz[i], id = base + i;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], x, y, z, id, b);
+            m[q] = key < m[q] ? key : m[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) best[q] = m[q];
+}
+
+// qmax: cells per thread, the same for the whole block (threads past the
+// tile's last occupied cell compute on a dummy cell)
 template <int DIST, typename K>
-__device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8], uint32_t nq,
+__device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8], uint32_t qmax,
                                    const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, uint32_t base, uint32_t n,
                                    uint32_t b)
 {
-#pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) {
-        if (q >= nq) break;
-        K m = best[q];
-        for (uint32_t i = 0; i < n; ++i) {
-            const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], sx[i], sy[i], sz[i], base + i, b);
-            m = key < m ? key : m;
-        }
-        best[q] = m;
-    }
+    if (qmax <= 1) naive_cells_q<DIST, K, 1>(best, cx, cy, cz, sx, sy, sz, base, n, b);
+    else if (qmax == 2) naive_cells_q<DIST, K, 2>(best, cx, cy, cz, sx, sy, sz, base, n, b);
+    else if (qmax == 3) naive_cells_q<DIST, K, 3>(best, cx, cy, cz, sx, sy, sz, base, n, b);
+    else if (qmax == 4) naive_cells_q<DIST, K, 4>(best, cx, cy, cz, sx, sy, sz, base, n, b);
+    else naive_cells_q<DIST, K, 8>(best, cx, cy, cz, sx, sy, sz, base, n, b);
 }
+
 
 // One tile per step: the tile's occupied cells are compacted in LDS, so the
 // seed loop runs over occupied cells only (shells occupy a few percent of
@@ -144,6 +110,13 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         const uint64_t t0 = (uint64_t)local * kTileCells;
         const uint64_t c0 = t0 + tid * kCellsPerThread;
         const uint64_t occ = c0 < J.V ? load8b(J.occ, c0, J.V) : 0ull;
+        if (!__syncthreads_or(occ != 0ull)) {  // an EMPTY tile (most of a vessel's box): labels stay EMPTY
+            if (c0 < J.V) {
+                const uint16_t zero[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+                store8(J.labels, c0, J.V, zero);
+            }
+            return;
+        }
         // block-wide exclusive scan of the occupied counts -> compacted positions
         const uint32_t n = (uint32_t)__popcll(occ & 0x0101010101010101ull);
         uint32_t inc = n;
@@ -165,6 +138,7 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         __syncthreads();
         if (total) {
             const uint32_t nq = tid < total ? (total - tid + kCellThreads - 1) / kCellThreads : 0u;
+            const uint32_t qmax = (total + kCellThreads - 1) / kCellThreads;  // block-uniform
             uint32_t cx[8], cy[8], cz[8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q) {
@@ -182,9 +156,9 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
                     stage(J, base, m);
                     staged = 0xFFFFFFFFu;
                 }
-                if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
-                else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
-                else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, nq, sx, sy, sz, base, m, b);
+                if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
+                else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
+                else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
             }
             const K mask = ((K)1 << b) - 1;
 #pragma unroll

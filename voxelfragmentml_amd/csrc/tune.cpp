@@ -18,11 +18,13 @@ namespace vf {
 namespace {
 
 std::mutex g_mu;
+// flood relaxation: 5120 one-wave workgroups = 5 waves per SIMD on 256 CUs, the most the kernel keeps
+// resident (a workgroup that is not resident takes its strided share of the worklist late)
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 2048, 4096, 8192, 4096 },
-    { 2048, 4096, 8192, 4096 },
-    { 2048, 4096, 8192, 4096 },
-    { 2048, 4096, 8192, 4096 },
+    { 2048, 5120, 8192, 4096 },
+    { 2048, 5120, 8192, 4096 },
+    { 2048, 5120, 8192, 4096 },
+    { 2048, 5120, 8192, 4096 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
@@ -80,11 +82,17 @@ using namespace vf;
 namespace {
 
 // A synthetic batch of the bucket: 16 vessel jobs (8 in quick mode) with edges
-// in (edge/2, edge], seeds 2-64, naive and flood alternating.
+// in (edge/2, edge], seeds 2-64, naive and flood alternating; meshes resident
+// in device memory, so a timed run is device work only.
 struct TuneJobs {
     std::vector<std::vector<float>> xyz;
     std::vector<std::vector<uint32_t>> tri;
+    std::vector<void*> dev;  // device copies of the meshes
     std::vector<vf_job> jobs;
+    ~TuneJobs()
+    {
+        for (void* p : dev) vf_device_free(p);
+    }
 };
 
 TuneJobs make_jobs(int bucket, bool quick)
@@ -111,8 +119,17 @@ TuneJobs make_jobs(int bucket, bool quick)
     }
     for (uint32_t i = 0; i < n; ++i) {
         vf_job j{};
-        j.xyz = T.xyz[i].data();
-        j.tri = T.tri[i].data();
+        const size_t bx = T.xyz[i].size() * sizeof(float), bt = T.tri[i].size() * sizeof(uint32_t);
+        void *dx = nullptr, *dt = nullptr;
+        if (vf_device_alloc(bx, &dx) != VF_OK || vf_device_alloc(bt, &dt) != VF_OK)
+            fail(VF_ERR_OUT_OF_MEMORY, "tuning meshes");
+        T.dev.push_back(dx);
+        T.dev.push_back(dt);
+        if (vf_memcpy_htod(dx, T.xyz[i].data(), bx) != VF_OK || vf_memcpy_htod(dt, T.tri[i].data(), bt) != VF_OK)
+            fail(VF_ERR_DEVICE, "tuning meshes");
+        j.xyz = static_cast<const float*>(dx);
+        j.tri = static_cast<const uint32_t*>(dt);
+        j.mesh_on_device = 1;
         j.nv = (uint32_t)(T.xyz[i].size() / 3);
         j.nt = (uint32_t)(T.tri[i].size() / 3);
         const uint32_t e = std::max<uint32_t>(16, (emax / 2 + (uint32_t)(next() % (emax / 2 + 1))) / 4 * 4);
@@ -181,10 +198,9 @@ int vf_tune_run(const char* path, int quick)
         if (!path) fail(VF_ERR_INVALID_ARG, "path is NULL");
         if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
         const std::vector<uint32_t> cand_cell = { 1024, 2048, 4096 };
-        const std::vector<uint32_t> cand_flood = { 2048, 4096, 8192 };
         const std::vector<uint32_t> cand_pair = { 4096, 8192, 16384 };
         const std::vector<uint32_t> cand_brick = { 2048, 4096, 8192 };
-        const int reps = quick ? 3 : 5;
+        const int reps = quick ? 5 : 7;
         for (int bucket = quick ? 1 : 0; bucket < kTuneBuckets; ++bucket) {
             TuneJobs T = make_jobs(bucket, quick != 0);
             vf_batch* b = nullptr;
@@ -209,7 +225,9 @@ int vf_tune_run(const char* path, int quick)
                 }
                 set_shape(bucket, best);
             };
-            sweep(&LaunchShape::flood_blocks, cand_flood);
+            // flood_blocks stays at the relaxation's resident maximum: its best value follows the
+            // length of the brick worklist, which grows with the number of jobs in a batch, not with
+            // the bucket's edge, so a bucket timed alone does not predict a mixed batch
             sweep(&LaunchShape::cell_blocks, cand_cell);
             sweep(&LaunchShape::pair_blocks, cand_pair);
             sweep(&LaunchShape::brick_blocks, cand_brick);
