@@ -18,13 +18,14 @@ namespace vf {
 namespace {
 
 std::mutex g_mu;
-// flood relaxation: 5120 one-wave workgroups = 5 waves per SIMD on 256 CUs, the most the kernel keeps
-// resident (a workgroup that is not resident takes its strided share of the worklist late)
+// flood relaxation: 4608 one-wave workgroups (18 per CU).  Each takes a fixed strided share of its
+// XCD's brick list, so workgroups beyond those resident at once run their whole share late; on
+// MI355X (mixed 64-job batch) 4096 and 4608 relax in 4.55 ms, 5120 and 6144 in 5.1-5.4 ms.
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 2048, 5120, 8192, 4096 },
-    { 2048, 5120, 8192, 4096 },
-    { 2048, 5120, 8192, 4096 },
-    { 2048, 5120, 8192, 4096 },
+    { 2048, 4608, 8192, 4096 },
+    { 2048, 4608, 8192, 4096 },
+    { 2048, 4608, 8192, 4096 },
+    { 2048, 4608, 8192, 4096 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
@@ -225,9 +226,9 @@ int vf_tune_run(const char* path, int quick)
                 }
                 set_shape(bucket, best);
             };
-            // flood_blocks stays at the relaxation's resident maximum: its best value follows the
-            // length of the brick worklist, which grows with the number of jobs in a batch, not with
-            // the bucket's edge, so a bucket timed alone does not predict a mixed batch
+            // flood_blocks keeps its default: its best value follows the length of the brick
+            // worklist, which grows with the number of jobs in a batch, not with the bucket's edge,
+            // so a bucket timed alone does not predict a mixed batch
             sweep(&LaunchShape::cell_blocks, cand_cell);
             sweep(&LaunchShape::pair_blocks, cand_pair);
             sweep(&LaunchShape::brick_blocks, cand_brick);
