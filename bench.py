@@ -36,6 +36,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = "fragmented Gvoxels/s + p50/p99 job latency, batched/served vs plain, bit-exact"
 BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 summary of the default workload: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a
+# 1 GiB copy's counts, written as "traffic_bytes_per_launch <kernel> <bytes>"
+TRAFFIC_SUMMARY = ROOT / "profiles" / "kernels_mixed64.md"
+DEFAULT_JOBS = 64
 
 
 def log(*a):
@@ -52,6 +56,17 @@ def job_specs(n, seed, emin=64, emax=384, smax=64, tmin=5000, tmax=300000):
                           mesh_seed=int(rng.integers(1, 1 << 30)), seed_key=int(rng.integers(1, 1 << 40)),
                           noise_seed=int(rng.integers(1, 1 << 40))))
     return specs
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default workload from the committed counter summary, or None."""
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    for line in TRAFFIC_SUMMARY.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
 
 
 def build_jobs(vf, specs, mc=False):
@@ -213,7 +228,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--jobs", type=int, default=DEFAULT_JOBS)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -313,6 +328,11 @@ def main():
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                            "bytes_per_launch": per_launch, "avg_launch_ms": prof[dom]["avg_ms"],
                            "launches_per_step": prof[dom]["launches"]}
+        traffic = measured_traffic(dom) if a.jobs == DEFAULT_JOBS else None
+        if traffic:
+            out["roofline"]["traffic"] = traffic
+            out["roofline"]["traffic_source"] = (f"{TRAFFIC_SUMMARY.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE and "
+                                                 "WRITE_SIZE of this workload, each scaled by a 1 GiB copy's count")
         if dom == "flood_relax":
             visits = sum(r["flood_bricks"] for r in pres)
             out["roofline"]["unit_work"] = {"brick_visits_per_step": visits, "bytes_per_visit": RELAX_BYTES_PER_VISIT,
