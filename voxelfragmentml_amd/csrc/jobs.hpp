@@ -125,9 +125,10 @@ struct Launch {
 // voxelize.hip
 struct VoxScratch {
     TriSetup* setup;                   // total triangles
-    unsigned long long* shell_off;     // total triangles + 1 exclusive prefix of the shell pair counts
+    unsigned long long* shell_off;     // total triangles + 1 exclusive prefix of the shell work counts
     unsigned long long* col_off;
-    uint32_t* counts;                  // 2*total triangles: shell pair counts, then column pair counts
+    uint32_t* counts;                  // 2*total triangles: shell work counts (column segments of one z word
+                                       // per candidate (x, y)), then parity column counts
     const uint32_t* tri_prefix;        // njobs+1 prefix over triangles (device)
     uint32_t total_tris;
     void* scan_tmp;                    // scan scratch

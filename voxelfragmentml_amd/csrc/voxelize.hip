@@ -6,8 +6,9 @@
 //   1. k_vox_setup   per triangle: SAT candidate cell box, snapped fixed-point
 //                    vertices, parity column range, pair counts
 //   2. scans         exclusive prefix of the pair counts (load balancing)
-//   3. k_vox_shell   one thread per (triangle, cell) pair: SAT
-//                    (Intersections3D.h:204-420 arithmetic), atomicOr shell bit
+//   3. k_vox_shell   one thread per (triangle, column, 32-cell z word): SAT
+//                    (Intersections3D.h:204-420 arithmetic) per cell with the
+//                    z-free terms once per column, one atomicOr per word
 //   4. k_vox_parity  one thread per (triangle, column) pair: top-left covered
 //                    column centres toggle the first cell centre above the
 //                    crossing (exact int64 edge functions), atomicXor
@@ -35,78 +36,80 @@ __device__ inline bool axis_sep(float pa, float pb, float rad)
     return mn > rad || mx < -rad;
 }
 
-__device__ bool sat_triangle_box(const float* a, const float* b, const float* c, const float ctr[3], const float half[3])
-{
-    float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        v0[q] = a[q] - ctr[q];
-        v1[q] = b[q] - ctr[q];
-        v2[q] = c[q] - ctr[q];
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        e0[q] = v1[q] - v0[q];
-        e1[q] = v2[q] - v1[q];
-        e2[q] = v0[q] - v2[q];
-    }
-    float fx, fy, fz, pa, pb, rad;
-    fx = fabsf(e0[0]); fy = fabsf(e0[1]); fz = fabsf(e0[2]);
-    pa = e0[2] * v0[1] - e0[1] * v0[2]; pb = e0[2] * v2[1] - e0[1] * v2[2];
-    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = -e0[2] * v0[0] + e0[0] * v0[2]; pb = -e0[2] * v2[0] + e0[0] * v2[2];
-    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = e0[1] * v1[0] - e0[0] * v1[1]; pb = e0[1] * v2[0] - e0[0] * v2[1];
-    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
-
-    fx = fabsf(e1[0]); fy = fabsf(e1[1]); fz = fabsf(e1[2]);
-    pa = e1[2] * v0[1] - e1[1] * v0[2]; pb = e1[2] * v2[1] - e1[1] * v2[2];
-    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = -e1[2] * v0[0] + e1[0] * v0[2]; pb = -e1[2] * v2[0] + e1[0] * v2[2];
-    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = e1[1] * v0[0] - e1[0] * v0[1]; pb = e1[1] * v1[0] - e1[0] * v1[1];
-    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
-
-    fx = fabsf(e2[0]); fy = fabsf(e2[1]); fz = fabsf(e2[2]);
-    pa = e2[2] * v0[1] - e2[1] * v0[2]; pb = e2[2] * v1[1] - e2[1] * v1[2];
-    rad = fz * half[1] + fy * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = -e2[2] * v0[0] + e2[0] * v0[2]; pb = -e2[2] * v1[0] + e2[0] * v1[2];
-    rad = fz * half[0] + fx * half[2]; if (axis_sep(pa, pb, rad)) return false;
-    pa = e2[1] * v1[0] - e2[0] * v1[1]; pb = e2[1] * v2[0] - e2[0] * v2[1];
-    rad = fy * half[0] + fx * half[1]; if (axis_sep(pa, pb, rad)) return false;
-
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        float mn = v0[q], mx = v0[q];
-        if (v1[q] < mn) mn = v1[q];
-        if (v1[q] > mx) mx = v1[q];
-        if (v2[q] < mn) mn = v2[q];
-        if (v2[q] > mx) mx = v2[q];
-        if (mn > half[q] || mx < -half[q]) return false;
-    }
-
-    float n[3];
-    n[0] = e0[1] * e1[2] - e1[1] * e0[2];
-    n[1] = e0[2] * e1[0] - e1[2] * e0[0];
-    n[2] = e0[0] * e1[1] - e1[0] * e0[1];
-    float vmin[3], vmax[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        if (n[q] > 0.0f) { vmin[q] = -half[q] - v0[q]; vmax[q] = half[q] - v0[q]; }
-        else             { vmin[q] = half[q] - v0[q];  vmax[q] = -half[q] - v0[q]; }
-    }
-    float dmin = n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2];
-    if (dmin > 0.0f) return false;
-    float dmax = n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2];
-    return dmax >= 0.0f;
-}
-
 __device__ inline void cell_box(int i, float mn, float cs, float* center, float* half)
 {
     float lo = mn + (float)i * cs;
     float hi = mn + (float)(i + 1) * cs;
     *center = (hi + lo) / 2.0f;
     *half = hi - *center;
+}
+
+// The triangle/box SAT of Intersections3D::intersect(Triangle3D&, AABB&) (Intersections3D.h:204-420:
+// nine edge-cross axes, the three box faces, the triangle plane) for the cells (x, y, z0..z1) of one
+// grid column, z0..z1 inside one 32-cell word: bit (z & 31) is set when cell (x, y, z) passes.  For
+// each cell every FP32 operation is the reference's (same operands, same order, no contraction), so
+// a cell passes exactly when the per-cell test passes; the operations that do not depend on z (x and
+// y offsets and edge components, the third axis of each edge, the x and y box faces, the normal's z
+// component) are evaluated once per column, and a cell's tests are combined without branches (which
+// axis separates first does not change the outcome).
+__device__ uint32_t sat_column(const float* a, const float* b, const float* c, float cx, float hx, float cy, float hy,
+                               int z0, int z1, float mnz, float csz)
+{
+    const float v0x = a[0] - cx, v1x = b[0] - cx, v2x = c[0] - cx;
+    const float v0y = a[1] - cy, v1y = b[1] - cy, v2y = c[1] - cy;
+    const float e0x = v1x - v0x, e1x = v2x - v1x, e2x = v0x - v2x;
+    const float e0y = v1y - v0y, e1y = v2y - v1y, e2y = v0y - v2y;
+    const float f0x = fabsf(e0x), f0y = fabsf(e0y), f1x = fabsf(e1x), f1y = fabsf(e1y), f2x = fabsf(e2x), f2y = fabsf(e2y);
+    bool sep = axis_sep(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, f0y * hx + f0x * hy);
+    sep |= axis_sep(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, f1y * hx + f1x * hy);
+    sep |= axis_sep(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, f2y * hx + f2x * hy);
+    {
+        float mn = v0x, mx = v0x;
+        if (v1x < mn) mn = v1x;
+        if (v1x > mx) mx = v1x;
+        if (v2x < mn) mn = v2x;
+        if (v2x > mx) mx = v2x;
+        sep |= mn > hx || mx < -hx;
+        mn = v0y; mx = v0y;
+        if (v1y < mn) mn = v1y;
+        if (v1y > mx) mx = v1y;
+        if (v2y < mn) mn = v2y;
+        if (v2y > mx) mx = v2y;
+        sep |= mn > hy || mx < -hy;
+    }
+    if (sep) return 0u;
+    const float n2 = e0x * e1y - e1x * e0y;
+    const float xlo = -hx - v0x, xhi = hx - v0x, ylo = -hy - v0y, yhi = hy - v0y;
+    uint32_t bits = 0;
+    for (int z = z0; z <= z1; ++z) {
+        float cz, hz;
+        cell_box(z, mnz, csz, &cz, &hz);
+        const float v0z = a[2] - cz, v1z = b[2] - cz, v2z = c[2] - cz;
+        const float e0z = v1z - v0z, e1z = v2z - v1z, e2z = v0z - v2z;
+        const float f0z = fabsf(e0z), f1z = fabsf(e1z), f2z = fabsf(e2z);
+        bool s = axis_sep(e0z * v0y - e0y * v0z, e0z * v2y - e0y * v2z, f0z * hy + f0y * hz);
+        s |= axis_sep(-e0z * v0x + e0x * v0z, -e0z * v2x + e0x * v2z, f0z * hx + f0x * hz);
+        s |= axis_sep(e1z * v0y - e1y * v0z, e1z * v2y - e1y * v2z, f1z * hy + f1y * hz);
+        s |= axis_sep(-e1z * v0x + e1x * v0z, -e1z * v2x + e1x * v2z, f1z * hx + f1x * hz);
+        s |= axis_sep(e2z * v0y - e2y * v0z, e2z * v1y - e2y * v1z, f2z * hy + f2y * hz);
+        s |= axis_sep(-e2z * v0x + e2x * v0z, -e2z * v1x + e2x * v1z, f2z * hx + f2x * hz);
+        float mn = v0z, mx = v0z;
+        if (v1z < mn) mn = v1z;
+        if (v1z > mx) mx = v1z;
+        if (v2z < mn) mn = v2z;
+        if (v2z > mx) mx = v2z;
+        s |= mn > hz || mx < -hz;
+        const float n0 = e0y * e1z - e1y * e0z;
+        const float n1 = e0z * e1x - e1z * e0x;
+        const float vmin0 = n0 > 0.0f ? xlo : xhi, vmax0 = n0 > 0.0f ? xhi : xlo;
+        const float vmin1 = n1 > 0.0f ? ylo : yhi, vmax1 = n1 > 0.0f ? yhi : ylo;
+        const float vmin2 = n2 > 0.0f ? -hz - v0z : hz - v0z, vmax2 = n2 > 0.0f ? hz - v0z : -hz - v0z;
+        const float dmin = n0 * vmin0 + n1 * vmin1 + n2 * vmin2;
+        const float dmax = n0 * vmax0 + n1 * vmax1 + n2 * vmax2;
+        s |= dmin > 0.0f || !(dmax >= 0.0f);
+        bits |= s ? 0u : 1u << (z & 31);
+    }
+    return bits;
 }
 
 __device__ inline int32_t snap_coord(float x, float mn, float cs)
@@ -234,20 +237,24 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                 }
             }
         }
-        // small triangles: test the candidates here; only large ones go to the pair kernels
+        // small triangles: test the candidates here; only large ones go to the pair kernels, as
+        // segments of a column (x, y) that lie in one 32-cell word of z
         if (nshell && nshell <= kInlineShell) {
             for (int x = S.lo[0]; x <= S.hi[0]; ++x)
-                for (int y = S.lo[1]; y <= S.hi[1]; ++y)
-                    for (int z = S.lo[2]; z <= S.hi[2]; ++z) {
-                        const size_t word = ((size_t)x * J->ny + (size_t)y) * J->wp + (size_t)(z >> 5);
-                        const uint32_t bit = 1u << (z & 31);
-                        float c[3], h[3];
-                        cell_box(x, J->mn[0], J->cs[0], &c[0], &h[0]);
-                        cell_box(y, J->mn[1], J->cs[1], &c[1], &h[1]);
-                        cell_box(z, J->mn[2], J->cs[2], &c[2], &h[2]);
-                        if (sat_triangle_box(S.P[0], S.P[1], S.P[2], c, h)) atomicOr(J->shell_bits + word, bit);
+                for (int y = S.lo[1]; y <= S.hi[1]; ++y) {
+                    float cx, hx, cy, hy;
+                    cell_box(x, J->mn[0], J->cs[0], &cx, &hx);
+                    cell_box(y, J->mn[1], J->cs[1], &cy, &hy);
+                    for (int w = S.lo[2] >> 5; w <= S.hi[2] >> 5; ++w) {
+                        const uint32_t bits = sat_column(S.P[0], S.P[1], S.P[2], cx, hx, cy, hy, max(S.lo[2], 32 * w),
+                                                         min(S.hi[2], 32 * w + 31), J->mn[2], J->cs[2]);
+                        if (bits) atomicOr(J->shell_bits + ((size_t)x * J->ny + (size_t)y) * J->wp + (size_t)w, bits);
                     }
+                }
             nshell = 0;
+        } else if (nshell) {
+            nshell = (uint32_t)(S.hi[0] - S.lo[0] + 1) * (uint32_t)(S.hi[1] - S.lo[1] + 1) *
+                     (uint32_t)((S.hi[2] >> 5) - (S.lo[2] >> 5) + 1);
         }
         if (ncol && ncol <= kInlineCols) {
             for (int i = S.i0; i <= S.i1; ++i)
@@ -324,51 +331,31 @@ __device__ inline void for_each_pair(const unsigned long long* off, uint32_t ntr
     }
 }
 
-// Wave-combined bit update: lanes holding the same target word merge their
-// bits (OR, or XOR for toggles) and the first lane of each run of equal
-// words issues one device atomic.  All 64 lanes must call it.
-template <bool XOR>
-__device__ inline void wave_bits(uint32_t* word, uint32_t bits)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t key = reinterpret_cast<uint64_t>(word);
-    uint32_t acc = bits;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t other = __shfl_down(acc, o, 64);
-        const uint64_t okey = __shfl_down(key, o, 64);
-        if (lane + (uint32_t)o < 64u && okey == key) acc = XOR ? (acc ^ other) : (acc | other);
-    }
-    const uint64_t pkey = __shfl_up(key, 1, 64);
-    const bool leader = lane == 0 || pkey != key;
-    if (leader && acc && word) {
-        if (XOR) atomicXor(word, acc);
-        else atomicOr(word, acc);
-    }
-}
-
+// One thread per column segment of a triangle's candidate box: the cells (x, y, z) with z in one
+// 32-cell word, tested by sat_column and merged into the shell word with one atomicOr.
 __global__ __launch_bounds__(kVoxThreads) void k_vox_shell(const JobDev* jobs, const TriSetup* setup,
                                                            const unsigned long long* off, uint32_t ntris)
 {
     for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
-        uint32_t* word = nullptr;
-        uint32_t bit = 0;
-        if (valid) {
-            const TriSetup* S = setup + g;
-            const JobDev* J = jobs + S->job;
-            const uint32_t ez = (uint32_t)(S->hi[2] - S->lo[2] + 1), ey = (uint32_t)(S->hi[1] - S->lo[1] + 1);
-            const uint32_t z = (uint32_t)S->lo[2] + (uint32_t)(local % ez);
-            const uint64_t rest = local / ez;
-            const uint32_t y = (uint32_t)S->lo[1] + (uint32_t)(rest % ey);
-            const uint32_t x = (uint32_t)S->lo[0] + (uint32_t)(rest / ey);
-            float c[3], h[3];
-            cell_box((int)x, J->mn[0], J->cs[0], &c[0], &h[0]);
-            cell_box((int)y, J->mn[1], J->cs[1], &c[1], &h[1]);
-            cell_box((int)z, J->mn[2], J->cs[2], &c[2], &h[2]);
-            word = J->shell_bits + ((size_t)x * J->ny + y) * J->wp + (z >> 5);
-            if (sat_triangle_box(S->P[0], S->P[1], S->P[2], c, h)) bit = 1u << (z & 31);
+        if (!valid) return;
+        const TriSetup* S = setup + g;
+        const JobDev* J = jobs + S->job;
+        const int lo2 = S->lo[2], hi2 = S->hi[2];
+        const uint32_t w0 = (uint32_t)lo2 >> 5, nseg = ((uint32_t)hi2 >> 5) - w0 + 1;
+        const uint32_t ey = (uint32_t)(S->hi[1] - S->lo[1] + 1);
+        uint32_t col = (uint32_t)local, seg = 0;  // < 2^32: columns x segments of one triangle
+        if (nseg > 1) {
+            seg = col % nseg;
+            col /= nseg;
         }
-        wave_bits<false>(word, bit);
+        const int y = S->lo[1] + (int)(col % ey), x = S->lo[0] + (int)(col / ey);
+        const uint32_t w = w0 + seg;
+        float cx, hx, cy, hy;
+        cell_box(x, J->mn[0], J->cs[0], &cx, &hx);
+        cell_box(y, J->mn[1], J->cs[1], &cy, &hy);
+        const uint32_t bits = sat_column(S->P[0], S->P[1], S->P[2], cx, hx, cy, hy, max(lo2, (int)(32 * w)),
+                                         min(hi2, (int)(32 * w + 31)), J->mn[2], J->cs[2]);
+        if (bits) atomicOr(J->shell_bits + ((size_t)x * J->ny + (size_t)y) * J->wp + w, bits);
     });
 }
 
