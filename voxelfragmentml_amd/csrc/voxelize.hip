@@ -112,9 +112,11 @@ __device__ uint32_t sat_column(const float* a, const float* b, const float* c, f
     return bits;
 }
 
-__device__ inline int32_t snap_coord(float x, float mn, float cs)
+// (x - mn) / cs in grid cells, the quotient both the candidate range and the snap start from
+__device__ inline float grid_coord(float x, float mn, float cs) { return (x - mn) / cs; }
+
+__device__ inline int32_t snap_coord(float q)
 {
-    float q = (x - mn) / cs;
     q = q * (float)kSub;
     if (!(q >= -kSnapLim)) q = -kSnapLim;
     if (q > kSnapLim) q = kSnapLim;
@@ -128,10 +130,20 @@ __device__ inline bool edge_incl(int64_t e, int64_t dx, int64_t dy)
     return e > 0 || (e == 0 && (dy < 0 || (dy == 0 && dx < 0)));
 }
 
+// floor(a / b) for b > 0: an FP32 estimate, then exact int64 remainder corrections (the estimate is
+// within one of the quotient wherever |a / b| < 2^20; the loops make it exact everywhere)
 __device__ inline int64_t floordiv64(int64_t a, int64_t b)
 {
-    int64_t q = a / b;
-    if ((a % b) != 0 && a < 0) q -= 1;
+    int64_t q = (int64_t)floorf((float)a * __builtin_amdgcn_rcpf((float)b));
+    int64_t r = a - q * b;
+    while (r < 0) {
+        --q;
+        r += b;
+    }
+    while (r >= b) {
+        ++q;
+        r -= b;
+    }
     return q;
 }
 
@@ -183,18 +195,23 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                 }
             const uint32_t dims[3] = { J->nx, J->ny, J->nz };
             if (finite) {
+                // vertex coordinates in cells; (v - mn) / cs is monotone in v, so the extremes
+                // of these are the quotients of the extreme coordinates
+                float G[3][3];
+                for (int k = 0; k < 3; ++k)
+                    for (int q = 0; q < 3; ++q) G[k][q] = grid_coord(P[k][q], J->mn[q], J->cs[q]);
                 bool empty = false;
                 for (int q = 0; q < 3; ++q) {
-                    float mn = P[0][q], mx = P[0][q];
-                    if (P[1][q] < mn) mn = P[1][q];
-                    if (P[1][q] > mx) mx = P[1][q];
-                    if (P[2][q] < mn) mn = P[2][q];
-                    if (P[2][q] > mx) mx = P[2][q];
+                    float mn = G[0][q], mx = G[0][q];
+                    if (G[1][q] < mn) mn = G[1][q];
+                    if (G[1][q] > mx) mx = G[1][q];
+                    if (G[2][q] < mn) mn = G[2][q];
+                    if (G[2][q] > mx) mx = G[2][q];
                     // candidate cells: those within 1/64 cell of the triangle's extent (the
                     // SAT's FP32 box test can only accept touching cells, and its rounding
                     // is far below the margin)
-                    float fl = floorf((mn - J->mn[q]) / J->cs[q] - kRangeMargin);
-                    float fh = floorf((mx - J->mn[q]) / J->cs[q] + kRangeMargin);
+                    float fl = floorf(mn - kRangeMargin);
+                    float fh = floorf(mx + kRangeMargin);
                     const float lim = (float)dims[q] + 1.0f;
                     fl = fl < -2.0f ? -2.0f : (fl > lim ? lim : fl);
                     fh = fh < -2.0f ? -2.0f : (fh > lim ? lim : fh);
@@ -207,9 +224,9 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                              (uint32_t)(S.hi[2] - S.lo[2] + 1);
                 int64_t X[3], Y[3], Z[3];
                 for (int k = 0; k < 3; ++k) {
-                    X[k] = snap_coord(P[k][0], J->mn[0], J->cs[0]);
-                    Y[k] = snap_coord(P[k][1], J->mn[1], J->cs[1]);
-                    Z[k] = snap_coord(P[k][2], J->mn[2], J->cs[2]);
+                    X[k] = snap_coord(G[k][0]);
+                    Y[k] = snap_coord(G[k][1]);
+                    Z[k] = snap_coord(G[k][2]);
                 }
                 int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (Y[1] - Y[0]) * (X[2] - X[0]);
                 if (area != 0) {
