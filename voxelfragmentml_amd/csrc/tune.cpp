@@ -18,14 +18,16 @@ namespace vf {
 namespace {
 
 std::mutex g_mu;
-// flood relaxation: 4608 one-wave workgroups (18 per CU).  Each takes a fixed strided share of its
-// XCD's brick list, so workgroups beyond those resident at once run their whole share late; on
-// MI355X (mixed 64-job batch) 4096 and 4608 relax in 4.55 ms, 5120 and 6144 in 5.1-5.4 ms.
+// Defaults measured on MI355X with the mixed 64-job batch.  Cell kernels: 8192 workgroups of 4 waves
+// (8 waves per SIMD for the streaming kernels; 2048 left them at 2 and latency-bound).  Boundary
+// bricks: 16384.  Flood relaxation: 4608 one-wave workgroups (18 per CU); each takes a fixed strided
+// share of its XCD's brick list, so workgroups beyond those resident at once run their whole share
+// late (4096 and 4608 relax in 4.55 ms, 5120 and 6144 in 5.1-5.4 ms).
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 2048, 4608, 8192, 4096 },
-    { 2048, 4608, 8192, 4096 },
-    { 2048, 4608, 8192, 4096 },
-    { 2048, 4608, 8192, 4096 },
+    { 8192, 4608, 8192, 16384 },
+    { 8192, 4608, 8192, 16384 },
+    { 8192, 4608, 8192, 16384 },
+    { 8192, 4608, 8192, 16384 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
@@ -198,9 +200,9 @@ int vf_tune_run(const char* path, int quick)
     return guard([&] {
         if (!path) fail(VF_ERR_INVALID_ARG, "path is NULL");
         if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
-        const std::vector<uint32_t> cand_cell = { 1024, 2048, 4096 };
-        const std::vector<uint32_t> cand_pair = { 4096, 8192, 16384 };
-        const std::vector<uint32_t> cand_brick = { 2048, 4096, 8192 };
+        const std::vector<uint32_t> cand_cell = { 4096, 8192, 16384 };
+        const std::vector<uint32_t> cand_pair = { 8192, 16384, 32768 };
+        const std::vector<uint32_t> cand_brick = { 4096, 8192, 16384 };
         const int reps = quick ? 5 : 7;
         for (int bucket = quick ? 1 : 0; bucket < kTuneBuckets; ++bucket) {
             TuneJobs T = make_jobs(bucket, quick != 0);
