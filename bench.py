@@ -38,7 +38,7 @@ BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 summary of the default workload: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a
 # 1 GiB copy's counts, written as "traffic_bytes_per_launch <kernel> <bytes>"
-TRAFFIC_SUMMARY = ROOT / "profiles" / "kernels_mixed64.md"
+TRAFFIC_SUMMARY = ROOT / "profiles" / "flood_relax_traffic_mixed64.txt"
 DEFAULT_JOBS = 64
 
 
