@@ -481,7 +481,8 @@ int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         uint64_t dmax = distance == VF_MANHATTAN ? ex + ey + ez
                         : distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
                                                    : ex * ex + ey * ey + ez * ez;
-        const bool key64 = ((dmax + 1) << b) > 0xFFFFFFFFull;
+        // 32-bit keys pack d * 2^b + i with a 24-bit multiply: 2^b must fit 24 bits
+        const bool key64 = b > 23 || ((dmax + 1) << b) > 0xFFFFFFFFull;
         g->job.key_bits = b;
         g->job.dist = distance;
         g->job.algo = VF_NAIVE;

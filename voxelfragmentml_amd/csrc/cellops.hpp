@@ -176,6 +176,38 @@ __device__ inline void walk_tiles_strided(const Launch& L, const uint32_t* prefi
     }
 }
 
+// Grid-stride over runs of N consecutive tiles: run r (tiles r*N .. r*N + N - 1, split where a job
+// ends) goes to block r % gridDim.x, and f(J, j, first_local_tile, n) handles n <= N tiles of job j,
+// so a stage whose tiles differ in cost keeps the round-robin balance and can still issue the
+// loads of a whole run before using any of them.
+template <uint32_t N, class F>
+__device__ inline void walk_tile_runs_strided(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ,
+                                              uint32_t* sp, F&& f)
+{
+    for (uint32_t i = threadIdx.x; i <= L.njobs && i <= kWalkMaxJobs; i += blockDim.x) sp[i] = prefix[i];
+    __syncthreads();
+    uint32_t cur = 0xFFFFFFFFu;
+    const uint32_t runs = (total + N - 1) / N;
+    for (uint32_t r = blockIdx.x; r < runs; r += gridDim.x) {
+        uint32_t t = r * N;
+        const uint32_t t1 = min(t + N, total);
+        while (t < t1) {
+            uint32_t lo = 0, hi = L.njobs;  // largest j with sp[j] <= t
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sp[mid] <= t) lo = mid; else hi = mid;
+            }
+            if (lo != cur) {
+                stage_job(sJ, L.jobs + lo);
+                cur = lo;
+            }
+            const uint32_t n = min(t1, sp[lo + 1]) - t;
+            f(*sJ, lo, t - sp[lo], n);
+            t += n;
+        }
+    }
+}
+
 __device__ inline void decompose(uint64_t c, const JobDev& J, uint32_t& x, uint32_t& y, uint32_t& z)
 {
     uint32_t yz = J.ny * J.nz;

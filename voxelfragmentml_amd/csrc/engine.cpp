@@ -151,7 +151,7 @@ void Engine::alloc(EJob& e)
         const uint64_t dmax = j.distance == VF_MANHATTAN ? ex + ey + ez
                               : j.distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
                                                            : ex * ex + ey * ey + ez * ez;
-        d.naive_key64 = ((dmax + 1) << d.key_bits) > 0xFFFFFFFFull ? 1 : 0;
+        d.naive_key64 = d.key_bits > 23 || ((dmax + 1) << d.key_bits) > 0xFFFFFFFFull ? 1 : 0;
     }
     if (j.mesh_on_device) {
         d.xyz = j.xyz;

@@ -17,22 +17,33 @@ namespace {
 
 constexpr uint32_t kSeedChunk = 1024;
 
+// (d << b) | i.  32-bit keys: d < 2^24 (at most 3 * 2047^2) and b <= 17, so it is one 24-bit
+// multiply-add, d * 2^b + i.
 template <int DIST, typename K>
 __device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
 {
-    return ((K)d << b) | (K)i;
+    if constexpr (sizeof(K) == 4) return (K)(__umul24(d, 1u << b) + i);
+    else return ((K)d << b) | (K)i;
 }
 
-// Distance key of one cell to one seed.
+// |a.lo - b.lo| + |a.hi - b.hi| + acc over the 16-bit halves (v_sad_u16, one instruction); cell and seed
+// coordinates are below 2^11 (grid edges of at most 2048 cells), so the halves never wrap.
+__device__ inline uint32_t sad16(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u16(a, b, acc); }
+
+// Distance key of one cell to one seed; xy and sxy hold x | y << 16.
 template <int DIST, typename K>
-__device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, uint32_t sy, uint32_t sz, uint32_t id,
-                             uint32_t b)
+__device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t xy, uint32_t sx, uint32_t sy, uint32_t sz,
+                             uint32_t sxy, uint32_t id, uint32_t b)
 {
-    const uint32_t ax = __usad(x, sx, 0u), ay = __usad(y, sy, 0u), az = __usad(z, sz, 0u);
     uint32_t d;
-    if (DIST == VF_MANHATTAN) d = ax + ay + az;
-    else if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
-    else d = __umul24(ax, ax) + __umul24(ay, ay) + __umul24(az, az);  // offsets < 2^24: full-rate multiplies
+    if (DIST == VF_MANHATTAN) {
+        d = sad16(xy, sxy, sad16(z, sz, 0u));
+    } else {
+        const uint32_t ax = sad16(x, sx, 0u), ay = sad16(y, sy, 0u), az = sad16(z, sz, 0u);
+        if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
+        else  // offsets < 2^11: 16-bit operands select the full-rate 24-bit multiply
+            d = (ax & 0xFFFFu) * (ax & 0xFFFFu) + (ay & 0xFFFFu) * (ay & 0xFFFFu) + (az & 0xFFFFu) * (az & 0xFFFFu);
+    }
     return pack_key<DIST, K>(d, id, b);
 }
 
@@ -45,13 +56,17 @@ __device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint
                                      uint32_t b)
 {
     K m[Q];
+    uint32_t cxy[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) m[q] = best[q];
+    for (int q = 0; q < Q; ++q) {
+        m[q] = best[q];
+        cxy[q] = cx[q] | (cy[q] << 16);
+    }
     for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t x = sx[i], y = sy[i], z = sz[i], id = base + i;
+        const uint32_t x = sx[i], y = sy[i], z = sz[i], xy = x | (y << 16), id = base + i;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], x, y, z, id, b);
+            const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], cxy[q], x, y, z, xy, id, b);
             m[q] = key < m[q] ? key : m[q];
         }
     }
@@ -59,7 +74,7 @@ __device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint
     for (int q = 0; q < Q; ++q) best[q] = m[q];
 }
 
-// qmax: cells per thread, the same for the whole block (threads past the
+// qmax: cells per thread, the same for the whole wave (lanes past the
 // tile's last occupied cell compute on a dummy cell)
 template <int DIST, typename K>
 __device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8], uint32_t qmax,
@@ -84,7 +99,9 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
     __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
     __shared__ uint16_t sw[kSeedChunk];
     __shared__ uint16_t cells[kTileCells];  // compacted offsets of the occupied cells
-    __shared__ uint16_t lab[kTileCells];    // the tile's output labels
+    __shared__ uint4 lab4[kTileCells / 8];  // the tile's output labels, 8 per thread
+    typedef uint16_t __attribute__((may_alias)) label_alias;  // single labels of lab4
+    label_alias* lab = reinterpret_cast<label_alias*>(lab4);
     __shared__ uint32_t wsum[kCellThreads / 64];
     __shared__ JobDev sJ;
     __shared__ uint32_t sp[kWalkMaxJobs + 1];
@@ -99,7 +116,9 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         }
         __syncthreads();
     };
-    walk_tiles_strided(L, L.tile_prefix, L.total_tiles, &sJ, sp, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t) {
+    constexpr uint32_t kRun = 4;  // tiles per run: their occupancy loads are issued together
+    walk_tile_runs_strided<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, sp, [&](const JobDev& J, uint32_t j, uint32_t first,
+                                                                              uint32_t nrun) {
         if (!J.active || J.algo != VF_NAIVE || J.naive_key64 != kWant || J.S == 0) return;
         const uint32_t S = J.S;
         const bool one_chunk = S <= kSeedChunk;
@@ -107,9 +126,15 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
             stage(J, 0, S);
             staged = j;
         }
+        uint64_t occs[kRun];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(first + r) * kTileCells + tid * kCellsPerThread;
+            occs[r] = r < nrun && c0 < J.V ? load8b(J.occ, c0, J.V) : 0ull;
+        }
+        auto tile = [&](uint32_t local, uint64_t occ) {
         const uint64_t t0 = (uint64_t)local * kTileCells;
         const uint64_t c0 = t0 + tid * kCellsPerThread;
-        const uint64_t occ = c0 < J.V ? load8b(J.occ, c0, J.V) : 0ull;
         if (!__syncthreads_or(occ != 0ull)) {  // an EMPTY tile (most of a vessel's box): labels stay EMPTY
             if (c0 < J.V) {
                 const uint16_t zero[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
@@ -126,7 +151,7 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
             if (lane >= (uint32_t)o) inc += t;
         }
         if (lane == 63u) wsum[wave] = inc;
-        for (int k = 0; k < 8; ++k) lab[tid * 8 + k] = 0;  // EMPTY unless written below
+        lab4[tid] = make_uint4(0u, 0u, 0u, 0u);  // EMPTY unless written below
         __syncthreads();
         uint32_t pos = inc - n, total = 0;
         for (uint32_t w = 0; w < kCellThreads / 64; ++w) {
@@ -137,8 +162,11 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
             if ((occ >> (8 * k)) & 0xFFu) cells[pos++] = (uint16_t)(tid * 8 + k);
         __syncthreads();
         if (total) {
+            // thread t holds compacted cells t, t + 256, ...: a wave's first lane holds the most, so
+            // waves past the last cell skip the seed loop
             const uint32_t nq = tid < total ? (total - tid + kCellThreads - 1) / kCellThreads : 0u;
-            const uint32_t qmax = (total + kCellThreads - 1) / kCellThreads;  // block-uniform
+            const uint32_t tid0 = wave * 64u;
+            const uint32_t qmax = tid0 < total ? (total - tid0 + kCellThreads - 1) / kCellThreads : 0u;  // wave-uniform
             uint32_t cx[8], cy[8], cz[8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q) {
@@ -156,6 +184,7 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
                     stage(J, base, m);
                     staged = 0xFFFFFFFFu;
                 }
+                if (qmax == 0) continue;  // staging above stays block-wide
                 if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
                 else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
                 else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
@@ -170,11 +199,16 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         }
         __syncthreads();
         if (c0 < J.V) {
-            uint16_t out[8];
-            for (int k = 0; k < 8; ++k) out[k] = lab[tid * 8 + k];
+            const uint4 v = lab4[tid];
+            const uint16_t out[8] = { (uint16_t)v.x, (uint16_t)(v.x >> 16), (uint16_t)v.y, (uint16_t)(v.y >> 16),
+                                      (uint16_t)v.z, (uint16_t)(v.z >> 16), (uint16_t)v.w, (uint16_t)(v.w >> 16) };
             store8(J.labels, c0, J.V, out);
         }
         __syncthreads();  // lab and cells are reused by the next tile
+        };
+#pragma unroll 1
+        for (uint32_t r = 0; r < nrun; ++r)
+            tile(first + r, r == 0 ? occs[0] : (r == 1 ? occs[1] : (r == 2 ? occs[2] : occs[3])));
     });
 }
 
