@@ -127,8 +127,9 @@ __device__ inline void walk_tiles(const Launch& L, const uint32_t* prefix, uint3
 // The same walk in runs of up to N consecutive tiles of one job:
 // f(J, j, first_local_tile, n) handles tiles first .. first + n - 1 (n <= N),
 // so a streaming stage can issue all its loads before using any of them.
-template <uint32_t N, class F>
-__device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f)
+// leave(j) runs when the block is done with job j (as in walk_tiles).
+template <uint32_t N, class F, class G>
+__device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f, G&& leave)
 {
     uint32_t tb, te;
     block_tile_range(total, tb, te);
@@ -138,6 +139,7 @@ __device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, u
     stage_job(sJ, L.jobs + j);
     for (uint32_t t = tb; t < te;) {
         if (t >= je) {
+            leave(j);
             do {
                 ++j;
                 jb = je;
@@ -149,6 +151,13 @@ __device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, u
         f(*sJ, j, t - jb, n);
         t += n;
     }
+    leave(j);
+}
+
+template <uint32_t N, class F>
+__device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ, F&& f)
+{
+    walk_tile_runs<N>(L, prefix, total, sJ, f, [](uint32_t) {});
 }
 
 // Grid-stride variant for compute-heavy stages whose tiles differ in cost:

@@ -272,13 +272,11 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
 #pragma unroll
         for (int k = 0; k < 8; ++k) out[k] = pk_max_u16(pk_max_u16(p[k], p[k + 1]), p[k + 2]);
     };
-    walk_tiles(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t) {
+    constexpr uint32_t kRun = 4;  // bricks per step: their interior loads are issued together
+    walk_tile_runs<kRun>(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t first, uint32_t nrun) {
         if (!J.active || J.bsize > 1) return;
         const int nx = (int)J.nx, ny = (int)J.ny, nz = (int)J.nz;
         const uint32_t tz = ceil_div_d(J.nz, kBTZ), tyn = ceil_div_d(J.ny, kBTY);
-        const int x0 = (int)(local / (tz * tyn)) * kBTX;
-        const int y0 = (int)((local / tz) % tyn) * kBTY;
-        const int z0 = (int)(local % tz) * kBTZ;
         const bool bound = J.bsize == 1;
         const uint16_t* lab = J.labels;
         // ten raw values of row (gx, gy) at z = zs - 1 .. zs + 8, clamped to the grid
@@ -298,94 +296,147 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
                 for (int k = 0; k < 10; ++k) w[k] = r[min(max(zs - 1 + k, 0), nz - 1)];
             }
         };
-        // interior
-        const int gx = x0 + tx, gy = y0 + ty, gz = z0 + seg;
-        const bool row_ok = gx < nx && gy < ny;
-        const int nval = row_ok ? max(0, min(8, nz - gz)) : 0;
-        uint16_t w[10];
-        load_row(gx, gy, gz, w);
-        uint16_t v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = k < nval ? w[k + 1] : (uint16_t)0;
-        bool mine = false;
-        for (int k = 0; k < 8; ++k) mine |= v[k] > kFree;
-        const uint32_t c0 = (uint32_t)(((size_t)(row_ok ? gx : 0) * ny + (row_ok ? gy : 0)) * nz + gz);
-        auto add_checksum = [&]() {
-            if (CSUM)
-                if (v[0] | v[1] | v[2] | v[3] | v[4] | v[5] | v[6] | v[7])  // EMPTY cells add nothing
-                    for (int k = 0; k < nval; ++k) csum += checksum_term(v[k], c0 + k);
+        // brick `local`: its origin, this thread's row segment (gx, gy, gz) and how many of its 8 cells lie in the grid
+        auto segment = [&](uint32_t local, int& x0, int& y0, int& z0, int& gx, int& gy, int& gz, int& nval) {
+            x0 = (int)(local / (tz * tyn)) * kBTX;
+            y0 = (int)((local / tz) % tyn) * kBTY;
+            z0 = (int)(local % tz) * kBTZ;
+            gx = x0 + tx;
+            gy = y0 + ty;
+            gz = z0 + seg;
+            nval = gx < nx && gy < ny ? max(0, min(8, nz - gz)) : 0;
         };
-        if (!__syncthreads_or(mine)) {
-            add_checksum();
-            return;
+        auto add_checksum = [&](const uint16_t v[8], int gx, int gy, int gz, int nval) {
+            if (CSUM)
+                if (nval && (v[0] | v[1] | v[2] | v[3] | v[4] | v[5] | v[6] | v[7])) {  // EMPTY cells add nothing
+                    const uint32_t c0 = (uint32_t)(((size_t)gx * ny + gy) * nz + gz);
+                    for (int k = 0; k < nval; ++k) csum += checksum_term(v[k], c0 + k);
+                }
+        };
+        // the 8 in-grid cells of a segment as packed pairs (cells beyond the grid read 0)
+        auto load_seg = [&](int gx, int gy, int gz, int nval, uint32_t p[4]) {
+            p[0] = p[1] = p[2] = p[3] = 0u;
+            const uint16_t* r = lab + ((size_t)(nval ? gx : 0) * ny + (nval ? gy : 0)) * nz + gz;
+            if (nval == 8 && (nz & 3) == 0) {
+                const uint2 a = gload2(r), q = gload2(r + 4);
+                p[0] = a.x; p[1] = a.y; p[2] = q.x; p[3] = q.y;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k < nval) p[k >> 1] |= (uint32_t)r[k] << (16 * (k & 1));
+            }
+        };
+
+        // phase 1: the interior segments of the whole run, loaded together; bricks without a label end here
+        uint32_t live = 0;
+        {
+            uint32_t pr[kRun][4];
+#pragma unroll
+            for (uint32_t r = 0; r < kRun; ++r) {
+                int x0, y0, z0, gx, gy, gz, nval;
+                segment(first + r, x0, y0, z0, gx, gy, gz, nval);
+                load_seg(gx, gy, gz, r < nrun ? nval : 0, pr[r]);
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kRun; ++r) {
+                bool mine = false;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) mine |= (pr[r][k] & 0xFFFFu) > kFree || (pr[r][k] >> 16) > kFree;
+                live |= mine ? 1u << r : 0u;
+            }
+            live = (uint32_t)__ockl_wgred_or_i32((int)live);  // block-uniform: bit r = brick r holds a label
+            if (CSUM) {
+#pragma unroll
+                for (uint32_t r = 0; r < kRun; ++r) {
+                    if (r >= nrun || ((live >> r) & 1u)) continue;
+                    int x0, y0, z0, gx, gy, gz, nval;
+                    segment(first + r, x0, y0, z0, gx, gy, gz, nval);
+                    uint16_t v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = (uint16_t)(pr[r][k >> 1] >> (16 * (k & 1)));
+                    add_checksum(v, gx, gy, gz, nval);
+                }
+            }
         }
-        if (HIST) {
-            // run-length over the 8 cells: one LDS atomic per run of equal labels
-            uint32_t cur = 0, run = 0;
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t m = v[k] > kFree ? (uint32_t)(v[k] & kMask15) : 0xFFFFFFFFu;
-                if (m != cur && run) {
+
+        // phase 2: bricks with a label
+        for (uint32_t r = 0; r < nrun; ++r) {
+            if (!((live >> r) & 1u)) continue;
+            int x0, y0, z0, gx, gy, gz, nval;
+            segment(first + r, x0, y0, z0, gx, gy, gz, nval);
+            uint16_t w[10];
+            load_row(gx, gy, gz, w);  // the segment again (a cache hit) with its z neighbours
+            uint16_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = k < nval ? w[k + 1] : (uint16_t)0;
+            if (HIST) {
+                // run-length over the 8 cells: one LDS atomic per run of equal labels
+                uint32_t cur = 0, run = 0;
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t m = v[k] > kFree ? (uint32_t)(v[k] & kMask15) : 0xFFFFFFFFu;
+                    if (m != cur && run) {
+                        if (cur < kTiledBins) atomicAdd(&bins[cur], run);
+                        else atomicAdd(J.hist + cur, run);
+                        run = 0;
+                    }
+                    if (m != 0xFFFFFFFFu) {
+                        cur = m;
+                        ++run;
+                    }
+                }
+                if (run) {
                     if (cur < kTiledBins) atomicAdd(&bins[cur], run);
                     else atomicAdd(J.hist + cur, run);
-                    run = 0;
-                }
-                if (m != 0xFFFFFFFFu) {
-                    cur = m;
-                    ++run;
                 }
             }
-            if (run) {
-                if (cur < kTiledBins) atomicAdd(&bins[cur], run);
-                else atomicAdd(J.hist + cur, run);
+            if (!bound) {
+                add_checksum(v, gx, gy, gz, nval);
+                continue;
             }
-        }
-        if (!bound) {
-            add_checksum();
-            return;
-        }
-        {
-            uint32_t zw[8];
-            zwin(w, zw);
-            uint32_t* dst = zmm + ((tx + 1) * kBHY + ty + 1) * kAZ + seg;
+            {
+                uint32_t zw[8];
+                zwin(w, zw);
+                uint32_t* dst = zmm + ((tx + 1) * kBHY + ty + 1) * kAZ + seg;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) dst[k] = zw[k];
-        }
-        if (halo_task) {
-            uint16_t hw[10];
-            load_row(x0 + hhx - 1, y0 + hhy - 1, z0 + seg, hw);
-            uint32_t zw[8];
-            zwin(hw, zw);
-            uint32_t* dst = zmm + (hhx * kBHY + hhy) * kAZ + seg;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) dst[k] = zw[k];
-        }
-        __syncthreads();
-        // x and y windows together: a labelled cell without the bit is interior iff the packed
-        // window (max | 0xFFFF - min) equals its own packed value; cells beyond the grid are 0
-        bool changed = false;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            uint32_t acc = 0;
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy) acc = pk_max_u16(acc, zmm[((tx + dx) * kBHY + ty + dy) * kAZ + seg + k]);
-            const uint32_t c = v[k];
-            const bool bnd = c > kFree && (c & 0x8000u) == 0 && acc != packed_self(c);
-            v[k] = bnd ? (uint16_t)(c | 0x8000u) : (uint16_t)c;
-            changed |= bnd;
-        }
-        add_checksum();
-        if (changed) {
-            uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
-            if (nval == 8 && (nz & 3) == 0) {
-                gstore2(dst, make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16)));
-                gstore2(dst + 4, make_uint2((uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16)));
-            } else {
-                for (int k = 0; k < nval; ++k) dst[k] = v[k];
+                for (int k = 0; k < 8; ++k) dst[k] = zw[k];
             }
+            if (halo_task) {
+                uint16_t hw[10];
+                load_row(x0 + hhx - 1, y0 + hhy - 1, z0 + seg, hw);
+                uint32_t zw[8];
+                zwin(hw, zw);
+                uint32_t* dst = zmm + (hhx * kBHY + hhy) * kAZ + seg;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dst[k] = zw[k];
+            }
+            __syncthreads();
+            // x and y windows together: a labelled cell without the bit is interior iff the packed
+            // window (max | 0xFFFF - min) equals its own packed value; cells beyond the grid are 0
+            bool changed = false;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy) acc = pk_max_u16(acc, zmm[((tx + dx) * kBHY + ty + dy) * kAZ + seg + k]);
+                const uint32_t c = v[k];
+                const bool bnd = c > kFree && (c & 0x8000u) == 0 && acc != packed_self(c);
+                v[k] = bnd ? (uint16_t)(c | 0x8000u) : (uint16_t)c;
+                changed |= bnd;
+            }
+            add_checksum(v, gx, gy, gz, nval);
+            if (changed) {
+                uint16_t* dst = J.labels + ((size_t)gx * ny + gy) * nz + gz;
+                if (nval == 8 && (nz & 3) == 0) {
+                    gstore2(dst, make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16)));
+                    gstore2(dst + 4, make_uint2((uint32_t)v[4] | ((uint32_t)v[5] << 16), (uint32_t)v[6] | ((uint32_t)v[7] << 16)));
+                } else {
+                    for (int k = 0; k < nval; ++k) dst[k] = v[k];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }, flush);
 }
 
