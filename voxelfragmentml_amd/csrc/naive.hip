@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         }
         __syncthreads();
     };
-    constexpr uint32_t kRun = 4;  // tiles per run: their occupancy loads are issued together
+    constexpr uint32_t kRun = 8;  // tiles per run: their occupancy loads are issued together
     walk_tile_runs_strided<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, sp, [&](const JobDev& J, uint32_t j, uint32_t first,
                                                                               uint32_t nrun) {
         if (!J.active || J.algo != VF_NAIVE || J.naive_key64 != kWant || J.S == 0) return;
@@ -207,8 +207,12 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         __syncthreads();  // lab and cells are reused by the next tile
         };
 #pragma unroll 1
-        for (uint32_t r = 0; r < nrun; ++r)
-            tile(first + r, r == 0 ? occs[0] : (r == 1 ? occs[1] : (r == 2 ? occs[2] : occs[3])));
+        for (uint32_t r = 0; r < nrun; ++r) {
+            uint64_t occ = occs[0];  // occs[r] by selects (a dynamic index would put occs in scratch)
+#pragma unroll
+            for (uint32_t i = 1; i < kRun; ++i) occ = r == i ? occs[i] : occ;
+            tile(first + r, occ);
+        }
     });
 }
 

@@ -396,47 +396,60 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
     __shared__ JobDev sJ;
     __shared__ unsigned long long part[kVoxThreads / 64];
     unsigned long long acc = 0;
-    walk_tiles(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t group, uint32_t) {
+    constexpr uint32_t kRun = 4;  // column groups per step: all their bit loads are issued first
+    walk_tile_runs<kRun>(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t first, uint32_t nrun) {
         if (!J.active) return;
         const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
-        const uint64_t w = (uint64_t)group * kVoxThreads + threadIdx.x;
-        const bool valid = w < nwords;
-        const uint32_t tg = valid ? gcast<const uint32_t>(J.toggle_bits)[w] : 0u;
-        const uint32_t sh = valid ? gcast<const uint32_t>(J.shell_bits)[w] : 0u;
-        uint32_t p = tg;
-        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8; p ^= p << 16;
-        const uint64_t par = __ballot(__popc(tg) & 1);
-        const uint32_t lane = threadIdx.x & 63;
-        const uint32_t seg = lane & ~(J.wp - 1);
-        const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull) & ~(seg ? (~0ull >> (64 - seg)) : 0ull);
-        if (__popcll(par & below) & 1) p = ~p;
-        uint32_t bits = 0;
-        uint32_t nb = 0;
-        uint64_t base = 0;
-        if (valid) {
-            const uint64_t col = w / J.wp;
-            const uint32_t z0 = (uint32_t)(w % J.wp) * 32u;
-            if (z0 < J.nz) {
-                nb = min(32u, J.nz - z0);
-                bits = (sh | p) & (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u));
-                base = col * J.nz + z0;
-            }
-        }
-        if (nb) {
-            uint8_t* o = J.occ + base;
-            if ((J.nz & 15u) == 0 && nb == 32) {
-                gstore4(o, make_uint4(spread4(bits), spread4(bits >> 4), spread4(bits >> 8), spread4(bits >> 12)));
-                gstore4(o + 16, make_uint4(spread4(bits >> 16), spread4(bits >> 20), spread4(bits >> 24), spread4(bits >> 28)));
-            } else if ((J.nz & 7u) == 0 && nb == 32) {
+        uint32_t tgs[kRun], shs[kRun];
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) gstore2(o + 8 * q, make_uint2(spread4(bits >> (8 * q)), spread4(bits >> (8 * q + 4))));
-            } else if ((J.nz & 3u) == 0) {
-                for (uint32_t q = 0; q < nb / 4; ++q) gcast<uint32_t>(o)[q] = spread4(bits >> (4 * q));
-            } else {
-                for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
-            }
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t w = (uint64_t)(first + r) * kVoxThreads + threadIdx.x;
+            const bool valid = r < nrun && w < nwords;
+            tgs[r] = valid ? gcast<const uint32_t>(J.toggle_bits)[w] : 0u;
+            shs[r] = valid ? gcast<const uint32_t>(J.shell_bits)[w] : 0u;
         }
-        acc += (unsigned long long)__popc(bits);
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            if (r >= nrun) break;
+            const uint64_t w = (uint64_t)(first + r) * kVoxThreads + threadIdx.x;
+            const bool valid = w < nwords;
+            const uint32_t tg = tgs[r];
+            const uint32_t sh = shs[r];
+            uint32_t p = tg;
+            p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8; p ^= p << 16;
+            const uint64_t par = __ballot(__popc(tg) & 1);
+            const uint32_t lane = threadIdx.x & 63;
+            const uint32_t seg = lane & ~(J.wp - 1);
+            const uint64_t below = (lane ? (~0ull >> (64 - lane)) : 0ull) & ~(seg ? (~0ull >> (64 - seg)) : 0ull);
+            if (__popcll(par & below) & 1) p = ~p;
+            uint32_t bits = 0;
+            uint32_t nb = 0;
+            uint64_t base = 0;
+            if (valid) {
+                const uint64_t col = w / J.wp;
+                const uint32_t z0 = (uint32_t)(w % J.wp) * 32u;
+                if (z0 < J.nz) {
+                    nb = min(32u, J.nz - z0);
+                    bits = (sh | p) & (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u));
+                    base = col * J.nz + z0;
+                }
+            }
+            if (nb) {
+                uint8_t* o = J.occ + base;
+                if ((J.nz & 15u) == 0 && nb == 32) {
+                    gstore4(o, make_uint4(spread4(bits), spread4(bits >> 4), spread4(bits >> 8), spread4(bits >> 12)));
+                    gstore4(o + 16, make_uint4(spread4(bits >> 16), spread4(bits >> 20), spread4(bits >> 24), spread4(bits >> 28)));
+                } else if ((J.nz & 7u) == 0 && nb == 32) {
+    #pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q) gstore2(o + 8 * q, make_uint2(spread4(bits >> (8 * q)), spread4(bits >> (8 * q + 4))));
+                } else if ((J.nz & 3u) == 0) {
+                    for (uint32_t q = 0; q < nb / 4; ++q) gcast<uint32_t>(o)[q] = spread4(bits >> (4 * q));
+                } else {
+                    for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
+                }
+            }
+            acc += (unsigned long long)__popc(bits);
+        }
     }, [&](uint32_t j) {
         const unsigned long long s = wave_sum_u64(acc);
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
