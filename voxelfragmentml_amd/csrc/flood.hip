@@ -323,20 +323,27 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         }
     }
 
-    // neighbour bricks that see a dropped cell: bit (dx+1)*9 + (dy+1)*3 + (dz+1)
+    // neighbour bricks that see a dropped cell: bit (dx+1)*9 + (dy+1)*3 + (dz+1).  Lane x*8 + y holds row
+    // (x, y): a brick (dx, dy, dz) sees a dropped cell of a row with x = 0 (dx = -1) or 7 (dx = +1) or any
+    // x (dx = 0), likewise for y, and a dropped end cell z = 0 (dz = -1) or 7 (dz = +1) or any cell
+    // (dz = 0); the 6-neighbour stencil sees faces only.  Built from three ballots with lane masks.
+    const uint64_t bzm[3] = { __ballot(changed && zlo), __ballot(changed), __ballot(changed && zhi) };
+    constexpr uint64_t kAll = ~0ull, kX0 = 0xFFull, kX7 = 0xFFull << 56, kY0 = 0x0101010101010101ull,
+                       kY7 = 0x8080808080808080ull;
+    const uint64_t mxs[3] = { kX0, kAll, kX7 }, mys[3] = { kY0, kAll, kY7 };
     uint32_t dirs = 0;
-    if (changed) {
-        const uint32_t sx = x == 0 ? 0u : (x == 7 ? 2u : 1u), sy = y == 0 ? 0u : (y == 7 ? 2u : 1u);
-        if (R.n26) {
-            const uint32_t mz = 2u | zlo | (zhi << 2);  // bit iz for iz in {1, 0 if z = 0 dropped, 2 if z = 7 dropped}
-            dirs = (mz << 12) | (mz << (sx * 9u + 3u)) | (mz << (9u + sy * 3u)) | (mz << (sx * 9u + sy * 3u));
-            dirs &= ~(1u << 13);  // the brick itself
-        } else {
-            dirs = (sx != 1u ? 1u << (sx * 9u + 4u) : 0u) | (sy != 1u ? 1u << (10u + sy * 3u) : 0u) | (zlo << 12) | (zhi << 14);
-        }
-    }
+    if (R.n26) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) dirs |= __shfl_xor(dirs, o, 64);
+        for (uint32_t ix = 0; ix < 3; ++ix)
+#pragma unroll
+            for (uint32_t iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (uint32_t iz = 0; iz < 3; ++iz)
+                    if (ix * 9 + iy * 3 + iz != 13 && (bzm[iz] & mxs[ix] & mys[iy])) dirs |= 1u << (ix * 9 + iy * 3 + iz);
+    } else {
+        dirs = ((bzm[1] & kX0) ? 1u << 4 : 0u) | ((bzm[1] & kX7) ? 1u << 22 : 0u) | ((bzm[1] & kY0) ? 1u << 10 : 0u) |
+               ((bzm[1] & kY7) ? 1u << 16 : 0u) | (bzm[0] ? 1u << 12 : 0u) | (bzm[2] ? 1u << 14 : 0u);
+    }
     if (lane < 27u && (dirs & (1u << lane))) {
         const int qx = (int)bx + (int)(lane / 9u) - 1, qy = (int)by + (int)((lane / 3u) % 3u) - 1,
                   qz = (int)bz + (int)(lane % 3u) - 1;
