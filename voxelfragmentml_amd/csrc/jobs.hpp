@@ -124,7 +124,7 @@ struct Launch {
 
 // voxelize.hip
 struct VoxScratch {
-    TriSetup* setup;                   // total triangles
+    TriSetup* setup;                   // total triangles, as 8 planes of 16-byte pieces (store_setup / load_setup)
     unsigned long long* shell_off;     // total triangles + 1 exclusive prefix of the shell work counts
     unsigned long long* col_off;
     uint32_t* counts;                  // 2*total triangles: shell work counts (column segments of one z word

@@ -125,6 +125,29 @@ __device__ inline int32_t snap_coord(float q)
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// TriSetup records are stored as 8 planes of 16-byte pieces: piece k of record g at plane k, index g.
+// A wave's stores of 64 consecutive records are then 8 contiguous 1 KiB runs instead of 16-byte pieces
+// 128 bytes apart; readers fetch the pieces they use.
+static_assert(sizeof(TriSetup) == 8 * sizeof(uint4), "TriSetup is eight 16-byte pieces");
+__device__ inline void store_setup(TriSetup* base, uint32_t total, uint32_t g, const TriSetup& S)
+{
+    uint4 piece[8];
+    __builtin_memcpy(piece, &S, sizeof(TriSetup));
+    uint4* dst = reinterpret_cast<uint4*>(base);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) gstore4(dst + (size_t)k * total + g, piece[k]);
+}
+__device__ inline TriSetup load_setup(const TriSetup* base, uint32_t total, uint32_t g)
+{
+    uint4 piece[8];
+    const uint4* src = reinterpret_cast<const uint4*>(base);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) piece[k] = src[(size_t)k * total + g];
+    TriSetup S;
+    __builtin_memcpy(&S, piece, sizeof(TriSetup));
+    return S;
+}
+
 __device__ inline bool edge_incl(int64_t e, int64_t dx, int64_t dy)
 {
     return e > 0 || (e == 0 && (dy < 0 || (dy == 0 && dx < 0)));
@@ -278,7 +301,7 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                 for (int jj = S.j0; jj <= S.j1; ++jj) parity_column(J, S, i, jj);
             ncol = 0;
         }
-        setup[g] = S;
+        store_setup(setup, total, g, S);
         cnt_shell[g] = nshell;
         cnt_col[g] = ncol;
     }
@@ -355,7 +378,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_shell(const JobDev* jobs, c
 {
     for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
         if (!valid) return;
-        const TriSetup* S = setup + g;
+        const TriSetup Sg = load_setup(setup, ntris, g);
+        const TriSetup* S = &Sg;
         const JobDev* J = jobs + S->job;
         const int lo2 = S->lo[2], hi2 = S->hi[2];
         const uint32_t w0 = (uint32_t)lo2 >> 5, nseg = ((uint32_t)hi2 >> 5) - w0 + 1;
@@ -381,7 +405,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_parity(const JobDev* jobs, 
 {
     for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
         if (!valid) return;
-        const TriSetup* S = setup + g;
+        const TriSetup Sg = load_setup(setup, ntris, g);
+        const TriSetup* S = &Sg;
         const JobDev* J = jobs + S->job;
         const uint32_t nj = (uint32_t)(S->j1 - S->j0 + 1);
         parity_column(J, *S, S->i0 + (int)(local / nj), S->j0 + (int)(local % nj));
