@@ -503,6 +503,11 @@ __global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
     }
 }
 
+// Cells of thread t in a tile of 2048 for the 4-byte-per-cell kernels: two quads, q = 0 / 1 at
+// 512 * wave + 256 * q + 4 * lane, so that every 16-byte access of a wave is one contiguous kilobyte
+// (8 consecutive cells per thread would put a wave's 16-byte accesses 32 bytes apart).
+__device__ inline uint32_t quad_cell(uint32_t q) { return 512u * (threadIdx.x >> 6) + 256u * q + 4u * (threadIdx.x & 63u); }
+
 __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
 {
     constexpr uint32_t kRun = 4;  // tiles per step: all occupancy loads are issued before any key is written
@@ -510,31 +515,42 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
     walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t n) {
         if (!J.active || J.algo != VF_FLOOD) return;
         const uint64_t V = J.V;
-        uint64_t o[kRun];
+        uint32_t o[kRun][2];  // occupancy bytes of the two quads
 #pragma unroll
-        for (uint32_t r = 0; r < kRun; ++r) {
-            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
-            o[r] = r < n && c0 < V ? load8b(J.occ, c0, V) : 0ull;
-        }
+        for (uint32_t r = 0; r < kRun; ++r)
 #pragma unroll
-        for (uint32_t r = 0; r < kRun; ++r) {
-            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
-            if (r >= n || c0 >= V) break;
-            uint32_t key[8] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
-            if (o[r]) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    key[k] = ((o[r] >> (8 * k)) & 0xFFu)
-                                 ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c0 + k), J.noise_amp))
-                                 : kEmptyKey;
+            for (uint32_t q = 0; q < 2; ++q) {
+                const uint64_t c = (uint64_t)(local + r) * kTileCells + quad_cell(q);
+                uint32_t v = 0;
+                if (r < n && c + 4 <= V) {
+                    v = *gcast<const uint32_t>(J.occ + c);
+                } else if (r < n) {
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (c + k < V) v |= (uint32_t)gcast<const uint8_t>(J.occ)[c + k] << (8 * k);
                 }
+                o[r][q] = v;
             }
-            if (c0 + 8 <= V) {
-                gstore4(J.keys + c0, make_uint4(key[0], key[1], key[2], key[3]));
-                gstore4(J.keys + c0 + 4, make_uint4(key[4], key[5], key[6], key[7]));
-            } else {
-                for (int k = 0; k < 8; ++k)
-                    if (c0 + k < V) gcast<uint32_t>(J.keys)[c0 + k] = key[k];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            if (r >= n) break;
+#pragma unroll
+            for (uint32_t q = 0; q < 2; ++q) {
+                const uint64_t c = (uint64_t)(local + r) * kTileCells + quad_cell(q);
+                if (c >= V) continue;
+                uint32_t key[4] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
+                if (o[r][q]) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        key[k] = ((o[r][q] >> (8 * k)) & 0xFFu)
+                                     ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c + k), J.noise_amp))
+                                     : kEmptyKey;
+                }
+                if (c + 4 <= V) {
+                    gstore4(J.keys + c, make_uint4(key[0], key[1], key[2], key[3]));
+                } else {
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (c + k < V) gcast<uint32_t>(J.keys)[c + k] = key[k];
+                }
             }
         }
     });
