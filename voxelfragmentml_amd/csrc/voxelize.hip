@@ -28,6 +28,8 @@ constexpr uint32_t kVoxThreads = 256;
 constexpr float kRangeMargin = 0.015625f;   // 1/64 cell
 constexpr uint32_t kInlineShell = 8;        // SAT candidates a setup thread tests itself
 constexpr uint32_t kInlineCols = 8;         // parity columns a setup thread toggles itself
+constexpr uint32_t kColBits = 4;            // column index bits in the setup kernel's owner tables
+static_assert(kInlineShell <= (1u << kColBits) && kInlineCols <= (1u << kColBits), "owner table encoding");
 
 __device__ inline bool axis_sep(float pa, float pb, float rad)
 {
@@ -194,18 +196,34 @@ __device__ inline void parity_column(const JobDev* J, const TriSetup& S, int i, 
 __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, uint32_t njobs, const uint32_t* tri_prefix,
                                                            uint32_t total, TriSetup* setup, uint32_t* cnt_shell, uint32_t* cnt_col)
 {
+    // Small triangles' candidate columns (shell) and covered columns (parity) are tested in this kernel,
+    // shared by the lanes of a wave: each lane stashes its triangle in the wave's slots and lists its
+    // columns as (lane << kColBits | column) in an owner table; the wave then takes the listed columns 64 at a
+    // time, instead of every lane looping over its own (as many steps as its largest triangle).
+    constexpr uint32_t kWaves = kVoxThreads / 64;
+    // 16-bit pairs hold cell indices (< 2048) and extents: lo0 | lo1 << 16, lo2 | hi2 << 16, ey | nj << 16, i0 | j0 << 16
+    enum { kLo01, kLo2Hi2, kEyNj, kI0J0, kJob, kX, kY = kX + 3, kZ = kY + 3, kAreaLo = kZ + 3, kAreaHi, kFields };
     __shared__ uint32_t sp[kWalkMaxJobs + 1];
+    __shared__ float sP[kWaves][9][64];
+    __shared__ int32_t sF[kWaves][kFields][64];
+    __shared__ uint16_t sOwnS[kWaves][64 * kInlineShell], sOwnP[kWaves][64 * kInlineCols];
     for (uint32_t i = threadIdx.x; i <= njobs; i += blockDim.x) sp[i] = tri_prefix[i];
     __syncthreads();
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        const uint32_t j = find_job(sp, njobs, g);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < total; g0 += gridDim.x * blockDim.x) {  // block-uniform
+        const uint32_t g = g0 + threadIdx.x;
+        const bool valid = g < total;
+        const uint32_t j = valid ? find_job(sp, njobs, g) : 0u;
         const JobDev* J = jobs + j;
         TriSetup S;
         S.job = j;
         S.area = 0;
+        S.i0 = S.j0 = 0;
+        S.i1 = S.j1 = -1;
+        for (int q = 0; q < 3; ++q) S.lo[q] = S.hi[q] = S.X[q] = S.Y[q] = S.Z[q] = 0;
         uint32_t nshell = 0, ncol = 0;
         const uint32_t t = g - sp[j];
-        if (J->active) {
+        if (valid && J->active) {
             const float* p[3] = { J->xyz + 3 * (size_t)J->tri[3 * (size_t)t], J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 1],
                                   J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 2] };
             float P[3][3];
@@ -277,33 +295,94 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
                 }
             }
         }
-        // small triangles: test the candidates here; only large ones go to the pair kernels, as
-        // segments of a column (x, y) that lie in one 32-cell word of z
+        // small triangles are tested here; the others go to the pair kernels, the shell as segments of
+        // a column (x, y) that lie in one 32-cell word of z
+        uint32_t nis = 0, nip = 0;  // this lane's inline shell columns and parity columns
         if (nshell && nshell <= kInlineShell) {
-            for (int x = S.lo[0]; x <= S.hi[0]; ++x)
-                for (int y = S.lo[1]; y <= S.hi[1]; ++y) {
-                    float cx, hx, cy, hy;
-                    cell_box(x, J->mn[0], J->cs[0], &cx, &hx);
-                    cell_box(y, J->mn[1], J->cs[1], &cy, &hy);
-                    for (int w = S.lo[2] >> 5; w <= S.hi[2] >> 5; ++w) {
-                        const uint32_t bits = sat_column(S.P[0], S.P[1], S.P[2], cx, hx, cy, hy, max(S.lo[2], 32 * w),
-                                                         min(S.hi[2], 32 * w + 31), J->mn[2], J->cs[2]);
-                        if (bits) atomicOr(J->shell_bits + ((size_t)x * J->ny + (size_t)y) * J->wp + (size_t)w, bits);
-                    }
-                }
+            nis = (uint32_t)(S.hi[0] - S.lo[0] + 1) * (uint32_t)(S.hi[1] - S.lo[1] + 1);
             nshell = 0;
         } else if (nshell) {
             nshell = (uint32_t)(S.hi[0] - S.lo[0] + 1) * (uint32_t)(S.hi[1] - S.lo[1] + 1) *
                      (uint32_t)((S.hi[2] >> 5) - (S.lo[2] >> 5) + 1);
         }
         if (ncol && ncol <= kInlineCols) {
-            for (int i = S.i0; i <= S.i1; ++i)
-                for (int jj = S.j0; jj <= S.j1; ++jj) parity_column(J, S, i, jj);
+            nip = ncol;
             ncol = 0;
         }
-        store_setup(setup, total, g, S);
-        cnt_shell[g] = nshell;
-        cnt_col[g] = ncol;
+        if (valid) {
+            store_setup(setup, total, g, S);
+            cnt_shell[g] = nshell;
+            cnt_col[g] = ncol;
+        }
+        // stash the triangle and list its columns
+        if (nis | nip) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) sP[wave][3 * k + q][lane] = S.P[k][q];
+            sF[wave][kLo01][lane] = S.lo[0] | S.lo[1] << 16;
+            sF[wave][kLo2Hi2][lane] = S.lo[2] | S.hi[2] << 16;
+            sF[wave][kEyNj][lane] = (S.hi[1] - S.lo[1] + 1) | (S.j1 - S.j0 + 1) << 16;
+            sF[wave][kI0J0][lane] = S.i0 | S.j0 << 16;
+            sF[wave][kJob][lane] = (int32_t)j;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sF[wave][kX + k][lane] = S.X[k];
+                sF[wave][kY + k][lane] = S.Y[k];
+                sF[wave][kZ + k][lane] = S.Z[k];
+            }
+            sF[wave][kAreaLo][lane] = (int32_t)(uint32_t)S.area;
+            sF[wave][kAreaHi][lane] = (int32_t)(uint32_t)((uint64_t)S.area >> 32);
+        }
+        uint32_t es = nis, ep = nip;  // wave inclusive scans
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t us = __shfl_up(es, o, 64), up = __shfl_up(ep, o, 64);
+            if (lane >= (uint32_t)o) {
+                es += us;
+                ep += up;
+            }
+        }
+        for (uint32_t k = 0; k < nis; ++k) sOwnS[wave][es - nis + k] = (uint16_t)(lane << kColBits | k);
+        for (uint32_t k = 0; k < nip; ++k) sOwnP[wave][ep - nip + k] = (uint16_t)(lane << kColBits | k);
+        const uint32_t ts = __shfl(es, 63, 64), tp = __shfl(ep, 63, 64);
+        __syncthreads();
+        for (uint32_t it = lane; it < ts; it += 64) {
+            const uint32_t e = sOwnS[wave][it], o = e >> kColBits, c = e & ((1u << kColBits) - 1u);
+            const int ey = sF[wave][kEyNj][o] & 0xFFFF, lo01 = sF[wave][kLo01][o], z2 = sF[wave][kLo2Hi2][o];
+            const int x = (lo01 & 0xFFFF) + (int)c / ey, y = (lo01 >> 16) + (int)c % ey;
+            const int lo2 = z2 & 0xFFFF, hi2 = z2 >> 16;
+            const JobDev* Jo = jobs + sF[wave][kJob][o];
+            float pa[3], pb[3], pc[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                pa[q] = sP[wave][q][o];
+                pb[q] = sP[wave][3 + q][o];
+                pc[q] = sP[wave][6 + q][o];
+            }
+            float cx, hx, cy, hy;
+            cell_box(x, Jo->mn[0], Jo->cs[0], &cx, &hx);
+            cell_box(y, Jo->mn[1], Jo->cs[1], &cy, &hy);
+            for (int w = lo2 >> 5; w <= hi2 >> 5; ++w) {
+                const uint32_t bits = sat_column(pa, pb, pc, cx, hx, cy, hy, max(lo2, 32 * w), min(hi2, 32 * w + 31),
+                                                 Jo->mn[2], Jo->cs[2]);
+                if (bits) atomicOr(Jo->shell_bits + ((size_t)x * Jo->ny + (size_t)y) * Jo->wp + (size_t)w, bits);
+            }
+        }
+        for (uint32_t it = lane; it < tp; it += 64) {
+            const uint32_t e = sOwnP[wave][it], o = e >> kColBits, c = e & ((1u << kColBits) - 1u);
+            TriSetup Q;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Q.X[k] = sF[wave][kX + k][o];
+                Q.Y[k] = sF[wave][kY + k][o];
+                Q.Z[k] = sF[wave][kZ + k][o];
+            }
+            Q.area = (int64_t)(((uint64_t)(uint32_t)sF[wave][kAreaHi][o] << 32) | (uint32_t)sF[wave][kAreaLo][o]);
+            const int nj = sF[wave][kEyNj][o] >> 16, ij = sF[wave][kI0J0][o];
+            parity_column(jobs + sF[wave][kJob][o], Q, (ij & 0xFFFF) + (int)c / nj, (ij >> 16) + (int)c % nj);
+        }
+        __syncthreads();  // the slots and owner tables are reused by the next 256 triangles
     }
 }
 
