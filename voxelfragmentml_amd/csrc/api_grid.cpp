@@ -328,7 +328,7 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
         X.total_tris = nt;
         X.scan_tmp = g->scan_tmp.p;
         X.scan_tmp_bytes = g->scan_tmp.bytes;
-        launch_voxelize(L, X, g->ctl.as<uint32_t>() + 2, groups);
+        launch_voxelize(L, X, g->ctl.as<uint32_t>() + 2, groups, nullptr);
         (void)nv;
         if (merge) {
             // labels = new ? FREE : old, occupancy = old | new
@@ -390,7 +390,7 @@ int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed
         g->seed_tile_off.ensure(((size_t)tiles + 1) * 8 + 64);
         g->scan_tmp.ensure(scan_tmp_bytes(tiles) + 64);
         Launch L = grid_launch(g);
-        launch_seed_count(L, g->seed_tile_cnt.as<uint32_t>());
+        launch_seed_count(L, g->seed_tile_cnt.as<uint32_t>(), false);
         scan_exclusive_u32_to_u64(g->seed_tile_cnt.as<uint32_t>(), g->seed_tile_off.as<unsigned long long>(), tiles,
                                   g->scan_tmp.p, g->stream);
         launch_seed_pick(L, g->seed_tile_off.as<unsigned long long>());

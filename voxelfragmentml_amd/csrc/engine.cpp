@@ -288,8 +288,8 @@ void Engine::front(Table& t)
     X.total_tris = nt;
     X.scan_tmp = scan_tmp_.p;
     X.scan_tmp_bytes = scan_tmp_.bytes;
-    launch_voxelize(L, X, t.col_prefix, t.total_groups);
-    launch_seed_count(L, seed_cnt_.as<uint32_t>());
+    launch_voxelize(L, X, t.col_prefix, t.total_groups, seed_cnt_.as<uint32_t>());  // counts tiles of VF_SEED_BOTH jobs
+    launch_seed_count(L, seed_cnt_.as<uint32_t>(), true);
     scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
     launch_seed_pick(L, seed_off_.as<unsigned long long>());
     if (t.any_naive32) launch_naive(L, false);

@@ -135,8 +135,11 @@ struct VoxScratch {
     size_t scan_tmp_bytes;
 };
 size_t vox_scan_tmp_bytes(uint32_t total_tris);
-void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_col_groups);
-void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_col_groups);
+// tile_cnt (or null): also count the occupied cells per tile (L.tile_prefix) of jobs that seed anywhere
+// (VF_SEED_BOTH), as launch_seed_count does; launch_seed_count(..., true) then skips those jobs
+void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_col_groups,
+                     uint32_t* tile_cnt);
+void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_col_groups, uint32_t* tile_cnt);
 
 // grid_ops.hip
 void launch_occ_to_labels(const Launch& L);
@@ -221,7 +224,7 @@ void launch_flood_sub(const Launch& L);
 
 // seed.hip
 constexpr uint32_t kMaxDeviceSeeds = 4096;
-void launch_seed_count(const Launch& L, uint32_t* tile_cnt);
+void launch_seed_count(const Launch& L, uint32_t* tile_cnt, bool skip_both);
 // tile_off: exclusive prefix over the global tiles (total_tiles + 1 entries)
 void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
 // Seeder::nearSeeds on job 0 of L; out: F + num_seeds entries; result[2]: count, draw-cap flag

@@ -36,12 +36,14 @@ __device__ inline bool candidate(const JobDev* J, uint64_t c, uint32_t x, uint32
     return (J->seed_location == VF_SEED_OUTER) == s;
 }
 
-__global__ __launch_bounds__(kCellThreads) void k_seed_count(Launch L, uint32_t* tile_cnt)
+// skip_both: jobs that seed anywhere were counted by the voxelizer's fill
+__global__ __launch_bounds__(kCellThreads) void k_seed_count(Launch L, uint32_t* tile_cnt, bool skip_both)
 {
     constexpr uint32_t kRun = 4;  // tiles per step: loads first, one block reduction for all of them
     __shared__ uint32_t wsum[kRun][kCellThreads / 64];
     __shared__ JobDev sJ;
     walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t j, uint32_t local, uint32_t nrun) {
+        if (skip_both && J.seed_location == VF_SEED_BOTH) return;  // block-uniform
         uint32_t n[kRun];
         uint64_t o[kRun];
 #pragma unroll
@@ -340,11 +342,11 @@ void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t 
     VF_CHECK_LAUNCH();
 }
 
-void launch_seed_count(const Launch& L, uint32_t* tile_cnt)
+void launch_seed_count(const Launch& L, uint32_t* tile_cnt, bool skip_both)
 {
     TimedLaunch tl("seed_count", L.s);
     const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
-    k_seed_count<<<blocks, kCellThreads, 0, L.s>>>(L, tile_cnt);
+    k_seed_count<<<blocks, kCellThreads, 0, L.s>>>(L, tile_cnt, skip_both);
     VF_CHECK_LAUNCH();
 }
 

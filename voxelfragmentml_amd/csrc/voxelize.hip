@@ -495,13 +495,17 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_parity(const JobDev* jobs, 
 // 4 bits -> 4 bytes of 0/1 (bit i -> byte i)
 __device__ inline uint32_t spread4(uint32_t b) { return ((b & 0xFu) * 0x00204081u) & 0x01010101u; }
 
-__global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
+// tile_cnt (or null): per 2048-cell tile (indexed by L.tile_prefix) occupied-cell counts for jobs that seed
+// anywhere, added here from the occupancy bits (zeroed by k_vox_clear); a wave's 64 words are whole
+// columns, at most 2048 consecutive cells, so they touch at most two tiles
+__global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32_t* col_prefix, uint32_t total_groups,
+                                                          uint32_t* tile_cnt)
 {
     __shared__ JobDev sJ;
     __shared__ unsigned long long part[kVoxThreads / 64];
     unsigned long long acc = 0;
     constexpr uint32_t kRun = 4;  // column groups per step: all their bit loads are issued first
-    walk_tile_runs<kRun>(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t first, uint32_t nrun) {
+    walk_tile_runs<kRun>(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t jidx, uint32_t first, uint32_t nrun) {
         if (!J.active) return;
         const uint64_t nwords = (uint64_t)J.nx * J.ny * J.wp;
         uint32_t tgs[kRun], shs[kRun];
@@ -553,6 +557,23 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
                 }
             }
             acc += (unsigned long long)__popc(bits);
+            if (tile_cnt && J.seed_location == VF_SEED_BOTH) {
+                const uint64_t b0 = __builtin_amdgcn_readfirstlane((uint32_t)base) |
+                                    ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32);
+                const uint64_t ta = b0 >> 11;  // the wave's first tile (lane 0 holds a column start)
+                const uint64_t split = (ta + 1) << 11;  // first cell of the next tile
+                uint32_t na = 0;
+                if (nb) {
+                    const uint32_t below = base >= split ? 0u : (uint32_t)min<uint64_t>(split - base, 32);
+                    na = __popc(bits & (below == 32 ? 0xFFFFFFFFu : ((1u << below) - 1u)));
+                }
+                const uint32_t sa = (uint32_t)wave_sum_u64(na), sb = (uint32_t)wave_sum_u64(__popc(bits) - na);
+                if ((threadIdx.x & 63u) == 0) {
+                    const uint32_t t0 = L.tile_prefix[jidx];
+                    if (sa) atomicAdd(tile_cnt + t0 + (uint32_t)ta, sa);
+                    if (sb) atomicAdd(tile_cnt + t0 + (uint32_t)ta + 1u, sb);
+                }
+            }
         }
     }, [&](uint32_t j) {
         const unsigned long long s = wave_sum_u64(acc);
@@ -568,8 +589,11 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
     });
 }
 
-__global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint32_t* col_prefix, uint32_t total_groups)
+__global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint32_t* col_prefix, uint32_t total_groups,
+                                                           uint32_t* tile_cnt)
 {
+    if (tile_cnt)
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < L.total_tiles; i += gridDim.x * blockDim.x) tile_cnt[i] = 0u;
     __shared__ JobDev sJ;
     walk_tiles(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t, uint32_t group, uint32_t) {
         if (!J.active) return;
@@ -587,19 +611,20 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint3
 
 size_t vox_scan_tmp_bytes(uint32_t total_tris) { return scan_tmp_bytes(total_tris); }
 
-void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_groups)
+void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_groups, uint32_t* tile_cnt)
 {
     TimedLaunch tl("vox_clear", L.s);
     const uint32_t blocks = std::max(1u, std::min(total_groups, active_shape().cell_blocks));
-    k_vox_clear<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
+    k_vox_clear<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups, tile_cnt);
     VF_CHECK_LAUNCH();
 }
 
 // X.setup / X.counts / X.shell_off / X.col_off / X.scan_tmp sized for X.total_tris.
-void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_groups)
+void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_groups,
+                     uint32_t* tile_cnt)
 {
     if (L.njobs > kWalkMaxJobs) fail(VF_ERR_INVALID_ARG, "voxelize launch with %u jobs (max %u)", L.njobs, kWalkMaxJobs);
-    launch_clear_bits(L, col_prefix, total_groups);
+    launch_clear_bits(L, col_prefix, total_groups, tile_cnt);
     if (X.total_tris) {
         uint32_t* cnt_shell = X.counts;
         uint32_t* cnt_col = cnt_shell + X.total_tris;
@@ -625,7 +650,7 @@ void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_p
     }
     TimedLaunch tl("vox_fill", L.s);
     const uint32_t blocks = std::max(1u, std::min(total_groups, active_shape().cell_blocks));
-    k_vox_fill<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups);
+    k_vox_fill<<<blocks, kVoxThreads, 0, L.s>>>(L, col_prefix, total_groups, tile_cnt);
     VF_CHECK_LAUNCH();
 }
 
