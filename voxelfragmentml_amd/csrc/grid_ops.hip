@@ -474,17 +474,33 @@ __global__ __launch_bounds__(kCellThreads) void k_count_occupied(Launch L)
 }
 
 // One block per job: number of labels >= 2 with a nonzero count.
+// kStatLabels: labels >= 2 with a nonzero count.  kSummaryParts blocks per job, each over 65536 /
+// kSummaryParts bins with 16-byte loads issued together, add their counts (the slot is zeroed by job_init).
+constexpr uint32_t kSummaryParts = 8;
 __global__ __launch_bounds__(256) void k_hist_summary(Launch L)
 {
+    constexpr uint32_t kBins = 65536 / kSummaryParts, kIters = kBins / 4 / 256;
     __shared__ uint32_t wsum[4];
-    const JobDev* J = L.jobs + blockIdx.x;
+    const JobDev* J = L.jobs + blockIdx.x / kSummaryParts;
     if (!J->active) return;
+    const uint32_t part = blockIdx.x % kSummaryParts;
+    const uint4* h = reinterpret_cast<const uint4*>(J->hist + part * kBins);
+    uint4 q[kIters];
+#pragma unroll
+    for (uint32_t i = 0; i < kIters; ++i) q[i] = gload4(h + i * 256 + threadIdx.x);
     uint32_t n = 0;
-    for (uint32_t i = 2 + threadIdx.x; i < 65536; i += blockDim.x) n += J->hist[i] != 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kIters; ++i) {
+        const bool first = part == 0 && i == 0 && threadIdx.x == 0;  // bins 0 (EMPTY) and 1 (FREE) are not labels
+        n += (q[i].x != 0 && !first) + (q[i].y != 0 && !first) + (q[i].z != 0) + (q[i].w != 0);
+    }
     for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
     __syncthreads();
-    if (threadIdx.x == 0) J->stats[kStatLabels] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (threadIdx.x == 0) {
+        const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(J->stats + kStatLabels, (unsigned long long)t);
+    }
 }
 
 // Block per job: zero the per-job statistics, histogram and flood control.
@@ -617,7 +633,7 @@ void launch_gather(const Launch& L, uint32_t* out)
 void launch_hist_summary(const Launch& L)
 {
     TimedLaunch tl("hist_summary", L.s);
-    k_hist_summary<<<L.njobs, 256, 0, L.s>>>(L);
+    k_hist_summary<<<L.njobs * kSummaryParts, 256, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

@@ -164,7 +164,7 @@ void launch_job_init(const Launch& L);     // zero stats, histogram and flood co
 void launch_gather(const Launch& L, uint32_t* out);  // 16 words per job: stats (low 32 bits) + flood state
 void launch_checksum(const Launch& L);
 void launch_count_occupied(const Launch& L);
-void launch_hist_summary(const Launch& L);  // block per job: kStatLabels
+void launch_hist_summary(const Launch& L);  // kStatLabels of every job (added: zeroed by job_init)
 
 // naive.hip
 // runs every job with algo == VF_NAIVE and matching key width; distance and
