@@ -62,7 +62,26 @@ __device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint
         m[q] = best[q];
         cxy[q] = cx[q] | (cy[q] << 16);
     }
-    for (uint32_t i = 0; i < n; ++i) {
+    // up to 4 cells per lane: four seeds per step, one 16-byte LDS read per coordinate array (the arrays
+    // are 16-byte aligned); more cells already amortize each seed's reads
+    uint32_t i = 0;
+    if constexpr (Q <= 4)
+    for (; i + 4 <= n; i += 4) {
+        const uint4 X4 = *reinterpret_cast<const uint4*>(sx + i), Y4 = *reinterpret_cast<const uint4*>(sy + i),
+                    Z4 = *reinterpret_cast<const uint4*>(sz + i);
+        const uint32_t xs[4] = { X4.x, X4.y, X4.z, X4.w }, ys[4] = { Y4.x, Y4.y, Y4.z, Y4.w },
+                       zs[4] = { Z4.x, Z4.y, Z4.z, Z4.w };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t xy = xs[k] | (ys[k] << 16), id = base + i + k;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const K key = seed_key<DIST, K>(cx[q], cy[q], cz[q], cxy[q], xs[k], ys[k], zs[k], xy, id, b);
+                m[q] = key < m[q] ? key : m[q];
+            }
+        }
+    }
+    for (; i < n; ++i) {
         const uint32_t x = sx[i], y = sy[i], z = sz[i], xy = x | (y << 16), id = base + i;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -96,7 +115,7 @@ __device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32
 template <typename K>
 __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
 {
-    __shared__ uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
+    __shared__ __attribute__((aligned(16))) uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
     __shared__ uint16_t sw[kSeedChunk];
     __shared__ uint16_t cells[kTileCells];  // compacted offsets of the occupied cells
     __shared__ uint4 lab4[kTileCells / 8];  // the tile's output labels, 8 per thread
