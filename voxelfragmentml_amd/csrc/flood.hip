@@ -31,16 +31,17 @@ namespace vf {
 
 namespace {
 
-// Flood key of a cell: 0 = EMPTY; otherwise (K << 2) | w with w = 1 + noise in
+// Flood key of a cell: 0xFFFFFFFF = EMPTY; otherwise (K << 2) | w with w = 1 + noise in
 // 1..3 the cell's node weight and K = (D << b) | seed index, or K = 0x3FFFFFFC
 // (key kUnreachedKey | w) while unreached.  Keys compare by K first (the weight bits only break ties
 // between equal K, which never changes K), so the minimum over neighbours is
-// still the (distance, seed) minimum, and a cell's weight travels with it.
-constexpr uint32_t kEmptyKey = 0u;
+// still the (distance, seed) minimum, and a cell's weight travels with it; EMPTY, all ones, never
+// wins a minimum, so the relaxation copies keys into LDS as they are.
+constexpr uint32_t kEmptyKey = kFloodEmptyKey;
 constexpr uint32_t kUnreachedKey = kFloodUnreachedKey;  // | w
-// LDS tile values are the keys with EMPTY stored as 0xFFFFFFFF, so an unsigned
-// min over neighbours never selects an EMPTY or unreached (>= kUnreachedKey) cell
+// LDS tile value of EMPTY cells and of cells outside the grid
 constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
+static_assert(kEmptyKey == kTileEmpty, "keys are copied into the tile as they are");
 // Halo z rows of 10 words (z = -1..8) at an odd stride, so that the row starts
 // of a wave's 64 lanes fall on distinct LDS banks.
 constexpr uint32_t kRowWords = 11;
@@ -163,13 +164,13 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             v1[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o1) * 4u);
             v2[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o2) * 4u);
         }
-        // outside the grid -> all ones; EMPTY (0) -> kTileEmpty
+        // outside the grid -> all ones (EMPTY keys are all ones already)
         const uint32_t bad1 = in1 ? 0u : kTileEmpty, bad2 = in2 ? 0u : kTileEmpty;
 #pragma unroll
         for (uint32_t hx = 0; hx < 10u; ++hx) {
             const uint32_t xbad = (uint32_t)(X0 + (int)hx - 1) < nx ? 0u : kTileEmpty;
-            tile[hx * 10u * kRowWords + t1] = (v1[hx] ? v1[hx] : kTileEmpty) | bad1 | xbad;
-            tile[hx * 10u * kRowWords + t2] = (v2[hx] ? v2[hx] : kTileEmpty) | bad2 | xbad;
+            tile[hx * 10u * kRowWords + t1] = v1[hx] | bad1 | xbad;
+            tile[hx * 10u * kRowWords + t2] = v2[hx] | bad2 | xbad;
         }
     }
     __syncthreads();
@@ -305,7 +306,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     for (int z = 0; z < 8; ++z) {
         if (k[z] < kUnreachedKey && k[z] >= headroom) error = true;
         const uint32_t w = (wmask >> (2 * z)) & 3u;
-        k[z] = w ? (k[z] | w) : kEmptyKey;  // the stored form: weight bits back, EMPTY as 0
+        k[z] = w ? (k[z] | w) : kEmptyKey;  // the stored form: weight bits back
     }
 
     // write back the rows with a dropped cell, whole: the other cells rewrite the values they hold

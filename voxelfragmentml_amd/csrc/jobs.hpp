@@ -68,7 +68,7 @@ struct JobDev {
     uint32_t noise_amp;
     int seed_location;
     uint64_t seed_key;
-    uint32_t* keys;         // flood keys, V: 0 = EMPTY, else (K << 2) | weight (1..3), K = (distance << key_bits) | seed; kFloodUnreachedKey | weight unreached
+    uint32_t* keys;         // flood keys, V: kFloodEmptyKey = EMPTY, else (K << 2) | weight (1..3), K = (distance << key_bits) | seed; kFloodUnreachedKey | weight unreached
     uint32_t* sub;          // sub-seed pass (null when the seeds carry one id per fragment): S union-find parents, S keep flags
     unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
@@ -81,7 +81,8 @@ struct JobDev {
     int naive_key64;        // naive packed keys need 64 bits
 };
 
-// Flood keys (see flood.hip): 0 = EMPTY, unreached occupied cells kFloodUnreachedKey | weight.
+// Flood keys (see flood.hip): all ones = EMPTY, unreached occupied cells kFloodUnreachedKey | weight.
+constexpr uint32_t kFloodEmptyKey = 0xFFFFFFFFu;
 constexpr uint32_t kFloodUnreachedKey = 0xFFFFFFF0u;
 __host__ __device__ inline uint32_t flood_weight(uint64_t noise_seed, uint32_t cell, uint32_t amp)
 {
