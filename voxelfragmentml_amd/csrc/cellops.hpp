@@ -160,30 +160,8 @@ __device__ inline void walk_tile_runs(const Launch& L, const uint32_t* prefix, u
     walk_tile_runs<N>(L, prefix, total, sJ, f, [](uint32_t) {});
 }
 
-// Grid-stride variant for compute-heavy stages whose tiles differ in cost:
-// tiles are dealt round-robin over the blocks (better balance); the tile
-// prefix (njobs <= 1024) is kept in LDS for the per-tile job search.
+// Job prefixes kept in LDS by the stages that search them per tile (njobs <= 1024).
 constexpr uint32_t kWalkMaxJobs = 1024;
-template <class F>
-__device__ inline void walk_tiles_strided(const Launch& L, const uint32_t* prefix, uint32_t total, JobDev* sJ,
-                                          uint32_t* sp, F&& f)
-{
-    for (uint32_t i = threadIdx.x; i <= L.njobs && i <= kWalkMaxJobs; i += blockDim.x) sp[i] = prefix[i];
-    __syncthreads();
-    uint32_t cur = 0xFFFFFFFFu;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t lo = 0, hi = L.njobs;  // largest j with sp[j] <= t
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (sp[mid] <= t) lo = mid; else hi = mid;
-        }
-        if (lo != cur) {
-            stage_job(sJ, L.jobs + lo);
-            cur = lo;
-        }
-        f(*sJ, lo, t - sp[lo], t);
-    }
-}
 
 // Grid-stride over runs of N consecutive tiles: run r (tiles r*N .. r*N + N - 1, split where a job
 // ends) goes to block r % gridDim.x, and f(J, j, first_local_tile, n) handles n <= N tiles of job j,
