@@ -312,9 +312,8 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
         uint32_t pre[4] = { 0, nt, 0, groups };
         g->vox_setup.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup));
         g->vox_counts.ensure((size_t)std::max<uint32_t>(nt, 1) * 8);
-        g->vox_shell_off.ensure(((size_t)nt + 1) * 8);
-        g->vox_col_off.ensure(((size_t)nt + 1) * 8);
-        g->scan_tmp.ensure(scan_tmp_bytes(nt) + 64);
+        g->vox_shell_off.ensure((2 * (size_t)nt + 1) * 8);
+        g->scan_tmp.ensure(scan_tmp_bytes(2 * (uint64_t)nt) + 64);
         g->ctl.ensure(256);
         std::memcpy(g->pinned + 512, pre, sizeof pre);
         VF_HIP(hipMemcpyAsync(g->ctl.p, g->pinned + 512, sizeof pre, hipMemcpyHostToDevice, g->stream));
@@ -323,7 +322,6 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
         X.setup = g->vox_setup.as<TriSetup>();
         X.counts = g->vox_counts.as<uint32_t>();
         X.shell_off = g->vox_shell_off.as<unsigned long long>();
-        X.col_off = g->vox_col_off.as<unsigned long long>();
         X.tri_prefix = g->ctl.as<uint32_t>();
         X.total_tris = nt;
         X.scan_tmp = g->scan_tmp.p;

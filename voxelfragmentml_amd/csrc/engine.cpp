@@ -265,9 +265,8 @@ void Engine::reserve_front(const Table& t)
     const uint32_t nt = t.total_tris;
     vox_setup_.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup));
     vox_counts_.ensure((size_t)std::max<uint32_t>(nt, 1) * 8);
-    vox_shell_off_.ensure(((size_t)nt + 1) * 8);
-    vox_col_off_.ensure(((size_t)nt + 1) * 8);
-    scan_tmp_.ensure(std::max(scan_tmp_bytes(nt), scan_tmp_bytes(t.L.total_tiles)) + 64);
+    vox_shell_off_.ensure((2 * (size_t)nt + 1) * 8);
+    scan_tmp_.ensure(std::max(scan_tmp_bytes(2 * (uint64_t)nt), scan_tmp_bytes(t.L.total_tiles)) + 64);
     seed_cnt_.ensure((size_t)t.L.total_tiles * 4 + 64);
     seed_off_.ensure(((size_t)t.L.total_tiles + 1) * 8 + 64);
 }
@@ -283,7 +282,6 @@ void Engine::front(Table& t)
     X.setup = vox_setup_.as<TriSetup>();
     X.counts = vox_counts_.as<uint32_t>();
     X.shell_off = vox_shell_off_.as<unsigned long long>();
-    X.col_off = vox_col_off_.as<unsigned long long>();
     X.tri_prefix = t.tri_prefix;
     X.total_tris = nt;
     X.scan_tmp = scan_tmp_.p;

@@ -84,7 +84,7 @@ public:
     static constexpr uint32_t kMaxTableJobs = 1024;
 
 private:
-    DevBuf vox_setup_, vox_counts_, vox_shell_off_, vox_col_off_, scan_tmp_, seed_cnt_, seed_off_;
+    DevBuf vox_setup_, vox_counts_, vox_shell_off_, scan_tmp_, seed_cnt_, seed_off_;
     McScratch mc_;
 };
 

@@ -13,7 +13,7 @@ struct vf_grid {
     vf::DevBuf labels, occ;          // uint16 cells, uint8 occupancy
     vf::DevBuf jobdev, prefix, stats, hist, seeds;
     vf::DevBuf bits;                 // shell + toggle bitmasks
-    vf::DevBuf vox_setup, vox_counts, vox_shell_off, vox_col_off, scan_tmp, occ_tmp;
+    vf::DevBuf vox_setup, vox_counts, vox_shell_off, scan_tmp, occ_tmp;
     vf::DevBuf mesh_xyz, mesh_tri;
     vf::DevBuf seed_tile_cnt, seed_tile_off;
     vf::DevBuf keys, buckets, fctl, ctl, flood_lists, flood_sub, near_in, near_out, erode_tmp, erode_mask;

@@ -126,8 +126,8 @@ struct Launch {
 // voxelize.hip
 struct VoxScratch {
     TriSetup* setup;                   // total triangles, as 8 planes of 16-byte pieces (store_setup / load_setup)
-    unsigned long long* shell_off;     // total triangles + 1 exclusive prefix of the shell work counts
-    unsigned long long* col_off;
+    unsigned long long* shell_off;     // 2 * total triangles + 1: exclusive prefix of counts (shell work counts,
+                                       // then parity column counts; the parity offsets start at shell_off + total)
     uint32_t* counts;                  // 2*total triangles: shell work counts (column segments of one z word
                                        // per candidate (x, y)), then parity column counts
     const uint32_t* tri_prefix;        // njobs+1 prefix over triangles (device)
@@ -135,7 +135,6 @@ struct VoxScratch {
     void* scan_tmp;                    // scan scratch
     size_t scan_tmp_bytes;
 };
-size_t vox_scan_tmp_bytes(uint32_t total_tris);
 // tile_cnt (or null): also count the occupied cells per tile (L.tile_prefix) of jobs that seed anywhere
 // (VF_SEED_BOTH), as launch_seed_count does; launch_seed_count(..., true) then skips those jobs
 void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_col_groups,

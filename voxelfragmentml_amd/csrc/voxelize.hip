@@ -418,9 +418,9 @@ __device__ inline void for_each_pair(const unsigned long long* off, uint32_t ntr
 {
     __shared__ unsigned long long soff[kOffCache + 1];
     __shared__ uint32_t sg[2];
-    const unsigned long long total = off[ntris];
+    const unsigned long long start = off[0], total = off[ntris];  // pairs start..total - 1
     const uint32_t wave = threadIdx.x >> 6;
-    for (unsigned long long p0 = (unsigned long long)blockIdx.x * kPairChunk; p0 < total;
+    for (unsigned long long p0 = start + (unsigned long long)blockIdx.x * kPairChunk; p0 < total;
          p0 += (unsigned long long)gridDim.x * kPairChunk) {
         const unsigned long long p1 = min(p0 + kPairChunk, total);
         if (wave < 2) {
@@ -609,7 +609,6 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_clear(Launch L, const uint3
 
 }  // namespace
 
-size_t vox_scan_tmp_bytes(uint32_t total_tris) { return scan_tmp_bytes(total_tris); }
 
 void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t total_groups, uint32_t* tile_cnt)
 {
@@ -619,7 +618,7 @@ void launch_clear_bits(const Launch& L, const uint32_t* col_prefix, uint32_t tot
     VF_CHECK_LAUNCH();
 }
 
-// X.setup / X.counts / X.shell_off / X.col_off / X.scan_tmp sized for X.total_tris.
+// X.setup / X.counts / X.shell_off / X.scan_tmp sized for X.total_tris (shell_off and the scan for twice that).
 void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_prefix, uint32_t total_groups,
                      uint32_t* tile_cnt)
 {
@@ -634,8 +633,9 @@ void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_p
                 L.jobs, L.njobs, X.tri_prefix, X.total_tris, X.setup, cnt_shell, cnt_col);
             VF_CHECK_LAUNCH();
         }
-        scan_exclusive_u32_to_u64(cnt_shell, X.shell_off, X.total_tris, X.scan_tmp, L.s);
-        scan_exclusive_u32_to_u64(cnt_col, X.col_off, X.total_tris, X.scan_tmp, L.s);
+        // one scan over both count arrays (contiguous): the parity offsets continue from the shell total
+        scan_exclusive_u32_to_u64(cnt_shell, X.shell_off, 2ull * X.total_tris, X.scan_tmp, L.s);
+        const unsigned long long* col_off = X.shell_off + X.total_tris;
         {
             TimedLaunch tl("vox_shell", L.s);
             k_vox_shell<<<active_shape().pair_blocks, kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.shell_off, X.total_tris);
@@ -643,7 +643,7 @@ void launch_voxelize(const Launch& L, const VoxScratch& X, const uint32_t* col_p
         }
         {
             TimedLaunch tl("vox_parity", L.s);
-            k_vox_parity<<<std::max(64u, active_shape().pair_blocks / 2), kVoxThreads, 0, L.s>>>(L.jobs, X.setup, X.col_off,
+            k_vox_parity<<<std::max(64u, active_shape().pair_blocks / 2), kVoxThreads, 0, L.s>>>(L.jobs, X.setup, col_off,
                                                                                                 X.total_tris);
             VF_CHECK_LAUNCH();
         }
