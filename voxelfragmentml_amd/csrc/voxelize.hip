@@ -534,8 +534,9 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
             uint32_t nb = 0;
             uint64_t base = 0;
             if (valid) {
-                const uint64_t col = w / J.wp;
-                const uint32_t z0 = (uint32_t)(w % J.wp) * 32u;
+                const uint32_t lgwp = (uint32_t)__builtin_ctz(J.wp);  // wp is a power of two
+                const uint64_t col = w >> lgwp;
+                const uint32_t z0 = (uint32_t)(w & (J.wp - 1u)) * 32u;
                 if (z0 < J.nz) {
                     nb = min(32u, J.nz - z0);
                     bits = (sh | p) & (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u));
@@ -567,7 +568,11 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
                     const uint32_t below = base >= split ? 0u : (uint32_t)min<uint64_t>(split - base, 32);
                     na = __popc(bits & (below == 32 ? 0xFFFFFFFFu : ((1u << below) - 1u)));
                 }
-                const uint32_t sa = (uint32_t)wave_sum_u64(na), sb = (uint32_t)wave_sum_u64(__popc(bits) - na);
+                // both counts (<= 32 per lane, <= 2048 per wave) packed in one 32-bit wave sum
+                uint32_t both = na | ((uint32_t)__popc(bits) - na) << 16;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) both += __shfl_xor(both, o, 64);
+                const uint32_t sa = both & 0xFFFFu, sb = both >> 16;
                 if ((threadIdx.x & 63u) == 0) {
                     const uint32_t t0 = L.tile_prefix[jidx];
                     if (sa) atomicAdd(tile_cnt + t0 + (uint32_t)ta, sa);
