@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 BOX = ([-1.03] * 3, [1.03] * 3)
 
 
-def mixed_jobs(vf, n, seed, keep, edges=(40, 96), mc_every=0):
+def mixed_jobs(vf, n, seed, keep, edges=(40, 96), mc_every=0, locations=None):
     rng = np.random.default_rng(seed)
     jobs, meta = [], []
     for i in range(n):
@@ -26,21 +26,23 @@ def mixed_jobs(vf, n, seed, keep, edges=(40, 96), mc_every=0):
         labels = np.zeros(e, np.uint16)
         hist = np.zeros(65536, np.uint32)
         keep.extend([labels, hist])
+        loc = locations[i % len(locations)] if locations else vf.SEED_BOTH
         j = vf.make_job(xyz, tri, e, *BOX, S, int(rng.integers(1, 1 << 40)), algo, dist,
                         noise_seed=int(rng.integers(1, 1 << 40)), noise_amp=amp,
-                        boundary_size=1, marching_cubes=1 if (mc_every and i % mc_every == 0) else 0, keep=keep)
+                        boundary_size=1, marching_cubes=1 if (mc_every and i % mc_every == 0) else 0,
+                        seed_location=loc, keep=keep)
         j.labels_out = labels.ctypes.data
         j.histogram_out = hist.ctypes.data
         jobs.append(j)
         meta.append(dict(xyz=xyz, tri=tri, dims=e, S=S, algo=algo, dist=dist, amp=amp, labels=labels, hist=hist,
-                         seed=j.seed, noise_seed=j.noise_seed))
+                         seed=j.seed, noise_seed=j.noise_seed, location=loc))
     return jobs, meta
 
 
 def oracle_job(m):
     occ = O.voxelize(m["xyz"], m["tri"], *BOX, m["dims"])
     g = O.occ_to_grid(occ)
-    seeds = O.seed_uniform(g, m["S"], m["seed"])
+    seeds = O.seed_uniform(g, m["S"], m["seed"], m.get("location", 2))
     if m["algo"] == 0:
         lab = O.naive(g, seeds, m["dist"])
     else:
@@ -70,6 +72,22 @@ def test_batch_matches_oracle_and_plain(vf):
     # without graphs
     b2 = vf.Batch(graphs=False)
     assert [r["checksum"] for r in b2.run(jobs)] == [r["checksum"] for r in res]
+
+
+def test_batch_seed_locations_match_oracle(vf):
+    # jobs seeding anywhere get their candidate counts from the voxelizer's fill, the others from
+    # the candidate-count stage: a mix of both in one batch
+    keep = []
+    jobs, meta = mixed_jobs(vf, 6, 21, keep, edges=(40, 72), locations=[vf.SEED_OUTER, vf.SEED_BOTH, vf.SEED_INNER])
+    b = vf.Batch()
+    res = b.run(jobs)
+    b.close()
+    for r, m in zip(res, meta):
+        print("seed location", m["location"], m["dims"], m["S"], flush=True)
+        assert r["status"] == 0, r
+        ref, occ = oracle_job(m)
+        np.testing.assert_array_equal(m["labels"], ref)
+        assert r["occupied"] == occ
 
 
 def test_batch_alone_equals_in_mix(vf):

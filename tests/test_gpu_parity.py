@@ -90,6 +90,20 @@ def test_naive_many_seeds_and_64bit_keys(vf):
         np.testing.assert_array_equal(g.read(), O.naive(base, seeds, dist))
 
 
+@pytest.mark.parametrize("dist", [EUC, MAN, CHE])
+def test_naive_dense_tiles(vf, dist):
+    # every cell occupied: 2048 cells per tile, 8 per lane, beside the 1-4 per lane of sparse grids
+    print("naive dense", dist, flush=True)
+    dims = (40, 24, 36)
+    base = np.ones(dims, np.uint16)
+    g = vf.Grid(dims, *BOX)
+    for S in (9, 37):
+        seeds = O.seed_uniform(base, S, 5 + S)
+        g.write(base)
+        g.fracture_naive(seeds, dist)
+        np.testing.assert_array_equal(g.read(), O.naive(base, seeds, dist))
+
+
 def test_naive_ties_and_seeds_outside_rejected(vf):
     g = vf.Grid((5, 1, 8), *BOX)
     g.write(np.ones((5, 1, 8), np.uint16))
