@@ -503,6 +503,10 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
 {
     __shared__ JobDev sJ;
     __shared__ unsigned long long part[kVoxThreads / 64];
+    // a wave's 64 words are whole columns (wp divides 64): at most 2048 consecutive occupancy bytes, staged
+    // here and stored as dwords with consecutive lanes on consecutive dwords (a lane's own 32 bytes would
+    // put a wave's 16-byte stores 32 bytes apart)
+    __shared__ uint32_t stage[kVoxThreads / 64][512];
     unsigned long long acc = 0;
     constexpr uint32_t kRun = 4;  // column groups per step: all their bit loads are issued first
     walk_tile_runs<kRun>(L, col_prefix, total_groups, &sJ, [&](const JobDev& J, uint32_t jidx, uint32_t first, uint32_t nrun) {
@@ -543,19 +547,30 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_fill(Launch L, const uint32
                     base = col * J.nz + z0;
                 }
             }
-            if (nb) {
-                uint8_t* o = J.occ + base;
-                if ((J.nz & 15u) == 0 && nb == 32) {
-                    gstore4(o, make_uint4(spread4(bits), spread4(bits >> 4), spread4(bits >> 8), spread4(bits >> 12)));
-                    gstore4(o + 16, make_uint4(spread4(bits >> 16), spread4(bits >> 20), spread4(bits >> 24), spread4(bits >> 28)));
-                } else if ((J.nz & 7u) == 0 && nb == 32) {
-    #pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q) gstore2(o + 8 * q, make_uint2(spread4(bits >> (8 * q)), spread4(bits >> (8 * q + 4))));
-                } else if ((J.nz & 3u) == 0) {
-                    for (uint32_t q = 0; q < nb / 4; ++q) gcast<uint32_t>(o)[q] = spread4(bits >> (4 * q));
-                } else {
-                    for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
+            if ((J.nz & 3u) == 0) {
+                // the wave's region starts at lane 0's cell (a column start when lane 0's word is valid;
+                // when it is not, no lane's is) and spans its valid columns
+                const uint32_t lgwp = (uint32_t)__builtin_ctz(J.wp);
+                const uint64_t w0 = w - lane;
+                const uint64_t col0 = w0 >> lgwp, cols = (uint64_t)J.nx * J.ny;
+                const uint32_t ncols = col0 < cols ? (uint32_t)min<uint64_t>(64u >> lgwp, cols - col0) : 0u;
+                const uint32_t ndw = ncols * (J.nz / 4);  // dwords of the region, <= 512
+                uint32_t* st = stage[threadIdx.x >> 6];
+                if (nb) {
+                    const uint32_t off = (uint32_t)((w & (64u - 1u)) >> lgwp) * (J.nz / 4) + (uint32_t)(w & (J.wp - 1u)) * 8u;
+#pragma unroll
+                    for (uint32_t q = 0; q < 8; ++q)
+                        if (4 * q < nb) st[off + q] = spread4(bits >> (4 * q));
                 }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                auto* dst = gcast<uint32_t>(J.occ + (col0 * J.nz));
+                for (uint32_t i = lane; i < ndw; i += 64) dst[i] = st[i];
+                __builtin_amdgcn_wave_barrier();  // the stage is rewritten by the next group
+            } else if (nb) {
+                uint8_t* o = J.occ + base;
+                for (uint32_t q = 0; q < nb; ++q) o[q] = (uint8_t)((bits >> q) & 1u);
             }
             acc += (unsigned long long)__popc(bits);
             if (tile_cnt && J.seed_location == VF_SEED_BOTH) {
