@@ -202,7 +202,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_post(Launch L, int bsize
 // load the halo and run separable max/min passes z, y, x in LDS.
 constexpr int kBTX = 8, kBTY = 8, kBTZ = 32;
 constexpr int kBHX = kBTX + 2, kBHY = kBTY + 2, kBHZ = kBTZ + 2;
-constexpr int kAZ = kBTZ + 1;  // z-pass row stride (words): odd, so rows spread over the LDS banks
+// z-pass row stride (words): a multiple of 4, so a thread's 8-value segment is two 16-byte LDS accesses;
+// 36 = 9 groups of 4 banks per row, so the 16-byte accesses of 8 consecutive lanes (2 rows x 4 segments)
+// fall on distinct bank groups
+constexpr int kAZ = kBTZ + 4;
 
 // CSUM: also accumulate the grid checksum of the final cells (jobs with a
 // window size <= 1; the others get theirs from k_checksum<true>).
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
 {
     // z-window max (low 16 bits) | 0xFFFF - min (high 16) over labelled cells (EMPTY / FREE contribute 0),
     // so one packed u16 max combines both halves; row = hx * kBHY + hy
-    __shared__ uint32_t zmm[kBHX * kBHY * kAZ];
+    __shared__ __attribute__((aligned(16))) uint32_t zmm[kBHX * kBHY * kAZ];
     __shared__ uint32_t bins[HIST ? kTiledBins : 1];
     __shared__ JobDev sJ;
     __shared__ BlockSum csum_part;
@@ -396,30 +399,38 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
             {
                 uint32_t zw[8];
                 zwin(w, zw);
-                uint32_t* dst = zmm + ((tx + 1) * kBHY + ty + 1) * kAZ + seg;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) dst[k] = zw[k];
+                uint4* dst = reinterpret_cast<uint4*>(zmm + ((tx + 1) * kBHY + ty + 1) * kAZ + seg);
+                dst[0] = make_uint4(zw[0], zw[1], zw[2], zw[3]);
+                dst[1] = make_uint4(zw[4], zw[5], zw[6], zw[7]);
             }
             if (halo_task) {
                 uint16_t hw[10];
                 load_row(x0 + hhx - 1, y0 + hhy - 1, z0 + seg, hw);
                 uint32_t zw[8];
                 zwin(hw, zw);
-                uint32_t* dst = zmm + (hhx * kBHY + hhy) * kAZ + seg;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) dst[k] = zw[k];
+                uint4* dst = reinterpret_cast<uint4*>(zmm + (hhx * kBHY + hhy) * kAZ + seg);
+                dst[0] = make_uint4(zw[0], zw[1], zw[2], zw[3]);
+                dst[1] = make_uint4(zw[4], zw[5], zw[6], zw[7]);
             }
             __syncthreads();
             // x and y windows together: a labelled cell without the bit is interior iff the packed
             // window (max | 0xFFFF - min) equals its own packed value; cells beyond the grid are 0
+            uint32_t accs[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const uint4* row = reinterpret_cast<const uint4*>(zmm + ((tx + dx) * kBHY + ty + dy) * kAZ + seg);
+                    const uint4 a = row[0], b = row[1];
+                    accs[0] = pk_max_u16(accs[0], a.x); accs[1] = pk_max_u16(accs[1], a.y);
+                    accs[2] = pk_max_u16(accs[2], a.z); accs[3] = pk_max_u16(accs[3], a.w);
+                    accs[4] = pk_max_u16(accs[4], b.x); accs[5] = pk_max_u16(accs[5], b.y);
+                    accs[6] = pk_max_u16(accs[6], b.z); accs[7] = pk_max_u16(accs[7], b.w);
+                }
             bool changed = false;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                    for (int dy = 0; dy < 3; ++dy) acc = pk_max_u16(acc, zmm[((tx + dx) * kBHY + ty + dy) * kAZ + seg + k]);
+                const uint32_t acc = accs[k];
                 const uint32_t c = v[k];
                 const bool bnd = c > kFree && (c & 0x8000u) == 0 && acc != packed_self(c);
                 v[k] = bnd ? (uint16_t)(c | 0x8000u) : (uint16_t)c;
