@@ -249,7 +249,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); jobs are independent, so several GPUs would "
-              f"each run their own share (see DESIGN.md)", file=sys.stderr)
+              f"each run their own share of the jobs, one process per GPU", file=sys.stderr)
         sys.exit(2)
     skip = set(x for x in a.skip.split(",") if x)
     from voxelfragmentml_amd import vfrag as vf
