@@ -51,7 +51,7 @@ __device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t xy, ui
 // a thread: each seed is read once (an LDS broadcast) and applied to all Q
 // cells held in registers.
 template <int DIST, typename K, int Q>
-__device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8],
+__device__ inline void naive_cells_q(K best[4], const uint32_t cx[4], const uint32_t cy[4], const uint32_t cz[4],
                                      const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, uint32_t base, uint32_t n,
                                      uint32_t b)
 {
@@ -62,10 +62,8 @@ __device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint
         m[q] = best[q];
         cxy[q] = cx[q] | (cy[q] << 16);
     }
-    // up to 4 cells per lane: four seeds per step, one 16-byte LDS read per coordinate array (the arrays
-    // are 16-byte aligned); more cells already amortize each seed's reads
+    // four seeds per step, one 16-byte LDS read per coordinate array (the arrays are 16-byte aligned)
     uint32_t i = 0;
-    if constexpr (Q <= 4)
     for (; i + 4 <= n; i += 4) {
         const uint4 X4 = *reinterpret_cast<const uint4*>(sx + i), Y4 = *reinterpret_cast<const uint4*>(sy + i),
                     Z4 = *reinterpret_cast<const uint4*>(sz + i);
@@ -93,27 +91,29 @@ __device__ inline void naive_cells_q(K best[8], const uint32_t cx[8], const uint
     for (int q = 0; q < Q; ++q) best[q] = m[q];
 }
 
-// qmax: cells per thread, the same for the whole wave (lanes past the
+// qmax (1..4): cells per thread, the same for the whole wave (lanes past the
 // tile's last occupied cell compute on a dummy cell)
 template <int DIST, typename K>
-__device__ inline void naive_cells(K best[8], const uint32_t cx[8], const uint32_t cy[8], const uint32_t cz[8], uint32_t qmax,
+__device__ inline void naive_cells(K best[4], const uint32_t cx[4], const uint32_t cy[4], const uint32_t cz[4], uint32_t qmax,
                                    const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, uint32_t base, uint32_t n,
                                    uint32_t b)
 {
     if (qmax <= 1) naive_cells_q<DIST, K, 1>(best, cx, cy, cz, sx, sy, sz, base, n, b);
     else if (qmax == 2) naive_cells_q<DIST, K, 2>(best, cx, cy, cz, sx, sy, sz, base, n, b);
     else if (qmax == 3) naive_cells_q<DIST, K, 3>(best, cx, cy, cz, sx, sy, sz, base, n, b);
-    else if (qmax == 4) naive_cells_q<DIST, K, 4>(best, cx, cy, cz, sx, sy, sz, base, n, b);
-    else naive_cells_q<DIST, K, 8>(best, cx, cy, cz, sx, sy, sz, base, n, b);
+    else naive_cells_q<DIST, K, 4>(best, cx, cy, cz, sx, sy, sz, base, n, b);
 }
 
 
 // One tile per step: the tile's occupied cells are compacted in LDS, so the
 // seed loop runs over occupied cells only (shells occupy a few percent of
 // their box); labels return through LDS and leave as one 16-byte store per
-// thread.
+// thread.  A tile costs several block barriers and dependent LDS steps, so
+// the register budget is set for more waves in flight: 5 per SIMD with
+// 32-bit keys (95 VGPRs instead of 127, 1.06 -> 0.94 ms on the mixed batch),
+// 4 with 64-bit keys, both without spills.
 template <typename K>
-__global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
+__global__ __launch_bounds__(kCellThreads) __attribute__((amdgpu_waves_per_eu(sizeof(K) == 4 ? 5 : 4))) void k_naive(Launch L)
 {
     __shared__ __attribute__((aligned(16))) uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
     __shared__ uint16_t sw[kSeedChunk];
@@ -182,38 +182,46 @@ __global__ __launch_bounds__(kCellThreads) void k_naive(Launch L)
         __syncthreads();
         if (total) {
             // thread t holds compacted cells t, t + 256, ...: a wave's first lane holds the most, so
-            // waves past the last cell skip the seed loop
+            // waves past the last cell skip the seed loop.  Cells are taken four per thread at a time
+            // (the second group only in tiles with more than 1024 occupied cells), so the registers
+            // hold four cells' coordinates and keys, not eight.
             const uint32_t nq = tid < total ? (total - tid + kCellThreads - 1) / kCellThreads : 0u;
             const uint32_t tid0 = wave * 64u;
             const uint32_t qmax = tid0 < total ? (total - tid0 + kCellThreads - 1) / kCellThreads : 0u;  // wave-uniform
-            uint32_t cx[8], cy[8], cz[8];
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-                cx[q] = cy[q] = cz[q] = 0;
-                if (q < nq) decompose(t0 + cells[tid + q * kCellThreads], J, cx[q], cy[q], cz[q]);
-            }
-            K best[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) best[q] = ~(K)0;
             const uint32_t b = J.key_bits;
             const int dist = J.dist;
-            for (uint32_t base = 0; base < S; base += kSeedChunk) {
-                const uint32_t m = min(kSeedChunk, S - base);
-                if (!one_chunk) {
-                    stage(J, base, m);
-                    staged = 0xFFFFFFFFu;
-                }
-                if (qmax == 0) continue;  // staging above stays block-wide
-                if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
-                else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
-                else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, qmax, sx, sy, sz, base, m, b);
-            }
             const K mask = ((K)1 << b) - 1;
+            const uint32_t groups = total > 4u * kCellThreads ? 2u : 1u;  // block-uniform
+            for (uint32_t g = 0; g < groups; ++g) {
+                const uint32_t q0 = 4u * g;
+                const uint32_t gq = nq > q0 ? min(nq - q0, 4u) : 0u;
+                const uint32_t gqmax = qmax > q0 ? min(qmax - q0, 4u) : 0u;  // wave-uniform
+                uint32_t cx[4], cy[4], cz[4];
 #pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-                if (q >= nq) break;
-                const uint32_t id = (uint32_t)(best[q] & mask);
-                lab[cells[tid + q * kCellThreads]] = one_chunk ? sw[id] : (uint16_t)J.seeds[id].w;
+                for (uint32_t q = 0; q < 4; ++q) {
+                    cx[q] = cy[q] = cz[q] = 0;
+                    if (q < gq) decompose(t0 + cells[tid + (q0 + q) * kCellThreads], J, cx[q], cy[q], cz[q]);
+                }
+                K best[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) best[q] = ~(K)0;
+                for (uint32_t base = 0; base < S; base += kSeedChunk) {
+                    const uint32_t m = min(kSeedChunk, S - base);
+                    if (!one_chunk) {
+                        stage(J, base, m);
+                        staged = 0xFFFFFFFFu;
+                    }
+                    if (gqmax == 0) continue;  // staging above stays block-wide
+                    if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
+                    else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
+                    else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    if (q >= gq) break;
+                    const uint32_t id = (uint32_t)(best[q] & mask);
+                    lab[cells[tid + (q0 + q) * kCellThreads]] = one_chunk ? sw[id] : (uint16_t)J.seeds[id].w;
+                }
             }
         }
         __syncthreads();
