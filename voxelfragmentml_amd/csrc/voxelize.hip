@@ -451,8 +451,9 @@ __device__ inline void for_each_pair(const unsigned long long* off, uint32_t ntr
 }
 
 // One thread per column segment of a triangle's candidate box: the cells (x, y, z) with z in one
-// 32-cell word, tested by sat_column and merged into the shell word with one atomicOr.
-__global__ __launch_bounds__(kVoxThreads) void k_vox_shell(const JobDev* jobs, const TriSetup* setup,
+// 32-cell word, tested by sat_column and merged into the shell word with one atomicOr.  Held to 64
+// VGPRs (8 waves per SIMD instead of 6, no spills: 1.97 -> 1.94 ms on the mixed batch).
+__global__ __launch_bounds__(kVoxThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_vox_shell(const JobDev* jobs, const TriSetup* setup,
                                                            const unsigned long long* off, uint32_t ntris)
 {
     for_each_pair(off, ntris, [&](uint32_t g, unsigned long long local, bool valid) {
