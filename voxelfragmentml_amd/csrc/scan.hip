@@ -9,6 +9,29 @@ namespace {
 constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+static_assert(kScanItems % 4 == 0, "a thread's items are loaded as 16-byte vectors");
+
+// a thread's items whole and both arrays 16-byte aligned: 16-byte loads and stores (a quarter and half
+// as many accesses as single items)
+__device__ inline bool vec_ok(const void* a, const void* b, uint64_t base, uint64_t n)
+{
+    return base + kScanItems <= n && (((uintptr_t)a | (uintptr_t)b) & 15u) == 0;
+}
+
+__device__ inline void load_items(const uint32_t* in, uint64_t base, uint64_t n, bool vec, uint32_t v[kScanItems])
+{
+    if (vec) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (uint32_t q = 0; q < kScanItems / 4; ++q) {
+            const uint4 a = p[q];
+            v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+    }
+}
 
 __device__ inline unsigned long long block_excl_scan(unsigned long long v, unsigned long long* smem,
                                                      unsigned long long* total)
@@ -42,9 +65,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* in
 {
     __shared__ unsigned long long smem[kScanThreads / 64 + 1];
     uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    load_items(in, base, n, vec_ok(in, in, base, n), v);
     unsigned long long s = 0;
-    for (uint32_t k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += in[base + k];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) s += v[k];
     unsigned long long total;
     (void)block_excl_scan(s, smem, &total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -70,17 +95,27 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, 
 {
     __shared__ unsigned long long smem[kScanThreads / 64 + 1];
     uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    const bool vec = vec_ok(in, out, base, n);
     uint32_t v[kScanItems];
+    load_items(in, base, n, vec, v);
     unsigned long long s = 0;
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        s += v[k];
-    }
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) s += v[k];
     unsigned long long total;
     unsigned long long e = block_excl_scan(s, smem, &total) + sums[blockIdx.x];
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = e;
-        e += v[k];
+    if (vec) {
+        uint4* o = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (uint32_t q = 0; q < kScanItems / 2; ++q) {
+            const unsigned long long e0 = e, e1 = e + v[2 * q];
+            e = e1 + v[2 * q + 1];
+            o[q] = make_uint4((uint32_t)e0, (uint32_t)(e0 >> 32), (uint32_t)e1, (uint32_t)(e1 >> 32));
+        }
+    } else {
+        for (uint32_t k = 0; k < kScanItems; ++k) {
+            if (base + k < n) out[base + k] = e;
+            e += v[k];
+        }
     }
     if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = sums[nb];
 }
