@@ -417,7 +417,7 @@ int vf_seed_near(vf_grid* g, const vf_seed* frags, uint32_t F, uint32_t impacts,
         VF_HIP(hipMemcpyAsync(g->near_in.p, frags, (size_t)F * sizeof(uint4), hipMemcpyHostToDevice, g->stream));
         Launch L = grid_launch(g);
         g->fctl.ensure(64 * 4);
-        uint32_t* dres = g->fctl.as<uint32_t>() + 48;  // words 48..49: count, draw-cap flag
+        uint32_t* dres = g->fctl.as<uint32_t>() + 40;  // words 40..41: count, draw-cap flag
         launch_seed_near(L, g->near_in.as<uint4>(), F, impacts, num_seeds, spreading, seed, g->near_out.as<uint4>(), dres);
         VF_HIP(hipMemcpyAsync(g->pinned, dres, 8, hipMemcpyDeviceToHost, g->stream));
         VF_HIP(hipStreamSynchronize(g->stream));
@@ -561,18 +561,18 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         J.algo = VF_FLOOD;
         g->job_dirty = true;
         Launch L = grid_launch(g);
-        // fctl words 32..33: task prefix of the one job; 40..47: worklist counters
+        // fctl words 32..33: task prefix of the one job; 48..63: worklist counters (two list sets)
         const uint32_t tasks = relax_tasks(g->dims[0], g->dims[1], g->dims[2]);
         FloodRoundScratch W{};
         W.task_prefix = g->fctl.as<uint32_t>() + 32;
         W.total_tasks = tasks;
         W.list_cap = flood_list_cap(tasks);
-        g->flood_lists.ensure((size_t)kFloodXcds * W.list_cap * 8);
+        g->flood_lists.ensure(2ull * kFloodXcds * W.list_cap * 8);
         W.lists = g->flood_lists.as<unsigned long long>();
-        W.counts = g->fctl.as<uint32_t>() + 40;
-        std::memset(g->pinned, 0, 64);
+        W.counts = g->fctl.as<uint32_t>() + 48;
+        std::memset(g->pinned, 0, 128);
         g->pinned[1] = tasks;
-        VF_HIP(hipMemcpyAsync(g->fctl.as<uint32_t>() + 32, g->pinned, 64, hipMemcpyHostToDevice, g->stream));
+        VF_HIP(hipMemcpyAsync(g->fctl.as<uint32_t>() + 32, g->pinned, 128, hipMemcpyHostToDevice, g->stream));
         VF_HIP(hipMemsetAsync(J.bflags, 0, 16ull * relax_words_per_parity(nbricks), g->stream));  // both flag parities
         VF_HIP(hipStreamSynchronize(g->stream));  // the pinned words are reused below
         launch_flood_init(L);

@@ -188,7 +188,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     const size_t o_gather = align_up(o_bt + (n + 1) * 4);
     const size_t o_done = align_up(o_gather + 16 * 4 * (size_t)n);
     const size_t o_fcnt = align_up(o_done + 4 * ((size_t)n + 1));
-    const size_t total = align_up(o_fcnt + 4 * kFloodXcds);
+    const size_t total = align_up(o_fcnt + 4 * 2 * kFloodXcds);
     std::vector<uint8_t> img(total, 0);
     auto* jd = reinterpret_cast<JobDev*>(img.data() + o_jobs);
     auto* tp = reinterpret_cast<uint32_t*>(img.data() + o_tile);
@@ -246,7 +246,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
     if (fchunks) {
         t->flood.list_cap = flood_list_cap((uint32_t)fchunks);
-        VF_HIP(hipMallocAsync(&t->flood_lists, (size_t)kFloodXcds * t->flood.list_cap * 8, s));
+        VF_HIP(hipMallocAsync(&t->flood_lists, 2ull * kFloodXcds * t->flood.list_cap * 8, s));
         t->flood.lists = static_cast<unsigned long long*>(t->flood_lists);
     }
     return t;

@@ -359,12 +359,10 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     return sweeps;
 }
 
-// Load the uniform per-job state of a round into scalar registers.
-__device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
+// The uniform per-job state of round r in scalar registers.
+__device__ inline void load_relax_state(const JobDev* J, uint32_t r, RelaxJob& R)
 {
-    const JobDev* J = L.jobs + j;
     R.J = uni_ptr(J);
-    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u && uni(J->fctl[1]) == 0u)) return false;
     R.nx = uni(J->nx); R.ny = uni(J->ny); R.nz = uni(J->nz);
     R.nbx = bdim(R.nx); R.nby = bdim(R.ny); R.nbz = bdim(R.nz);
     R.nb = uni(J->nbricks);
@@ -372,16 +370,56 @@ __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
     R.amp = uni(J->noise_amp);
     R.noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
     R.keys = gcast<uint32_t>(uni_ptr(J->keys));
-    R.r = uni(J->fctl[0]);
+    R.r = r;
     R.nw = relax_words_per_parity(R.nb);
     unsigned long long* flags = uni_ptr(J->bflags);
     R.cur = gcast<unsigned long long>(flags + (size_t)R.nw * (R.r & 1u));
     R.nxt = gcast<unsigned long long>(flags + (size_t)R.nw * ((R.r + 1u) & 1u));
     R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
+}
+
+// A running job's state for the relaxation step (its round was set by this round's compaction).
+__device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
+{
+    const JobDev* J = L.jobs + j;
+    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u && uni(J->fctl[1]) == 0u)) return false;
+    load_relax_state(J, uni(J->fctl[0]), R);
     return true;
 }
 
-// Round step 1: turn the flag bitmasks of every running job into worklists.
+// Start job j's next round (in the compaction of chunk round i): the round the previous
+// compaction started is read from slot fctl[4 + ((i + 1) & 1)]; when the relaxation of that round
+// flagged nothing the job is drained there, else round rp + 1 starts: it goes into slot
+// fctl[4 + (i & 1)] and fctl[0], and the flags-non-empty word of round rp is cleared (round
+// rp + 1's relaxation writes it for round rp + 2).  Every workgroup of the job computes the same
+// values from words no workgroup writes in this launch, and writes them: no ordering is needed.
+__device__ inline bool start_round(const Launch& L, uint32_t j, uint32_t i, RelaxJob& R)
+{
+    const JobDev* J = L.jobs + j;
+    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u)) return false;
+    uint32_t* ctl = J->fctl;
+    const uint32_t done = uni(ctl[1]), rp = uni(ctl[4 + ((i + 1u) & 1u)]);
+    if (done) return false;
+    const bool lead = threadIdx.x == 0;
+    if (uni(ctl[2 + ((rp + 1u) & 1u)]) == 0u) {
+        if (lead) {
+            ctl[1] = rp;
+            J->stats[kStatFloodDone] = rp;
+        }
+        return false;
+    }
+    const uint32_t r = rp + 1u;
+    if (lead) {
+        ctl[4 + (i & 1u)] = r;
+        ctl[0] = r;
+        ctl[2 + (rp & 1u)] = 0u;
+    }
+    load_relax_state(J, r, R);
+    return true;
+}
+
+// Round step 1: start every running job's next round (or find it drained) and
+// turn its flag bitmasks into worklists.
 // A task is 64 flag words (4096 bricks, one word per lane) of one job; each
 // lane clears its word and appends its flagged bricks as (job << 32 | brick)
 // to the worklist of the task's XCD, with one list reservation per wave.
@@ -389,12 +427,16 @@ __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
 // which serves list x from the workgroups of XCD x, finds the halos of
 // neighbouring bricks (mostly in the same task) in the same L2.
 // prefix: njobs + 1 task prefix over the jobs (0 tasks for non-flood jobs);
-// lists: kXcds lists of list_cap entries; counts: kXcds counters, zero on entry.
+// lists: kXcds lists of list_cap entries; counts: kXcds counters, zero on entry;
+// stale: the other list set's kXcds counters (the previous round's), cleared here;
+// i: the round's index in its chunk.
 __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* prefix, uint32_t total_tasks,
-                                                      unsigned long long* lists, uint32_t list_cap, uint32_t* counts)
+                                                      unsigned long long* lists, uint32_t list_cap, uint32_t* counts,
+                                                      uint32_t* stale, uint32_t i)
 {
     __shared__ uint32_t sp[kFloodMaxJobs + 1];
     const uint32_t lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane < kXcds) stale[lane] = 0u;
     for (uint32_t i = lane; i <= L.njobs; i += 64u) sp[i] = prefix[i];
     __syncthreads();
     const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, local = blockIdx.x / kXcds;
@@ -413,7 +455,7 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
         const uint32_t j = uni(lo);
         if (j != cur_job) {
             cur_job = j;
-            running = load_relax_job(L, j, R);
+            running = start_round(L, j, i, R);
         }
         if (!running) continue;
         const uint32_t w = (task - sp[j]) * 64u + lane;
@@ -482,15 +524,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     flush();
 }
 
-// Round step 3 (single block): advance every running flood job to its next
-// round, or mark it drained when no brick was flagged for that round, and
-// clear the worklist counters.  The flags of the round just run were cleared
-// as they were listed; its non-empty word is cleared here because the round
-// after next writes it.  njobs <= 1024.
-__global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
+// End of a chunk (single block): mark drained every running flood job whose
+// last round flagged nothing, so the host sees it after the chunk (the next
+// chunk's first compaction would find the same).  njobs <= 1024.
+__global__ __launch_bounds__(1024) void k_flood_plan(Launch L)
 {
     const uint32_t j = threadIdx.x;
-    if (j < kXcds) counts[j] = 0u;
     if (j >= L.njobs) return;
     const JobDev* J = L.jobs + j;
     if (!J->active || J->algo != VF_FLOOD || J->fctl[1] != 0u) return;
@@ -498,9 +537,6 @@ __global__ __launch_bounds__(1024) void k_flood_plan(Launch L, uint32_t* counts)
     if (J->fctl[2 + ((r + 1u) & 1u)] == 0u) {
         J->fctl[1] = r;
         J->stats[kStatFloodDone] = r;
-    } else {
-        J->fctl[2 + (r & 1u)] = 0u;
-        J->fctl[0] = r + 1u;
     }
 }
 
@@ -748,6 +784,7 @@ __global__ __launch_bounds__(1024) void k_flood_sub_keep(Launch L)
         J->fctl[1] = 0;
         J->fctl[2] = 0;
         J->fctl[3] = 1;
+        J->fctl[4] = J->fctl[5] = 0;  // the next compaction starts round 1
     }
 }
 
@@ -814,24 +851,28 @@ void launch_flood_rounds(const Launch& L, const FloodRoundScratch& W, uint32_t r
 {
     if (L.njobs > kFloodMaxJobs) fail(VF_ERR_INVALID_ARG, "flood launch with %u jobs (max %u)", L.njobs, kFloodMaxJobs);
     if (W.list_cap < flood_list_cap(W.total_tasks)) fail(VF_ERR_INVALID_ARG, "flood worklists too small");
+    if (rounds == 0) return;
+    rounds += rounds & 1u;  // even chunks: a job's round slots (fctl[4..5]) follow the round's index in its chunk
     blocks = (blocks + kXcds - 1) / kXcds * kXcds;
     // one compaction wave per task is enough (tasks are dealt round-robin over the XCDs)
     const uint32_t cblocks = std::max(kXcds, std::min(blocks, (W.total_tasks + kXcds - 1) / kXcds * kXcds));
+    unsigned long long* lists[2] = { W.lists, W.lists + (size_t)kXcds * W.list_cap };
+    uint32_t* counts[2] = { W.counts, W.counts + kXcds };
     for (uint32_t i = 0; i < rounds; ++i) {
+        const uint32_t p = i & 1u;  // round i lists into set p; set p ^ 1 (the previous round's) is cleared
         {
             TimedLaunch tl("flood_compact", L.s);
-            k_flood_compact<<<cblocks, 64, 0, L.s>>>(L, W.task_prefix, W.total_tasks, W.lists, W.list_cap, W.counts);
+            k_flood_compact<<<cblocks, 64, 0, L.s>>>(L, W.task_prefix, W.total_tasks, lists[p], W.list_cap, counts[p],
+                                                     counts[p ^ 1u], i);
             VF_CHECK_LAUNCH();
         }
-        {
-            TimedLaunch tl("flood_relax", L.s);
-            k_flood_relax<<<blocks, 64, 0, L.s>>>(L, W.lists, W.list_cap, W.counts);
-            VF_CHECK_LAUNCH();
-        }
-        TimedLaunch tl("flood_plan", L.s);
-        k_flood_plan<<<1, 1024, 0, L.s>>>(L, W.counts);
+        TimedLaunch tl("flood_relax", L.s);
+        k_flood_relax<<<blocks, 64, 0, L.s>>>(L, lists[p], W.list_cap, counts[p]);
         VF_CHECK_LAUNCH();
     }
+    TimedLaunch tl("flood_plan", L.s);
+    k_flood_plan<<<1, 1024, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
 }
 
 void launch_flood_sub(const Launch& L)

@@ -72,7 +72,7 @@ struct JobDev {
     uint32_t* sub;          // sub-seed pass (null when the seeds carry one id per fragment): S union-find parents, S keep flags
     unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
-    uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty
+    uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty, [4..5] round started by the compaction of an even / odd round of a chunk
     unsigned long long* stats;  // kStatCount slots
     uint32_t* hist;         // 65536 counts
     int active;             // 0: the job is skipped by multi-job launches
@@ -175,7 +175,7 @@ void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint
                             uint32_t* host_counter);
 
 // flood.hip
-constexpr uint32_t kFloodCtlWords = 4;
+constexpr uint32_t kFloodCtlWords = 6;
 constexpr uint32_t kFloodMaxJobs = 1024;  // jobs per multi-job flood launch
 inline uint32_t relax_bricks(uint32_t nx, uint32_t ny, uint32_t nz)
 {
@@ -196,9 +196,9 @@ void launch_flood_seed(const Launch& L);
 struct FloodRoundScratch {
     const uint32_t* task_prefix;   // njobs + 1: prefix of relax_tasks over the jobs (0 for non-flood jobs)
     uint32_t total_tasks;
-    unsigned long long* lists;     // per-XCD brick worklists, list_cap entries each
+    unsigned long long* lists;     // 2 x kFloodXcds brick worklists (even and odd rounds of a chunk), list_cap entries each
     uint32_t list_cap;             // >= flood_list_cap(total_tasks)
-    uint32_t* counts;              // per-XCD list lengths; zero before the first round
+    uint32_t* counts;              // 2 x kFloodXcds list lengths; zero before the first round (a chunk leaves them zero)
 };
 constexpr uint32_t kFloodXcds = 8;
 uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
