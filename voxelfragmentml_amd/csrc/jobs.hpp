@@ -202,7 +202,7 @@ struct FloodRoundScratch {
 };
 constexpr uint32_t kFloodXcds = 8;
 uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
-// `rounds` relaxation rounds for every running flood job of L.  Each job
+// `rounds` (rounded up to an even count) relaxation rounds for every running flood job of L.  Each job
 // continues at its own round, so jobs may join a table between calls; rounds
 // after a job drained do nothing for it, so the launches can be replayed from
 // a graph.
