@@ -609,11 +609,23 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
         }
         const uint64_t V = J.V;
         const uint32_t mask = (1u << J.key_bits) - 1u;
+        // a cell's key is EMPTY exactly where its occupancy byte is 0 (the init writes EMPTY there, relax never
+        // lowers an EMPTY key, a seed cell gets its occupancy byte set): keys are read only for the 8-cell
+        // segments that hold an occupied cell, a small part of a vessel's box
+        uint64_t occ[kRun];
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
+            occ[r] = r < n && c0 < V ? load8b(J.occ, c0, V) : 0ull;
+        }
         uint32_t key[kRun][8];
 #pragma unroll
         for (uint32_t r = 0; r < kRun; ++r) {
             const uint64_t c0 = (uint64_t)(local + r) * kTileCells + threadIdx.x * kCellsPerThread;
-            if (r < n && c0 + 8 <= V) {
+            if (occ[r] == 0ull) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) key[r][k] = kEmptyKey;
+            } else if (c0 + 8 <= V) {
                 const uint4 a = gload4(J.keys + c0);
                 const uint4 b = gload4(J.keys + c0 + 4);
                 key[r][0] = a.x; key[r][1] = a.y; key[r][2] = a.z; key[r][3] = a.w;
@@ -629,7 +641,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
             if (r >= n || c0 >= V) break;
             uint16_t out[8];
             for (int k = 0; k < 8; ++k) {
-                if (key[r][k] == kEmptyKey) out[k] = kEmpty;
+                if (((occ[r] >> (8 * k)) & 0xFFull) == 0ull || key[r][k] == kEmptyKey) out[k] = kEmpty;
                 else if (key[r][k] >= kUnreachedKey) out[k] = kFree;
                 else out[k] = lab[(key[r][k] >> 2) & mask];
             }
