@@ -45,6 +45,27 @@ static_assert(kEmptyKey == kTileEmpty, "keys are copied into the tile as they ar
 // Halo z rows of 10 words (z = -1..8) at an odd stride, so that the row starts
 // of a wave's 64 lanes fall on distinct LDS banks.
 constexpr uint32_t kRowWords = 11;
+// Lanes whose halo-plane position (hy, hz) = (p / 10, p % 10) falls in (y, z) class c = cy * 3 + cz
+// (0: below the brick, 1: beside it, 2: above it): the first position p = lane, the second p = lane + 64
+// (lanes 0..35; lanes 36..63 repeat their first).
+constexpr uint32_t halo_class(uint32_t p)
+{
+    const uint32_t hy = p / 10u, hz = p % 10u;
+    return (hy == 0u ? 0u : (hy == 9u ? 2u : 1u)) * 3u + (hz == 0u ? 0u : (hz == 9u ? 2u : 1u));
+}
+constexpr uint64_t halo_lanes(uint32_t c, uint32_t off, uint32_t n)
+{
+    uint64_t m = 0;
+    for (uint32_t l = 0; l < n; ++l)
+        if (halo_class(l + off) == c) m |= 1ull << l;
+    return m;
+}
+constexpr uint64_t kHaloLanes1[9] = { halo_lanes(0, 0, 64), halo_lanes(1, 0, 64), halo_lanes(2, 0, 64),
+                                      halo_lanes(3, 0, 64), halo_lanes(4, 0, 64), halo_lanes(5, 0, 64),
+                                      halo_lanes(6, 0, 64), halo_lanes(7, 0, 64), halo_lanes(8, 0, 64) };
+constexpr uint64_t kHaloLanes2[9] = { halo_lanes(0, 64, 36), halo_lanes(1, 64, 36), halo_lanes(2, 64, 36),
+                                      halo_lanes(3, 64, 36), halo_lanes(4, 64, 36), halo_lanes(5, 64, 36),
+                                      halo_lanes(6, 64, 36), halo_lanes(7, 64, 36), halo_lanes(8, 64, 36) };
 constexpr uint32_t kTileWords = 10 * 10 * kRowWords;
 // 26-neighbour minima, separably: zt = z-window minima of the 100 halo-plane rows (8 per row),
 // yt = y-window minima of the 10 x-slabs' 8 rows; rows at an odd stride of 9 words.  yt reuses
@@ -139,6 +160,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const uint32_t bz = brick % R.nbz, t1 = brick / R.nbz, by = t1 % R.nby, bx = t1 / R.nby;
     const int X0 = (int)(bx * kRelaxBrick), Y0 = (int)(by * kRelaxBrick), Z0 = (int)(bz * kRelaxBrick);
 
+    uint32_t halo_live;  // 27 bits: halo classes holding a cell that is not EMPTY
     // brick + halo -> LDS (cells outside the grid and EMPTY cells as kTileEmpty).  Each lane owns
     // one or two (y, z) positions of the 10 x 10 halo plane and loads them in all 10 x slices at
     // once; addresses are clamped into the grid so every load is unconditional, and cells outside
@@ -166,12 +188,36 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         }
         // outside the grid -> all ones (EMPTY keys are all ones already)
         const uint32_t bad1 = in1 ? 0u : kTileEmpty, bad2 = in2 ? 0u : kTileEmpty;
+        // halo_live bit ix * 9 + iy * 3 + iz: a cell of that halo class (x, y, z each below the brick, beside
+        // it, above it) is not EMPTY; a neighbour brick in direction (ix, iy, iz) can only take a drop from
+        // this brick through those cells (EMPTY keys never change).  Per position and x class a ballot of
+        // "not all EMPTY" (across x slices 1..8 the AND of the keys is all ones iff all are EMPTY), then the
+        // lanes of each (y, z) class picked by constant masks.
+        uint32_t m1 = ~0u, m2 = ~0u;
+        uint64_t b1[3], b2[3];
 #pragma unroll
         for (uint32_t hx = 0; hx < 10u; ++hx) {
             const uint32_t xbad = (uint32_t)(X0 + (int)hx - 1) < nx ? 0u : kTileEmpty;
-            tile[hx * 10u * kRowWords + t1] = v1[hx] | bad1 | xbad;
-            tile[hx * 10u * kRowWords + t2] = v2[hx] | bad2 | xbad;
+            const uint32_t a1 = v1[hx] | bad1 | xbad, a2 = v2[hx] | bad2 | xbad;
+            tile[hx * 10u * kRowWords + t1] = a1;
+            tile[hx * 10u * kRowWords + t2] = a2;
+            if (hx == 0u || hx == 9u) {
+                b1[hx == 0u ? 0 : 2] = __ballot(a1 != ~0u);
+                b2[hx == 0u ? 0 : 2] = __ballot(a2 != ~0u);
+            } else {
+                m1 &= a1;
+                m2 &= a2;
+            }
         }
+        b1[1] = __ballot(m1 != ~0u);
+        b2[1] = __ballot(m2 != ~0u);
+        uint32_t live = 0;
+#pragma unroll
+        for (uint32_t ix = 0; ix < 3u; ++ix)
+#pragma unroll
+            for (uint32_t c = 0; c < 9u; ++c)
+                if ((b1[ix] & kHaloLanes1[c]) | (b2[ix] & kHaloLanes2[c])) live |= 1u << (ix * 9u + c);
+        halo_live = live;
     }
     __syncthreads();
 
@@ -345,6 +391,7 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         dirs = ((bzm[1] & kX0) ? 1u << 4 : 0u) | ((bzm[1] & kX7) ? 1u << 22 : 0u) | ((bzm[1] & kY0) ? 1u << 10 : 0u) |
                ((bzm[1] & kY7) ? 1u << 16 : 0u) | (bzm[0] ? 1u << 12 : 0u) | (bzm[2] ? 1u << 14 : 0u);
     }
+    dirs &= halo_live;  // a neighbour whose cells facing this brick are all EMPTY cannot take a drop from it
     if (lane < 27u && (dirs & (1u << lane))) {
         const int qx = (int)bx + (int)(lane / 9u) - 1, qy = (int)by + (int)((lane / 3u) % 3u) - 1,
                   qz = (int)bz + (int)(lane % 3u) - 1;
