@@ -303,7 +303,10 @@ int vf_batch_create(vf_batch** out);
 int vf_batch_destroy(vf_batch* b);
 /* Runs n jobs together: every stage is one launch over all jobs' tiles (the
  * flood runs all jobs' BFS levels in shared launches), captured in a HIP graph
- * per launch shape.  results: n entries. */
+ * per launch shape.  results: n entries.  Device state and graphs are kept
+ * while a call passes the same job descriptors as the last; host meshes are
+ * validated and uploaded again on every call, so buffers refilled in place
+ * are seen. */
 int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results);
 /* Plain form: the same jobs one after another, one job per launch. */
 int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results);
@@ -326,6 +329,13 @@ int vf_server_poll(vf_server* s, uint64_t ticket, vf_result* result);
 int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result);
 /* bytes currently reserved by admitted jobs and number of jobs in flight */
 int vf_server_stats(vf_server* s, size_t* reserved_bytes, uint32_t* in_flight, uint64_t* completed);
+/* Device memory audit of the scheduler: bytes reserved by admitted jobs (each
+ * job reserves its own grid buffers plus its share of the scratch its launches
+ * use), the engine's device allocations at the last allocation step, and the
+ * largest excess of allocations over reservations seen so far (0 when the
+ * budget accounting holds).  No reference counterpart: the reference has no
+ * server. */
+int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* max_unreserved_bytes);
 
 /* ---- autotuner ---------------------------------------------------------------- */
 /* Times each tunable stage over candidate block/tile shapes per grid-edge

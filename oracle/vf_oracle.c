@@ -344,14 +344,21 @@ void vfo_histogram(const uint16_t* grid, size_t n, uint32_t* counts)
 static inline int64_t seed_dist(int64_t dx, int64_t dy, int64_t dz, int dist)
 {
     /* NaiveFracturer distance lambdas (MF/Source/Fracturer/NaiveFracturer.cpp:12-23) and
-     * distance.glsl:1-20 in their exact integer form: squared Euclidean (sqrt is
-     * monotone and injective on these integers), L1 and L-infinity. */
+     * distance.glsl:1-20.  L1 and L-infinity are integers (exact as the reference's floats: below
+     * 2^24).  Euclidean is the reference's float sqrt(x*x + y*y + z*z): the sum of squares is an
+     * exact float (below 3 * 2047^2 < 2^24) and sqrtf rounds it correctly; the returned value is
+     * the bit pattern of that float, which orders as the float does (non-negative).  Distinct
+     * integers can share a float root (from 4197200 on), and such seeds tie as in the reference. */
     if (dx < 0) dx = -dx;
     if (dy < 0) dy = -dy;
     if (dz < 0) dz = -dz;
     if (dist == VFO_MANHATTAN) return dx + dy + dz;
     if (dist == VFO_CHEBYSHEV) return dx > dy ? (dx > dz ? dx : dz) : (dy > dz ? dy : dz);
-    return dx * dx + dy * dy + dz * dz;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    const float r = sqrtf(fx * fx + fy * fy + fz * fz);
+    uint32_t bits;
+    memcpy(&bits, &r, 4);
+    return (int64_t)bits;
 }
 
 void vfo_naive(const uint16_t* in, uint16_t* out, const uint32_t dims[3],
@@ -893,27 +900,30 @@ int vfo_seed_uniform(const uint16_t* grid, const uint32_t dims[3], uint32_t S,
     return 0;
 }
 
-static int64_t seed_dist2(const uint32_t* a, const uint32_t* b, int dist)
+static float merge_dist(const uint32_t* a, const uint32_t* b, int dist)
 {
-    int64_t dx = (int64_t)a[0] - b[0], dy = (int64_t)a[1] - b[1], dz = (int64_t)a[2] - b[2];
-    return seed_dist(dx, dy, dz, dist);
+    /* Seeder::mergeSeeds' float distances on vec3 coordinates (Seeder.cpp:126-140): glm::distance
+     * for Euclidean, sums and maxima of float absolute differences otherwise */
+    const float dx = (float)a[0] - (float)b[0], dy = (float)a[1] - (float)b[1], dz = (float)a[2] - (float)b[2];
+    if (dist == VFO_MANHATTAN) return fabsf(dx) + fabsf(dy) + fabsf(dz);
+    if (dist == VFO_CHEBYSHEV) return fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
+    return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
 int vfo_seed_merge(const uint32_t* frags, uint32_t F, uint32_t* seeds, uint32_t N, int dist)
 {
     /* Seeder::mergeSeeds (Seeder.cpp:115-152): each seed takes the label of its
      * nearest fragment seed (first on ties) with a per-fragment running sub-id
-     * in bits 8..15.  Integer distances order exactly as the reference's float
-     * ones (squared Euclidean for glm::distance). */
+     * in bits 8..15, with the reference's float distances and strict '<'. */
     uint32_t next[256] = { 0 };
     if (N && !F) return -1;
     for (uint32_t f = 0; f < F; ++f)
         if (frags[4 * f + 3] > 255) return -1;
     for (uint32_t i = 0; i < N; ++i) {
-        int64_t best = INT64_MAX;
+        float best = FLT_MAX;
         uint32_t nearest = 0;
         for (uint32_t f = 0; f < F; ++f) {
-            int64_t d = seed_dist2(seeds + 4 * i, frags + 4 * f, dist);
+            float d = merge_dist(seeds + 4 * i, frags + 4 * f, dist);
             if (d < best) { best = d; nearest = f; }
         }
         uint32_t w = frags[4 * nearest + 3];
@@ -1053,6 +1063,25 @@ int vfo_marching_cubes(const uint16_t* grid, const uint32_t dims[3],
 #define PADV(px, py, pz) (((px) <= 0 || (py) <= 0 || (pz) <= 0 || (px) >= N[0] - 1 || (py) >= N[1] - 1 || (pz) >= N[2] - 1) \
                             ? (uint16_t)VFO_FREE : grid[gidx((px) - 1, (py) - 1, (pz) - 1, dims)])
 
+    /* each label's bounding box in grid coordinates: a cube none of whose corners lies in it has
+     * configuration 255, which emits nothing, so the cube loop is limited to the cubes touching it */
+    int32_t* bb = (int32_t*)malloc(65536 * 6 * sizeof(int32_t));
+    for (uint32_t L = 0; L < 65536; ++L) {
+        bb[6 * L + 0] = bb[6 * L + 1] = bb[6 * L + 2] = INT32_MAX;
+        bb[6 * L + 3] = bb[6 * L + 4] = bb[6 * L + 5] = -1;
+    }
+    for (uint32_t x = 0; x < dims[0]; ++x)
+        for (uint32_t y = 0; y < dims[1]; ++y)
+            for (uint32_t z = 0; z < dims[2]; ++z) {
+                int32_t* b = bb + 6 * (grid[gidx(x, y, z, dims)] & VFO_MASK15);
+                if ((int32_t)x < b[0]) b[0] = (int32_t)x;
+                if ((int32_t)y < b[1]) b[1] = (int32_t)y;
+                if ((int32_t)z < b[2]) b[2] = (int32_t)z;
+                if ((int32_t)x > b[3]) b[3] = (int32_t)x;
+                if ((int32_t)y > b[4]) b[4] = (int32_t)y;
+                if ((int32_t)z > b[5]) b[5] = (int32_t)z;
+            }
+
     u64vec tri_edges = { 0 }, tri_meta = { 0 }, verts = { 0 };
     uint32_t nl = 0;
     uint16_t* labels = (uint16_t*)malloc(65536 * sizeof(uint16_t));
@@ -1064,9 +1093,14 @@ int vfo_marching_cubes(const uint16_t* grid, const uint32_t dims[3],
         if (want != 0xFFFFFFFFu && want != L) continue;
         labels[nl] = (uint16_t)L;
         size_t tbegin = tri_meta.n;
-        for (int64_t cx = 1; cx <= N[0] - 1; ++cx)
-            for (int64_t cy = 0; cy <= N[1] - 2; ++cy)
-                for (int64_t cz = 0; cz <= N[2] - 2; ++cz) {
+        /* cube (cx, cy, cz) has corners at grid x in {cx - 1, cx - 2}, y in {cy - 1, cy}, z in {cz - 1, cz} */
+        const int32_t* b = bb + 6 * L;
+        const int64_t x0 = b[0] + 1 > 1 ? b[0] + 1 : 1, x1 = b[3] + 2 < N[0] - 1 ? b[3] + 2 : N[0] - 1;
+        const int64_t y0 = b[1], y1 = b[4] + 1 < N[1] - 2 ? b[4] + 1 : N[1] - 2;
+        const int64_t z0 = b[2], z1 = b[5] + 1 < N[2] - 2 ? b[5] + 1 : N[2] - 2;
+        for (int64_t cx = x0; cx <= x1; ++cx)
+            for (int64_t cy = y0; cy <= y1; ++cy)
+                for (int64_t cz = z0; cz <= z1; ++cz) {
                     int conf = 0;
                     uint16_t c0 = 0;
                     for (int i = 0; i < 8; ++i) {
@@ -1148,6 +1182,7 @@ int vfo_marching_cubes(const uint16_t* grid, const uint32_t dims[3],
     free(tri_edges.v);
     free(tri_meta.v);
     free(counts);
+    free(bb);
 #undef PADV
     return 0;
 }

@@ -5,6 +5,7 @@ The oracle is the checker only: nothing in the product imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -12,7 +13,8 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_DIR = ROOT / "oracle"
-LIB_PATH = ORACLE_DIR / "libvf_oracle.so"
+# VFO_ORACLE_LIB selects another build of the same sources (the sanitizer build)
+LIB_PATH = Path(os.environ.get("VFO_ORACLE_LIB", ORACLE_DIR / "libvf_oracle.so"))
 
 _lib = None
 

@@ -122,19 +122,84 @@ def test_batch_marching_cubes_counts(vf):
         assert r["mc_triangles"] == mesh["tri"].shape[0] and r["mc_vertices"] == mesh["xyz"].shape[0]
 
 
-def test_server_results_match_alone(vf):
+def test_server_results_match_alone_and_oracle(vf):
     keep = []
     jobs, meta = mixed_jobs(vf, 16, 5, keep)
     alone = vf.Batch().run(jobs)
+    for m in meta:  # the served run below must write every output again
+        m["labels"][...] = 0xDEAD
+        m["hist"][...] = 0
     srv = vf.Server(64 << 30, max_slots=6)
     tickets = [srv.submit(j) for j in jobs]
     out = [srv.wait(t) for t in tickets]
     for a, o in zip(alone, out):
         assert o["status"] == 0 and o["checksum"] == a["checksum"] and o["occupied"] == a["occupied"]
         assert o["latency_ms"] >= o["queue_ms"] >= 0
+    for m, o in zip(meta, out):  # served results against the oracle, cell for cell
+        ref, occ = oracle_job(m)
+        np.testing.assert_array_equal(m["labels"], ref)
+        np.testing.assert_array_equal(m["hist"], O.histogram(ref))
+        assert o["occupied"] == occ and o["checksum"] == O.checksum(ref)
     st = srv.stats()
     assert st["in_flight"] == 0 and st["reserved_bytes"] == 0 and st["completed"] == 16
+    mem = srv.memory()
+    assert mem["max_unreserved_bytes"] == 0, mem  # allocations never exceeded the admitted jobs' reservations
     srv.close()
+
+
+def test_server_budget_covers_allocations_under_pressure(vf):
+    # a budget that admits only a few of these jobs at once: the scheduler's allocations (job blocks,
+    # tables, worklists, voxelizer and seed scratch) must stay within what the admitted jobs reserved
+    keep = []
+    jobs, meta = mixed_jobs(vf, 12, 17, keep, edges=(64, 128))
+    alone = vf.Batch().run(jobs)
+    srv = vf.Server(160 << 20, max_slots=8)
+    tickets = [srv.submit(j) for j in jobs]
+    out = [srv.wait(t) for t in tickets]
+    assert [o["checksum"] for o in out] == [a["checksum"] for a in alone]
+    mem = srv.memory()
+    assert mem["max_unreserved_bytes"] == 0, mem
+    assert mem["reserved_bytes"] == 0
+    srv.close()
+
+
+def test_batch_rereads_host_meshes_refilled_in_place(vf):
+    # same descriptors, new mesh contents: the batch uploads the host meshes again on every run
+    keep = []
+    jobs, meta = mixed_jobs(vf, 3, 41, keep, edges=(48, 64))
+    for j, m in zip(jobs, meta):
+        assert j.xyz == m["xyz"].ctypes.data  # the job points at the test's own buffer
+    b = vf.Batch()
+    first = b.run(jobs)
+    for m in meta:
+        m["xyz"][:, 0] *= np.float32(0.6)  # squeeze the vessels in x, in place
+    second = b.run(jobs)
+    for m, r1, r2 in zip(meta, first, second):
+        assert r2["status"] == 0
+        ref, occ = oracle_job(m)
+        np.testing.assert_array_equal(m["labels"], ref)
+        assert r2["occupied"] == occ and r1["occupied"] != occ
+    b.close()
+
+
+def test_device_mesh_with_bad_index_fails_alone(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 3, 43, keep, edges=(40, 56))
+    tri = meta[1]["tri"].copy()
+    tri[5, 1] = meta[1]["xyz"].shape[0]  # one past the last vertex
+    dev = vf.DeviceMesh(meta[1]["xyz"], tri)
+    bad = dev.job(meta[1]["dims"], *BOX, meta[1]["S"], meta[1]["seed"], meta[1]["algo"], meta[1]["dist"],
+                  noise_seed=meta[1]["noise_seed"], noise_amp=meta[1]["amp"], boundary_size=1)
+    res = vf.Batch().run([jobs[0], bad, jobs[2]])
+    assert res[1]["status"] == -1
+    good = vf.Batch().run([jobs[0], jobs[2]])
+    assert [res[0]["checksum"], res[2]["checksum"]] == [r["checksum"] for r in good]
+    with vf.Grid(meta[1]["dims"], *BOX) as g:
+        with pytest.raises(vf.VfError):
+            g.voxelize_device(dev)
+        with pytest.raises(vf.VfError):
+            g.voxelize(meta[1]["xyz"], tri)
+    dev.close()
 
 
 def test_server_budget_poison_cancel(vf):

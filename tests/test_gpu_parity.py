@@ -35,6 +35,25 @@ def test_voxelize_matches_oracle(vf, dims):
     assert g.count_occupied() == int(ref.sum())
 
 
+def test_set_aabb_then_voxelize_matches_oracle(vf):
+    # RegularGrid::setAABB: a new box and new dims on an existing grid (cells cleared, cell size = box / dims)
+    xyz, tri = vessel(vf, 12, 6000)
+    g = vf.Grid((40, 40, 40), *BOX)
+    g.voxelize(xyz, tri)
+    mn, mx, dims = [-1.2, -0.9, -1.1], [0.8, 1.15, 0.95], (52, 36, 60)
+    g.set_aabb(mn, mx, dims)
+    assert g.dims == dims
+    assert not g.read().any()  # cleared
+    g.voxelize(xyz, tri)
+    ref = O.voxelize(xyz, tri, mn, mx, dims)
+    assert ref.sum() > 0
+    np.testing.assert_array_equal(g.occupancy(), ref)
+    seeds = g.seed_uniform(5, 13)
+    g.fracture_naive(seeds, 0)
+    np.testing.assert_array_equal(g.read(), O.naive(O.occ_to_grid(ref), seeds, 0))
+    g.close()
+
+
 def test_voxelize_merges_into_nonempty_grid(vf):
     dims = (48, 48, 48)
     g, xyz, tri = voxelized(vf, dims, seed=3, tris=4000)

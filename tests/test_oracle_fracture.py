@@ -52,6 +52,51 @@ def test_naive_matches_float_reference_semantics(dist, seed):
     np.testing.assert_array_equal(got, naive_numpy_float(g, seeds, dist))
 
 
+def naive_float32_cells(cells, seeds, dist):
+    """Labels of the given cells by the reference's float arithmetic carried out in float32
+    (coordinate differences, squares summed in order, sqrt), first minimum wins."""
+    c = cells.astype(np.float32)
+    s = seeds[:, :3].astype(np.float32)
+    d = c[:, None, :] - s[None, :, :]
+    if dist == EUC:
+        D = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+    elif dist == MAN:
+        D = np.abs(d[..., 0]) + np.abs(d[..., 1]) + np.abs(d[..., 2])
+    else:
+        D = np.maximum(np.abs(d[..., 0]), np.maximum(np.abs(d[..., 1]), np.abs(d[..., 2])))
+    assert D.dtype == np.float32
+    return seeds[np.argmin(D, axis=1), 3]
+
+
+def test_naive_euclid_float_roots_tie_as_in_reference():
+    # two squared distances one apart whose float32 roots are equal: the reference's strict '<'
+    # keeps the first seed although the second is nearer in exact arithmetic
+    grid = np.zeros((2048, 2048, 1), np.uint16)
+    grid[0, 0, 0] = 1
+    seeds = np.array([[659, 1951, 0, 2], [616, 1965, 0, 3]], np.uint32)
+    d2 = [int(s[0]) ** 2 + int(s[1]) ** 2 for s in seeds]
+    assert d2 == [4240682, 4240681]
+    assert np.sqrt(np.float32(d2[0])) == np.sqrt(np.float32(d2[1]))
+    got = O.naive(grid, seeds, EUC)
+    assert got[0, 0, 0] == 2
+    assert naive_float32_cells(np.array([[0, 0, 0]]), seeds, EUC)[0] == 2
+
+
+@pytest.mark.parametrize("dist", [EUC, MAN, CHE])
+def test_naive_large_grid_matches_float32_reference(dist):
+    # a 2048 x 2048 x 2 grid: squared distances up to 8.4M, where float roots of distinct integers collide
+    rng = np.random.default_rng(11)
+    shape = (2048, 2048, 2)
+    grid = np.zeros(shape, np.uint16)
+    cells = rng.integers(0, [2048, 2048, 2], size=(4000, 3))
+    grid[cells[:, 0], cells[:, 1], cells[:, 2]] = 1
+    seeds = random_seeds(grid, 96, 12)
+    got = O.naive(grid, seeds, dist)
+    occ = np.argwhere(grid != 0)
+    np.testing.assert_array_equal(got[occ[:, 0], occ[:, 1], occ[:, 2]], naive_float32_cells(occ, seeds, dist))
+    assert np.count_nonzero(got) == len(occ)
+
+
 @pytest.mark.parametrize("dist", [EUC, MAN, CHE])
 def test_naive_invariants(dist):
     occ = random_occupancy((16, 16, 16), 0.3, 7)

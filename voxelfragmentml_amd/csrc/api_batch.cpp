@@ -12,6 +12,7 @@
 //                  stages together, lets new flood jobs join the shared round
 //                  launches at their own round, and retires drained jobs, so
 //                  work of jobs that arrived at different times overlaps.
+#include <algorithm>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -133,7 +134,19 @@ static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
         b->tune_gen = gen;
     }
     const uint64_t sig = job_signature(jobs, n);
-    if (sig == b->sig && b->jobs.size() == n) return;
+    if (sig == b->sig && b->jobs.size() == n) {
+        // the same descriptors: device state is reused, but host meshes are read again (a caller may
+        // have refilled the buffers in place), after validating them again
+        bool same = true;
+        for (uint32_t i = 0; i < n && same; ++i) {
+            std::string why;
+            if (!jobs[i].mesh_on_device) same = job_validate(jobs[i], why) == b->jobs[i].vstatus;
+        }
+        if (same) {
+            for (auto* e : b->live) b->eng.upload_mesh(*e);
+            return;
+        }
+    }
     b->clear();
     b->jobs.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -141,7 +154,7 @@ static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
         e.spec = jobs[i];
         std::string why;
         e.res = vf_result{};
-        e.res.status = job_validate(e.spec, why);
+        e.res.status = e.vstatus = job_validate(e.spec, why);
         e.res.voxels = (uint64_t)e.spec.dims[0] * e.spec.dims[1] * e.spec.dims[2];
         if (e.res.status == VF_OK) b->live.push_back(&e);
     }
@@ -278,6 +291,8 @@ struct vf_server {
     std::deque<EJob*> queue;
     std::unordered_map<uint64_t, EJob*> jobs;  // every ticket not yet collected
     size_t reserved = 0;
+    size_t device_bytes = 0;          // engine allocations at the last audit
+    size_t max_unreserved = 0;        // largest excess of engine allocations over reserved bytes seen
     uint32_t in_flight = 0;
     uint64_t completed = 0;
     uint64_t next_ticket = 1;
@@ -294,6 +309,14 @@ void vf_server::loop()
     (void)hipSetDevice(device);
     try {
         Engine E;
+        // engine allocations never exceed the bytes reserved by admitted jobs: checked after every step
+        // that allocates (job blocks, tables, worklists, front scratch)
+        auto audit = [&] {
+            const size_t dev = E.device_bytes();
+            std::lock_guard<std::mutex> lk(mu);
+            device_bytes = dev;
+            if (dev > reserved) max_unreserved = std::max(max_unreserved, dev - reserved);
+        };
         std::vector<EJob*> leveling;  // flood jobs between front and drain
         Table* lev = nullptr;
         bool lev_dirty = false;
@@ -359,8 +382,10 @@ void vf_server::loop()
                 }
                 Table* T = E.make_table(admitted);
                 E.reserve_front(*T);
+                audit();
                 E.front(*T);
                 E.drop_table(T);
+                E.release_front();  // admitted jobs reserve their share of it only while they are in flight
                 std::vector<EJob*> naive_done;
                 for (auto* e : admitted) {
                     if (e->d.algo == VF_FLOOD) {
@@ -372,6 +397,7 @@ void vf_server::loop()
                 }
                 if (!naive_done.empty()) {
                     Table* P = E.make_table(naive_done);
+                    audit();
                     E.post(*P);
                     VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * naive_done.size(), hipMemcpyDeviceToHost, E.s));
                     VF_HIP(hipStreamSynchronize(E.s));
@@ -390,6 +416,7 @@ void vf_server::loop()
                     if (lev) E.drop_table(lev);
                     lev = E.make_table(leveling);
                     lev_dirty = false;
+                    audit();
                 }
                 E.rounds(*lev, kServerRounds);
                 E.gather(*lev, E.pinned);
@@ -403,6 +430,7 @@ void vf_server::loop()
                 }
                 if (!drained.empty()) {
                     Table* P = E.make_table(drained);
+                    audit();
                     E.post(*P);
                     VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * drained.size(), hipMemcpyDeviceToHost, E.s));
                     VF_HIP(hipStreamSynchronize(E.s));
@@ -472,7 +500,7 @@ int vf_server_submit(vf_server* s, const vf_job* job, uint64_t* ticket)
         e->spec = *job;
         std::string why;
         const int st = job_validate(e->spec, why);
-        if (!e->spec.mesh_on_device && e->spec.nt) {
+        if (st == VF_OK && !e->spec.mesh_on_device && e->spec.nt) {
             e->xyz_copy.assign(job->xyz, job->xyz + 3 * (size_t)job->nv);
             e->tri_copy.assign(job->tri, job->tri + 3 * (size_t)job->nt);
             e->spec.xyz = e->xyz_copy.data();
@@ -544,6 +572,17 @@ int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result)
         return VF_ERR_INVALID_ARG;
     }
     return collect(s, ticket, result, true);
+}
+
+int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* max_unreserved_bytes)
+{
+    return guard([&] {
+        if (!s) fail(VF_ERR_INVALID_ARG, "server is NULL");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (reserved_bytes) *reserved_bytes = s->reserved;
+        if (device_bytes) *device_bytes = s->device_bytes;
+        if (max_unreserved_bytes) *max_unreserved_bytes = s->max_unreserved;
+    });
 }
 
 int vf_server_stats(vf_server* s, size_t* reserved_bytes, uint32_t* in_flight, uint64_t* completed)

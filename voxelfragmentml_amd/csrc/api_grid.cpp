@@ -4,6 +4,7 @@
 // support), launches the multi-job kernels with a one-job table, and reports
 // failures through vf_last_error().
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -40,6 +41,24 @@ uint32_t key_bits_for(uint32_t S)
     uint32_t b = 1;
     while ((1ull << b) < S) ++b;
     return b;
+}
+
+NaiveKeyForm naive_key_form(const uint32_t dims[3], int distance, uint32_t S)
+{
+    const uint32_t b = key_bits_for(S);
+    const uint64_t ex = dims[0] - 1, ey = dims[1] - 1, ez = dims[2] - 1;
+    const uint64_t dmax = distance == VF_MANHATTAN ? ex + ey + ez
+                          : distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
+                                                     : ex * ex + ey * ey + ez * ez;
+    NaiveKeyForm f{ distance, 0 };
+    if (distance == VF_EUCLIDEAN && dmax > kSqrtInjectiveMax) {
+        f.dist = kDistEuclidF32;  // float root bits (below 2^31) << b: 64-bit keys
+        f.key64 = 1;
+        return f;
+    }
+    // 32-bit keys pack d * 2^b + i with a 24-bit multiply: 2^b must fit 24 bits
+    f.key64 = b > 23 || ((dmax + 1) << b) > 0xFFFFFFFFull ? 1 : 0;
+    return f;
 }
 
 static void grid_setup_shape(vf_grid* g, const uint32_t dims[3], const float mn[3], const float mx[3])
@@ -123,6 +142,7 @@ void check_device_error(vf_grid* g, const char* stage)
     if (e & kDevErrSeeding) fail(VF_ERR_SEEDING, "%s: fewer candidate cells than seeds", stage);
     if (e & kDevErrKeyOverflow) fail(VF_ERR_OVERFLOW, "%s: flood distance exceeds the packed key range", stage);
     if (e & kDevErrRelax) fail(VF_ERR_OVERFLOW, "%s: flood brick did not settle", stage);
+    if (e & kDevErrMesh) fail(VF_ERR_INVALID_ARG, "%s: a triangle references a vertex index >= nv", stage);
     fail(VF_ERR_OVERFLOW, "%s: device error flag 0x%llx", stage, (unsigned long long)e);
 }
 
@@ -300,6 +320,7 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
         J.xyz = d_xyz;
         J.tri = d_tri;
         J.nt = nt;
+        J.nv = nv;
         // RegularGrid::fill only sets occupied cells to FREE; on a grid that
         // already holds values the new occupancy is merged afterwards.
         const bool merge = !g->clean;
@@ -327,7 +348,7 @@ int vf_voxelize_device(vf_grid* g, const float* d_xyz, uint32_t nv, const uint32
         X.scan_tmp = g->scan_tmp.p;
         X.scan_tmp_bytes = g->scan_tmp.bytes;
         launch_voxelize(L, X, g->ctl.as<uint32_t>() + 2, groups, nullptr);
-        (void)nv;
+        check_device_error(g, "voxelize");
         if (merge) {
             // labels = new ? FREE : old, occupancy = old | new
             std::vector<uint16_t> old(g->V);
@@ -437,22 +458,22 @@ int vf_seed_merge(const vf_seed* frags, uint32_t F, vf_seed* seeds, uint32_t N, 
         if (distance < VF_EUCLIDEAN || distance > VF_CHEBYSHEV) fail(VF_ERR_INVALID_ARG, "unknown distance %d", distance);
         for (uint32_t f = 0; f < F; ++f)
             if (frags[f].w > 255) fail(VF_ERR_INVALID_ARG, "fragment label %u does not fit 8 bits", frags[f].w);
-        // integer distances order as the reference's float ones (squared Euclidean for glm::distance:
-        // the float square root of an integer below 2^24 is strictly increasing)
-        auto d = [&](const vf_seed& a, const vf_seed& b) -> int64_t {
-            const int64_t dx = std::llabs((int64_t)a.x - b.x), dy = std::llabs((int64_t)a.y - b.y),
-                          dz = std::llabs((int64_t)a.z - b.z);
-            if (distance == VF_EUCLIDEAN) return dx * dx + dy * dy + dz * dz;
-            if (distance == VF_MANHATTAN) return dx + dy + dz;
-            return std::max(dx, std::max(dy, dz));
+        // the reference's float distances on vec3 coordinates (glm::distance for Euclidean), compiled
+        // without contraction: integer coordinates below 2^24 convert exactly, sums are rounded in
+        // the order written
+        auto d = [&](const vf_seed& a, const vf_seed& b) -> float {
+            const float dx = (float)a.x - (float)b.x, dy = (float)a.y - (float)b.y, dz = (float)a.z - (float)b.z;
+            if (distance == VF_EUCLIDEAN) return std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (distance == VF_MANHATTAN) return std::fabs(dx) + std::fabs(dy) + std::fabs(dz);
+            return std::max(std::fabs(dx), std::max(std::fabs(dy), std::fabs(dz)));
         };
         uint32_t next[256] = {};
         std::vector<uint32_t> label(N);
         for (uint32_t i = 0; i < N; ++i) {
             uint32_t nearest = 0;
-            int64_t best = INT64_MAX;
+            float best = FLT_MAX;
             for (uint32_t f = 0; f < F; ++f) {
-                const int64_t v = d(seeds[i], frags[f]);
+                const float v = d(seeds[i], frags[f]);
                 if (v < best) {
                     best = v;
                     nearest = f;
@@ -474,17 +495,12 @@ int vf_fracture_naive(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         validate_seeds(g, seeds, S);
         if (S == 0) return;  // buildCPU leaves every cell untouched without seeds
         upload_seeds(g, seeds, S);
-        const uint32_t b = key_bits_for(S);
-        const uint64_t ex = g->dims[0] - 1, ey = g->dims[1] - 1, ez = g->dims[2] - 1;
-        uint64_t dmax = distance == VF_MANHATTAN ? ex + ey + ez
-                        : distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
-                                                   : ex * ex + ey * ey + ez * ez;
-        // 32-bit keys pack d * 2^b + i with a 24-bit multiply: 2^b must fit 24 bits
-        const bool key64 = b > 23 || ((dmax + 1) << b) > 0xFFFFFFFFull;
-        g->job.key_bits = b;
-        g->job.dist = distance;
+        const NaiveKeyForm kf = naive_key_form(g->dims, distance, S);
+        const bool key64 = kf.key64 != 0;
+        g->job.key_bits = key_bits_for(S);
+        g->job.dist = kf.dist;
         g->job.algo = VF_NAIVE;
-        g->job.naive_key64 = key64 ? 1 : 0;
+        g->job.naive_key64 = kf.key64;
         g->job_dirty = true;
         launch_naive(grid_launch(g), key64);
         g->labels_from_occ = false;

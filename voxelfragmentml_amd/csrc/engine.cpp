@@ -56,7 +56,23 @@ Layout layout_of(const vf_job& j)
 
 }  // namespace
 
-size_t job_footprint(const vf_job& j) { return layout_of(j).total; }
+size_t job_footprint(const vf_job& j)
+{
+    const Layout Lo = layout_of(j);
+    const uint64_t tiles = ceil_div(Lo.V, kTileCells);
+    // the job's share of the engine scratch its launches use: the voxelizer's per-triangle setup records,
+    // work counts and their 64-bit scans, the seed-candidate counts per tile and their scan (each buffer
+    // at least 256 bytes plus 64 bytes of slack), the launch table's descriptor and prefix entries (with
+    // the alignment of its nine arrays), and the flood worklists (2 sets x 8 XCD lists of
+    // ceil(tasks / 8) x 4096 eight-byte entries: 64 KiB per task, plus one rounding step)
+    const size_t front = (size_t)j.nt * (sizeof(TriSetup) + 8 + 16) + scan_tmp_bytes(2ull * j.nt) +
+                         (size_t)tiles * 12 + scan_tmp_bytes(tiles) + 6 * (256 + 64);
+    const size_t table = sizeof(JobDev) + 16 * 4 + 6 * 4 + 9 * kAlign;
+    const size_t lists = j.algorithm == VF_FLOOD
+                             ? 65536ull * relax_tasks(j.dims[0], j.dims[1], j.dims[2]) + 2ull * kFloodXcds * 4096 * 8
+                             : 0;
+    return Lo.total + front + table + lists;
+}
 
 int job_validate(const vf_job& j, std::string& why)
 {
@@ -110,6 +126,7 @@ void Engine::alloc(EJob& e)
     const Layout Lo = layout_of(j);
     e.bytes = Lo.total;
     VF_HIP(hipMallocAsync(&e.mem, e.bytes, s));
+    live_bytes_ += e.bytes;
     auto* base = static_cast<uint8_t*>(e.mem);
     JobDev& d = e.d;
     d = JobDev{};
@@ -147,11 +164,9 @@ void Engine::alloc(EJob& e)
         d.bflags = reinterpret_cast<unsigned long long*>(base + Lo.bricks);
         d.nbricks = relax_bricks(d.nx, d.ny, d.nz);
     } else {
-        const uint64_t ex = d.nx - 1, ey = d.ny - 1, ez = d.nz - 1;
-        const uint64_t dmax = j.distance == VF_MANHATTAN ? ex + ey + ez
-                              : j.distance == VF_CHEBYSHEV ? std::max(ex, std::max(ey, ez))
-                                                           : ex * ex + ey * ey + ez * ez;
-        d.naive_key64 = d.key_bits > 23 || ((dmax + 1) << d.key_bits) > 0xFFFFFFFFull ? 1 : 0;
+        const NaiveKeyForm kf = naive_key_form(j.dims, j.distance, j.num_seeds);
+        d.dist = kf.dist;
+        d.naive_key64 = kf.key64;
     }
     if (j.mesh_on_device) {
         d.xyz = j.xyz;
@@ -159,18 +174,29 @@ void Engine::alloc(EJob& e)
     } else {
         d.xyz = reinterpret_cast<const float*>(base + Lo.mesh_xyz);
         d.tri = reinterpret_cast<const uint32_t*>(base + Lo.mesh_tri);
-        if (j.nv) VF_HIP(hipMemcpyAsync(base + Lo.mesh_xyz, j.xyz, (size_t)j.nv * 12, hipMemcpyHostToDevice, s));
-        if (j.nt) VF_HIP(hipMemcpyAsync(base + Lo.mesh_tri, j.tri, (size_t)j.nt * 12, hipMemcpyHostToDevice, s));
+        upload_mesh(e);
     }
     d.nt = j.nt;
+    d.nv = j.nv;
     e.words = Lo.words;
     e.tiles = ceil_div(Lo.V, kTileCells);
     e.groups = ceil_div(Lo.words, 256);
 }
 
+void Engine::upload_mesh(EJob& e)
+{
+    const vf_job& j = e.spec;
+    if (j.mesh_on_device) return;
+    if (j.nv) VF_HIP(hipMemcpyAsync(const_cast<float*>(e.d.xyz), j.xyz, (size_t)j.nv * 12, hipMemcpyHostToDevice, s));
+    if (j.nt) VF_HIP(hipMemcpyAsync(const_cast<uint32_t*>(e.d.tri), j.tri, (size_t)j.nt * 12, hipMemcpyHostToDevice, s));
+}
+
 void Engine::release(EJob& e)
 {
-    if (e.mem) VF_HIP(hipFreeAsync(e.mem, s));
+    if (e.mem) {
+        VF_HIP(hipFreeAsync(e.mem, s));
+        live_bytes_ -= e.bytes;
+    }
     e.mem = nullptr;
 }
 
@@ -225,6 +251,8 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     bp[n] = (uint32_t)btiles;
     fp[n] = (uint32_t)fchunks;
     VF_HIP(hipMallocAsync(&t->dev, total, s));
+    t->bytes = total;
+    live_bytes_ += total;
     VF_HIP(hipMemcpyAsync(t->dev, img.data(), total, hipMemcpyHostToDevice, s));
     VF_HIP(hipStreamSynchronize(s));  // the pageable source must outlive the copy
     auto* base = static_cast<uint8_t*>(t->dev);
@@ -247,6 +275,8 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     if (fchunks) {
         t->flood.list_cap = flood_list_cap((uint32_t)fchunks);
         VF_HIP(hipMallocAsync(&t->flood_lists, 2ull * kFloodXcds * t->flood.list_cap * 8, s));
+        t->bytes += 2ull * kFloodXcds * t->flood.list_cap * 8;
+        live_bytes_ += 2ull * kFloodXcds * t->flood.list_cap * 8;
         t->flood.lists = static_cast<unsigned long long*>(t->flood_lists);
     }
     return t;
@@ -257,6 +287,7 @@ void Engine::drop_table(Table* t)
     if (!t) return;
     if (t->dev) VF_HIP(hipFreeAsync(t->dev, s));
     if (t->flood_lists) VF_HIP(hipFreeAsync(t->flood_lists, s));
+    live_bytes_ -= t->bytes;
     delete t;
 }
 
@@ -269,6 +300,24 @@ void Engine::reserve_front(const Table& t)
     scan_tmp_.ensure(std::max(scan_tmp_bytes(2 * (uint64_t)nt), scan_tmp_bytes(t.L.total_tiles)) + 64);
     seed_cnt_.ensure((size_t)t.L.total_tiles * 4 + 64);
     seed_off_.ensure(((size_t)t.L.total_tiles + 1) * 8 + 64);
+}
+
+void Engine::release_front()
+{
+    vox_setup_.release();
+    vox_counts_.release();
+    vox_shell_off_.release();
+    scan_tmp_.release();
+    seed_cnt_.release();
+    seed_off_.release();
+}
+
+size_t Engine::device_bytes() const
+{
+    size_t n = live_bytes_;
+    for (const DevBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) n += b->bytes;
+    for (const DevBuf& b : mc_.buf) n += b.bytes;
+    return n;
 }
 
 // Launch-only (safe inside stream capture): reserve_front must have sized the scratch.
@@ -333,7 +382,8 @@ void Engine::finish(EJob& e, const uint32_t* w)
     r.flood_sweeps = (uint32_t)(stat(kStatFloodWork) >> 32);
     const uint64_t err = stat(kStatError);
     r.status = VF_OK;
-    if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;
+    if (err & kDevErrMesh) r.status = VF_ERR_INVALID_ARG;
+    else if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;
     else if (err) r.status = VF_ERR_OVERFLOW;
     if (r.status == VF_OK && e.spec.marching_cubes) {
         vf_mesh_info info{};

@@ -31,12 +31,15 @@ struct EJob {
     uint64_t ticket = 0;
     double t_submit = 0, t_admit = 0;
     int phase = 0;
+    int vstatus = 0;                 // job_validate's status when the job was prepared
     bool cancelled = false;
     bool admitted = false;           // server: holds a slot and budget until completed
 };
 
-// Device bytes a job needs while resident (grid, occupancy, bitmasks, flood
-// keys and buckets, seeds, histogram, mesh copy).
+// Device bytes a job needs while resident: its own block (grid, occupancy,
+// bitmasks, flood keys and brick flags, seeds, histogram, mesh copy) plus its
+// share of the engine scratch its launches use (voxelizer setup records and
+// scans, seed counts, launch table entries, flood worklists).
 size_t job_footprint(const vf_job& j);
 // VF_OK or the status the job fails with before touching the device.
 int job_validate(const vf_job& j, std::string& why);
@@ -55,6 +58,7 @@ struct Table {
     FloodRoundScratch flood{};      // chunk prefix + worklists of the flood rounds
     int bucket = 0;                 // launch-shape bucket (largest grid edge of the table)
     void* flood_lists = nullptr;    // separate allocation of flood.lists
+    size_t bytes = 0;               // device bytes of dev and flood_lists
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
 };
 
@@ -69,10 +73,14 @@ public:
     int device = 0;
 
     void alloc(EJob& j);    // stream-ordered allocation + descriptor + mesh upload
+    void upload_mesh(EJob& j);  // host mesh -> the job's device copy (nothing for device meshes)
     void release(EJob& j);  // stream-ordered free
     Table* make_table(const std::vector<EJob*>& js);
     void drop_table(Table* t);
     void reserve_front(const Table& t);  // size the front scratch (not capturable)
+    void release_front();                // free the front scratch (the next reserve_front allocates it again)
+    // device bytes held: job blocks, launch tables, worklists and scratch buffers
+    size_t device_bytes() const;
     void front(Table& t);               // launches only
     void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
@@ -84,6 +92,7 @@ public:
     static constexpr uint32_t kMaxTableJobs = 1024;
 
 private:
+    size_t live_bytes_ = 0;  // stream-ordered allocations: job blocks and tables
     DevBuf vox_setup_, vox_counts_, vox_shell_off_, scan_tmp_, seed_cnt_, seed_off_;
     McScratch mc_;
 };

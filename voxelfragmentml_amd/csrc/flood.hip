@@ -175,16 +175,18 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
         const uint32_t o1 = (uint32_t)min(max(gy1, 0), (int)ny - 1) * nz + (uint32_t)min(max(gz1, 0), (int)nz - 1);
         const uint32_t o2 = (uint32_t)min(max(gy2, 0), (int)ny - 1) * nz + (uint32_t)min(max(gz2, 0), (int)nz - 1);
         const uint32_t t1 = hy1 * kRowWords + hz1, t2 = hy2 * kRowWords + hz2;  // tile offsets within an x slice
-        const uint32_t slice = ny * nz;
-        // byte offsets from the keys base fit 32 bits (grids below 2^30 cells): a uniform base plus a
-        // 32-bit lane offset per load
+        const uint64_t slice = (uint64_t)ny * nz;
+        // each x slice's base address is uniform (64-bit, scalar); the lane offsets within a slice
+        // (below 2^22 cells, 2^24 bytes) are 32-bit: a scalar base plus a 32-bit offset per load
+        const uint32_t ob1 = o1 * 4u, ob2 = o2 * 4u;
         const VF_AS_GLOBAL uint8_t* kbytes = (const VF_AS_GLOBAL uint8_t*)R.keys;
         uint32_t v1[10], v2[10];
 #pragma unroll
         for (uint32_t hx = 0; hx < 10u; ++hx) {
-            const uint32_t xs = (uint32_t)min(max(X0 + (int)hx - 1, 0), (int)nx - 1) * slice;
-            v1[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o1) * 4u);
-            v2[hx] = *(const VF_AS_GLOBAL uint32_t*)(kbytes + (xs + o2) * 4u);
+            const uint32_t sx = uni((uint32_t)min(max(X0 + (int)hx - 1, 0), (int)nx - 1));
+            const VF_AS_GLOBAL uint8_t* sb = kbytes + sx * slice * 4u;
+            v1[hx] = *(const VF_AS_GLOBAL uint32_t*)(sb + ob1);
+            v2[hx] = *(const VF_AS_GLOBAL uint32_t*)(sb + ob2);
         }
         // outside the grid -> all ones (EMPTY keys are all ones already)
         const uint32_t bad1 = in1 ? 0u : kTileEmpty, bad2 = in2 ? 0u : kTileEmpty;

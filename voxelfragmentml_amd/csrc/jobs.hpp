@@ -35,6 +35,7 @@ enum DevError : uint32_t {
     kDevErrSeeding = 1,     // fewer candidates than seeds
     kDevErrKeyOverflow = 2, // flood distance no longer fits the packed key
     kDevErrRelax = 4,       // a flood brick did not settle within its sweep limit
+    kDevErrMesh = 8,        // a triangle references a vertex index >= nv (the triangle is skipped)
 };
 
 struct TriSetup {
@@ -55,13 +56,13 @@ struct JobDev {
     // voxelizer input / scratch
     const float* xyz;
     const uint32_t* tri;
-    uint32_t nt;
+    uint32_t nt, nv;
     uint32_t* shell_bits;   // nx*ny*wp words
     uint32_t* toggle_bits;  // nx*ny*wp words
     // seeds {x,y,z,label}
     uint4* seeds;
     uint32_t S;
-    int dist;
+    int dist;               // VF_* distance, or kDistEuclidF32 for naive Euclidean ranked by float roots
     uint32_t key_bits;      // bits for the seed index in packed keys
     int neighbourhood;      // 6 or 26
     uint64_t noise_seed;
@@ -90,6 +91,19 @@ __host__ __device__ inline uint32_t flood_weight(uint64_t noise_seed, uint32_t c
 }
 
 uint32_t key_bits_for(uint32_t S);  // bits of the seed index in packed keys
+
+// Naive Euclidean distances: the reference ranks seeds by the float sqrt of the exact integer
+// squared distance (NaiveFracturer.cpp:12-15).  Below kSqrtInjectiveMax distinct integers have
+// distinct float roots, so the squared distance orders identically; grids whose largest squared
+// distance exceeds it rank by the float root's bit pattern (distance code kDistEuclidF32, 64-bit
+// keys), where distinct squared distances can tie as they do in the reference.
+constexpr uint64_t kSqrtInjectiveMax = 4197200;  // sqrtf(4197200) == sqrtf(4197201)
+constexpr int kDistEuclidF32 = 3;
+struct NaiveKeyForm {
+    int dist;   // distance code the kernel evaluates (VF_* or kDistEuclidF32)
+    int key64;  // packed keys need 64 bits
+};
+NaiveKeyForm naive_key_form(const uint32_t dims[3], int distance, uint32_t S);
 
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t njobs, uint32_t t)
 {

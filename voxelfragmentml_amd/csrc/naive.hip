@@ -4,7 +4,9 @@
 //
 // Every non-EMPTY cell takes the label of argmin over seeds of (distance,
 // seed index).  Distances are exact integers: squared Euclidean, L1, L-inf
-// (the reference's float distances on integer offsets order identically).
+// (the reference's float distances on integer offsets order identically),
+// except Euclidean on grids large enough for float roots to collide, which
+// ranks by the float root itself.
 // The argmin is a single unsigned min over packed keys (distance << b | i),
 // so ties go to the lowest index exactly as the reference's strict '<' does.
 // Seeds are staged through LDS in chunks; each tile's occupied cells are
@@ -30,7 +32,36 @@ __device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
 // coordinates are below 2^11 (grid edges of at most 2048 cells), so the halves never wrap.
 __device__ inline uint32_t sad16(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u16(a, b, acc); }
 
-// Distance key of one cell to one seed; xy and sxy hold x | y << 16.
+// Bits of the correctly rounded float square root of an integer n < 2^24, exactly: the estimate
+// moves to a float neighbour while sqrt(n) lies beyond the midpoint towards it.  Midpoints of
+// adjacent floats have at most 25 significant bits, so their sums and squares are exact in double,
+// and no midpoint squares to an integer below 2^48, so there are no ties.
+__device__ inline uint32_t sqrt_rn_bits(uint32_t n)
+{
+    uint32_t b = __float_as_uint(__builtin_sqrtf((float)n));
+    const double dn = (double)n;
+    for (int it = 0; it < 4; ++it) {
+        const double f = (double)__uint_as_float(b);
+        if (b > 0u) {
+            const double lo = 0.5 * ((double)__uint_as_float(b - 1u) + f);
+            if (lo * lo > dn) {
+                --b;
+                continue;
+            }
+        }
+        const double hi = 0.5 * (f + (double)__uint_as_float(b + 1u));
+        if (hi * hi < dn) {
+            ++b;
+            continue;
+        }
+        break;
+    }
+    return b;
+}
+
+// Distance key of one cell to one seed; xy and sxy hold x | y << 16.  kDistEuclidF32 (64-bit keys
+// only) ranks by the bits of the correctly rounded float root of the squared distance, which is
+// below 2^24 and so converts exactly: the reference's float sqrt (NaiveFracturer.cpp:12-15).
 template <int DIST, typename K>
 __device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t xy, uint32_t sx, uint32_t sy, uint32_t sz,
                              uint32_t sxy, uint32_t id, uint32_t b)
@@ -43,6 +74,7 @@ __device__ inline K seed_key(uint32_t x, uint32_t y, uint32_t z, uint32_t xy, ui
         if (DIST == VF_CHEBYSHEV) d = max(ax, max(ay, az));
         else  // offsets < 2^11: 16-bit operands select the full-rate 24-bit multiply
             d = (ax & 0xFFFFu) * (ax & 0xFFFFu) + (ay & 0xFFFFu) * (ay & 0xFFFFu) + (az & 0xFFFFu) * (az & 0xFFFFu);
+        if (DIST == kDistEuclidF32) d = sqrt_rn_bits(d);
     }
     return pack_key<DIST, K>(d, id, b);
 }
@@ -214,7 +246,8 @@ __global__ __launch_bounds__(kCellThreads) __attribute__((amdgpu_waves_per_eu(si
                     if (gqmax == 0) continue;  // staging above stays block-wide
                     if (dist == VF_MANHATTAN) naive_cells<VF_MANHATTAN, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
                     else if (dist == VF_CHEBYSHEV) naive_cells<VF_CHEBYSHEV, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
-                    else naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
+                    else if (dist == VF_EUCLIDEAN) naive_cells<VF_EUCLIDEAN, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
+                    else if constexpr (sizeof(K) == 8) naive_cells<kDistEuclidF32, K>(best, cx, cy, cz, gqmax, sx, sy, sz, base, m, b);
                 }
 #pragma unroll
                 for (uint32_t q = 0; q < 4; ++q) {

@@ -223,9 +223,18 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
         for (int q = 0; q < 3; ++q) S.lo[q] = S.hi[q] = S.X[q] = S.Y[q] = S.Z[q] = 0;
         uint32_t nshell = 0, ncol = 0;
         const uint32_t t = g - sp[j];
+        uint32_t vi[3] = { 0, 0, 0 };
+        bool in_range = true;
         if (valid && J->active) {
-            const float* p[3] = { J->xyz + 3 * (size_t)J->tri[3 * (size_t)t], J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 1],
-                                  J->xyz + 3 * (size_t)J->tri[3 * (size_t)t + 2] };
+            for (int k = 0; k < 3; ++k) {
+                vi[k] = J->tri[3 * (size_t)t + k];
+                in_range &= vi[k] < J->nv;
+            }
+            if (!in_range)  // a device mesh with a bad index: flag the job and skip the triangle
+                atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrMesh);
+        }
+        if (valid && J->active && in_range) {
+            const float* p[3] = { J->xyz + 3 * (size_t)vi[0], J->xyz + 3 * (size_t)vi[1], J->xyz + 3 * (size_t)vi[2] };
             float P[3][3];
             bool finite = true;
             for (int k = 0; k < 3; ++k)

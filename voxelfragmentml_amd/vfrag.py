@@ -101,6 +101,7 @@ def lib():
         "vf_server_submit": (I, [P, P, C.POINTER(U64)]), "vf_server_cancel": (I, [P, U64]),
         "vf_server_poll": (I, [P, U64, P]), "vf_server_wait": (I, [P, U64, P]),
         "vf_server_stats": (I, [P, C.POINTER(C.c_size_t), C.POINTER(U32), C.POINTER(U64)]),
+        "vf_server_memory": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "vf_device_alloc": (I, [C.c_size_t, C.POINTER(P)]), "vf_device_free": (I, [P]),
         "vf_memcpy_htod": (I, [P, P, C.c_size_t]),
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
@@ -221,6 +222,9 @@ class Grid:
         xyz = np.ascontiguousarray(xyz, np.float32)
         tri = np.ascontiguousarray(tri, np.uint32)
         check(lib().vf_voxelize(self._h, _ptr(xyz), xyz.shape[0], _ptr(tri), tri.shape[0]))
+
+    def voxelize_device(self, mesh: "DeviceMesh"):
+        check(lib().vf_voxelize_device(self._h, mesh.d_xyz, mesh.nv, mesh.d_tri, mesh.nt))
 
     def seed_uniform(self, n: int, seed: int, location: int = SEED_BOTH) -> np.ndarray:
         out = np.zeros((n, 4), np.uint32)
@@ -468,6 +472,11 @@ class Server:
         b, f, c = C.c_size_t(), C.c_uint32(), C.c_uint64()
         check(lib().vf_server_stats(self._h, C.byref(b), C.byref(f), C.byref(c)))
         return {"reserved_bytes": b.value, "in_flight": f.value, "completed": c.value}
+
+    def memory(self):
+        r, d, m = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(lib().vf_server_memory(self._h, C.byref(r), C.byref(d), C.byref(m)))
+        return {"reserved_bytes": r.value, "device_bytes": d.value, "max_unreserved_bytes": m.value}
 
     def close(self):
         if self._h:
