@@ -222,6 +222,27 @@ int vf_mesh_smooth(vf_grid* g, uint32_t nonboundary_iterations, float nonboundar
  * Writes at most cap bytes; *size gets the full size. */
 int vf_mesh_export_binm(vf_grid* g, uint32_t label_index, uint8_t* out, size_t cap, size_t* size);
 
+/* ---- no device allocation --------------------------------------------------- */
+/* Largest grid, and the device scratch held per GPU, of vf_fracture_naive_noalloc. */
+#define VF_NOALLOC_MAX_CELLS (1ull << 24)
+#define VF_NOALLOC_SCRATCH_BYTES (66560ull + VF_NOALLOC_MAX_CELLS)
+/* vf_fracture_naive on a caller-owned device grid, with no device allocation
+ * and no allocating library call.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops, harnesses that
+ * forbid hipMalloc, code that must not allocate while it runs).
+ * d_cells: device pointer (16-byte aligned) to dims[0]*dims[1]*dims[2] uint16
+ * cells in the reference's layout, relabelled in place as vf_fracture_naive
+ * relabels a grid; seeds, S and distance as for vf_fracture_naive (host
+ * array); stream: a hipStream_t, or NULL for the default stream; the call
+ * returns when the work on that stream has finished.  Everything beyond the
+ * cells (the grid's occupancy bytes, the launch descriptor, the seeds) lives in
+ * one static device array of VF_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends: grids of more than VF_NOALLOC_MAX_CELLS cells fail with
+ * VF_ERR_INVALID_ARG.  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU race. */
+int vf_fracture_naive_noalloc(uint16_t* d_cells, const uint32_t dims[3], const vf_seed* seeds, uint32_t S,
+                              int distance, void* stream);
+
 /* ---- grid file formats ----------------------------------------------------- */
 /* RegularGrid::exportRLE (RegularGrid.cpp:672-714): uvec3 dims + {uint16 value,
  * uint32 count} runs.  Writes at most cap bytes; *size gets the full size. */

@@ -105,7 +105,10 @@ void drain_locked()
 }
 }  // namespace
 
-bool timing_enabled() { return g_timing; }
+thread_local int t_timing_off = 0;
+bool timing_enabled() { return g_timing && !t_timing_off; }
+TimingOff::TimingOff() { ++t_timing_off; }
+TimingOff::~TimingOff() { --t_timing_off; }
 
 bool capturing(hipStream_t s)
 {

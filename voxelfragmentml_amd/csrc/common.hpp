@@ -57,6 +57,11 @@ bool timing_enabled();
 void timing_begin(const char* name, hipStream_t s);
 void timing_end(const char* name, hipStream_t s);
 bool capturing(hipStream_t s);
+// While one lives, the calling thread's launches record no timing events (which would create events).
+struct TimingOff {
+    TimingOff();
+    ~TimingOff();
+};
 
 struct TimedLaunch {
     const char* name;
