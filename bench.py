@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
-"""Fragmented Gvoxels/s of the voxelize -> fracture path on one MI355X.
+"""Fragmented Gvoxels/s and p50/p99 job latency of the voxelize -> fracture
+path on one MI355X.
 
-Workload (default): the mixed batch — 64 independent fracture jobs, each a
+Headline workload: the mixed batch — 64 independent fracture jobs, each a
 closed lathe vessel of 5k-300k triangles voxelized to a cubic grid with edge
 uniform in [64, 384] (rounded to a multiple of 4, as the reference rounds its
 voxelization sizes), 2-64 seeds drawn uniformly from the occupied cells,
@@ -12,17 +13,24 @@ timed region.
 
 One step = one pass of every job through the path.  `value` is the batched
 form (every stage one launch over all 64 jobs, captured in HIP graphs):
-sum of grid cells (n^3) over the jobs / wall time.  The same JSON line carries
-the plain form (one job per launch), the served form (Poisson arrivals with
-continuous batching: p50/p99 latency), the roofline of the dominant kernel
-and the CPU oracle's rate on a bounded sample.
+sum of grid cells (n^3) over the jobs / wall time.  The same JSON line
+carries the plain form (one job per launch), the served form (a Poisson
+stream of 10k requests at 200 jobs/s, each drawn fresh from the mixed
+distribution extended to 512^3 and 128 seeds, with continuous batching:
+p50/p99 latency, and a sample checked against the oracle), the single-grid
+configurations (128^3 naive, 256^3 flood, 512^3 naive + flood + marching
+cubes) with per-stage times, the roofline of the dominant kernel and of
+every stage, and the reference logic (the CPU oracle) timed on the host's
+cores.
 """
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
-import math
 import os
+import platform
+import statistics
 import sys
 import threading
 import time
@@ -37,9 +45,10 @@ METRIC = "fragmented Gvoxels/s + p50/p99 job latency, batched/served vs plain, b
 BOX = ([-1.03, -1.03, -1.03], [1.03, 1.03, 1.03])
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 summary of the default workload: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a
-# 1 GiB copy's counts, written as "traffic_bytes_per_launch <kernel> <bytes>"
+# plain copy's counts, written as "traffic_bytes_per_launch <kernel> <bytes>"
 TRAFFIC_SUMMARY = ROOT / "profiles" / "flood_relax_traffic_mixed64.txt"
 DEFAULT_JOBS = 64
+HOST_WORKERS = 16  # the host's CPU share of a one-GPU box
 
 
 def log(*a):
@@ -69,56 +78,55 @@ def measured_traffic(kernel):
     return None
 
 
+def make_spec_job(vf, mesh, s, mc=False):
+    e = s["edge"]
+    return mesh.job((e, e, e), *BOX, s["seeds"], s["seed_key"], s["algo"], s["dist"],
+                    noise_seed=s["noise_seed"], noise_amp=2 if s["algo"] == 1 else 0, boundary_size=1,
+                    marching_cubes=1 if mc else 0)
+
+
 def build_jobs(vf, specs, mc=False):
     meshes, jobs = [], []
     for s in specs:
         xyz, tri = vf.vessel_mesh(s["mesh_seed"], *vf.vessel_segments(s["tris"]))
         m = vf.DeviceMesh(xyz, tri)
         meshes.append(m)
-        e = s["edge"]
-        jobs.append(m.job((e, e, e), *BOX, s["seeds"], s["seed_key"], s["algo"], s["dist"],
-                          noise_seed=s["noise_seed"], noise_amp=2 if s["algo"] == 1 else 0, boundary_size=1,
-                          marching_cubes=1 if mc else 0))
+        jobs.append(make_spec_job(vf, m, s, mc))
     return meshes, jobs
 
 
-# Algorithmic HBM bytes of each stage per step (the minimum traffic of the
-# stage's definition, not of this implementation): cell stages count the
-# bytes per cell they must read and write (labels 2 B, occupancy 1 B, flood
-# keys 4 B); the voxelizer counts the triangle vertices (36 B per triangle);
-# a flood relaxation round counts 4 KiB per brick visit (its 512 keys read
-# and written once).
-RELAX_BYTES_PER_VISIT = 512 * 4 * 2
+# ---------------------------------------------------------------- roofline ---
+# Algorithmic HBM bytes per stage, the fixed definitions of the stage (not of
+# this implementation): voxelize 36 B per triangle (FP32 vertices read) + 1 B
+# per cell (occupancy written); naive or flood 3 B per cell (occupancy read,
+# uint16 labels written); boundaries 4 B per cell (labels read and written);
+# histogram 2 B per cell (labels read); marching cubes 2 B per cell + 12 B per
+# vertex and per triangle.  A kernel that fuses stages counts each stage's bytes.
+STAGES = {
+    "voxelize": ["job_init", "vox_clear", "vox_setup", "scan", "vox_shell", "vox_parity", "vox_fill"],
+    "seed": ["seed_count", "seed_pick"],
+    "naive": ["naive", "naive64"],
+    "flood": ["flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan", "flood_finish"],
+    "boundaries+histogram": ["boundaries_hist", "hist_summary"],
+    "marching_cubes": ["mc_count", "mc_emit", "mc_finish"],
+}
+KERNELS = [k for ks in STAGES.values() for k in ks] + ["gather"]
 
 
-def kernel_bytes(name, specs, results=None):
+def stage_bytes(specs, mc_out=None):
     V = [s["edge"] ** 3 for s in specs]
-    T = [s["tris"] for s in specs]
     Vn = sum(v for v, s in zip(V, specs) if s["algo"] == 0)
     Vf = sum(v for v, s in zip(V, specs) if s["algo"] == 1)
-    table = {
-        "naive": 3 * Vn, "naive64": 3 * Vn,             # occupancy in, labels out
-        "boundaries_hist": 4 * sum(V),                  # labels in and out
-        "checksum": 2 * sum(V),                         # labels in
-        "vox_fill": sum(V),                             # occupancy out
-        "vox_setup": 36 * sum(T), "vox_shell": 36 * sum(T), "vox_parity": 36 * sum(T),
-        "seed_count": sum(V),                           # occupancy in
-        "flood_init": 5 * Vf,                           # occupancy in, keys out
-        "flood_finish": 6 * Vf,                         # keys in, labels out
-    }
-    if name == "flood_relax" and results is not None:
-        return RELAX_BYTES_PER_VISIT * sum(r["flood_bricks"] for r in results)
-    return table.get(name)
+    out = {"voxelize": 36 * sum(s["tris"] for s in specs) + sum(V), "seed": 0, "naive": 3 * Vn, "flood": 3 * Vf,
+           "boundaries+histogram": 4 * sum(V) + 2 * sum(V)}
+    if mc_out is not None:
+        out["marching_cubes"] = 2 * sum(V) + 12 * mc_out
+    return out
 
 
 def path_bytes(specs):
-    """Algorithmic bytes of the whole path per step: voxelize 36 T + V, fracture 3 V, boundaries 4 V."""
-    return sum(8 * s["edge"] ** 3 + 36 * s["tris"] for s in specs)
-
-
-KERNELS = ["job_init", "vox_clear", "vox_setup", "scan", "vox_shell", "vox_parity", "vox_fill", "seed_count",
-           "seed_pick", "naive", "naive64", "flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan",
-           "flood_finish", "boundaries_hist", "hist_summary", "checksum", "mc_count", "mc_emit", "mc_finish"]
+    """Per step: voxelize 36 T + V, fracture 3 V, boundaries 4 V, histogram 2 V."""
+    return sum(10 * s["edge"] ** 3 + 36 * s["tris"] for s in specs)
 
 
 def kernel_profile(vf, jobs, steps):
@@ -140,22 +148,66 @@ def kernel_profile(vf, jobs, steps):
     return prof, res
 
 
-def served(vf, specs, rate, n_req, budget, slots, seed):
-    """Poisson arrivals at `rate` jobs/s; returns latencies and throughput."""
-    meshes, jobs = build_jobs(vf, specs)
+def roofline(prof, specs, res, traffic_ok):
+    sb = stage_bytes(specs)
+    stages = {}
+    for st, ks in STAGES.items():
+        ms = sum(prof[k]["total_ms"] for k in ks if k in prof)
+        if ms <= 0 or st not in sb:
+            continue
+        gbs = sb[st] / (ms * 1e-3) / 1e9
+        stages[st] = {"ms_per_step": round(ms, 4), "algorithmic_bytes": sb[st], "achieved_gbs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max((k for k in prof if k != "gather"), key=lambda k: prof[k]["total_ms"])
+    st = next(s for s, ks in STAGES.items() if dom in ks)
+    launches = prof[dom]["launches"]
+    per_launch = sb[st] / launches  # the stage's bytes spread over the dominant kernel's launches
+    achieved = per_launch / (prof[dom]["avg_ms"] * 1e-3) / 1e9
+    out = {"kernel": dom, "stage": st, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "bytes_per_launch": per_launch, "avg_launch_ms": prof[dom]["avg_ms"], "launches_per_step": launches,
+           "definition": f"{st} stage bytes per step ({'3 B per flood cell' if st == 'flood' else 'per-stage rule'}) "
+                         f"/ launches per step / average launch time (HIP events on the launch stream)"}
+    if dom == "flood_relax":
+        visits = sum(r["flood_bricks"] for r in res)
+        out["limiter"] = "VALU/SALU issue (SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_BUSY_CYCLES), not HBM"
+        out["unit_work"] = {"brick_visits_per_step": visits, "visit_bytes_per_step": 4096 * visits,
+                            "sweeps_per_step": sum(r["flood_sweeps"] for r in res),
+                            "visit_gbs": round(4096 * visits / launches / (prof[dom]["avg_ms"] * 1e-3) / 1e9, 1)}
+    traffic = measured_traffic(dom) if traffic_ok else None
+    if traffic:
+        out["traffic"] = traffic
+        out["traffic_source"] = (f"{TRAFFIC_SUMMARY.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                 "of this workload, each scaled by a plain copy's count")
+    return out, stages
+
+
+# ------------------------------------------------------------------ served ---
+def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n):
+    """Poisson arrivals at `rate` jobs/s.  Each request is a fresh draw from the mixed distribution extended
+    to 512^3 and 128 seeds (edge, seeds, distance, algorithm, seed and noise keys), on a mesh taken from a pool
+    of `pool_size` device-resident vessels of 5k-300k triangles (uploaded before the stream starts)."""
+    pool_specs = job_specs(pool_size, seed + 1000)
+    pool, host = [], []
+    for s in pool_specs:
+        xyz, tri = vf.vessel_mesh(s["mesh_seed"], *vf.vessel_segments(s["tris"]))
+        pool.append(vf.DeviceMesh(xyz, tri))
+        host.append((xyz, tri))
     rng = np.random.default_rng(seed)
+    reqs = job_specs(n_req, seed, emax=512, smax=128)
+    which = rng.integers(0, pool_size, size=n_req)
+    for s, w in zip(reqs, which):
+        s["tris"] = pool_specs[w]["tris"]
+    jobs = [make_spec_job(vf, pool[w], s) for s, w in zip(reqs, which)]
     gaps = rng.exponential(1.0 / rate, size=n_req)
-    order = rng.integers(0, len(jobs), size=n_req)
     srv = vf.Server(budget, slots)
     tickets, results = [None] * n_req, [None] * n_req
-    done = threading.Event()
 
     def collect():
         for i in range(n_req):
             while tickets[i] is None:
                 time.sleep(0.0002)
             results[i] = srv.wait(tickets[i])
-        done.set()
 
     th = threading.Thread(target=collect)
     th.start()
@@ -166,61 +218,201 @@ def served(vf, specs, rate, n_req, budget, slots, seed):
         d = t_next - time.perf_counter()
         if d > 0:
             time.sleep(d)
-        tickets[i] = srv.submit(jobs[order[i]])
+        tickets[i] = srv.submit(jobs[i])
+        if (i + 1) % 2000 == 0:
+            log(f"served: {i + 1} of {n_req} requests submitted")
     th.join()
     t1 = time.perf_counter()
+    mem = srv.memory()
     srv.close()
     lat = np.array([r["latency_ms"] for r in results])
-    ok = all(r["status"] == 0 for r in results)
-    vox = sum(specs[o]["edge"] ** 3 for o in order)
+    vox = sum(s["edge"] ** 3 for s in reqs)
     out = {
         "requests": n_req, "rate_jobs_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-        "gvoxels_per_s": vox / (t1 - t0) / 1e9, "wall_s": t1 - t0, "all_ok": ok,
-        "hbm_budget_gb": budget / 2**30, "slots": slots,
+        "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "gvoxels_per_s": vox / (t1 - t0) / 1e9, "wall_s": t1 - t0, "all_ok": all(r["status"] == 0 for r in results),
+        "hbm_budget_gb": budget / 2**30, "slots": slots, "mesh_pool": pool_size,
+        "distribution": "edge U[64,512] (x4), seeds U[2,128], distance uniform, naive/flood 50:50, flood noise 2, "
+                        "boundaries + histogram",
+        "max_unreserved_bytes": mem["max_unreserved_bytes"],
     }
-    # each served result equals the same job run alone (bit-exact)
-    sample = sorted(set(int(o) for o in order[:8]))
-    alone = vf.Batch().run([jobs[o] for o in sample])
-    first = {}
-    for i, o in enumerate(order):
-        first.setdefault(int(o), results[i]["checksum"])
-    out["alone_equals_served"] = all(first[o] == a["checksum"] for o, a in zip(sample, alone))
-    for m in meshes:
+    # a sample of requests: the served result equals the job run alone and the oracle
+    pick = sorted(rng.choice(n_req, size=min(check_n, n_req), replace=False).tolist())
+    alone = vf.Batch().run([jobs[i] for i in pick])
+    out["alone_equals_served"] = all(a["checksum"] == results[i]["checksum"] for a, i in zip(alone, pick))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle as O
+
+    def ref(i):
+        s = reqs[i]
+        xyz, tri = host[which[i]]
+        e = s["edge"]
+        g = O.occ_to_grid(O.voxelize(xyz, tri, *BOX, (e, e, e)))
+        seeds = O.seed_uniform(g, s["seeds"], s["seed_key"])
+        lab = O.naive(g, seeds, s["dist"]) if s["algo"] == 0 else O.flood(g, seeds, 6 if s["dist"] == 1 else 26,
+                                                                         s["noise_seed"], 2)
+        return O.checksum(O.detect_boundaries(lab, 1))
+
+    t2 = time.perf_counter()
+    with cf.ThreadPoolExecutor(HOST_WORKERS) as ex:
+        refs = list(ex.map(ref, pick))
+    out["oracle_sample"] = {"jobs": len(pick), "equal": sum(int(r == results[i]["checksum"]) for r, i in zip(refs, pick)),
+                            "seconds": round(time.perf_counter() - t2, 1)}
+    for m in pool:
         m.close()
     return out
 
 
-def cpu_baseline(specs, budget_s):
-    """The CPU oracle (single thread) on the same jobs, in order, until the
-    time budget is used: voxelize, seed, fracture, boundaries, histogram."""
+# ---------------------------------------------------------- configurations ---
+CONFIGS = [
+    dict(name="128^3 naive, 8 seeds, Manhattan (~20k-triangle vessel)", edge=128, tris=20000, mesh_seed=1, seeds=8,
+         seed_key=1, dist=1, algos=[0], mc=False),
+    dict(name="256^3 flood, 32 seeds, Euclidean -> 26 neighbours, noise 2, histogram (~200k triangles)", edge=256,
+         tris=200000, mesh_seed=2, seeds=32, seed_key=2, dist=0, algos=[1], mc=False),
+    dict(name="512^3 naive and flood, 128 seeds, Chebyshev, marching cubes (~1M triangles)", edge=512,
+         tris=1000000, mesh_seed=3, seeds=128, seed_key=3, dist=2, algos=[0, 1], mc=True),
+]
+
+
+def config_runs(vf, reps):
+    """Single-grid form of each configuration: every call synchronises, so the wall time of each call is its
+    stage time.  Median of `reps` runs per stage."""
+    out = []
+    for c in CONFIGS:
+        e = c["edge"]
+        dims = (e, e, e)
+        xyz, tri = vf.vessel_mesh(c["mesh_seed"], *vf.vessel_segments(c["tris"]))
+        mesh = vf.DeviceMesh(xyz, tri)
+        times = {}
+
+        def clock(name, fn):
+            t0 = time.perf_counter()
+            r = fn()
+            times.setdefault(name, []).append((time.perf_counter() - t0) * 1e3)
+            return r
+
+        mc_info = None
+        with vf.Grid(dims, *BOX) as g:
+            for rep in range(reps + 1):
+                g.set_aabb(*BOX, dims)  # RegularGrid::setAABB: an empty grid, so the voxelizer writes it whole
+                clock("voxelize", lambda: g.voxelize_device(mesh))
+                base = g.read()
+                seeds = clock("seed", lambda: g.seed_uniform(c["seeds"], c["seed_key"]))
+                for a in c["algos"]:
+                    if a != c["algos"][0]:
+                        g.write(base)
+                    if a == 0:
+                        clock("naive", lambda: g.fracture_naive(seeds, c["dist"]))
+                    else:
+                        clock("flood", lambda: g.fracture_flood(seeds, c["dist"], noise_seed=c["seed_key"],
+                                                                noise_amp=2))
+                clock("boundaries", lambda: g.detect_boundaries(1))
+                clock("histogram", lambda: g.histogram())
+                if c["mc"]:
+                    mc_info = clock("marching_cubes", lambda: g.marching_cubes_info())
+                if rep == 0:
+                    times.clear()  # the first pass warms up
+        med = {k: statistics.median(v) for k, v in times.items()}
+        total = sum(med.values())
+        V = e ** 3
+        occ = int((base != 0).sum())
+        # batched form of the same job (flood when the configuration runs it)
+        job = mesh.job(dims, *BOX, c["seeds"], c["seed_key"], c["algos"][-1], c["dist"], noise_seed=c["seed_key"],
+                       noise_amp=2 if c["algos"][-1] == 1 else 0, boundary_size=1, marching_cubes=1 if c["mc"] else 0)
+        bt = []
+        with vf.Batch() as b:
+            b.run([job])
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                b.run([job])
+                bt.append((time.perf_counter() - t0) * 1e3)
+        entry = {"config": c["name"], "voxels": V, "occupied": occ, "stage_ms": {k: round(v, 3) for k, v in med.items()},
+                 "plain_gvoxels_per_s": round(V * len(c["algos"]) / (total * 1e-3) / 1e9, 3),
+                 "plain_occupied_gvoxels_per_s": round(occ * len(c["algos"]) / (total * 1e-3) / 1e9, 4),
+                 "batched_ms": round(statistics.median(bt), 3),
+                 "batched_gvoxels_per_s": round(V / (statistics.median(bt) * 1e-3) / 1e9, 3)}
+        if mc_info:
+            entry["mc_triangles"] = mc_info["num_triangles"]
+            entry["mc_vertices"] = mc_info["num_vertices"]
+            entry["mc_triangles_per_s"] = round(mc_info["num_triangles"] / (med["marching_cubes"] * 1e-3), 1)
+        out.append(entry)
+        log(f"config {c['name']}: {entry}")
+        mesh.close()
+    return out
+
+
+# -------------------------------------------------------------- CPU leg ---
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(specs, n_jobs, runs, single_jobs):
+    """The oracle (the reference logic restated in C) on the host's cores: jobs of the mixed workload spread
+    over a pool of threads (the oracle's C calls release the interpreter lock), median wall time of `runs`
+    passes over the same jobs, per-stage CPU seconds; plus one single-threaded pass over fewer jobs."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle as O
     from voxelfragmentml_amd import vfrag as vf
 
-    done, vox, t_all = [], 0, 0.0
-    for i, s in enumerate(specs):
-        xyz, tri = vf.vessel_mesh(s["mesh_seed"], *vf.vessel_segments(s["tris"]))
+    sample = specs[:n_jobs]
+    meshes = [vf.vessel_mesh(s["mesh_seed"], *vf.vessel_segments(s["tris"])) for s in sample]
+    lock = threading.Lock()
+
+    def one(i, stages):
+        s, (xyz, tri) = sample[i], meshes[i]
         e = s["edge"]
-        t0 = time.perf_counter()
+        t = [time.perf_counter()]
         occ = O.voxelize(xyz, tri, *BOX, (e, e, e))
+        t.append(time.perf_counter())
         g = O.occ_to_grid(occ)
         seeds = O.seed_uniform(g, s["seeds"], s["seed_key"])
+        t.append(time.perf_counter())
         if s["algo"] == 0:
             lab = O.naive(g, seeds, s["dist"])
         else:
             lab = O.flood(g, seeds, 6 if s["dist"] == 1 else 26, s["noise_seed"], 2)
+        t.append(time.perf_counter())
         b = O.detect_boundaries(lab, 1)
+        t.append(time.perf_counter())
         O.histogram(b)
-        t_all += time.perf_counter() - t0
-        vox += e ** 3
-        done.append(i)
-        if t_all > budget_s:
-            break
-    return {"value": vox / t_all / 1e9, "unit": "Gvoxels/s", "cores": 1, "kind": "port",
-            "sample": f"jobs {done[0]}-{done[-1]} of the mixed workload ({len(done)} jobs, {vox / 1e6:.1f} Mvoxels, "
-                      f"{t_all:.1f} s), single-threaded C oracle, full path without marching cubes",
-            "host_cpus": os.cpu_count()}
+        t.append(time.perf_counter())
+        names = ["voxelize", "seed", "naive" if s["algo"] == 0 else "flood", "boundaries", "histogram"]
+        with lock:
+            for k, a, c in zip(names, t, t[1:]):
+                stages[k] = stages.get(k, 0.0) + (c - a)
+
+    workers = min(HOST_WORKERS, len(os.sched_getaffinity(0)))
+    walls, stages = [], {}
+    for r in range(runs):
+        st = {}
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(workers) as ex:
+            list(ex.map(lambda i: one(i, st), range(len(sample))))
+        walls.append(time.perf_counter() - t0)
+        stages = st
+        log(f"cpu baseline: pass {r + 1} of {runs}: {walls[-1]:.2f} s")
+    vox = sum(s["edge"] ** 3 for s in sample)
+    st1 = {}
+    t0 = time.perf_counter()
+    for i in range(single_jobs):
+        one(i, st1)
+    t1 = time.perf_counter() - t0
+    vox1 = sum(s["edge"] ** 3 for s in sample[:single_jobs])
+    return {"value": vox / statistics.median(walls) / 1e9, "unit": "Gvoxels/s", "cores": workers, "kind": "port",
+            "sample": f"jobs 0-{len(sample) - 1} of the mixed workload ({len(sample)} jobs, {vox / 1e6:.1f} Mvoxels), "
+                      f"oracle C restatement (gcc -O2) on {workers} threads (one job per thread at a time), median "
+                      f"of {runs} passes; voxelize, seed, fracture, boundaries, histogram",
+            "pass_seconds": [round(w, 3) for w in walls],
+            "stage_cpu_seconds": {k: round(v, 3) for k, v in stages.items()},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "single_thread": {"value": vox1 / t1 / 1e9, "jobs": single_jobs, "seconds": round(t1, 2),
+                              "stage_cpu_seconds": {k: round(v, 3) for k, v in st1.items()}}}
 
 
 def main():
@@ -229,10 +421,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--jobs", type=int, default=DEFAULT_JOBS)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--skip", default="", help="comma list of: tune,plain,served,cpu,profile")
+    ap.add_argument("--serve-check", type=int, default=32, help="served requests compared with the oracle")
+    ap.add_argument("--cpu-jobs", type=int, default=32)
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--skip", default="", help="comma list of: tune,plain,served,cpu,profile,configs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a reference dispatch of known "
                          "traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -273,12 +467,10 @@ def main():
 
     # ---- batched form (the headline)
     b = vf.Batch()
-    for _ in range(a.warmup):
+    ref = None
+    for _ in range(max(1, a.warmup)):
         ref = b.run(jobs)
-    if a.warmup == 0:
-        ref = None
-    assert all(r["status"] == 0 for r in (ref or b.run(jobs))), "a job failed"
-    ref = ref or b.run(jobs)
+    assert all(r["status"] == 0 for r in ref), "a job failed"
     t0 = time.perf_counter()
     for _ in range(a.steps):
         res = b.run(jobs)
@@ -286,9 +478,11 @@ def main():
     ms_step = (t1 - t0) * 1e3 / a.steps
     value = voxels / ((t1 - t0) / a.steps) / 1e9
     deterministic = [r["checksum"] for r in res] == [r["checksum"] for r in ref]
+    occupied = sum(r["occupied"] for r in res)
     log(f"batched: {ms_step:.2f} ms/step, {value:.2f} Gvoxels/s, deterministic={deterministic}")
     b.close()
 
+    pb = path_bytes(specs)
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gvoxels/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -296,11 +490,11 @@ def main():
         "config": {"workload": "mixed batch: 64 vessel jobs, edge U[64,384] (x4), 5k-300k triangles, 2-64 seeds, "
                                "distance uniform, naive/flood 50:50, flood noise 2, boundaries + histogram",
                    "jobs": len(specs), "voxels_per_step": voxels, "rng_seed": 4, "form": "batched (HIP graphs)"},
-        "batched": {"gvoxels_per_s": value, "ms_per_step": ms_step, "deterministic": deterministic},
-        "path_bytes": {"algorithmic_bytes_per_step": path_bytes(specs),
-                       "achieved_gbs": round(path_bytes(specs) / (ms_step * 1e-3) / 1e9, 2),
-                       "frac_of_hbm_peak": round(path_bytes(specs) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "definition": "8 B per cell + 36 B per triangle per job"},
+        "batched": {"gvoxels_per_s": value, "ms_per_step": ms_step, "deterministic": deterministic,
+                    "occupied_gvoxels_per_s": occupied / (ms_step * 1e-3) / 1e9},
+        "path_bytes": {"algorithmic_bytes_per_step": pb, "achieved_gbs": round(pb / (ms_step * 1e-3) / 1e9, 2),
+                       "frac_of_hbm_peak": round(pb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "definition": "per job: voxelize 36 T + V, fracture 3 V, boundaries 4 V, histogram 2 V"},
         "tuning": tuning,
     }
 
@@ -320,32 +514,19 @@ def main():
     if "profile" not in skip:
         prof, pres = kernel_profile(vf, jobs, 2)
         out["kernels_ms_per_step"] = {k: round(v["total_ms"], 4) for k, v in prof.items()}
-        cand = {k: v for k, v in prof.items() if kernel_bytes(k, specs, pres)}
-        dom = max(cand, key=lambda k: cand[k]["total_ms"])
-        per_launch = kernel_bytes(dom, specs, pres) / max(1.0, prof[dom]["launches"])
-        achieved = per_launch / (prof[dom]["avg_ms"] * 1e-3) / 1e9
-        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                           "bytes_per_launch": per_launch, "avg_launch_ms": prof[dom]["avg_ms"],
-                           "launches_per_step": prof[dom]["launches"]}
-        traffic = measured_traffic(dom) if a.jobs == DEFAULT_JOBS else None
-        if traffic:
-            out["roofline"]["traffic"] = traffic
-            out["roofline"]["traffic_source"] = (f"{TRAFFIC_SUMMARY.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE and "
-                                                 "WRITE_SIZE of this workload, each scaled by a 1 GiB copy's count")
-        if dom == "flood_relax":
-            visits = sum(r["flood_bricks"] for r in pres)
-            out["roofline"]["unit_work"] = {"brick_visits_per_step": visits, "bytes_per_visit": RELAX_BYTES_PER_VISIT,
-                                            "sweeps_per_step": sum(r["flood_sweeps"] for r in pres)}
-        log(f"dominant kernel {dom}: {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+        out["roofline"], out["stage_roofline"] = roofline(prof, specs, pres, a.jobs == DEFAULT_JOBS)
+        log(f"roofline: {out['roofline']}")
+        log(f"stages: {out['stage_roofline']}")
+
+    if "configs" not in skip:
+        out["configs"] = config_runs(vf, 3)
 
     if "served" not in skip:
-        sspecs = job_specs(48, 5, emax=512, smax=128, tmax=300000)
-        out["served"] = served(vf, sspecs, a.serve_rate, a.serve_requests, 200 << 30, 64, 5)
+        out["served"] = served(vf, a.serve_rate, a.serve_requests, 200 << 30, 64, 5, 64, a.serve_check)
         log(f"served: {out['served']}")
 
     if "cpu" not in skip:
-        out["cpu_baseline"] = cpu_baseline(specs, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(specs, a.cpu_jobs, a.cpu_runs, 4)
         log(f"cpu baseline: {out['cpu_baseline']}")
 
     for m in meshes:

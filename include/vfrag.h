@@ -351,12 +351,15 @@ int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result);
 /* bytes currently reserved by admitted jobs and number of jobs in flight */
 int vf_server_stats(vf_server* s, size_t* reserved_bytes, uint32_t* in_flight, uint64_t* completed);
 /* Device memory audit of the scheduler: bytes reserved by admitted jobs (each
- * job reserves its own grid buffers plus its share of the scratch its launches
- * use), the engine's device allocations at the last allocation step, and the
- * largest excess of allocations over reservations seen so far (0 when the
- * budget accounting holds).  No reference counterpart: the reference has no
- * server. */
-int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* max_unreserved_bytes);
+ * job reserves its own grid block, rounded to its size class, plus its share of
+ * the scratch its launches use), the device memory the scheduler held at its
+ * last allocation step (blocks in use plus blocks cached for reuse), the peak of
+ * that figure (cached blocks are freed before it would pass the budget), and the
+ * largest excess of in-use bytes over reservations seen so far (0 when the
+ * budget accounting holds).  Any pointer may be NULL.  No reference
+ * counterpart: the reference has no server. */
+int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* peak_device_bytes,
+                     size_t* max_unreserved_bytes);
 
 /* ---- autotuner ---------------------------------------------------------------- */
 /* Times each tunable stage over candidate block/tile shapes per grid-edge

@@ -160,6 +160,7 @@ def test_server_budget_covers_allocations_under_pressure(vf):
     mem = srv.memory()
     assert mem["max_unreserved_bytes"] == 0, mem
     assert mem["reserved_bytes"] == 0
+    assert mem["peak_device_bytes"] <= 160 << 20, mem  # cached blocks included
     srv.close()
 
 

@@ -14,6 +14,8 @@
 //                  work of jobs that arrived at different times overlaps.
 #include <algorithm>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -29,6 +31,7 @@ namespace {
 
 constexpr uint32_t kRoundChunk = 8;   // flood rounds per captured chunk
 constexpr uint32_t kServerRounds = 4; // flood rounds per scheduler tick
+constexpr size_t kPlainCacheKeep = 16ull << 30;  // device bytes the plain form keeps cached between calls
 
 uint64_t mix64(uint64_t h, uint64_t v)
 {
@@ -276,6 +279,7 @@ int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results)
             results[i].latency_ms = now_ms() - t0;
             results[i].queue_ms = 0;
         }
+        E->trim_cache(kPlainCacheKeep);  // blocks are reused across jobs and calls, up to a bound
     });
 }
 
@@ -291,8 +295,9 @@ struct vf_server {
     std::deque<EJob*> queue;
     std::unordered_map<uint64_t, EJob*> jobs;  // every ticket not yet collected
     size_t reserved = 0;
-    size_t device_bytes = 0;          // engine allocations at the last audit
-    size_t max_unreserved = 0;        // largest excess of engine allocations over reserved bytes seen
+    size_t device_bytes = 0;          // engine device memory at the last audit (in use + cached)
+    size_t peak_device_bytes = 0;     // largest such figure seen
+    size_t max_unreserved = 0;        // largest excess of the engine's in-use bytes over reserved bytes seen
     uint32_t in_flight = 0;
     uint64_t completed = 0;
     uint64_t next_ticket = 1;
@@ -304,18 +309,49 @@ struct vf_server {
     void loop();
 };
 
+// Wall time of the scheduler's phases, printed at shutdown when VF_SERVER_TRACE is set.
+struct ServerTrace {
+    bool on = std::getenv("VF_SERVER_TRACE") != nullptr;
+    double t[12] = {};
+    uint64_t n[12] = {};
+    uint64_t ticks = 0, round_jobs = 0;
+    static constexpr const char* names[12] = { "admit: job blocks", "front", "naive post+sync", "flood table",
+                                                "rounds+gather+sync", "drained post+sync", "idle wait", "other",
+                                                "admit: table", "admit: scratch", "drained: table", "drained: release" };
+    double mark = now_ms();
+    void step(int k)
+    {
+        if (!on) return;
+        const double t1 = now_ms();
+        t[k] += t1 - mark;
+        ++n[k];
+        mark = t1;
+    }
+    void print() const
+    {
+        if (!on) return;
+        std::fprintf(stderr, "vf_server trace: %llu ticks, %llu job-rounds chunks\n", (unsigned long long)ticks,
+                     (unsigned long long)round_jobs);
+        for (int k = 0; k < 12; ++k)
+            if (n[k]) std::fprintf(stderr, "  %-22s %10.1f ms  %8llu times  %8.3f ms each\n", names[k], t[k],
+                                   (unsigned long long)n[k], t[k] / n[k]);
+    }
+};
+
 void vf_server::loop()
 {
     (void)hipSetDevice(device);
+    ServerTrace tr;
     try {
         Engine E;
         // engine allocations never exceed the bytes reserved by admitted jobs: checked after every step
         // that allocates (job blocks, tables, worklists, front scratch)
         auto audit = [&] {
-            const size_t dev = E.device_bytes();
+            const size_t dev = E.device_bytes(), in_use = dev - E.cached_bytes();
             std::lock_guard<std::mutex> lk(mu);
             device_bytes = dev;
-            if (dev > reserved) max_unreserved = std::max(max_unreserved, dev - reserved);
+            peak_device_bytes = std::max(peak_device_bytes, dev);
+            if (in_use > reserved) max_unreserved = std::max(max_unreserved, in_use - reserved);
         };
         std::vector<EJob*> leveling;  // flood jobs between front and drain
         Table* lev = nullptr;
@@ -336,6 +372,8 @@ void vf_server::loop()
         };
         for (;;) {
             std::vector<EJob*> admitted;
+            ++tr.ticks;
+            tr.step(7);
             {
                 std::unique_lock<std::mutex> lk(mu);
                 if (queue.empty() && leveling.empty())
@@ -374,18 +412,26 @@ void vf_server::loop()
                 }
                 in_flight += (uint32_t)admitted.size();
             }
+            tr.step(6);
             // front stages of the newly admitted jobs, in one set of launches
             if (!admitted.empty()) {
+                size_t need = 0;
+                for (auto* e : admitted) need += job_footprint(e->spec);
+                E.make_room(need, budget);  // cached blocks give way to the budget
                 for (auto* e : admitted) {
                     e->t_admit = now_ms();
                     E.alloc(*e);
                 }
+                tr.step(0);
                 Table* T = E.make_table(admitted);
+                tr.step(8);
                 E.reserve_front(*T);
                 audit();
+                tr.step(9);
                 E.front(*T);
                 E.drop_table(T);
                 E.release_front();  // admitted jobs reserve their share of it only while they are in flight
+                tr.step(1);
                 std::vector<EJob*> naive_done;
                 for (auto* e : admitted) {
                     if (e->d.algo == VF_FLOOD) {
@@ -408,6 +454,7 @@ void vf_server::loop()
                         complete(naive_done[i], naive_done[i]->res.status);
                     }
                     E.drop_table(P);
+                    tr.step(2);
                 }
             }
             // one chunk of relaxation rounds for every flood job in flight
@@ -417,10 +464,13 @@ void vf_server::loop()
                     lev = E.make_table(leveling);
                     lev_dirty = false;
                     audit();
+                    tr.step(3);
                 }
                 E.rounds(*lev, kServerRounds);
                 E.gather(*lev, E.pinned);
                 VF_HIP(hipStreamSynchronize(E.s));
+                tr.step(4);
+                tr.round_jobs += leveling.size();
                 std::vector<uint32_t> w(E.pinned, E.pinned + 16 * leveling.size());
                 std::vector<EJob*> drained, still;
                 for (size_t i = 0; i < leveling.size(); ++i) {
@@ -431,23 +481,32 @@ void vf_server::loop()
                 if (!drained.empty()) {
                     Table* P = E.make_table(drained);
                     audit();
+                    tr.step(10);
                     E.post(*P);
                     VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * drained.size(), hipMemcpyDeviceToHost, E.s));
                     VF_HIP(hipStreamSynchronize(E.s));
+                    tr.step(5);
                     std::vector<uint32_t> wd(E.pinned, E.pinned + 16 * drained.size());
                     for (size_t i = 0; i < drained.size(); ++i) {
                         E.finish(*drained[i], wd.data() + 16 * i);
                         E.release(*drained[i]);
                         complete(drained[i], drained[i]->res.status);
                     }
+                    tr.step(11);
                     E.drop_table(P);
                     leveling.swap(still);
+                    // the table (and its worklists) of the jobs just retired goes now: their reservations
+                    // were returned above
+                    E.drop_table(lev);
+                    lev = nullptr;
                     lev_dirty = true;
+                    tr.step(11);
                 }
             }
         }
         if (lev) E.drop_table(lev);
         VF_HIP(hipStreamSynchronize(E.s));
+        tr.print();
     } catch (const Exception& ex) {
         std::lock_guard<std::mutex> lk(mu);
         fatal = vf_last_error();
@@ -574,13 +633,15 @@ int vf_server_wait(vf_server* s, uint64_t ticket, vf_result* result)
     return collect(s, ticket, result, true);
 }
 
-int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* max_unreserved_bytes)
+int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes, size_t* peak_device_bytes,
+                     size_t* max_unreserved_bytes)
 {
     return guard([&] {
         if (!s) fail(VF_ERR_INVALID_ARG, "server is NULL");
         std::lock_guard<std::mutex> lk(s->mu);
         if (reserved_bytes) *reserved_bytes = s->reserved;
         if (device_bytes) *device_bytes = s->device_bytes;
+        if (peak_device_bytes) *peak_device_bytes = s->peak_device_bytes;
         if (max_unreserved_bytes) *max_unreserved_bytes = s->max_unreserved;
     });
 }

@@ -71,7 +71,9 @@ size_t job_footprint(const vf_job& j)
     const size_t lists = j.algorithm == VF_FLOOD
                              ? 65536ull * relax_tasks(j.dims[0], j.dims[1], j.dims[2]) + 2ull * kFloodXcds * 4096 * 8
                              : 0;
-    return Lo.total + front + table + lists;
+    // blocks come from the engine's size classes: at most an eighth more than asked, and 256 bytes at least
+    const size_t shared = front + table + lists;
+    return BlockCache::class_size(Lo.total) + shared + shared / 8 + 8 * 256;
 }
 
 int job_validate(const vf_job& j, std::string& why)
@@ -100,6 +102,73 @@ int job_validate(const vf_job& j, std::string& why)
     return VF_OK;
 }
 
+size_t BlockCache::class_size(size_t n)
+{
+    if (n <= 256) return 256;
+    size_t p = 1;
+    while (p <= n / 2) p <<= 1;  // largest power of two <= n
+    const size_t step = std::max<size_t>(p / 8, 256);
+    return (n + step - 1) / step * step;
+}
+
+void* BlockCache::get(size_t n, size_t* got)
+{
+    const size_t c = class_size(n);
+    *got = c;
+    auto it = free_.find(c);
+    if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        cached_ -= c;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess && cached_) {  // out of memory with blocks cached: give them back and try once more
+        (void)hipGetLastError();
+        trim(0);
+        e = hipMalloc(&p, c);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(VF_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed: %s", c, hipGetErrorString(e));
+    }
+    held_ += c;
+    return p;
+}
+
+void BlockCache::put(void* p, size_t bytes)
+{
+    if (!p) return;
+    free_[bytes].push_back(p);
+    cached_ += bytes;
+}
+
+void BlockCache::trim(size_t keep)
+{
+    for (auto it = free_.rbegin(); it != free_.rend() && cached_ > keep; ++it)
+        while (!it->second.empty() && cached_ > keep) {
+            (void)hipFree(it->second.back());  // synchronises: every queued use has finished
+            it->second.pop_back();
+            cached_ -= it->first;
+            held_ -= it->first;
+        }
+}
+
+void CacheBuf::ensure(size_t n, BlockCache& c)
+{
+    if (n <= bytes && p) return;
+    release(c);
+    p = c.get(n, &bytes);
+}
+
+void CacheBuf::release(BlockCache& c)
+{
+    c.put(p, bytes);
+    p = nullptr;
+    bytes = 0;
+}
+
 Engine::Engine()
 {
     VF_HIP(hipGetDevice(&device));
@@ -115,7 +184,10 @@ Engine::Engine()
 
 Engine::~Engine()
 {
-    if (s) (void)hipStreamSynchronize(s);
+    if (s) {
+        for (CacheBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) b->release(cache_);
+        (void)hipStreamSynchronize(s);
+    }
     if (pinned) (void)hipHostFree(pinned);
     if (s) (void)hipStreamDestroy(s);
 }
@@ -125,8 +197,7 @@ void Engine::alloc(EJob& e)
     const vf_job& j = e.spec;
     const Layout Lo = layout_of(j);
     e.bytes = Lo.total;
-    VF_HIP(hipMallocAsync(&e.mem, e.bytes, s));
-    live_bytes_ += e.bytes;
+    e.mem = cache_.get(Lo.total, &e.bytes);
     auto* base = static_cast<uint8_t*>(e.mem);
     JobDev& d = e.d;
     d = JobDev{};
@@ -193,10 +264,7 @@ void Engine::upload_mesh(EJob& e)
 
 void Engine::release(EJob& e)
 {
-    if (e.mem) {
-        VF_HIP(hipFreeAsync(e.mem, s));
-        live_bytes_ -= e.bytes;
-    }
+    cache_.put(e.mem, e.bytes);  // reused only by work queued after the job's own
     e.mem = nullptr;
 }
 
@@ -250,9 +318,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     cp[n] = (uint32_t)groups;
     bp[n] = (uint32_t)btiles;
     fp[n] = (uint32_t)fchunks;
-    VF_HIP(hipMallocAsync(&t->dev, total, s));
-    t->bytes = total;
-    live_bytes_ += total;
+    t->dev = cache_.get(total, &t->dev_bytes);
     VF_HIP(hipMemcpyAsync(t->dev, img.data(), total, hipMemcpyHostToDevice, s));
     VF_HIP(hipStreamSynchronize(s));  // the pageable source must outlive the copy
     auto* base = static_cast<uint8_t*>(t->dev);
@@ -274,9 +340,7 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
     if (fchunks) {
         t->flood.list_cap = flood_list_cap((uint32_t)fchunks);
-        VF_HIP(hipMallocAsync(&t->flood_lists, 2ull * kFloodXcds * t->flood.list_cap * 8, s));
-        t->bytes += 2ull * kFloodXcds * t->flood.list_cap * 8;
-        live_bytes_ += 2ull * kFloodXcds * t->flood.list_cap * 8;
+        t->flood_lists = cache_.get(2ull * kFloodXcds * t->flood.list_cap * 8, &t->list_bytes);
         t->flood.lists = static_cast<unsigned long long*>(t->flood_lists);
     }
     return t;
@@ -285,39 +349,40 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
 void Engine::drop_table(Table* t)
 {
     if (!t) return;
-    if (t->dev) VF_HIP(hipFreeAsync(t->dev, s));
-    if (t->flood_lists) VF_HIP(hipFreeAsync(t->flood_lists, s));
-    live_bytes_ -= t->bytes;
+    cache_.put(t->dev, t->dev_bytes);
+    cache_.put(t->flood_lists, t->list_bytes);
     delete t;
 }
 
 void Engine::reserve_front(const Table& t)
 {
     const uint32_t nt = t.total_tris;
-    vox_setup_.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup));
-    vox_counts_.ensure((size_t)std::max<uint32_t>(nt, 1) * 8);
-    vox_shell_off_.ensure((2 * (size_t)nt + 1) * 8);
-    scan_tmp_.ensure(std::max(scan_tmp_bytes(2 * (uint64_t)nt), scan_tmp_bytes(t.L.total_tiles)) + 64);
-    seed_cnt_.ensure((size_t)t.L.total_tiles * 4 + 64);
-    seed_off_.ensure(((size_t)t.L.total_tiles + 1) * 8 + 64);
+    vox_setup_.ensure((size_t)std::max<uint32_t>(nt, 1) * sizeof(TriSetup), cache_);
+    vox_counts_.ensure((size_t)std::max<uint32_t>(nt, 1) * 8, cache_);
+    vox_shell_off_.ensure((2 * (size_t)nt + 1) * 8, cache_);
+    scan_tmp_.ensure(std::max(scan_tmp_bytes(2 * (uint64_t)nt), scan_tmp_bytes(t.L.total_tiles)) + 64, cache_);
+    seed_cnt_.ensure((size_t)t.L.total_tiles * 4 + 64, cache_);
+    seed_off_.ensure(((size_t)t.L.total_tiles + 1) * 8 + 64, cache_);
 }
 
 void Engine::release_front()
 {
-    vox_setup_.release();
-    vox_counts_.release();
-    vox_shell_off_.release();
-    scan_tmp_.release();
-    seed_cnt_.release();
-    seed_off_.release();
+    for (CacheBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) b->release(cache_);
 }
 
 size_t Engine::device_bytes() const
 {
-    size_t n = live_bytes_;
-    for (const DevBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) n += b->bytes;
+    size_t n = cache_.held();
     for (const DevBuf& b : mc_.buf) n += b.bytes;
     return n;
+}
+
+void Engine::make_room(size_t need, size_t budget)
+{
+    const size_t held = device_bytes();
+    if (held + need <= budget) return;
+    const size_t excess = held + need - budget;
+    cache_.trim(cache_.cached() > excess ? cache_.cached() - excess : 0);
 }
 
 // Launch-only (safe inside stream capture): reserve_front must have sized the scratch.

@@ -10,6 +10,7 @@
 // tick has in that phase) shares each launch.
 #pragma once
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,40 @@
 #include "tune.hpp"
 
 namespace vf {
+
+// Device blocks by size class, kept for reuse.  A class is the size rounded up
+// to an eighth of its power of two (at most 12.5 % more); a released block
+// waits in its class's list for the next request of that class, so a steady
+// stream of jobs allocates nothing once each class has been seen.  Every use
+// is ordered on the engine's one stream, so a block released after queued work
+// is reused only by work queued later.
+class BlockCache {
+public:
+    static size_t class_size(size_t n);
+    void* get(size_t n, size_t* got);  // a block of class_size(n) bytes; hipMalloc on a miss
+    void put(void* p, size_t bytes);   // bytes: the block's class size
+    size_t held() const { return held_; }      // in use + cached
+    size_t cached() const { return cached_; }
+    void trim(size_t keep);                    // hipFree cached blocks, largest first, until at most keep bytes stay cached
+    BlockCache() = default;
+    BlockCache(const BlockCache&) = delete;
+    BlockCache& operator=(const BlockCache&) = delete;
+    ~BlockCache() { trim(0); }
+
+private:
+    std::map<size_t, std::vector<void*>> free_;
+    size_t held_ = 0, cached_ = 0;
+};
+
+// A grow-only scratch buffer from the engine's block cache.
+struct CacheBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n, BlockCache& c);
+    void release(BlockCache& c);
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
 
 struct EJob {
     vf_job spec{};
@@ -58,7 +93,7 @@ struct Table {
     FloodRoundScratch flood{};      // chunk prefix + worklists of the flood rounds
     int bucket = 0;                 // launch-shape bucket (largest grid edge of the table)
     void* flood_lists = nullptr;    // separate allocation of flood.lists
-    size_t bytes = 0;               // device bytes of dev and flood_lists
+    size_t dev_bytes = 0, list_bytes = 0;  // class sizes of the two blocks
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
 };
 
@@ -78,9 +113,13 @@ public:
     Table* make_table(const std::vector<EJob*>& js);
     void drop_table(Table* t);
     void reserve_front(const Table& t);  // size the front scratch (not capturable)
-    void release_front();                // free the front scratch (the next reserve_front allocates it again)
-    // device bytes held: job blocks, launch tables, worklists and scratch buffers
+    void release_front();                // return the front scratch (the next reserve_front takes it again)
+    // device bytes held: job blocks, launch tables, worklists and scratch buffers, in use or cached
     size_t device_bytes() const;
+    size_t cached_bytes() const { return cache_.cached(); }
+    // free cached blocks until `need` more bytes fit in `budget` (or nothing is left to free)
+    void make_room(size_t need, size_t budget);
+    void trim_cache(size_t keep = 0) { cache_.trim(keep); }
     void front(Table& t);               // launches only
     void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
@@ -92,8 +131,8 @@ public:
     static constexpr uint32_t kMaxTableJobs = 1024;
 
 private:
-    size_t live_bytes_ = 0;  // stream-ordered allocations: job blocks and tables
-    DevBuf vox_setup_, vox_counts_, vox_shell_off_, scan_tmp_, seed_cnt_, seed_off_;
+    BlockCache cache_;       // job blocks, tables, worklists and front scratch
+    CacheBuf vox_setup_, vox_counts_, vox_shell_off_, scan_tmp_, seed_cnt_, seed_off_;
     McScratch mc_;
 };
 

@@ -101,7 +101,7 @@ def lib():
         "vf_server_submit": (I, [P, P, C.POINTER(U64)]), "vf_server_cancel": (I, [P, U64]),
         "vf_server_poll": (I, [P, U64, P]), "vf_server_wait": (I, [P, U64, P]),
         "vf_server_stats": (I, [P, C.POINTER(C.c_size_t), C.POINTER(U32), C.POINTER(U64)]),
-        "vf_server_memory": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "vf_server_memory": (I, [P] + [C.POINTER(C.c_size_t)] * 4),
         "vf_device_alloc": (I, [C.c_size_t, C.POINTER(P)]), "vf_device_free": (I, [P]),
         "vf_memcpy_htod": (I, [P, P, C.c_size_t]),
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
@@ -275,6 +275,13 @@ class Grid:
         out = np.zeros(n, np.uint32)
         check(lib().vf_histogram(self._h, _ptr(out), n))
         return out
+
+    def marching_cubes_info(self, label: int = ALL_LABELS) -> dict:
+        """Marching cubes on the device; only the sizes come back (read_mesh copies the mesh out)."""
+        info = MeshInfo()
+        check(lib().vf_marching_cubes(self._h, label, C.byref(info)))
+        self._mesh_sizes = (info.num_vertices, info.num_triangles, info.num_labels)
+        return {"num_vertices": info.num_vertices, "num_triangles": info.num_triangles, "num_labels": info.num_labels}
 
     def marching_cubes(self, label: int = ALL_LABELS) -> dict:
         info = MeshInfo()
@@ -474,9 +481,10 @@ class Server:
         return {"reserved_bytes": b.value, "in_flight": f.value, "completed": c.value}
 
     def memory(self):
-        r, d, m = C.c_size_t(), C.c_size_t(), C.c_size_t()
-        check(lib().vf_server_memory(self._h, C.byref(r), C.byref(d), C.byref(m)))
-        return {"reserved_bytes": r.value, "device_bytes": d.value, "max_unreserved_bytes": m.value}
+        r, d, p, m = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(lib().vf_server_memory(self._h, C.byref(r), C.byref(d), C.byref(p), C.byref(m)))
+        return {"reserved_bytes": r.value, "device_bytes": d.value, "peak_device_bytes": p.value,
+                "max_unreserved_bytes": m.value}
 
     def close(self):
         if self._h:
