@@ -198,7 +198,7 @@ def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n):
     which = rng.integers(0, pool_size, size=n_req)
     for s, w in zip(reqs, which):
         s["tris"] = pool_specs[w]["tris"]
-    jobs = [make_spec_job(vf, pool[w], s) for s, w in zip(reqs, which)]
+    jobs = [make_spec_job(vf, pool[w], s, mc=True) for s, w in zip(reqs, which)]
     gaps = rng.exponential(1.0 / rate, size=n_req)
     srv = vf.Server(budget, slots)
     tickets, results = [None] * n_req, [None] * n_req
@@ -233,7 +233,8 @@ def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n):
         "gvoxels_per_s": vox / (t1 - t0) / 1e9, "wall_s": t1 - t0, "all_ok": all(r["status"] == 0 for r in results),
         "hbm_budget_gb": budget / 2**30, "slots": slots, "mesh_pool": pool_size,
         "distribution": "edge U[64,512] (x4), seeds U[2,128], distance uniform, naive/flood 50:50, flood noise 2, "
-                        "boundaries + histogram",
+                        "boundaries + histogram + marching cubes of every fragment",
+        "mc_triangles": sum(r["mc_triangles"] for r in results),
         "max_unreserved_bytes": mem["max_unreserved_bytes"],
     }
     # a sample of requests: the served result equals the job run alone and the oracle
@@ -426,7 +427,7 @@ def main():
     ap.add_argument("--serve-check", type=int, default=32, help="served requests compared with the oracle")
     ap.add_argument("--cpu-jobs", type=int, default=32)
     ap.add_argument("--cpu-runs", type=int, default=5)
-    ap.add_argument("--skip", default="", help="comma list of: tune,plain,served,cpu,profile,configs")
+    ap.add_argument("--skip", default="", help="comma list of: tune,extract,plain,served,cpu,profile,configs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a reference dispatch of known "
                          "traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -497,6 +498,27 @@ def main():
                        "definition": "per job: voxelize 36 T + V, fracture 3 V, boundaries 4 V, histogram 2 V"},
         "tuning": tuning,
     }
+
+    if "extract" not in skip:
+        # the same batch with the extract stage: every job's fragment surfaces by marching cubes (all jobs in
+        # one launch set after the fracture), grids resident throughout
+        mjobs = [make_spec_job(vf, m, s, mc=True) for m, s in zip(meshes, specs)]
+        bx = vf.Batch()
+        xr = bx.run(mjobs)
+        assert all(r["status"] == 0 for r in xr), "a job failed"
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            xr = bx.run(mjobs)
+        t1 = time.perf_counter()
+        bx.close()
+        xms = (t1 - t0) * 1e3 / a.steps
+        tris = sum(r["mc_triangles"] for r in xr)
+        out["batched_extract"] = {"gvoxels_per_s": voxels / (xms * 1e-3) / 1e9, "ms_per_step": xms,
+                                  "mc_triangles_per_step": tris, "mc_vertices_per_step": sum(r["mc_vertices"] for r in xr),
+                                  "extract_ms_per_step": xms - ms_step,
+                                  "mc_triangles_per_s": tris / ((xms - ms_step) * 1e-3) if xms > ms_step else None,
+                                  "same_labels_as_batched": [r["checksum"] for r in xr] == [r["checksum"] for r in ref]}
+        log(f"batched with extract: {out['batched_extract']}")
 
     if "plain" not in skip:
         plain = vf.plain_run(jobs)

@@ -275,6 +275,23 @@ int vf_timing_reset(void);
 int vf_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
 
 /* ---- batched form ------------------------------------------------------------ */
+/* Host buffers for one job's marching-cubes mesh, in vf_mesh_read's layout:
+ * positions, triangles with indices local to their label's vertex range,
+ * boundary flags, labels ascending and per-label vertex / triangle offsets.
+ * The run writes num_*; the mesh is copied (copied = 1) only when it fits the
+ * capacities. */
+typedef struct {
+    float* xyz;              /* 3 * cap_vertices */
+    uint32_t* tri;           /* 3 * cap_triangles */
+    uint8_t* tri_flag;       /* cap_triangles, may be NULL */
+    uint16_t* labels;        /* cap_labels, may be NULL */
+    uint32_t* vert_offsets;  /* cap_labels + 1, may be NULL */
+    uint32_t* tri_offsets;   /* cap_labels + 1, may be NULL */
+    uint32_t cap_vertices, cap_triangles, cap_labels;
+    uint32_t num_vertices, num_triangles, num_labels;
+    int copied;
+} vf_mesh_out;
+
 /* One fracture job: voxelize a mesh, seed, fracture, then boundaries, histogram
  * and (optionally) marching cubes, with the grid resident on the device.
  * The same sequence CADScene::fractureModel + prepareScene drives
@@ -294,7 +311,12 @@ typedef struct {
     uint64_t noise_seed;   /* flood only */
     uint32_t noise_amp;    /* flood only */
     int boundary_size;     /* detectBoundaries size; 0 skips the stage */
-    int marching_cubes;    /* 1: extract all labels */
+    int marching_cubes;    /* 1: extract every label's surface (RegularGrid::toTriangleMesh) */
+    /* Laplacian smoothing of the extracted meshes (vf_mesh_smooth): non-boundary
+     * steps and weight, then boundary steps and weight; zero steps skip it */
+    uint32_t smooth_iterations[2];
+    float smooth_weights[2];
+    vf_mesh_out* mesh_out; /* optional host copy of the mesh (marching_cubes = 1) */
     int mesh_on_device;
     int poison;            /* failure-path testing: the job fails with VF_ERR_POISON */
     uint16_t* labels_out;  /* optional host copy of the final grid (V entries) */

@@ -122,6 +122,62 @@ def test_batch_marching_cubes_counts(vf):
         assert r["mc_triangles"] == mesh["tri"].shape[0] and r["mc_vertices"] == mesh["xyz"].shape[0]
 
 
+def _mesh_equal(m, r):
+    for k in ("labels", "voff", "toff", "tri", "flag", "xyz"):
+        np.testing.assert_array_equal(m[k], r[k], err_msg=k)
+
+
+def test_batch_marching_cubes_meshes_match_oracle(vf):
+    # every job's surfaces extracted in one launch set; meshes copied to the jobs' buffers; one job smoothed
+    keep = []
+    jobs, meta = mixed_jobs(vf, 4, 31, keep, edges=(40, 72), mc_every=1)
+    it = 3
+    sm = (it, 0.9, it, 0.2)
+    bufs = []
+    for i, j in enumerate(jobs):
+        b = vf.MeshBuffers(400000, 800000)
+        b.attach(j)
+        bufs.append(b)
+        if i == 2:
+            j.smooth_iterations[:] = [sm[0], sm[2]]
+            j.smooth_weights[:] = [sm[1], sm[3]]
+    with vf.Batch() as bt:
+        res = bt.run(jobs)
+        res2 = bt.run(jobs)  # replay: the same meshes
+    for i, (m, r, b) in enumerate(zip(meta, res, bufs)):
+        assert r["status"] == 0, r
+        ref, _ = oracle_job(m)
+        want = O.marching_cubes(ref, *BOX, smooth=sm if i == 2 else None)
+        got = b.mesh()
+        assert got is not None, "mesh not copied"
+        assert r["mc_triangles"] == want["tri"].shape[0] and r["mc_vertices"] == want["xyz"].shape[0]
+        _mesh_equal(got, want)
+    assert [r["mc_triangles"] for r in res2] == [r["mc_triangles"] for r in res]
+    # a buffer too small: counts reported, nothing copied
+    small = vf.MeshBuffers(10, 10)
+    small.attach(jobs[0])
+    r = vf.Batch().run([jobs[0]])[0]
+    assert r["status"] == 0 and small.out.copied == 0 and small.out.num_triangles == res[0]["mc_triangles"]
+
+
+def test_server_marching_cubes_equals_batch(vf):
+    keep = []
+    jobs, meta = mixed_jobs(vf, 6, 33, keep, edges=(40, 64), mc_every=1)
+    bufs = [vf.MeshBuffers(300000, 600000) for _ in jobs]
+    for b, j in zip(bufs, jobs):
+        b.attach(j)
+    srv = vf.Server(64 << 30, max_slots=3)
+    out = [srv.wait(t) for t in [srv.submit(j) for j in jobs]]
+    served = [b.mesh() for b in bufs]
+    mem = srv.memory()
+    srv.close()
+    assert mem["max_unreserved_bytes"] == 0, mem
+    for m, o, got in zip(meta, out, served):
+        assert o["status"] == 0 and got is not None
+        ref, _ = oracle_job(m)
+        _mesh_equal(got, O.marching_cubes(ref, *BOX))
+
+
 def test_server_results_match_alone_and_oracle(vf):
     keep = []
     jobs, meta = mixed_jobs(vf, 16, 5, keep)

@@ -138,7 +138,10 @@ def test_cfg_512_chebyshev_naive_flood_mc_exact(vf):
     job.labels_out = out.ctypes.data
     with vf.Batch() as b:
         res = b.run([job])[0]
+    print("batched 512^3 flood:", res, flush=True)
     assert res["status"] == 0
+    assert res["occupied"] == int((base != 0).sum())
+    assert res["num_labels"] == 128
     np.testing.assert_array_equal(out, flood)
     dev.close()
 

@@ -222,6 +222,7 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
             VF_HIP(hipStreamSynchronize(E.s));
             std::vector<uint32_t> w(E.pinned, E.pinned + 16 * T.L.njobs);
             for (size_t i = 0; i < T.jobs.size(); ++i) E.finish(*T.jobs[i], w.data() + 16 * i);
+            E.extract(T.jobs);  // marching cubes of the jobs that ask for it, all in one launch set
         }
         const double t1 = now_ms();
         for (uint32_t i = 0; i < n; ++i) {
@@ -271,6 +272,8 @@ int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results)
                 VF_HIP(hipStreamSynchronize(E->s));
                 std::vector<uint32_t> w(E->pinned, E->pinned + 16);
                 E->finish(e, w.data());
+                E->extract({ &e });
+                E->release_extract();
                 E->drop_table(T);
                 E->release(e);
                 VF_HIP(hipStreamSynchronize(E->s));
@@ -352,6 +355,26 @@ void vf_server::loop()
             device_bytes = dev;
             peak_device_bytes = std::max(peak_device_bytes, dev);
             if (in_use > reserved) max_unreserved = std::max(max_unreserved, in_use - reserved);
+        };
+        // marching cubes of a group of finished jobs: its buffers are sized only once the surfaces are
+        // counted, so they are reserved then (cached blocks give way first); a group that cannot fit the
+        // budget fails with VF_ERR_BUDGET.  The meshes leave with the results, the buffers go back.
+        size_t mc_reserved = 0;
+        auto extract = [&](std::vector<EJob*>& group) {
+            auto reserve = [&](size_t bytes) {
+                E.make_room(bytes, budget);
+                std::lock_guard<std::mutex> lk(mu);
+                if (reserved + bytes > budget) return false;
+                reserved += bytes;
+                mc_reserved += bytes;
+                return true;
+            };
+            E.extract(group, reserve);
+            audit();
+            E.release_extract();
+            std::lock_guard<std::mutex> lk(mu);
+            reserved -= mc_reserved;
+            mc_reserved = 0;
         };
         std::vector<EJob*> leveling;  // flood jobs between front and drain
         Table* lev = nullptr;
@@ -448,10 +471,11 @@ void vf_server::loop()
                     VF_HIP(hipMemcpyAsync(E.pinned, P->gather, 16 * 4 * naive_done.size(), hipMemcpyDeviceToHost, E.s));
                     VF_HIP(hipStreamSynchronize(E.s));
                     std::vector<uint32_t> w(E.pinned, E.pinned + 16 * naive_done.size());
-                    for (size_t i = 0; i < naive_done.size(); ++i) {
-                        E.finish(*naive_done[i], w.data() + 16 * i);
-                        E.release(*naive_done[i]);
-                        complete(naive_done[i], naive_done[i]->res.status);
+                    for (size_t i = 0; i < naive_done.size(); ++i) E.finish(*naive_done[i], w.data() + 16 * i);
+                    extract(naive_done);
+                    for (auto* e : naive_done) {
+                        E.release(*e);
+                        complete(e, e->res.status);
                     }
                     E.drop_table(P);
                     tr.step(2);
@@ -487,10 +511,11 @@ void vf_server::loop()
                     VF_HIP(hipStreamSynchronize(E.s));
                     tr.step(5);
                     std::vector<uint32_t> wd(E.pinned, E.pinned + 16 * drained.size());
-                    for (size_t i = 0; i < drained.size(); ++i) {
-                        E.finish(*drained[i], wd.data() + 16 * i);
-                        E.release(*drained[i]);
-                        complete(drained[i], drained[i]->res.status);
+                    for (size_t i = 0; i < drained.size(); ++i) E.finish(*drained[i], wd.data() + 16 * i);
+                    extract(drained);
+                    for (auto* e : drained) {
+                        E.release(*e);
+                        complete(e, e->res.status);
                     }
                     tr.step(11);
                     E.drop_table(P);

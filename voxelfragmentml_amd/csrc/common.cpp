@@ -38,6 +38,73 @@ void fail_hip(hipError_t e, const char* what, const char* file, int line)
     throw Exception{ e == hipErrorOutOfMemory ? VF_ERR_OUT_OF_MEMORY : VF_ERR_DEVICE };
 }
 
+size_t BlockCache::class_size(size_t n)
+{
+    if (n <= 256) return 256;
+    size_t p = 1;
+    while (p <= n / 2) p <<= 1;  // largest power of two <= n
+    const size_t step = std::max<size_t>(p / 8, 256);
+    return (n + step - 1) / step * step;
+}
+
+void* BlockCache::get(size_t n, size_t* got)
+{
+    const size_t c = class_size(n);
+    *got = c;
+    auto it = free_.find(c);
+    if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        cached_ -= c;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess && cached_) {  // out of memory with blocks cached: give them back and try once more
+        (void)hipGetLastError();
+        trim(0);
+        e = hipMalloc(&p, c);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(VF_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed: %s", c, hipGetErrorString(e));
+    }
+    held_ += c;
+    return p;
+}
+
+void BlockCache::put(void* p, size_t bytes)
+{
+    if (!p) return;
+    free_[bytes].push_back(p);
+    cached_ += bytes;
+}
+
+void BlockCache::trim(size_t keep)
+{
+    for (auto it = free_.rbegin(); it != free_.rend() && cached_ > keep; ++it)
+        while (!it->second.empty() && cached_ > keep) {
+            (void)hipFree(it->second.back());  // synchronises: every queued use has finished
+            it->second.pop_back();
+            cached_ -= it->first;
+            held_ -= it->first;
+        }
+}
+
+void CacheBuf::ensure(size_t n, BlockCache& c)
+{
+    if (n <= bytes && p) return;
+    release(c);
+    p = c.get(n, &bytes);
+}
+
+void CacheBuf::release(BlockCache& c)
+{
+    c.put(p, bytes);
+    p = nullptr;
+    bytes = 0;
+}
+
 void DevBuf::ensure(size_t n)
 {
     if (n <= bytes && p) return;

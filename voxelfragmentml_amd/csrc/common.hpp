@@ -8,8 +8,10 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "vfrag.h"
 
@@ -90,6 +92,40 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Device blocks by size class, kept for reuse.  A class is the size rounded up
+// to an eighth of its power of two (at most 12.5 % more); a released block
+// waits in its class's list for the next request of that class, so a steady
+// stream of jobs allocates nothing once each class has been seen.  Every use
+// is ordered on the engine's one stream, so a block released after queued work
+// is reused only by work queued later.
+class BlockCache {
+public:
+    static size_t class_size(size_t n);
+    void* get(size_t n, size_t* got);  // a block of class_size(n) bytes; hipMalloc on a miss
+    void put(void* p, size_t bytes);   // bytes: the block's class size
+    size_t held() const { return held_; }      // in use + cached
+    size_t cached() const { return cached_; }
+    void trim(size_t keep);                    // hipFree cached blocks, largest first, until at most keep bytes stay cached
+    BlockCache() = default;
+    BlockCache(const BlockCache&) = delete;
+    BlockCache& operator=(const BlockCache&) = delete;
+    ~BlockCache() { trim(0); }
+
+private:
+    std::map<size_t, std::vector<void*>> free_;
+    size_t held_ = 0, cached_ = 0;
+};
+
+// A grow-only scratch buffer from the engine's block cache.
+struct CacheBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n, BlockCache& c);
+    void release(BlockCache& c);
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
 };
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }

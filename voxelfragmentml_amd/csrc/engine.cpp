@@ -102,78 +102,12 @@ int job_validate(const vf_job& j, std::string& why)
     return VF_OK;
 }
 
-size_t BlockCache::class_size(size_t n)
-{
-    if (n <= 256) return 256;
-    size_t p = 1;
-    while (p <= n / 2) p <<= 1;  // largest power of two <= n
-    const size_t step = std::max<size_t>(p / 8, 256);
-    return (n + step - 1) / step * step;
-}
-
-void* BlockCache::get(size_t n, size_t* got)
-{
-    const size_t c = class_size(n);
-    *got = c;
-    auto it = free_.find(c);
-    if (it != free_.end() && !it->second.empty()) {
-        void* p = it->second.back();
-        it->second.pop_back();
-        cached_ -= c;
-        return p;
-    }
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, c);
-    if (e != hipSuccess && cached_) {  // out of memory with blocks cached: give them back and try once more
-        (void)hipGetLastError();
-        trim(0);
-        e = hipMalloc(&p, c);
-    }
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        fail(VF_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed: %s", c, hipGetErrorString(e));
-    }
-    held_ += c;
-    return p;
-}
-
-void BlockCache::put(void* p, size_t bytes)
-{
-    if (!p) return;
-    free_[bytes].push_back(p);
-    cached_ += bytes;
-}
-
-void BlockCache::trim(size_t keep)
-{
-    for (auto it = free_.rbegin(); it != free_.rend() && cached_ > keep; ++it)
-        while (!it->second.empty() && cached_ > keep) {
-            (void)hipFree(it->second.back());  // synchronises: every queued use has finished
-            it->second.pop_back();
-            cached_ -= it->first;
-            held_ -= it->first;
-        }
-}
-
-void CacheBuf::ensure(size_t n, BlockCache& c)
-{
-    if (n <= bytes && p) return;
-    release(c);
-    p = c.get(n, &bytes);
-}
-
-void CacheBuf::release(BlockCache& c)
-{
-    c.put(p, bytes);
-    p = nullptr;
-    bytes = 0;
-}
-
 Engine::Engine()
 {
     VF_HIP(hipGetDevice(&device));
     VF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     VF_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), 16 * 4 * kMaxTableJobs));
+    mc_.cache = &cache_;  // marching-cubes scratch and meshes come from the same block cache
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
         uint64_t thr = ~0ull;  // keep freed blocks cached in the pool
@@ -370,12 +304,7 @@ void Engine::release_front()
     for (CacheBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) b->release(cache_);
 }
 
-size_t Engine::device_bytes() const
-{
-    size_t n = cache_.held();
-    for (const DevBuf& b : mc_.buf) n += b.bytes;
-    return n;
-}
+size_t Engine::device_bytes() const { return cache_.held(); }
 
 void Engine::make_room(size_t need, size_t budget)
 {
@@ -450,18 +379,55 @@ void Engine::finish(EJob& e, const uint32_t* w)
     if (err & kDevErrMesh) r.status = VF_ERR_INVALID_ARG;
     else if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;
     else if (err) r.status = VF_ERR_OVERFLOW;
-    if (r.status == VF_OK && e.spec.marching_cubes) {
-        vf_mesh_info info{};
-        const uint32_t dims[3] = { e.d.nx, e.d.ny, e.d.nz };
-        mc_run(mc_, s, e.d.labels, dims, e.d.mn, e.d.cs, e.d.hist, VF_ALL_LABELS, &info);
-        r.mc_vertices = info.num_vertices;
-        r.mc_triangles = info.num_triangles;
-    }
+    r.mc_vertices = r.mc_triangles = 0;
     if (e.spec.labels_out)
         VF_HIP(hipMemcpyAsync(e.spec.labels_out, e.d.labels, e.d.V * 2, hipMemcpyDeviceToHost, s));
     if (e.spec.histogram_out)
         VF_HIP(hipMemcpyAsync(e.spec.histogram_out, e.d.hist, 65536 * 4, hipMemcpyDeviceToHost, s));
     if (e.spec.labels_out || e.spec.histogram_out) VF_HIP(hipStreamSynchronize(s));
 }
+
+void Engine::extract(const std::vector<EJob*>& js, const std::function<bool(size_t)>& reserve)
+{
+    std::vector<EJob*> mj;
+    for (EJob* e : js)
+        if (e->spec.marching_cubes && e->res.status == VF_OK) mj.push_back(e);
+    if (mj.empty()) return;
+    const uint32_t n = (uint32_t)mj.size();
+    std::vector<McJobSpec> spec(n);
+    std::vector<McJobResult> res(n);
+    std::vector<McSmooth> sm(n);
+    std::vector<vf_mesh_out*> outs(n);
+    bool smooth = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        const EJob& e = *mj[i];
+        McJobSpec& q = spec[i];
+        q.grid = e.d.labels;
+        q.hist = e.d.hist;
+        q.dims[0] = e.d.nx;
+        q.dims[1] = e.d.ny;
+        q.dims[2] = e.d.nz;
+        for (int a = 0; a < 3; ++a) {
+            q.mn[a] = e.d.mn[a];
+            q.cs[a] = e.d.cs[a];
+        }
+        q.want = VF_ALL_LABELS;
+        sm[i] = McSmooth{ e.spec.smooth_iterations[0], e.spec.smooth_weights[0], e.spec.smooth_iterations[1],
+                          e.spec.smooth_weights[1] };
+        smooth |= sm[i].nb_iters || sm[i].b_iters;
+        outs[i] = e.spec.mesh_out;
+    }
+    mc_run_jobs(mc_, s, spec.data(), n, res.data(), reserve);
+    if (mc_.valid && smooth) mc_smooth_jobs(mc_, s, sm.data(), n);
+    mc_read_jobs(mc_, s, res.data(), outs.data(), n);
+    for (uint32_t i = 0; i < n; ++i) {
+        vf_result& r = mj[i]->res;
+        r.mc_vertices = res[i].nv;
+        r.mc_triangles = res[i].nt;
+        if (res[i].status != VF_OK) r.status = res[i].status;
+    }
+}
+
+void Engine::release_extract() { mc_.release_all(); }
 
 }  // namespace vf

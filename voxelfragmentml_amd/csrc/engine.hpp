@@ -19,40 +19,6 @@
 
 namespace vf {
 
-// Device blocks by size class, kept for reuse.  A class is the size rounded up
-// to an eighth of its power of two (at most 12.5 % more); a released block
-// waits in its class's list for the next request of that class, so a steady
-// stream of jobs allocates nothing once each class has been seen.  Every use
-// is ordered on the engine's one stream, so a block released after queued work
-// is reused only by work queued later.
-class BlockCache {
-public:
-    static size_t class_size(size_t n);
-    void* get(size_t n, size_t* got);  // a block of class_size(n) bytes; hipMalloc on a miss
-    void put(void* p, size_t bytes);   // bytes: the block's class size
-    size_t held() const { return held_; }      // in use + cached
-    size_t cached() const { return cached_; }
-    void trim(size_t keep);                    // hipFree cached blocks, largest first, until at most keep bytes stay cached
-    BlockCache() = default;
-    BlockCache(const BlockCache&) = delete;
-    BlockCache& operator=(const BlockCache&) = delete;
-    ~BlockCache() { trim(0); }
-
-private:
-    std::map<size_t, std::vector<void*>> free_;
-    size_t held_ = 0, cached_ = 0;
-};
-
-// A grow-only scratch buffer from the engine's block cache.
-struct CacheBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t n, BlockCache& c);
-    void release(BlockCache& c);
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
 struct EJob {
     vf_job spec{};
     std::vector<float> xyz_copy;     // server: the job owns a copy of its mesh
@@ -124,8 +90,12 @@ public:
     void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
     void gather(Table& t, uint32_t* host16);  // async copy of the gathered stats
-    // fill j.res from 16 gathered words; runs marching cubes and optional copies
+    // fill j.res from 16 gathered words and make the optional grid and histogram copies
     void finish(EJob& j, const uint32_t* w16);
+    // marching cubes (and smoothing) of every listed job with marching_cubes set and no error, in one
+    // launch set: counts into the results, meshes into the jobs' mesh_out.  reserve: see mc_run_jobs.
+    void extract(const std::vector<EJob*>& js, const std::function<bool(size_t)>& reserve = nullptr);
+    void release_extract();  // the meshes and their scratch back to the block cache
 
     uint32_t* pinned = nullptr;     // gathered stats staging (16 words x kMaxTableJobs)
     static constexpr uint32_t kMaxTableJobs = 1024;

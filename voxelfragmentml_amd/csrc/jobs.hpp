@@ -7,6 +7,7 @@
 // once for all jobs of a batch, so uneven grids share one launch.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -245,19 +246,60 @@ void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
 void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
                       uint32_t spreading, uint64_t key, uint4* out, uint32_t* result);
 
-// mc.hip: all-labels marching cubes on a raw device grid.  hist_dev holds the
-// grid's countValues histogram (65536 counts); results stay in W until read.
+// mc.hip: marching cubes of every label of one or many device grids (one
+// launch set for all of them).  Scratch and results live in W's buffers, taken
+// from `cache` (its owner's block cache, or W's own, which release_all trims).
 struct McScratch {
-    DevBuf buf[16];
-    uint32_t nv = 0, nt = 0, nl = 0;
+    static constexpr int kBuffers = 32;
+    CacheBuf buf[kBuffers];
+    BlockCache own;
+    BlockCache* cache = &own;
+    uint32_t nv = 0, nt = 0, nl = 0;  // totals of the last run; nl counts (grid, label) segments
     bool valid = false;
+    std::vector<uint32_t> label_base;  // first segment of each grid of the last run, and the total
+    void ensure(int slot, size_t n);
+    void release_all();               // every buffer back to the cache (the result is dropped)
+    McScratch() = default;
+    McScratch(const McScratch&) = delete;
+    McScratch& operator=(const McScratch&) = delete;
+    ~McScratch();
 };
+struct McJobSpec {
+    const uint16_t* grid;
+    const uint32_t* hist;  // the grid's countValues histogram (65536 counts, device)
+    uint32_t dims[3];
+    float mn[3], cs[3];
+    uint32_t want;         // VF_ALL_LABELS or one label
+};
+struct McJobResult {
+    uint32_t nv, nt, nl;   // the grid's vertices, triangles and labels
+    uint64_t v0, t0;       // its first vertex and triangle in W
+    uint32_t seg0;         // its first segment in W
+    int status;            // VF_ERR_INVALID_ARG: more labels than marching cubes supports
+};
+// reserve (optional): asked for the bytes of each of the two phases' buffers before they are taken; a
+// refusal ends the run with every grid's status VF_ERR_BUDGET
+void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* jobs, uint32_t n, McJobResult* out,
+                 const std::function<bool(size_t)>& reserve);
+// copies each grid's mesh of the last run to its vf_mesh_out (NULL entries skipped) when it fits
+void mc_read_jobs(McScratch& W, hipStream_t s, const McJobResult* res, vf_mesh_out* const* outs, uint32_t n);
 void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
             const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info);
 void mc_read(McScratch& W, hipStream_t s, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
              uint32_t* toff);
-// Laplacian smoothing of the last mesh in W, in place (non-boundary pass, then boundary pass)
+// segments [seg0, seg0 + nseg) of the last run: offsets relative to the first of them
+void mc_read_range(McScratch& W, hipStream_t s, uint32_t seg0, uint32_t nseg, float* xyz, uint32_t* tri, uint8_t* tri_flag,
+                   uint16_t* labels, uint32_t* voff, uint32_t* toff);
+// Laplacian smoothing of the last mesh in W, in place (non-boundary pass, then boundary pass); with
+// several grids, each with its own steps and weights
+struct McSmooth {
+    uint32_t nb_iters;
+    float nb_weight;
+    uint32_t b_iters;
+    float b_weight;
+};
 void mc_smooth(McScratch& W, hipStream_t s, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight);
+void mc_smooth_jobs(McScratch& W, hipStream_t s, const McSmooth* per_job, uint32_t njobs);
 
 // scan (scan.hip): exclusive prefix sums, graph-capturable, no host sync
 size_t scan_tmp_bytes(uint64_t n);

@@ -1,29 +1,42 @@
-// All-labels marching cubes with an exact weld
+// Marching cubes of every fragment of one or many grids, with an exact weld
 // (RegularGrid::toTriangleMesh RegularGrid.cpp:443-486 ->
 //  MarchingCubes::triangulateFieldGPU MarchingCubes.cpp:364-433,
 //  marchingCubes-comp.glsl:96-156).
 //
 // The reference runs one full-grid dispatch per label, appends triangles with
 // an atomic counter (nondeterministic order) and welds with 30-bit Morton codes.
-// Here one pass over the padded lattice serves every label.  For label L the
-// field is (cell & 0x7FFF) == L on the grid padded by FREE cells, isolevel 0.5,
-// so every vertex is the midpoint of a lattice edge whose two ends disagree
-// about L: the weld is exact by edge id (3 * padded index + axis).
-// Output is canonical: labels ascending; per label, vertices in edge-id order
-// and triangles in (cube, table slot) order, corners emitted (0, 2, 1).
+// For label L the field is (cell & 0x7FFF) == L on the grid padded by FREE
+// cells, isolevel 0.5, so every vertex is the midpoint of a lattice edge whose
+// two ends disagree about L: the weld is exact by edge id (3 * padded index +
+// axis).  Output is canonical: per grid, labels ascending; per label, vertices
+// in edge-id order and triangles in (cube, table slot) order, corners emitted
+// (0, 2, 1).
 //
 // Lattice point p = (px, py, pz) owns its three +axis edges and, when it is a
-// valid corner-0, the cube spanning x in [px-1, px], y in [py, py+1],
-// z in [pz, pz+1].  Kernels:
-//   k_mc_count  per tile: per-label triangle / vertex counts -> (label, tile) matrices
-//   scans       label-major exclusive prefix of both matrices (counting sort)
-//   k_mc_emit   per tile, per present label: block scan of per-thread counts,
-//               write triangle edge ids + flags and vertex edge ids in order
-//   k_mc_finish vertex positions (FP32, aabb_min + cell_size * (p - 1)) and
-//               triangle indices (binary search of the label's sorted edges)
+// valid corner 0, the cube spanning x in [px-1, px], y in [py, py+1],
+// z in [pz, pz+1].  Only points next to a label change produce output, a few
+// per cent of the lattice, so the work is sparse:
+//   k_mc_labels / k_mc_base  each grid's target labels (nonzero histogram
+//                 counts above FREE) -> dense index, and one global segment
+//                 per (grid, label)
+//   k_mc_count   one thread per unit (8 consecutive points of a z column):
+//                 the (segment, point) pairs with output, and their triangle
+//                 and vertex counts; uniform units cost five column loads
+//   scan         pair offsets per tile of units
+//   (one host synchronisation: totals size the outputs)
+//   k_mc_pairs   the pairs, in (grid, point) order, as (segment, point | counts)
+//   radix sort   stable by segment -> (grid, label, point) order
+//   scans        triangle and vertex offsets of every pair
+//   k_mc_emit    each pair's triangles (edge ids, boundary flag) and vertices
+//   k_mc_segments per-segment vertex and triangle offsets
+//   k_mc_finish  vertex positions (FP32, aabb_min + cell_size * (p - 1)) and
+//                 triangle indices (binary search in the label's sorted edges)
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "cellops.hpp"
 #include "grid.hpp"
@@ -34,10 +47,11 @@ namespace vf {
 namespace {
 
 constexpr uint32_t kMcThreads = 256;
-constexpr uint32_t kMcPts = 8;                       // lattice points per thread per sub-chunk
-constexpr uint32_t kMcChunk = kMcThreads * kMcPts;   // points per sub-chunk
-constexpr uint32_t kMcMaxLabels = 4096;
-constexpr uint32_t kMcNoLabel = 0xFFFFu;
+constexpr uint32_t kMcZ = 8;                       // lattice points per unit (a z run of one column)
+constexpr uint32_t kMcMaxLabels = 4096;            // labels per grid
+constexpr uint16_t kMcNoLabel = 0xFFFFu;           // no dense index; also "beyond the lattice" (masked values <= 0x7FFF)
+constexpr uint32_t kLindexWords = 65536 + kMcMaxLabels;  // per grid: label -> dense index, then its labels
+constexpr uint32_t kMcMaxJobs = 1024;              // grids per run
 
 __constant__ int c_mc_nb[8][3] = { { 0, 0, 0 }, { 0, 0, 1 }, { -1, 0, 1 }, { -1, 0, 0 },
                                    { 0, 1, 0 }, { 0, 1, 1 }, { -1, 1, 1 }, { -1, 1, 0 } };
@@ -45,142 +59,209 @@ __constant__ int c_mc_edge[12][2] = { { 0, 1 }, { 1, 2 }, { 2, 3 }, { 3, 0 }, { 
                                       { 6, 7 }, { 7, 4 }, { 0, 4 }, { 1, 5 }, { 2, 6 }, { 3, 7 } };
 __constant__ unsigned long long c_mc_tri[256];
 
-struct McDev {
+struct McJob {
     const uint16_t* grid;
-    uint32_t nx, ny, nz;      // grid dims
-    uint32_t Nx, Ny, Nz;      // padded lattice dims
-    uint64_t P;               // lattice points
-    uint32_t tiles, k;        // tiles, sub-chunks per tile
-    uint32_t nl;
-    const uint16_t* lindex;   // 65536: label -> dense index or kMcNoLabel
-    const uint16_t* labels;   // nl labels ascending
-    uint32_t* mtri;           // nl * tiles
-    uint32_t* mvert;          // nl * tiles
-    uint32_t* tile_total;     // tiles
-    const unsigned long long* otri;   // nl * tiles + 1 exclusive prefix
-    const unsigned long long* overt;
-    unsigned long long* tri_edge;     // 3 per triangle
+    const uint32_t* hist;      // countValues histogram (65536 counts)
+    uint16_t* lindex;          // kLindexWords: label -> dense index or kMcNoLabel, then the labels ascending
+    uint32_t nx, ny, nz;       // grid dims
+    uint32_t Nx, Ny, Nz;       // padded lattice dims
+    uint32_t cz;               // units per z column
+    uint32_t want;             // VF_ALL_LABELS or one label
+    uint32_t min_target;       // smallest target label (65536 when none): units below it emit nothing
+    float mn[3], cs[3];
+};
+
+struct McRun {
+    McJob* jobs;
+    uint32_t njobs;
+    const unsigned long long* unit_prefix;  // njobs + 1
+    unsigned long long total_units;
+    uint32_t tiles;                         // of kMcThreads units
+    uint32_t* label_base;                   // njobs + 1: first segment of each grid
+    uint32_t* job_nl;                       // njobs
+    uint32_t* job_err;                      // njobs: more than kMcMaxLabels labels
+    uint16_t* labels_all;                   // one label per segment
+    unsigned long long* job_tot;            // njobs x 4: pairs, triangles, vertices
+    uint8_t* unit_pairs;                    // total_units
+    uint32_t* tile_pairs;                   // tiles
+    const unsigned long long* tile_off;     // tiles + 1
+    uint32_t* keys;                         // pairs: segment
+    unsigned long long* vals;               // pairs: point | triangles << 32 | vertices << 36
+    const uint32_t* skeys;                  // sorted by segment
+    const unsigned long long* svals;
+    uint32_t* cnt_t;                        // triangles of each sorted pair
+    uint32_t* cnt_v;
+    const unsigned long long* tpre;         // npairs + 1
+    const unsigned long long* vpre;
+    unsigned long long npairs;
+    uint32_t segments;
+    unsigned long long* voff;               // segments + 1
+    unsigned long long* toff;
+    unsigned long long* tri_edge;           // 3 per triangle
     uint8_t* tri_flag;
     unsigned long long* vert_edge;
-    float mn[3], cs[3];
     float* xyz;
     uint32_t* tri_idx;
-    uint32_t nv, nt;
+    unsigned long long nv, nt;
 };
 
-__device__ inline uint16_t raw_at(const McDev& M, int px, int py, int pz)
+// Raw value of lattice point (x, y, z): FREE on the padding, kMcNoLabel beyond the lattice.
+__device__ inline uint16_t lat_raw(const McJob& J, int x, int y, int z)
 {
-    if (px <= 0 || py <= 0 || pz <= 0 || px >= (int)M.Nx - 1 || py >= (int)M.Ny - 1 || pz >= (int)M.Nz - 1) return kFree;
-    return M.grid[((size_t)(px - 1) * M.ny + (py - 1)) * M.nz + (pz - 1)];
+    if (x < 0 || y < 0 || z < 0 || x >= (int)J.Nx || y >= (int)J.Ny || z >= (int)J.Nz) return kMcNoLabel;
+    if (x == 0 || y == 0 || z == 0 || x == (int)J.Nx - 1 || y == (int)J.Ny - 1 || z == (int)J.Nz - 1) return kFree;
+    return gcast<const uint16_t>(J.grid)[((size_t)(x - 1) * J.ny + (size_t)(y - 1)) * J.nz + (size_t)(z - 1)];
 }
+__device__ inline uint16_t mask15(uint16_t raw) { return raw == kMcNoLabel ? kMcNoLabel : (uint16_t)(raw & kMask15); }
 
-// Everything a lattice point contributes, for all labels.
-struct PointInfo {
-    bool cube;               // p is a valid corner-0
-    uint16_t corner[8];      // masked corner values
-    uint16_t c0raw;          // raw value of corner 0 (boundary flag)
-    uint16_t own;            // masked value of p
-    uint16_t nbr[3];         // masked value at p + e_axis (kMcNoLabel beyond the lattice)
-    int px, py, pz;
-};
-
-__device__ inline void load_point(const McDev& M, uint64_t p, PointInfo& I)
+// Column (x, y) from z0 on: n raw values.  Interior columns are one contiguous row of the grid.
+template <int N>
+__device__ inline void load_column(const McJob& J, int x, int y, int z0, uint16_t v[N])
 {
-    const uint32_t yz = M.Ny * M.Nz;
-    I.px = (int)(p / yz);
-    const uint32_t r = (uint32_t)(p - (uint64_t)I.px * yz);
-    I.py = (int)(r / M.Nz);
-    I.pz = (int)(r - (uint32_t)I.py * M.Nz);
-    const uint16_t own_raw = raw_at(M, I.px, I.py, I.pz);
-    I.own = own_raw & kMask15;
-    I.nbr[0] = I.px + 1 < (int)M.Nx ? (uint16_t)(raw_at(M, I.px + 1, I.py, I.pz) & kMask15) : (uint16_t)kMcNoLabel;
-    I.nbr[1] = I.py + 1 < (int)M.Ny ? (uint16_t)(raw_at(M, I.px, I.py + 1, I.pz) & kMask15) : (uint16_t)kMcNoLabel;
-    I.nbr[2] = I.pz + 1 < (int)M.Nz ? (uint16_t)(raw_at(M, I.px, I.py, I.pz + 1) & kMask15) : (uint16_t)kMcNoLabel;
-    I.cube = I.px >= 1 && I.py <= (int)M.Ny - 2 && I.pz <= (int)M.Nz - 2;
-    I.c0raw = own_raw;
-    if (I.cube) {
+    const bool inside = x >= 1 && y >= 1 && x <= (int)J.Nx - 2 && y <= (int)J.Ny - 2;
+    if (inside && z0 >= 1 && z0 + N <= (int)J.Nz - 1) {
+        const VF_AS_GLOBAL uint16_t* row =
+            gcast<const uint16_t>(J.grid) + ((size_t)(x - 1) * J.ny + (size_t)(y - 1)) * J.nz + (size_t)(z0 - 1);
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            I.corner[i] = raw_at(M, I.px + c_mc_nb[i][0], I.py + c_mc_nb[i][1], I.pz + c_mc_nb[i][2]) & kMask15;
+        for (int k = 0; k < N; ++k) v[k] = row[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = lat_raw(J, x, y, z0 + k);
     }
 }
 
-__device__ inline uint32_t tri_count_for(const PointInfo& I, uint16_t L)
+// The labels with output at one point and their counts.  c: the eight masked cube corners (corner 0 is
+// the point itself, corner 1 its +z neighbour, corner 4 its +y neighbour; kMcNoLabel beyond the lattice),
+// nbx: its masked +x neighbour; cube: the point is a valid corner 0; ntri: the table's triangle count
+// per configuration.  f(dense index, label, triangles, vertices) for every label with output; the
+// vertices are the point's own +axis edges with the label on exactly one end.  Fully unrolled: the
+// candidates stay in registers.
+template <class F>
+__device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const uint16_t c[8], uint16_t nbx, bool cube, F&& f)
 {
-    if (!I.cube) return 0;
-    uint32_t conf = 0;
+    const uint16_t own = c[0];
+    const uint16_t nb[3] = { nbx, c[4], c[1] };
+    const uint16_t cand[9] = { c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], nbx };
+    // candidates: every corner of a valid cube, else the point and its in-lattice +y / +z neighbours; +x
+    uint32_t valid = cube ? 0xFFu : (1u | (c[1] != kMcNoLabel ? 2u : 0u) | (c[4] != kMcNoLabel ? 16u : 0u));
+    valid |= nbx != kMcNoLabel ? 256u : 0u;
+    // nothing changes around the point: no surface, no vertex
+    bool uniform = true;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) conf |= (uint32_t)(I.corner[i] != L) << i;
-    return (uint32_t)(c_mc_tri[conf] >> 60);
-}
-
-__device__ inline uint32_t vert_count_for(const PointInfo& I, uint16_t L)
-{
-    uint32_t n = 0;
+    for (int i = 1; i < 9; ++i) uniform &= !((valid >> i) & 1u) || cand[i] == own;
+    if (uniform) return;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        n += (I.nbr[a] != kMcNoLabel && I.own != I.nbr[a] && (I.own == L || I.nbr[a] == L)) ? 1u : 0u;
-    return n;
-}
-
-__global__ __launch_bounds__(kMcThreads) void k_mc_count(McDev M)
-{
-    extern __shared__ uint32_t sh[];
-    uint32_t* ct = sh;             // nl
-    uint32_t* cv = sh + M.nl;      // nl
-    __shared__ uint32_t any;
-    for (uint32_t t = blockIdx.x; t < M.tiles; t += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < 2 * M.nl; i += blockDim.x) sh[i] = 0;
-        if (threadIdx.x == 0) any = 0;
-        __syncthreads();
-        for (uint32_t c = 0; c < M.k; ++c) {
-            const uint64_t p0 = ((uint64_t)t * M.k + c) * kMcChunk + (uint64_t)threadIdx.x * kMcPts;
-            for (uint32_t q = 0; q < kMcPts; ++q) {
-                const uint64_t p = p0 + q;
-                if (p >= M.P) break;
-                PointInfo I;
-                load_point(M, p, I);
-                // labels touching this point: cube corners and edge ends
-                uint16_t cand[11];
-                int nc = 0;
-                if (I.cube)
-                    for (int i = 0; i < 8; ++i) cand[nc++] = I.corner[i];
-                cand[nc++] = I.own;
-                for (int a = 0; a < 3; ++a)
-                    if (I.nbr[a] != kMcNoLabel) cand[nc++] = I.nbr[a];
-                for (int i = 0; i < nc; ++i) {
-                    const uint16_t L = cand[i];
-                    bool seen = false;
-                    for (int j = 0; j < i; ++j) seen |= cand[j] == L;
-                    if (seen) continue;
-                    const uint16_t li = M.lindex[L];
-                    if (li == kMcNoLabel) continue;
-                    const uint32_t nt = tri_count_for(I, L), nv = vert_count_for(I, L);
-                    if (nt) atomicAdd(ct + li, nt);
-                    if (nv) atomicAdd(cv + li, nv);
-                    if (nt | nv) any = 1;
-                }
-            }
+    for (int i = 0; i < 9; ++i) {
+        const uint16_t L = cand[i];
+        if (!((valid >> i) & 1u) || L < J.min_target) continue;
+        bool seen = false;
+#pragma unroll
+        for (int k = 0; k < i; ++k) seen |= ((valid >> k) & 1u) && cand[k] == L;
+        if (seen) continue;
+        const uint16_t li = gcast<const uint16_t>(J.lindex)[L];
+        if (li == kMcNoLabel) continue;
+        uint32_t nt = 0;
+        if (cube) {
+            uint32_t conf = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) conf |= (uint32_t)(c[k] != L) << k;
+            nt = ntri[conf];
         }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < M.nl; i += blockDim.x) {
-            M.mtri[(size_t)i * M.tiles + t] = ct[i];
-            M.mvert[(size_t)i * M.tiles + t] = cv[i];
-        }
-        if (threadIdx.x == 0) M.tile_total[t] = any;
-        __syncthreads();
+        uint32_t nv = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nv += (nb[a] != kMcNoLabel && own != nb[a] && (own == L || nb[a] == L)) ? 1u : 0u;
+        if (nt | nv) f(li, L, nt, nv);
     }
+}
+
+// Walks the points of unit `local` of job J: f(point index, masked corners, masked +x neighbour, valid
+// cube, raw corner 0) for each of its (up to) 8 points, unless the unit is uniform or holds no value above
+// FREE, when no point can have output.
+template <class F>
+__device__ inline void walk_unit(const McJob& J, unsigned long long local, F&& f)
+{
+    const uint32_t col = (uint32_t)(local / J.cz);
+    const int z0 = (int)((uint32_t)(local - (unsigned long long)col * J.cz) * kMcZ);
+    const int px = (int)(col / J.Ny), py = (int)(col - (uint32_t)px * J.Ny);
+    uint16_t A[kMcZ + 1], B[kMcZ + 1], C[kMcZ + 1], D[kMcZ + 1], E[kMcZ];
+    load_column<kMcZ + 1>(J, px, py, z0, A);
+    load_column<kMcZ + 1>(J, px - 1, py, z0, B);
+    load_column<kMcZ + 1>(J, px, py + 1, z0, C);
+    load_column<kMcZ + 1>(J, px - 1, py + 1, z0, D);
+    load_column<kMcZ>(J, px + 1, py, z0, E);
+    // uniform units (outside the vessel, inside one fragment) and units without a label emit nothing
+    uint32_t lo = 0xFFFFu, hi = 0;
+    auto acc = [&](uint16_t raw) {
+        const uint16_t m = mask15(raw);
+        if (m != kMcNoLabel) {
+            lo = min(lo, (uint32_t)m);
+            hi = max(hi, (uint32_t)m);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k <= (int)kMcZ; ++k) {
+        acc(A[k]);
+        acc(B[k]);
+        acc(C[k]);
+        acc(D[k]);
+        if (k < (int)kMcZ) acc(E[k]);
+    }
+    if (hi < J.min_target || lo == hi) return;
+    const uint32_t yz = J.Ny * J.Nz;
+#pragma unroll 1
+    for (int k = 0; k < (int)kMcZ; ++k) {
+        const int pz = z0 + k;
+        if (pz >= (int)J.Nz) break;
+        const uint16_t c[8] = { mask15(A[k]), mask15(A[k + 1]), mask15(B[k + 1]), mask15(B[k]),
+                                mask15(C[k]), mask15(C[k + 1]), mask15(D[k + 1]), mask15(D[k]) };
+        const uint16_t nbx = mask15(E[k]);
+        const bool cube = px >= 1 && py <= (int)J.Ny - 2 && pz <= (int)J.Nz - 2;
+        const uint32_t p = (uint32_t)px * yz + (uint32_t)py * J.Nz + (uint32_t)pz;
+        f(p, c, nbx, cube, A[k]);
+    }
+}
+
+__device__ inline uint32_t find_job_u64(const unsigned long long* prefix, uint32_t n, unsigned long long u)
+{
+    uint32_t lo = 0, hi = n;  // largest j with prefix[j] <= u
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t find_seg_job(const uint32_t* base, uint32_t n, uint32_t s)
+{
+    uint32_t lo = 0, hi = n;  // largest j with base[j] <= s
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= s) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t block_sum_u32(uint32_t v, uint32_t* wsum)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum[wave] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kMcThreads / 64; ++w) t += wsum[w];
+    __syncthreads();
+    return t;
 }
 
 __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wsum, uint32_t* total)
 {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t x = v;
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
+        const uint32_t y = __shfl_up(x, o, 64);
         if (lane >= (uint32_t)o) x += y;
     }
-    if (lane == 63) wsum[wave] = x;
+    if (lane == 63u) wsum[wave] = x;
     __syncthreads();
     uint32_t off = 0, tot = 0;
     for (uint32_t w = 0; w < kMcThreads / 64; ++w) {
@@ -192,139 +273,272 @@ __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wsum, uint3
     return off + x - v;
 }
 
-__device__ inline uint64_t edge_id(const McDev& M, int px, int py, int pz, int a, int b)
+// One block per grid: dense indices of its target labels, the values countValues reports (a nonzero
+// count; raw values above FREE are counted unmasked, so a flagged EMPTY or FREE cell makes 0 or 1 a
+// label too, as in the reference), or the one wanted.
+__global__ __launch_bounds__(kMcThreads) void k_mc_labels(McRun R)
+{
+    __shared__ uint32_t wsum[kMcThreads / 64];
+    __shared__ uint32_t first;
+    McJob& J = R.jobs[blockIdx.x];
+    constexpr uint32_t kPer = 65536 / kMcThreads;
+    const uint32_t l0 = threadIdx.x * kPer;
+    if (threadIdx.x == 0) first = 65536u;
+    auto target = [&](uint32_t l) { return J.hist[l] != 0 && (J.want == VF_ALL_LABELS || J.want == l); };
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < kPer; ++i) c += target(l0 + i) ? 1u : 0u;
+    uint32_t total;
+    uint32_t idx = block_excl_scan_u32(c, wsum, &total);
+    const bool ok = total <= kMcMaxLabels;
+    for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t l = l0 + i;
+        uint16_t li = kMcNoLabel;
+        if (ok && target(l)) {
+            J.lindex[65536 + idx] = (uint16_t)l;
+            li = (uint16_t)idx++;
+            atomicMin(&first, l);
+        }
+        J.lindex[l] = li;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R.job_nl[blockIdx.x] = ok ? total : 0u;
+        R.job_err[blockIdx.x] = ok ? 0u : 1u;
+        J.min_target = first;
+    }
+}
+
+// One block: segment bases (prefix of the label counts) and the label of every segment; clears the totals.
+__global__ __launch_bounds__(1024) void k_mc_base(McRun R)
+{
+    __shared__ uint32_t wsum[1024 / 64];
+    const uint32_t j = threadIdx.x, lane = j & 63u, wave = j >> 6;
+    const uint32_t v = j < R.njobs ? R.job_nl[j] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) wsum[wave] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < 1024 / 64; ++w) {
+        if (w < wave) off += wsum[w];
+        tot += wsum[w];
+    }
+    if (j < R.njobs) {
+        R.label_base[j] = off + x - v;
+        for (int k = 0; k < 4; ++k) R.job_tot[4 * j + k] = 0ull;
+    }
+    if (j == 0) R.label_base[R.njobs] = tot;
+    __syncthreads();
+    for (uint32_t g = 0; g < R.njobs; ++g) {
+        const uint32_t base = R.label_base[g], n = R.job_nl[g];
+        for (uint32_t i = j; i < n; i += blockDim.x) R.labels_all[base + i] = R.jobs[g].lindex[65536 + i];
+    }
+}
+
+// One thread per unit: pairs with output, their triangle and vertex counts.  A tile's sums go to its
+// first unit's grid with one atomic each (a few tiles straddle two grids: their other grid's units add
+// their own), never one atomic per unit on three shared words.
+__global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
+{
+    __shared__ unsigned long long sp[kMcMaxJobs + 1];
+    __shared__ uint32_t wsum[kMcThreads / 64];
+    __shared__ uint8_t ntri[256];
+    for (uint32_t i = threadIdx.x; i <= R.njobs; i += blockDim.x) sp[i] = R.unit_prefix[i];
+    ntri[threadIdx.x] = (uint8_t)(c_mc_tri[threadIdx.x] >> 60);
+    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
+        const unsigned long long u = (unsigned long long)t * kMcThreads + threadIdx.x;
+        const uint32_t j0 = find_job_u64(sp, R.njobs, (unsigned long long)t * kMcThreads);  // block-uniform
+        uint32_t pairs = 0, nts = 0, nvs = 0, j = j0;
+        if (u < R.total_units) {
+            j = find_job_u64(sp, R.njobs, u);
+            const McJob J = R.jobs[j];
+            walk_unit(J, u - sp[j], [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
+                point_labels(J, ntri, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv) {
+                    ++pairs;
+                    nts += nt;
+                    nvs += nv;
+                });
+            });
+            R.unit_pairs[u] = (uint8_t)pairs;
+            if (pairs && j != j0) {
+                atomicAdd(R.job_tot + 4 * j, (unsigned long long)pairs);
+                atomicAdd(R.job_tot + 4 * j + 1, (unsigned long long)nts);
+                atomicAdd(R.job_tot + 4 * j + 2, (unsigned long long)nvs);
+            }
+        }
+        const uint32_t tp = block_sum_u32(pairs, wsum);
+        if (tp == 0) {  // block-uniform: nothing in this tile
+            if (threadIdx.x == 0) R.tile_pairs[t] = 0;
+            continue;
+        }
+        const bool mine = j == j0;
+        const uint32_t p0 = block_sum_u32(mine ? pairs : 0u, wsum);
+        const uint32_t t0 = block_sum_u32(mine ? nts : 0u, wsum);
+        const uint32_t v0 = block_sum_u32(mine ? nvs : 0u, wsum);
+        if (threadIdx.x == 0) {
+            R.tile_pairs[t] = tp;
+            if (p0) {
+                atomicAdd(R.job_tot + 4 * j0, (unsigned long long)p0);
+                atomicAdd(R.job_tot + 4 * j0 + 1, (unsigned long long)t0);
+                atomicAdd(R.job_tot + 4 * j0 + 2, (unsigned long long)v0);
+            }
+        }
+    }
+}
+
+// The pairs of every unit, in unit order, from the tile offsets.
+__global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
+{
+    __shared__ unsigned long long sp[kMcMaxJobs + 1];
+    __shared__ uint32_t wsum[kMcThreads / 64];
+    __shared__ uint8_t ntri[256];
+    for (uint32_t i = threadIdx.x; i <= R.njobs; i += blockDim.x) sp[i] = R.unit_prefix[i];
+    ntri[threadIdx.x] = (uint8_t)(c_mc_tri[threadIdx.x] >> 60);
+    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
+        if (R.tile_pairs[t] == 0) continue;  // block-uniform
+        const unsigned long long u = (unsigned long long)t * kMcThreads + threadIdx.x;
+        const uint32_t n = u < R.total_units ? R.unit_pairs[u] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan_u32(n, wsum, &total);
+        if (n == 0) continue;
+        unsigned long long o = R.tile_off[t] + ex;
+        const uint32_t j = find_job_u64(sp, R.njobs, u);
+        const McJob J = R.jobs[j];
+        const uint32_t base = R.label_base[j];
+        walk_unit(J, u - sp[j], [&](uint32_t p, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
+            point_labels(J, ntri, c, nbx, cube, [&](uint16_t li, uint16_t, uint32_t nt, uint32_t nv) {
+                R.keys[o] = base + li;
+                R.vals[o] = (unsigned long long)p | ((unsigned long long)nt << 32) | ((unsigned long long)nv << 36);
+                ++o;
+            });
+        });
+    }
+}
+
+__global__ void k_mc_split(McRun R)
+{
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < R.npairs;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long v = R.svals[i];
+        R.cnt_t[i] = (uint32_t)((v >> 32) & 15u);
+        R.cnt_v[i] = (uint32_t)((v >> 36) & 15u);
+    }
+}
+
+__device__ inline unsigned long long cube_edge_id(const McJob& J, int px, int py, int pz, int a, int b)
 {
     const int ax = px + c_mc_nb[a][0], ay = py + c_mc_nb[a][1], az = pz + c_mc_nb[a][2];
     const int bx = px + c_mc_nb[b][0], by = py + c_mc_nb[b][1], bz = pz + c_mc_nb[b][2];
     const int axis = ax != bx ? 0 : (ay != by ? 1 : 2);
-    const uint64_t lo = ((uint64_t)min(ax, bx) * M.Ny + (uint64_t)min(ay, by)) * M.Nz + (uint64_t)min(az, bz);
-    return 3ull * lo + (uint64_t)axis;
+    const unsigned long long lo = ((unsigned long long)min(ax, bx) * J.Ny + (unsigned long long)min(ay, by)) * J.Nz +
+                                  (unsigned long long)min(az, bz);
+    return 3ull * lo + (unsigned long long)axis;
 }
 
-__global__ __launch_bounds__(kMcThreads) void k_mc_emit(McDev M)
+// Each sorted pair writes its vertices (own +axis edges, axis order) and triangles.
+__global__ void k_mc_emit(McRun R)
 {
-    __shared__ uint16_t present[256];
-    __shared__ uint32_t npresent;
-    __shared__ unsigned long long tbase[256], vbase[256];
-    __shared__ uint32_t wsum[kMcThreads / 64];
-    for (uint32_t t = blockIdx.x; t < M.tiles; t += gridDim.x) {
-        if (M.tile_total[t] == 0) continue;
-        // labels with output in this tile, in groups of 256
-        for (uint32_t g0 = 0; g0 < M.nl; g0 += 256) {
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t n = 0;
-                for (uint32_t li = g0; li < min(M.nl, g0 + 256); ++li) {
-                    const size_t m = (size_t)li * M.tiles + t;
-                    if (M.mtri[m] | M.mvert[m]) {
-                        present[n] = (uint16_t)li;
-                        tbase[n] = M.otri[m];
-                        vbase[n] = M.overt[m];
-                        ++n;
-                    }
-                }
-                npresent = n;
-            }
-            __syncthreads();
-            const uint32_t np = npresent;
-            if (np == 0) continue;
-            for (uint32_t c = 0; c < M.k; ++c) {
-                const uint64_t p0 = ((uint64_t)t * M.k + c) * kMcChunk + (uint64_t)threadIdx.x * kMcPts;
-                PointInfo I[kMcPts];
-                bool valid[kMcPts];
-                for (uint32_t q = 0; q < kMcPts; ++q) {
-                    valid[q] = p0 + q < M.P;
-                    if (valid[q]) load_point(M, p0 + q, I[q]);
-                }
-                for (uint32_t li_k = 0; li_k < np; ++li_k) {
-                    const uint32_t li = present[li_k];
-                    const uint16_t L = M.labels[li];
-                    uint32_t ntl = 0, nvl = 0;
-                    for (uint32_t q = 0; q < kMcPts; ++q)
-                        if (valid[q]) {
-                            ntl += tri_count_for(I[q], L);
-                            nvl += vert_count_for(I[q], L);
-                        }
-                    uint32_t tot;
-                    const uint32_t ex = block_excl_scan_u32((ntl << 16) | nvl, wsum, &tot);
-                    unsigned long long to = tbase[li_k] + (ex >> 16), vo = vbase[li_k] + (ex & 0xFFFFu);
-                    for (uint32_t q = 0; q < kMcPts; ++q) {
-                        if (!valid[q]) continue;
-                        const PointInfo& P = I[q];
-                        // vertices: +x, +y, +z edges in axis order
-                        for (int a = 0; a < 3; ++a) {
-                            if (P.nbr[a] != kMcNoLabel && P.own != P.nbr[a] && (P.own == L || P.nbr[a] == L)) {
-                                const uint64_t lo = ((uint64_t)P.px * M.Ny + (uint64_t)P.py) * M.Nz + (uint64_t)P.pz;
-                                M.vert_edge[vo++] = 3ull * lo + (uint64_t)a;
-                            }
-                        }
-                        if (P.cube) {
-                            uint32_t conf = 0;
-                            for (int i = 0; i < 8; ++i) conf |= (uint32_t)(P.corner[i] != L) << i;
-                            const unsigned long long w = c_mc_tri[conf];
-                            const int n = (int)(w >> 60);
-                            for (int s = 0; s < n; ++s) {
-                                const int e0 = (int)((w >> (12 * s)) & 15), e1 = (int)((w >> (12 * s + 4)) & 15),
-                                          e2 = (int)((w >> (12 * s + 8)) & 15);
-                                unsigned long long* dst = M.tri_edge + 3 * to;
-                                dst[0] = edge_id(M, P.px, P.py, P.pz, c_mc_edge[e0][0], c_mc_edge[e0][1]);
-                                dst[1] = edge_id(M, P.px, P.py, P.pz, c_mc_edge[e2][0], c_mc_edge[e2][1]);
-                                dst[2] = edge_id(M, P.px, P.py, P.pz, c_mc_edge[e1][0], c_mc_edge[e1][1]);
-                                M.tri_flag[to] = (uint8_t)((P.c0raw >> 15) & 1u);
-                                ++to;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                    if (threadIdx.x == 0) {
-                        tbase[li_k] += tot >> 16;
-                        vbase[li_k] += tot & 0xFFFFu;
-                    }
-                    __syncthreads();
-                }
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < R.npairs;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint32_t s = R.skeys[i];
+        const unsigned long long val = R.svals[i];
+        const uint32_t p = (uint32_t)val;
+        const uint32_t nt = (uint32_t)((val >> 32) & 15u);
+        const uint32_t j = find_seg_job(R.label_base, R.njobs, s);
+        const McJob& J = R.jobs[j];
+        const uint16_t L = R.labels_all[s];
+        const uint32_t yz = J.Ny * J.Nz;
+        const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
+        const uint16_t raw0 = lat_raw(J, px, py, pz);
+        const uint16_t own = mask15(raw0);
+        const uint16_t nb[3] = { mask15(lat_raw(J, px + 1, py, pz)), mask15(lat_raw(J, px, py + 1, pz)),
+                                 mask15(lat_raw(J, px, py, pz + 1)) };
+        unsigned long long vo = R.vpre[i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (nb[a] != kMcNoLabel && own != nb[a] && (own == L || nb[a] == L)) R.vert_edge[vo++] = 3ull * p + (unsigned long long)a;
+        if (nt) {
+            uint32_t conf = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                conf |= (uint32_t)(mask15(lat_raw(J, px + c_mc_nb[k][0], py + c_mc_nb[k][1], pz + c_mc_nb[k][2])) != L) << k;
+            const unsigned long long w = c_mc_tri[conf];
+            unsigned long long to = R.tpre[i];
+            for (uint32_t q = 0; q < nt; ++q, ++to) {
+                const int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
+                          e2 = (int)((w >> (12 * q + 8)) & 15);
+                unsigned long long* dst = R.tri_edge + 3 * to;
+                dst[0] = cube_edge_id(J, px, py, pz, c_mc_edge[e0][0], c_mc_edge[e0][1]);
+                dst[1] = cube_edge_id(J, px, py, pz, c_mc_edge[e2][0], c_mc_edge[e2][1]);
+                dst[2] = cube_edge_id(J, px, py, pz, c_mc_edge[e1][0], c_mc_edge[e1][1]);
+                R.tri_flag[to] = (uint8_t)((raw0 >> 15) & 1u);
             }
         }
     }
 }
 
-__global__ void k_mc_finish(McDev M, const unsigned long long* voff_l, const unsigned long long* toff_l)
+// voff / toff of every segment: the offsets of its first pair (pairs are sorted by segment).
+__global__ void k_mc_segments(McRun R)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < M.nv; v += stride) {
-        const uint64_t e = M.vert_edge[v];
-        const int axis = (int)(e % 3);
-        const uint64_t p = e / 3;
-        const uint32_t yz = M.Ny * M.Nz;
-        const uint32_t P[3] = { (uint32_t)(p / yz), (uint32_t)((p % yz) / M.Nz), (uint32_t)(p % M.Nz) };
-        for (int q = 0; q < 3; ++q) {
-            const float tq = (float)P[q] - 1.0f + (q == axis ? 0.5f : 0.0f);
-            M.xyz[3 * v + q] = M.mn[q] + M.cs[q] * tq;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= R.segments; s += gridDim.x * blockDim.x) {
+        unsigned long long lo = 0, hi = R.npairs;  // first pair with key >= s
+        while (lo < hi) {
+            const unsigned long long mid = (lo + hi) >> 1;
+            if (R.skeys[mid] < s) lo = mid + 1; else hi = mid;
         }
+        R.voff[s] = R.vpre[lo];
+        R.toff[s] = R.tpre[lo];
     }
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < M.nt; t += stride) {
-        // label of triangle t: largest l with toff_l[l] <= t
-        uint32_t lo = 0, hi = M.nl;
+}
+
+// Vertex positions and triangle indices local to their segment's vertices.
+__global__ void k_mc_finish(McRun R)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long v = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; v < R.nv; v += stride) {
+        uint32_t lo = 0, hi = R.segments;  // segment of v: largest s with voff[s] <= v
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (toff_l[mid] <= t) lo = mid; else hi = mid;
+            if (R.voff[mid] <= v) lo = mid; else hi = mid;
         }
-        const unsigned long long vb = voff_l[lo], ve = voff_l[lo + 1];
+        const McJob& J = R.jobs[find_seg_job(R.label_base, R.njobs, lo)];
+        const unsigned long long e = R.vert_edge[v];
+        const int axis = (int)(e % 3);
+        const unsigned long long p = e / 3;
+        const uint32_t yz = J.Ny * J.Nz;
+        const uint32_t P[3] = { (uint32_t)(p / yz), (uint32_t)((p % yz) / J.Nz), (uint32_t)(p % J.Nz) };
+        for (int q = 0; q < 3; ++q) {
+            const float tq = (float)P[q] - 1.0f + (q == axis ? 0.5f : 0.0f);
+            R.xyz[3 * v + q] = J.mn[q] + J.cs[q] * tq;
+        }
+    }
+    for (unsigned long long t = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; t < R.nt; t += stride) {
+        uint32_t lo = 0, hi = R.segments;  // segment of t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (R.toff[mid] <= t) lo = mid; else hi = mid;
+        }
+        const unsigned long long vb = R.voff[lo], ve = R.voff[lo + 1];
         for (int k = 0; k < 3; ++k) {
-            const unsigned long long e = M.tri_edge[3 * t + k];
+            const unsigned long long e = R.tri_edge[3 * t + k];
             unsigned long long a = vb, b = ve;
             while (a < b) {
                 const unsigned long long mid = (a + b) >> 1;
-                if (M.vert_edge[mid] < e) a = mid + 1; else b = mid;
+                if (R.vert_edge[mid] < e) a = mid + 1; else b = mid;
             }
-            M.tri_idx[3 * t + k] = (uint32_t)(a - vb);
+            R.tri_idx[3 * t + k] = (uint32_t)(a - vb);
         }
     }
 }
 
-__global__ void k_mc_offsets(const unsigned long long* o, uint32_t nl, uint32_t tiles, unsigned long long* per_label)
+__global__ void k_mc_zero_u64(unsigned long long* p, uint32_t n)
 {
-    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l <= nl; l += gridDim.x * blockDim.x)
-        per_label[l] = o[(size_t)l * tiles];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 bool g_table_loaded = false;
@@ -349,6 +563,10 @@ struct SmoothDev {
     uint32_t* deg;                       // nv incidence counts, then fill cursors
     const unsigned long long* adj_off;   // nv + 1
     uint32_t* adj;                       // 3 * nt triangle ids
+    const uint32_t* label_base;          // njobs + 1: first segment of each grid
+    uint32_t njobs;
+    uint32_t* vjob;                      // nv: grid of each vertex
+    const uint4* params;                 // per grid: non-boundary steps, boundary steps, their weights (float bits)
 };
 
 __global__ void k_smooth_prep(SmoothDev D)
@@ -360,9 +578,11 @@ __global__ void k_smooth_prep(SmoothDev D)
             if (D.toff[mid] <= t) lo = mid; else hi = mid;
         }
         const uint32_t base = (uint32_t)D.voff[lo];
+        const uint32_t job = find_seg_job(D.label_base, D.njobs, lo);
         for (int i = 0; i < 3; ++i) {
             const uint32_t g = base + D.tri[3 * (size_t)t + i];
             D.tri_g[3 * (size_t)t + i] = g;
+            D.vjob[g] = job;  // every triangle of a vertex belongs to the same grid
             atomicAdd(D.deg + g, 1u);
             if (D.tri_flag[t]) atomicOr(D.vflag + g, 1u);
         }
@@ -379,13 +599,19 @@ __global__ void k_smooth_fill(SmoothDev D)
         }
 }
 
-__global__ void k_smooth_step(SmoothDev D, const float* src, float* dst, uint32_t target, int check, float weight)
+// Step `it` of a pass: a vertex of the pass's class moves when its grid runs more than `it` steps of
+// that pass (pass 0: non-boundary vertices over faces of non-boundary vertices, pass 1: boundary
+// vertices over all faces), with its grid's weight; every other vertex keeps its position.
+__global__ void k_smooth_step(SmoothDev D, const float* src, float* dst, uint32_t target, int check, uint32_t it)
 {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < D.nv; v += gridDim.x * blockDim.x) {
         const float* p = src + 3 * (size_t)v;
         float* o = dst + 3 * (size_t)v;
         float px = p[0], py = p[1], pz = p[2];
-        if (D.vflag[v] == target) {
+        const uint4 prm = D.params[D.vjob[v]];
+        const uint32_t steps = target == 0 ? prm.x : prm.y;
+        const float weight = __uint_as_float(target == 0 ? prm.z : prm.w);
+        if (D.vflag[v] == target && it < steps) {
             int32_t sx = 0, sy = 0, sz = 0, cnt = 0;
             for (unsigned long long e = D.adj_off[v]; e < D.adj_off[v + 1]; ++e) {
                 const uint32_t* f = D.tri_g + 3 * (size_t)D.adj[e];
@@ -415,195 +641,397 @@ __global__ void k_smooth_step(SmoothDev D, const float* src, float* dst, uint32_
 
 }  // namespace
 
-void mc_smooth(McScratch& W, hipStream_t s, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight)
-{
-    if (!W.valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes mesh to smooth");
-    if (!(nb_weight >= 0.0f && nb_weight <= 1.0f && b_weight >= 0.0f && b_weight <= 1.0f))
-        fail(VF_ERR_INVALID_ARG, "smoothing weights must lie in [0, 1]");
-    if (W.nv == 0 || W.nt == 0 || (nb_iters == 0 && b_iters == 0)) return;
-    auto& B = W.buf;
-    SmoothDev D{};
-    D.nv = W.nv;
-    D.nt = W.nt;
-    D.nl = W.nl;
-    D.tri = B[12].as<uint32_t>();
-    D.tri_flag = B[8].as<uint8_t>();
-    D.voff = B[11].as<unsigned long long>();
-    D.toff = D.voff + (W.nl + 1);
-    B[14].ensure((size_t)W.nt * 12 + 64);
-    B[15].ensure((size_t)W.nv * 8 + 64);         // vflag, deg
-    B[13].ensure(std::max(scan_tmp_bytes(W.nv), (size_t)W.nt * 12) + 64);  // scan scratch, then adjacency
-    D.tri_g = B[14].as<uint32_t>();
-    D.vflag = B[15].as<uint32_t>();
-    D.deg = D.vflag + W.nv;
-    DevBuf& offs = W.buf[5];                     // reuse: nv + 1 adjacency offsets
-    offs.ensure(((size_t)W.nv + 1) * 8 + 64);
-    DevBuf& tmp = W.buf[2];                      // ping-pong positions
-    tmp.ensure((size_t)W.nv * 12 + 64);
-    VF_HIP(hipMemsetAsync(D.vflag, 0, (size_t)W.nv * 8, s));
-    const uint32_t gt = stride_blocks(W.nt, 256, 8192), gv = stride_blocks(W.nv, 256, 8192);
-    k_smooth_prep<<<gt, 256, 0, s>>>(D);
-    VF_CHECK_LAUNCH();
-    scan_exclusive_u32_to_u64(D.deg, offs.as<unsigned long long>(), W.nv, B[13].p, s);
-    VF_HIP(hipMemsetAsync(D.deg, 0, (size_t)W.nv * 4, s));
-    D.adj_off = offs.as<unsigned long long>();
-    D.adj = B[13].as<uint32_t>();                // the scan scratch is free again
-    k_smooth_fill<<<gt, 256, 0, s>>>(D);
-    VF_CHECK_LAUNCH();
-    float* a = B[10].as<float>();
-    float* b = tmp.as<float>();
-    auto pass = [&](uint32_t iters, float weight, uint32_t target, int check) {
-        for (uint32_t i = 0; i < iters; ++i) {
-            TimedLaunch tl("mc_smooth", s);
-            k_smooth_step<<<gv, 256, 0, s>>>(D, a, b, target, check, weight);
-            VF_CHECK_LAUNCH();
-            std::swap(a, b);
-        }
-    };
-    pass(nb_iters, nb_weight, 0, 1);
-    pass(b_iters, b_weight, 1, 0);
-    if (a != B[10].as<float>()) VF_HIP(hipMemcpyAsync(B[10].p, a, (size_t)W.nv * 12, hipMemcpyDeviceToDevice, s));
-    VF_HIP(hipStreamSynchronize(s));
-}
+namespace {
 
+// scratch slots
+enum : int {
+    kBJobs, kBLidx, kBUnitPre, kBBase, kBLabels, kBTot, kBUnit, kBTile, kBTileOff, kBScan, kBKeys0, kBKeys1, kBVals0,
+    kBVals1, kBSort, kBCntT, kBCntV, kBTpre, kBVpre, kBSeg, kBTEdge, kBFlag, kBVEdge, kBXyz, kBTri,
+    // smoothing
+    kBSmTriG, kBSmFlag, kBSmAdjOff, kBSmTmp, kBSmAdj, kBSmScan, kBSmPrm, kBCount
+};
+static_assert(kBCount <= McScratch::kBuffers, "scratch slots");
 
-void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
-            const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info)
+void load_table()
 {
     if (!g_table_loaded) {
         VF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri), kMcTriPacked, sizeof(kMcTriPacked)));
         g_table_loaded = true;
     }
-    // target labels: countValues (RegularGrid.cpp:601-625) -> labels with a nonzero count
-    std::vector<uint32_t> hist(65536);
-    VF_HIP(hipMemcpyAsync(hist.data(), hist_dev, 65536 * 4, hipMemcpyDeviceToHost, s));
-    VF_HIP(hipStreamSynchronize(s));
-    std::vector<uint16_t> labels, lindex(65536, (uint16_t)kMcNoLabel);
-    for (uint32_t l = 0; l < 65536; ++l)
-        if (hist[l] && (want == VF_ALL_LABELS || want == l)) {
-            lindex[l] = (uint16_t)labels.size();
-            labels.push_back((uint16_t)l);
-        }
-    const uint32_t nl = (uint32_t)labels.size();
-    if (nl > kMcMaxLabels) fail(VF_ERR_INVALID_ARG, "marching cubes supports at most %u labels (grid has %u)", kMcMaxLabels, nl);
-    const uint64_t Nx = dims[0] + 2ull, Ny = dims[1] + 2ull, Nz = dims[2] + 2ull;
-    const uint64_t P = Nx * Ny * Nz;
-    if (3 * P >= (1ull << 40)) fail(VF_ERR_INVALID_ARG, "grid too large for marching cubes");
-    W.nl = nl;
-    W.nv = W.nt = 0;
+}
+
+}  // namespace
+
+void McScratch::ensure(int slot, size_t n) { buf[slot].ensure(n, *cache); }
+
+void McScratch::release_all()
+{
+    for (auto& b : buf) b.release(*cache);
+    if (cache == &own) own.trim(0);
+    valid = false;
+}
+
+McScratch::~McScratch()
+{
+    for (auto& b : buf) b.release(*cache);
+}
+
+void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n, McJobResult* out,
+                 const std::function<bool(size_t)>& reserve)
+{
+    if (n == 0) return;
+    auto refuse = [&] {
+        for (uint32_t j = 0; j < n; ++j) out[j] = McJobResult{ 0, 0, 0, 0, 0, 0, VF_ERR_BUDGET };
+        W.valid = false;
+    };
+    if (n > kMcMaxJobs) fail(VF_ERR_INVALID_ARG, "marching cubes of %u grids at once (at most %u)", n, kMcMaxJobs);
+    load_table();
     W.valid = false;
-    auto& B = W.buf;
-    B[0].ensure(65536 * 2);
-    B[1].ensure((size_t)std::max<uint32_t>(nl, 1) * 2);
-    VF_HIP(hipMemcpyAsync(B[0].p, lindex.data(), 65536 * 2, hipMemcpyHostToDevice, s));
-    if (nl) VF_HIP(hipMemcpyAsync(B[1].p, labels.data(), (size_t)nl * 2, hipMemcpyHostToDevice, s));
-    B[11].ensure((size_t)(nl + 1) * 16 + 64);  // per-label offsets (voff | toff), u64
-    if (nl == 0) {
-        std::vector<unsigned long long> z(2, 0);
-        VF_HIP(hipMemcpyAsync(B[11].p, z.data(), 16, hipMemcpyHostToDevice, s));
-        VF_HIP(hipStreamSynchronize(s));
-        info->num_vertices = info->num_triangles = info->num_labels = 0;
-        W.valid = true;
+    // per grid: lattice, units, label index
+    W.ensure(kBLidx, (size_t)n * kLindexWords * 2);
+    std::vector<McJob> jobs(n);
+    std::vector<unsigned long long> upre(n + 1, 0);
+    for (uint32_t j = 0; j < n; ++j) {
+        const McJobSpec& q = spec[j];
+        McJob& J = jobs[j];
+        J.grid = q.grid;
+        J.hist = q.hist;
+        J.lindex = W.buf[kBLidx].as<uint16_t>() + (size_t)j * kLindexWords;
+        J.nx = q.dims[0]; J.ny = q.dims[1]; J.nz = q.dims[2];
+        J.Nx = q.dims[0] + 2; J.Ny = q.dims[1] + 2; J.Nz = q.dims[2] + 2;
+        const uint64_t P = (uint64_t)J.Nx * J.Ny * J.Nz;
+        if (P > 0xFFFFFFFFull) fail(VF_ERR_INVALID_ARG, "grid too large for marching cubes");
+        J.cz = (J.Nz + kMcZ - 1) / kMcZ;
+        J.want = q.want;
+        for (int a = 0; a < 3; ++a) {
+            J.mn[a] = q.mn[a];
+            J.cs[a] = q.cs[a];
+        }
+        upre[j + 1] = upre[j] + (unsigned long long)J.Nx * J.Ny * J.cz;
+    }
+    const unsigned long long units = upre[n];
+    const uint64_t tiles64 = (units + kMcThreads - 1) / kMcThreads;
+    if (tiles64 > 0xFFFFFFFFull) fail(VF_ERR_INVALID_ARG, "marching-cubes lattice too large");
+    // the first phase's buffers: label indices, per-unit pair counts, per-tile offsets and their scan
+    const size_t phase1 = (size_t)n * kLindexWords * 2 + n * sizeof(McJob) + (size_t)n * (8 + 12 + 32 + 2 * kMcMaxLabels) +
+                          units + tiles64 * 12 + scan_tmp_bytes(tiles64) + 10 * 320;
+    if (reserve && !reserve(BlockCache::class_size(phase1) + phase1 / 8)) {
+        refuse();
         return;
     }
-    // tiles: keep the (label, tile) matrices near 2M entries
-    const uint64_t tiles0 = (P + kMcChunk - 1) / kMcChunk;
-    uint32_t k = (uint32_t)std::max<uint64_t>(1, (nl * tiles0 + (2u << 20) - 1) / (2u << 20));
-    const uint32_t tiles = (uint32_t)((tiles0 + k - 1) / k);
-    const size_t mat = (size_t)nl * tiles;
-    B[2].ensure(mat * 4 + 64);
-    B[3].ensure(mat * 4 + 64);
-    B[4].ensure((size_t)tiles * 4 + 64);
-    B[5].ensure((mat + 1) * 8 + 64);
-    B[6].ensure((mat + 1) * 8 + 64);
-    B[13].ensure(scan_tmp_bytes(mat) + 64);
-
-    McDev M{};
-    M.grid = grid;
-    M.nx = dims[0]; M.ny = dims[1]; M.nz = dims[2];
-    M.Nx = (uint32_t)Nx; M.Ny = (uint32_t)Ny; M.Nz = (uint32_t)Nz;
-    M.P = P;
-    M.tiles = tiles;
-    M.k = k;
-    M.nl = nl;
-    M.lindex = B[0].as<uint16_t>();
-    M.labels = B[1].as<uint16_t>();
-    M.mtri = B[2].as<uint32_t>();
-    M.mvert = B[3].as<uint32_t>();
-    M.tile_total = B[4].as<uint32_t>();
-    M.otri = B[5].as<unsigned long long>();
-    M.overt = B[6].as<unsigned long long>();
-    for (int q = 0; q < 3; ++q) {
-        M.mn[q] = mn[q];
-        M.cs[q] = cs[q];
-    }
-    {
-        TimedLaunch tl("mc_count", s);
-        const uint32_t blocks = std::min<uint32_t>(tiles, 4096);
-        k_mc_count<<<blocks, kMcThreads, 2 * nl * sizeof(uint32_t), s>>>(M);
-        VF_CHECK_LAUNCH();
-    }
-    scan_exclusive_u32_to_u64(M.mtri, B[5].as<unsigned long long>(), mat, B[13].p, s);
-    scan_exclusive_u32_to_u64(M.mvert, B[6].as<unsigned long long>(), mat, B[13].p, s);
-    unsigned long long tot[2];
-    VF_HIP(hipMemcpyAsync(&tot[0], B[5].as<unsigned long long>() + mat, 8, hipMemcpyDeviceToHost, s));
-    VF_HIP(hipMemcpyAsync(&tot[1], B[6].as<unsigned long long>() + mat, 8, hipMemcpyDeviceToHost, s));
-    VF_HIP(hipStreamSynchronize(s));
-    if (tot[0] > 0xFFFFFFFFull || tot[1] > 0xFFFFFFFFull) fail(VF_ERR_OVERFLOW, "marching-cubes output exceeds 2^32 elements");
-    M.nt = (uint32_t)tot[0];
-    M.nv = (uint32_t)tot[1];
-    B[7].ensure((size_t)M.nt * 24 + 64);
-    B[8].ensure((size_t)M.nt + 64);
-    B[9].ensure((size_t)M.nv * 8 + 64);
-    B[10].ensure((size_t)M.nv * 12 + 64);
-    B[12].ensure((size_t)M.nt * 12 + 64);
-    M.tri_edge = B[7].as<unsigned long long>();
-    M.tri_flag = B[8].as<uint8_t>();
-    M.vert_edge = B[9].as<unsigned long long>();
-    M.xyz = B[10].as<float>();
-    M.tri_idx = B[12].as<uint32_t>();
-    {
-        TimedLaunch tl("mc_emit", s);
-        const uint32_t blocks = std::min<uint32_t>(tiles, 4096);
-        k_mc_emit<<<blocks, kMcThreads, 0, s>>>(M);
-        VF_CHECK_LAUNCH();
-    }
-    auto* voff_l = B[11].as<unsigned long long>();
-    auto* toff_l = voff_l + (nl + 1);
-    k_mc_offsets<<<1, 256, 0, s>>>(M.overt, nl, tiles, voff_l);
-    k_mc_offsets<<<1, 256, 0, s>>>(M.otri, nl, tiles, toff_l);
+    McRun R{};
+    R.njobs = n;
+    R.total_units = units;
+    R.tiles = (uint32_t)tiles64;
+    W.ensure(kBJobs, n * sizeof(McJob));
+    W.ensure(kBUnitPre, (n + 1) * 8);
+    W.ensure(kBBase, (3 * (size_t)n + 1) * 4);
+    W.ensure(kBTot, (size_t)n * 32);
+    W.ensure(kBLabels, (size_t)n * kMcMaxLabels * 2);
+    W.ensure(kBUnit, units + 64);
+    W.ensure(kBTile, (size_t)R.tiles * 4 + 64);
+    W.ensure(kBTileOff, ((size_t)R.tiles + 1) * 8 + 64);
+    VF_HIP(hipMemcpyAsync(W.buf[kBJobs].p, jobs.data(), n * sizeof(McJob), hipMemcpyHostToDevice, s));
+    VF_HIP(hipMemcpyAsync(W.buf[kBUnitPre].p, upre.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+    R.jobs = W.buf[kBJobs].as<McJob>();
+    R.unit_prefix = W.buf[kBUnitPre].as<unsigned long long>();
+    R.label_base = W.buf[kBBase].as<uint32_t>();
+    R.job_nl = R.label_base + n + 1;
+    R.job_err = R.job_nl + n;
+    R.labels_all = W.buf[kBLabels].as<uint16_t>();
+    R.job_tot = W.buf[kBTot].as<unsigned long long>();
+    R.unit_pairs = W.buf[kBUnit].as<uint8_t>();
+    R.tile_pairs = W.buf[kBTile].as<uint32_t>();
+    R.tile_off = W.buf[kBTileOff].as<unsigned long long>();
+    k_mc_labels<<<n, kMcThreads, 0, s>>>(R);
+    VF_CHECK_LAUNCH();
+    k_mc_base<<<1, 1024, 0, s>>>(R);
     VF_CHECK_LAUNCH();
     {
-        TimedLaunch tl("mc_finish", s);
-        const uint64_t n = std::max<uint64_t>(M.nv, M.nt);
-        k_mc_finish<<<stride_blocks(n, 256, 8192), 256, 0, s>>>(M, voff_l, toff_l);
+        TimedLaunch tl("mc_count", s);
+        k_mc_count<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, 0, s>>>(R);
         VF_CHECK_LAUNCH();
     }
+    W.ensure(kBScan, scan_tmp_bytes(R.tiles) + 64);
+    scan_exclusive_u32_to_u64(R.tile_pairs, W.buf[kBTileOff].as<unsigned long long>(), R.tiles, W.buf[kBScan].p, s);
+    // totals: pairs, triangles and vertices of every grid, segments, label-count errors
+    std::vector<unsigned long long> tot(4 * (size_t)n);
+    std::vector<uint32_t> base(3 * (size_t)n + 1);
+    VF_HIP(hipMemcpyAsync(tot.data(), R.job_tot, tot.size() * 8, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipMemcpyAsync(base.data(), R.label_base, base.size() * 4, hipMemcpyDeviceToHost, s));
     VF_HIP(hipStreamSynchronize(s));
-    W.nv = M.nv;
-    W.nt = M.nt;
-    info->num_vertices = M.nv;
-    info->num_triangles = M.nt;
-    info->num_labels = nl;
+    unsigned long long npairs = 0, NT = 0, NV = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        out[j].nl = base[n + 1 + j];
+        out[j].seg0 = base[j];
+        out[j].status = base[2 * n + 1 + j] ? VF_ERR_INVALID_ARG : VF_OK;
+        out[j].nt = (uint32_t)tot[4 * j + 1];
+        out[j].nv = (uint32_t)tot[4 * j + 2];
+        out[j].t0 = NT;
+        out[j].v0 = NV;
+        if (tot[4 * j + 1] > 0xFFFFFFFFull || tot[4 * j + 2] > 0xFFFFFFFFull)
+            fail(VF_ERR_OVERFLOW, "marching-cubes output of one grid exceeds 2^32 elements");
+        npairs += tot[4 * j];
+        NT += tot[4 * j + 1];
+        NV += tot[4 * j + 2];
+    }
+    const uint32_t S = base[n];
+    W.label_base.assign(base.begin(), base.begin() + n + 1);
+    if (reserve) {  // the second phase's buffers: pairs (double-buffered, counts, prefixes) and the meshes
+        size_t sort_bytes = 0;
+        if (npairs)
+            VF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                      (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                                      (int)npairs, 0, 32, s));
+        const size_t phase2 = (size_t)npairs * (4 + 4 + 8 + 8 + 4 + 4 + 8 + 8) + sort_bytes + scan_tmp_bytes(npairs) +
+                              (size_t)NT * (24 + 12 + 1) + (size_t)NV * (8 + 12) + 2 * ((size_t)S + 1) * 8 + 14 * 320;
+        if (!reserve(phase2 + phase2 / 8)) {
+            refuse();
+            return;
+        }
+    }
+    R.segments = S;
+    R.npairs = npairs;
+    R.nt = NT;
+    R.nv = NV;
+    if (npairs > 0xFFFFFFFFull || NT > 0xFFFFFFFFull || NV > 0xFFFFFFFFull)
+        fail(VF_ERR_OVERFLOW, "marching-cubes output exceeds 2^32 elements");
+    W.ensure(kBSeg, 2 * ((size_t)S + 1) * 8 + 64);
+    R.voff = W.buf[kBSeg].as<unsigned long long>();
+    R.toff = R.voff + (S + 1);
+    W.ensure(kBXyz, (size_t)NV * 12 + 64);
+    W.ensure(kBTri, (size_t)NT * 12 + 64);
+    W.ensure(kBFlag, (size_t)NT + 64);
+    R.xyz = W.buf[kBXyz].as<float>();
+    R.tri_idx = W.buf[kBTri].as<uint32_t>();
+    R.tri_flag = W.buf[kBFlag].as<uint8_t>();
+    if (npairs == 0) {
+        k_mc_zero_u64<<<1, 256, 0, s>>>(R.voff, 2 * (S + 1));
+        VF_CHECK_LAUNCH();
+    } else {
+        const size_t np = (size_t)npairs;
+        W.ensure(kBKeys0, np * 4 + 64);
+        W.ensure(kBKeys1, np * 4 + 64);
+        W.ensure(kBVals0, np * 8 + 64);
+        W.ensure(kBVals1, np * 8 + 64);
+        R.keys = W.buf[kBKeys0].as<uint32_t>();
+        R.vals = W.buf[kBVals0].as<unsigned long long>();
+        {
+            TimedLaunch tl("mc_pairs", s);
+            k_mc_pairs<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, 0, s>>>(R);
+            VF_CHECK_LAUNCH();
+        }
+        // stable sort by segment: (grid, label, point) order
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < (unsigned long long)S) ++bits;
+        size_t sort_bytes = 0;
+        VF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, R.keys, W.buf[kBKeys1].as<uint32_t>(), R.vals,
+                                                  W.buf[kBVals1].as<unsigned long long>(), (int)np, 0, bits, s));
+        W.ensure(kBSort, sort_bytes + 64);
+        {
+            TimedLaunch tl("mc_sort", s);
+            VF_HIP(hipcub::DeviceRadixSort::SortPairs(W.buf[kBSort].p, sort_bytes, R.keys, W.buf[kBKeys1].as<uint32_t>(),
+                                                      R.vals, W.buf[kBVals1].as<unsigned long long>(), (int)np, 0, bits,
+                                                      s));
+        }
+        R.skeys = W.buf[kBKeys1].as<uint32_t>();
+        R.svals = W.buf[kBVals1].as<unsigned long long>();
+        W.ensure(kBCntT, np * 4 + 64);
+        W.ensure(kBCntV, np * 4 + 64);
+        W.ensure(kBTpre, (np + 1) * 8 + 64);
+        W.ensure(kBVpre, (np + 1) * 8 + 64);
+        R.cnt_t = W.buf[kBCntT].as<uint32_t>();
+        R.cnt_v = W.buf[kBCntV].as<uint32_t>();
+        const uint32_t gp = stride_blocks(np, 256, 16384);
+        k_mc_split<<<gp, 256, 0, s>>>(R);
+        VF_CHECK_LAUNCH();
+        W.ensure(kBScan, scan_tmp_bytes(np) + 64);
+        scan_exclusive_u32_to_u64(R.cnt_t, W.buf[kBTpre].as<unsigned long long>(), np, W.buf[kBScan].p, s);
+        scan_exclusive_u32_to_u64(R.cnt_v, W.buf[kBVpre].as<unsigned long long>(), np, W.buf[kBScan].p, s);
+        R.tpre = W.buf[kBTpre].as<unsigned long long>();
+        R.vpre = W.buf[kBVpre].as<unsigned long long>();
+        W.ensure(kBTEdge, (size_t)NT * 24 + 64);
+        W.ensure(kBVEdge, (size_t)NV * 8 + 64);
+        R.tri_edge = W.buf[kBTEdge].as<unsigned long long>();
+        R.vert_edge = W.buf[kBVEdge].as<unsigned long long>();
+        {
+            TimedLaunch tl("mc_emit", s);
+            k_mc_emit<<<gp, 256, 0, s>>>(R);
+            VF_CHECK_LAUNCH();
+        }
+        k_mc_segments<<<stride_blocks((uint64_t)S + 1, 256, 4096), 256, 0, s>>>(R);
+        VF_CHECK_LAUNCH();
+        {
+            TimedLaunch tl("mc_finish", s);
+            k_mc_finish<<<stride_blocks(std::max(NV, NT), 256, 16384), 256, 0, s>>>(R);
+            VF_CHECK_LAUNCH();
+        }
+    }
+    VF_HIP(hipStreamSynchronize(s));
+    W.nv = (uint32_t)NV;
+    W.nt = (uint32_t)NT;
+    W.nl = S;
     W.valid = true;
+}
+
+void mc_run(McScratch& W, hipStream_t s, const uint16_t* grid, const uint32_t dims[3], const float mn[3], const float cs[3],
+            const uint32_t* hist_dev, uint32_t want, vf_mesh_info* info)
+{
+    McJobSpec q{};
+    q.grid = grid;
+    q.hist = hist_dev;
+    for (int a = 0; a < 3; ++a) {
+        q.dims[a] = dims[a];
+        q.mn[a] = mn[a];
+        q.cs[a] = cs[a];
+    }
+    q.want = want;
+    McJobResult r{};
+    mc_run_jobs(W, s, &q, 1, &r, nullptr);
+    if (r.status != VF_OK) {
+        W.valid = false;
+        fail(VF_ERR_INVALID_ARG, "marching cubes supports at most %u labels per grid", kMcMaxLabels);
+    }
+    info->num_vertices = r.nv;
+    info->num_triangles = r.nt;
+    info->num_labels = r.nl;
+}
+
+void mc_read_range(McScratch& W, hipStream_t s, uint32_t seg0, uint32_t nseg, float* xyz, uint32_t* tri, uint8_t* tri_flag,
+                   uint16_t* labels, uint32_t* voff, uint32_t* toff)
+{
+    auto& B = W.buf;
+    if ((uint64_t)seg0 + nseg > W.nl) fail(VF_ERR_INVALID_ARG, "mesh segment range out of bounds");
+    std::vector<unsigned long long> vo(nseg + 1), to(nseg + 1);
+    const auto* seg = B[kBSeg].as<unsigned long long>();
+    VF_HIP(hipMemcpyAsync(vo.data(), seg + seg0, (nseg + 1) * 8, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipMemcpyAsync(to.data(), seg + (W.nl + 1) + seg0, (nseg + 1) * 8, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipStreamSynchronize(s));
+    const unsigned long long v0 = vo[0], nv = vo[nseg] - v0, t0 = to[0], nt = to[nseg] - t0;
+    if (xyz && nv) VF_HIP(hipMemcpyAsync(xyz, B[kBXyz].as<float>() + 3 * v0, nv * 12, hipMemcpyDeviceToHost, s));
+    if (tri && nt) VF_HIP(hipMemcpyAsync(tri, B[kBTri].as<uint32_t>() + 3 * t0, nt * 12, hipMemcpyDeviceToHost, s));
+    if (tri_flag && nt) VF_HIP(hipMemcpyAsync(tri_flag, B[kBFlag].as<uint8_t>() + t0, nt, hipMemcpyDeviceToHost, s));
+    if (labels && nseg) VF_HIP(hipMemcpyAsync(labels, B[kBLabels].as<uint16_t>() + seg0, (size_t)nseg * 2, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipStreamSynchronize(s));
+    for (uint32_t l = 0; l <= nseg; ++l) {
+        if (voff) voff[l] = (uint32_t)(vo[l] - v0);
+        if (toff) toff[l] = (uint32_t)(to[l] - t0);
+    }
+}
+
+void mc_read_jobs(McScratch& W, hipStream_t s, const McJobResult* res, vf_mesh_out* const* outs, uint32_t n)
+{
+    if (!W.valid) return;
+    auto& B = W.buf;
+    std::vector<unsigned long long> seg(2 * ((size_t)W.nl + 1));
+    VF_HIP(hipMemcpyAsync(seg.data(), B[kBSeg].p, seg.size() * 8, hipMemcpyDeviceToHost, s));
+    VF_HIP(hipStreamSynchronize(s));
+    const unsigned long long* vo = seg.data();
+    const unsigned long long* to = seg.data() + W.nl + 1;
+    for (uint32_t j = 0; j < n; ++j) {
+        vf_mesh_out* o = outs[j];
+        if (!o) continue;
+        const McJobResult& r = res[j];
+        o->num_vertices = r.nv;
+        o->num_triangles = r.nt;
+        o->num_labels = r.nl;
+        o->copied = 0;
+        if (r.status != VF_OK || r.nv > o->cap_vertices || r.nt > o->cap_triangles || r.nl > o->cap_labels) continue;
+        if ((r.nv && !o->xyz) || (r.nt && !o->tri)) continue;
+        const unsigned long long v0 = vo[r.seg0], t0 = to[r.seg0];
+        if (r.nv) VF_HIP(hipMemcpyAsync(o->xyz, B[kBXyz].as<float>() + 3 * v0, (size_t)r.nv * 12, hipMemcpyDeviceToHost, s));
+        if (r.nt) VF_HIP(hipMemcpyAsync(o->tri, B[kBTri].as<uint32_t>() + 3 * t0, (size_t)r.nt * 12, hipMemcpyDeviceToHost, s));
+        if (r.nt && o->tri_flag) VF_HIP(hipMemcpyAsync(o->tri_flag, B[kBFlag].as<uint8_t>() + t0, r.nt, hipMemcpyDeviceToHost, s));
+        if (r.nl && o->labels)
+            VF_HIP(hipMemcpyAsync(o->labels, B[kBLabels].as<uint16_t>() + r.seg0, (size_t)r.nl * 2, hipMemcpyDeviceToHost, s));
+        for (uint32_t l = 0; l <= r.nl; ++l) {
+            if (o->vert_offsets) o->vert_offsets[l] = (uint32_t)(vo[r.seg0 + l] - v0);
+            if (o->tri_offsets) o->tri_offsets[l] = (uint32_t)(to[r.seg0 + l] - t0);
+        }
+        o->copied = 1;
+    }
+    VF_HIP(hipStreamSynchronize(s));
 }
 
 void mc_read(McScratch& W, hipStream_t s, float* xyz, uint32_t* tri, uint8_t* tri_flag, uint16_t* labels, uint32_t* voff,
              uint32_t* toff)
 {
-    auto& B = W.buf;
-    const uint32_t nl = W.nl;
-    if (xyz && W.nv) VF_HIP(hipMemcpyAsync(xyz, B[10].p, (size_t)W.nv * 12, hipMemcpyDeviceToHost, s));
-    if (tri && W.nt) VF_HIP(hipMemcpyAsync(tri, B[12].p, (size_t)W.nt * 12, hipMemcpyDeviceToHost, s));
-    if (tri_flag && W.nt) VF_HIP(hipMemcpyAsync(tri_flag, B[8].p, (size_t)W.nt, hipMemcpyDeviceToHost, s));
-    if (labels && nl) VF_HIP(hipMemcpyAsync(labels, B[1].p, (size_t)nl * 2, hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> off(2 * (size_t)(nl + 1));
-    VF_HIP(hipMemcpyAsync(off.data(), B[11].p, off.size() * 8, hipMemcpyDeviceToHost, s));
-    VF_HIP(hipStreamSynchronize(s));
-    for (uint32_t l = 0; l <= nl; ++l) {
-        if (voff) voff[l] = (uint32_t)off[l];
-        if (toff) toff[l] = (uint32_t)off[nl + 1 + l];
+    mc_read_range(W, s, 0, W.nl, xyz, tri, tri_flag, labels, voff, toff);
+}
+
+void mc_smooth_jobs(McScratch& W, hipStream_t s, const McSmooth* per_job, uint32_t njobs)
+{
+    if (!W.valid) fail(VF_ERR_INVALID_ARG, "no marching-cubes mesh to smooth");
+    if (njobs + 1 != W.label_base.size()) fail(VF_ERR_INVALID_ARG, "smoothing parameters for %u grids, mesh of %zu", njobs,
+                                                W.label_base.size() - 1);
+    uint32_t max_nb = 0, max_b = 0;
+    std::vector<uint4> prm(njobs);
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const McSmooth& q = per_job[j];
+        if (!(q.nb_weight >= 0.0f && q.nb_weight <= 1.0f && q.b_weight >= 0.0f && q.b_weight <= 1.0f))
+            fail(VF_ERR_INVALID_ARG, "smoothing weights must lie in [0, 1]");
+        uint32_t wn, wb;
+        std::memcpy(&wn, &q.nb_weight, 4);
+        std::memcpy(&wb, &q.b_weight, 4);
+        prm[j] = make_uint4(q.nb_iters, q.b_iters, wn, wb);
+        max_nb = std::max(max_nb, q.nb_iters);
+        max_b = std::max(max_b, q.b_iters);
     }
+    if (W.nv == 0 || W.nt == 0 || (max_nb == 0 && max_b == 0)) return;
+    auto& B = W.buf;
+    SmoothDev D{};
+    D.nv = W.nv;
+    D.nt = W.nt;
+    D.nl = W.nl;
+    D.tri = B[kBTri].as<uint32_t>();
+    D.tri_flag = B[kBFlag].as<uint8_t>();
+    D.voff = B[kBSeg].as<unsigned long long>();
+    D.toff = D.voff + (W.nl + 1);
+    D.label_base = B[kBBase].as<uint32_t>();
+    D.njobs = njobs;
+    W.ensure(kBSmTriG, (size_t)W.nt * 12 + 64);
+    W.ensure(kBSmFlag, (size_t)W.nv * 12 + 64);  // vflag, deg, vjob
+    W.ensure(kBSmScan, scan_tmp_bytes(W.nv) + 64);
+    W.ensure(kBSmAdj, (size_t)W.nt * 12 + 64);
+    W.ensure(kBSmAdjOff, ((size_t)W.nv + 1) * 8 + 64);
+    W.ensure(kBSmTmp, (size_t)W.nv * 12 + 64);  // ping-pong positions
+    W.ensure(kBSmPrm, (size_t)njobs * sizeof(uint4) + 64);
+    VF_HIP(hipMemcpyAsync(B[kBSmPrm].p, prm.data(), (size_t)njobs * sizeof(uint4), hipMemcpyHostToDevice, s));
+    D.params = B[kBSmPrm].as<uint4>();
+    D.tri_g = B[kBSmTriG].as<uint32_t>();
+    D.vflag = B[kBSmFlag].as<uint32_t>();
+    D.deg = D.vflag + W.nv;
+    D.vjob = D.deg + W.nv;
+    VF_HIP(hipMemsetAsync(D.vflag, 0, (size_t)W.nv * 8, s));
+    const uint32_t gt = stride_blocks(W.nt, 256, 8192), gv = stride_blocks(W.nv, 256, 8192);
+    k_smooth_prep<<<gt, 256, 0, s>>>(D);
+    VF_CHECK_LAUNCH();
+    scan_exclusive_u32_to_u64(D.deg, B[kBSmAdjOff].as<unsigned long long>(), W.nv, B[kBSmScan].p, s);
+    VF_HIP(hipMemsetAsync(D.deg, 0, (size_t)W.nv * 4, s));
+    D.adj_off = B[kBSmAdjOff].as<unsigned long long>();
+    D.adj = B[kBSmAdj].as<uint32_t>();
+    k_smooth_fill<<<gt, 256, 0, s>>>(D);
+    VF_CHECK_LAUNCH();
+    float* a = B[kBXyz].as<float>();
+    float* b = B[kBSmTmp].as<float>();
+    auto pass = [&](uint32_t iters, uint32_t target, int check) {
+        for (uint32_t i = 0; i < iters; ++i) {
+            TimedLaunch tl("mc_smooth", s);
+            k_smooth_step<<<gv, 256, 0, s>>>(D, a, b, target, check, i);
+            VF_CHECK_LAUNCH();
+            std::swap(a, b);
+        }
+    };
+    pass(max_nb, 0, 1);
+    pass(max_b, 1, 0);
+    if (a != B[kBXyz].as<float>()) VF_HIP(hipMemcpyAsync(B[kBXyz].p, a, (size_t)W.nv * 12, hipMemcpyDeviceToDevice, s));
+    VF_HIP(hipStreamSynchronize(s));
+}
+
+void mc_smooth(McScratch& W, hipStream_t s, uint32_t nb_iters, float nb_weight, uint32_t b_iters, float b_weight)
+{
+    const McSmooth q{ nb_iters, nb_weight, b_iters, b_weight };
+    mc_smooth_jobs(W, s, &q, 1);
 }
 
 void grid_marching_cubes(vf_grid* g, uint32_t want, vf_mesh_info* info)
@@ -625,8 +1053,8 @@ void grid_mesh_slice(vf_grid* g, uint32_t v0, uint32_t nv, uint32_t t0, uint32_t
 {
     auto& B = g->mc.buf;
     if ((uint64_t)v0 + nv > g->mc.nv || (uint64_t)t0 + nt > g->mc.nt) fail(VF_ERR_INVALID_ARG, "mesh slice out of range");
-    if (nv) VF_HIP(hipMemcpyAsync(xyz, B[10].as<float>() + 3ull * v0, (size_t)nv * 12, hipMemcpyDeviceToHost, g->stream));
-    if (nt) VF_HIP(hipMemcpyAsync(tri, B[12].as<uint32_t>() + 3ull * t0, (size_t)nt * 12, hipMemcpyDeviceToHost, g->stream));
+    if (nv) VF_HIP(hipMemcpyAsync(xyz, B[kBXyz].as<float>() + 3ull * v0, (size_t)nv * 12, hipMemcpyDeviceToHost, g->stream));
+    if (nt) VF_HIP(hipMemcpyAsync(tri, B[kBTri].as<uint32_t>() + 3ull * t0, (size_t)nt * 12, hipMemcpyDeviceToHost, g->stream));
     VF_HIP(hipStreamSynchronize(g->stream));
 }
 

@@ -41,6 +41,16 @@ class MeshInfo(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_labels", C.c_uint32)]
 
 
+class MeshOut(C.Structure):
+    _fields_ = [
+        ("xyz", C.c_void_p), ("tri", C.c_void_p), ("tri_flag", C.c_void_p), ("labels", C.c_void_p),
+        ("vert_offsets", C.c_void_p), ("tri_offsets", C.c_void_p),
+        ("cap_vertices", C.c_uint32), ("cap_triangles", C.c_uint32), ("cap_labels", C.c_uint32),
+        ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_labels", C.c_uint32),
+        ("copied", C.c_int),
+    ]
+
+
 class Job(C.Structure):
     _fields_ = [
         ("xyz", C.c_void_p), ("tri", C.c_void_p), ("nv", C.c_uint32), ("nt", C.c_uint32),
@@ -48,7 +58,8 @@ class Job(C.Structure):
         ("num_seeds", C.c_uint32), ("seed", C.c_uint64), ("seed_location", C.c_int),
         ("algorithm", C.c_int), ("distance", C.c_int), ("neighbourhood", C.c_int),
         ("noise_seed", C.c_uint64), ("noise_amp", C.c_uint32), ("boundary_size", C.c_int),
-        ("marching_cubes", C.c_int), ("mesh_on_device", C.c_int), ("poison", C.c_int),
+        ("marching_cubes", C.c_int), ("smooth_iterations", C.c_uint32 * 2), ("smooth_weights", C.c_float * 2),
+        ("mesh_out", C.c_void_p), ("mesh_on_device", C.c_int), ("poison", C.c_int),
         ("labels_out", C.c_void_p), ("histogram_out", C.c_void_p),
     ]
 
@@ -373,9 +384,11 @@ def read_tuning(path):
 # ------------------------------------------------------------ batched / served
 def make_job(xyz, tri, dims, aabb_min, aabb_max, num_seeds, seed, algorithm, distance,
              neighbourhood=NEIGH_FROM_DISTANCE, noise_seed=0, noise_amp=0, boundary_size=1,
-             marching_cubes=0, seed_location=SEED_BOTH, poison=0, keep=None):
+             marching_cubes=0, seed_location=SEED_BOTH, poison=0, keep=None, smooth=None):
     """Fill a vf_job.  `keep` (a list) receives the arrays the job points to so
-    they outlive the call."""
+    they outlive the call.  smooth: optional (nonboundary_iterations,
+    nonboundary_weight, boundary_iterations, boundary_weight) for the extracted
+    meshes."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     tri = np.ascontiguousarray(tri, np.uint32)
     if keep is not None:
@@ -391,8 +404,41 @@ def make_job(xyz, tri, dims, aabb_min, aabb_max, num_seeds, seed, algorithm, dis
     j.algorithm, j.distance, j.neighbourhood = algorithm, distance, neighbourhood
     j.noise_seed, j.noise_amp = noise_seed, noise_amp
     j.boundary_size, j.marching_cubes, j.poison = boundary_size, marching_cubes, poison
+    if smooth is not None:
+        j.smooth_iterations[:] = [int(smooth[0]), int(smooth[2])]
+        j.smooth_weights[:] = [float(smooth[1]), float(smooth[3])]
     j.mesh_on_device = 0
     return j
+
+
+class MeshBuffers:
+    """Host buffers a job's marching-cubes mesh is copied into (vf_mesh_out), with numpy views."""
+
+    def __init__(self, cap_vertices, cap_triangles, cap_labels=4096):
+        self.xyz = np.zeros((cap_vertices, 3), np.float32)
+        self.tri = np.zeros((cap_triangles, 3), np.uint32)
+        self.flag = np.zeros(cap_triangles, np.uint8)
+        self.labels = np.zeros(cap_labels, np.uint16)
+        self.voff = np.zeros(cap_labels + 1, np.uint32)
+        self.toff = np.zeros(cap_labels + 1, np.uint32)
+        self.out = MeshOut()
+        o = self.out
+        o.xyz, o.tri, o.tri_flag = self.xyz.ctypes.data, self.tri.ctypes.data, self.flag.ctypes.data
+        o.labels, o.vert_offsets, o.tri_offsets = self.labels.ctypes.data, self.voff.ctypes.data, self.toff.ctypes.data
+        o.cap_vertices, o.cap_triangles, o.cap_labels = cap_vertices, cap_triangles, cap_labels
+
+    def attach(self, job):
+        job.mesh_out = C.addressof(self.out)
+        return job
+
+    def mesh(self):
+        """The copied mesh in the layout Grid.marching_cubes returns (None when it was not copied)."""
+        o = self.out
+        if not o.copied:
+            return None
+        nv, nt, nl = o.num_vertices, o.num_triangles, o.num_labels
+        return {"xyz": self.xyz[:nv].copy(), "tri": self.tri[:nt].copy(), "flag": self.flag[:nt].copy(),
+                "labels": self.labels[:nl].copy(), "voff": self.voff[:nl + 1].copy(), "toff": self.toff[:nl + 1].copy()}
 
 
 def _job_array(jobs):
