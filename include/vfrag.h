@@ -317,6 +317,24 @@ typedef struct {
     uint32_t smooth_iterations[2];
     float smooth_weights[2];
     vf_mesh_out* mesh_out; /* optional host copy of the mesh (marching_cubes = 1) */
+    /* Seeder::mergeSeeds as CADScene::fractureModel runs it (CADScene.cpp:647-655):
+     * extra_seeds more seeds drawn anywhere (VF_SEED_BOTH, key extra_seed) follow
+     * copies of the num_seeds fragment seeds; each of those takes its nearest
+     * fragment's label (merge_distance, first on ties) with a running sub-seed id
+     * in bits 8..15, and the job runs on fragments + merged list.  The flood then
+     * runs its sub-seed pass.  0 skips it. */
+    uint32_t extra_seeds;
+    uint64_t extra_seed;
+    int merge_distance;
+    /* RegularGrid::erode in place of detectBoundaries (CADScene.cpp:675-687):
+     * erosion_iterations steps of boundary detection and erosion with the
+     * VF_EROSION_* window of side erosion_size, then the isolated-cell pass
+     * (vf_erode); boundary_size is not used then. */
+    int erode;
+    int erosion_convolution;
+    uint32_t erosion_size, erosion_iterations;
+    float erosion_probability, erosion_threshold;
+    uint64_t erosion_seed;
     int mesh_on_device;
     int poison;            /* failure-path testing: the job fails with VF_ERR_POISON */
     uint16_t* labels_out;  /* optional host copy of the final grid (V entries) */

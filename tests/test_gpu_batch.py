@@ -122,6 +122,64 @@ def test_batch_marching_cubes_counts(vf):
         assert r["mc_triangles"] == mesh["tri"].shape[0] and r["mc_vertices"] == mesh["xyz"].shape[0]
 
 
+def _production_reference(m, S0, loc, extra, extra_key, mdist, erode):
+    """CADScene::fractureModel on the oracle: fragment seeds, mergeSeeds with the extra draw (the merged
+    list follows the fragments), the fracture, then erosion or boundary detection."""
+    g = O.occ_to_grid(O.voxelize(m["xyz"], m["tri"], *BOX, m["dims"]))
+    frags = O.seed_uniform(g, S0, m["seed"], loc)
+    seeds = frags
+    if extra:
+        ex = O.seed_uniform(g, extra, extra_key, vf_SEED_BOTH)
+        merged = O.seed_merge(frags, np.concatenate([frags, ex]), mdist)
+        seeds = np.concatenate([frags, merged])
+    if m["algo"] == 0:
+        lab = O.naive(g, seeds, m["dist"])
+    else:
+        lab = O.flood(g, seeds, 6 if m["dist"] == 1 else 26, m["noise_seed"], m["amp"])
+    if erode:
+        kind, size, its, prob, thr, key = erode
+        return O.erode(lab, kind, size, its, prob, thr, key)
+    return O.detect_boundaries(lab, 1)
+
+
+vf_SEED_BOTH = 2
+
+
+def test_batch_and_server_production_seeding_and_erosion_match_oracle(vf):
+    """The reference's default fracture sequence in batched and served jobs: fragment seeds on the surface,
+    twice as many extra seeds merged as sub-seeds (the flood's sub-seed pass), erosion on some jobs."""
+    keep = []
+    jobs, meta = mixed_jobs(vf, 6, 51, keep, edges=(40, 72), locations=[vf.SEED_OUTER])
+    erosions = [None, (1, 3, 3, 0.5, 0.5, 77), None, (0, 5, 2, 0.7, 0.4, 78), None, (2, 3, 1, 0.5, 0.5, 79)]
+    cfg = []
+    for i, (j, m, er) in enumerate(zip(jobs, meta, erosions)):
+        j.extra_seeds = 2 * m["S"] if i != 4 else 0
+        j.extra_seed = 1000 + i
+        j.merge_distance = i % 3
+        if er:
+            j.erode, j.erosion_convolution, j.erosion_size, j.erosion_iterations = 1, er[0], er[1], er[2]
+            j.erosion_probability, j.erosion_threshold, j.erosion_seed = er[3], er[4], er[5]
+        cfg.append((m["S"], vf.SEED_OUTER, j.extra_seeds, j.extra_seed, j.merge_distance, er))
+    with vf.Batch() as b:
+        res = b.run(jobs)
+    batched = [m["labels"].copy() for m in meta]
+    for i, (m, r, c) in enumerate(zip(meta, res, cfg)):
+        assert r["status"] == 0, (i, r)
+        ref = _production_reference(m, *c)
+        np.testing.assert_array_equal(batched[i], ref, err_msg=f"job {i}")
+        np.testing.assert_array_equal(m["hist"], O.histogram(ref), err_msg=f"job {i}")
+        assert r["checksum"] == O.checksum(ref)
+    # the plain and served forms give the same grids
+    plain = vf.plain_run(jobs)
+    assert [p["checksum"] for p in plain] == [r["checksum"] for r in res]
+    srv = vf.Server(64 << 30, max_slots=4)
+    served = [srv.wait(t) for t in [srv.submit(j) for j in jobs]]
+    srv.close()
+    assert [x["checksum"] for x in served] == [r["checksum"] for r in res]
+    for i, m in enumerate(meta):
+        np.testing.assert_array_equal(m["labels"], batched[i])
+
+
 def _mesh_equal(m, r):
     for k in ("labels", "voff", "toff", "tri", "flag", "xyz"):
         np.testing.assert_array_equal(m[k], r[k], err_msg=k)

@@ -206,14 +206,21 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
             Table& T = *b->table;
             run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T); });
             if (T.any_flood) {
-                // the first chunks run without a check; then one readback per chunk
+                // the first chunks run without a check; then one readback per chunk.  Jobs with merged
+                // seeds run the sub-seed pass after the first drain and grow again where it freed cells.
                 uint32_t chunks = 0;
+                bool sub_done = !T.any_sub;
                 for (;;) {
                     run_captured(E.s, b->g_rounds, b->use_graphs, [&] { E.rounds(T, kRoundChunk); });
                     if (++chunks < 3) continue;
                     E.gather(T, E.pinned);
                     VF_HIP(hipStreamSynchronize(E.s));
-                    if (all_drained(T, E.pinned)) break;
+                    if (all_drained(T, E.pinned)) {
+                        if (sub_done) break;
+                        E.flood_sub(T);
+                        sub_done = true;
+                        chunks = 2;  // check after the next chunk
+                    }
                     if ((uint64_t)chunks * kRoundChunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                 }
             }
@@ -257,12 +264,21 @@ int vf_plain_run(const vf_job* jobs, uint32_t n, vf_result* results)
                 E->front(*T);
                 if (T->any_flood) {
                     uint32_t chunk = 4, total = 0;
+                    bool sub_done = !T->any_sub;
                     for (;;) {
                         E->rounds(*T, chunk);
                         total += chunk;
                         E->gather(*T, E->pinned);
                         VF_HIP(hipStreamSynchronize(E->s));
-                        if (all_drained(*T, E->pinned)) break;
+                        if (all_drained(*T, E->pinned)) {
+                            if (sub_done) break;
+                            E->flood_sub(*T);  // merged seeds: sub-seed pass, then grow again
+                            sub_done = true;
+                            E->gather(*T, E->pinned);
+                            VF_HIP(hipStreamSynchronize(E->s));
+                            if (all_drained(*T, E->pinned)) break;
+                            chunk = 4;
+                        }
                         chunk = std::min<uint32_t>(chunk * 2, 32);
                         if (total > (1u << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                     }
@@ -496,11 +512,32 @@ void vf_server::loop()
                 tr.step(4);
                 tr.round_jobs += leveling.size();
                 std::vector<uint32_t> w(E.pinned, E.pinned + 16 * leveling.size());
-                std::vector<EJob*> drained, still;
+                std::vector<EJob*> drained, still, subs;
                 for (size_t i = 0; i < leveling.size(); ++i) {
                     const uint32_t* wi = w.data() + 16 * i;
                     const bool done = wi[2 * kStatFloodDone] || wi[2 * kStatFloodDone + 1] || wi[2 * kStatError];
-                    (done ? drained : still).push_back(leveling[i]);
+                    const bool sub = done && !wi[2 * kStatError] && leveling[i]->d.sub && !leveling[i]->sub_done;
+                    (sub ? subs : done ? drained : still).push_back(leveling[i]);
+                }
+                if (!subs.empty()) {  // merged seeds: the sub-seed pass, then they grow again or are done
+                    Table* U = E.make_table(subs);
+                    audit();
+                    E.flood_sub(*U);
+                    E.gather(*U, E.pinned);
+                    VF_HIP(hipStreamSynchronize(E.s));
+                    for (size_t i = 0; i < subs.size(); ++i) {
+                        subs[i]->sub_done = true;
+                        const uint32_t* wi = E.pinned + 16 * i;
+                        const bool done = wi[2 * kStatFloodDone] || wi[2 * kStatFloodDone + 1] || wi[2 * kStatError];
+                        (done ? drained : still).push_back(subs[i]);
+                    }
+                    E.drop_table(U);
+                    if (drained.empty()) {  // the restarted jobs rejoin the rounds with a fresh table
+                        leveling.swap(still);
+                        E.drop_table(lev);
+                        lev = nullptr;
+                        lev_dirty = true;
+                    }
                 }
                 if (!drained.empty()) {
                     Table* P = E.make_table(drained);

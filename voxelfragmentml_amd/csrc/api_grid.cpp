@@ -164,6 +164,7 @@ static void upload_seeds(vf_grid* g, const vf_seed* seeds, uint32_t S)
         g->job.S = S;
         g->job_dirty = true;
     }
+    g->job.S_draw = S;
     VF_HIP(hipStreamSynchronize(g->stream));
 }
 
@@ -401,6 +402,7 @@ int vf_seed_uniform(vf_grid* g, uint32_t S, uint64_t seed, int location, vf_seed
         g->seeds.ensure((size_t)S * sizeof(uint4));
         g->job.seeds = g->seeds.as<uint4>();
         g->job.S = S;
+        g->job.S_draw = S;
         g->job.seed_key = seed;
         g->job.seed_location = location;
         g->job_dirty = true;

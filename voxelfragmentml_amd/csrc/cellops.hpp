@@ -11,6 +11,34 @@
 
 namespace vf {
 
+// Bits of the correctly rounded float square root of an integer n < 2^24, exactly: the estimate
+// moves to a float neighbour while sqrt(n) lies beyond the midpoint towards it.  Midpoints of
+// adjacent floats have at most 25 significant bits, so their sums and squares are exact in double,
+// and no midpoint squares to an integer below 2^48, so there are no ties.
+__device__ inline uint32_t sqrt_rn_bits(uint32_t n)
+{
+    uint32_t b = __float_as_uint(__builtin_sqrtf((float)n));
+    const double dn = (double)n;
+    for (int it = 0; it < 4; ++it) {
+        const double f = (double)__uint_as_float(b);
+        if (b > 0u) {
+            const double lo = 0.5 * ((double)__uint_as_float(b - 1u) + f);
+            if (lo * lo > dn) {
+                --b;
+                continue;
+            }
+        }
+        const double hi = 0.5 * (f + (double)__uint_as_float(b + 1u));
+        if (hi * hi < dn) {
+            ++b;
+            continue;
+        }
+        break;
+    }
+    return b;
+}
+
+
 constexpr uint32_t kCellThreads = 256;
 constexpr uint32_t kCellsPerThread = 8;
 static_assert(kCellThreads * kCellsPerThread == kTileCells, "tile shape");

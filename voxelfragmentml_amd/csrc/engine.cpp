@@ -21,9 +21,14 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-    size_t labels, occ, bits, seeds, hist, stats, fctl, keys, bricks, mesh_xyz, mesh_tri, total;
+    size_t labels, occ, bits, seeds, hist, stats, fctl, keys, bricks, sub, erode, mask, mesh_xyz, mesh_tri, total;
     uint64_t V, words;
 };
+
+// seeds the fracture uses: the drawn fragments, then (mergeSeeds) their copies and the extra draw
+uint32_t seed_total(const vf_job& j) { return j.num_seeds + (j.extra_seeds ? j.num_seeds + j.extra_seeds : 0); }
+constexpr uint32_t kMaxMergeSeeds = 2048;  // fragments + extra seeds of one job (k_seed_merge)
+constexpr uint32_t kMaxErosionIterations = 64;
 
 int resolved_neigh(const vf_job& j)
 {
@@ -42,12 +47,15 @@ Layout layout_of(const vf_job& j)
     L.labels = o; o = align_up(o + L.V * 2);
     L.occ = o; o = align_up(o + L.V);
     L.bits = o; o = align_up(o + L.words * 8);
-    L.seeds = o; o = align_up(o + (size_t)std::max<uint32_t>(j.num_seeds, 1) * 16);
+    L.seeds = o; o = align_up(o + (size_t)std::max<uint32_t>(seed_total(j), 1) * 16);
     L.hist = o; o = align_up(o + 65536 * 4);
     L.stats = o; o = align_up(o + kStatCount * 8);
     L.fctl = o; o = align_up(o + 64 * 4);
     L.keys = o; if (flood) o = align_up(o + L.V * 4);
     L.bricks = o; if (flood) o = align_up(o + 16ull * relax_words_per_parity(relax_bricks(j.dims[0], j.dims[1], j.dims[2])));
+    L.sub = o; if (flood && j.extra_seeds) o = align_up(o + 8ull * seed_total(j));  // union-find parents + keep flags
+    L.erode = o; if (j.erode) o = align_up(o + L.V * 2);                          // erosion output grid
+    L.mask = o; if (j.erode) o = align_up(o + (size_t)kMaxErosionSide * kMaxErosionSide * kMaxErosionSide);
     L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
     L.mesh_tri = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nt * 12);
     L.total = o;
@@ -86,6 +94,22 @@ int job_validate(const vf_job& j, std::string& why)
         if (!(j.aabb_max[q] > j.aabb_min[q])) { why = "empty aabb"; return VF_ERR_INVALID_ARG; }
     if (j.nt && (!j.xyz || !j.tri)) { why = "mesh pointer is NULL"; return VF_ERR_INVALID_ARG; }
     if (j.num_seeds == 0 || j.num_seeds > kMaxDeviceSeeds) { why = "seed count out of range"; return VF_ERR_INVALID_ARG; }
+    if (j.extra_seeds) {
+        if (seed_total(j) > kMaxDeviceSeeds || j.num_seeds + j.extra_seeds > kMaxMergeSeeds) {
+            why = "too many seeds to merge";
+            return VF_ERR_INVALID_ARG;
+        }
+        if (j.merge_distance < VF_EUCLIDEAN || j.merge_distance > VF_CHEBYSHEV) { why = "unknown merge distance"; return VF_ERR_INVALID_ARG; }
+    }
+    if (j.erode) {
+        std::vector<uint8_t> mask;
+        float act = 0;
+        if (j.erosion_size == 0 || j.erosion_size > kMaxErosionSide || erosion_mask(j.erosion_convolution, j.erosion_size, mask, act) < 0) {
+            why = "bad erosion window";
+            return VF_ERR_INVALID_ARG;
+        }
+        if (j.erosion_iterations > kMaxErosionIterations) { why = "too many erosion iterations"; return VF_ERR_INVALID_ARG; }
+    }
     if (j.algorithm != VF_NAIVE && j.algorithm != VF_FLOOD) { why = "unknown algorithm"; return VF_ERR_INVALID_ARG; }
     if (j.distance < VF_EUCLIDEAN || j.distance > VF_CHEBYSHEV) { why = "unknown distance"; return VF_ERR_INVALID_ARG; }
     if (j.seed_location < VF_SEED_INNER || j.seed_location > VF_SEED_BOTH) { why = "unknown seed location"; return VF_ERR_INVALID_ARG; }
@@ -150,9 +174,13 @@ void Engine::alloc(EJob& e)
     d.shell_bits = reinterpret_cast<uint32_t*>(base + Lo.bits);
     d.toggle_bits = d.shell_bits + Lo.words;
     d.seeds = reinterpret_cast<uint4*>(base + Lo.seeds);
-    d.S = j.num_seeds;
+    d.S = seed_total(j);
+    d.S_draw = j.num_seeds;
+    d.S0 = j.num_seeds;
+    d.extra = j.extra_seeds;
+    d.merge_dist = j.merge_distance;
     d.dist = j.distance;
-    d.key_bits = key_bits_for(j.num_seeds);
+    d.key_bits = key_bits_for(d.S);
     d.neighbourhood = resolved_neigh(j);
     d.noise_seed = j.noise_seed;
     d.noise_amp = j.noise_amp;
@@ -163,13 +191,14 @@ void Engine::alloc(EJob& e)
     d.fctl = reinterpret_cast<uint32_t*>(base + Lo.fctl);
     d.active = 1;
     d.algo = j.algorithm;
-    d.bsize = j.boundary_size;
+    d.bsize = j.erode ? 0 : j.boundary_size;  // erosion runs its own boundary detection
     if (j.algorithm == VF_FLOOD) {
         d.keys = reinterpret_cast<uint32_t*>(base + Lo.keys);
         d.bflags = reinterpret_cast<unsigned long long*>(base + Lo.bricks);
         d.nbricks = relax_bricks(d.nx, d.ny, d.nz);
+        d.sub = j.extra_seeds ? reinterpret_cast<uint32_t*>(base + Lo.sub) : nullptr;
     } else {
-        const NaiveKeyForm kf = naive_key_form(j.dims, j.distance, j.num_seeds);
+        const NaiveKeyForm kf = naive_key_form(j.dims, j.distance, d.S);
         d.dist = kf.dist;
         d.naive_key64 = kf.key64;
     }
@@ -183,6 +212,17 @@ void Engine::alloc(EJob& e)
     }
     d.nt = j.nt;
     d.nv = j.nv;
+    e.erode_out = nullptr;
+    if (j.erode) {  // RegularGrid::erode's window and output grid
+        std::vector<uint8_t> mask;
+        float act = 0;
+        e.erode_side = (uint32_t)erosion_mask(j.erosion_convolution, j.erosion_size, mask, act);
+        e.erode_limit = act * j.erosion_threshold;
+        e.erode_out = reinterpret_cast<uint16_t*>(base + Lo.erode);
+        e.erode_mask = base + Lo.mask;
+        VF_HIP(hipMemcpyAsync(e.erode_mask, mask.data(), mask.size(), hipMemcpyHostToDevice, s));
+        VF_HIP(hipStreamSynchronize(s));  // the pageable mask goes out of scope
+    }
     e.words = Lo.words;
     e.tiles = ceil_div(Lo.V, kTileCells);
     e.groups = ceil_div(Lo.words, 256);
@@ -216,8 +256,14 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     const size_t o_gather = align_up(o_bt + (n + 1) * 4);
     const size_t o_done = align_up(o_gather + 16 * 4 * (size_t)n);
     const size_t o_fcnt = align_up(o_done + 4 * ((size_t)n + 1));
-    const size_t total = align_up(o_fcnt + 4 * 2 * kFloodXcds);
+    const size_t o_xjobs = align_up(o_fcnt + 4 * 2 * kFloodXcds);  // extra-seed draws (mergeSeeds)
+    const size_t o_ejobs = align_up(o_xjobs + n * sizeof(JobDev)); // erosion passes (boundary size 1)
+    const size_t o_local = align_up(o_ejobs + n * sizeof(JobDev)); // per job: {0, tiles}, {0, bricks}
+    const size_t total = align_up(o_local + 16 * (size_t)n);
     std::vector<uint8_t> img(total, 0);
+    auto* xj = reinterpret_cast<JobDev*>(img.data() + o_xjobs);
+    auto* ej = reinterpret_cast<JobDev*>(img.data() + o_ejobs);
+    auto* lp = reinterpret_cast<uint32_t*>(img.data() + o_local);
     auto* jd = reinterpret_cast<JobDev*>(img.data() + o_jobs);
     auto* tp = reinterpret_cast<uint32_t*>(img.data() + o_tile);
     auto* rp = reinterpret_cast<uint32_t*>(img.data() + o_tri);
@@ -244,6 +290,23 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         t->any_flood |= e.d.algo == VF_FLOOD;
         t->any_naive32 |= e.d.algo == VF_NAIVE && !e.d.naive_key64;
         t->any_naive64 |= e.d.algo == VF_NAIVE && e.d.naive_key64;
+        // the extra draw: S_draw = extra seeds anywhere, with their own key, written after the fragments'
+        // copies; jobs without extra seeds sit it out
+        xj[i] = e.d;
+        xj[i].active = e.d.extra ? 1 : 0;
+        xj[i].S_draw = e.d.extra;
+        xj[i].seed_key = e.spec.extra_seed;
+        xj[i].seed_location = VF_SEED_BOTH;
+        xj[i].seeds = e.d.seeds + 2 * (size_t)e.d.S0;
+        t->any_extra |= e.d.extra != 0;
+        t->any_sub |= e.d.sub != nullptr;
+        ej[i] = e.d;
+        ej[i].bsize = 1;
+        t->any_erode |= e.spec.erode != 0;
+        lp[4 * i + 0] = 0;
+        lp[4 * i + 1] = e.tiles;
+        lp[4 * i + 2] = 0;
+        lp[4 * i + 3] = brick_count(e.d.nx, e.d.ny, e.d.nz);
     }
     if (tiles > 0xFFFFFFFFull || tris > 0xFFFFFFFFull || groups > 0xFFFFFFFFull || fchunks > 0xFFFFFFFFull)
         fail(VF_ERR_INVALID_ARG, "table too large for 32-bit prefixes");
@@ -269,6 +332,10 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
     t->total_groups = (uint32_t)groups;
     t->gather = reinterpret_cast<uint32_t*>(base + o_gather);
     t->bucket = tune_bucket(max_edge);
+    t->xL = t->L;
+    t->xL.jobs = reinterpret_cast<const JobDev*>(base + o_xjobs);
+    t->ejobs = reinterpret_cast<const JobDev*>(base + o_ejobs);
+    t->local_prefix = reinterpret_cast<const uint32_t*>(base + o_local);
     t->flood.task_prefix = reinterpret_cast<const uint32_t*>(base + o_done);
     t->flood.total_tasks = (uint32_t)fchunks;
     t->flood.counts = reinterpret_cast<uint32_t*>(base + o_fcnt);
@@ -333,6 +400,12 @@ void Engine::front(Table& t)
     launch_seed_count(L, seed_cnt_.as<uint32_t>(), true);
     scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
     launch_seed_pick(L, seed_off_.as<unsigned long long>());
+    if (t.any_extra) {  // mergeSeeds: the extra draw over all occupied cells, then the merge
+        launch_seed_count(t.xL, seed_cnt_.as<uint32_t>(), false);
+        scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
+        launch_seed_pick(t.xL, seed_off_.as<unsigned long long>());
+        launch_seed_merge(L);
+    }
     if (t.any_naive32) launch_naive(L, false);
     if (t.any_naive64) launch_naive(L, true);
     if (t.any_flood) {
@@ -352,9 +425,42 @@ void Engine::post(Table& t)
     set_active_bucket(t.bucket);
     const Launch& L = t.L;
     if (t.any_flood) launch_flood_finish(L);
+    if (t.any_erode) erode(t);
     launch_boundaries_tiled(L, t.bt_prefix, t.total_btiles, t.any_wide, true);  // + grid checksum
     launch_hist_summary(L);
     launch_gather(L, t.gather);
+}
+
+// RegularGrid::erode (RegularGrid.cpp:82-159) for each job that asks for it, in place of boundary detection:
+// iterations x (detectBoundaries(1), one erosion step), then the isolated-cell pass; its histogram is
+// cleared afterwards (the boundary passes count into it) and recounted by the post stage.  Launch-only.
+void Engine::erode(Table& t)
+{
+    for (uint32_t i = 0; i < t.L.njobs; ++i) {
+        const EJob& e = *t.jobs[i];
+        if (!e.spec.erode) continue;
+        Launch one = t.L;
+        one.jobs = t.ejobs + i;
+        one.njobs = 1;
+        one.tile_prefix = t.local_prefix + 4 * i;
+        one.total_tiles = e.tiles;
+        const uint32_t bricks = brick_count(e.d.nx, e.d.ny, e.d.nz);
+        for (uint32_t it = 0; it < e.spec.erosion_iterations; ++it) {
+            launch_boundaries_tiled(one, t.local_prefix + 4 * i + 2, bricks, false, false);
+            launch_erode_step(one, e.erode_out, e.erode_mask, e.erode_side, e.erode_limit, e.spec.erosion_probability,
+                              e.spec.erosion_seed, e.d.V);
+            VF_HIP(hipMemcpyAsync(e.d.labels, e.erode_out, e.d.V * 2, hipMemcpyDeviceToDevice, s));
+        }
+        launch_isolated_cells(one, e.erode_out, e.d.V);
+        VF_HIP(hipMemcpyAsync(e.d.labels, e.erode_out, e.d.V * 2, hipMemcpyDeviceToDevice, s));
+        launch_clear_hist(one);
+    }
+}
+
+void Engine::flood_sub(Table& t)
+{
+    set_active_bucket(t.bucket);
+    launch_flood_sub(t.L);
 }
 
 void Engine::gather(Table& t, uint32_t* host16)

@@ -33,6 +33,11 @@ struct EJob {
     double t_submit = 0, t_admit = 0;
     int phase = 0;
     int vstatus = 0;                 // job_validate's status when the job was prepared
+    bool sub_done = false;           // the flood's sub-seed pass has run (served form)
+    uint16_t* erode_out = nullptr;   // erosion output grid and window (jobs with erode set)
+    uint8_t* erode_mask = nullptr;
+    uint32_t erode_side = 0;
+    float erode_limit = 0;
     bool cancelled = false;
     bool admitted = false;           // server: holds a slot and budget until completed
 };
@@ -61,6 +66,10 @@ struct Table {
     void* flood_lists = nullptr;    // separate allocation of flood.lists
     size_t dev_bytes = 0, list_bytes = 0;  // class sizes of the two blocks
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
+    bool any_extra = false, any_sub = false, any_erode = false;
+    Launch xL{};                          // the jobs as their extra-seed draw sees them
+    const JobDev* ejobs = nullptr;        // the jobs as their erosion passes see them (boundary size 1)
+    const uint32_t* local_prefix = nullptr;  // per job {0, tiles, 0, bricks}: one-job launches
 };
 
 class Engine {
@@ -89,6 +98,8 @@ public:
     void front(Table& t);               // launches only
     void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
+    void erode(Table& t);      // per-job erosion (post calls it)
+    void flood_sub(Table& t);  // the flood's sub-seed pass for the jobs with merged seeds
     void gather(Table& t, uint32_t* host16);  // async copy of the gathered stats
     // fill j.res from 16 gathered words and make the optional grid and histogram copies
     void finish(EJob& j, const uint32_t* w16);

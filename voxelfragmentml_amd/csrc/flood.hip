@@ -846,6 +846,7 @@ __global__ __launch_bounds__(1024) void k_flood_sub_keep(Launch L)
         J->fctl[2] = 0;
         J->fctl[3] = 1;
         J->fctl[4] = J->fctl[5] = 0;  // the next compaction starts round 1
+        J->stats[kStatFloodDone] = 0;  // running again (multi-job drivers watch this word)
     }
 }
 

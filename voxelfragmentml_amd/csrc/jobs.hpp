@@ -37,6 +37,7 @@ enum DevError : uint32_t {
     kDevErrKeyOverflow = 2, // flood distance no longer fits the packed key
     kDevErrRelax = 4,       // a flood brick did not settle within its sweep limit
     kDevErrMesh = 8,        // a triangle references a vertex index >= nv (the triangle is skipped)
+    kDevErrMerge = 16,      // mergeSeeds: a fragment would get more than 255 sub-seeds
 };
 
 struct TriSetup {
@@ -62,7 +63,10 @@ struct JobDev {
     uint32_t* toggle_bits;  // nx*ny*wp words
     // seeds {x,y,z,label}
     uint4* seeds;
-    uint32_t S;
+    uint32_t S;             // seeds the fracture uses
+    uint32_t S_draw;        // seeds a uniform draw picks (k_seed_pick), written from seeds[0]
+    uint32_t S0, extra;     // mergeSeeds: fragment seeds and extra seeds (extra = 0: no merge)
+    int merge_dist;         // mergeSeeds distance (VF_*)
     int dist;               // VF_* distance, or kDistEuclidF32 for naive Euclidean ranked by float roots
     uint32_t key_bits;      // bits for the seed index in packed keys
     int neighbourhood;      // 6 or 26
@@ -242,6 +246,11 @@ constexpr uint32_t kMaxDeviceSeeds = 4096;
 void launch_seed_count(const Launch& L, uint32_t* tile_cnt, bool skip_both);
 // tile_off: exclusive prefix over the global tiles (total_tiles + 1 entries)
 void launch_seed_pick(const Launch& L, const unsigned long long* tile_off);
+// Seeder::mergeSeeds for every job with extra seeds (one block each): seeds[S0..2 S0) get copies of the
+// fragment seeds [0, S0), seeds[2 S0..2 S0 + extra) hold the extra draw; each of these S0 + extra seeds
+// takes its nearest fragment's label with a running sub-seed id in bits 8..15 (more than 255 on one
+// fragment flags kDevErrMerge).
+void launch_seed_merge(const Launch& L);
 // Seeder::nearSeeds on job 0 of L; out: F + num_seeds entries; result[2]: count, draw-cap flag
 void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
                       uint32_t spreading, uint64_t key, uint4* out, uint32_t* result);

@@ -32,33 +32,6 @@ __device__ inline K pack_key(uint32_t d, uint32_t i, uint32_t b)
 // coordinates are below 2^11 (grid edges of at most 2048 cells), so the halves never wrap.
 __device__ inline uint32_t sad16(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u16(a, b, acc); }
 
-// Bits of the correctly rounded float square root of an integer n < 2^24, exactly: the estimate
-// moves to a float neighbour while sqrt(n) lies beyond the midpoint towards it.  Midpoints of
-// adjacent floats have at most 25 significant bits, so their sums and squares are exact in double,
-// and no midpoint squares to an integer below 2^48, so there are no ties.
-__device__ inline uint32_t sqrt_rn_bits(uint32_t n)
-{
-    uint32_t b = __float_as_uint(__builtin_sqrtf((float)n));
-    const double dn = (double)n;
-    for (int it = 0; it < 4; ++it) {
-        const double f = (double)__uint_as_float(b);
-        if (b > 0u) {
-            const double lo = 0.5 * ((double)__uint_as_float(b - 1u) + f);
-            if (lo * lo > dn) {
-                --b;
-                continue;
-            }
-        }
-        const double hi = 0.5 * (f + (double)__uint_as_float(b + 1u));
-        if (hi * hi < dn) {
-            ++b;
-            continue;
-        }
-        break;
-    }
-    return b;
-}
-
 // Distance key of one cell to one seed; xy and sxy hold x | y << 16.  kDistEuclidF32 (64-bit keys
 // only) ranks by the bits of the correctly rounded float root of the squared distance, which is
 // below 2^24 and so converts exactly: the reference's float sqrt (NaiveFracturer.cpp:12-15).

@@ -105,6 +105,7 @@ extern "C" int vf_fracture_naive_noalloc(uint16_t* d_cells, const uint32_t dims[
         J.occ = base + kOffOcc;
         J.seeds = reinterpret_cast<uint4*>(base + kOffSeeds);
         J.S = S;
+        J.S_draw = S;
         J.dist = kf.dist;
         J.key_bits = key_bits_for(S);
         J.naive_key64 = kf.key64;

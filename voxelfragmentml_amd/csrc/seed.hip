@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
     const uint32_t j = blockIdx.x;
     const JobDev* J = L.jobs + j;
     if (!J->active) return;
-    const uint32_t S = J->S;
+    const uint32_t S = J->S_draw;
     const uint32_t t0 = L.tile_prefix[j], t1 = L.tile_prefix[j + 1];
     const unsigned long long base = tile_off[t0];
     const unsigned long long O = tile_off[t1] - base;
@@ -221,6 +221,55 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
     }
 }
 
+// Seeder::mergeSeeds (Seeder.cpp:115-152) in CADScene::fractureModel's order (CADScene.cpp:647-655): the
+// merged list is the fragment seeds followed by the extra draw; each entry takes the label of its nearest
+// fragment seed (the reference's float distances on the coordinates, strict '<': the first fragment wins
+// ties) with that fragment's running sub-seed id (1, 2, ... in list order) in bits 8..15.  The fragments
+// stay first, the merged list follows them.  One block per job; entries staged in LDS.
+constexpr uint32_t kMaxMerge = 2048;  // fragments + extra seeds of one job
+__global__ __launch_bounds__(kSeedThreads) void k_seed_merge(Launch L)
+{
+    __shared__ uint32_t px[kMaxMerge], py[kMaxMerge], pz[kMaxMerge];
+    __shared__ uint16_t near[kMaxMerge];
+    const JobDev* J = L.jobs + blockIdx.x;
+    if (!J->active || J->extra == 0) return;
+    const uint32_t F = J->S0, M = J->S0 + J->extra;
+    uint4* seeds = J->seeds;
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        const uint4 q = i < F ? seeds[i] : seeds[2 * F + (i - F)];
+        px[i] = q.x;
+        py[i] = q.y;
+        pz[i] = q.z;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        uint32_t best = 0xFFFFFFFFu, arg = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            const uint32_t dx = px[i] > px[f] ? px[i] - px[f] : px[f] - px[i];
+            const uint32_t dy = py[i] > py[f] ? py[i] - py[f] : py[f] - py[i];
+            const uint32_t dz = pz[i] > pz[f] ? pz[i] - pz[f] : pz[f] - pz[i];
+            // integer forms of the float distances (coordinates below 2^11): Manhattan and Chebyshev
+            // exact, Euclidean as the correctly rounded float root of the squared distance
+            const uint32_t d = J->merge_dist == VF_MANHATTAN ? dx + dy + dz
+                               : J->merge_dist == VF_CHEBYSHEV ? max(dx, max(dy, dz))
+                                                               : sqrt_rn_bits(dx * dx + dy * dy + dz * dz);
+            if (d < best) {
+                best = d;
+                arg = f;
+            }
+        }
+        near[i] = (uint16_t)arg;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        const uint32_t f = near[i];
+        uint32_t id = 1;
+        for (uint32_t k = 0; k < i; ++k) id += near[k] == f ? 1u : 0u;
+        if (id > 255u) atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrMerge);
+        seeds[F + i] = make_uint4(px[i], py[i], pz[i], (seeds[f].w & 0xFFu) | ((id & 0xFFu) << 8));
+    }
+}
+
 // floor(mn + (mx - mn) * h / 2^32): static_cast<int>(min + (max - min) * u) of
 // RandomUtilities::getUniformRandomInt (RandomUtilities.h:141-144) for
 // u = h / 2^32 wherever that is not negative.
@@ -330,6 +379,13 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_near(const JobDev* J, con
 }
 
 }  // namespace
+
+void launch_seed_merge(const Launch& L)
+{
+    TimedLaunch tl("seed_merge", L.s);
+    k_seed_merge<<<L.njobs, kSeedThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
 
 void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
                       uint32_t spreading, uint64_t key, uint4* out, uint32_t* result)

@@ -59,7 +59,12 @@ class Job(C.Structure):
         ("algorithm", C.c_int), ("distance", C.c_int), ("neighbourhood", C.c_int),
         ("noise_seed", C.c_uint64), ("noise_amp", C.c_uint32), ("boundary_size", C.c_int),
         ("marching_cubes", C.c_int), ("smooth_iterations", C.c_uint32 * 2), ("smooth_weights", C.c_float * 2),
-        ("mesh_out", C.c_void_p), ("mesh_on_device", C.c_int), ("poison", C.c_int),
+        ("mesh_out", C.c_void_p),
+        ("extra_seeds", C.c_uint32), ("extra_seed", C.c_uint64), ("merge_distance", C.c_int),
+        ("erode", C.c_int), ("erosion_convolution", C.c_int), ("erosion_size", C.c_uint32),
+        ("erosion_iterations", C.c_uint32), ("erosion_probability", C.c_float), ("erosion_threshold", C.c_float),
+        ("erosion_seed", C.c_uint64),
+        ("mesh_on_device", C.c_int), ("poison", C.c_int),
         ("labels_out", C.c_void_p), ("histogram_out", C.c_void_p),
     ]
 
