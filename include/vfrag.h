@@ -335,6 +335,18 @@ typedef struct {
     uint32_t erosion_size, erosion_iterations;
     float erosion_probability, erosion_threshold;
     uint64_t erosion_seed;
+    /* The caller's fragment seeds (num_seeds entries, host memory, labels in .w)
+     * in place of the uniform draw: CADScene::fractureModel's impact seeds
+     * (_impactSeeds, CADScene.cpp:643-646).  NULL draws them (seed,
+     * seed_location).  Flood labels keep the fragment id in the low 8 bits. */
+    const vf_seed* seeds;
+    /* Seeder::nearSeeds on the fragment seeds (CADScene.cpp:636-646, vf_seed_near):
+     * impacts times, up to impact_seeds new surface seeds in all around a
+     * fragment seed, spreading impact_spreading (1..64), key impact_seed.  The
+     * new seeds join the fragments, labelled after the last one, before
+     * mergeSeeds.  0 impacts skips it. */
+    uint32_t impacts, impact_seeds, impact_spreading;
+    uint64_t impact_seed;
     int mesh_on_device;
     int poison;            /* failure-path testing: the job fails with VF_ERR_POISON */
     uint16_t* labels_out;  /* optional host copy of the final grid (V entries) */

@@ -122,11 +122,15 @@ def test_batch_marching_cubes_counts(vf):
         assert r["mc_triangles"] == mesh["tri"].shape[0] and r["mc_vertices"] == mesh["xyz"].shape[0]
 
 
-def _production_reference(m, S0, loc, extra, extra_key, mdist, erode):
-    """CADScene::fractureModel on the oracle: fragment seeds, mergeSeeds with the extra draw (the merged
-    list follows the fragments), the fracture, then erosion or boundary detection."""
+def _production_reference(m, S0, loc, extra, extra_key, mdist, erode, frags=None, near=None):
+    """CADScene::fractureModel on the oracle: fragment seeds (drawn, or the given impact seeds), nearSeeds
+    when near = (impacts, seeds, spreading, key), mergeSeeds with the extra draw (the merged list follows
+    the fragments), the fracture, then erosion or boundary detection."""
     g = O.occ_to_grid(O.voxelize(m["xyz"], m["tri"], *BOX, m["dims"]))
-    frags = O.seed_uniform(g, S0, m["seed"], loc)
+    if frags is None:
+        frags = O.seed_uniform(g, S0, m["seed"], loc)
+    if near:
+        frags = O.seed_near(g, frags, *near)
     seeds = frags
     if extra:
         ex = O.seed_uniform(g, extra, extra_key, vf_SEED_BOTH)
@@ -178,6 +182,58 @@ def test_batch_and_server_production_seeding_and_erosion_match_oracle(vf):
     assert [x["checksum"] for x in served] == [r["checksum"] for r in res]
     for i, m in enumerate(meta):
         np.testing.assert_array_equal(m["labels"], batched[i])
+
+
+def test_batch_and_server_impact_seeding_match_oracle(vf):
+    """CADScene::fractureModel's impact paths in batched and served jobs: drawn fragment seeds followed by
+    nearSeeds (numImpacts > 0), and the caller's impact seeds followed by one nearSeeds impact, some jobs
+    with mergeSeeds after it.  nearSeeds' seed count is known only on the device."""
+    keep = []
+    jobs, meta = mixed_jobs(vf, 8, 61, keep, edges=(40, 72), locations=[vf.SEED_OUTER])
+    cfg = []
+    for i, (j, m) in enumerate(zip(jobs, meta)):
+        frags = None
+        if i % 2:  # the caller's seeds: surface cells of another draw, labels 2..S+1
+            g = O.occ_to_grid(O.voxelize(m["xyz"], m["tri"], *BOX, m["dims"]))
+            frags = np.ascontiguousarray(O.seed_uniform(g, m["S"], 500 + i, vf.SEED_OUTER))
+            keep.append(frags)
+            j.seeds = frags.ctypes.data
+        j.impacts = 1 if i % 2 else 1 + i % 3
+        j.impact_seeds = 4 + i
+        j.impact_spreading = 2 + i % 3
+        j.impact_seed = 900 + i
+        if i in (2, 3, 6):
+            j.extra_seeds = m["S"]
+            j.extra_seed = 70 + i
+            j.merge_distance = i % 3
+        near = (j.impacts, j.impact_seeds, j.impact_spreading, j.impact_seed)
+        cfg.append(((m["S"], vf.SEED_OUTER, j.extra_seeds, j.extra_seed, j.merge_distance, None), frags, near))
+    refs = [_production_reference(m, *c, frags=f, near=nr) for m, (c, f, nr) in zip(meta, cfg)]
+    with vf.Batch() as b:
+        res = b.run(jobs)
+        for i, (m, r) in enumerate(zip(meta, res)):
+            print("impact job", i, m["dims"], m["algo"], m["S"], flush=True)
+            assert r["status"] == 0, (i, r)
+            np.testing.assert_array_equal(m["labels"], refs[i], err_msg=f"job {i}")
+            np.testing.assert_array_equal(m["hist"], O.histogram(refs[i]), err_msg=f"job {i}")
+        again = b.run(jobs)  # replay: nearSeeds runs again from the same fragments
+    assert [x["checksum"] for x in again] == [r["checksum"] for r in res]
+    plain = vf.plain_run(jobs)
+    assert [p["checksum"] for p in plain] == [r["checksum"] for r in res]
+    srv = vf.Server(64 << 30, max_slots=3)
+    served = [srv.wait(t) for t in [srv.submit(j) for j in jobs]]
+    srv.close()
+    assert [x["status"] for x in served] == [0] * len(jobs)
+    assert [x["checksum"] for x in served] == [r["checksum"] for r in res]
+    # bad input is refused per job: a given seed outside the grid, spreading 0
+    bad = np.zeros((meta[0]["S"], 4), np.uint32)
+    bad[:, 0] = 4096
+    bad[:, 3] = 2
+    keep.append(bad)
+    jobs[0].seeds = bad.ctypes.data
+    jobs[2].impact_spreading = 0
+    res = vf.Batch().run(jobs[:3])
+    assert res[0]["status"] == -1 and res[2]["status"] == -1 and res[1]["status"] == 0
 
 
 def _mesh_equal(m, r):

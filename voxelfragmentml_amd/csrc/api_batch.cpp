@@ -138,15 +138,15 @@ static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
     }
     const uint64_t sig = job_signature(jobs, n);
     if (sig == b->sig && b->jobs.size() == n) {
-        // the same descriptors: device state is reused, but host meshes are read again (a caller may
-        // have refilled the buffers in place), after validating them again
+        // the same descriptors: device state is reused, but host meshes and seeds are read again (a
+        // caller may have refilled the buffers in place), after validating them again
         bool same = true;
         for (uint32_t i = 0; i < n && same; ++i) {
             std::string why;
-            if (!jobs[i].mesh_on_device) same = job_validate(jobs[i], why) == b->jobs[i].vstatus;
+            if (!jobs[i].mesh_on_device || jobs[i].seeds) same = job_validate(jobs[i], why) == b->jobs[i].vstatus;
         }
         if (same) {
-            for (auto* e : b->live) b->eng.upload_mesh(*e);
+            for (auto* e : b->live) b->eng.upload_inputs(*e);
             return;
         }
     }
@@ -626,6 +626,10 @@ int vf_server_submit(vf_server* s, const vf_job* job, uint64_t* ticket)
             e->tri_copy.assign(job->tri, job->tri + 3 * (size_t)job->nt);
             e->spec.xyz = e->xyz_copy.data();
             e->spec.tri = e->tri_copy.data();
+        }
+        if (st == VF_OK && job->seeds) {  // the caller's fragment seeds, kept with the job
+            e->seeds_copy.assign(job->seeds, job->seeds + job->num_seeds);
+            e->spec.seeds = e->seeds_copy.data();
         }
         e->spec.labels_out = job->labels_out;
         e->spec.histogram_out = job->histogram_out;

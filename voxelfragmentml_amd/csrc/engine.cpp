@@ -25,8 +25,20 @@ struct Layout {
     uint64_t V, words;
 };
 
-// seeds the fracture uses: the drawn fragments, then (mergeSeeds) their copies and the extra draw
-uint32_t seed_total(const vf_job& j) { return j.num_seeds + (j.extra_seeds ? j.num_seeds + j.extra_seeds : 0); }
+// fragment seeds at most: the drawn or given ones, then those nearSeeds adds
+uint32_t frag_max(const vf_job& j) { return j.num_seeds + (j.impacts ? j.impact_seeds : 0); }
+// seed slots: the fragments, then (mergeSeeds) their copies and the extra draw
+uint32_t seed_total(const vf_job& j) { return frag_max(j) + (j.extra_seeds ? frag_max(j) + j.extra_seeds : 0); }
+// the flood's sub-seed pass: merged sub-seeds, or given seeds that carry sub-seed ids (bits 8..15)
+bool needs_sub(const vf_job& j)
+{
+    if (j.algorithm != VF_FLOOD) return false;
+    if (j.extra_seeds) return true;
+    if (j.seeds)
+        for (uint32_t i = 0; i < j.num_seeds; ++i)
+            if (j.seeds[i].w >> 8) return true;
+    return false;
+}
 constexpr uint32_t kMaxMergeSeeds = 2048;  // fragments + extra seeds of one job (k_seed_merge)
 constexpr uint32_t kMaxErosionIterations = 64;
 
@@ -53,7 +65,7 @@ Layout layout_of(const vf_job& j)
     L.fctl = o; o = align_up(o + 64 * 4);
     L.keys = o; if (flood) o = align_up(o + L.V * 4);
     L.bricks = o; if (flood) o = align_up(o + 16ull * relax_words_per_parity(relax_bricks(j.dims[0], j.dims[1], j.dims[2])));
-    L.sub = o; if (flood && j.extra_seeds) o = align_up(o + 8ull * seed_total(j));  // union-find parents + keep flags
+    L.sub = o; if (needs_sub(j)) o = align_up(o + 8ull * seed_total(j));  // union-find parents + keep flags
     L.erode = o; if (j.erode) o = align_up(o + L.V * 2);                          // erosion output grid
     L.mask = o; if (j.erode) o = align_up(o + (size_t)kMaxErosionSide * kMaxErosionSide * kMaxErosionSide);
     L.mesh_xyz = o; if (!j.mesh_on_device) o = align_up(o + (size_t)j.nv * 12);
@@ -94,8 +106,31 @@ int job_validate(const vf_job& j, std::string& why)
         if (!(j.aabb_max[q] > j.aabb_min[q])) { why = "empty aabb"; return VF_ERR_INVALID_ARG; }
     if (j.nt && (!j.xyz || !j.tri)) { why = "mesh pointer is NULL"; return VF_ERR_INVALID_ARG; }
     if (j.num_seeds == 0 || j.num_seeds > kMaxDeviceSeeds) { why = "seed count out of range"; return VF_ERR_INVALID_ARG; }
+    if (j.seeds) {  // the caller's fragment seeds
+        for (uint32_t i = 0; i < j.num_seeds; ++i) {
+            const vf_seed& q = j.seeds[i];
+            if (q.x >= j.dims[0] || q.y >= j.dims[1] || q.z >= j.dims[2]) { why = "seed outside the grid"; return VF_ERR_INVALID_ARG; }
+            if (q.w > 0xFFFF) { why = "seed label above 16 bits"; return VF_ERR_INVALID_ARG; }
+            if (j.algorithm == VF_FLOOD && (q.w & 0xFFu) < 2) { why = "flood fragment id (low 8 bits) below 2"; return VF_ERR_TOO_MANY_SEEDS; }
+        }
+    }
+    if (j.impacts) {
+        const uint32_t e = std::min(j.dims[0], std::min(j.dims[1], j.dims[2]));
+        if (j.impact_spreading == 0 || j.impact_spreading > 64 || j.impact_spreading > e) {
+            why = "impact spreading outside 1..min(64, smallest edge)";
+            return VF_ERR_INVALID_ARG;
+        }
+        if (frag_max(j) > kMaxDeviceSeeds) { why = "too many impact seeds"; return VF_ERR_INVALID_ARG; }
+        const uint32_t last = j.seeds ? j.seeds[j.num_seeds - 1].w : j.num_seeds + 1;
+        if ((uint64_t)last + j.impact_seeds > 0xFFFF) { why = "impact seed labels above 16 bits"; return VF_ERR_INVALID_ARG; }
+    }
     if (j.extra_seeds) {
-        if (seed_total(j) > kMaxDeviceSeeds || j.num_seeds + j.extra_seeds > kMaxMergeSeeds) {
+        // mergeSeeds keeps the fragment label in 8 bits
+        const uint32_t last = j.seeds ? j.seeds[j.num_seeds - 1].w : j.num_seeds + 1;
+        uint32_t top = last + (j.impacts ? j.impact_seeds : 0);
+        for (uint32_t i = 0; j.seeds && i < j.num_seeds; ++i) top = std::max(top, j.seeds[i].w);
+        if (top > 255) { why = "fragment labels above 8 bits cannot take sub-seeds"; return VF_ERR_TOO_MANY_SEEDS; }
+        if (seed_total(j) > kMaxDeviceSeeds || frag_max(j) + j.extra_seeds > kMaxMergeSeeds) {
             why = "too many seeds to merge";
             return VF_ERR_INVALID_ARG;
         }
@@ -115,7 +150,11 @@ int job_validate(const vf_job& j, std::string& why)
     if (j.seed_location < VF_SEED_INNER || j.seed_location > VF_SEED_BOTH) { why = "unknown seed location"; return VF_ERR_INVALID_ARG; }
     if (j.boundary_size < 0 || j.boundary_size > 64) { why = "boundary size out of range"; return VF_ERR_INVALID_ARG; }
     if (j.algorithm == VF_FLOOD) {
-        if (j.num_seeds > 254) { why = "flood labels must fit 8 bits"; return VF_ERR_TOO_MANY_SEEDS; }
+        if (!j.seeds && frag_max(j) > 254) { why = "flood labels must fit 8 bits"; return VF_ERR_TOO_MANY_SEEDS; }
+        if (j.seeds && j.impacts && (j.seeds[j.num_seeds - 1].w & 0xFFu) + j.impact_seeds > 255) {
+            why = "flood impact seed labels must fit 8 bits";
+            return VF_ERR_TOO_MANY_SEEDS;
+        }
         const int nb = resolved_neigh(j);
         if (nb != 6 && nb != 26) { why = "neighbourhood must be 0, 6 or 26"; return VF_ERR_INVALID_ARG; }
         if (j.noise_amp > kMaxNoiseAmp) { why = "noise amplitude too large"; return VF_ERR_INVALID_ARG; }
@@ -174,13 +213,19 @@ void Engine::alloc(EJob& e)
     d.shell_bits = reinterpret_cast<uint32_t*>(base + Lo.bits);
     d.toggle_bits = d.shell_bits + Lo.words;
     d.seeds = reinterpret_cast<uint4*>(base + Lo.seeds);
-    d.S = seed_total(j);
-    d.S_draw = j.num_seeds;
+    d.S = j.num_seeds + (j.extra_seeds ? j.num_seeds + j.extra_seeds : 0);  // nearSeeds resets it on the device
+    d.S_draw = j.seeds ? 0 : j.num_seeds;  // the caller's seeds are copied, not drawn
     d.S0 = j.num_seeds;
+    d.F_in = j.num_seeds;
     d.extra = j.extra_seeds;
+    d.xoff = 2 * frag_max(j);
+    d.impacts = j.impacts;
+    d.near_seeds = j.impact_seeds;
+    d.spreading = j.impact_spreading;
+    d.near_key = j.impact_seed;
     d.merge_dist = j.merge_distance;
     d.dist = j.distance;
-    d.key_bits = key_bits_for(d.S);
+    d.key_bits = key_bits_for(seed_total(j));  // any count nearSeeds leaves fits
     d.neighbourhood = resolved_neigh(j);
     d.noise_seed = j.noise_seed;
     d.noise_amp = j.noise_amp;
@@ -196,9 +241,9 @@ void Engine::alloc(EJob& e)
         d.keys = reinterpret_cast<uint32_t*>(base + Lo.keys);
         d.bflags = reinterpret_cast<unsigned long long*>(base + Lo.bricks);
         d.nbricks = relax_bricks(d.nx, d.ny, d.nz);
-        d.sub = j.extra_seeds ? reinterpret_cast<uint32_t*>(base + Lo.sub) : nullptr;
+        d.sub = needs_sub(j) ? reinterpret_cast<uint32_t*>(base + Lo.sub) : nullptr;
     } else {
-        const NaiveKeyForm kf = naive_key_form(j.dims, j.distance, d.S);
+        const NaiveKeyForm kf = naive_key_form(j.dims, j.distance, seed_total(j));
         d.dist = kf.dist;
         d.naive_key64 = kf.key64;
     }
@@ -208,8 +253,8 @@ void Engine::alloc(EJob& e)
     } else {
         d.xyz = reinterpret_cast<const float*>(base + Lo.mesh_xyz);
         d.tri = reinterpret_cast<const uint32_t*>(base + Lo.mesh_tri);
-        upload_mesh(e);
     }
+    upload_inputs(e);
     d.nt = j.nt;
     d.nv = j.nv;
     e.erode_out = nullptr;
@@ -228,9 +273,10 @@ void Engine::alloc(EJob& e)
     e.groups = ceil_div(Lo.words, 256);
 }
 
-void Engine::upload_mesh(EJob& e)
+void Engine::upload_inputs(EJob& e)
 {
     const vf_job& j = e.spec;
+    if (j.seeds) VF_HIP(hipMemcpyAsync(e.d.seeds, j.seeds, (size_t)j.num_seeds * sizeof(vf_seed), hipMemcpyHostToDevice, s));
     if (j.mesh_on_device) return;
     if (j.nv) VF_HIP(hipMemcpyAsync(const_cast<float*>(e.d.xyz), j.xyz, (size_t)j.nv * 12, hipMemcpyHostToDevice, s));
     if (j.nt) VF_HIP(hipMemcpyAsync(const_cast<uint32_t*>(e.d.tri), j.tri, (size_t)j.nt * 12, hipMemcpyHostToDevice, s));
@@ -297,8 +343,9 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         xj[i].S_draw = e.d.extra;
         xj[i].seed_key = e.spec.extra_seed;
         xj[i].seed_location = VF_SEED_BOTH;
-        xj[i].seeds = e.d.seeds + 2 * (size_t)e.d.S0;
+        xj[i].seeds = e.d.seeds + e.d.xoff;
         t->any_extra |= e.d.extra != 0;
+        t->any_near |= e.d.impacts != 0;
         t->any_sub |= e.d.sub != nullptr;
         ej[i] = e.d;
         ej[i].bsize = 1;
@@ -344,6 +391,9 @@ Table* Engine::make_table(const std::vector<EJob*>& js)
         t->flood_lists = cache_.get(2ull * kFloodXcds * t->flood.list_cap * 8, &t->list_bytes);
         t->flood.lists = static_cast<unsigned long long*>(t->flood_lists);
     }
+    bool counts = false;  // jobs whose nearSeeds already ran: take its seed counts
+    for (const EJob* e : js) counts |= e->d.impacts && e->fronted;
+    if (counts) launch_seed_counts(t->L);
     return t;
 }
 
@@ -385,6 +435,7 @@ void Engine::make_room(size_t need, size_t budget)
 void Engine::front(Table& t)
 {
     set_active_bucket(t.bucket);
+    for (EJob* e : t.jobs) e->fronted = true;
     const Launch& L = t.L;
     launch_job_init(L);
     VoxScratch X;
@@ -400,6 +451,7 @@ void Engine::front(Table& t)
     launch_seed_count(L, seed_cnt_.as<uint32_t>(), true);
     scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
     launch_seed_pick(L, seed_off_.as<unsigned long long>());
+    if (t.any_near) launch_seed_near_jobs(L);  // nearSeeds after the fragment seeds
     if (t.any_extra) {  // mergeSeeds: the extra draw over all occupied cells, then the merge
         launch_seed_count(t.xL, seed_cnt_.as<uint32_t>(), false);
         scan_exclusive_u32_to_u64(seed_cnt_.as<uint32_t>(), seed_off_.as<unsigned long long>(), L.total_tiles, scan_tmp_.p, s);
@@ -484,6 +536,7 @@ void Engine::finish(EJob& e, const uint32_t* w)
     r.status = VF_OK;
     if (err & kDevErrMesh) r.status = VF_ERR_INVALID_ARG;
     else if (err & kDevErrSeeding) r.status = VF_ERR_SEEDING;
+    else if (err & kDevErrMerge) r.status = VF_ERR_TOO_MANY_SEEDS;
     else if (err) r.status = VF_ERR_OVERFLOW;
     r.mc_vertices = r.mc_triangles = 0;
     if (e.spec.labels_out)

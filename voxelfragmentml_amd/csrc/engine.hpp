@@ -23,6 +23,7 @@ struct EJob {
     vf_job spec{};
     std::vector<float> xyz_copy;     // server: the job owns a copy of its mesh
     std::vector<uint32_t> tri_copy;
+    std::vector<vf_seed> seeds_copy;  // server: and of the caller's fragment seeds
     JobDev d{};
     void* mem = nullptr;             // one device block per job
     size_t bytes = 0;
@@ -34,6 +35,7 @@ struct EJob {
     int phase = 0;
     int vstatus = 0;                 // job_validate's status when the job was prepared
     bool sub_done = false;           // the flood's sub-seed pass has run (served form)
+    bool fronted = false;            // the front stage is queued: nearSeeds' counts are in fctl
     uint16_t* erode_out = nullptr;   // erosion output grid and window (jobs with erode set)
     uint8_t* erode_mask = nullptr;
     uint32_t erode_side = 0;
@@ -66,7 +68,7 @@ struct Table {
     void* flood_lists = nullptr;    // separate allocation of flood.lists
     size_t dev_bytes = 0, list_bytes = 0;  // class sizes of the two blocks
     bool any_flood = false, any_naive32 = false, any_naive64 = false;
-    bool any_extra = false, any_sub = false, any_erode = false;
+    bool any_extra = false, any_sub = false, any_erode = false, any_near = false;
     Launch xL{};                          // the jobs as their extra-seed draw sees them
     const JobDev* ejobs = nullptr;        // the jobs as their erosion passes see them (boundary size 1)
     const uint32_t* local_prefix = nullptr;  // per job {0, tiles, 0, bricks}: one-job launches
@@ -83,7 +85,7 @@ public:
     int device = 0;
 
     void alloc(EJob& j);    // stream-ordered allocation + descriptor + mesh upload
-    void upload_mesh(EJob& j);  // host mesh -> the job's device copy (nothing for device meshes)
+    void upload_inputs(EJob& j);  // host mesh and given seeds -> the job's device copies
     void release(EJob& j);  // stream-ordered free
     Table* make_table(const std::vector<EJob*>& js);
     void drop_table(Table* t);

@@ -66,6 +66,10 @@ struct JobDev {
     uint32_t S;             // seeds the fracture uses
     uint32_t S_draw;        // seeds a uniform draw picks (k_seed_pick), written from seeds[0]
     uint32_t S0, extra;     // mergeSeeds: fragment seeds and extra seeds (extra = 0: no merge)
+    uint32_t F_in;          // fragment seeds before nearSeeds (drawn or given); nearSeeds sets S0 and S
+    uint32_t xoff;          // mergeSeeds: the extra draw sits at seeds[xoff .. xoff + extra)
+    uint32_t impacts, near_seeds, spreading;  // nearSeeds (impacts = 0: none)
+    uint64_t near_key;
     int merge_dist;         // mergeSeeds distance (VF_*)
     int dist;               // VF_* distance, or kDistEuclidF32 for naive Euclidean ranked by float roots
     uint32_t key_bits;      // bits for the seed index in packed keys
@@ -195,6 +199,7 @@ void launch_remove_isolated(hipStream_t s, uint16_t* labels, uint16_t* out, uint
 
 // flood.hip
 constexpr uint32_t kFloodCtlWords = 6;
+constexpr uint32_t kCtlSeedCounts = 16;  // fctl words 16..17: S and S0 as nearSeeds left them
 constexpr uint32_t kFloodMaxJobs = 1024;  // jobs per multi-job flood launch
 inline uint32_t relax_bricks(uint32_t nx, uint32_t ny, uint32_t nz)
 {
@@ -254,6 +259,12 @@ void launch_seed_merge(const Launch& L);
 // Seeder::nearSeeds on job 0 of L; out: F + num_seeds entries; result[2]: count, draw-cap flag
 void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t impacts, uint32_t num_seeds,
                       uint32_t spreading, uint64_t key, uint4* out, uint32_t* result);
+// nearSeeds for every job of L with impacts, in place on its fragment seeds (one block per job); sets
+// the job's S0 and S in the table.  A draw cap raises kDevErrSeeding.
+void launch_seed_near_jobs(const Launch& L);
+// copies S and S0 that nearSeeds left in each job's control words into the table L (jobs with impacts whose
+// front stage ran before: tables made after it)
+void launch_seed_counts(const Launch& L);
 
 // mc.hip: marching cubes of every label of one or many device grids (one
 // launch set for all of them).  Scratch and results live in W's buffers, taken

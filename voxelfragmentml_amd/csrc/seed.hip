@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_pick(Launch L, const unsi
 }
 
 // Seeder::mergeSeeds (Seeder.cpp:115-152) in CADScene::fractureModel's order (CADScene.cpp:647-655): the
-// merged list is the fragment seeds followed by the extra draw; each entry takes the label of its nearest
+// merged list is the fragment seeds followed by the extra draw (at seeds[xoff..]); each entry takes the label of its nearest
 // fragment seed (the reference's float distances on the coordinates, strict '<': the first fragment wins
 // ties) with that fragment's running sub-seed id (1, 2, ... in list order) in bits 8..15.  The fragments
 // stay first, the merged list follows them.  One block per job; entries staged in LDS.
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_merge(Launch L)
     const uint32_t F = J->S0, M = J->S0 + J->extra;
     uint4* seeds = J->seeds;
     for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
-        const uint4 q = i < F ? seeds[i] : seeds[2 * F + (i - F)];
+        const uint4 q = i < F ? seeds[i] : seeds[J->xoff + (i - F)];
         px[i] = q.x;
         py[i] = q.y;
         pz[i] = q.z;
@@ -288,11 +288,11 @@ __device__ inline int64_t uniform_int(int64_t mn, int64_t mx, uint32_t h)
 // candidate is accepted, in draw order, when it lies within min(n) / 2 of the
 // fragment seed, is occupied, on the surface (isBoundary) and new.  Batches of
 // kSeedThreads candidates are tested in parallel; one lane accepts them in
-// order.  result: [0] seeds written (fragments first), [1] 1 if an impact hit
-// the draw cap.
-__global__ __launch_bounds__(kSeedThreads) void k_seed_near(const JobDev* J, const uint4* frags, uint32_t F,
-                                                             uint32_t impacts, uint32_t num_seeds, uint32_t spreading,
-                                                             uint64_t key, uint4* out, uint32_t* result)
+// order.  out receives the fragments (unless out == frags) and then the new
+// seeds; returns the number of new seeds, err = 1 if an impact hit the draw cap.
+__device__ __forceinline__ uint32_t near_draw(const JobDev* J, const uint4* frags, uint32_t F, uint32_t impacts,
+                                              uint32_t num_seeds, uint32_t spreading, uint64_t key, uint4* out,
+                                              uint32_t& err)
 {
     __shared__ uint32_t chosen[kMaxDeviceSeeds];
     __shared__ uint32_t sorted[kMaxDeviceSeeds];
@@ -366,16 +366,58 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_near(const JobDev* J, con
         sorted[p] = v;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) out[i] = frags[i];
+    if (out != frags)
+        for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) out[i] = frags[i];
     const uint32_t label0 = frags[F - 1].w;
     for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
         const uint32_t v = sorted[i];
         out[F + i] = make_uint4(v / (n[1] * n[2]), (v / n[2]) % n[1], v % n[2], label0 + 1u + i);
     }
+    err = s_err;
+    return N;
+}
+
+// One grid's nearSeeds.  result: [0] seeds written (fragments first), [1] 1 if an impact hit the draw cap.
+__global__ __launch_bounds__(kSeedThreads) void k_seed_near(const JobDev* J, const uint4* frags, uint32_t F,
+                                                             uint32_t impacts, uint32_t num_seeds, uint32_t spreading,
+                                                             uint64_t key, uint4* out, uint32_t* result)
+{
+    uint32_t err = 0;
+    const uint32_t N = near_draw(J, frags, F, impacts, num_seeds, spreading, key, out, err);
     if (threadIdx.x == 0) {
         result[0] = F + N;
-        result[1] = s_err;
+        result[1] = err;
     }
+}
+
+// nearSeeds of every job with impacts (CADScene::fractureModel, CADScene.cpp:636-646), one block per job, in
+// place: the new seeds follow the F_in fragment seeds, and the job's fragment count S0 and seed count S
+// (fragments, then their merged copies and the extra draw when mergeSeeds follows) are set in the table.
+__global__ __launch_bounds__(kSeedThreads) void k_seed_near_jobs(Launch L)
+{
+    JobDev* J = const_cast<JobDev*>(L.jobs + blockIdx.x);
+    if (!J->active || J->impacts == 0) return;
+    const uint32_t F = J->F_in;
+    uint32_t err = 0;
+    const uint32_t N = near_draw(J, J->seeds, F, J->impacts, J->near_seeds, J->spreading, J->near_key, J->seeds, err);
+    if (threadIdx.x == 0) {
+        const uint32_t S0 = F + N;
+        J->S0 = S0;
+        J->S = S0 + (J->extra ? S0 + J->extra : 0u);
+        J->fctl[kCtlSeedCounts] = J->S;  // for tables made later
+        J->fctl[kCtlSeedCounts + 1] = S0;
+        if (err) atomicOr(reinterpret_cast<unsigned int*>(J->stats + kStatError), (unsigned int)kDevErrSeeding);
+    }
+}
+
+__global__ void k_seed_counts(Launch L)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L.njobs) return;
+    JobDev* J = const_cast<JobDev*>(L.jobs + j);
+    if (!J->impacts) return;
+    J->S = J->fctl[kCtlSeedCounts];
+    J->S0 = J->fctl[kCtlSeedCounts + 1];
 }
 
 }  // namespace
@@ -395,6 +437,20 @@ void launch_seed_near(const Launch& L, const uint4* frags, uint32_t F, uint32_t 
              kMaxDeviceSeeds);
     TimedLaunch tl("seed_near", L.s);
     k_seed_near<<<1, kSeedThreads, 0, L.s>>>(L.jobs, frags, F, impacts, num_seeds, spreading, key, out, result);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_seed_near_jobs(const Launch& L)
+{
+    TimedLaunch tl("seed_near", L.s);
+    k_seed_near_jobs<<<L.njobs, kSeedThreads, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+}
+
+void launch_seed_counts(const Launch& L)
+{
+    TimedLaunch tl("seed_counts", L.s);
+    k_seed_counts<<<ceil_div(L.njobs, 64u), 64, 0, L.s>>>(L);
     VF_CHECK_LAUNCH();
 }
 

@@ -64,6 +64,8 @@ class Job(C.Structure):
         ("erode", C.c_int), ("erosion_convolution", C.c_int), ("erosion_size", C.c_uint32),
         ("erosion_iterations", C.c_uint32), ("erosion_probability", C.c_float), ("erosion_threshold", C.c_float),
         ("erosion_seed", C.c_uint64),
+        ("seeds", C.c_void_p), ("impacts", C.c_uint32), ("impact_seeds", C.c_uint32),
+        ("impact_spreading", C.c_uint32), ("impact_seed", C.c_uint64),
         ("mesh_on_device", C.c_int), ("poison", C.c_int),
         ("labels_out", C.c_void_p), ("histogram_out", C.c_void_p),
     ]
