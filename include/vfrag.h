@@ -414,10 +414,22 @@ int vf_server_memory(vf_server* s, size_t* reserved_bytes, size_t* device_bytes,
                      size_t* max_unreserved_bytes);
 
 /* ---- autotuner ---------------------------------------------------------------- */
-/* Times each tunable stage over candidate block/tile shapes per grid-edge
- * bucket on this device and writes the winners to path (text); vf_tune_load
- * installs a table for subsequent launches. */
+/* Launch shapes per grid-edge bucket (largest edge <= 64, <= 128, <= 256,
+ * larger): workgroup counts of the cell, voxelizer-pair, boundary-brick and
+ * flood stages, walker run lengths (naive tiles, boundary bricks per step)
+ * and the batched flood's round chunking.  A batch uses the bucket of its
+ * largest edge.  No reference counterpart: the reference sizes its GL
+ * dispatches from fixed work-group sizes (ComputeShader.cpp:152-155).
+ *
+ * vf_tune_run times the batched form on synthetic vessel jobs of each bucket
+ * (coordinate descent; quick = 1: fewer jobs and repeats, buckets above 64),
+ * installs the winners and writes the table to path (text).  vf_tune_jobs
+ * tunes the bucket of the caller's own batch on that batch, reporting its
+ * median time before and after (either may be NULL) and, when path is not
+ * NULL, writing the table.  vf_tune_load installs a table for later launches.
+ * Shapes change timing only, never results. */
 int vf_tune_run(const char* path, int quick);
+int vf_tune_jobs(const vf_job* jobs, uint32_t n, const char* path, int quick, double* before_ms, double* after_ms);
 int vf_tune_load(const char* path);
 
 #ifdef __cplusplus

@@ -29,7 +29,6 @@ using namespace vf;
 
 namespace {
 
-constexpr uint32_t kRoundChunk = 8;   // flood rounds per captured chunk
 constexpr uint32_t kServerRounds = 4; // flood rounds per scheduler tick
 constexpr size_t kPlainCacheKeep = 16ull << 30;  // device bytes the plain form keeps cached between calls
 
@@ -208,20 +207,21 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
             if (T.any_flood) {
                 // the first chunks run without a check; then one readback per chunk.  Jobs with merged
                 // seeds run the sub-seed pass after the first drain and grow again where it freed cells.
+                const LaunchShape sh = get_shape(T.bucket);
                 uint32_t chunks = 0;
                 bool sub_done = !T.any_sub;
                 for (;;) {
-                    run_captured(E.s, b->g_rounds, b->use_graphs, [&] { E.rounds(T, kRoundChunk); });
-                    if (++chunks < 3) continue;
+                    run_captured(E.s, b->g_rounds, b->use_graphs, [&] { E.rounds(T, sh.round_chunk); });
+                    if (++chunks < sh.lead_chunks) continue;
                     E.gather(T, E.pinned);
                     VF_HIP(hipStreamSynchronize(E.s));
                     if (all_drained(T, E.pinned)) {
                         if (sub_done) break;
                         E.flood_sub(T);
                         sub_done = true;
-                        chunks = 2;  // check after the next chunk
+                        chunks = sh.lead_chunks - 1;  // check after the next chunk
                     }
-                    if ((uint64_t)chunks * kRoundChunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
+                    if ((uint64_t)chunks * sh.round_chunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                 }
             }
             run_captured(E.s, b->g_post, b->use_graphs, [&] { E.post(T); });

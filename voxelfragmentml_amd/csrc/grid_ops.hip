@@ -2,6 +2,8 @@
 // boundary detection (optionally fused with the per-fragment histogram),
 // histogram, occupied count and the order-independent grid checksum.
 // All are HBM-bound streaming passes; each thread owns 8 cells.
+#include <type_traits>
+
 #include "cellops.hpp"
 
 namespace vf {
@@ -224,7 +226,7 @@ __device__ inline uint32_t pk_max_u16(uint32_t a, uint32_t b)
 // a label as the packed pair (label | 0xFFFF - label)
 __device__ inline uint32_t packed_self(uint32_t m) { return m | ((0xFFFFu - m) << 16); }
 
-template <bool HIST, bool CSUM>
+template <bool HIST, bool CSUM, uint32_t RUN>
 __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32_t* bt_prefix, uint32_t total)
 {
     // z-window max (low 16 bits) | 0xFFFF - min (high 16) over labelled cells (EMPTY / FREE contribute 0),
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(256) void k_boundaries_tiled(Launch L, const uint32
 #pragma unroll
         for (int k = 0; k < 8; ++k) out[k] = pk_max_u16(pk_max_u16(p[k], p[k + 1]), p[k + 2]);
     };
-    constexpr uint32_t kRun = 4;  // bricks per step: their interior loads are issued together
+    constexpr uint32_t kRun = RUN;  // bricks per step: their interior loads are issued together
     walk_tile_runs<kRun>(L, bt_prefix, total, &sJ, [&](const JobDev& J, uint32_t, uint32_t first, uint32_t nrun) {
         if (!J.active || J.bsize > 1) return;
         const int nx = (int)J.nx, ny = (int)J.ny, nz = (int)J.nz;
@@ -604,9 +606,16 @@ void launch_boundaries_tiled(const Launch& L, const uint32_t* bt_prefix, uint32_
 {
     {
         TimedLaunch tl("boundaries_hist", L.s);
-        const uint32_t blocks = std::max(1u, std::min(total_btiles, active_shape().brick_blocks));
-        if (with_checksum) k_boundaries_tiled<true, true><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
-        else k_boundaries_tiled<true, false><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+        const LaunchShape sh = active_shape();
+        const uint32_t blocks = std::max(1u, std::min(total_btiles, sh.brick_blocks));
+        auto go = [&](auto run) {
+            constexpr uint32_t R = decltype(run)::value;
+            if (with_checksum) k_boundaries_tiled<true, true, R><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+            else k_boundaries_tiled<true, false, R><<<blocks, 256, 0, L.s>>>(L, bt_prefix, total_btiles);
+        };
+        if (sh.brick_run == 2) go(std::integral_constant<uint32_t, 2>{});
+        else if (sh.brick_run == 8) go(std::integral_constant<uint32_t, 8>{});
+        else go(std::integral_constant<uint32_t, 4>{});
         VF_CHECK_LAUNCH();
     }
     if (any_wide) {

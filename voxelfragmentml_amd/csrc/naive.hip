@@ -116,8 +116,9 @@ __device__ inline void naive_cells(K best[4], const uint32_t cx[4], const uint32
 // thread.  A tile costs several block barriers and dependent LDS steps, so
 // the register budget is set for more waves in flight: 5 per SIMD with
 // 32-bit keys (95 VGPRs instead of 127, 1.06 -> 0.94 ms on the mixed batch),
-// 4 with 64-bit keys, both without spills.
-template <typename K>
+// 4 with 64-bit keys, both without spills.  RUN tiles form a walker run whose
+// occupancy loads are issued together (a launch shape the tuner picks).
+template <typename K, uint32_t RUN>
 __global__ __launch_bounds__(kCellThreads) __attribute__((amdgpu_waves_per_eu(sizeof(K) == 4 ? 5 : 4))) void k_naive(Launch L)
 {
     __shared__ __attribute__((aligned(16))) uint32_t sx[kSeedChunk], sy[kSeedChunk], sz[kSeedChunk];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kCellThreads) __attribute__((amdgpu_waves_per_eu(si
         }
         __syncthreads();
     };
-    constexpr uint32_t kRun = 8;  // tiles per run: their occupancy loads are issued together
+    constexpr uint32_t kRun = RUN;
     walk_tile_runs_strided<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, sp, [&](const JobDev& J, uint32_t j, uint32_t first,
                                                                               uint32_t nrun) {
         if (!J.active || J.algo != VF_NAIVE || J.naive_key64 != kWant || J.S == 0) return;
@@ -313,8 +314,16 @@ void launch_naive(const Launch& L, bool key64)
     if (L.njobs > kWalkMaxJobs) fail(VF_ERR_INVALID_ARG, "naive launch with %u jobs (max %u)", L.njobs, kWalkMaxJobs);
     const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
     TimedLaunch tl(key64 ? "naive64" : "naive", L.s);
-    if (key64) k_naive<unsigned long long><<<blocks, kCellThreads, 0, L.s>>>(L);
-    else k_naive<uint32_t><<<blocks, kCellThreads, 0, L.s>>>(L);
+    const uint32_t run = active_shape().naive_run;
+    if (key64) {
+        if (run == 4) k_naive<unsigned long long, 4><<<blocks, kCellThreads, 0, L.s>>>(L);
+        else if (run == 16) k_naive<unsigned long long, 16><<<blocks, kCellThreads, 0, L.s>>>(L);
+        else k_naive<unsigned long long, 8><<<blocks, kCellThreads, 0, L.s>>>(L);
+    } else {
+        if (run == 4) k_naive<uint32_t, 4><<<blocks, kCellThreads, 0, L.s>>>(L);
+        else if (run == 16) k_naive<uint32_t, 16><<<blocks, kCellThreads, 0, L.s>>>(L);
+        else k_naive<uint32_t, 8><<<blocks, kCellThreads, 0, L.s>>>(L);
+    }
     VF_CHECK_LAUNCH();
 }
 

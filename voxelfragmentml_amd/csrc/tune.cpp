@@ -1,6 +1,8 @@
-// Autotuner: picks launch shapes per grid-edge bucket by timing the batched
-// form on synthetic vessel jobs of that bucket (coordinate descent over the
-// parameters, median of repeated runs), and stores / loads the table as text.
+// Autotuner: picks launch shapes per grid-edge bucket (workgroup counts, walker
+// run lengths, flood round chunking) by timing the batched form on synthetic
+// vessel jobs of that bucket, or on the caller's own batch (coordinate descent
+// over the parameters, median of repeated runs), and stores / loads the table
+// as text.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -23,24 +25,28 @@ std::mutex g_mu;
 // bricks: 16384.  Flood relaxation: 4608 one-wave workgroups (18 per CU); each takes a fixed strided
 // share of its XCD's brick list, so workgroups beyond those resident at once run their whole share
 // late (4096 and 4608 relax in 4.55 ms, 5120 and 6144 in 5.1-5.4 ms).
+// Walker runs: 8 naive tiles, 4 boundary bricks.  Flood: chunks of 8 rounds, the first drain check after 3.
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 8192, 4608, 8192, 16384 },
-    { 8192, 4608, 8192, 16384 },
-    { 8192, 4608, 8192, 16384 },
-    { 8192, 4608, 8192, 16384 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
 
 constexpr uint32_t kEdgeOfBucket[kTuneBuckets] = { 64, 128, 256, 384 };
 
+}  // namespace
+
 bool shape_ok(const LaunchShape& s)
 {
     auto ok = [](uint32_t v) { return v >= 64 && v <= (1u << 20); };
-    return ok(s.cell_blocks) && ok(s.flood_blocks) && ok(s.pair_blocks) && ok(s.brick_blocks);
+    return ok(s.cell_blocks) && ok(s.flood_blocks) && ok(s.pair_blocks) && ok(s.brick_blocks) &&
+           (s.naive_run == 4 || s.naive_run == 8 || s.naive_run == 16) &&
+           (s.brick_run == 2 || s.brick_run == 4 || s.brick_run == 8) && s.round_chunk >= 2 && s.round_chunk <= 32 &&
+           s.lead_chunks >= 1 && s.lead_chunks <= 8;
 }
-
-}  // namespace
 
 int tune_bucket(uint32_t max_edge)
 {
@@ -182,14 +188,65 @@ void write_table(const char* path)
 {
     FILE* f = std::fopen(path, "w");
     if (!f) fail(VF_ERR_INVALID_ARG, "cannot write %s", path);
-    std::fprintf(f, "# vfrag launch shapes: bucket max_edge cell_blocks flood_blocks pair_blocks brick_blocks\n");
+    std::fprintf(f, "# vfrag launch shapes: bucket max_edge cell_blocks flood_blocks pair_blocks brick_blocks "
+                    "naive_run brick_run round_chunk lead_chunks\n");
     for (int b = 0; b < kTuneBuckets; ++b) {
         const LaunchShape s = get_shape(b);
-        std::fprintf(f, "%d %u %u %u %u %u\n", b, kEdgeOfBucket[b], s.cell_blocks, s.flood_blocks, s.pair_blocks,
-                     s.brick_blocks);
+        std::fprintf(f, "%d %u %u %u %u %u %u %u %u %u\n", b, kEdgeOfBucket[b], s.cell_blocks, s.flood_blocks,
+                     s.pair_blocks, s.brick_blocks, s.naive_run, s.brick_run, s.round_chunk, s.lead_chunks);
     }
     std::fclose(f);
 }
+
+// Coordinate descent over the bucket's shape on one batch: each parameter in turn takes each of its
+// candidates, a change is kept only when the median batch time drops by more than 2 %; the walk over
+// all parameters runs twice (once in quick mode).  Returns the starting and the final median.
+// flood_blocks is swept only on a caller's batch: its best value follows the length of the brick worklist,
+// which grows with the number of jobs in a batch, not with the bucket's edge, so a bucket's synthetic
+// jobs timed alone do not predict it.
+void tune_bucket_on(vf_batch* b, const std::vector<vf_job>& jobs, int bucket, bool quick, bool flood_blocks,
+                    double* before, double* after)
+{
+    const int reps = quick ? 5 : 7;
+    LaunchShape best = get_shape(bucket);
+    double best_ms = time_batch(b, jobs, reps);
+    if (before) *before = best_ms;
+    auto sweep = [&](uint32_t LaunchShape::*field, const std::vector<uint32_t>& cands) {
+        for (uint32_t v : cands) {
+            if (best.*field == v) continue;
+            LaunchShape s = best;
+            s.*field = v;
+            set_shape(bucket, s);
+            const double ms = time_batch(b, jobs, reps);
+            if (ms < best_ms * 0.98) {
+                best_ms = ms;
+                best = s;
+            }
+        }
+        set_shape(bucket, best);
+    };
+    for (int pass = 0; pass < (quick ? 1 : 2); ++pass) {
+        sweep(&LaunchShape::cell_blocks, { 4096, 8192, 16384 });
+        sweep(&LaunchShape::naive_run, { 4, 8, 16 });
+        sweep(&LaunchShape::pair_blocks, { 8192, 16384, 32768 });
+        sweep(&LaunchShape::brick_blocks, { 4096, 8192, 16384 });
+        sweep(&LaunchShape::brick_run, { 2, 4, 8 });
+        if (flood_blocks) sweep(&LaunchShape::flood_blocks, { 3072, 4096, 4608, 6144 });
+        sweep(&LaunchShape::round_chunk, { 4, 8, 16 });
+        sweep(&LaunchShape::lead_chunks, { 1, 2, 3, 4, 6 });
+    }
+    // the final shape timed again, so the figure reported is not the luckiest of the sweep's runs
+    if (after) *after = time_batch(b, jobs, reps);
+}
+
+struct BatchHandle {
+    vf_batch* b = nullptr;
+    BatchHandle()
+    {
+        if (vf_batch_create(&b) != VF_OK) fail(VF_ERR_DEVICE, "cannot create a tuning batch");
+    }
+    ~BatchHandle() { vf_batch_destroy(b); }
+};
 
 }  // namespace
 
@@ -200,42 +257,27 @@ int vf_tune_run(const char* path, int quick)
     return guard([&] {
         if (!path) fail(VF_ERR_INVALID_ARG, "path is NULL");
         if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
-        const std::vector<uint32_t> cand_cell = { 4096, 8192, 16384 };
-        const std::vector<uint32_t> cand_pair = { 8192, 16384, 32768 };
-        const std::vector<uint32_t> cand_brick = { 4096, 8192, 16384 };
-        const int reps = quick ? 5 : 7;
         for (int bucket = quick ? 1 : 0; bucket < kTuneBuckets; ++bucket) {
             TuneJobs T = make_jobs(bucket, quick != 0);
-            vf_batch* b = nullptr;
-            if (vf_batch_create(&b) != VF_OK) fail(VF_ERR_DEVICE, "cannot create a tuning batch");
-            struct Destroy {
-                vf_batch* b;
-                ~Destroy() { vf_batch_destroy(b); }
-            } destroy{ b };
-            LaunchShape best = get_shape(bucket);
-            double best_ms = time_batch(b, T.jobs, reps);
-            auto sweep = [&](uint32_t LaunchShape::*field, const std::vector<uint32_t>& cands) {
-                for (uint32_t v : cands) {
-                    if (best.*field == v) continue;
-                    LaunchShape s = best;
-                    s.*field = v;
-                    set_shape(bucket, s);
-                    const double ms = time_batch(b, T.jobs, reps);
-                    if (ms < best_ms * 0.98) {  // keep a change only when it is clearly faster
-                        best_ms = ms;
-                        best = s;
-                    }
-                }
-                set_shape(bucket, best);
-            };
-            // flood_blocks keeps its default: its best value follows the length of the brick
-            // worklist, which grows with the number of jobs in a batch, not with the bucket's edge,
-            // so a bucket timed alone does not predict a mixed batch
-            sweep(&LaunchShape::cell_blocks, cand_cell);
-            sweep(&LaunchShape::pair_blocks, cand_pair);
-            sweep(&LaunchShape::brick_blocks, cand_brick);
+            BatchHandle h;
+            tune_bucket_on(h.b, T.jobs, bucket, quick != 0, false, nullptr, nullptr);
         }
         write_table(path);
+    });
+}
+
+int vf_tune_jobs(const vf_job* jobs, uint32_t n, const char* path, int quick, double* before_ms, double* after_ms)
+{
+    return guard([&] {
+        if (!jobs || n == 0) fail(VF_ERR_INVALID_ARG, "no jobs to tune on");
+        if (vf_device_count() < 1) fail(VF_ERR_DEVICE, "no HIP device available");
+        uint32_t edge = 1;
+        for (uint32_t i = 0; i < n; ++i)
+            edge = std::max(edge, std::max(jobs[i].dims[0], std::max(jobs[i].dims[1], jobs[i].dims[2])));
+        const std::vector<vf_job> js(jobs, jobs + n);
+        BatchHandle h;
+        tune_bucket_on(h.b, js, tune_bucket(edge), quick != 0, true, before_ms, after_ms);
+        if (path) write_table(path);
     });
 }
 
@@ -251,9 +293,11 @@ int vf_tune_load(const char* path)
             if (line[0] == '#') continue;
             int b = -1;
             unsigned edge = 0;
-            LaunchShape s{};
-            if (std::sscanf(line, "%d %u %u %u %u %u", &b, &edge, &s.cell_blocks, &s.flood_blocks, &s.pair_blocks,
-                            &s.brick_blocks) != 6) {
+            LaunchShape s = get_shape(kTuneBuckets - 1);
+            const int got = std::sscanf(line, "%d %u %u %u %u %u %u %u %u %u", &b, &edge, &s.cell_blocks, &s.flood_blocks,
+                                        &s.pair_blocks, &s.brick_blocks, &s.naive_run, &s.brick_run, &s.round_chunk,
+                                        &s.lead_chunks);
+            if (got != 10 || b < 0 || b >= kTuneBuckets || !shape_ok(s)) {
                 std::fclose(f);
                 fail(VF_ERR_INVALID_ARG, "malformed tuning line: %s", line);
             }

@@ -125,6 +125,7 @@ def lib():
         "vf_timing_enable": (I, [I]), "vf_timing_reset": (I, []),
         "vf_timing_read": (I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),
         "vf_tune_run": (I, [C.c_char_p, I]), "vf_tune_load": (I, [C.c_char_p]),
+        "vf_tune_jobs": (I, [P, U32, C.c_char_p, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -377,15 +378,35 @@ def tune_load(path):
     check(lib().vf_tune_load(str(path).encode()))
 
 
+def tune_jobs(jobs, path=None, quick: bool = True):
+    """Tune the launch shapes of the bucket this batch uses on the batch itself; returns the median batch
+    milliseconds (before, after) and writes the table to path when given."""
+    arr = (Job * len(jobs))(*jobs)
+    before, after = C.c_double(), C.c_double()
+    check(lib().vf_tune_jobs(C.cast(arr, C.c_void_p), len(jobs), str(path).encode() if path else None,
+                             1 if quick else 0, C.byref(before), C.byref(after)))
+    return before.value, after.value
+
+
+TUNING_FIELDS = ("max_edge", "cell_blocks", "flood_blocks", "pair_blocks", "brick_blocks", "naive_run", "brick_run",
+                 "round_chunk", "lead_chunks")
+
+
 def read_tuning(path):
-    """The table of a tuning file: {bucket: {max_edge, cell_blocks, flood_blocks, pair_blocks, brick_blocks}}."""
+    """The table of a tuning file: {bucket: {max_edge, cell_blocks, ..., lead_chunks}}."""
     out = {}
     for line in Path(path).read_text().splitlines():
         if line.startswith("#") or not line.strip():
             continue
-        b, e, c, f, p, k = (int(x) for x in line.split())
-        out[b] = {"max_edge": e, "cell_blocks": c, "flood_blocks": f, "pair_blocks": p, "brick_blocks": k}
+        v = [int(x) for x in line.split()]
+        out[v[0]] = dict(zip(TUNING_FIELDS, v[1:]))
     return out
+
+
+def write_tuning(path, table):
+    """Write a table in read_tuning's form (for vf_tune_load)."""
+    lines = [" ".join(str(x) for x in [b] + [table[b][k] for k in TUNING_FIELDS]) for b in sorted(table)]
+    Path(path).write_text("\n".join(lines) + "\n")
 
 
 # ------------------------------------------------------------ batched / served
