@@ -19,9 +19,11 @@
 //   k_mc_labels / k_mc_base  each grid's target labels (nonzero histogram
 //                 counts above FREE) -> dense index, and one global segment
 //                 per (grid, label)
-//   k_mc_count   one thread per unit (8 consecutive points of a z column):
-//                 the (segment, point) pairs with output, and their triangle
-//                 and vertex counts; uniform units cost five column loads
+//   k_mc_count   one thread per unit (8 consecutive points of a z column)
+//                 loads its columns; units that can have output are queued
+//                 in LDS and worked one lane per point: the (segment, point)
+//                 pairs with output, and their triangle and vertex counts;
+//                 uniform units cost five column loads
 //   scan         pair offsets per tile of units
 //   (one host synchronisation: totals size the outputs)
 //   k_mc_pairs   the pairs, in (grid, point) order, as (segment, point | counts)
@@ -52,6 +54,7 @@ constexpr uint32_t kMcMaxLabels = 4096;            // labels per grid
 constexpr uint16_t kMcNoLabel = 0xFFFFu;           // no dense index; also "beyond the lattice" (masked values <= 0x7FFF)
 constexpr uint32_t kLindexWords = 65536 + kMcMaxLabels;  // per grid: label -> dense index, then its labels
 constexpr uint32_t kMcMaxJobs = 1024;              // grids per run
+constexpr uint32_t kMcLdsLabels = 1024;            // label-index entries a block keeps in LDS
 
 __constant__ int c_mc_nb[8][3] = { { 0, 0, 0 }, { 0, 0, 1 }, { -1, 0, 1 }, { -1, 0, 0 },
                                    { 0, 1, 0 }, { 0, 1, 1 }, { -1, 1, 1 }, { -1, 1, 0 } };
@@ -68,6 +71,7 @@ struct McJob {
     uint32_t cz;               // units per z column
     uint32_t want;             // VF_ALL_LABELS or one label
     uint32_t min_target;       // smallest target label (65536 when none): units below it emit nothing
+    uint32_t vec;              // grid rows 16-byte aligned (2), 8-byte aligned (1) or neither (0)
     float mn[3], cs[3];
 };
 
@@ -114,30 +118,15 @@ __device__ inline uint16_t lat_raw(const McJob& J, int x, int y, int z)
 }
 __device__ inline uint16_t mask15(uint16_t raw) { return raw == kMcNoLabel ? kMcNoLabel : (uint16_t)(raw & kMask15); }
 
-// Column (x, y) from z0 on: n raw values.  Interior columns are one contiguous row of the grid.
-template <int N>
-__device__ inline void load_column(const McJob& J, int x, int y, int z0, uint16_t v[N])
-{
-    const bool inside = x >= 1 && y >= 1 && x <= (int)J.Nx - 2 && y <= (int)J.Ny - 2;
-    if (inside && z0 >= 1 && z0 + N <= (int)J.Nz - 1) {
-        const VF_AS_GLOBAL uint16_t* row =
-            gcast<const uint16_t>(J.grid) + ((size_t)(x - 1) * J.ny + (size_t)(y - 1)) * J.nz + (size_t)(z0 - 1);
-#pragma unroll
-        for (int k = 0; k < N; ++k) v[k] = row[k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < N; ++k) v[k] = lat_raw(J, x, y, z0 + k);
-    }
-}
-
 // The labels with output at one point and their counts.  c: the eight masked cube corners (corner 0 is
 // the point itself, corner 1 its +z neighbour, corner 4 its +y neighbour; kMcNoLabel beyond the lattice),
 // nbx: its masked +x neighbour; cube: the point is a valid corner 0; ntri: the table's triangle count
 // per configuration.  f(dense index, label, triangles, vertices) for every label with output; the
-// vertices are the point's own +axis edges with the label on exactly one end.  Fully unrolled: the
-// candidates stay in registers.
+// vertices are the point's own +axis edges with the label on exactly one end; slx holds the first
+// kMcLdsLabels entries of J's label index when staged.  Fully unrolled: the candidates stay in registers.
 template <class F>
-__device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const uint16_t c[8], uint16_t nbx, bool cube, F&& f)
+__device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const uint16_t* slx, bool staged, const uint16_t c[8],
+                                    uint16_t nbx, bool cube, F&& f)
 {
     const uint16_t own = c[0];
     const uint16_t nb[3] = { nbx, c[4], c[1] };
@@ -158,7 +147,7 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
 #pragma unroll
         for (int k = 0; k < i; ++k) seen |= ((valid >> k) & 1u) && cand[k] == L;
         if (seen) continue;
-        const uint16_t li = gcast<const uint16_t>(J.lindex)[L];
+        const uint16_t li = staged && L < kMcLdsLabels ? slx[L] : gcast<const uint16_t>(J.lindex)[L];
         if (li == kMcNoLabel) continue;
         uint32_t nt = 0;
         if (cube) {
@@ -174,51 +163,136 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
     }
 }
 
-// Walks the points of unit `local` of job J: f(point index, masked corners, masked +x neighbour, valid
-// cube, raw corner 0) for each of its (up to) 8 points, unless the unit is uniform or holds no value above
-// FREE, when no point can have output.
-template <class F>
-__device__ inline void walk_unit(const McJob& J, unsigned long long local, F&& f)
+// Raw values of a unit's five columns: A = (px, py), B = (px - 1, py), C = (px, py + 1), D = (px - 1, py + 1),
+// each at lattice z0 .. z0 + 8, and E = (px + 1, py) at z0 .. z0 + 7.
+constexpr uint32_t kMcCols = 4 * (kMcZ + 1) + kMcZ;
+
+__device__ inline void unit_origin(const McJob& J, unsigned long long local, int& px, int& py, int& z0)
 {
     const uint32_t col = (uint32_t)(local / J.cz);
-    const int z0 = (int)((uint32_t)(local - (unsigned long long)col * J.cz) * kMcZ);
-    const int px = (int)(col / J.Ny), py = (int)(col - (uint32_t)px * J.Ny);
-    uint16_t A[kMcZ + 1], B[kMcZ + 1], C[kMcZ + 1], D[kMcZ + 1], E[kMcZ];
-    load_column<kMcZ + 1>(J, px, py, z0, A);
-    load_column<kMcZ + 1>(J, px - 1, py, z0, B);
-    load_column<kMcZ + 1>(J, px, py + 1, z0, C);
-    load_column<kMcZ + 1>(J, px - 1, py + 1, z0, D);
-    load_column<kMcZ>(J, px + 1, py, z0, E);
-    // uniform units (outside the vessel, inside one fragment) and units without a label emit nothing
-    uint32_t lo = 0xFFFFu, hi = 0;
-    auto acc = [&](uint16_t raw) {
-        const uint16_t m = mask15(raw);
-        if (m != kMcNoLabel) {
-            lo = min(lo, (uint32_t)m);
-            hi = max(hi, (uint32_t)m);
-        }
-    };
+    z0 = (int)((uint32_t)(local - (unsigned long long)col * J.cz) * kMcZ);
+    px = (int)(col / J.Ny);
+    py = (int)(col - (uint32_t)px * J.Ny);
+}
+
+// Zeros that lanes off the fast path of load_unit read in place of a grid row (16-byte aligned at +8).
+__device__ __align__(16) const uint16_t c_mc_zero[32] = {};
+
+// Loads unit `local`'s columns into v; false when no point of the unit can have output: the unit is uniform
+// (outside the vessel, inside one fragment) or holds no target label.  Every lane issues all of its loads
+// before waiting on any (interior columns are contiguous grid rows, lattice z0 + k being grid z0 - 1 + k;
+// z0 is a multiple of 8, so with 16-byte aligned rows lattice z0 + 1 .. z0 + 8 are one 16-byte load, with
+// 8-byte aligned rows two); lanes whose columns touch the padding or the lattice end read zeros and then
+// take those columns value by value.
+__device__ inline bool load_unit(const McJob& J, unsigned long long local, uint16_t v[kMcCols])
+{
+    int px, py, z0;
+    unit_origin(J, local, px, py, z0);
+    constexpr int K = kMcZ + 1;
+    const int cx[5] = { px, px - 1, px, px - 1, px + 1 }, cy[5] = { py, py, py + 1, py + 1, py };
+    const bool zin = z0 >= 1 && z0 + K <= (int)J.Nz - 1;
+    bool fast[5];
+    const uint16_t* row[5];
 #pragma unroll
-    for (int k = 0; k <= (int)kMcZ; ++k) {
-        acc(A[k]);
-        acc(B[k]);
-        acc(C[k]);
-        acc(D[k]);
-        if (k < (int)kMcZ) acc(E[k]);
+    for (int c = 0; c < 5; ++c) {
+        fast[c] = zin && cx[c] >= 1 && cy[c] >= 1 && cx[c] <= (int)J.Nx - 2 && cy[c] <= (int)J.Ny - 2;
+        row[c] = fast[c] ? J.grid + (((size_t)(cx[c] - 1) * J.ny + (size_t)(cy[c] - 1)) * J.nz + (size_t)(z0 - 1))
+                         : c_mc_zero + 7;
     }
-    if (hi < J.min_target || lo == hi) return;
-    const uint32_t yz = J.Ny * J.Nz;
-#pragma unroll 1
-    for (int k = 0; k < (int)kMcZ; ++k) {
-        const int pz = z0 + k;
-        if (pz >= (int)J.Nz) break;
-        const uint16_t c[8] = { mask15(A[k]), mask15(A[k + 1]), mask15(B[k + 1]), mask15(B[k]),
-                                mask15(C[k]), mask15(C[k + 1]), mask15(D[k + 1]), mask15(D[k]) };
-        const uint16_t nbx = mask15(E[k]);
-        const bool cube = px >= 1 && py <= (int)J.Ny - 2 && pz <= (int)J.Nz - 2;
-        const uint32_t p = (uint32_t)px * yz + (uint32_t)py * J.Nz + (uint32_t)pz;
-        f(p, c, nbx, cube, A[k]);
+    auto put = [&](int c, const uint32_t w[4]) {  // lattice z0 + 1 .. from four packed words
+        const int n = c < 4 ? K : (int)kMcZ;
+#pragma unroll
+        for (int k = 1; k < n; ++k) v[c * K + k] = (uint16_t)(w[(k - 1) >> 1] >> (16 * ((k - 1) & 1)));
+    };
+    if (J.vec == 2) {
+        uint4 q[5];
+        uint16_t h[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            h[c] = gcast<const uint16_t>(row[c])[0];
+            q[c] = gload4(row[c] + 1);
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            v[c * K] = h[c];
+            const uint32_t w[4] = { q[c].x, q[c].y, q[c].z, q[c].w };
+            put(c, w);
+        }
+    } else if (J.vec == 1) {
+        uint2 qa[5], qb[5];
+        uint16_t h[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            h[c] = gcast<const uint16_t>(row[c])[0];
+            qa[c] = gload2(row[c] + 1);
+            qb[c] = gload2(row[c] + 5);
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            v[c * K] = h[c];
+            const uint32_t w[4] = { qa[c].x, qa[c].y, qb[c].x, qb[c].y };
+            put(c, w);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const VF_AS_GLOBAL uint16_t* r = gcast<const uint16_t>(row[c]);
+#pragma unroll
+            for (int k = 0; k < (c < 4 ? K : (int)kMcZ); ++k) v[c * K + k] = r[k];
+        }
     }
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        if (!fast[c])
+            for (int k = 0; k < (c < 4 ? K : (int)kMcZ); ++k) v[c * K + k] = lat_raw(J, cx[c], cy[c], z0 + k);
+    uint32_t lo = 0xFFFFu, hi = 0;
+#pragma unroll
+    for (uint32_t m = 0; m < kMcCols; ++m) {
+        const uint16_t x = mask15(v[m]);
+        if (x != kMcNoLabel) {
+            lo = min(lo, (uint32_t)x);
+            hi = max(hi, (uint32_t)x);
+        }
+    }
+    return !(hi < J.min_target || lo == hi);
+}
+
+// Point k of unit `local` from the unit's columns v: f(point index, masked corners, masked +x neighbour,
+// valid cube); nothing for a point beyond the lattice.
+template <class F>
+__device__ inline void unit_point(const McJob& J, unsigned long long local, const uint16_t* v, int k, F&& f)
+{
+    int px, py, z0;
+    unit_origin(J, local, px, py, z0);
+    const int pz = z0 + k;
+    if (pz >= (int)J.Nz) return;
+    constexpr int K = kMcZ + 1;
+    const uint16_t *A = v, *B = v + K, *C = v + 2 * K, *D = v + 3 * K, *E = v + 4 * K;
+    const uint16_t c[8] = { mask15(A[k]), mask15(A[k + 1]), mask15(B[k + 1]), mask15(B[k]),
+                            mask15(C[k]), mask15(C[k + 1]), mask15(D[k + 1]), mask15(D[k]) };
+    const uint16_t nbx = mask15(E[k]);
+    const bool cube = px >= 1 && py <= (int)J.Ny - 2 && pz <= (int)J.Nz - 2;
+    const uint32_t p = (uint32_t)px * (J.Ny * J.Nz) + (uint32_t)py * J.Nz + (uint32_t)pz;
+    f(p, c, nbx, cube);
+}
+
+// The 8 lanes of a point group (lanes 8g .. 8g + 7, one per point of a unit): sum, and exclusive prefix.
+__device__ inline uint32_t group8_sum(uint32_t x)
+{
+    x += __shfl_xor(x, 1, 8);
+    x += __shfl_xor(x, 2, 8);
+    x += __shfl_xor(x, 4, 8);
+    return x;
+}
+__device__ inline uint32_t group8_excl(uint32_t x, uint32_t k)
+{
+    uint32_t inc = x;
+#pragma unroll
+    for (uint32_t o = 1; o < 8; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 8);
+        if (k >= o) inc += y;
+    }
+    return inc - x;
 }
 
 __device__ inline uint32_t find_job_u64(const unsigned long long* prefix, uint32_t n, unsigned long long u)
@@ -251,6 +325,29 @@ __device__ inline uint32_t block_sum_u32(uint32_t v, uint32_t* wsum)
     for (uint32_t w = 0; w < kMcThreads / 64; ++w) t += wsum[w];
     __syncthreads();
     return t;
+}
+
+// Four block sums in one pass; every thread receives them.
+__device__ inline void block_sum4_u32(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d, uint4* wsum4)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+        d += __shfl_xor(d, o, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum4[wave] = make_uint4(a, b, c, d);
+    __syncthreads();
+    a = b = c = d = 0;
+    for (uint32_t w = 0; w < kMcThreads / 64; ++w) {
+        const uint4 q = wsum4[w];
+        a += q.x;
+        b += q.y;
+        c += q.z;
+        d += q.w;
+    }
+    __syncthreads();
 }
 
 __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t* wsum, uint32_t* total)
@@ -338,85 +435,191 @@ __global__ __launch_bounds__(1024) void k_mc_base(McRun R)
     }
 }
 
-// One thread per unit: pairs with output, their triangle and vertex counts.  A tile's sums go to its
-// first unit's grid with one atomic each (a few tiles straddle two grids: their other grid's units add
-// their own), never one atomic per unit on three shared words.
+// A tile is kMcThreads consecutive units.  Each thread loads one unit's columns; the units that can have
+// output are queued in LDS in unit order (a block scan), and the queue is worked 8 lanes per unit, one lane
+// per point, so a surface unit costs one pass of its 8 points in parallel and a uniform unit its loads only.
+struct McQueue {
+    uint16_t v[kMcThreads * kMcCols];  // columns of the queued units
+    uint16_t unit[kMcThreads];         // their unit index within the tile
+    uint32_t off[kMcThreads];          // k_mc_pairs: their first pair's offset within the tile
+};
+
+// Pairs with output, their triangle and vertex counts, per unit.  A tile's sums go to its first unit's grid
+// through the block's running sums (a few tiles straddle two grids: their other grid's points add their own).
+// The grid of a tile's first unit, staged in LDS with the first kMcLdsLabels entries of its label index
+// when it changes (block-uniform; the caller's last barrier has freed the previous one).
+struct McStage {
+    McJob J;
+    uint16_t lx[kMcLdsLabels];
+};
+__device__ inline void stage_grid(McStage& G, const McRun& R, uint32_t j0, uint32_t& staged)
+{
+    if (j0 == staged) return;
+    if (threadIdx.x == 0) G.J = R.jobs[j0];
+    const uint16_t* lx = R.jobs[j0].lindex;
+    for (uint32_t i = threadIdx.x; i < kMcLdsLabels; i += blockDim.x) G.lx[i] = lx[i];
+    __syncthreads();
+    staged = j0;
+}
+
+// Grid j's descriptor: from LDS when the whole tile lies in the staged grid (one, block-uniform).
+__device__ inline McJob job_of(const McStage& G, const McRun& R, bool one, uint32_t j)
+{
+    if (one) return G.J;
+    return R.jobs[j];
+}
+
 __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
 {
-    __shared__ unsigned long long sp[kMcMaxJobs + 1];
+    extern __shared__ unsigned long long sp[];  // njobs + 1 unit prefixes
     __shared__ uint32_t wsum[kMcThreads / 64];
+    __shared__ uint4 wsum4[kMcThreads / 64];
     __shared__ uint8_t ntri[256];
+    __shared__ McQueue Q;
+    __shared__ McStage G;
     for (uint32_t i = threadIdx.x; i <= R.njobs; i += blockDim.x) sp[i] = R.unit_prefix[i];
     ntri[threadIdx.x] = (uint8_t)(c_mc_tri[threadIdx.x] >> 60);
     __syncthreads();
-    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
-        const unsigned long long u = (unsigned long long)t * kMcThreads + threadIdx.x;
-        const uint32_t j0 = find_job_u64(sp, R.njobs, (unsigned long long)t * kMcThreads);  // block-uniform
-        uint32_t pairs = 0, nts = 0, nvs = 0, j = j0;
-        if (u < R.total_units) {
-            j = find_job_u64(sp, R.njobs, u);
-            const McJob J = R.jobs[j];
-            walk_unit(J, u - sp[j], [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
-                point_labels(J, ntri, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv) {
-                    ++pairs;
-                    nts += nt;
-                    nvs += nv;
-                });
-            });
-            R.unit_pairs[u] = (uint8_t)pairs;
-            if (pairs && j != j0) {
-                atomicAdd(R.job_tot + 4 * j, (unsigned long long)pairs);
-                atomicAdd(R.job_tot + 4 * j + 1, (unsigned long long)nts);
-                atomicAdd(R.job_tot + 4 * j + 2, (unsigned long long)nvs);
-            }
+    uint32_t staged = 0xFFFFFFFFu;
+    // thread 0: the block's running sums for grid accj, added to the grid's totals when the block moves on
+    // to another grid and at its end (same-address atomics serialise: one flush per block and grid)
+    uint32_t accj = 0xFFFFFFFFu;
+    unsigned long long acc_p = 0, acc_t = 0, acc_v = 0;
+    auto flush = [&]() {
+        if (accj != 0xFFFFFFFFu && acc_p) {
+            atomicAdd(R.job_tot + 4 * accj, acc_p);
+            atomicAdd(R.job_tot + 4 * accj + 1, acc_t);
+            atomicAdd(R.job_tot + 4 * accj + 2, acc_v);
         }
-        const uint32_t tp = block_sum_u32(pairs, wsum);
-        if (tp == 0) {  // block-uniform: nothing in this tile
+        acc_p = acc_t = acc_v = 0;
+    };
+    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
+        const unsigned long long ubase = (unsigned long long)t * kMcThreads, u = ubase + threadIdx.x;
+        const uint32_t j0 = find_job_u64(sp, R.njobs, ubase);  // block-uniform
+        const bool one = sp[j0 + 1] >= ubase + kMcThreads;
+        stage_grid(G, R, j0, staged);
+        bool live = false;
+        uint16_t v[kMcCols];
+        if (u < R.total_units) {
+            const uint32_t j = find_job_u64(sp, R.njobs, u);
+            const McJob J = job_of(G, R, one, j);
+            live = load_unit(J, u - sp[j], v);
+        }
+        uint32_t nq = 0;
+        const uint32_t pos = block_excl_scan_u32(live ? 1u : 0u, wsum, &nq);
+        if (nq == 0) {  // block-uniform: nothing in this tile (k_mc_pairs skips it, unit counts unread)
             if (threadIdx.x == 0) R.tile_pairs[t] = 0;
             continue;
         }
-        const bool mine = j == j0;
-        const uint32_t p0 = block_sum_u32(mine ? pairs : 0u, wsum);
-        const uint32_t t0 = block_sum_u32(mine ? nts : 0u, wsum);
-        const uint32_t v0 = block_sum_u32(mine ? nvs : 0u, wsum);
-        if (threadIdx.x == 0) {
-            R.tile_pairs[t] = tp;
-            if (p0) {
-                atomicAdd(R.job_tot + 4 * j0, (unsigned long long)p0);
-                atomicAdd(R.job_tot + 4 * j0 + 1, (unsigned long long)t0);
-                atomicAdd(R.job_tot + 4 * j0 + 2, (unsigned long long)v0);
+        if (!live && u < R.total_units) R.unit_pairs[u] = 0;
+        if (live) {
+#pragma unroll
+            for (uint32_t m = 0; m < kMcCols; ++m) Q.v[pos * kMcCols + m] = v[m];
+            Q.unit[pos] = (uint16_t)threadIdx.x;
+        }
+        __syncthreads();
+        uint32_t tp = 0, p0 = 0, t0 = 0, v0 = 0;
+        for (uint32_t i = threadIdx.x; i < nq * kMcZ; i += kMcThreads) {  // whole 8-lane groups
+            const uint32_t q = i / kMcZ, k = i % kMcZ;
+            const unsigned long long uq = ubase + Q.unit[q];
+            const uint32_t j = find_job_u64(sp, R.njobs, uq);
+            const bool mine = j == j0;
+            const McJob J = job_of(G, R, one, j);
+            uint32_t pc = 0, tc = 0, vc = 0;
+            unit_point(J, uq - sp[j], Q.v + q * kMcCols, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube) {
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv) {
+                    ++pc;
+                    tc += nt;
+                    vc += nv;
+                });
+            });
+            const uint32_t up = group8_sum(pc);
+            if (k == 0) R.unit_pairs[uq] = (uint8_t)up;
+            tp += pc;
+            if (mine) {
+                p0 += pc;
+                t0 += tc;
+                v0 += vc;
+            } else if (pc) {
+                atomicAdd(R.job_tot + 4 * j, (unsigned long long)pc);
+                atomicAdd(R.job_tot + 4 * j + 1, (unsigned long long)tc);
+                atomicAdd(R.job_tot + 4 * j + 2, (unsigned long long)vc);
             }
         }
+        block_sum4_u32(tp, p0, t0, v0, wsum4);  // ends with a barrier: the queue is free for the next tile
+        if (threadIdx.x == 0) {
+            R.tile_pairs[t] = tp;
+            if (j0 != accj) {
+                flush();
+                accj = j0;
+            }
+            acc_p += p0;
+            acc_t += t0;
+            acc_v += v0;
+        }
     }
+    if (threadIdx.x == 0) flush();
 }
 
-// The pairs of every unit, in unit order, from the tile offsets.
+// The pairs of every unit, in unit order (points in z order, a point's labels in candidate order), from the
+// tile offsets and the per-unit counts.
 __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
 {
-    __shared__ unsigned long long sp[kMcMaxJobs + 1];
+    extern __shared__ unsigned long long sp[];  // njobs + 1 unit prefixes
     __shared__ uint32_t wsum[kMcThreads / 64];
     __shared__ uint8_t ntri[256];
+    __shared__ McQueue Q;
+    __shared__ McStage G;
     for (uint32_t i = threadIdx.x; i <= R.njobs; i += blockDim.x) sp[i] = R.unit_prefix[i];
     ntri[threadIdx.x] = (uint8_t)(c_mc_tri[threadIdx.x] >> 60);
     __syncthreads();
+    uint32_t staged = 0xFFFFFFFFu;
     for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
         if (R.tile_pairs[t] == 0) continue;  // block-uniform
-        const unsigned long long u = (unsigned long long)t * kMcThreads + threadIdx.x;
+        const unsigned long long ubase = (unsigned long long)t * kMcThreads, u = ubase + threadIdx.x;
+        const uint32_t j0 = find_job_u64(sp, R.njobs, ubase);  // block-uniform
+        const bool one = sp[j0 + 1] >= ubase + kMcThreads;
+        stage_grid(G, R, j0, staged);
         const uint32_t n = u < R.total_units ? R.unit_pairs[u] : 0u;
-        uint32_t total;
+        uint32_t total = 0, nq = 0;
         const uint32_t ex = block_excl_scan_u32(n, wsum, &total);
-        if (n == 0) continue;
-        unsigned long long o = R.tile_off[t] + ex;
-        const uint32_t j = find_job_u64(sp, R.njobs, u);
-        const McJob J = R.jobs[j];
-        const uint32_t base = R.label_base[j];
-        walk_unit(J, u - sp[j], [&](uint32_t p, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
-            point_labels(J, ntri, c, nbx, cube, [&](uint16_t li, uint16_t, uint32_t nt, uint32_t nv) {
-                R.keys[o] = base + li;
-                R.vals[o] = (unsigned long long)p | ((unsigned long long)nt << 32) | ((unsigned long long)nv << 36);
-                ++o;
+        const uint32_t pos = block_excl_scan_u32(n ? 1u : 0u, wsum, &nq);
+        if (n) {
+            const uint32_t j = find_job_u64(sp, R.njobs, u);
+            const McJob J = job_of(G, R, one, j);
+            uint16_t v[kMcCols];
+            load_unit(J, u - sp[j], v);
+#pragma unroll
+            for (uint32_t m = 0; m < kMcCols; ++m) Q.v[pos * kMcCols + m] = v[m];
+            Q.unit[pos] = (uint16_t)threadIdx.x;
+            Q.off[pos] = ex;
+        }
+        __syncthreads();
+        const unsigned long long tile_off = R.tile_off[t];
+        for (uint32_t i = threadIdx.x; i < nq * kMcZ; i += kMcThreads) {  // whole 8-lane groups
+            const uint32_t q = i / kMcZ, k = i % kMcZ;
+            const unsigned long long uq = ubase + Q.unit[q];
+            const uint32_t j = find_job_u64(sp, R.njobs, uq);
+            const bool mine = j == j0;
+            const McJob J = job_of(G, R, one, j);
+            const uint32_t base = R.label_base[j];
+            const unsigned long long local = uq - sp[j];
+            const uint16_t* v = Q.v + q * kMcCols;
+            uint32_t pc = 0;
+            unit_point(J, local, v, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube) {
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t, uint32_t) { ++pc; });
             });
-        });
+            unsigned long long o = tile_off + Q.off[q] + group8_excl(pc, k);
+            if (pc == 0) continue;
+            unit_point(J, local, v, (int)k, [&](uint32_t p, const uint16_t* c, uint16_t nbx, bool cube) {
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t li, uint16_t, uint32_t nt, uint32_t nv) {
+                    R.keys[o] = base + li;
+                    R.vals[o] = (unsigned long long)p | ((unsigned long long)nt << 32) | ((unsigned long long)nv << 36);
+                    ++o;
+                });
+            });
+        }
+        __syncthreads();  // the queue is free for the next tile
     }
 }
 
@@ -702,6 +905,8 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         const uint64_t P = (uint64_t)J.Nx * J.Ny * J.Nz;
         if (P > 0xFFFFFFFFull) fail(VF_ERR_INVALID_ARG, "grid too large for marching cubes");
         J.cz = (J.Nz + kMcZ - 1) / kMcZ;
+        const uintptr_t a = reinterpret_cast<uintptr_t>(q.grid);
+        J.vec = (a % 16 == 0 && J.nz % 8 == 0) ? 2u : (a % 8 == 0 && J.nz % 4 == 0) ? 1u : 0u;
         J.want = q.want;
         for (int a = 0; a < 3; ++a) {
             J.mn[a] = q.mn[a];
@@ -749,7 +954,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     VF_CHECK_LAUNCH();
     {
         TimedLaunch tl("mc_count", s);
-        k_mc_count<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, 0, s>>>(R);
+        k_mc_count<<<std::max(1u, std::min(R.tiles, 2048u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
         VF_CHECK_LAUNCH();
     }
     W.ensure(kBScan, scan_tmp_bytes(R.tiles) + 64);
@@ -818,7 +1023,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         R.vals = W.buf[kBVals0].as<unsigned long long>();
         {
             TimedLaunch tl("mc_pairs", s);
-            k_mc_pairs<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, 0, s>>>(R);
+            k_mc_pairs<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
             VF_CHECK_LAUNCH();
         }
         // stable sort by segment: (grid, label, point) order
