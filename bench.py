@@ -113,7 +113,7 @@ STAGES = {
     "naive": ["naive", "naive64"],
     "flood": ["flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan", "flood_finish"],
     "boundaries+histogram": ["boundaries_hist", "hist_summary"],
-    "marching_cubes": ["mc_count", "mc_emit", "mc_finish"],
+    "marching_cubes": ["mc_count", "mc_pairs", "mc_sort", "mc_emit", "mc_finish"],
 }
 KERNELS = [k for ks in STAGES.values() for k in ks] + ["gather"]
 
@@ -546,6 +546,16 @@ def main():
         prof, pres = kernel_profile(vf, jobs, 2)
         out["kernels_ms_per_step"] = {k: round(v["total_ms"], 4) for k, v in prof.items()}
         out["roofline"], out["stage_roofline"] = roofline(prof, specs, pres, a.jobs == DEFAULT_JOBS)
+        if "extract" not in skip:
+            # the extract stage's own roofline, from the same batch with marching cubes
+            mprof, mres = kernel_profile(vf, mjobs, 2)
+            mc_out = sum(r["mc_vertices"] + r["mc_triangles"] for r in mres)
+            ms = sum(mprof[k]["total_ms"] for k in STAGES["marching_cubes"] if k in mprof)
+            mb = stage_bytes(specs, mc_out)["marching_cubes"]
+            out["stage_roofline"]["marching_cubes"] = {
+                "ms_per_step": round(ms, 4), "algorithmic_bytes": mb, "achieved_gbs": round(mb / (ms * 1e-3) / 1e9, 1),
+                "frac": round(mb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "kernels_ms": {k: round(mprof[k]["total_ms"], 4) for k in STAGES["marching_cubes"] if k in mprof}}
         log(f"roofline: {out['roofline']}")
         log(f"stages: {out['stage_roofline']}")
 

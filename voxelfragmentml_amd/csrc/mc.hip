@@ -90,7 +90,7 @@ struct McRun {
     uint32_t* tile_pairs;                   // tiles
     const unsigned long long* tile_off;     // tiles + 1
     uint32_t* keys;                         // pairs: segment
-    unsigned long long* vals;               // pairs: point | triangles << 32 | vertices << 36
+    unsigned long long* vals;               // pairs: point | triangles << 32 | vertices << 36 | configuration << 40 | vertex axes << 48 | flag << 51
     const uint32_t* skeys;                  // sorted by segment
     const unsigned long long* svals;
     uint32_t* cnt_t;                        // triangles of each sorted pair
@@ -121,8 +121,8 @@ __device__ inline uint16_t mask15(uint16_t raw) { return raw == kMcNoLabel ? kMc
 // The labels with output at one point and their counts.  c: the eight masked cube corners (corner 0 is
 // the point itself, corner 1 its +z neighbour, corner 4 its +y neighbour; kMcNoLabel beyond the lattice),
 // nbx: its masked +x neighbour; cube: the point is a valid corner 0; ntri: the table's triangle count
-// per configuration.  f(dense index, label, triangles, vertices) for every label with output; the
-// vertices are the point's own +axis edges with the label on exactly one end; slx holds the first
+// per configuration.  f(dense index, label, triangles, vertices, cube configuration, vertex axes) for every
+// label with output; the vertices are the point's own +axis edges with the label on exactly one end; slx holds the first
 // kMcLdsLabels entries of J's label index when staged.  Fully unrolled: the candidates stay in registers.
 template <class F>
 __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const uint16_t* slx, bool staged, const uint16_t c[8],
@@ -149,17 +149,20 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
         if (seen) continue;
         const uint16_t li = staged && L < kMcLdsLabels ? slx[L] : gcast<const uint16_t>(J.lindex)[L];
         if (li == kMcNoLabel) continue;
-        uint32_t nt = 0;
+        uint32_t nt = 0, conf = 0;
         if (cube) {
-            uint32_t conf = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) conf |= (uint32_t)(c[k] != L) << k;
             nt = ntri[conf];
         }
-        uint32_t nv = 0;
+        uint32_t nv = 0, vax = 0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) nv += (nb[a] != kMcNoLabel && own != nb[a] && (own == L || nb[a] == L)) ? 1u : 0u;
-        if (nt | nv) f(li, L, nt, nv);
+        for (int a = 0; a < 3; ++a)
+            if (nb[a] != kMcNoLabel && own != nb[a] && (own == L || nb[a] == L)) {
+                ++nv;
+                vax |= 1u << a;
+            }
+        if (nt | nv) f(li, L, nt, nv, conf, vax);
     }
 }
 
@@ -258,7 +261,7 @@ __device__ inline bool load_unit(const McJob& J, unsigned long long local, uint1
 }
 
 // Point k of unit `local` from the unit's columns v: f(point index, masked corners, masked +x neighbour,
-// valid cube); nothing for a point beyond the lattice.
+// valid cube, raw value); nothing for a point beyond the lattice.
 template <class F>
 __device__ inline void unit_point(const McJob& J, unsigned long long local, const uint16_t* v, int k, F&& f)
 {
@@ -273,7 +276,7 @@ __device__ inline void unit_point(const McJob& J, unsigned long long local, cons
     const uint16_t nbx = mask15(E[k]);
     const bool cube = px >= 1 && py <= (int)J.Ny - 2 && pz <= (int)J.Nz - 2;
     const uint32_t p = (uint32_t)px * (J.Ny * J.Nz) + (uint32_t)py * J.Nz + (uint32_t)pz;
-    f(p, c, nbx, cube);
+    f(p, c, nbx, cube, A[k]);
 }
 
 // The 8 lanes of a point group (lanes 8g .. 8g + 7, one per point of a unit): sum, and exclusive prefix.
@@ -462,6 +465,19 @@ __device__ inline void stage_grid(McStage& G, const McRun& R, uint32_t j0, uint3
     staged = j0;
 }
 
+// The tiles of the calling block: workgroups are dealt to the 8 XCDs round-robin, so XCD x takes the x-th
+// eighth of the tiles and its blocks stride through it together; the columns a tile shares with its
+// neighbours in x (the next slice of units) are then read by the same XCD at about the same time, from its
+// L2.  Needs gridDim.x >= 8.
+constexpr uint32_t kMcXcds = 8;
+__device__ inline void xcd_tiles(uint32_t tiles, uint32_t& t0, uint32_t& t1, uint32_t& step)
+{
+    const uint32_t x = blockIdx.x % kMcXcds;
+    t0 = (uint32_t)((unsigned long long)tiles * x / kMcXcds) + blockIdx.x / kMcXcds;
+    t1 = (uint32_t)((unsigned long long)tiles * (x + 1) / kMcXcds);
+    step = (gridDim.x - x + kMcXcds - 1) / kMcXcds;  // blocks of this XCD
+}
+
 // Grid j's descriptor: from LDS when the whole tile lies in the staged grid (one, block-uniform).
 __device__ inline McJob job_of(const McStage& G, const McRun& R, bool one, uint32_t j)
 {
@@ -493,7 +509,9 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
         }
         acc_p = acc_t = acc_v = 0;
     };
-    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
+    uint32_t t0, t1, tstep;
+    xcd_tiles(R.tiles, t0, t1, tstep);
+    for (uint32_t t = t0; t < t1; t += tstep) {
         const unsigned long long ubase = (unsigned long long)t * kMcThreads, u = ubase + threadIdx.x;
         const uint32_t j0 = find_job_u64(sp, R.njobs, ubase);  // block-uniform
         const bool one = sp[j0 + 1] >= ubase + kMcThreads;
@@ -526,8 +544,8 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
             const bool mine = j == j0;
             const McJob J = job_of(G, R, one, j);
             uint32_t pc = 0, tc = 0, vc = 0;
-            unit_point(J, uq - sp[j], Q.v + q * kMcCols, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube) {
-                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv) {
+            unit_point(J, uq - sp[j], Q.v + q * kMcCols, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv, uint32_t, uint32_t) {
                     ++pc;
                     tc += nt;
                     vc += nv;
@@ -574,7 +592,9 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
     ntri[threadIdx.x] = (uint8_t)(c_mc_tri[threadIdx.x] >> 60);
     __syncthreads();
     uint32_t staged = 0xFFFFFFFFu;
-    for (uint32_t t = blockIdx.x; t < R.tiles; t += gridDim.x) {
+    uint32_t t0, t1, tstep;
+    xcd_tiles(R.tiles, t0, t1, tstep);
+    for (uint32_t t = t0; t < t1; t += tstep) {
         if (R.tile_pairs[t] == 0) continue;  // block-uniform
         const unsigned long long ubase = (unsigned long long)t * kMcThreads, u = ubase + threadIdx.x;
         const uint32_t j0 = find_job_u64(sp, R.njobs, ubase);  // block-uniform
@@ -606,17 +626,22 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
             const unsigned long long local = uq - sp[j];
             const uint16_t* v = Q.v + q * kMcCols;
             uint32_t pc = 0;
-            unit_point(J, local, v, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube) {
-                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t, uint32_t) { ++pc; });
+            unit_point(J, local, v, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube,
+                             [&](uint16_t, uint16_t, uint32_t, uint32_t, uint32_t, uint32_t) { ++pc; });
             });
             unsigned long long o = tile_off + Q.off[q] + group8_excl(pc, k);
             if (pc == 0) continue;
-            unit_point(J, local, v, (int)k, [&](uint32_t p, const uint16_t* c, uint16_t nbx, bool cube) {
-                point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t li, uint16_t, uint32_t nt, uint32_t nv) {
-                    R.keys[o] = base + li;
-                    R.vals[o] = (unsigned long long)p | ((unsigned long long)nt << 32) | ((unsigned long long)nv << 36);
-                    ++o;
-                });
+            unit_point(J, local, v, (int)k, [&](uint32_t p, const uint16_t* c, uint16_t nbx, bool cube, uint16_t raw) {
+                const unsigned long long flag = (unsigned long long)((raw >> 15) & 1u) << 51;
+                point_labels(J, ntri, G.lx, mine, c, nbx, cube,
+                             [&](uint16_t li, uint16_t, uint32_t nt, uint32_t nv, uint32_t conf, uint32_t vax) {
+                                 R.keys[o] = base + li;
+                                 R.vals[o] = (unsigned long long)p | ((unsigned long long)nt << 32) |
+                                             ((unsigned long long)nv << 36) | ((unsigned long long)conf << 40) |
+                                             ((unsigned long long)vax << 48) | flag;
+                                 ++o;
+                             });
             });
         }
         __syncthreads();  // the queue is free for the next tile
@@ -643,7 +668,9 @@ __device__ inline unsigned long long cube_edge_id(const McJob& J, int px, int py
     return 3ull * lo + (unsigned long long)axis;
 }
 
-// Each sorted pair writes its vertices (own +axis edges, axis order) and triangles.
+// Each sorted pair writes its vertices (own +axis edges, axis order) and triangles, from what the pair
+// carries: point | triangles << 32 | vertices << 36 | cube configuration << 40 | vertex axes << 48 |
+// boundary flag << 51 (no grid reads).
 __global__ void k_mc_emit(McRun R)
 {
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < R.npairs;
@@ -652,24 +679,17 @@ __global__ void k_mc_emit(McRun R)
         const unsigned long long val = R.svals[i];
         const uint32_t p = (uint32_t)val;
         const uint32_t nt = (uint32_t)((val >> 32) & 15u);
+        const uint32_t conf = (uint32_t)((val >> 40) & 0xFFu), vax = (uint32_t)((val >> 48) & 7u);
+        const uint8_t flag = (uint8_t)((val >> 51) & 1u);
         const uint32_t j = find_seg_job(R.label_base, R.njobs, s);
         const McJob& J = R.jobs[j];
-        const uint16_t L = R.labels_all[s];
         const uint32_t yz = J.Ny * J.Nz;
         const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
-        const uint16_t raw0 = lat_raw(J, px, py, pz);
-        const uint16_t own = mask15(raw0);
-        const uint16_t nb[3] = { mask15(lat_raw(J, px + 1, py, pz)), mask15(lat_raw(J, px, py + 1, pz)),
-                                 mask15(lat_raw(J, px, py, pz + 1)) };
         unsigned long long vo = R.vpre[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            if (nb[a] != kMcNoLabel && own != nb[a] && (own == L || nb[a] == L)) R.vert_edge[vo++] = 3ull * p + (unsigned long long)a;
+            if ((vax >> a) & 1u) R.vert_edge[vo++] = 3ull * p + (unsigned long long)a;
         if (nt) {
-            uint32_t conf = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                conf |= (uint32_t)(mask15(lat_raw(J, px + c_mc_nb[k][0], py + c_mc_nb[k][1], pz + c_mc_nb[k][2])) != L) << k;
             const unsigned long long w = c_mc_tri[conf];
             unsigned long long to = R.tpre[i];
             for (uint32_t q = 0; q < nt; ++q, ++to) {
@@ -679,7 +699,7 @@ __global__ void k_mc_emit(McRun R)
                 dst[0] = cube_edge_id(J, px, py, pz, c_mc_edge[e0][0], c_mc_edge[e0][1]);
                 dst[1] = cube_edge_id(J, px, py, pz, c_mc_edge[e2][0], c_mc_edge[e2][1]);
                 dst[2] = cube_edge_id(J, px, py, pz, c_mc_edge[e1][0], c_mc_edge[e1][1]);
-                R.tri_flag[to] = (uint8_t)((raw0 >> 15) & 1u);
+                R.tri_flag[to] = flag;
             }
         }
     }
@@ -954,7 +974,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     VF_CHECK_LAUNCH();
     {
         TimedLaunch tl("mc_count", s);
-        k_mc_count<<<std::max(1u, std::min(R.tiles, 2048u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
+        k_mc_count<<<std::max(kMcXcds, std::min(R.tiles, 2048u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
         VF_CHECK_LAUNCH();
     }
     W.ensure(kBScan, scan_tmp_bytes(R.tiles) + 64);
@@ -1023,7 +1043,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         R.vals = W.buf[kBVals0].as<unsigned long long>();
         {
             TimedLaunch tl("mc_pairs", s);
-            k_mc_pairs<<<std::max(1u, std::min(R.tiles, 16384u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
+            k_mc_pairs<<<std::max(kMcXcds, std::min(R.tiles, 16384u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
             VF_CHECK_LAUNCH();
         }
         // stable sort by segment: (grid, label, point) order
