@@ -139,16 +139,10 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
 #pragma unroll
     for (int i = 1; i < 9; ++i) uniform &= !((valid >> i) & 1u) || cand[i] == own;
     if (uniform) return;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const uint16_t L = cand[i];
-        if (!((valid >> i) & 1u) || L < J.min_target) continue;
-        bool seen = false;
-#pragma unroll
-        for (int k = 0; k < i; ++k) seen |= ((valid >> k) & 1u) && cand[k] == L;
-        if (seen) continue;
+    auto one = [&](uint16_t L) {
+        if (L < J.min_target) return;
         const uint16_t li = staged && L < kMcLdsLabels ? slx[L] : gcast<const uint16_t>(J.lindex)[L];
-        if (li == kMcNoLabel) continue;
+        if (li == kMcNoLabel) return;
         uint32_t nt = 0, conf = 0;
         if (cube) {
 #pragma unroll
@@ -163,6 +157,32 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
                 vax |= 1u << a;
             }
         if (nt | nv) f(li, L, nt, nv, conf, vax);
+    };
+    // the common case: two values meet at the point.  A point's labels may come in any order: each goes to
+    // its own segment, and the segments are ordered by the stable sort that follows.
+    uint32_t lo = 0xFFFFu, hi = 0u;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        if ((valid >> i) & 1u) {
+            lo = min(lo, (uint32_t)cand[i]);
+            hi = max(hi, (uint32_t)cand[i]);
+        }
+    bool two = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) two &= !((valid >> i) & 1u) || cand[i] == lo || cand[i] == hi;
+    if (two) {
+        one((uint16_t)lo);
+        one((uint16_t)hi);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint16_t L = cand[i];
+        if (!((valid >> i) & 1u)) continue;
+        bool seen = false;
+#pragma unroll
+        for (int k = 0; k < i; ++k) seen |= ((valid >> k) & 1u) && cand[k] == L;
+        if (!seen) one(L);
     }
 }
 
