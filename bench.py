@@ -461,16 +461,19 @@ def main():
     log(f"mixed workload: {len(specs)} jobs, {voxels / 1e9:.3f} Gvoxels per step")
     tuning = None
     meshes, jobs = build_jobs(vf, specs)
+    # the same batch with the extract stage: every job's fragment surfaces by marching cubes
+    mjobs = [make_spec_job(vf, m, s, mc=True) for m, s in zip(meshes, specs)] if "extract" not in skip else None
     if "tune" not in skip:
-        # launch shapes from the autotuner, tuned on this batch (outside the timed region): the median batch
-        # time with the default shapes and with the tuned ones
+        # launch shapes from the autotuner, tuned on this batch with its extract stage (every stage the bench
+        # times), outside the timed region: the batch's median time with the default shapes and the tuned ones
         import tempfile
         t0 = time.perf_counter()
         with tempfile.TemporaryDirectory() as d:
             path = Path(d) / "tuning.txt"
-            before, after = vf.tune_jobs(jobs, path, quick=True)
+            before, after = vf.tune_jobs(mjobs or jobs, path, quick=True)
             table = vf.read_tuning(path)
-        tuning = {"default_ms_per_step": round(before, 3), "tuned_ms_per_step": round(after, 3),
+        tuning = {"tuned_on": "the batch with marching cubes" if mjobs else "the batch",
+                  "default_ms_per_step": round(before, 3), "tuned_ms_per_step": round(after, 3),
                   "shape": table[edge_bucket(max(sp["edge"] for sp in specs))],
                   "seconds": round(time.perf_counter() - t0, 2)}
         log(f"autotuner: {tuning}")
@@ -509,9 +512,7 @@ def main():
     }
 
     if "extract" not in skip:
-        # the same batch with the extract stage: every job's fragment surfaces by marching cubes (all jobs in
-        # one launch set after the fracture), grids resident throughout
-        mjobs = [make_spec_job(vf, m, s, mc=True) for m, s in zip(meshes, specs)]
+        # the extract stage: all jobs' marching cubes in one launch set after the fracture, grids resident
         bx = vf.Batch()
         xr = bx.run(mjobs)
         assert all(r["status"] == 0 for r in xr), "a job failed"

@@ -87,9 +87,12 @@ def test_tuner_changes_shapes_not_results(vf, tmp_path):
         xyz, tri = vf.vessel_mesh(40 + i, *vf.vessel_segments(10000))
         m = vf.DeviceMesh(xyz, tri)
         meshes.append(m)
-        jobs.append(m.job((e, e, e), *BOX, S, 100 + i, algo, dist, noise_seed=9 + i, noise_amp=2 if algo else 0))
+        jobs.append(m.job((e, e, e), *BOX, S, 100 + i, algo, dist, noise_seed=9 + i, noise_amp=2 if algo else 0,
+                          marching_cubes=1))
     with vf.Batch() as b:
-        before = [r["checksum"] for r in b.run(jobs)]
+        res = b.run(jobs)
+    before = [r["checksum"] for r in res]
+    tris = [r["mc_triangles"] for r in res]
     path = tmp_path / "tuning.txt"
     vf.tune_run(path, quick=True)
     table = vf.read_tuning(path)
@@ -100,19 +103,23 @@ def test_tuner_changes_shapes_not_results(vf, tmp_path):
     assert after == before
     # every walker run length and round chunking at its extremes: the same grids
     for run, brun, chunk, lead in ((4, 2, 2, 1), (16, 8, 32, 8), (16, 2, 4, 1)):
-        t = {k: dict(v, naive_run=run, brick_run=brun, round_chunk=chunk, lead_chunks=lead, cell_blocks=64 if run == 4 else 16384)
+        t = {k: dict(v, naive_run=run, brick_run=brun, round_chunk=chunk, lead_chunks=lead, cell_blocks=64 if run == 4 else 16384,
+                     mc_blocks=64 if run == 4 else 8192)
              for k, v in table.items()}
         vf.write_tuning(path, t)
         vf.tune_load(path)
         with vf.Batch() as b:
-            assert [r["checksum"] for r in b.run(jobs)] == before, (run, brun, chunk, lead)
+            res = b.run(jobs)
+        assert [r["checksum"] for r in res] == before, (run, brun, chunk, lead)
+        assert [r["mc_triangles"] for r in res] == tris
         assert [r["checksum"] for r in vf.plain_run(jobs)] == before
     # tuning on the batch itself
     t0, t1 = vf.tune_jobs(jobs, path, quick=True)
     assert t0 > 0 and t1 > 0 and sorted(vf.read_tuning(path)) == [0, 1, 2, 3]
     with vf.Batch() as b:
         assert [r["checksum"] for r in b.run(jobs)] == before
-    for bad_line in ("0 64 1\n", "0 64 8192 4608 8192 16384 5 4 8 3\n", "7 64 8192 4608 8192 16384 8 4 8 3\n"):
+    for bad_line in ("0 64 1\n", "0 64 8192 4608 8192 16384 5 4 8 3 2048\n", "7 64 8192 4608 8192 16384 8 4 8 3 2048\n",
+                     "0 64 8192 4608 8192 16384 8 4 8 3 2050\n"):
         bad = tmp_path / "bad.txt"
         bad.write_text(bad_line)
         with pytest.raises(vf.VfError):

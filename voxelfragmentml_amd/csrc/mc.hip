@@ -43,6 +43,7 @@
 #include "cellops.hpp"
 #include "grid.hpp"
 #include "mc_table.hpp"
+#include "tune.hpp"
 
 namespace vf {
 
@@ -994,7 +995,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     VF_CHECK_LAUNCH();
     {
         TimedLaunch tl("mc_count", s);
-        k_mc_count<<<std::max(kMcXcds, std::min(R.tiles, 2048u)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
+        k_mc_count<<<std::max(kMcXcds, std::min(R.tiles, active_shape().mc_blocks)), kMcThreads, (R.njobs + 1) * 8, s>>>(R);
         VF_CHECK_LAUNCH();
     }
     W.ensure(kBScan, scan_tmp_bytes(R.tiles) + 64);

@@ -27,10 +27,10 @@ std::mutex g_mu;
 // late (4096 and 4608 relax in 4.55 ms, 5120 and 6144 in 5.1-5.4 ms).
 // Walker runs: 8 naive tiles, 4 boundary bricks.  Flood: chunks of 8 rounds, the first drain check after 3.
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
@@ -45,7 +45,7 @@ bool shape_ok(const LaunchShape& s)
     return ok(s.cell_blocks) && ok(s.flood_blocks) && ok(s.pair_blocks) && ok(s.brick_blocks) &&
            (s.naive_run == 4 || s.naive_run == 8 || s.naive_run == 16) &&
            (s.brick_run == 2 || s.brick_run == 4 || s.brick_run == 8) && s.round_chunk >= 2 && s.round_chunk <= 32 &&
-           s.lead_chunks >= 1 && s.lead_chunks <= 8;
+           s.lead_chunks >= 1 && s.lead_chunks <= 8 && ok(s.mc_blocks) && s.mc_blocks % 8 == 0;
 }
 
 int tune_bucket(uint32_t max_edge)
@@ -189,11 +189,12 @@ void write_table(const char* path)
     FILE* f = std::fopen(path, "w");
     if (!f) fail(VF_ERR_INVALID_ARG, "cannot write %s", path);
     std::fprintf(f, "# vfrag launch shapes: bucket max_edge cell_blocks flood_blocks pair_blocks brick_blocks "
-                    "naive_run brick_run round_chunk lead_chunks\n");
+                    "naive_run brick_run round_chunk lead_chunks mc_blocks\n");
     for (int b = 0; b < kTuneBuckets; ++b) {
         const LaunchShape s = get_shape(b);
-        std::fprintf(f, "%d %u %u %u %u %u %u %u %u %u\n", b, kEdgeOfBucket[b], s.cell_blocks, s.flood_blocks,
-                     s.pair_blocks, s.brick_blocks, s.naive_run, s.brick_run, s.round_chunk, s.lead_chunks);
+        std::fprintf(f, "%d %u %u %u %u %u %u %u %u %u %u\n", b, kEdgeOfBucket[b], s.cell_blocks, s.flood_blocks,
+                     s.pair_blocks, s.brick_blocks, s.naive_run, s.brick_run, s.round_chunk, s.lead_chunks,
+                     s.mc_blocks);
     }
     std::fclose(f);
 }
@@ -234,6 +235,7 @@ void tune_bucket_on(vf_batch* b, const std::vector<vf_job>& jobs, int bucket, bo
         if (flood_blocks) sweep(&LaunchShape::flood_blocks, { 3072, 4096, 4608, 6144 });
         sweep(&LaunchShape::round_chunk, { 4, 8, 16 });
         sweep(&LaunchShape::lead_chunks, { 1, 2, 3, 4, 6 });
+        sweep(&LaunchShape::mc_blocks, { 1024, 1280, 2048, 2560, 4096 });
     }
     // the final shape timed again, so the figure reported is not the luckiest of the sweep's runs
     if (after) *after = time_batch(b, jobs, reps);
@@ -294,10 +296,10 @@ int vf_tune_load(const char* path)
             int b = -1;
             unsigned edge = 0;
             LaunchShape s = get_shape(kTuneBuckets - 1);
-            const int got = std::sscanf(line, "%d %u %u %u %u %u %u %u %u %u", &b, &edge, &s.cell_blocks, &s.flood_blocks,
-                                        &s.pair_blocks, &s.brick_blocks, &s.naive_run, &s.brick_run, &s.round_chunk,
-                                        &s.lead_chunks);
-            if (got != 10 || b < 0 || b >= kTuneBuckets || !shape_ok(s)) {
+            const int got = std::sscanf(line, "%d %u %u %u %u %u %u %u %u %u %u", &b, &edge, &s.cell_blocks,
+                                        &s.flood_blocks, &s.pair_blocks, &s.brick_blocks, &s.naive_run, &s.brick_run,
+                                        &s.round_chunk, &s.lead_chunks, &s.mc_blocks);
+            if (got != 11 || b < 0 || b >= kTuneBuckets || !shape_ok(s)) {
                 std::fclose(f);
                 fail(VF_ERR_INVALID_ARG, "malformed tuning line: %s", line);
             }

@@ -16,6 +16,7 @@ struct LaunchShape {
     uint32_t brick_run;     // bricks per step of the tiled boundary kernel, interiors loaded together (2, 4, 8)
     uint32_t round_chunk;   // flood rounds per captured chunk of the batched form (2 .. 32)
     uint32_t lead_chunks;   // chunks the batched form runs before its first drain check (1 .. 8)
+    uint32_t mc_blocks;     // marching cubes' count pass workgroups (a multiple of 8, for the XCD split)
 };
 
 constexpr int kTuneBuckets = 4;               // largest edge <= 64, <= 128, <= 256, larger

@@ -389,7 +389,7 @@ def tune_jobs(jobs, path=None, quick: bool = True):
 
 
 TUNING_FIELDS = ("max_edge", "cell_blocks", "flood_blocks", "pair_blocks", "brick_blocks", "naive_run", "brick_run",
-                 "round_chunk", "lead_chunks")
+                 "round_chunk", "lead_chunks", "mc_blocks")
 
 
 def read_tuning(path):
