@@ -26,13 +26,14 @@
 //                 uniform units cost five column loads
 //   scan         pair offsets per tile of units
 //   (one host synchronisation: totals size the outputs)
-//   k_mc_pairs   the pairs, in (grid, point) order, as (segment, point | counts)
+//   k_mc_pairs   the pairs, in (grid, point) order, as (segment, point |
+//                 counts | cube configuration | vertex axes | boundary flag)
 //   radix sort   stable by segment -> (grid, label, point) order
 //   scans        triangle and vertex offsets of every pair
-//   k_mc_emit    each pair's triangles (edge ids, boundary flag) and vertices
-//   k_mc_segments per-segment vertex and triangle offsets
-//   k_mc_finish  vertex positions (FP32, aabb_min + cell_size * (p - 1)) and
-//                 triangle indices (binary search in the label's sorted edges)
+//   k_mc_segments per-segment vertex, triangle and pair offsets
+//   k_mc_emit    each pair's vertex positions (FP32, aabb_min + cell_size *
+//                 (p - 1)) and triangles, with indices local to the label's
+//                 vertices (found through the corners' owner pairs)
 #include <algorithm>
 #include <cstring>
 #include <functional>
@@ -102,9 +103,8 @@ struct McRun {
     uint32_t segments;
     unsigned long long* voff;               // segments + 1
     unsigned long long* toff;
-    unsigned long long* tri_edge;           // 3 per triangle
     uint8_t* tri_flag;
-    unsigned long long* vert_edge;
+    unsigned long long* pstart;             // segments + 1: first pair of each segment
     float* xyz;
     uint32_t* tri_idx;
     unsigned long long nv, nt;
@@ -679,54 +679,114 @@ __global__ void k_mc_split(McRun R)
     }
 }
 
-__device__ inline unsigned long long cube_edge_id(const McJob& J, int px, int py, int pz, int a, int b)
+// The lower bound of (sg, t) among the sorted pairs (the first pair not below it, npairs if none), given a
+// k0 no further than it: a forward gallop, then a binary search.  Lower bounds of increasing targets chain.
+__device__ inline unsigned long long gallop_pair(const McRun& R, unsigned long long k0, uint32_t sg, uint32_t t)
 {
-    const int ax = px + c_mc_nb[a][0], ay = py + c_mc_nb[a][1], az = pz + c_mc_nb[a][2];
-    const int bx = px + c_mc_nb[b][0], by = py + c_mc_nb[b][1], bz = pz + c_mc_nb[b][2];
-    const int axis = ax != bx ? 0 : (ay != by ? 1 : 2);
-    const unsigned long long lo = ((unsigned long long)min(ax, bx) * J.Ny + (unsigned long long)min(ay, by)) * J.Nz +
-                                  (unsigned long long)min(az, bz);
-    return 3ull * lo + (unsigned long long)axis;
+    auto below = [&](unsigned long long k) {
+        const uint32_t sk = R.skeys[k];
+        return sk < sg || (sk == sg && (uint32_t)R.svals[k] < t);
+    };
+    if (k0 >= R.npairs || !below(k0)) return k0;
+    unsigned long long lo = k0, hi = k0, step = 1;  // below(lo)
+    for (;;) {
+        hi = lo + step;
+        if (hi >= R.npairs) {
+            hi = R.npairs;
+            break;
+        }
+        if (!below(hi)) break;
+        lo = hi;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {  // the answer lies in (lo, hi]
+        const unsigned long long mid = (lo + hi) >> 1;
+        if (below(mid)) lo = mid; else hi = mid;
+    }
+    return hi;
 }
 
-// Each sorted pair writes its vertices (own +axis edges, axis order) and triangles, from what the pair
+// Each sorted pair writes its vertices (own +axis edges, axis order: positions aabb_min + cell_size * (p - 1),
+// the edge's axis at + 0.5) and its triangles (table order (0, 2, 1), boundary flag), from what the pair
 // carries: point | triangles << 32 | vertices << 36 | cube configuration << 40 | vertex axes << 48 |
-// boundary flag << 51 (no grid reads).
+// boundary flag << 51, without reading the grid.  A triangle corner is an edge of the cube whose owner
+// (its lower end) is one of the cube's corners; its vertex index within the label is the owner pair's
+// vertex offset plus the rank of the edge's axis among the owner's vertex axes.  Pairs are sorted by
+// (label, point), so the owners at the pair's own x lie just after it and those at x - 1 just after the
+// lower bound of point - Ny * Nz: one binary search per pair, then six short gallops for the cube's corners.
 __global__ void k_mc_emit(McRun R)
 {
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < R.npairs;
          i += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t s = R.skeys[i];
+        const uint32_t sg = R.skeys[i];
         const unsigned long long val = R.svals[i];
         const uint32_t p = (uint32_t)val;
         const uint32_t nt = (uint32_t)((val >> 32) & 15u);
         const uint32_t conf = (uint32_t)((val >> 40) & 0xFFu), vax = (uint32_t)((val >> 48) & 7u);
         const uint8_t flag = (uint8_t)((val >> 51) & 1u);
-        const uint32_t j = find_seg_job(R.label_base, R.njobs, s);
+        const uint32_t j = find_seg_job(R.label_base, R.njobs, sg);
         const McJob& J = R.jobs[j];
         const uint32_t yz = J.Ny * J.Nz;
         const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
+        const int P[3] = { px, py, pz };
         unsigned long long vo = R.vpre[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            if ((vax >> a) & 1u) R.vert_edge[vo++] = 3ull * p + (unsigned long long)a;
-        if (nt) {
-            const unsigned long long w = c_mc_tri[conf];
-            unsigned long long to = R.tpre[i];
-            for (uint32_t q = 0; q < nt; ++q, ++to) {
-                const int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
-                          e2 = (int)((w >> (12 * q + 8)) & 15);
-                unsigned long long* dst = R.tri_edge + 3 * to;
-                dst[0] = cube_edge_id(J, px, py, pz, c_mc_edge[e0][0], c_mc_edge[e0][1]);
-                dst[1] = cube_edge_id(J, px, py, pz, c_mc_edge[e2][0], c_mc_edge[e2][1]);
-                dst[2] = cube_edge_id(J, px, py, pz, c_mc_edge[e1][0], c_mc_edge[e1][1]);
-                R.tri_flag[to] = flag;
+            if ((vax >> a) & 1u) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const float tq = (float)P[q] - 1.0f + (q == a ? 0.5f : 0.0f);
+                    R.xyz[3 * vo + q] = J.mn[q] + J.cs[q] * tq;
+                }
+                ++vo;
             }
+        if (nt == 0) continue;
+        const unsigned long long v0 = R.voff[sg];
+        // the pairs of the cube's 8 corners as edge owners, o = (x side) * 4 + dy * 2 + dz, x side 1 for the
+        // pair's own x and 0 for x - 1: lower bounds, each galloping from the previous one (increasing points)
+        unsigned long long own[8];
+        own[4] = i;
+        own[5] = gallop_pair(R, own[4], sg, p + 1u);
+        own[6] = gallop_pair(R, own[5], sg, p + J.Nz);
+        own[7] = gallop_pair(R, own[6], sg, p + J.Nz + 1u);
+        if (px >= 1) {
+            const uint32_t t = p - yz;
+            unsigned long long lo = R.pstart[sg], hi = i;
+            while (lo < hi) {
+                const unsigned long long mid = (lo + hi) >> 1;
+                if ((uint32_t)R.svals[mid] < t) lo = mid + 1; else hi = mid;
+            }
+            own[0] = lo;
+            own[1] = gallop_pair(R, own[0], sg, t + 1u);
+            own[2] = gallop_pair(R, own[1], sg, t + J.Nz);
+            own[3] = gallop_pair(R, own[2], sg, t + J.Nz + 1u);
+        } else {
+            own[0] = own[1] = own[2] = own[3] = i;  // no cube reaches x - 1 < 0
+        }
+        auto corner = [&](int ca, int cb) -> uint32_t {
+            const int dxa = c_mc_nb[ca][0], dya = c_mc_nb[ca][1], dza = c_mc_nb[ca][2];
+            const int dxb = c_mc_nb[cb][0], dyb = c_mc_nb[cb][1], dzb = c_mc_nb[cb][2];
+            const int axis = dxa != dxb ? 0 : (dya != dyb ? 1 : 2);
+            const int o = (min(dxa, dxb) + 1) * 4 + min(dya, dyb) * 2 + min(dza, dzb);
+            const unsigned long long k = own[o];
+            const uint32_t kax = (uint32_t)((R.svals[k] >> 48) & 7u);
+            return (uint32_t)(R.vpre[k] - v0) + (uint32_t)__popc(kax & ((1u << axis) - 1u));
+        };
+        const unsigned long long w = c_mc_tri[conf];
+        unsigned long long to = R.tpre[i];
+        for (uint32_t q = 0; q < nt; ++q, ++to) {
+            const int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
+                      e2 = (int)((w >> (12 * q + 8)) & 15);
+            uint32_t* dst = R.tri_idx + 3 * to;
+            dst[0] = corner(c_mc_edge[e0][0], c_mc_edge[e0][1]);
+            dst[1] = corner(c_mc_edge[e2][0], c_mc_edge[e2][1]);
+            dst[2] = corner(c_mc_edge[e1][0], c_mc_edge[e1][1]);
+            R.tri_flag[to] = flag;
         }
     }
 }
 
-// voff / toff of every segment: the offsets of its first pair (pairs are sorted by segment).
+// voff / toff / pstart of every segment: the offsets of its first pair (pairs are sorted by segment).
 __global__ void k_mc_segments(McRun R)
 {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= R.segments; s += gridDim.x * blockDim.x) {
@@ -737,46 +797,7 @@ __global__ void k_mc_segments(McRun R)
         }
         R.voff[s] = R.vpre[lo];
         R.toff[s] = R.tpre[lo];
-    }
-}
-
-// Vertex positions and triangle indices local to their segment's vertices.
-__global__ void k_mc_finish(McRun R)
-{
-    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    for (unsigned long long v = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; v < R.nv; v += stride) {
-        uint32_t lo = 0, hi = R.segments;  // segment of v: largest s with voff[s] <= v
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (R.voff[mid] <= v) lo = mid; else hi = mid;
-        }
-        const McJob& J = R.jobs[find_seg_job(R.label_base, R.njobs, lo)];
-        const unsigned long long e = R.vert_edge[v];
-        const int axis = (int)(e % 3);
-        const unsigned long long p = e / 3;
-        const uint32_t yz = J.Ny * J.Nz;
-        const uint32_t P[3] = { (uint32_t)(p / yz), (uint32_t)((p % yz) / J.Nz), (uint32_t)(p % J.Nz) };
-        for (int q = 0; q < 3; ++q) {
-            const float tq = (float)P[q] - 1.0f + (q == axis ? 0.5f : 0.0f);
-            R.xyz[3 * v + q] = J.mn[q] + J.cs[q] * tq;
-        }
-    }
-    for (unsigned long long t = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; t < R.nt; t += stride) {
-        uint32_t lo = 0, hi = R.segments;  // segment of t
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (R.toff[mid] <= t) lo = mid; else hi = mid;
-        }
-        const unsigned long long vb = R.voff[lo], ve = R.voff[lo + 1];
-        for (int k = 0; k < 3; ++k) {
-            const unsigned long long e = R.tri_edge[3 * t + k];
-            unsigned long long a = vb, b = ve;
-            while (a < b) {
-                const unsigned long long mid = (a + b) >> 1;
-                if (R.vert_edge[mid] < e) a = mid + 1; else b = mid;
-            }
-            R.tri_idx[3 * t + k] = (uint32_t)(a - vb);
-        }
+        R.pstart[s] = lo;
     }
 }
 
@@ -890,7 +911,7 @@ namespace {
 // scratch slots
 enum : int {
     kBJobs, kBLidx, kBUnitPre, kBBase, kBLabels, kBTot, kBUnit, kBTile, kBTileOff, kBScan, kBKeys0, kBKeys1, kBVals0,
-    kBVals1, kBSort, kBCntT, kBCntV, kBTpre, kBVpre, kBSeg, kBTEdge, kBFlag, kBVEdge, kBXyz, kBTri,
+    kBVals1, kBSort, kBCntT, kBCntV, kBTpre, kBVpre, kBSeg, kBFlag, kBXyz, kBTri,
     // smoothing
     kBSmTriG, kBSmFlag, kBSmAdjOff, kBSmTmp, kBSmAdj, kBSmScan, kBSmPrm, kBCount
 };
@@ -1030,7 +1051,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
                                                       (unsigned long long*)nullptr, (unsigned long long*)nullptr,
                                                       (int)npairs, 0, 32, s));
         const size_t phase2 = (size_t)npairs * (4 + 4 + 8 + 8 + 4 + 4 + 8 + 8) + sort_bytes + scan_tmp_bytes(npairs) +
-                              (size_t)NT * (24 + 12 + 1) + (size_t)NV * (8 + 12) + 2 * ((size_t)S + 1) * 8 + 14 * 320;
+                              (size_t)NT * (12 + 1) + (size_t)NV * 12 + 3 * ((size_t)S + 1) * 8 + 12 * 320;
         if (!reserve(phase2 + phase2 / 8)) {
             refuse();
             return;
@@ -1042,9 +1063,10 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     R.nv = NV;
     if (npairs > 0xFFFFFFFFull || NT > 0xFFFFFFFFull || NV > 0xFFFFFFFFull)
         fail(VF_ERR_OVERFLOW, "marching-cubes output exceeds 2^32 elements");
-    W.ensure(kBSeg, 2 * ((size_t)S + 1) * 8 + 64);
+    W.ensure(kBSeg, 3 * ((size_t)S + 1) * 8 + 64);
     R.voff = W.buf[kBSeg].as<unsigned long long>();
     R.toff = R.voff + (S + 1);
+    R.pstart = R.voff + 2 * ((size_t)S + 1);
     W.ensure(kBXyz, (size_t)NV * 12 + 64);
     W.ensure(kBTri, (size_t)NT * 12 + 64);
     W.ensure(kBFlag, (size_t)NT + 64);
@@ -1052,7 +1074,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     R.tri_idx = W.buf[kBTri].as<uint32_t>();
     R.tri_flag = W.buf[kBFlag].as<uint8_t>();
     if (npairs == 0) {
-        k_mc_zero_u64<<<1, 256, 0, s>>>(R.voff, 2 * (S + 1));
+        k_mc_zero_u64<<<1, 256, 0, s>>>(R.voff, 3 * (S + 1));
         VF_CHECK_LAUNCH();
     } else {
         const size_t np = (size_t)npairs;
@@ -1096,20 +1118,11 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         scan_exclusive_u32_to_u64(R.cnt_v, W.buf[kBVpre].as<unsigned long long>(), np, W.buf[kBScan].p, s);
         R.tpre = W.buf[kBTpre].as<unsigned long long>();
         R.vpre = W.buf[kBVpre].as<unsigned long long>();
-        W.ensure(kBTEdge, (size_t)NT * 24 + 64);
-        W.ensure(kBVEdge, (size_t)NV * 8 + 64);
-        R.tri_edge = W.buf[kBTEdge].as<unsigned long long>();
-        R.vert_edge = W.buf[kBVEdge].as<unsigned long long>();
-        {
-            TimedLaunch tl("mc_emit", s);
-            k_mc_emit<<<gp, 256, 0, s>>>(R);
-            VF_CHECK_LAUNCH();
-        }
         k_mc_segments<<<stride_blocks((uint64_t)S + 1, 256, 4096), 256, 0, s>>>(R);
         VF_CHECK_LAUNCH();
         {
-            TimedLaunch tl("mc_finish", s);
-            k_mc_finish<<<stride_blocks(std::max(NV, NT), 256, 16384), 256, 0, s>>>(R);
+            TimedLaunch tl("mc_emit", s);
+            k_mc_emit<<<gp, 256, 0, s>>>(R);
             VF_CHECK_LAUNCH();
         }
     }
