@@ -110,13 +110,6 @@ struct McRun {
     unsigned long long nv, nt;
 };
 
-// Raw value of lattice point (x, y, z): FREE on the padding, kMcNoLabel beyond the lattice.
-__device__ inline uint16_t lat_raw(const McJob& J, int x, int y, int z)
-{
-    if (x < 0 || y < 0 || z < 0 || x >= (int)J.Nx || y >= (int)J.Ny || z >= (int)J.Nz) return kMcNoLabel;
-    if (x == 0 || y == 0 || z == 0 || x == (int)J.Nx - 1 || y == (int)J.Ny - 1 || z == (int)J.Nz - 1) return kFree;
-    return gcast<const uint16_t>(J.grid)[((size_t)(x - 1) * J.ny + (size_t)(y - 1)) * J.nz + (size_t)(z - 1)];
-}
 __device__ inline uint16_t mask15(uint16_t raw) { return raw == kMcNoLabel ? kMcNoLabel : (uint16_t)(raw & kMask15); }
 
 // The labels with output at one point and their counts.  c: the eight masked cube corners (corner 0 is
@@ -191,12 +184,16 @@ __device__ inline void point_labels(const McJob& J, const uint8_t* ntri, const u
 // each at lattice z0 .. z0 + 8, and E = (px + 1, py) at z0 .. z0 + 7.
 constexpr uint32_t kMcCols = 4 * (kMcZ + 1) + kMcZ;
 
-__device__ inline void unit_origin(const McJob& J, unsigned long long local, int& px, int& py, int& z0)
+// A unit's lattice origin (px, py, z0) packed as px | py << 16 and z0: units of a grid number fewer than
+// 2^32 (its lattice points do), so 32-bit divisions suffice.
+struct UnitOrigin {
+    uint32_t pxy, z0;
+};
+__device__ inline UnitOrigin unit_origin(const McJob& J, uint32_t local)
 {
-    const uint32_t col = (uint32_t)(local / J.cz);
-    z0 = (int)((uint32_t)(local - (unsigned long long)col * J.cz) * kMcZ);
-    px = (int)(col / J.Ny);
-    py = (int)(col - (uint32_t)px * J.Ny);
+    const uint32_t col = local / J.cz;
+    const uint32_t px = col / J.Ny;
+    return UnitOrigin{ px | ((col - px * J.Ny) << 16), (local - col * J.cz) * kMcZ };
 }
 
 // Zeros that lanes off the fast path of load_unit read in place of a grid row (16-byte aligned at +8).
@@ -208,10 +205,9 @@ __device__ __align__(16) const uint16_t c_mc_zero[32] = {};
 // z0 is a multiple of 8, so with 16-byte aligned rows lattice z0 + 1 .. z0 + 8 are one 16-byte load, with
 // 8-byte aligned rows two); lanes whose columns touch the padding or the lattice end read zeros and then
 // take those columns value by value.
-__device__ inline bool load_unit(const McJob& J, unsigned long long local, uint16_t v[kMcCols])
+__device__ inline bool load_unit(const McJob& J, UnitOrigin uo, uint16_t v[kMcCols])
 {
-    int px, py, z0;
-    unit_origin(J, local, px, py, z0);
+    const int px = (int)(uo.pxy & 0xFFFFu), py = (int)(uo.pxy >> 16), z0 = (int)uo.z0;
     constexpr int K = kMcZ + 1;
     const int cx[5] = { px, px - 1, px, px - 1, px + 1 }, cy[5] = { py, py, py + 1, py + 1, py };
     const bool zin = z0 >= 1 && z0 + K <= (int)J.Nz - 1;
@@ -265,10 +261,24 @@ __device__ inline bool load_unit(const McJob& J, unsigned long long local, uint1
             for (int k = 0; k < (c < 4 ? K : (int)kMcZ); ++k) v[c * K + k] = r[k];
         }
     }
+    // the units at a column's ends and the columns at or beyond the padding (a few lanes of most tiles): the
+    // column's state once, then each value from its lattice z alone
 #pragma unroll
     for (int c = 0; c < 5; ++c)
-        if (!fast[c])
-            for (int k = 0; k < (c < 4 ? K : (int)kMcZ); ++k) v[c * K + k] = lat_raw(J, cx[c], cy[c], z0 + k);
+        if (!fast[c]) {
+            const bool beyond = cx[c] < 0 || cy[c] < 0 || cx[c] >= (int)J.Nx || cy[c] >= (int)J.Ny;
+            const bool pad = cx[c] == 0 || cy[c] == 0 || cx[c] == (int)J.Nx - 1 || cy[c] == (int)J.Ny - 1;
+            const VF_AS_GLOBAL uint16_t* r =
+                gcast<const uint16_t>(J.grid) + (beyond || pad ? 0 : ((size_t)(cx[c] - 1) * J.ny + (size_t)(cy[c] - 1)) * J.nz);
+            for (int k = 0; k < (c < 4 ? K : (int)kMcZ); ++k) {
+                const int z = z0 + k;
+                uint16_t x;
+                if (beyond || z >= (int)J.Nz) x = kMcNoLabel;
+                else if (pad || z == 0 || z == (int)J.Nz - 1) x = kFree;
+                else x = r[z - 1];
+                v[c * K + k] = x;
+            }
+        }
     uint32_t lo = 0xFFFFu, hi = 0;
 #pragma unroll
     for (uint32_t m = 0; m < kMcCols; ++m) {
@@ -284,10 +294,9 @@ __device__ inline bool load_unit(const McJob& J, unsigned long long local, uint1
 // Point k of unit `local` from the unit's columns v: f(point index, masked corners, masked +x neighbour,
 // valid cube, raw value); nothing for a point beyond the lattice.
 template <class F>
-__device__ inline void unit_point(const McJob& J, unsigned long long local, const uint16_t* v, int k, F&& f)
+__device__ inline void unit_point(const McJob& J, UnitOrigin uo, const uint16_t* v, int k, F&& f)
 {
-    int px, py, z0;
-    unit_origin(J, local, px, py, z0);
+    const int px = (int)(uo.pxy & 0xFFFFu), py = (int)(uo.pxy >> 16), z0 = (int)uo.z0;
     const int pz = z0 + k;
     if (pz >= (int)J.Nz) return;
     constexpr int K = kMcZ + 1;
@@ -465,6 +474,7 @@ __global__ __launch_bounds__(1024) void k_mc_base(McRun R)
 struct McQueue {
     uint16_t v[kMcThreads * kMcCols];  // columns of the queued units
     uint16_t unit[kMcThreads];         // their unit index within the tile
+    UnitOrigin org[kMcThreads];        // their lattice origins
     uint32_t off[kMcThreads];          // k_mc_pairs: their first pair's offset within the tile
 };
 
@@ -539,10 +549,12 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
         stage_grid(G, R, j0, staged);
         bool live = false;
         uint16_t v[kMcCols];
+        UnitOrigin uo{};
         if (u < R.total_units) {
             const uint32_t j = find_job_u64(sp, R.njobs, u);
             const McJob J = job_of(G, R, one, j);
-            live = load_unit(J, u - sp[j], v);
+            uo = unit_origin(J, (uint32_t)(u - sp[j]));
+            live = load_unit(J, uo, v);
         }
         uint32_t nq = 0;
         const uint32_t pos = block_excl_scan_u32(live ? 1u : 0u, wsum, &nq);
@@ -555,6 +567,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
 #pragma unroll
             for (uint32_t m = 0; m < kMcCols; ++m) Q.v[pos * kMcCols + m] = v[m];
             Q.unit[pos] = (uint16_t)threadIdx.x;
+            Q.org[pos] = uo;
         }
         __syncthreads();
         uint32_t tp = 0, p0 = 0, t0 = 0, v0 = 0;
@@ -565,7 +578,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
             const bool mine = j == j0;
             const McJob J = job_of(G, R, one, j);
             uint32_t pc = 0, tc = 0, vc = 0;
-            unit_point(J, uq - sp[j], Q.v + q * kMcCols, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
+            unit_point(J, Q.org[q], Q.v + q * kMcCols, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
                 point_labels(J, ntri, G.lx, mine, c, nbx, cube, [&](uint16_t, uint16_t, uint32_t nt, uint32_t nv, uint32_t, uint32_t) {
                     ++pc;
                     tc += nt;
@@ -629,10 +642,12 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
             const uint32_t j = find_job_u64(sp, R.njobs, u);
             const McJob J = job_of(G, R, one, j);
             uint16_t v[kMcCols];
-            load_unit(J, u - sp[j], v);
+            const UnitOrigin uo = unit_origin(J, (uint32_t)(u - sp[j]));
+            load_unit(J, uo, v);
 #pragma unroll
             for (uint32_t m = 0; m < kMcCols; ++m) Q.v[pos * kMcCols + m] = v[m];
             Q.unit[pos] = (uint16_t)threadIdx.x;
+            Q.org[pos] = uo;
             Q.off[pos] = ex;
         }
         __syncthreads();
@@ -644,7 +659,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
             const bool mine = j == j0;
             const McJob J = job_of(G, R, one, j);
             const uint32_t base = R.label_base[j];
-            const unsigned long long local = uq - sp[j];
+            const UnitOrigin local = Q.org[q];
             const uint16_t* v = Q.v + q * kMcCols;
             uint32_t pc = 0;
             unit_point(J, local, v, (int)k, [&](uint32_t, const uint16_t* c, uint16_t nbx, bool cube, uint16_t) {
