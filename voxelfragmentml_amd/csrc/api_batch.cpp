@@ -203,7 +203,8 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
         Engine& E = b->eng;
         if (b->table) {
             Table& T = *b->table;
-            run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T); });
+            run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T, !T.any_flood); });
+            if (T.any_flood) E.naive_side(T);  // beside the flood rounds
             if (T.any_flood) {
                 // the first chunks run without a check; then one readback per chunk.  Jobs with merged
                 // seeds run the sub-seed pass after the first drain and grow again where it freed cells.
@@ -224,6 +225,7 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
                     if ((uint64_t)chunks * sh.round_chunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                 }
             }
+            E.join_side();
             run_captured(E.s, b->g_post, b->use_graphs, [&] { E.post(T); });
             VF_HIP(hipMemcpyAsync(E.pinned, T.gather, 16 * 4 * (size_t)T.L.njobs, hipMemcpyDeviceToHost, E.s));
             VF_HIP(hipStreamSynchronize(E.s));

@@ -169,6 +169,9 @@ Engine::Engine()
 {
     VF_HIP(hipGetDevice(&device));
     VF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    VF_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    VF_HIP(hipEventCreateWithFlags(&side_go, hipEventDisableTiming));
+    VF_HIP(hipEventCreateWithFlags(&side_done, hipEventDisableTiming));
     VF_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), 16 * 4 * kMaxTableJobs));
     mc_.cache = &cache_;  // marching-cubes scratch and meshes come from the same block cache
     hipMemPool_t pool;
@@ -185,7 +188,11 @@ Engine::~Engine()
         for (CacheBuf* b : { &vox_setup_, &vox_counts_, &vox_shell_off_, &scan_tmp_, &seed_cnt_, &seed_off_ }) b->release(cache_);
         (void)hipStreamSynchronize(s);
     }
+    if (side) (void)hipStreamSynchronize(side);
     if (pinned) (void)hipHostFree(pinned);
+    if (side_go) (void)hipEventDestroy(side_go);
+    if (side_done) (void)hipEventDestroy(side_done);
+    if (side) (void)hipStreamDestroy(side);
     if (s) (void)hipStreamDestroy(s);
 }
 
@@ -432,7 +439,7 @@ void Engine::make_room(size_t need, size_t budget)
 }
 
 // Launch-only (safe inside stream capture): reserve_front must have sized the scratch.
-void Engine::front(Table& t)
+void Engine::front(Table& t, bool with_naive)
 {
     set_active_bucket(t.bucket);
     for (EJob* e : t.jobs) e->fronted = true;
@@ -458,12 +465,37 @@ void Engine::front(Table& t)
         launch_seed_pick(t.xL, seed_off_.as<unsigned long long>());
         launch_seed_merge(L);
     }
-    if (t.any_naive32) launch_naive(L, false);
-    if (t.any_naive64) launch_naive(L, true);
+    if (with_naive) {
+        if (t.any_naive32) launch_naive(L, false);
+        if (t.any_naive64) launch_naive(L, true);
+    }
     if (t.any_flood) {
         launch_flood_init(L);
         launch_flood_seed(L);
     }
+}
+
+// The naive jobs and the flood jobs of a table share no memory, so the naive fracture can run on a second
+// stream while the flood's rounds, whose later ones leave most of the device idle, run on s.
+void Engine::naive_side(Table& t)
+{
+    if (!t.any_naive32 && !t.any_naive64) return;
+    set_active_bucket(t.bucket);
+    VF_HIP(hipEventRecord(side_go, s));
+    VF_HIP(hipStreamWaitEvent(side, side_go, 0));
+    Launch L = t.L;
+    L.s = side;
+    if (t.any_naive32) launch_naive(L, false);
+    if (t.any_naive64) launch_naive(L, true);
+    VF_HIP(hipEventRecord(side_done, side));
+    side_pending_ = true;
+}
+
+void Engine::join_side()
+{
+    if (!side_pending_) return;
+    VF_HIP(hipStreamWaitEvent(s, side_done, 0));
+    side_pending_ = false;
 }
 
 void Engine::rounds(Table& t, uint32_t n)

@@ -82,6 +82,8 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     hipStream_t s = nullptr;
+    hipStream_t side = nullptr;        // the batched form's naive fracture, beside the flood rounds
+    hipEvent_t side_go = nullptr, side_done = nullptr;
     int device = 0;
 
     void alloc(EJob& j);    // stream-ordered allocation + descriptor + mesh upload
@@ -97,7 +99,11 @@ public:
     // free cached blocks until `need` more bytes fit in `budget` (or nothing is left to free)
     void make_room(size_t need, size_t budget);
     void trim_cache(size_t keep = 0) { cache_.trim(keep); }
-    void front(Table& t);               // launches only
+    void front(Table& t, bool with_naive = true);  // launches only
+    // the naive fracture of t's naive jobs on the side stream, after the work queued so far on s; s waits
+    // for it at join_side()
+    void naive_side(Table& t);
+    void join_side();
     void rounds(Table& t, uint32_t n);  // flood relaxation rounds
     void post(Table& t);
     void erode(Table& t);      // per-job erosion (post calls it)
@@ -114,6 +120,7 @@ public:
     static constexpr uint32_t kMaxTableJobs = 1024;
 
 private:
+    bool side_pending_ = false;  // naive work queued on the side stream that s has not joined yet
     BlockCache cache_;       // job blocks, tables, worklists and front scratch
     CacheBuf vox_setup_, vox_counts_, vox_shell_off_, scan_tmp_, seed_cnt_, seed_off_;
     McScratch mc_;
