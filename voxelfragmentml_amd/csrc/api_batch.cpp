@@ -108,11 +108,12 @@ struct vf_batch {
     std::vector<EJob> jobs;          // every submitted job (invalid ones keep their status)
     std::vector<EJob*> live;         // jobs on the device
     Table* table = nullptr;
-    Graph g_front, g_rounds, g_post;
+    Graph g_front, g_fstart, g_rounds, g_post;
 
     void clear()
     {
         g_front.reset();
+        g_fstart.reset();
         g_rounds.reset();
         g_post.reset();
         if (table) eng.drop_table(table);
@@ -131,6 +132,7 @@ static void batch_prepare(vf_batch* b, const vf_job* jobs, uint32_t n)
     const uint64_t gen = tune_generation();
     if (gen != b->tune_gen) {  // launch shapes changed: recapture
         b->g_front.reset();
+        b->g_fstart.reset();
         b->g_rounds.reset();
         b->g_post.reset();
         b->tune_gen = gen;
@@ -203,9 +205,10 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
         Engine& E = b->eng;
         if (b->table) {
             Table& T = *b->table;
-            run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T, !T.any_flood); });
-            if (T.any_flood) E.naive_side(T);  // beside the flood rounds
+            run_captured(E.s, b->g_front, b->use_graphs, [&] { E.front(T, T.any_flood); });
             if (T.any_flood) {
+                E.naive_side(T);  // beside the flood's init and rounds
+                run_captured(E.s, b->g_fstart, b->use_graphs, [&] { E.flood_start(T); });
                 // the first chunks run without a check; then one readback per chunk.  Jobs with merged
                 // seeds run the sub-seed pass after the first drain and grow again where it freed cells.
                 const LaunchShape sh = get_shape(T.bucket);

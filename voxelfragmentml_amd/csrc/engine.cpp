@@ -439,7 +439,7 @@ void Engine::make_room(size_t need, size_t budget)
 }
 
 // Launch-only (safe inside stream capture): reserve_front must have sized the scratch.
-void Engine::front(Table& t, bool with_naive)
+void Engine::front(Table& t, bool split)
 {
     set_active_bucket(t.bucket);
     for (EJob* e : t.jobs) e->fronted = true;
@@ -465,18 +465,23 @@ void Engine::front(Table& t, bool with_naive)
         launch_seed_pick(t.xL, seed_off_.as<unsigned long long>());
         launch_seed_merge(L);
     }
-    if (with_naive) {
-        if (t.any_naive32) launch_naive(L, false);
-        if (t.any_naive64) launch_naive(L, true);
-    }
-    if (t.any_flood) {
-        launch_flood_init(L);
-        launch_flood_seed(L);
-    }
+    if (split) return;
+    if (t.any_naive32) launch_naive(L, false);
+    if (t.any_naive64) launch_naive(L, true);
+    flood_start(t);
 }
 
-// The naive jobs and the flood jobs of a table share no memory, so the naive fracture can run on a second
-// stream while the flood's rounds, whose later ones leave most of the device idle, run on s.
+void Engine::flood_start(Table& t)
+{
+    if (!t.any_flood) return;
+    set_active_bucket(t.bucket);
+    launch_flood_init(t.L);
+    launch_flood_seed(t.L);
+}
+
+// The naive jobs and the flood jobs of a table share no memory, so the naive fracture (VALU-bound) can run
+// on a second stream while the flood's key init (HBM-bound) and its rounds, whose later ones leave most of
+// the device idle, run on s.
 void Engine::naive_side(Table& t)
 {
     if (!t.any_naive32 && !t.any_naive64) return;

@@ -99,7 +99,9 @@ public:
     // free cached blocks until `need` more bytes fit in `budget` (or nothing is left to free)
     void make_room(size_t need, size_t budget);
     void trim_cache(size_t keep = 0) { cache_.trim(keep); }
-    void front(Table& t, bool with_naive = true);  // launches only
+    // launches only; split: without the fracture (the batched form's naive_side and flood_start)
+    void front(Table& t, bool split = false);
+    void flood_start(Table& t);  // the flood's key init and seeding
     // the naive fracture of t's naive jobs on the side stream, after the work queued so far on s; s waits
     // for it at join_side()
     void naive_side(Table& t);
