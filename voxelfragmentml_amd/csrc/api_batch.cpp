@@ -109,6 +109,7 @@ struct vf_batch {
     std::vector<EJob*> live;         // jobs on the device
     Table* table = nullptr;
     Graph g_front, g_fstart, g_rounds, g_post;
+    uint32_t drain_chunks[2] = { 0, 0 };  // chunks the last run needed before its drain and after the sub-seed pass
 
     void clear()
     {
@@ -123,6 +124,7 @@ struct vf_batch {
         jobs.clear();
         (void)hipStreamSynchronize(eng.s);
         sig = 0;
+        drain_chunks[0] = drain_chunks[1] = 0;
     }
     ~vf_batch() { clear(); }
 };
@@ -211,19 +213,24 @@ int vf_batch_run(vf_batch* b, const vf_job* jobs, uint32_t n, vf_result* results
                 run_captured(E.s, b->g_fstart, b->use_graphs, [&] { E.flood_start(T); });
                 // the first chunks run without a check; then one readback per chunk.  Jobs with merged
                 // seeds run the sub-seed pass after the first drain and grow again where it freed cells.
+                // A batch run again runs as many chunks before its first check as its last run needed
+                // (the rounds a flood needs hardly change between runs of the same jobs; rounds after a
+                // drain find nothing to do), so a replay waits on the host about once per phase.
                 const LaunchShape sh = get_shape(T.bucket);
-                uint32_t chunks = 0;
+                uint32_t chunks = 0, lead = std::max(sh.lead_chunks, b->drain_chunks[0]);
                 bool sub_done = !T.any_sub;
                 for (;;) {
                     run_captured(E.s, b->g_rounds, b->use_graphs, [&] { E.rounds(T, sh.round_chunk); });
-                    if (++chunks < sh.lead_chunks) continue;
+                    if (++chunks < lead) continue;
                     E.gather(T, E.pinned);
                     VF_HIP(hipStreamSynchronize(E.s));
                     if (all_drained(T, E.pinned)) {
+                        b->drain_chunks[sub_done && T.any_sub ? 1 : 0] = chunks;
                         if (sub_done) break;
                         E.flood_sub(T);
                         sub_done = true;
-                        chunks = sh.lead_chunks - 1;  // check after the next chunk
+                        chunks = 0;
+                        lead = std::max(1u, b->drain_chunks[1]);  // check after the next chunk, or as the last run
                     }
                     if ((uint64_t)chunks * sh.round_chunk > (1ull << 24)) fail(VF_ERR_OVERFLOW, "flood did not drain");
                 }
