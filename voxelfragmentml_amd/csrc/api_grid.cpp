@@ -566,7 +566,7 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         J.sub = sub ? g->flood_sub.as<uint32_t>() : nullptr;
         const uint32_t nbricks = relax_bricks(g->dims[0], g->dims[1], g->dims[2]);
         g->keys.ensure(g->V * 4 + 64);
-        g->buckets.ensure(16ull * relax_words_per_parity(nbricks));
+        g->buckets.ensure(relax_flag_bytes(g->dims[0], g->dims[1], g->dims[2]));  // two flag parities, brick liveness
         g->fctl.ensure(64 * 4);
         J.keys = g->keys.as<uint32_t>();
         J.bflags = g->buckets.as<unsigned long long>();
@@ -593,8 +593,8 @@ int vf_fracture_flood(vf_grid* g, const vf_seed* seeds, uint32_t S, int distance
         VF_HIP(hipMemcpyAsync(g->fctl.as<uint32_t>() + 32, g->pinned, 128, hipMemcpyHostToDevice, g->stream));
         VF_HIP(hipMemsetAsync(J.bflags, 0, 16ull * relax_words_per_parity(nbricks), g->stream));  // both flag parities
         VF_HIP(hipStreamSynchronize(g->stream));  // the pinned words are reused below
-        launch_flood_init(L);
-        launch_flood_seed(L);
+        launch_flood_init(L, W);
+        launch_flood_seed(L, W);
         auto drain = [&]() {
             uint32_t chunk = 4, rounds = 0;
             for (;;) {

@@ -64,7 +64,7 @@ Layout layout_of(const vf_job& j)
     L.stats = o; o = align_up(o + kStatCount * 8);
     L.fctl = o; o = align_up(o + 64 * 4);
     L.keys = o; if (flood) o = align_up(o + L.V * 4);
-    L.bricks = o; if (flood) o = align_up(o + 16ull * relax_words_per_parity(relax_bricks(j.dims[0], j.dims[1], j.dims[2])));
+    L.bricks = o; if (flood) o = align_up(o + relax_flag_bytes(j.dims[0], j.dims[1], j.dims[2]));
     L.sub = o; if (needs_sub(j)) o = align_up(o + 8ull * seed_total(j));  // union-find parents + keep flags
     L.erode = o; if (j.erode) o = align_up(o + L.V * 2);                          // erosion output grid
     L.mask = o; if (j.erode) o = align_up(o + (size_t)kMaxErosionSide * kMaxErosionSide * kMaxErosionSide);
@@ -475,8 +475,8 @@ void Engine::flood_start(Table& t)
 {
     if (!t.any_flood) return;
     set_active_bucket(t.bucket);
-    launch_flood_init(t.L);
-    launch_flood_seed(t.L);
+    launch_flood_init(t.L, t.flood);
+    launch_flood_seed(t.L, t.flood);
 }
 
 // The naive jobs and the flood jobs of a table share no memory, so the naive fracture (VALU-bound) can run

@@ -83,13 +83,47 @@ __device__ inline uint32_t bdim(uint32_t n) { return (n + kRelaxBrick - 1) / kRe
 // One block per job: stamp the seeds (a later seed on the same cell wins, as
 // FloodFracturer.cpp:99-105 writes them in list order), mark seed cells
 // occupied and flag, for round 1, every brick that holds a seed cell or one of
-// its neighbours.  Both flag arrays were zeroed by an earlier launch.
+// its neighbours.  Both flag arrays were zeroed by an earlier launch.  A seed
+// on an EMPTY cell can sit in a brick the key init left without keys: such a
+// brick gets EMPTY keys and its liveness bit first.
 __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
 {
+    __shared__ uint32_t todo[256];
+    __shared__ uint32_t ntodo;
     const JobDev* J = L.jobs + blockIdx.x;
     if (!J->active || J->algo != VF_FLOOD) return;
     const uint32_t S = J->S, b = J->key_bits;
+    const uint32_t nby = bdim(J->ny), nbz = bdim(J->nz);
+    unsigned long long* live = J->bflags + 2u * relax_words_per_parity(J->nbricks);
+    const uint32_t wpc = live_words_per_column(nbz);
+    auto brick_of = [&](const uint4& q) {
+        return ((q.x / kRelaxBrick) * nby + q.y / kRelaxBrick) * nbz + q.z / kRelaxBrick;
+    };
+    // liveness word and bit of a brick id
+    auto live_word = [&](uint32_t id) { return (size_t)(id / nbz) * wpc + (id % nbz) / 64u; };
+    auto live_bit = [&](uint32_t id) { return 1ull << ((id % nbz) & 63u); };
+    for (uint32_t s0 = 0; s0 < S; s0 += blockDim.x) {
+        if (threadIdx.x == 0) ntodo = 0;
+        __syncthreads();
+        if (s0 + threadIdx.x < S) {
+            const uint32_t id = brick_of(J->seeds[s0 + threadIdx.x]);
+            if (!(live[live_word(id)] & live_bit(id))) todo[atomicAdd(&ntodo, 1u)] = id;
+        }
+        __syncthreads();
+        const uint32_t nt = ntodo;  // a brick listed twice is written twice with the same values
+        for (uint32_t t = 0; t < nt; ++t) {
+            const uint32_t id = todo[t];
+            const uint32_t X0 = id / nbz / nby * kRelaxBrick, Y0 = id / nbz % nby * kRelaxBrick, Z0 = id % nbz * kRelaxBrick;
+            for (uint32_t e = threadIdx.x; e < 512u; e += blockDim.x) {
+                const uint32_t x = X0 + (e >> 6), y = Y0 + ((e >> 3) & 7u), z = Z0 + (e & 7u);
+                if (x < J->nx && y < J->ny && z < J->nz) J->keys[((size_t)x * J->ny + y) * J->nz + z] = kEmptyKey;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < nt) atomicOr(live + live_word(todo[threadIdx.x]), live_bit(todo[threadIdx.x]));
+    }
     if (threadIdx.x < kFloodCtlWords) J->fctl[threadIdx.x] = 0;
+    __syncthreads();
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
         J->keys[((size_t)q.x * J->ny + q.y) * J->nz + q.z] = 0u;
@@ -104,7 +138,6 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
         atomicMax(J->keys + c, seed_key(s, c));
     }
     __syncthreads();
-    const uint32_t nby = bdim(J->ny), nbz = bdim(J->nz);
     unsigned long long* flags1 = J->bflags + relax_words_per_parity(J->nbricks);  // round 1 consumes parity 1
     for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
         const uint4 q = J->seeds[s];
@@ -129,6 +162,40 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
     }
 }
 
+// Per brick: the liveness bits of the brick and its 26 neighbours (bit (dx + 1) * 9 + (dy + 1) * 3 +
+// (dz + 1); neighbours outside the grid are not live), read by every relaxation visit.  One thread per
+// brick; a job's bricks are dealt over its compaction tasks, 4096 per task.
+__global__ __launch_bounds__(256) void k_flood_nbr(Launch L, const uint32_t* task_prefix, uint32_t total_tasks)
+{
+    const uint64_t slots = (uint64_t)total_tasks * 4096u;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < slots; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t task = (uint32_t)(g >> 12);
+        const uint32_t j = find_job(task_prefix, L.njobs, task);
+        const JobDev* J = L.jobs + j;
+        if (!J->active || J->algo != VF_FLOOD) continue;
+        const uint32_t id = (uint32_t)(g - (uint64_t)task_prefix[j] * 4096u);
+        const uint32_t nb = J->nbricks;
+        if (id >= nb) continue;
+        const uint32_t nbx = bdim(J->nx), nby = bdim(J->ny), nbz = bdim(J->nz), wpc = live_words_per_column(nbz);
+        unsigned long long* live = J->bflags + 2u * relax_words_per_parity(nb);
+        const uint32_t bz = id % nbz, t1 = id / nbz, by = t1 % nby, bx = t1 / nby;
+        uint32_t m = 0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int qx = (int)bx + dx, qy = (int)by + dy;
+                if (qx < 0 || qy < 0 || (uint32_t)qx >= nbx || (uint32_t)qy >= nby) continue;
+                const size_t col = ((size_t)qx * nby + (uint32_t)qy) * wpc;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int qz = (int)bz + dz;
+                    if (qz < 0 || (uint32_t)qz >= nbz) continue;
+                    if ((live[col + (uint32_t)qz / 64u] >> ((uint32_t)qz & 63u)) & 1ull)
+                        m |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+                }
+            }
+        reinterpret_cast<uint32_t*>(live + live_words(nbx, nby, nbz))[id] = m;
+    }
+}
+
 // Uniform per-job state of a round, held in scalar registers.
 struct RelaxJob {
     const JobDev* J;
@@ -138,6 +205,7 @@ struct RelaxJob {
     uint32_t nw;                // flag words per parity
     VF_AS_GLOBAL unsigned long long* cur;  // brick flag bits consumed this round
     VF_AS_GLOBAL unsigned long long* nxt;  // brick flag bits for the next round
+    const VF_AS_GLOBAL uint32_t* nmask;  // per brick: liveness of the brick and its 26 neighbours
     bool n26;
 };
 
@@ -161,10 +229,14 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     const int X0 = (int)(bx * kRelaxBrick), Y0 = (int)(by * kRelaxBrick), Z0 = (int)(bz * kRelaxBrick);
 
     uint32_t halo_live;  // 27 bits: halo classes holding a cell that is not EMPTY
-    // brick + halo -> LDS (cells outside the grid and EMPTY cells as kTileEmpty).  Each lane owns
-    // one or two (y, z) positions of the 10 x 10 halo plane and loads them in all 10 x slices at
-    // once; addresses are clamped into the grid so every load is unconditional, and cells outside
-    // the grid are masked afterwards.
+    // liveness of the brick and its 26 neighbours, bit ix * 9 + iy * 3 + iz (each 0: below, 1: the
+    // brick's own range, 2: above), loaded with the halo.  Cells of a brick whose bit is clear (or
+    // outside the grid) hold no keys and count as EMPTY.
+    const uint32_t nlive = R.nmask[brick];
+    // brick + halo -> LDS (cells outside the grid, of bricks without keys and EMPTY cells as
+    // kTileEmpty).  Each lane owns one or two (y, z) positions of the 10 x 10 halo plane and loads
+    // them in all 10 x slices at once; addresses are clamped into the grid so every load is
+    // unconditional, and cells outside the grid or of bricks without keys are masked afterwards.
     {
         // lanes 36..63 have no second position and repeat their first (same cell, same word)
         const uint32_t p1 = lane, p2 = lane < 36u ? lane + 64u : lane;
@@ -188,8 +260,15 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             v1[hx] = *(const VF_AS_GLOBAL uint32_t*)(sb + ob1);
             v2[hx] = *(const VF_AS_GLOBAL uint32_t*)(sb + ob2);
         }
-        // outside the grid -> all ones (EMPTY keys are all ones already)
-        const uint32_t bad1 = in1 ? 0u : kTileEmpty, bad2 = in2 ? 0u : kTileEmpty;
+        // outside the grid or in a brick without keys -> all ones (EMPTY keys are all ones already);
+        // per x class ix (slice 0, slices 1..8, slice 9) of the position's (y, z) class
+        const uint32_t c1 = halo_class(p1), c2 = halo_class(p2);
+        uint32_t bad1[3], bad2[3];
+#pragma unroll
+        for (uint32_t ix = 0; ix < 3u; ++ix) {
+            bad1[ix] = in1 && ((nlive >> (ix * 9u + c1)) & 1u) ? 0u : kTileEmpty;
+            bad2[ix] = in2 && ((nlive >> (ix * 9u + c2)) & 1u) ? 0u : kTileEmpty;
+        }
         // halo_live bit ix * 9 + iy * 3 + iz: a cell of that halo class (x, y, z each below the brick, beside
         // it, above it) is not EMPTY; a neighbour brick in direction (ix, iy, iz) can only take a drop from
         // this brick through those cells (EMPTY keys never change).  Per position and x class a ballot of
@@ -200,7 +279,8 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
 #pragma unroll
         for (uint32_t hx = 0; hx < 10u; ++hx) {
             const uint32_t xbad = (uint32_t)(X0 + (int)hx - 1) < nx ? 0u : kTileEmpty;
-            const uint32_t a1 = v1[hx] | bad1 | xbad, a2 = v2[hx] | bad2 | xbad;
+            const uint32_t ix = hx == 0u ? 0u : (hx == 9u ? 2u : 1u);
+            const uint32_t a1 = v1[hx] | bad1[ix] | xbad, a2 = v2[hx] | bad2[ix] | xbad;
             tile[hx * 10u * kRowWords + t1] = a1;
             tile[hx * 10u * kRowWords + t2] = a2;
             if (hx == 0u || hx == 9u) {
@@ -424,6 +504,7 @@ __device__ inline void load_relax_state(const JobDev* J, uint32_t r, RelaxJob& R
     unsigned long long* flags = uni_ptr(J->bflags);
     R.cur = gcast<unsigned long long>(flags + (size_t)R.nw * (R.r & 1u));
     R.nxt = gcast<unsigned long long>(flags + (size_t)R.nw * ((R.r + 1u) & 1u));
+    R.nmask = gcast<const uint32_t>(reinterpret_cast<const uint32_t*>(flags + (size_t)R.nw * 2u + live_words(R.nbx, R.nby, R.nbz)));
     R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
 }
 
@@ -589,57 +670,142 @@ __global__ __launch_bounds__(1024) void k_flood_plan(Launch L)
     }
 }
 
-// Cells of thread t in a tile of 2048 for the 4-byte-per-cell kernels: two quads, q = 0 / 1 at
-// 512 * wave + 256 * q + 4 * lane, so that every 16-byte access of a wave is one contiguous kilobyte
-// (8 consecutive cells per thread would put a wave's 16-byte accesses 32 bytes apart).
-__device__ inline uint32_t quad_cell(uint32_t q) { return 512u * (threadIdx.x >> 6) + 256u * q + 4u * (threadIdx.x & 63u); }
+// Key init.  A brick with an occupied cell is live: its bit in the job's liveness bitmap is set and
+// its keys are written (EMPTY, or unreached with the cell's weight).  The keys of the other bricks
+// (most of a vessel's box) are never written and never read: the relaxation takes the cells of
+// bricks whose bit is clear as EMPTY, and the other kernels read a key only where the cell's
+// occupancy byte is set.
+//
+// One wave per bitmap word, i.e. per column chunk (bx, by, 64 bricks along z).  First lane l reads
+// brick bz = 64 * chunk + l over the column's 64 rows (x, y), each load of the wave 512 contiguous
+// bytes of one row, and packs its 512 occupancy bits into 16 words in LDS.  Then the live bricks are
+// written one after another, lane x * 8 + y writing row (x, y) of the brick from the packed bits.
+// Every word of the bitmap is written whole.  A job's words are dealt over the 64 slots of each of
+// its compaction tasks (a task is 64 flag words).
+constexpr uint32_t kInitBitStride = 65;  // LDS words between a wave's packed-bit planes (odd: no bank conflicts)
 
-__global__ __launch_bounds__(kCellThreads) void k_flood_init(Launch L)
+// one bit per byte of v: bit k set iff byte k is not zero
+__device__ inline uint32_t nonzero_bytes4(uint32_t v)
 {
-    constexpr uint32_t kRun = 4;  // tiles per step: all occupancy loads are issued before any key is written
-    __shared__ JobDev sJ;
-    walk_tile_runs<kRun>(L, L.tile_prefix, L.total_tiles, &sJ, [&](const JobDev& J, uint32_t, uint32_t local, uint32_t n) {
-        if (!J.active || J.algo != VF_FLOOD) return;
-        const uint64_t V = J.V;
-        uint32_t o[kRun][2];  // occupancy bytes of the two quads
+    const uint32_t h = ((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u) >> 7;  // 0 or 1 per byte
+    return (h * 0x01020408u) >> 24;  // byte k lands on bit 24 + k; the other products fall on distinct lower bits
+}
+
+__global__ __launch_bounds__(256) void k_flood_init(Launch L, const uint32_t* task_prefix, uint32_t total_tasks)
+{
+    constexpr uint32_t kRows = 16;  // rows per step of the packing pass: their loads are issued together
+    __shared__ uint32_t packed[4 * 16 * kInitBitStride];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* pw = packed + (threadIdx.x >> 6) * 16u * kInitBitStride;  // this wave's 16 planes
+    const uint32_t slots = total_tasks * 64u;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t g = uni(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); g < slots; g += waves) {
+        const uint32_t j = uni(find_job(task_prefix, L.njobs, g >> 6));
+        const JobDev* J = uni_ptr(L.jobs + j);
+        if (uni((uint32_t)(J->active && J->algo == VF_FLOOD)) == 0u) continue;
+        const uint32_t t0 = uni(task_prefix[j]), nslots = (uni(task_prefix[j + 1]) - t0) * 64u;
+        const uint32_t nx = uni(J->nx), ny = uni(J->ny), nz = uni(J->nz), amp = uni(J->noise_amp);
+        const uint32_t nby = bdim(ny), nbz = bdim(nz), wpc = live_words_per_column(nbz);
+        const uint32_t nwords = bdim(nx) * nby * wpc;
+        const uint64_t noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
+        const VF_AS_GLOBAL uint8_t* occ = gcast<const uint8_t>(uni_ptr(J->occ));
+        VF_AS_GLOBAL uint32_t* keys = gcast<uint32_t>(uni_ptr(J->keys));
+        VF_AS_GLOBAL unsigned long long* live =
+            gcast<unsigned long long>(uni_ptr(J->bflags) + 2u * relax_words_per_parity(uni(J->nbricks)));
+        const bool quads = (nz & 3u) == 0u;  // rows of full bricks start on 4-cell boundaries
+        for (uint32_t w = g - t0 * 64u; w < nwords; w += nslots) {
+            const uint32_t chunk = w % wpc, col = w / wpc, by = col % nby, bx = col / nby;
+            // packing: lane l holds brick bz = 64 * chunk + l, cells z0 .. z0 + 7 of each row.  With rows of
+            // a multiple of 4 cells every load is unconditional (no wait between rows): two 4-cell loads at
+            // addresses clamped into the row, masked where they leave the brick column or the grid.
+            const uint32_t bz = chunk * 64u + lane, z0 = bz * kRelaxBrick;
+            const bool inz = bz < nbz;
+            uint32_t bits[16];
+            if (quads) {
+                const uint32_t zlo = min(z0, nz - 4u), zhi = min(z0 + 4u, nz - 4u);
+                const uint32_t mlo = inz ? ~0u : 0u, mhi = inz && z0 + 4u < nz ? ~0u : 0u;
 #pragma unroll
-        for (uint32_t r = 0; r < kRun; ++r)
+                for (uint32_t r0 = 0; r0 < 64u; r0 += kRows) {
+                    uint32_t lo[kRows], hi[kRows];
 #pragma unroll
-            for (uint32_t q = 0; q < 2; ++q) {
-                const uint64_t c = (uint64_t)(local + r) * kTileCells + quad_cell(q);
-                uint32_t v = 0;
-                if (r < n && c + 4 <= V) {
-                    v = *gcast<const uint32_t>(J.occ + c);
-                } else if (r < n) {
-                    for (uint32_t k = 0; k < 4; ++k)
-                        if (c + k < V) v |= (uint32_t)gcast<const uint8_t>(J.occ)[c + k] << (8 * k);
+                    for (uint32_t r = 0; r < kRows; ++r) {
+                        const uint32_t x = bx * kRelaxBrick + ((r0 + r) >> 3), y = by * kRelaxBrick + ((r0 + r) & 7u);
+                        const uint64_t base = ((uint64_t)min(x, nx - 1u) * ny + min(y, ny - 1u)) * nz;
+                        lo[r] = *(const VF_AS_GLOBAL uint32_t*)(occ + base + zlo);
+                        hi[r] = *(const VF_AS_GLOBAL uint32_t*)(occ + base + zhi);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);  // every load of the step is issued before the first wait
+#pragma unroll
+                    for (uint32_t r = 0; r < kRows; ++r) {
+                        const uint32_t x = bx * kRelaxBrick + ((r0 + r) >> 3), y = by * kRelaxBrick + ((r0 + r) & 7u);
+                        const uint32_t rm = x < nx && y < ny ? ~0u : 0u;
+                        lo[r] &= mlo & rm;
+                        hi[r] &= mhi & rm;
+                    }
+#pragma unroll
+                    for (uint32_t r = 0; r < kRows; r += 4) {
+                        uint32_t word = 0;
+#pragma unroll
+                        for (uint32_t q = 0; q < 4u; ++q)
+                            word |= (nonzero_bytes4(lo[r + q]) | (nonzero_bytes4(hi[r + q]) << 4)) << (8u * q);
+                        bits[(r0 + r) >> 2] = word;
+                    }
                 }
-                o[r][q] = v;
+            } else {
+                const uint32_t zn = inz ? min(kRelaxBrick, nz - z0) : 0u;
+#pragma unroll
+                for (uint32_t i = 0; i < 16u; ++i) bits[i] = 0u;
+                for (uint32_t r = 0; r < 64u; ++r) {
+                    const uint32_t x = bx * kRelaxBrick + (r >> 3), y = by * kRelaxBrick + (r & 7u);
+                    if (x >= nx || y >= ny) continue;
+                    const uint64_t c = ((uint64_t)x * ny + y) * nz + z0;
+                    uint32_t b8 = 0;
+                    for (uint32_t k = 0; k < zn; ++k) b8 |= (occ[c + k] ? 1u : 0u) << k;
+#pragma unroll
+                    for (uint32_t i = 0; i < 16u; ++i)
+                        if (i == (r >> 2)) bits[i] |= b8 << (8u * (r & 3u));
+                }
             }
+            uint32_t any = 0;
 #pragma unroll
-        for (uint32_t r = 0; r < kRun; ++r) {
-            if (r >= n) break;
+            for (uint32_t i = 0; i < 16u; ++i) {
+                pw[i * kInitBitStride + lane] = bits[i];
+                any |= bits[i];
+            }
+            const uint64_t m0 = __ballot(any != 0u);
+            if (lane == 0u) live[(size_t)col * wpc + chunk] = m0;
+            if (m0 == 0ull) continue;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // keys of the live bricks; lane = row (x, y) of the brick, its 8 bits from the brick's lane
+            const uint32_t x = bx * kRelaxBrick + (lane >> 3), y = by * kRelaxBrick + (lane & 7u);
+            const bool row = x < nx && y < ny;
+            const uint64_t rowc = ((uint64_t)x * ny + y) * nz;
+            for (uint64_t m = m0; m; m &= m - 1ull) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(m);
+                const uint32_t zb = (chunk * 64u + l) * kRelaxBrick, zr = min(kRelaxBrick, nz - zb);
+                const uint32_t b8 = (pw[(lane >> 2) * kInitBitStride + l] >> (8u * (lane & 3u))) & 0xFFu;
+                if (!row) continue;
+                const uint64_t c = rowc + zb;
+                uint32_t key[8];
 #pragma unroll
-            for (uint32_t q = 0; q < 2; ++q) {
-                const uint64_t c = (uint64_t)(local + r) * kTileCells + quad_cell(q);
-                if (c >= V) continue;
-                uint32_t key[4] = { kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey };
-                if (o[r][q]) {  // most cells of a vessel grid are EMPTY: whole waves skip the weight hashes
+                for (uint32_t k = 0; k < 8u; ++k) key[k] = kEmptyKey;
+                if (b8) {
 #pragma unroll
-                    for (uint32_t k = 0; k < 4; ++k)
-                        key[k] = ((o[r][q] >> (8 * k)) & 0xFFu)
-                                     ? (kUnreachedKey | flood_weight(J.noise_seed, (uint32_t)(c + k), J.noise_amp))
-                                     : kEmptyKey;
+                    for (uint32_t k = 0; k < 8u; ++k)
+                        if ((b8 >> k) & 1u) key[k] = kUnreachedKey | flood_weight(noise_seed, (uint32_t)(c + k), amp);
                 }
-                if (c + 4 <= V) {
-                    gstore4(J.keys + c, make_uint4(key[0], key[1], key[2], key[3]));
+                if (zr == kRelaxBrick && quads) {
+                    *(VF_AS_GLOBAL vf_u32x4*)(keys + c) = vf_u32x4{ key[0], key[1], key[2], key[3] };
+                    *(VF_AS_GLOBAL vf_u32x4*)(keys + c + 4) = vf_u32x4{ key[4], key[5], key[6], key[7] };
                 } else {
-                    for (uint32_t k = 0; k < 4; ++k)
-                        if (c + k < V) gcast<uint32_t>(J.keys)[c + k] = key[k];
+                    for (uint32_t k = 0; k < zr; ++k) keys[c + k] = key[k];
                 }
             }
+            __builtin_amdgcn_wave_barrier();  // the planes are rewritten for the next word
         }
-    });
+    }
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_flood_finish(Launch L)
@@ -778,6 +944,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_sub_touch(Launch L)
         decompose(c0, J, x, y, z);
         uint32_t last_s = 0xFFFFFFFFu, last_t = 0xFFFFFFFFu;
         for (int k = 0; k < kCellsPerThread && c0 + k < J.V; ++k, advance(x, y, z, J)) {
+            if (!J.occ[c0 + k]) continue;  // keys exist only in bricks with an occupied cell
             const uint32_t key = J.keys[c0 + k];
             if (!reached_key(key)) continue;
             const uint32_t s = (key >> 2) & mask;
@@ -791,7 +958,9 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_sub_touch(Launch L)
                 else { dx = 0; dy = 0; dz = 1; }
                 const int xx = (int)x + dx, yy = (int)y + dy, zz = (int)z + dz;
                 if (xx >= (int)J.nx || yy < 0 || yy >= (int)J.ny || zz < 0 || zz >= (int)J.nz) continue;
-                const uint32_t kn = J.keys[((size_t)xx * J.ny + (uint32_t)yy) * J.nz + (uint32_t)zz];
+                const size_t cn = ((size_t)xx * J.ny + (uint32_t)yy) * J.nz + (uint32_t)zz;
+                if (!J.occ[cn]) continue;
+                const uint32_t kn = J.keys[cn];
                 if (!reached_key(kn)) continue;
                 const uint32_t t = (kn >> 2) & mask;
                 if (t == s || frag[t] != frag[s] || (s == last_s && t == last_t)) continue;
@@ -873,6 +1042,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_sub_reseed(Launch L)
         uint32_t x, y, z, last = 0xFFFFFFFFu;
         decompose(c0, J, x, y, z);
         for (int k = 0; k < kCellsPerThread && c0 + k < J.V; ++k, advance(x, y, z, J)) {
+            if (!J.occ[c0 + k]) continue;  // keys exist only in bricks with an occupied cell
             const uint32_t key = J.keys[c0 + k];
             if (!reached_key(key)) continue;
             const uint32_t s = (key >> 2) & mask;
@@ -892,18 +1062,23 @@ __global__ __launch_bounds__(kCellThreads) void k_flood_sub_reseed(Launch L)
 
 }  // namespace
 
-void launch_flood_init(const Launch& L)
+void launch_flood_init(const Launch& L, const FloodRoundScratch& W)
 {
     TimedLaunch tl("flood_init", L.s);
-    const uint32_t blocks = std::max(1u, std::min(L.total_tiles, active_shape().cell_blocks));
-    k_flood_init<<<blocks, kCellThreads, 0, L.s>>>(L);
+    if (W.total_tasks == 0) return;
+    const uint32_t blocks = std::max(1u, std::min(W.total_tasks * 16u, active_shape().cell_blocks));  // 4 waves each
+    k_flood_init<<<blocks, 256, 0, L.s>>>(L, W.task_prefix, W.total_tasks);
     VF_CHECK_LAUNCH();
 }
 
-void launch_flood_seed(const Launch& L)
+void launch_flood_seed(const Launch& L, const FloodRoundScratch& W)
 {
     TimedLaunch tl("flood_seed", L.s);
     k_flood_seed<<<L.njobs, 256, 0, L.s>>>(L);
+    VF_CHECK_LAUNCH();
+    if (W.total_tasks == 0) return;
+    const uint32_t blocks = std::max(1u, std::min(W.total_tasks * 16u, active_shape().cell_blocks));  // 4096 bricks a task
+    k_flood_nbr<<<blocks, 256, 0, L.s>>>(L, W.task_prefix, W.total_tasks);
     VF_CHECK_LAUNCH();
 }
 

@@ -80,7 +80,7 @@ struct JobDev {
     uint64_t seed_key;
     uint32_t* keys;         // flood keys, V: kFloodEmptyKey = EMPTY, else (K << 2) | weight (1..3), K = (distance << key_bits) | seed; kFloodUnreachedKey | weight unreached
     uint32_t* sub;          // sub-seed pass (null when the seeds carry one id per fragment): S union-find parents, S keep flags
-    unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each)
+    unsigned long long* bflags;  // brick flag bits for even / odd rounds (relax_words_per_parity words each), then the brick liveness words
     uint32_t nbricks;       // relaxation bricks (kRelaxBrick^3 cells each)
     uint32_t* fctl;         // flood control: [0] round, [1] rounds run once drained (0 = running), [2..3] flags non-empty, [4..5] round started by the compaction of an even / odd round of a chunk
     unsigned long long* stats;  // kStatCount slots
@@ -212,10 +212,21 @@ inline uint32_t relax_tasks(uint32_t nx, uint32_t ny, uint32_t nz)
 {
     return (relax_words_per_parity(relax_bricks(nx, ny, nz)) + 63u) / 64u;
 }
-// Both brick flag parities of every flood job must be zero before
-// launch_flood_seed (launch_job_init clears them).
-void launch_flood_init(const Launch& L);
-void launch_flood_seed(const Launch& L);
+// Brick liveness (bricks holding an occupied cell, whose keys the init writes), after the two flag
+// parities: a bitmap with one 64-bit word per 64 bricks of a brick column (bx, by), bit bz % 64 of word
+// (bx * nby + by) * wpc + bz / 64, then one 32-bit word per brick with the liveness of the brick and its
+// 26 neighbours, bit (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1).
+__host__ __device__ inline uint32_t live_words_per_column(uint32_t nbz) { return (nbz + 63u) / 64u; }
+__host__ __device__ inline uint32_t live_words(uint32_t nbx, uint32_t nby, uint32_t nbz)
+{
+    return nbx * nby * live_words_per_column(nbz);
+}
+inline uint64_t relax_flag_bytes(uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint32_t nbx = (nx + kRelaxBrick - 1) / kRelaxBrick, nby = (ny + kRelaxBrick - 1) / kRelaxBrick,
+                   nbz = (nz + kRelaxBrick - 1) / kRelaxBrick, nb = relax_bricks(nx, ny, nz);
+    return 16ull * relax_words_per_parity(nb) + 8ull * live_words(nbx, nby, nbz) + 4ull * nb;
+}
 // Device state of the flood rounds of one launch table.
 struct FloodRoundScratch {
     const uint32_t* task_prefix;   // njobs + 1: prefix of relax_tasks over the jobs (0 for non-flood jobs)
@@ -226,6 +237,11 @@ struct FloodRoundScratch {
 };
 constexpr uint32_t kFloodXcds = 8;
 uint32_t flood_list_cap(uint32_t total_tasks);  // entries per XCD list
+// Key init and brick liveness, then seed stamping and the bricks' neighbour liveness (W's task prefix
+// deals the work: a task is 4096 bricks of a job).  Both brick flag parities of every flood job must be
+// zero before launch_flood_seed (launch_job_init clears them).
+void launch_flood_init(const Launch& L, const FloodRoundScratch& W);
+void launch_flood_seed(const Launch& L, const FloodRoundScratch& W);
 // `rounds` (rounded up to an even count) relaxation rounds for every running flood job of L.  Each job
 // continues at its own round, so jobs may join a table between calls; rounds
 // after a job drained do nothing for it, so the launches can be replayed from
