@@ -113,7 +113,7 @@ STAGES = {
     "naive": ["naive", "naive64"],
     "flood": ["flood_init", "flood_seed", "flood_compact", "flood_relax", "flood_plan", "flood_finish"],
     "boundaries+histogram": ["boundaries_hist", "hist_summary"],
-    "marching_cubes": ["mc_count", "mc_pairs", "mc_sort", "mc_emit", "mc_finish"],
+    "marching_cubes": ["mc_count", "mc_pairs", "mc_sort", "mc_emit", "mc_smooth"],
 }
 KERNELS = [k for ks in STAGES.values() for k in ks] + ["gather"]
 

@@ -178,6 +178,25 @@ def test_flood_edge_cases(vf):
     np.testing.assert_array_equal(g.read(), O.flood(base, np.zeros((0, 4), np.uint32), 26))
 
 
+@pytest.mark.parametrize("nb", [6, 26])
+def test_flood_seed_in_empty_brick(vf, nb):
+    """Seeds on EMPTY cells of 8^3 bricks without an occupied cell: the key init writes no keys there,
+    so the seed kernel gives such a brick its keys; one seed touches the occupied block across the brick
+    boundary at x = 16 and grows into it, one sits alone."""
+    dims = (40, 36, 44)
+    base = np.zeros(dims, np.uint16)
+    base[:16, 4:30, 2:40] = 1
+    seeds = np.array([[5, 10, 10, 2], [16, 12, 20, 3], [33, 30, 41, 4], [0, 4, 2, 5]], np.uint32)
+    g = vf.Grid(dims, *BOX)
+    g.write(base)
+    g.fracture_flood(seeds, MAN if nb == 6 else CHE, nb, noise_seed=9, noise_amp=1)
+    out = g.read()
+    ref = O.flood(base, seeds, nb, 9, 1)
+    np.testing.assert_array_equal(out, ref)
+    assert out[33, 30, 41] == 4 and (out == 3).sum() > 100
+    g.close()
+
+
 @pytest.mark.parametrize("size", [1, 2])
 def test_boundaries_histogram_undo(vf, size):
     g, _, _ = voxelized(vf, (64, 64, 64), seed=13)

@@ -80,6 +80,16 @@ constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgr
 
 __device__ inline uint32_t bdim(uint32_t n) { return (n + kRelaxBrick - 1) / kRelaxBrick; }
 
+// wave-uniform values in scalar registers
+__device__ inline uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+template <typename T>
+__device__ inline T* uni_ptr(T* p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint64_t lo = uni((uint32_t)v), hi = uni((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(lo | (hi << 32));
+}
+
 // One block per job: stamp the seeds (a later seed on the same cell wins, as
 // FloodFracturer.cpp:99-105 writes them in list order), mark seed cells
 // occupied and flag, for round 1, every brick that holds a seed cell or one of
@@ -168,31 +178,50 @@ __global__ __launch_bounds__(256) void k_flood_seed(Launch L)
 __global__ __launch_bounds__(256) void k_flood_nbr(Launch L, const uint32_t* task_prefix, uint32_t total_tasks)
 {
     const uint64_t slots = (uint64_t)total_tasks * 4096u;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < slots; g += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t task = (uint32_t)(g >> 12);
-        const uint32_t j = find_job(task_prefix, L.njobs, task);
-        const JobDev* J = L.jobs + j;
-        if (!J->active || J->algo != VF_FLOOD) continue;
-        const uint32_t id = (uint32_t)(g - (uint64_t)task_prefix[j] * 4096u);
-        const uint32_t nb = J->nbricks;
-        if (id >= nb) continue;
-        const uint32_t nbx = bdim(J->nx), nby = bdim(J->ny), nbz = bdim(J->nz), wpc = live_words_per_column(nbz);
-        unsigned long long* live = J->bflags + 2u * relax_words_per_parity(nb);
-        const uint32_t bz = id % nbz, t1 = id / nbz, by = t1 % nby, bx = t1 / nby;
-        uint32_t m = 0;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int qx = (int)bx + dx, qy = (int)by + dy;
-                if (qx < 0 || qy < 0 || (uint32_t)qx >= nbx || (uint32_t)qy >= nby) continue;
-                const size_t col = ((size_t)qx * nby + (uint32_t)qy) * wpc;
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int qz = (int)bz + dz;
-                    if (qz < 0 || (uint32_t)qz >= nbz) continue;
-                    if ((live[col + (uint32_t)qz / 64u] >> ((uint32_t)qz & 63u)) & 1ull)
-                        m |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
-                }
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x; g0 < slots; g0 += (uint64_t)gridDim.x * blockDim.x) {
+        // the block's 256 bricks lie in one task (4096 bricks): one job per block
+        const uint32_t task = (uint32_t)(g0 >> 12);
+        const uint32_t j = uni(find_job(task_prefix, L.njobs, task));
+        const JobDev* J = uni_ptr(L.jobs + j);
+        if (uni((uint32_t)(J->active && J->algo == VF_FLOOD)) == 0u) continue;
+        const uint32_t id = (uint32_t)(g0 - (uint64_t)uni(task_prefix[j]) * 4096u) + threadIdx.x;
+        const uint32_t nb = uni(J->nbricks);
+        const uint32_t nbx = bdim(uni(J->nx)), nby = bdim(uni(J->ny)), nbz = bdim(uni(J->nz)), wpc = live_words_per_column(nbz);
+        unsigned long long* live = uni_ptr(J->bflags) + 2u * relax_words_per_parity(nb);
+        const uint32_t idc = min(id, nb - 1u);
+        const uint32_t bz = idc % nbz, t1 = idc / nbz, by = t1 % nby, bx = t1 / nby;
+        const uint32_t q = bz & 63u, wi = bz >> 6;
+        // per neighbouring column: the words holding bz and, where bz sits at a word's end, the next
+        // or previous one (only for columns of more than 64 bricks); every index is clamped into range
+        uint64_t w0[9], wm[9], wp[9];
+        bool in[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int qx = (int)bx + c / 3 - 1, qy = (int)by + c % 3 - 1;
+            in[c] = qx >= 0 && qy >= 0 && (uint32_t)qx < nbx && (uint32_t)qy < nby;
+            const size_t base = ((size_t)min(max(qx, 0), (int)nbx - 1) * nby + (uint32_t)min(max(qy, 0), (int)nby - 1)) * wpc;
+            w0[c] = live[base + wi];
+            wm[c] = wp[c] = 0ull;
+        }
+        if (wpc > 1u) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int qx = (int)bx + c / 3 - 1, qy = (int)by + c % 3 - 1;
+                const size_t base = ((size_t)min(max(qx, 0), (int)nbx - 1) * nby + (uint32_t)min(max(qy, 0), (int)nby - 1)) * wpc;
+                wm[c] = live[base + (wi ? wi - 1u : 0u)];
+                wp[c] = live[base + min(wi + 1u, wpc - 1u)];
             }
-        reinterpret_cast<uint32_t*>(live + live_words(nbx, nby, nbz))[id] = m;
+        }
+        uint32_t m = 0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            // bits of bz - 1, bz, bz + 1 (bits past the column's last brick are clear in the bitmap)
+            const uint32_t lo = q ? (uint32_t)(w0[c] >> (q - 1u)) & 1u : (wi ? (uint32_t)(wm[c] >> 63) : 0u);
+            const uint32_t mid = (uint32_t)(w0[c] >> q) & 1u;
+            const uint32_t hi = q < 63u ? (uint32_t)(w0[c] >> (q + 1u)) & 1u : (wi + 1u < wpc ? (uint32_t)wp[c] & 1u : 0u);
+            if (in[c]) m |= (lo | (mid << 1) | (hi << 2)) << (3 * c);
+        }
+        if (id < nb) reinterpret_cast<uint32_t*>(live + live_words(nbx, nby, nbz))[id] = m;
     }
 }
 
@@ -209,14 +238,6 @@ struct RelaxJob {
     bool n26;
 };
 
-__device__ inline uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-template <typename T>
-__device__ inline T* uni_ptr(T* p)
-{
-    const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint64_t lo = uni((uint32_t)v), hi = uni((uint32_t)(v >> 32));
-    return reinterpret_cast<T*>(lo | (hi << 32));
-}
 
 // Relax one brick to local stability against its halo, write the cells that
 // dropped and flag the neighbour bricks that see one of them.  Returns the
