@@ -729,75 +729,99 @@ __device__ inline unsigned long long gallop_pair(const McRun& R, unsigned long l
 // vertex offset plus the rank of the edge's axis among the owner's vertex axes.  Pairs are sorted by
 // (label, point), so the owners at the pair's own x lie just after it and those at x - 1 just after the
 // lower bound of point - Ny * Nz: one binary search per pair, then six short gallops for the cube's corners.
-__global__ void k_mc_emit(McRun R)
+// A block takes chunks of 256 consecutive pairs, whose vertices and triangles are two contiguous output ranges: they
+// are built in LDS and copied out with consecutive lanes on consecutive words (scattered 4-byte stores
+// straight from each pair's thread reached HBM as partial-line writes, several times the output bytes).
+constexpr uint32_t kEmitPairs = 256;
+__global__ __launch_bounds__(kEmitPairs) void k_mc_emit(McRun R)
 {
-    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < R.npairs;
-         i += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t sg = R.skeys[i];
-        const unsigned long long val = R.svals[i];
-        const uint32_t p = (uint32_t)val;
-        const uint32_t nt = (uint32_t)((val >> 32) & 15u);
-        const uint32_t conf = (uint32_t)((val >> 40) & 0xFFu), vax = (uint32_t)((val >> 48) & 7u);
-        const uint8_t flag = (uint8_t)((val >> 51) & 1u);
-        const uint32_t j = find_seg_job(R.label_base, R.njobs, sg);
-        const McJob& J = R.jobs[j];
-        const uint32_t yz = J.Ny * J.Nz;
-        const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
-        const int P[3] = { px, py, pz };
-        unsigned long long vo = R.vpre[i];
+    __shared__ float sx[kEmitPairs * 3 * 3];      // at most 3 vertices per pair
+    __shared__ uint32_t st[kEmitPairs * 5 * 3];   // at most 5 triangles per pair
+    __shared__ uint8_t sf[kEmitPairs * 5];
+    // each XCD takes a contiguous eighth of the chunks, so the pairs its searches read (the segment's pairs at
+    // x - 1 and the next rows) are mostly in its own L2
+    uint32_t c0, c1, cstep;
+    xcd_tiles((uint32_t)((R.npairs + kEmitPairs - 1) / kEmitPairs), c0, c1, cstep);
+    for (uint32_t c = c0; c < c1; c += cstep) {
+        const unsigned long long b0 = (unsigned long long)c * kEmitPairs, b1 = min(b0 + kEmitPairs, R.npairs);
+        const unsigned long long vb = R.vpre[b0], tb = R.tpre[b0];
+        const unsigned long long i = b0 + threadIdx.x;
+        if (i < b1) {
+            const uint32_t sg = R.skeys[i];
+            const unsigned long long val = R.svals[i];
+            const uint32_t p = (uint32_t)val;
+            const uint32_t nt = (uint32_t)((val >> 32) & 15u);
+            const uint32_t conf = (uint32_t)((val >> 40) & 0xFFu), vax = (uint32_t)((val >> 48) & 7u);
+            const uint8_t flag = (uint8_t)((val >> 51) & 1u);
+            const uint32_t j = find_seg_job(R.label_base, R.njobs, sg);
+            const McJob& J = R.jobs[j];
+            const uint32_t yz = J.Ny * J.Nz;
+            const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
+            const int P[3] = { px, py, pz };
+            uint32_t vo = (uint32_t)(R.vpre[i] - vb);
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if ((vax >> a) & 1u) {
+            for (int a = 0; a < 3; ++a)
+                if ((vax >> a) & 1u) {
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const float tq = (float)P[q] - 1.0f + (q == a ? 0.5f : 0.0f);
-                    R.xyz[3 * vo + q] = J.mn[q] + J.cs[q] * tq;
+                    for (int q = 0; q < 3; ++q) {
+                        const float tq = (float)P[q] - 1.0f + (q == a ? 0.5f : 0.0f);
+                        sx[3 * vo + q] = J.mn[q] + J.cs[q] * tq;
+                    }
+                    ++vo;
                 }
-                ++vo;
+            if (nt) {
+                const unsigned long long v0 = R.voff[sg];
+                // the pairs of the cube's 8 corners as edge owners, o = (x side) * 4 + dy * 2 + dz, x side 1 for
+                // the pair's own x and 0 for x - 1: lower bounds, each galloping from the previous one
+                unsigned long long own[8];
+                own[4] = i;
+                own[5] = gallop_pair(R, own[4], sg, p + 1u);
+                own[6] = gallop_pair(R, own[5], sg, p + J.Nz);
+                own[7] = gallop_pair(R, own[6], sg, p + J.Nz + 1u);
+                if (px >= 1) {
+                    const uint32_t t = p - yz;
+                    unsigned long long lo = R.pstart[sg], hi = i;
+                    while (lo < hi) {
+                        const unsigned long long mid = (lo + hi) >> 1;
+                        if ((uint32_t)R.svals[mid] < t) lo = mid + 1; else hi = mid;
+                    }
+                    own[0] = lo;
+                    own[1] = gallop_pair(R, own[0], sg, t + 1u);
+                    own[2] = gallop_pair(R, own[1], sg, t + J.Nz);
+                    own[3] = gallop_pair(R, own[2], sg, t + J.Nz + 1u);
+                } else {
+                    own[0] = own[1] = own[2] = own[3] = i;  // no cube reaches x - 1 < 0
+                }
+                auto corner = [&](int ca, int cb) -> uint32_t {
+                    const int dxa = c_mc_nb[ca][0], dya = c_mc_nb[ca][1], dza = c_mc_nb[ca][2];
+                    const int dxb = c_mc_nb[cb][0], dyb = c_mc_nb[cb][1], dzb = c_mc_nb[cb][2];
+                    const int axis = dxa != dxb ? 0 : (dya != dyb ? 1 : 2);
+                    const int o = (min(dxa, dxb) + 1) * 4 + min(dya, dyb) * 2 + min(dza, dzb);
+                    const unsigned long long k = own[o];
+                    const uint32_t kax = (uint32_t)((R.svals[k] >> 48) & 7u);
+                    return (uint32_t)(R.vpre[k] - v0) + (uint32_t)__popc(kax & ((1u << axis) - 1u));
+                };
+                const unsigned long long w = c_mc_tri[conf];
+                uint32_t to = (uint32_t)(R.tpre[i] - tb);
+                for (uint32_t q = 0; q < nt; ++q, ++to) {
+                    const int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
+                              e2 = (int)((w >> (12 * q + 8)) & 15);
+                    st[3 * to + 0] = corner(c_mc_edge[e0][0], c_mc_edge[e0][1]);
+                    st[3 * to + 1] = corner(c_mc_edge[e2][0], c_mc_edge[e2][1]);
+                    st[3 * to + 2] = corner(c_mc_edge[e1][0], c_mc_edge[e1][1]);
+                    sf[to] = flag;
+                }
             }
-        if (nt == 0) continue;
-        const unsigned long long v0 = R.voff[sg];
-        // the pairs of the cube's 8 corners as edge owners, o = (x side) * 4 + dy * 2 + dz, x side 1 for the
-        // pair's own x and 0 for x - 1: lower bounds, each galloping from the previous one (increasing points)
-        unsigned long long own[8];
-        own[4] = i;
-        own[5] = gallop_pair(R, own[4], sg, p + 1u);
-        own[6] = gallop_pair(R, own[5], sg, p + J.Nz);
-        own[7] = gallop_pair(R, own[6], sg, p + J.Nz + 1u);
-        if (px >= 1) {
-            const uint32_t t = p - yz;
-            unsigned long long lo = R.pstart[sg], hi = i;
-            while (lo < hi) {
-                const unsigned long long mid = (lo + hi) >> 1;
-                if ((uint32_t)R.svals[mid] < t) lo = mid + 1; else hi = mid;
-            }
-            own[0] = lo;
-            own[1] = gallop_pair(R, own[0], sg, t + 1u);
-            own[2] = gallop_pair(R, own[1], sg, t + J.Nz);
-            own[3] = gallop_pair(R, own[2], sg, t + J.Nz + 1u);
-        } else {
-            own[0] = own[1] = own[2] = own[3] = i;  // no cube reaches x - 1 < 0
         }
-        auto corner = [&](int ca, int cb) -> uint32_t {
-            const int dxa = c_mc_nb[ca][0], dya = c_mc_nb[ca][1], dza = c_mc_nb[ca][2];
-            const int dxb = c_mc_nb[cb][0], dyb = c_mc_nb[cb][1], dzb = c_mc_nb[cb][2];
-            const int axis = dxa != dxb ? 0 : (dya != dyb ? 1 : 2);
-            const int o = (min(dxa, dxb) + 1) * 4 + min(dya, dyb) * 2 + min(dza, dzb);
-            const unsigned long long k = own[o];
-            const uint32_t kax = (uint32_t)((R.svals[k] >> 48) & 7u);
-            return (uint32_t)(R.vpre[k] - v0) + (uint32_t)__popc(kax & ((1u << axis) - 1u));
-        };
-        const unsigned long long w = c_mc_tri[conf];
-        unsigned long long to = R.tpre[i];
-        for (uint32_t q = 0; q < nt; ++q, ++to) {
-            const int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
-                      e2 = (int)((w >> (12 * q + 8)) & 15);
-            uint32_t* dst = R.tri_idx + 3 * to;
-            dst[0] = corner(c_mc_edge[e0][0], c_mc_edge[e0][1]);
-            dst[1] = corner(c_mc_edge[e2][0], c_mc_edge[e2][1]);
-            dst[2] = corner(c_mc_edge[e1][0], c_mc_edge[e1][1]);
-            R.tri_flag[to] = flag;
-        }
+        __syncthreads();
+        const uint32_t nxw = (uint32_t)(3 * (R.vpre[b1] - vb)), ntt = (uint32_t)(R.tpre[b1] - tb);
+        float* gx = R.xyz + 3 * vb;
+        uint32_t* gt = R.tri_idx + 3 * tb;
+        uint8_t* gf = R.tri_flag + tb;
+        for (uint32_t k = threadIdx.x; k < nxw; k += kEmitPairs) gx[k] = sx[k];
+        for (uint32_t k = threadIdx.x; k < 3 * ntt; k += kEmitPairs) gt[k] = st[k];
+        for (uint32_t k = threadIdx.x; k < ntt; k += kEmitPairs) gf[k] = sf[k];
+        __syncthreads();  // the staging arrays are reused by the next chunk
     }
 }
 
@@ -1137,7 +1161,7 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         VF_CHECK_LAUNCH();
         {
             TimedLaunch tl("mc_emit", s);
-            k_mc_emit<<<gp, 256, 0, s>>>(R);
+            k_mc_emit<<<stride_blocks(np, kEmitPairs, 16384), kEmitPairs, 0, s>>>(R);
             VF_CHECK_LAUNCH();
         }
     }
