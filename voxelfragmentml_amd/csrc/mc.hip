@@ -74,6 +74,7 @@ struct McJob {
     uint32_t want;             // VF_ALL_LABELS or one label
     uint32_t min_target;       // smallest target label (65536 when none): units below it emit nothing
     uint32_t vec;              // grid rows 16-byte aligned (2), 8-byte aligned (1) or neither (0)
+    float inv_cz, inv_Ny;      // 1 / cz and 1 / Ny (small_div)
     float mn[3], cs[3];
 };
 
@@ -194,6 +195,25 @@ __device__ inline UnitOrigin unit_origin(const McJob& J, uint32_t local)
     const uint32_t col = local / J.cz;
     const uint32_t px = col / J.Ny;
     return UnitOrigin{ px | ((col - px * J.Ny) << 16), (local - col * J.cz) * kMcZ };
+}
+
+// x / d for x below 2^17: a float estimate off by at most one, corrected.
+__device__ inline uint32_t small_div(uint32_t x, uint32_t d, float inv)
+{
+    uint32_t q = (uint32_t)((float)x * inv);
+    const int r = (int)x - (int)(q * d);
+    if (r < 0) --q;
+    else if (r >= (int)d) ++q;
+    return q;
+}
+// The origin of the unit `lane` units after unit `local0` of grid J, from local0's column (col0 = local0 / cz,
+// split as px0 = col0 / Ny, py0 = col0 % Ny) and its remainder rz0 = local0 % cz: the divisions left are of
+// numbers below 2^17 (lane < 256, rz0 < cz, py0 < Ny, all below 2^16).
+__device__ inline UnitOrigin unit_origin_after(const McJob& J, uint32_t px0, uint32_t py0, uint32_t rz0, uint32_t lane)
+{
+    const uint32_t t = rz0 + lane, dc = small_div(t, J.cz, J.inv_cz);
+    const uint32_t yy = py0 + dc, dx = small_div(yy, J.Ny, J.inv_Ny);
+    return UnitOrigin{ (px0 + dx) | ((yy - dx * J.Ny) << 16), (t - dc * J.cz) * kMcZ };
 }
 
 // Zeros that lanes off the fast path of load_unit read in place of a grid row (16-byte aligned at +8).
@@ -550,10 +570,21 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
         bool live = false;
         uint16_t v[kMcCols];
         UnitOrigin uo{};
+        // a tile in one grid: its first unit's column once (scalar), each lane's origin from it
+        uint32_t px0 = 0, py0 = 0, rz0 = 0;
+        if (one) {
+            const uint32_t l0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ubase - sp[j0]));
+            const uint32_t cz = (uint32_t)__builtin_amdgcn_readfirstlane((int)G.J.cz),
+                           Ny = (uint32_t)__builtin_amdgcn_readfirstlane((int)G.J.Ny);
+            const uint32_t col0 = l0 / cz;
+            rz0 = l0 - col0 * cz;
+            px0 = col0 / Ny;
+            py0 = col0 - px0 * Ny;
+        }
         if (u < R.total_units) {
-            const uint32_t j = find_job_u64(sp, R.njobs, u);
+            const uint32_t j = one ? j0 : find_job_u64(sp, R.njobs, u);
             const McJob J = job_of(G, R, one, j);
-            uo = unit_origin(J, (uint32_t)(u - sp[j]));
+            uo = one ? unit_origin_after(J, px0, py0, rz0, threadIdx.x) : unit_origin(J, (uint32_t)(u - sp[j]));
             live = load_unit(J, uo, v);
         }
         uint32_t nq = 0;
@@ -574,7 +605,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
         for (uint32_t i = threadIdx.x; i < nq * kMcZ; i += kMcThreads) {  // whole 8-lane groups
             const uint32_t q = i / kMcZ, k = i % kMcZ;
             const unsigned long long uq = ubase + Q.unit[q];
-            const uint32_t j = find_job_u64(sp, R.njobs, uq);
+            const uint32_t j = one ? j0 : find_job_u64(sp, R.njobs, uq);
             const bool mine = j == j0;
             const McJob J = job_of(G, R, one, j);
             uint32_t pc = 0, tc = 0, vc = 0;
@@ -639,7 +670,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
         const uint32_t ex = block_excl_scan_u32(n, wsum, &total);
         const uint32_t pos = block_excl_scan_u32(n ? 1u : 0u, wsum, &nq);
         if (n) {
-            const uint32_t j = find_job_u64(sp, R.njobs, u);
+            const uint32_t j = one ? j0 : find_job_u64(sp, R.njobs, u);
             const McJob J = job_of(G, R, one, j);
             uint16_t v[kMcCols];
             const UnitOrigin uo = unit_origin(J, (uint32_t)(u - sp[j]));
@@ -655,7 +686,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
         for (uint32_t i = threadIdx.x; i < nq * kMcZ; i += kMcThreads) {  // whole 8-lane groups
             const uint32_t q = i / kMcZ, k = i % kMcZ;
             const unsigned long long uq = ubase + Q.unit[q];
-            const uint32_t j = find_job_u64(sp, R.njobs, uq);
+            const uint32_t j = one ? j0 : find_job_u64(sp, R.njobs, uq);
             const bool mine = j == j0;
             const McJob J = job_of(G, R, one, j);
             const uint32_t base = R.label_base[j];
@@ -1005,9 +1036,12 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
         J.Nx = q.dims[0] + 2; J.Ny = q.dims[1] + 2; J.Nz = q.dims[2] + 2;
         const uint64_t P = (uint64_t)J.Nx * J.Ny * J.Nz;
         if (P > 0xFFFFFFFFull) fail(VF_ERR_INVALID_ARG, "grid too large for marching cubes");
+        if (J.Nx > 0xFFFFu || J.Ny > 0xFFFFu) fail(VF_ERR_INVALID_ARG, "marching cubes: x and y dims must stay below 65534");
         J.cz = (J.Nz + kMcZ - 1) / kMcZ;
         const uintptr_t a = reinterpret_cast<uintptr_t>(q.grid);
         J.vec = (a % 16 == 0 && J.nz % 8 == 0) ? 2u : (a % 8 == 0 && J.nz % 4 == 0) ? 1u : 0u;
+        J.inv_cz = 1.0f / (float)J.cz;
+        J.inv_Ny = 1.0f / (float)J.Ny;
         J.want = q.want;
         for (int a = 0; a < 3; ++a) {
             J.mn[a] = q.mn[a];
