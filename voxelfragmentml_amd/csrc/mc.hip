@@ -536,7 +536,8 @@ __device__ inline McJob job_of(const McStage& G, const McRun& R, bool one, uint3
     return R.jobs[j];
 }
 
-__global__ __launch_bounds__(kMcThreads) void k_mc_count(McRun R)
+// 96 registers: 5 waves per SIMD (the compiler's own choice, 103, allows 4; the count is latency-bound)
+__global__ __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_mc_count(McRun R)
 {
     extern __shared__ unsigned long long sp[];  // njobs + 1 unit prefixes
     __shared__ uint32_t wsum[kMcThreads / 64];
