@@ -352,6 +352,9 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     uint32_t* zt = tile + kTileWords;
     uint32_t* yt = tile;
     uint32_t hzlo = 0, hzhi = 0;  // the own row's halo keys at z = -1 and z = 8
+    // 26 neighbours: the rows at y = 0 and y = 7 take their outer y neighbour from a halo row of zt (fixed
+    // during the visit), the others from the neighbouring lane
+    const uint32_t hrow = ((x + 1u) * 10u + (y == 0u ? 0u : 9u)) * kSepStride;
     if (R.n26) {
         // own row's z halo, then the minima that stay fixed during the visit: zt of the 36 halo
         // rows (x-slab 0 or 9, or y-row 0 or 9), then yt of the halo x-slabs 0 and 9
@@ -381,18 +384,20 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     for (;;) {
         uint32_t m[8];
         if (R.n26) {
-            // z pass in registers, then y and x passes through LDS; every window includes the cell
+            // z pass in registers, the y pass across lanes (lane x * 8 + y: the y neighbours are lanes -1 / +1,
+            // by DPP row shifts), then the x pass through LDS; every window includes the cell
             uint32_t zo[8], ym[8];
             zo[0] = min(min(hzlo, k[0]), k[1]);
 #pragma unroll
             for (int z = 1; z < 7; ++z) zo[z] = min(min(k[z - 1], k[z]), k[z + 1]);
             zo[7] = min(min(k[6], k[7]), hzhi);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) zt[((x + 1u) * 10u + y + 1u) * kSepStride + z] = zo[z];
-            __syncthreads();
-#pragma unroll
-            for (int z = 0; z < 8; ++z)
-                ym[z] = min(min(zt[((x + 1u) * 10u + y) * kSepStride + z], zo[z]), zt[((x + 1u) * 10u + y + 2u) * kSepStride + z]);
+            for (int z = 0; z < 8; ++z) {
+                const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)zo[z], (int)zo[z], 0x111, 0xF, 0xF, false);
+                const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)zo[z], (int)zo[z], 0x101, 0xF, 0xF, false);
+                const uint32_t hz = zt[hrow + z];
+                ym[z] = min(min(y == 0u ? hz : dn, zo[z]), y == 7u ? hz : up);
+            }
 #pragma unroll
             for (int z = 0; z < 8; ++z) yt[((x + 1u) * 8u + y) * kSepStride + z] = ym[z];
             __syncthreads();
