@@ -355,11 +355,13 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     // 26 neighbours: the rows at y = 0 and y = 7 take their outer y neighbour from a halo row of zt (fixed
     // during the visit), the others from the neighbouring lane
     const uint32_t hrow = ((x + 1u) * 10u + (y == 0u ? 0u : 9u)) * kSepStride;
+    hzlo = tile[own] & ~3u;
+    hzhi = tile[own + 9] & ~3u;
+    // 6 neighbours: the halo row beside y = 0 or y = 7 in the tile (not rewritten during the visit)
+    const uint32_t hrow6 = ((x + 1u) * 10u + (y == 0u ? 0u : 9u)) * kRowWords + 1u;
     if (R.n26) {
-        // own row's z halo, then the minima that stay fixed during the visit: zt of the 36 halo
-        // rows (x-slab 0 or 9, or y-row 0 or 9), then yt of the halo x-slabs 0 and 9
-        hzlo = tile[own] & ~3u;
-        hzhi = tile[own + 9] & ~3u;
+        // the minima that stay fixed during the visit: zt of the 36 halo rows (x-slab 0 or 9, or y-row
+        // 0 or 9), then yt of the halo x-slabs 0 and 9
         if (lane < 36u) {
             const uint32_t hx = lane < 10u ? 0u : (lane < 20u ? 9u : (lane < 28u ? lane - 19u : lane - 27u));
             const uint32_t hy = lane < 10u ? lane : (lane < 20u ? lane - 10u : (lane < 28u ? 0u : 9u));
@@ -405,24 +407,25 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
             for (int z = 0; z < 8; ++z)
                 m[z] = min(min(yt[(x * 8u + y) * kSepStride + z], ym[z]), yt[((x + 2u) * 8u + y) * kSepStride + z]);
         } else {
-            uint32_t cc[10], fm[8];
-            {
-                const uint32_t* row = tile + own;
+            // the row's own z neighbours from registers, its y neighbours from the neighbouring lanes (DPP row
+            // shifts; the rows at y = 0 and y = 7 take a halo row, fixed during the visit), its x neighbours
+            // from the tile, where every row is written back after a sweep that changed it
+            uint32_t fm[8];
 #pragma unroll
-                for (int o = 0; o < 10; ++o) cc[o] = row[o];
+            for (int z = 0; z < 8; ++z) {
+                const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)k[z], (int)k[z], 0x111, 0xF, 0xF, false);
+                const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)k[z], (int)k[z], 0x101, 0xF, 0xF, false);
+                const uint32_t hz = tile[hrow6 + z];
+                fm[z] = min(y == 0u ? hz : dn, y == 7u ? hz : up);
             }
 #pragma unroll
-            for (int o = 0; o < 8; ++o) fm[o] = 0xFFFFFFFFu;
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const uint32_t fx = f == 0 ? x : (f == 1 ? x + 2u : x + 1u);
-                const uint32_t fy = f < 2 ? y + 1u : (f == 2 ? y : y + 2u);
-                const uint32_t* row = tile + (fx * 10u + fy) * kRowWords + 1u;
+            for (int f = 0; f < 2; ++f) {
+                const uint32_t* row = tile + ((f == 0 ? x : x + 2u) * 10u + y + 1u) * kRowWords + 1u;
 #pragma unroll
                 for (int o = 0; o < 8; ++o) fm[o] = min(fm[o], row[o]);
             }
 #pragma unroll
-            for (int z = 0; z < 8; ++z) m[z] = min(min(cc[z], cc[z + 2]), fm[z]) & ~3u;
+            for (int z = 0; z < 8; ++z) m[z] = min(min(z == 0 ? hzlo : k[z - 1], z == 7 ? hzhi : k[z + 1]), fm[z]) & ~3u;
         }
         uint32_t prev[8];
 #pragma unroll
