@@ -106,6 +106,7 @@ struct McRun {
     unsigned long long* toff;
     uint8_t* tri_flag;
     unsigned long long* pstart;             // segments + 1: first pair of each segment
+    const uint32_t* seg_job;                // segments: the grid of each segment
     float* xyz;
     uint32_t* tri_idx;
     unsigned long long nv, nt;
@@ -785,7 +786,7 @@ __global__ __launch_bounds__(kEmitPairs) void k_mc_emit(McRun R)
             const uint32_t nt = (uint32_t)((val >> 32) & 15u);
             const uint32_t conf = (uint32_t)((val >> 40) & 0xFFu), vax = (uint32_t)((val >> 48) & 7u);
             const uint8_t flag = (uint8_t)((val >> 51) & 1u);
-            const uint32_t j = find_seg_job(R.label_base, R.njobs, sg);
+            const uint32_t j = R.seg_job[sg];
             const McJob& J = R.jobs[j];
             const uint32_t yz = J.Ny * J.Nz;
             const int px = (int)(p / yz), py = (int)((p - (uint32_t)px * yz) / J.Nz), pz = (int)(p - (uint32_t)px * yz - (uint32_t)py * J.Nz);
@@ -811,6 +812,7 @@ __global__ __launch_bounds__(kEmitPairs) void k_mc_emit(McRun R)
                 own[6] = gallop_pair(R, own[5], sg, p + J.Nz);
                 own[7] = gallop_pair(R, own[6], sg, p + J.Nz + 1u);
                 if (px >= 1) {
+                    // the lower bound of point p - yz among the segment's pairs at x - 1
                     const uint32_t t = p - yz;
                     unsigned long long lo = R.pstart[sg], hi = i;
                     while (lo < hi) {
@@ -982,7 +984,7 @@ namespace {
 // scratch slots
 enum : int {
     kBJobs, kBLidx, kBUnitPre, kBBase, kBLabels, kBTot, kBUnit, kBTile, kBTileOff, kBScan, kBKeys0, kBKeys1, kBVals0,
-    kBVals1, kBSort, kBCntT, kBCntV, kBTpre, kBVpre, kBSeg, kBFlag, kBXyz, kBTri,
+    kBVals1, kBSort, kBCntT, kBCntV, kBTpre, kBVpre, kBSeg, kBFlag, kBXyz, kBTri, kBSegJob,
     // smoothing
     kBSmTriG, kBSmFlag, kBSmAdjOff, kBSmTmp, kBSmAdj, kBSmScan, kBSmPrm, kBCount
 };
@@ -1118,6 +1120,10 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     }
     const uint32_t S = base[n];
     W.label_base.assign(base.begin(), base.begin() + n + 1);
+    // the grid of every segment (the emit reads it per pair)
+    std::vector<uint32_t> sjob((size_t)base[n] + 1, 0);
+    for (uint32_t j = 0; j < n; ++j)
+        for (uint32_t g = base[j]; g < base[j + 1]; ++g) sjob[g] = j;
     if (reserve) {  // the second phase's buffers: pairs (double-buffered, counts, prefixes) and the meshes
         size_t sort_bytes = 0;
         if (npairs)
@@ -1125,7 +1131,8 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
                                                       (unsigned long long*)nullptr, (unsigned long long*)nullptr,
                                                       (int)npairs, 0, 32, s));
         const size_t phase2 = (size_t)npairs * (4 + 4 + 8 + 8 + 4 + 4 + 8 + 8) + sort_bytes + scan_tmp_bytes(npairs) +
-                              (size_t)NT * (12 + 1) + (size_t)NV * 12 + 3 * ((size_t)S + 1) * 8 + 12 * 320;
+                              (size_t)NT * (12 + 1) + (size_t)NV * 12 + 3 * ((size_t)S + 1) * 8 + ((size_t)S + 1) * 4 +
+                              13 * 320;
         if (!reserve(phase2 + phase2 / 8)) {
             refuse();
             return;
@@ -1141,6 +1148,9 @@ void mc_run_jobs(McScratch& W, hipStream_t s, const McJobSpec* spec, uint32_t n,
     R.voff = W.buf[kBSeg].as<unsigned long long>();
     R.toff = R.voff + (S + 1);
     R.pstart = R.voff + 2 * ((size_t)S + 1);
+    W.ensure(kBSegJob, sjob.size() * 4 + 64);
+    VF_HIP(hipMemcpyAsync(W.buf[kBSegJob].p, sjob.data(), sjob.size() * 4, hipMemcpyHostToDevice, s));
+    R.seg_job = W.buf[kBSegJob].as<uint32_t>();
     W.ensure(kBXyz, (size_t)NV * 12 + 64);
     W.ensure(kBTri, (size_t)NT * 12 + 64);
     W.ensure(kBFlag, (size_t)NT + 64);
