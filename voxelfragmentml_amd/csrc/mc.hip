@@ -826,14 +826,27 @@ __global__ __launch_bounds__(kEmitPairs) void k_mc_emit(McRun R)
                 } else {
                     own[0] = own[1] = own[2] = own[3] = i;  // no cube reaches x - 1 < 0
                 }
+                // each owner's first vertex within the label and its vertex axes, all loads issued together; a
+                // corner picks its owner's by selects on the owner's index bits (own[] indexed by a value
+                // would live in scratch memory, several bytes of traffic per pair)
+                uint32_t vb[8], ax[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    vb[c] = (uint32_t)(R.vpre[own[c]] - v0);
+                    ax[c] = (uint32_t)((R.svals[own[c]] >> 48) & 7u);
+                }
+                auto pick = [](const uint32_t (&a)[8], int o) {
+                    const bool b0 = o & 1, b1 = o & 2, b2 = o & 4;
+                    const uint32_t l0 = b0 ? a[1] : a[0], l1 = b0 ? a[3] : a[2], l2 = b0 ? a[5] : a[4], l3 = b0 ? a[7] : a[6];
+                    const uint32_t m0 = b1 ? l1 : l0, m1 = b1 ? l3 : l2;
+                    return b2 ? m1 : m0;
+                };
                 auto corner = [&](int ca, int cb) -> uint32_t {
                     const int dxa = c_mc_nb[ca][0], dya = c_mc_nb[ca][1], dza = c_mc_nb[ca][2];
                     const int dxb = c_mc_nb[cb][0], dyb = c_mc_nb[cb][1], dzb = c_mc_nb[cb][2];
                     const int axis = dxa != dxb ? 0 : (dya != dyb ? 1 : 2);
                     const int o = (min(dxa, dxb) + 1) * 4 + min(dya, dyb) * 2 + min(dza, dzb);
-                    const unsigned long long k = own[o];
-                    const uint32_t kax = (uint32_t)((R.svals[k] >> 48) & 7u);
-                    return (uint32_t)(R.vpre[k] - v0) + (uint32_t)__popc(kax & ((1u << axis) - 1u));
+                    return pick(vb, o) + (uint32_t)__popc(pick(ax, o) & ((1u << axis) - 1u));
                 };
                 const unsigned long long w = c_mc_tri[conf];
                 uint32_t to = (uint32_t)(R.tpre[i] - tb);
