@@ -175,7 +175,8 @@ def roofline(prof, specs, res, traffic_ok):
                          f"/ launches per step / average launch time (HIP events on the launch stream)"}
     if dom == "flood_relax":
         visits = sum(r["flood_bricks"] for r in res)
-        out["limiter"] = "VALU/SALU issue (SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_BUSY_CYCLES), not HBM"
+        out["limiter"] = ("dependent sweeps within a visit and dependent rounds: a few waves per CU resident on average "
+                          "(SQ_WAVE_CYCLES / SQ_BUSY_CYCLES), VALU about a third busy (SQ_INSTS_VALU); not HBM")
         out["unit_work"] = {"brick_visits_per_step": visits, "visit_bytes_per_step": 4096 * visits,
                             "sweeps_per_step": sum(r["flood_sweeps"] for r in res),
                             "visit_gbs": round(4096 * visits / launches / (prof[dom]["avg_ms"] * 1e-3) / 1e9, 1)}
