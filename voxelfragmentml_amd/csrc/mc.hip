@@ -668,9 +668,10 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_pairs(McRun R)
         const bool one = sp[j0 + 1] >= ubase + kMcThreads;
         stage_grid(G, R, j0, staged);
         const uint32_t n = u < R.total_units ? R.unit_pairs[u] : 0u;
-        uint32_t total = 0, nq = 0;
-        const uint32_t ex = block_excl_scan_u32(n, wsum, &total);
-        const uint32_t pos = block_excl_scan_u32(n ? 1u : 0u, wsum, &nq);
+        // one scan of both counts: pairs in the low 16 bits (at most 256 x 72), queued units in the high 16
+        uint32_t both = 0;
+        const uint32_t exb = block_excl_scan_u32(n | (n ? 0x10000u : 0u), wsum, &both);
+        const uint32_t ex = exb & 0xFFFFu, pos = exb >> 16, nq = both >> 16;
         if (n) {
             const uint32_t j = one ? j0 : find_job_u64(sp, R.njobs, u);
             const McJob J = job_of(G, R, one, j);
