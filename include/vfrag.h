@@ -178,6 +178,28 @@ int vf_homogenize(vf_grid* g);
  * not reported. */
 int vf_histogram(vf_grid* g, uint32_t* counts, uint32_t ncounts);
 
+/* The per-fragment metadata RegularGrid::toTriangleMesh fills from countValues
+ * (RegularGrid.cpp:443-471; FragmentationProcedure::FragmentMetadata,
+ * FragmentationProcedure.h:22-37; written by CADScene::exportMetadata,
+ * CADScene.cpp:580-622): one entry per value above FREE (counts[0] and counts[1]
+ * are ignored) with a nonzero count, in ascending
+ * value order (the reference assigns entry value - 2, i.e. the same order when the
+ * labels are 2, 3, ...), with id = the entry's index, voxels = its count,
+ * occupied_voxels = the sum of all counts and percentage = (float)voxels /
+ * (float)occupied_voxels.  Host-only (no device work): counts are the 65536 counts
+ * vf_histogram returns.  *n receives the number of entries; at most cap are
+ * written. */
+typedef struct {
+    uint32_t id;
+    uint32_t value;             /* the fragment's label */
+    uint32_t voxels;
+    uint32_t occupied_voxels;
+    float percentage;
+    uint32_t voxelization_size[3];
+} vf_fragment_meta;
+int vf_fragment_metadata(const uint32_t* counts, const uint32_t dims[3], vf_fragment_meta* out, uint32_t cap,
+                         uint32_t* n);
+
 /* ---- marching cubes -------------------------------------------------------- */
 /* RegularGrid::toTriangleMesh -> MarchingCubes::triangulateFieldGPU
  * (RegularGrid.cpp:443-486, MF/Source/Graphics/Core/MarchingCubes.cpp:364-433,

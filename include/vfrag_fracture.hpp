@@ -190,6 +190,21 @@ public:
         return total;
     }
 
+    // The FragmentationProcedure::FragmentMetadata entries (FragmentationProcedure.h:22-37)
+    // RegularGrid::toTriangleMesh fills (RegularGrid.cpp:443-471): one per label in
+    // ascending order, percentage = voxels / occupied voxels
+    std::vector<vf_fragment_meta> fragmentMetadata()
+    {
+        std::vector<uint32_t> counts(65536);
+        check(vf_histogram(g_, counts.data(), 65536));
+        uint32_t d[3], n = 0;
+        check(vf_grid_dims(g_, d));
+        check(vf_fragment_metadata(counts.data(), d, nullptr, 0, &n));
+        std::vector<vf_fragment_meta> out(n);
+        check(vf_fragment_metadata(counts.data(), d, out.data(), n, &n));
+        return out;
+    }
+
     // RegularGrid::numOccupiedVoxels
     unsigned numOccupiedVoxels()
     {

@@ -94,3 +94,39 @@ def test_seed_merge_host_entry_matches_oracle(dist):
     with pytest.raises(vfrag.VfError) as e:
         vfrag.seed_merge(frags[:1], many, dist)
     assert e.value.code == -4
+
+
+def test_fragment_metadata_host_entry():
+    """vf_fragment_metadata is host code over the label counts: the entries of
+    RegularGrid::toTriangleMesh (RegularGrid.cpp:443-471), one per label above
+    FREE, percentage = voxels / occupied voxels in float32."""
+    if not LIB.exists():
+        pytest.skip("libvfrag.so not built")
+    import numpy as np
+    from voxelfragmentml_amd import vfrag
+
+    counts = np.zeros(65536, np.uint32)
+    counts[[0, 1]] = [1000, 77]          # EMPTY and FREE: never part of the metadata
+    counts[2], counts[3], counts[9] = 3, 1, 4
+    meta = vfrag.fragment_metadata(counts, (10, 20, 30))
+    assert [m["value"] for m in meta] == [2, 3, 9] and [m["id"] for m in meta] == [0, 1, 2]
+    assert [m["voxels"] for m in meta] == [3, 1, 4]
+    assert all(m["occupied_voxels"] == 8 and m["voxelization_size"] == (10, 20, 30) for m in meta)
+    assert [m["percentage"] for m in meta] == [0.375, 0.125, 0.5]   # exact in binary
+
+    rng = np.random.default_rng(9)
+    labels = rng.integers(0, 40, 200000).astype(np.uint16)
+    counts = np.bincount(labels, minlength=65536).astype(np.uint32)
+    counts[[0, 1, 17]] = 0
+    meta = vfrag.fragment_metadata(counts, (64, 64, 64))
+    keep = np.flatnonzero(counts[2:]) + 2
+    total = int(counts[2:].sum())
+    assert [m["value"] for m in meta] == keep.tolist()
+    expect = (counts[keep].astype(np.float32) / np.float32(total)).tolist()
+    assert [m["percentage"] for m in meta] == expect
+    assert abs(sum(m["percentage"] for m in meta) - 1.0) < 1e-5
+    n = ctypes.c_uint32(0)
+    out = (vfrag.FragmentMeta * 4)()
+    dims = np.array([64, 64, 64], np.uint32)
+    assert vfrag.lib().vf_fragment_metadata(vfrag._ptr(counts), vfrag._ptr(dims), out, 4, ctypes.byref(n)) == 0
+    assert n.value == len(keep) and [out[i].value for i in range(4)] == keep[:4].tolist()

@@ -51,11 +51,19 @@ def test_cfg_256_flood_noise_histogram_exact(vf):
         g.fracture_flood(seeds, EUC, noise_seed=2, noise_amp=2)
         got = g.read()
         hist = g.histogram()
+        meta = g.fragment_metadata()
     ref_base = O.occ_to_grid(O.voxelize(xyz, tri, *BOX, (256, 256, 256)))
     np.testing.assert_array_equal(base, ref_base)
     ref = O.flood(ref_base, seeds, 26, 2, 2)
     np.testing.assert_array_equal(got, ref)
-    np.testing.assert_array_equal(hist, O.histogram(ref))
+    ref_hist = O.histogram(ref)
+    np.testing.assert_array_equal(hist, ref_hist)
+    keep = np.flatnonzero(ref_hist[2:]) + 2
+    total = int(ref_hist[2:].sum())
+    assert [m["value"] for m in meta] == keep.tolist()
+    assert [m["voxels"] for m in meta] == ref_hist[keep].tolist()
+    assert all(m["occupied_voxels"] == total for m in meta)
+    assert [m["percentage"] for m in meta] == (ref_hist[keep].astype(np.float32) / np.float32(total)).tolist()
 
 
 def fragments_connected(lab, labels):

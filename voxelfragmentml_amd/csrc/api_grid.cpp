@@ -731,6 +731,35 @@ int vf_histogram(vf_grid* g, uint32_t* counts, uint32_t ncounts)
     });
 }
 
+int vf_fragment_metadata(const uint32_t* counts, const uint32_t dims[3], vf_fragment_meta* out, uint32_t cap,
+                         uint32_t* n)
+{
+    return guard([&] {
+        if (!counts || !dims || !n) fail(VF_ERR_INVALID_ARG, "counts, dims or n is NULL");
+        if (cap && !out) fail(VF_ERR_INVALID_ARG, "out is NULL");
+        // labels above FREE only, as countValues reports them; the reference sums
+        // into an unsigned
+        uint32_t total = 0, m = 0;
+        for (uint32_t v = 2; v < 65536; ++v) {
+            total += counts[v];
+            m += counts[v] != 0;
+        }
+        uint32_t k = 0;
+        for (uint32_t v = 2; v < 65536 && k < cap; ++v) {
+            if (!counts[v]) continue;
+            vf_fragment_meta& f = out[k];
+            f.id = k;
+            f.value = v;
+            f.voxels = counts[v];
+            f.occupied_voxels = total;
+            f.percentage = (float)counts[v] / (float)total;
+            for (int a = 0; a < 3; ++a) f.voxelization_size[a] = dims[a];
+            ++k;
+        }
+        *n = m;
+    });
+}
+
 int vf_marching_cubes(vf_grid* g, uint32_t label, vf_mesh_info* info)
 {
     return guard([&] {

@@ -41,6 +41,13 @@ class MeshInfo(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_labels", C.c_uint32)]
 
 
+class FragmentMeta(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32), ("value", C.c_uint32), ("voxels", C.c_uint32), ("occupied_voxels", C.c_uint32),
+        ("percentage", C.c_float), ("voxelization_size", C.c_uint32 * 3),
+    ]
+
+
 class MeshOut(C.Structure):
     _fields_ = [
         ("xyz", C.c_void_p), ("tri", C.c_void_p), ("tri_flag", C.c_void_p), ("labels", C.c_void_p),
@@ -104,7 +111,7 @@ def lib():
         "vf_fracture_flood": (I, [P, P, U32, I, I, U64, U32]),
         "vf_detect_boundaries": (I, [P, I]), "vf_undo_mask": (I, [P]), "vf_homogenize": (I, [P]),
         "vf_erode": (I, [P, I, U32, U32, F, F, U64]), "vf_remove_isolated_cells": (I, [P]),
-        "vf_histogram": (I, [P, P, U32]),
+        "vf_histogram": (I, [P, P, U32]), "vf_fragment_metadata": (I, [P, P, P, U32, C.POINTER(U32)]),
         "vf_marching_cubes": (I, [P, U32, C.POINTER(MeshInfo)]),
         "vf_mesh_read": (I, [P, P, P, P, P, P, P]),
         "vf_mesh_smooth": (I, [P, U32, F, U32, F]),
@@ -164,6 +171,23 @@ def vessel_segments(target_triangles: int):
     r = max(3, int(round((target_triangles / 10.0) ** 0.5)))
     h = max(1, int(round(2.5 * r)))
     return r, h
+
+
+def fragment_metadata(counts, dims) -> list:
+    """FragmentationProcedure::FragmentMetadata entries (FragmentationProcedure.h:22-37) from the
+    65536 label counts Grid.histogram returns: one dict per label above FREE with a nonzero count,
+    in ascending label order (RegularGrid.cpp:443-471)."""
+    c = np.ascontiguousarray(counts, np.uint32)
+    if c.size != 65536:
+        raise ValueError("counts must have 65536 entries")
+    d = np.asarray(dims, np.uint32)
+    n = C.c_uint32(0)
+    check(lib().vf_fragment_metadata(_ptr(c), _ptr(d), None, 0, C.byref(n)))
+    out = (FragmentMeta * max(1, n.value))()
+    check(lib().vf_fragment_metadata(_ptr(c), _ptr(d), out, n.value, C.byref(n)))
+    return [{"id": f.id, "value": f.value, "voxels": f.voxels, "occupied_voxels": f.occupied_voxels,
+             "percentage": f.percentage, "voxelization_size": tuple(f.voxelization_size)}
+            for f in out[:n.value]]
 
 
 def seeds_array(seeds) -> np.ndarray:
@@ -294,6 +318,10 @@ class Grid:
         out = np.zeros(n, np.uint32)
         check(lib().vf_histogram(self._h, _ptr(out), n))
         return out
+
+    def fragment_metadata(self) -> list:
+        """Per-fragment records of RegularGrid::toTriangleMesh (RegularGrid.cpp:443-471)."""
+        return fragment_metadata(self.histogram(), self.dims)
 
     def marching_cubes_info(self, label: int = ALL_LABELS) -> dict:
         """Marching cubes on the device; only the sizes come back (read_mesh copies the mesh out)."""
