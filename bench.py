@@ -153,6 +153,54 @@ def kernel_profile(vf, jobs, steps):
     return prof, res
 
 
+class Records:
+    """JSON-lines records, one per job (dims, seeds, stage times, results), then a run summary; no-op without a
+    path."""
+
+    def __init__(self, path):
+        self.f = open(path, "w") if path else None
+
+    def __bool__(self):
+        return self.f is not None
+
+    def emit(self, rec):
+        if self.f:
+            self.f.write(json.dumps(rec) + "\n")
+
+    def close(self):
+        if self.f:
+            self.f.close()
+            self.f = None
+
+
+def spec_record(s):
+    e = s["edge"]
+    return {"dims": [e, e, e], "triangles": s["tris"], "seeds": s["seeds"], "seed_key": s["seed_key"],
+            "algorithm": "naive" if s["algo"] == 0 else "flood", "distance": ["euclidean", "manhattan", "chebyshev"][s["dist"]],
+            "noise_seed": s["noise_seed"] if s["algo"] == 1 else None}
+
+
+RESULT_KEYS = ["status", "occupied", "num_labels", "flood_levels", "flood_bricks", "flood_sweeps", "checksum",
+               "mc_vertices", "mc_triangles"]
+
+
+def job_stage_times(vf, job):
+    """One job alone (graphs off): device ms per stage from HIP events around every launch."""
+    b = vf.Batch(graphs=False)
+    b.run([job])
+    vf.timing_reset()
+    vf.timing_enable(True)
+    r = b.run([job])[0]
+    vf.timing_enable(False)
+    b.close()
+    stage = {}
+    for st, ks in STAGES.items():
+        ms = sum(vf.timing_read(k)[0] for k in ks)
+        if ms:
+            stage[st] = round(ms, 4)
+    return stage, r
+
+
 def roofline(prof, specs, res, traffic_ok):
     sb = stage_bytes(specs)
     stages = {}
@@ -189,7 +237,7 @@ def roofline(prof, specs, res, traffic_ok):
 
 
 # ------------------------------------------------------------------ served ---
-def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n):
+def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n, records=None):
     """Poisson arrivals at `rate` jobs/s.  Each request is a fresh draw from the mixed distribution extended
     to 512^3 and 128 seeds (edge, seeds, distance, algorithm, seed and noise keys), on a mesh taken from a pool
     of `pool_size` device-resident vessels of 5k-300k triangles (uploaded before the stream starts)."""
@@ -233,6 +281,10 @@ def served(vf, rate, n_req, budget, slots, seed, pool_size, check_n):
     srv.close()
     lat = np.array([r["latency_ms"] for r in results])
     vox = sum(s["edge"] ** 3 for s in reqs)
+    if records:
+        for i, (s, r) in enumerate(zip(reqs, results)):
+            records.emit({"form": "served", "request": i, **spec_record(s), "latency_ms": round(r["latency_ms"], 4),
+                          "queue_ms": round(r["queue_ms"], 4), **{k: r[k] for k in RESULT_KEYS}})
     out = {
         "requests": n_req, "rate_jobs_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
@@ -434,6 +486,9 @@ def main():
     ap.add_argument("--cpu-jobs", type=int, default=32)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--skip", default="", help="comma list of: tune,extract,plain,served,cpu,profile,configs")
+    ap.add_argument("--records", default="", metavar="PATH",
+                    help="write JSON-lines records: one per batched job (dims, seeds, per-stage device ms of the job "
+                         "run alone, results), one per served request (latency, queue time), then the run summary")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a reference dispatch of known "
                          "traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -561,11 +616,25 @@ def main():
         log(f"roofline: {out['roofline']}")
         log(f"stages: {out['stage_roofline']}")
 
+    records = Records(a.records)
+    if records:
+        # per-job records: every job of the batch alone with events on (outside every timed region)
+        xres = xr if "extract" not in skip else None
+        for i, (s, job, r) in enumerate(zip(specs, mjobs or jobs, res)):
+            stage, alone = job_stage_times(vf, job)
+            rec = {"form": "batched", "job": i, **spec_record(s), "voxels": s["edge"] ** 3,
+                   **{k: r[k] for k in RESULT_KEYS}, "stage_ms_alone": stage,
+                   "extract": bool(mjobs), "alone_equals_batched": alone["checksum"] == r["checksum"]}
+            if xres is not None:
+                rec["mc_vertices"], rec["mc_triangles"] = xres[i]["mc_vertices"], xres[i]["mc_triangles"]
+            records.emit(rec)
+        log(f"records: {len(specs)} batched jobs written to {a.records}")
+
     if "configs" not in skip:
         out["configs"] = config_runs(vf, 3)
 
     if "served" not in skip:
-        out["served"] = served(vf, a.serve_rate, a.serve_requests, 200 << 30, 64, 5, 64, a.serve_check)
+        out["served"] = served(vf, a.serve_rate, a.serve_requests, 200 << 30, 64, 5, 64, a.serve_check, records)
         log(f"served: {out['served']}")
 
     if "cpu" not in skip:
@@ -574,6 +643,15 @@ def main():
 
     for m in meshes:
         m.close()
+    if records:
+        summary = {"form": "summary", "gvoxels_per_s": out["value"], "ms_per_step": out["ms_per_step"],
+                   "hbm_frac": out["path_bytes"]["frac_of_hbm_peak"]}
+        if "served" in out:
+            summary.update(served_p50_ms=out["served"]["p50_ms"], served_p99_ms=out["served"]["p99_ms"])
+        if "plain" in out:
+            summary.update(plain_job_p50_ms=out["plain"]["job_p50_ms"], plain_job_p99_ms=out["plain"]["job_p99_ms"])
+        records.emit(summary)
+        records.close()
     print(json.dumps(out), flush=True)
 
 
