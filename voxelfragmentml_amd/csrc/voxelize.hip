@@ -319,7 +319,8 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_setup(const JobDev* jobs, u
             ncol = 0;
         }
         if (valid) {
-            store_setup(setup, total, g, S);
+            // only the pair kernels read a record, and only a triangle with pairs left
+            if (nshell | ncol) store_setup(setup, total, g, S);
             cnt_shell[g] = nshell;
             cnt_col[g] = ncol;
         }
