@@ -669,7 +669,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         work = 0;
         activated = error = false;
     };
-    for (uint32_t e = local; e < n; e += per_xcd) {
+    // each workgroup takes a contiguous run of the list: consecutive entries are mostly one job's (its
+    // state is loaded once per run, not per visit) and neighbouring bricks of one flag word
+    const uint32_t per = (n + per_xcd - 1u) / per_xcd, e0 = min(local * per, n), e1 = min(e0 + per, n);
+    for (uint32_t e = e0; e < e1; ++e) {
         const unsigned long long entry = list[e];
         const uint32_t j = uni((uint32_t)(entry >> 32)), brick = uni((uint32_t)entry);
         if (j != cur_job) {
