@@ -75,6 +75,9 @@ constexpr uint32_t kZtWords = 100 * kSepStride;
 constexpr uint32_t kYtWords = 80 * kSepStride;
 static_assert(kYtWords <= kTileWords, "yt reuses the tile");
 constexpr uint32_t kRelaxLdsWords = kTileWords + kZtWords;
+// Consecutive worklist entries a relax workgroup takes at a time (batch step 11.47 ms with one run per
+// workgroup, 11.28 / 11.25 / 11.30 / 11.50 / 11.64 ms with runs of 2 / 3 / 4 / 5 / 6, 11.83 with 8)
+constexpr uint32_t kRelaxRun = 3;
 constexpr uint32_t kMaxSweeps = 2048;              // a brick of 512 cells is stable after at most 513 sweeps
 constexpr uint32_t kXcds = kFloodXcds;             // XCDs of the device; workgroups are dealt round-robin
 
@@ -669,10 +672,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         work = 0;
         activated = error = false;
     };
-    // each workgroup takes a contiguous run of the list: consecutive entries are mostly one job's (its
-    // state is loaded once per run, not per visit) and neighbouring bricks of one flag word
-    const uint32_t per = (n + per_xcd - 1u) / per_xcd, e0 = min(local * per, n), e1 = min(e0 + per, n);
-    for (uint32_t e = e0; e < e1; ++e) {
+    // the list is cut into runs of kRelaxRun consecutive entries, dealt round-robin to the workgroups: a
+    // run is mostly one job's (its state is loaded once per run) and neighbouring bricks of one flag word,
+    // so a later brick of a run reads the cells an earlier one just dropped, while the bricks relaxed at
+    // one time stay close together in the list and share their halos in L2 (longer runs, up to one run
+    // per workgroup, spread them over the whole list)
+    for (uint32_t e = local * kRelaxRun; e < n;
+         e = (e % kRelaxRun == kRelaxRun - 1u) ? e + 1u + (per_xcd - 1u) * kRelaxRun : e + 1u) {
         const unsigned long long entry = list[e];
         const uint32_t j = uni((uint32_t)(entry >> 32)), brick = uni((uint32_t)entry);
         if (j != cur_job) {
