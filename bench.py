@@ -223,8 +223,10 @@ def roofline(prof, specs, res, traffic_ok):
                          f"/ launches per step / average launch time (HIP events on the launch stream)"}
     if dom == "flood_relax":
         visits = sum(r["flood_bricks"] for r in res)
-        out["limiter"] = ("dependent sweeps within a visit and dependent rounds: a few waves per CU resident on average "
-                          "(SQ_WAVE_CYCLES / SQ_BUSY_CYCLES), VALU about a third busy (SQ_INSTS_VALU); not HBM")
+        out["limiter"] = ("latency of each visit's dependent chain (halo loads, then sweeps of LDS round trips) over "
+                          "dependent rounds: 13 of 18 waves per CU resident (4 x SQ_WAVE_CYCLES / SQ_BUSY_CYCLES per "
+                          "CU), waiting on results 42 % and on issue 30 % of their time (SQ_WAIT_ANY, SQ_WAIT_INST_ANY), "
+                          "VALU at 40 % of the rate k_vox_shell sustains (SQ_INSTS_VALU); not HBM")
         out["unit_work"] = {"brick_visits_per_step": visits, "visit_bytes_per_step": 4096 * visits,
                             "sweeps_per_step": sum(r["flood_sweeps"] for r in res),
                             "visit_gbs": round(4096 * visits / launches / (prof[dom]["avg_ms"] * 1e-3) / 1e9, 1)}
