@@ -520,8 +520,9 @@ __device__ inline uint32_t relax_brick(const RelaxJob& R, uint32_t brick, uint32
     return sweeps;
 }
 
-// The uniform per-job state of round r in scalar registers.
-__device__ inline void load_relax_state(const JobDev* J, uint32_t r, RelaxJob& R)
+// The round-independent part of a flood job's relaxation state, in scalar registers, read in one batch
+// of loads; returns the job's flag words (two parities, then the liveness words and neighbour masks).
+__device__ inline unsigned long long* load_relax_fields(const JobDev* J, RelaxJob& R)
 {
     R.J = uni_ptr(J);
     R.nx = uni(J->nx); R.ny = uni(J->ny); R.nz = uni(J->nz);
@@ -531,21 +532,39 @@ __device__ inline void load_relax_state(const JobDev* J, uint32_t r, RelaxJob& R
     R.amp = uni(J->noise_amp);
     R.noise_seed = ((uint64_t)uni((uint32_t)(J->noise_seed >> 32)) << 32) | uni((uint32_t)J->noise_seed);
     R.keys = gcast<uint32_t>(uni_ptr(J->keys));
-    R.r = r;
     R.nw = relax_words_per_parity(R.nb);
     unsigned long long* flags = uni_ptr(J->bflags);
-    R.cur = gcast<unsigned long long>(flags + (size_t)R.nw * (R.r & 1u));
-    R.nxt = gcast<unsigned long long>(flags + (size_t)R.nw * ((R.r + 1u) & 1u));
     R.nmask = gcast<const uint32_t>(reinterpret_cast<const uint32_t*>(flags + (size_t)R.nw * 2u + live_words(R.nbx, R.nby, R.nbz)));
     R.n26 = uni((uint32_t)J->neighbourhood) != 6u;
+    return flags;
 }
 
-// A running job's state for the relaxation step (its round was set by this round's compaction).
+// Round r's flag words: round r consumes parity r & 1 and flags into the other.
+__device__ inline void set_relax_round(RelaxJob& R, unsigned long long* flags, uint32_t r)
+{
+    R.r = r;
+    R.cur = gcast<unsigned long long>(flags + (size_t)R.nw * (r & 1u));
+    R.nxt = gcast<unsigned long long>(flags + (size_t)R.nw * ((r + 1u) & 1u));
+}
+
+// A flood job is running and not drained: its record's flag (read with its fields, before any branch).
+__device__ inline uint32_t flood_job_active(const JobDev* J)
+{
+    return uni((uint32_t)((J->active != 0) & (J->algo == VF_FLOOD)));
+}
+
+// A running job's state for the relaxation step (its round was set by this round's compaction): the job's
+// fields, then its control words, two dependent batches of loads.
 __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
 {
     const JobDev* J = L.jobs + j;
-    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u && uni(J->fctl[1]) == 0u)) return false;
-    load_relax_state(J, uni(J->fctl[0]), R);
+    const uint32_t act = flood_job_active(J);
+    const uint32_t* ctl = uni_ptr(J->fctl);
+    unsigned long long* flags = load_relax_fields(J, R);
+    if (!act) return false;
+    const uint32_t done = uni(ctl[1]), r = uni(ctl[0]);
+    if (done) return false;
+    set_relax_round(R, flags, r);
     return true;
 }
 
@@ -555,15 +574,22 @@ __device__ inline bool load_relax_job(const Launch& L, uint32_t j, RelaxJob& R)
 // fctl[4 + (i & 1)] and fctl[0], and the flags-non-empty word of round rp is cleared (round
 // rp + 1's relaxation writes it for round rp + 2).  Every workgroup of the job computes the same
 // values from words no workgroup writes in this launch, and writes them: no ordering is needed.
-__device__ inline bool start_round(const Launch& L, uint32_t j, uint32_t i, RelaxJob& R)
+// The loads come in two dependent batches: the job's fields, then its control words together with
+// both parities' flag word w (lane w of the task), of which *m receives the new round's.
+__device__ inline bool start_round(const Launch& L, uint32_t j, uint32_t i, uint32_t w, RelaxJob& R, uint64_t& m)
 {
     const JobDev* J = L.jobs + j;
-    if (!(uni((uint32_t)(J->active && J->algo == VF_FLOOD)) != 0u)) return false;
-    uint32_t* ctl = J->fctl;
-    const uint32_t done = uni(ctl[1]), rp = uni(ctl[4 + ((i + 1u) & 1u)]);
+    const uint32_t act = flood_job_active(J);
+    uint32_t* ctl = uni_ptr(J->fctl);
+    unsigned long long* flags = load_relax_fields(J, R);
+    if (!act) return false;
+    const uint32_t done = uni(ctl[1]), s0 = uni(ctl[4]), s1 = uni(ctl[5]), f0 = uni(ctl[2]), f1 = uni(ctl[3]);
+    const bool in = w < R.nw;
+    const uint64_t m0 = in ? flags[w] : 0ull, m1 = in ? flags[(size_t)R.nw + w] : 0ull;
     if (done) return false;
+    const uint32_t rp = ((i + 1u) & 1u) ? s1 : s0;
     const bool lead = threadIdx.x == 0;
-    if (uni(ctl[2 + ((rp + 1u) & 1u)]) == 0u) {
+    if ((((rp + 1u) & 1u) ? f1 : f0) == 0u) {
         if (lead) {
             ctl[1] = rp;
             J->stats[kStatFloodDone] = rp;
@@ -576,7 +602,8 @@ __device__ inline bool start_round(const Launch& L, uint32_t j, uint32_t i, Rela
         ctl[0] = r;
         ctl[2 + (rp & 1u)] = 0u;
     }
-    load_relax_state(J, r, R);
+    set_relax_round(R, flags, r);
+    m = (r & 1u) ? m1 : m0;
     return true;
 }
 
@@ -615,13 +642,15 @@ __global__ __launch_bounds__(64) void k_flood_compact(Launch L, const uint32_t* 
             if (sp[mid] <= task) lo = mid; else hi = mid;
         }
         const uint32_t j = uni(lo);
+        const uint32_t w = (task - sp[j]) * 64u + lane;
+        uint64_t m = 0ull;
         if (j != cur_job) {
             cur_job = j;
-            running = start_round(L, j, i, R);
+            running = start_round(L, j, i, w, R, m);
+        } else if (running) {
+            m = w < R.nw ? R.cur[w] : 0ull;
         }
         if (!running) continue;
-        const uint32_t w = (task - sp[j]) * 64u + lane;
-        const uint64_t m = w < R.nw ? R.cur[w] : 0ull;
         const uint32_t n = (uint32_t)__popcll(m);
         uint32_t inc = n;  // wave inclusive scan of the per-lane counts
 #pragma unroll
