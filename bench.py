@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # plain copy's counts, written as "traffic_bytes_per_launch <kernel> <bytes>"
 TRAFFIC_SUMMARY = ROOT / "profiles" / "flood_relax_traffic_mixed64.txt"
 DEFAULT_JOBS = 64
-HOST_WORKERS = 16  # the host's CPU share of a one-GPU box
+HOST_WORKERS = 16  # host threads the CPU legs use (the CPU share of one GPU on an 8-GPU, 128-core node)
 
 
 def edge_bucket(edge):
