@@ -22,15 +22,15 @@ namespace {
 std::mutex g_mu;
 // Defaults measured on MI355X with the mixed 64-job batch.  Cell kernels: 8192 workgroups of 4 waves
 // (8 waves per SIMD for the streaming kernels; 2048 left them at 2 and latency-bound).  Boundary
-// bricks: 16384.  Flood relaxation: 4608 one-wave workgroups (18 per CU), all resident at once, taking
-// short runs of their XCD's brick list round-robin (with one contiguous run per workgroup the batch step
-// was 11.41 ms at 4096 and 4608, 11.51 at 3072, 12.12 at 2048, 11.46-11.59 at 5120 and 6144).
+// bricks: 16384.  Flood relaxation: 4096 one-wave workgroups (16 per CU), all resident at once, taking
+// short runs of their XCD's brick list round-robin (batch step on one box: 10.91-10.93 ms at 4096,
+// 10.95-10.98 at 4608, 11.02 at 3584, 11.30 at 5120).
 // Walker runs: 8 naive tiles, 4 boundary bricks.  Flood: chunks of 8 rounds, the first drain check after 3.
 LaunchShape g_shapes[kTuneBuckets] = {
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
-    { 8192, 4608, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4096, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4096, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4096, 8192, 16384, 8, 4, 8, 3, 2048 },
+    { 8192, 4096, 8192, 16384, 8, 4, 8, 3, 2048 },
 };
 uint64_t g_generation = 1;
 thread_local int t_bucket = kTuneBuckets - 1;
